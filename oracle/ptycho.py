@@ -1,0 +1,436 @@
+"""PyNX operators and the AP, DM, RAAR, ML and rPIE loops of the reference's CPU engine in
+FP64, without features (no position offsets, scale or background).
+
+Test infrastructure (see oracle/__init__.py). Layout: exit waves `psi[k]` shifted, `k` in
+table order; objects `[nobj, nyo, nxo]`, probes `[nprobe, ny, nx]`, natural layout.
+"""
+
+import math
+
+import numpy as np
+
+from .base import AddrTable, observed, pynx_preprocess, random_batches, to_natural, to_shifted
+from .optics import Optics, fft2
+
+AP, DM, RAAR, RPIE, ML, BH, LSQML = "ap", "dm", "raar", "rpie", "ml", "bh", "lsqml"
+
+
+def obj_probe_to_psi(addr, views, obj, probe, quad):
+    """`psi[k] = fftshift(P[probe_k]·O[obj_k, win_k]·q)`. ptyrs/src/engine/ops.rs:42-71"""
+    ny, nx = probe.shape[-2:]
+    out = np.empty((len(views), ny, nx), dtype=np.complex128)
+    for k, (_, _, p, o, oy, ox) in enumerate(views):
+        out[k] = to_shifted(probe[p] * obj[o, oy:oy + ny, ox:ox + nx] * quad)
+    return out
+
+
+def _mode_sum(psi, nmode):
+    n = psi.shape[0] // nmode
+    a = psi.reshape(n, nmode, *psi.shape[1:])
+    return (a.real ** 2 + a.imag ** 2).sum(axis=1)
+
+
+def apply_amplitude(iobs, nmode, psi):
+    """`psi *= sqrt(iobs)/max(1e-12, sqrt(Σ_m|psi|²))` at unmasked pixels.
+    ptyrs/src/engine/ops.rs:73-107"""
+    calc = np.sqrt(_mode_sum(psi, nmode))
+    r = np.sqrt(iobs) / np.maximum(calc, 1e-12)
+    r = np.where(np.isnan(iobs), 1.0, r)
+    n = psi.shape[0] // nmode
+    return (psi.reshape(n, nmode, *psi.shape[1:]) * r[:, None]).reshape(psi.shape)
+
+
+class Llk:
+    """Poisson, Gaussian, Euclidian sums and `nb_photons_calc`. ptyrs/src/engine/ops.rs:255-277"""
+
+    def __init__(self, poisson=0.0, gaussian=0.0, euclidian=0.0, nb_photons_calc=0.0):
+        self.poisson, self.gaussian = poisson, gaussian
+        self.euclidian, self.nb_photons_calc = euclidian, nb_photons_calc
+
+    def normalised(self, nb_obs):
+        n = float(nb_obs)
+        return Llk(self.poisson / n, self.gaussian / n, self.euclidian / n, self.nb_photons_calc)
+
+    def as_dict(self):
+        return dict(poisson=self.poisson, gaussian=self.gaussian, euclidian=self.euclidian,
+                    nb_photons_calc=self.nb_photons_calc)
+
+
+def llk(iobs, nmode, psi):
+    """LLK sums of detector-plane psi; masked pixels excluded. ptyrs/src/engine/ops.rs:109-154"""
+    icalc = _mode_sum(psi, nmode)
+    keep = ~np.isnan(iobs)
+    io, ic = iobs[keep], icalc[keep]
+    pos = io > 0.0
+    safe = np.where(pos, io, 1.0)
+    pois = np.where(pos, ic - io + io * np.log(safe / np.where(pos, ic, 1.0)), ic)
+    return Llk(float(pois.sum()), float(((io - ic) ** 2 / (io + 1.0)).sum()),
+               float((4.0 * (np.sqrt(io) - np.sqrt(ic)) ** 2).sum()), float(ic.sum()))
+
+
+def poisson_grad(iobs, nmode, psi):
+    """`psi *= 2(1 − iobs/max(1e-12, icalc))`, masked pixels 0. ptyrs/src/engine/ops.rs:156-195"""
+    icalc = _mode_sum(psi, nmode)
+    r = 2.0 * (1.0 - iobs / np.maximum(icalc, 1e-12))
+    r = np.where(np.isnan(iobs), 0.0, r)
+    n = psi.shape[0] // nmode
+    return (psi.reshape(n, nmode, *psi.shape[1:]) * r[:, None]).reshape(psi.shape)
+
+
+def cg_poisson_gamma(iobs, nmode, po, a, b):
+    """Numerator and denominator of the Newton step along a direction.
+    ptyrs/src/engine/ops.rs:197-253"""
+    n = po.shape[0] // nmode
+    sh = (n, nmode) + po.shape[1:]
+    po, a, b = po.reshape(sh), a.reshape(sh), b.reshape(sh)
+    calc = (po.real ** 2 + po.imag ** 2).sum(axis=1)
+    r = (np.conj(po) * a).real.sum(axis=1)
+    s = (a.real ** 2 + a.imag ** 2 + 2.0 * (np.conj(po) * b).real).sum(axis=1)
+    keep = ~np.isnan(iobs)
+    calc = np.maximum(calc[keep], 1e-32)
+    r, s, io = r[keep], s[keep], iobs[keep]
+    d = 1.0 - io / calc
+    return float((-2.0 * r * d).sum()), float((2.0 * s * d + io * (2.0 * r / calc) ** 2).sum())
+
+
+def accumulate_obj(views, obj_shape, probe, quad, psi):
+    """Object back-projection over views in order. ptyrs/src/engine/ops.rs:279-310"""
+    ny, nx = probe.shape[-2:]
+    num = np.zeros(obj_shape, dtype=np.complex128)
+    norm = np.zeros(obj_shape, dtype=np.float64)
+    for k, (_, _, p, o, oy, ox) in enumerate(views):
+        w = to_natural(psi[k])
+        num[o, oy:oy + ny, ox:ox + nx] += w * np.conj(quad * probe[p])
+        norm[o, oy:oy + ny, ox:ox + nx] += probe[p].real ** 2 + probe[p].imag ** 2
+    return num, norm
+
+
+def accumulate_probe(views, obj, probe_shape, quad, psi):
+    """Probe back-projection over views in order. ptyrs/src/engine/ops.rs:312-343"""
+    ny, nx = probe_shape[-2:]
+    num = np.zeros(probe_shape, dtype=np.complex128)
+    norm = np.zeros(probe_shape, dtype=np.float64)
+    for k, (_, _, p, o, oy, ox) in enumerate(views):
+        w = to_natural(psi[k])
+        ob = obj[o, oy:oy + ny, ox:ox + nx]
+        num[p] += w * np.conj(quad * ob)
+        norm[p] += ob.real ** 2 + ob.imag ** 2
+    return num, norm
+
+
+def blend(old, num, norm, inertia):
+    """`(num + reg·old)/max(norm + reg, 1e-12)`, `reg = inertia·max(norm)`.
+    ptyrs/src/engine/ops.rs:367-381"""
+    reg = max(float(norm.max()), 0.0) * inertia
+    return (num + old * reg) / np.maximum(norm + reg, 1e-12)
+
+
+def psi_to_obj_probe(views, quad, psi, obj, probe, obj_inertia, probe_inertia):
+    """Psi2ObjProbe, both sums from the arrays on entry. ptyrs/src/engine/ops.rs:345-365"""
+    obj_acc = None
+    if obj_inertia is not None:
+        obj_acc = accumulate_obj(views, obj.shape, probe, quad, psi)
+    if probe_inertia is not None:
+        num, norm = accumulate_probe(views, obj, probe.shape, quad, psi)
+        probe = blend(probe, num, norm, probe_inertia)
+    if obj_inertia is not None:
+        obj = blend(obj, obj_acc[0], obj_acc[1], obj_inertia)
+    return obj, probe
+
+
+def llk_due(calc_llk, ic, ncycle):
+    """ptyrs/src/engine/ptycho.rs:43-46"""
+    return calc_llk > 0 and (ic % calc_llk == 0 or ic + 1 == ncycle)
+
+
+def zero_phase_ramp(obj, probe):
+    """PyNX ZeroPhaseRamp. ptyrs/src/engine/ptycho.rs:733-782"""
+    ny, nx = probe.shape[-2:]
+    far = fft2(to_shifted(probe))
+    v_nat = to_natural((far.real ** 2 + far.imag ** 2).sum(axis=0))
+    w = v_nat.sum()
+    yy, xx = np.mgrid[:ny, :nx]
+    dy = (v_nat * yy).sum() / w - (ny // 2)
+    dx = (v_nat * xx).sum() / w - (nx // 2)
+
+    def ramp(length, half, scale, d, sign):
+        return sign * 2.0 * math.pi * d * (np.arange(length) - half) / scale
+
+    def apply(a, ry, rx):
+        ph = ry[:, None] + rx[None, :]
+        return a * (np.cos(ph) + 1j * np.sin(ph))
+
+    probe = apply(probe, ramp(ny, ny // 2, ny, dy, -1.0), ramp(nx, nx // 2, nx, dx, -1.0))
+    nyo, nxo = obj.shape[-2:]
+    obj = apply(obj, ramp(nyo, nyo // 2, ny, dy, 1.0), ramp(nxo, nxo // 2, nx, dx, 1.0))
+    return obj, probe
+
+
+class Ptycho:
+    """Reconstruction state: object, probe, carried exit waves, LLK history.
+    ptyrs/src/engine/ptycho.rs:188-289 (no features)"""
+
+    def __init__(self, counts, y, x, obj, probe, optics):
+        probe = np.array(probe, dtype=np.complex128)
+        obj = np.array(obj, dtype=np.complex128)
+        frame = list(np.asarray(counts).shape[-2:])
+        if list(probe.shape[-2:]) != frame:
+            raise ValueError("probe plane differs from frame")
+        if optics.nslice() != 1:
+            raise ValueError("Ptycho runs one object slice")
+        if not optics.propagator.adjoint_inverts():
+            raise ValueError("Ptycho needs a unitary propagator")
+        py_, px_ = pynx_preprocess(y, x, optics.pixel)
+        self.addr = AddrTable(py_, px_, optics.pixel, frame, obj.shape[-2:],
+                              probe.shape[0], obj.shape[0])
+        self.optics = optics
+        self.iobs, self.nb_obs = observed(counts)
+        self.obj, self.probe = obj, probe
+        self.views_all = self.addr.views()
+        self.psi = np.zeros((len(self.views_all),) + tuple(frame), dtype=np.complex128)
+        self.cycle = 0
+        self.history = []
+
+    @property
+    def quad(self):
+        return self.optics.quad
+
+    def calc_psi(self):
+        return obj_probe_to_psi(self.addr, self.views_all, self.obj, self.probe, self.quad)
+
+    def llk(self):
+        """ptyrs/src/engine/ptycho.rs:453-466"""
+        psi = self.optics.exit_forward(self.calc_psi())
+        return llk(self.iobs, self.addr.nmode, psi).normalised(self.nb_obs)
+
+    def _record(self, algorithm, value):
+        self.history.append((self.cycle, algorithm, value))
+
+    def grad_psi(self):
+        """ptyrs/src/engine/ptycho.rs:148-162"""
+        psi = self.optics.exit_forward(self.calc_psi())
+        psi = poisson_grad(self.iobs, self.addr.nmode, psi)
+        return self.optics.exit_adjoint(psi)
+
+    def grads(self, obj, probe):
+        """2·∂L/∂O* and 2·∂L/∂P*. ptyrs/src/engine/ptycho.rs:93-114"""
+        g = self.grad_psi()
+        go = (accumulate_obj(self.views_all, self.obj.shape, self.probe, self.quad, g)[0]
+              if obj else np.zeros_like(self.obj))
+        gp = (accumulate_probe(self.views_all, self.obj, self.probe.shape, self.quad, g)[0]
+              if probe else np.zeros_like(self.probe))
+        return go, gp
+
+    def cg_gamma(self, d_obj, d_probe):
+        """ptyrs/src/engine/ptycho.rs:116-146 (no offsets)"""
+        v, q = self.views_all, self.quad
+        po = obj_probe_to_psi(self.addr, v, self.obj, self.probe, q)
+        a = (obj_probe_to_psi(self.addr, v, d_obj, self.probe, q)
+             + obj_probe_to_psi(self.addr, v, self.obj, d_probe, q))
+        b = obj_probe_to_psi(self.addr, v, d_obj, d_probe, q)
+        po, a, b = (self.optics.exit_forward(x) for x in (po, a, b))
+        return cg_poisson_gamma(self.iobs, self.addr.nmode, po, a, b)
+
+    def scale_obj_probe(self):
+        """PyNX ScaleObjProbe(absolute=True). ptyrs/src/engine/ptycho.rs:172-182"""
+        calc = self.llk().nb_photons_calc
+        total = float(np.nansum(self.iobs))
+        self.split_scale(math.sqrt(total / calc))
+
+    def split_scale(self, s):
+        """ptyrs/src/engine/ptycho.rs:184-192"""
+        os_ = float(np.abs(self.obj).sum())
+        ps = float(np.abs(self.probe).sum())
+        fp, fo = math.sqrt(os_ / ps * s), math.sqrt(ps / os_ * s)
+        self.probe = self.probe * fp
+        self.obj = self.obj * fo
+
+    def zero_phase_ramp(self):
+        self.obj, self.probe = zero_phase_ramp(self.obj, self.probe)
+
+    def ap(self, ncycle, obj_inertia=0.01, probe_inertia=None, calc_llk=0,
+           zero_phase_ramp=True):
+        """PyNX AP. ptyrs/src/engine/ptycho.rs:206-242"""
+        nmode = self.addr.nmode
+        for ic in range(ncycle):
+            self.psi = self.optics.exit_forward(self.calc_psi())
+            rec = None
+            if llk_due(calc_llk, ic, ncycle):
+                rec = llk(self.iobs, nmode, self.psi).normalised(self.nb_obs)
+            self.psi = apply_amplitude(self.iobs, nmode, self.psi)
+            self.psi = self.optics.exit_adjoint(self.psi)
+            self.obj, self.probe = psi_to_obj_probe(self.views_all, self.quad, self.psi,
+                                                    self.obj, self.probe, obj_inertia,
+                                                    probe_inertia)
+            if rec is not None:
+                self._record(AP, rec)
+            self.cycle += 1
+        if zero_phase_ramp:
+            self.zero_phase_ramp()
+
+    def _update_obj_probe(self, inertia, loops):
+        """ptyrs/src/engine/ptycho.rs:334-356 (no positions)"""
+        for _ in range(loops):
+            for o, p in ((inertia[0], None), (None, inertia[1])):
+                if o is not None or p is not None:
+                    self.obj, self.probe = psi_to_obj_probe(
+                        self.views_all, self.quad, self.psi, self.obj, self.probe, o, p)
+
+    def dm(self, ncycle, obj_inertia=0.01, probe_inertia=0.001, alpha=0.02, loop_obj_probe=1,
+           calc_llk=0, zero_phase_ramp=True):
+        """PyNX DM. ptyrs/src/engine/ptycho.rs:244-292"""
+        nmode, a = self.addr.nmode, alpha
+        refl = None
+        for ic in range(ncycle):
+            rec = self.llk() if llk_due(calc_llk, ic, ncycle) else None
+            po = self.calc_psi()
+            if ic == 0:
+                self.psi = po.copy()
+            else:
+                self.psi = po * (2.0 - a) - self.psi * (1.0 - a)
+                refl = self.psi.copy()
+            psi = self.optics.exit_forward(self.psi)
+            psi = apply_amplitude(self.iobs, nmode, psi)
+            self.psi = self.optics.exit_adjoint(psi)
+            if ic > 0:
+                self.psi = self.psi + (po - refl) * (1.0 - a)
+            self._update_obj_probe((obj_inertia, probe_inertia), loop_obj_probe)
+            if rec is not None:
+                self._record(DM, rec)
+                self.split_scale(1.0)
+            self.cycle += 1
+        if zero_phase_ramp:
+            self.zero_phase_ramp()
+
+    def raar(self, ncycle, obj_inertia=0.01, probe_inertia=0.001, beta=0.9, loop_obj_probe=1,
+             calc_llk=0, zero_phase_ramp=True):
+        """PyNX RAAR. ptyrs/src/engine/ptycho.rs:294-332"""
+        nmode, h = self.addr.nmode, 0.5 * beta
+        iterate = None
+        for ic in range(ncycle):
+            rec = self.llk() if llk_due(calc_llk, ic, ncycle) else None
+            po = self.calc_psi()
+            if ic == 0:
+                self.psi = po.copy()
+            else:
+                self.psi = self.psi * (1.0 - h) + (po * 2.0 - iterate) * h
+            iterate = self.psi.copy()
+            psi = self.optics.exit_forward(self.psi)
+            psi = apply_amplitude(self.iobs, nmode, psi)
+            self.psi = self.optics.exit_adjoint(psi)
+            self._update_obj_probe((obj_inertia, probe_inertia), loop_obj_probe)
+            if rec is not None:
+                self._record(RAAR, rec)
+                self.split_scale(1.0)
+            self.cycle += 1
+        if zero_phase_ramp:
+            self.zero_phase_ramp()
+
+    def ml(self, ncycle, update_object=True, update_probe=False, calc_llk=0,
+           zero_phase_ramp=True):
+        """PyNX ML Poisson conjugate gradient. ptyrs/src/engine/ptycho.rs:358-443"""
+        self.scale_obj_probe()
+        last = None
+        d = [np.zeros_like(self.obj), np.zeros_like(self.probe)]
+        for ic in range(ncycle):
+            rec = self.llk() if llk_due(calc_llk, ic, ncycle) else None
+            g = self.grads(update_object, update_probe)
+            beta = 0.0
+            if last is not None:
+                num = sum(float((np.conj(gi) * (gi - hi)).real.sum()) for gi, hi in zip(g, last))
+                den = sum(float((hi.real ** 2 + hi.imag ** 2).sum()) for hi in last)
+                b = num / den if den != 0 else math.nan
+                beta = b if (math.isfinite(b) and b >= 0.0) else 0.0
+            d = [di * beta - gi for di, gi in zip(d, g)]
+            num, den = self.cg_gamma(d[0], d[1])
+            gamma = num / den if den != 0 else math.inf
+            if math.isfinite(gamma):
+                self.obj = self.obj + d[0] * gamma
+                self.probe = self.probe + d[1] * gamma
+            else:
+                d = [-gi for gi in g]
+            last = g
+            if rec is not None:
+                self._record(ML, rec)
+            self.cycle += 1
+        if zero_phase_ramp:
+            self.zero_phase_ramp()
+
+    def rpie(self, nepoch, obj_step=1.0, probe_step=1.0, obj_alpha=0.1, probe_alpha=0.1,
+             batch_size=100, calc_llk=0, seed=7):
+        """rPIE after pty-chi with the overlap-normalised object step.
+        ptyrs/src/engine/ptycho.rs:445-499"""
+        if self.addr.nobj != 1:
+            raise ValueError("rPIE needs exactly one object mode")
+        if batch_size <= 0:
+            raise ValueError("rPIE batch_size must be >= 1")
+        for ic in range(nepoch):
+            for frames in random_batches(self.addr.nframe, batch_size, seed, self.cycle):
+                self.rpie_batch(frames, obj_step, probe_step, obj_alpha, probe_alpha)
+            if llk_due(calc_llk, ic, nepoch):
+                self._record(RPIE, self.llk())
+            self.cycle += 1
+
+    def rpie_delta(self, frames):
+        """The batch's `psi` and `Δ = IFT(ψ') − ψ` (shifted layout).
+        ptyrs/src/engine/ptycho.rs:518-546"""
+        nmode = self.addr.nmode
+        views = [v for f in frames for v in self.views_all[f * nmode:(f + 1) * nmode]]
+        psi = obj_probe_to_psi(self.addr, views, self.obj, self.probe, self.quad)
+        delta = self.optics.exit_forward(psi)
+        io = self.iobs[frames]
+        calc = np.sqrt(_mode_sum(delta, nmode))
+        r = np.sqrt(io + 1e-7) / (calc + 1e-7)
+        r = np.where(np.isnan(io), 1.0, r)
+        nb = len(frames)
+        delta = (delta.reshape(nb, nmode, *delta.shape[1:]) * r[:, None]).reshape(delta.shape)
+        delta = self.optics.exit_adjoint(delta) - psi
+        return views, psi, delta
+
+    def rpie_batch(self, frames, obj_step, probe_step, obj_alpha, probe_alpha):
+        """One rPIE batch from the batch-start arrays. ptyrs/src/engine/ptycho.rs:501-687"""
+        ny, nx = self.probe.shape[-2:]
+        nmode = self.addr.nmode
+        nprobe = self.probe.shape[0]
+        nbatch = len(frames)
+        views, psi, delta = self.rpie_delta(frames)
+        delta_nat = to_natural(delta)
+        w_nat = self.probe * self.quad  # W_m = P_m q, natural
+        w2_nat = self.probe.real ** 2 + self.probe.imag ** 2
+        d_obj = None
+        if obj_step is not None:
+            nyo, nxo = self.obj.shape[-2:]
+            ov = np.zeros((nprobe, nyo, nxo))
+            for (_, _, p, _, oy, ox) in views:
+                ov[p, oy:oy + ny, ox:ox + nx] += w2_nat[p]
+            dens = np.stack([ov[p, oy:oy + ny, ox:ox + nx] for (_, _, p, _, oy, ox) in views])
+            pmax = float(dens.reshape(nbatch, nmode, ny, nx).sum(axis=1).max())
+            a = obj_alpha
+            d_obj = np.zeros_like(self.obj)
+            for k, (_, _, p, _, oy, ox) in enumerate(views):
+                step = np.conj(w_nat[p]) / ((1.0 - a) * dens[k] + a * pmax) * delta_nat[k]
+                d_obj[0, oy:oy + ny, ox:ox + nx] += step
+        d_probe = None
+        if probe_step is not None:
+            a = probe_alpha
+            d_probe = np.zeros_like(self.probe)
+            for b in range(nbatch):
+                fv = views[b * nmode:(b + 1) * nmode]
+                oy, ox = fv[0][4], fv[0][5]
+                win = self.obj[0, oy:oy + ny, ox:ox + nx]
+                w2 = win.real ** 2 + win.imag ** 2
+                omax = float(w2.max())
+                wgt = np.conj(win * self.quad) / ((1.0 - a) * w2 + a * omax)
+                for m, v in enumerate(fv):
+                    d_probe[v[2]] += wgt * delta_nat[b * nmode + m]
+        if obj_step is not None:
+            self.obj = self.obj + d_obj * obj_step
+        if probe_step is not None:
+            self.probe = self.probe + d_probe * (probe_step / nbatch)
+
+
+def far_field_ptycho(counts, y, x, obj, probe, wavelength, distance, detector_pixel):
+    """`Ptycho::new`: the far field of the scan geometry. ptyrs/src/engine/ptycho.rs:230-233"""
+    frame = list(np.asarray(counts).shape[-2:])
+    optics = Optics.far_field(wavelength, distance, detector_pixel, frame)
+    return Ptycho(counts, y, x, obj, probe, optics)
