@@ -1,0 +1,182 @@
+"""Benchmark of the per-position reconstruction step on one MI355X.
+
+Workload (the headline configuration): rPIE, 256×256 probe with 4 incoherent Hermite–Gauss
+modes, 1024×1024 object, 32×32 raster of 1024 positions (24 px step, ±2 px jitter), Poisson
+data at 1e7 photons per pattern, batches of 128 positions. A step is one rPIE iteration (epoch)
+over all 1024 positions; `value` is positions·iterations per second of the complex-float
+(FP32) engine with the scan resident in HBM. The plain complex-double (FP64) engine runs the
+same epochs from the same start; `nrmse` compares the two objects after phase-ramp and
+global-phase alignment on the illuminated region.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+BATCH = 128
+N, M, NPOS = 256, 4, 1024
+
+
+def aligned_nrmse(a, b, weight):
+    """NRMSE of `a` against `b` after removing a linear phase ramp and a global phase
+    (least squares, weighted by `weight`)."""
+    w = weight / weight.sum()
+    r = a * np.conj(b)
+    # Phase ramp: weighted least-squares plane fit of the angle of a·conj(b).
+    ny, nx = r.shape[-2:]
+    yy, xx = np.mgrid[:ny, :nx]
+    ph = np.angle(r[0])
+    A = np.stack([np.ones(ny * nx), yy.ravel(), xx.ravel()], axis=1) * np.sqrt(w.ravel())[:, None]
+    coef = np.linalg.lstsq(A, ph.ravel() * np.sqrt(w.ravel()), rcond=None)[0]
+    a2 = a * np.exp(-1j * (coef[1] * yy + coef[2] * xx))
+    g = (w * a2 * np.conj(b)).sum()
+    a2 = a2 * np.exp(-1j * np.angle(g))
+    return float(np.sqrt((w * np.abs(a2 - b) ** 2).sum() / (w * np.abs(b) ** 2).sum()))
+
+
+def illumination(corners, probe, shape):
+    ov = np.zeros(shape[-2:])
+    p2 = (np.abs(probe) ** 2).sum(axis=0)
+    for oy, ox in corners:
+        ov[oy:oy + N, ox:ox + N] += p2
+    return ov
+
+
+def cpu_baseline(d, sample=256):
+    """The FP64 oracle (numpy, one thread for its FFTs and loops) on one batch of `sample`
+    positions of the same workload: positions per second of one rPIE batch."""
+    from oracle.ptycho import far_field_ptycho
+    o = far_field_ptycho(d["counts"].astype(np.float64), d["y"], d["x"], d["start_obj"],
+                         d["start_probe"], d["wavelength"], d["distance"], d["detector_pixel"])
+    frames = list(range(sample))
+    t0 = time.perf_counter()
+    o.rpie_batch(frames, 1.0, 1.0, 0.1, 0.1)
+    dt = time.perf_counter() - t0
+    return dict(value=sample / dt, unit="positions·iter/s", cores=1, kind="port",
+                sample=f"one rPIE batch of {sample} positions at 256²×4 modes, 1024² object "
+                       f"(numpy FP64 oracle, {dt:.1f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps64", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every GPU box of this build has one, and the "
+              "reconstruction step is not sharded here (see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import ptyrs_amd as pa
+    from ptyrs_amd import sim
+
+    t0 = time.perf_counter()
+    d = sim.scan("headline")
+    print(f"[bench] synthetic scan ready in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    scan = pa.Scan(d["counts"], d["y"], d["x"], d["wavelength"], d["distance"],
+                   d["detector_pixel"])
+    params = dict(batch_size=BATCH, obj_step=1.0, probe_step=1.0, obj_alpha=0.1, probe_alpha=0.1,
+                  seed=7)
+
+    results = {}
+    for dtype, steps in (("c64", a.steps), ("c128", a.steps64)):
+        g = pa.GpuPtycho(scan, d["start_obj"], d["start_probe"], dtype=dtype)
+        g.rpie(a.warmup, **params)
+        g.synchronize()
+        if dtype == "c64":
+            g.profile(True)
+        t = time.perf_counter()
+        g.rpie(steps, **params)
+        g.synchronize()
+        dt = time.perf_counter() - t
+        stats = g.kernel_stats() if dtype == "c64" else None
+        results[dtype] = dict(engine=g, seconds=dt, steps=steps, stats=stats)
+        print(f"[bench] {dtype}: {steps} epochs in {dt:.3f} s", file=sys.stderr, flush=True)
+
+    # Same number of epochs from the same start for the NRMSE comparison.
+    g32, g64 = results["c64"]["engine"], results["c128"]["engine"]
+    extra = a.steps - a.steps64
+    if extra > 0:
+        g64.rpie(extra, **params)
+    o32, o64 = g32.object, g64.object
+    weight = illumination(g64.corners, g64.probe, o64.shape)
+    nrmse_obj = aligned_nrmse(o32, o64, weight)
+    p32, p64 = g32.probe, g64.probe
+    nrmse_probe = float(np.linalg.norm(p32 - p64) / np.linalg.norm(p64))
+    llk32, llk64 = g32.llk()["poisson"], g64.llk()["poisson"]
+
+    r32, r64 = results["c64"], results["c128"]
+    pos32 = NPOS * r32["steps"] / r32["seconds"]
+    pos64 = NPOS * r64["steps"] / r64["seconds"]
+
+    # Dominant kernel: the largest total time under HIP events over the timed region.
+    stats = r32["stats"]
+    kern = max(stats, key=lambda k: stats[k][1])
+    launches, ms = stats[kern]
+    c = 8  # bytes per complex64
+    n = N * N
+    per_frame = {
+        "pass_a": n * c + M * n * c,                 # O window in, rows-transformed Ψ out
+        "pass_b": 2 * M * n * c + 4 * n,             # Ψ in and out, float counts in
+        "pass_c": 2 * M * n * c + n * c,             # Ψ' in, Δ out, O window in
+        "obj_update": M * n * c,                     # Δ in (object read-modify-write amortised)
+    }
+    frames_per_launch = NPOS * r32["steps"] / max(launches, 1)
+    alg_bytes = per_frame.get(kern, 0) * frames_per_launch
+    avg_s = ms / 1e3 / max(launches, 1)
+    achieved = alg_bytes / avg_s / 1e9 if alg_bytes else None
+
+    out = {
+        "metric": "positions/sec/iter at 256² probe ×4 modes: FP32 vs FP64 plain at fixed NRMSE",
+        "value": round(pos32, 1),
+        "unit": "positions·iter/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * r32["seconds"] / r32["steps"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": "rPIE epoch, 256x256 probe x 4 modes, 1024 positions (32x32 raster), "
+                               "1024x1024 object, batch 128, 1e7 photons/pattern",
+                   "global_batch": BATCH, "positions": NPOS, "probe": [M, N, N],
+                   "object": [1024, 1024]},
+        "fp64": {"value": round(pos64, 1), "ms_per_step": round(1e3 * r64["seconds"] / r64["steps"], 3),
+                 "steps": r64["steps"]},
+        "speedup_fp32_vs_fp64": round(pos32 / pos64, 3),
+        "nrmse": {"object": nrmse_obj, "probe": nrmse_probe, "epochs": a.warmup + a.steps,
+                  "llk_poisson_c64": llk32, "llk_poisson_c128": llk64},
+        "kernels_ms_per_step": {k: round(v[1] / r32["steps"], 4) for k, v in stats.items()},
+        "roofline": {"kernel": kern, "bound": "hbm",
+                     "achieved": round(achieved, 1) if achieved else None,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(achieved * 1e9 / HBM_PEAK, 4) if achieved else None,
+                     "traffic": None,
+                     "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_s * 1e3, 4)},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(d)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,152 @@
+/* ptyrs_hip: the per-position ptychographic reconstruction step on an AMD Instinct MI355X
+ * (gfx950), behind a C ABI.
+ *
+ * It replaces the device layer of ptyrs's GPU engine: `crate::gpu` (checked kernel launches,
+ * ptyrs/src/gpu/mod.rs:1-492), `crate::fft` (cuFFT plans, ptyrs/src/fft/mod.rs:114+) and the
+ * cuda-core/cuda-async runtime under `ptyrs::engine::GpuPtycho`
+ * (ptyrs/src/engine/gpu/mod.rs:597-912). Each entry point below names the reference interface
+ * it stands for. The engine owns every device buffer; one engine is driven from one thread.
+ *
+ * Arrays are row-major. Complex arrays are interleaved (re, im) doubles, as numpy complex128.
+ * Counts are [nframe, ny, nx] in detector order (zero frequency at [ny/2, nx/2]).
+ *
+ * Status codes: PTYRS_OK; PTYRS_EINVAL (bad input; nothing changed, as
+ * EngineError::Core/Invalid); PTYRS_EDEVICE (a device call failed part-way; the engine is then
+ * aborted, ptyrs/src/engine/gpu/gate.rs:14-71); PTYRS_EABORTED (refused: an earlier call
+ * failed part-way; downloads still run, as EngineError::Aborted, ptyrs/src/engine/mod.rs:81-85).
+ * ptyrs_last_error() gives the message of the last failure on this thread.
+ */
+#ifndef PTYRS_HIP_H
+#define PTYRS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PTYRS_OK = 0,
+  PTYRS_EINVAL = 1,
+  PTYRS_EDEVICE = 2,
+  PTYRS_EABORTED = 3,
+};
+
+/* Exit transform kinds (ptyrs/src/engine/propagate.rs:301-366). */
+enum { PTYRS_FOURIER = 0, PTYRS_ANGULAR_SPECTRUM = 1, PTYRS_FRESNEL = 2 };
+
+/* Element types of the engine's arithmetic. */
+enum { PTYRS_C64 = 0, PTYRS_C128 = 1 };
+
+/* Algorithms of LLK records (ptyrs/src/engine/ptycho.rs:22-31). */
+enum { PTYRS_AP = 0, PTYRS_DM = 1, PTYRS_RAAR = 2, PTYRS_RPIE = 3, PTYRS_ML = 4, PTYRS_BH = 5,
+       PTYRS_LSQML = 6 };
+
+/* A far-field scan, as ptyrs::base::Scan (ptyrs/src/base/scan.rs, ptyrs-py/src/lib.rs:433-463).
+ * Exactly one of counts_u16 (65535 = masked, ptyrs/src/base/counts.rs:3-7) and counts_f32
+ * (a negative value is masked; values above 65534 are kept, which the u16 path refuses) is
+ * non-null. y, x are the sample positions in metres. */
+typedef struct {
+  size_t nframe, ny, nx;
+  const uint16_t* counts_u16;
+  const float* counts_f32;
+  const double* y;
+  const double* x;
+  double wavelength, distance, detector_pixel;
+} ptyrs_scan;
+
+/* The exit-wave model (ptyrs::engine::Optics, ptyrs/src/engine/propagate.rs:24-186):
+ * free_space = 0 takes the far field of the scan geometry (Optics::far_field); free_space = 1
+ * takes pty-chi's planar model of one Wavefield (Optics::free_space) of wavelength, object
+ * pixel [pixel_y, pixel_x] and signed distance (INFINITY for the far field). */
+typedef struct {
+  int free_space;
+  double wavelength, pixel_y, pixel_x, distance;
+} ptyrs_optics;
+
+/* GpuOptions (ptyrs/src/engine/gpu/mod.rs:61-108) plus the element type. stack_size = 0
+ * lets the engine choose. */
+typedef struct {
+  size_t stack_size;
+  int graph;
+  size_t llk_slots;
+  int dtype;
+} ptyrs_gpu_options;
+
+/* RpieParams (ptyrs/src/engine/ptycho.rs:145-186); a step with has_* = 0 leaves that array
+ * fixed. Defaults: steps 1.0, alphas 0.1, batch_size 100, calc_llk 0, seed 7. */
+typedef struct {
+  int has_obj_step;
+  double obj_step;
+  int has_probe_step;
+  double probe_step;
+  double obj_alpha, probe_alpha;
+  size_t batch_size;
+  size_t calc_llk;
+  uint64_t seed;
+} ptyrs_rpie_params;
+
+/* One LLK record (LlkRecord, ptyrs/src/engine/ptycho.rs:33-41), normalised by nb_obs
+ * (Llk::normalised, ptyrs/src/engine/ops.rs:268-276). */
+typedef struct {
+  size_t cycle;
+  int algorithm;
+  double poisson, gaussian, euclidian, nb_photons_calc;
+} ptyrs_llk_record;
+
+typedef struct ptyrs_engine ptyrs_engine;
+
+/* GpuPtycho::with_optics (ptyrs/src/engine/gpu/mod.rs:619-760): object [nobj, nyo, nxo] and
+ * probe [nprobe, ny, nx] complex128; positions go through PyNX preprocessing and become view
+ * corners. optics may be NULL (far field of the scan). device is the HIP device ordinal. */
+int ptyrs_engine_create(const ptyrs_scan* scan, const ptyrs_optics* optics, const double* obj,
+                        size_t nobj, size_t nyo, size_t nxo, const double* probe, size_t nprobe,
+                        const ptyrs_gpu_options* options, int device, ptyrs_engine** out);
+
+/* GpuPtycho::rpie (ptyrs/src/engine/gpu/rpie.rs:31-66): nepoch epochs of rPIE. */
+int ptyrs_engine_rpie(ptyrs_engine* e, const ptyrs_rpie_params* params, size_t nepoch);
+
+/* GpuPtycho::llk (ptyrs/src/engine/gpu/mod.rs:853-868): normalised LLK of the current state. */
+int ptyrs_engine_llk(ptyrs_engine* e, ptyrs_llk_record* out);
+
+/* GpuPtycho::object / probe (ptyrs/src/engine/gpu/mod.rs:762-815): complex128 downloads of
+ * len doubles (2·nobj·nyo·nxo, 2·nprobe·ny·nx). Allowed on an aborted engine. */
+int ptyrs_engine_object(ptyrs_engine* e, double* out, size_t len);
+int ptyrs_engine_probe(ptyrs_engine* e, double* out, size_t len);
+
+/* GpuPtycho::set_object (ptyrs/src/engine/gpu/mod.rs:816-840) and its probe counterpart. */
+int ptyrs_engine_set_object(ptyrs_engine* e, const double* obj, size_t len);
+int ptyrs_engine_set_probe(ptyrs_engine* e, const double* probe, size_t len);
+
+/* GpuPtycho::history (ptyrs/src/engine/gpu/mod.rs:842-846): copies up to cap records and sets
+ * *n to the number held. */
+int ptyrs_engine_history(ptyrs_engine* e, ptyrs_llk_record* out, size_t cap, size_t* n);
+
+/* GpuPtycho::cycle (ptyrs/src/engine/gpu/mod.rs:848-851). */
+int ptyrs_engine_cycle(ptyrs_engine* e, size_t* cycle);
+
+/* The view corners [nframe, 2] (oy, ox) the engine built (AddrTable, ptyrs/src/base/addr.rs). */
+int ptyrs_engine_corners(ptyrs_engine* e, int64_t* out, size_t len);
+
+/* GpuPtycho::synchronize (ptyrs/src/engine/gpu/mod.rs:870-881). */
+int ptyrs_engine_synchronize(ptyrs_engine* e);
+
+/* Kernel timing with HIP events on the engine's stream, for measurement: enable = 1 starts
+ * recording every launch, 0 stops. ptyrs_engine_kernel_stats reports, for kernel name `name`
+ * (as listed by ptyrs_kernel_names), the launches recorded and their summed duration in ms.
+ * No reference counterpart (the reference times with nsys/ncu, DESIGN.md:47). */
+int ptyrs_engine_profile(ptyrs_engine* e, int enable);
+int ptyrs_engine_kernel_stats(ptyrs_engine* e, const char* name, size_t* launches, double* total_ms);
+const char* ptyrs_kernel_names(void);
+
+void ptyrs_engine_destroy(ptyrs_engine* e);
+
+/* Message of the last failed call on this thread; returns its length. */
+size_t ptyrs_last_error(char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PTYRS_HIP_H */

@@ -1,0 +1,386 @@
+"""ptyrs_amd: the per-position ptychographic reconstruction step on an MI355X, with the Python
+surface of ptyrs-py's GPU engine (ptyrs-py/src/gpu.rs, ptyrs-py/src/lib.rs).
+
+The work runs in libptyrs_hip.so (hand-written HIP kernels for gfx950) through the C ABI of
+include/ptyrs_hip.h. There is no CPU fallback: without the built library every engine call
+raises.
+
+    scan = Scan(counts, y, x, wavelength, distance, detector_pixel)
+    g = GpuPtycho(scan, obj, probe, dtype="c64")
+    g.rpie(10, batch_size=64, calc_llk=1)
+    g.object, g.probe, g.history, g.llk()
+
+Errors follow ptyrs-py (ptyrs-py/src/lib.rs:27-50, :52-108): bad input is a ValueError, a
+device failure a RuntimeError, an unknown keyword a TypeError.
+"""
+
+import ctypes
+import math
+import os
+import pathlib
+
+import numpy as np
+
+__all__ = ["Scan", "Wavefield", "Optics", "GpuPtycho", "library", "MASKED"]
+
+MASKED = 65535
+_HERE = pathlib.Path(__file__).resolve().parent
+_LIB_PATH = _HERE / "libptyrs_hip.so"
+
+_OK, _EINVAL, _EDEVICE, _EABORTED = 0, 1, 2, 3
+_C64, _C128 = 0, 1
+_ALGORITHMS = ["ap", "dm", "raar", "rpie", "ml", "bh", "lsqml"]
+
+
+class _ScanC(ctypes.Structure):
+    _fields_ = [("nframe", ctypes.c_size_t), ("ny", ctypes.c_size_t), ("nx", ctypes.c_size_t),
+                ("counts_u16", ctypes.POINTER(ctypes.c_uint16)),
+                ("counts_f32", ctypes.POINTER(ctypes.c_float)),
+                ("y", ctypes.POINTER(ctypes.c_double)), ("x", ctypes.POINTER(ctypes.c_double)),
+                ("wavelength", ctypes.c_double), ("distance", ctypes.c_double),
+                ("detector_pixel", ctypes.c_double)]
+
+
+class _OpticsC(ctypes.Structure):
+    _fields_ = [("free_space", ctypes.c_int), ("wavelength", ctypes.c_double),
+                ("pixel_y", ctypes.c_double), ("pixel_x", ctypes.c_double),
+                ("distance", ctypes.c_double)]
+
+
+class _OptionsC(ctypes.Structure):
+    _fields_ = [("stack_size", ctypes.c_size_t), ("graph", ctypes.c_int),
+                ("llk_slots", ctypes.c_size_t), ("dtype", ctypes.c_int)]
+
+
+class _RpieC(ctypes.Structure):
+    _fields_ = [("has_obj_step", ctypes.c_int), ("obj_step", ctypes.c_double),
+                ("has_probe_step", ctypes.c_int), ("probe_step", ctypes.c_double),
+                ("obj_alpha", ctypes.c_double), ("probe_alpha", ctypes.c_double),
+                ("batch_size", ctypes.c_size_t), ("calc_llk", ctypes.c_size_t),
+                ("seed", ctypes.c_uint64)]
+
+
+class _RecordC(ctypes.Structure):
+    _fields_ = [("cycle", ctypes.c_size_t), ("algorithm", ctypes.c_int),
+                ("poisson", ctypes.c_double), ("gaussian", ctypes.c_double),
+                ("euclidian", ctypes.c_double), ("nb_photons_calc", ctypes.c_double)]
+
+
+_lib = None
+
+# (name, argtypes, restype) of every entry point of include/ptyrs_hip.h.
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_SZ = ctypes.c_size_t
+_SIGNATURES = [
+    ("ptyrs_engine_create", [ctypes.POINTER(_ScanC), ctypes.POINTER(_OpticsC), _D, _SZ, _SZ, _SZ,
+                             _D, _SZ, ctypes.POINTER(_OptionsC), ctypes.c_int,
+                             ctypes.POINTER(_P)], ctypes.c_int),
+    ("ptyrs_engine_rpie", [_P, ctypes.POINTER(_RpieC), _SZ], ctypes.c_int),
+    ("ptyrs_engine_llk", [_P, ctypes.POINTER(_RecordC)], ctypes.c_int),
+    ("ptyrs_engine_object", [_P, _D, _SZ], ctypes.c_int),
+    ("ptyrs_engine_probe", [_P, _D, _SZ], ctypes.c_int),
+    ("ptyrs_engine_set_object", [_P, _D, _SZ], ctypes.c_int),
+    ("ptyrs_engine_set_probe", [_P, _D, _SZ], ctypes.c_int),
+    ("ptyrs_engine_history", [_P, ctypes.POINTER(_RecordC), _SZ, ctypes.POINTER(_SZ)],
+     ctypes.c_int),
+    ("ptyrs_engine_cycle", [_P, ctypes.POINTER(_SZ)], ctypes.c_int),
+    ("ptyrs_engine_corners", [_P, ctypes.POINTER(ctypes.c_int64), _SZ], ctypes.c_int),
+    ("ptyrs_engine_synchronize", [_P], ctypes.c_int),
+    ("ptyrs_engine_profile", [_P, ctypes.c_int], ctypes.c_int),
+    ("ptyrs_engine_kernel_stats", [_P, ctypes.c_char_p, ctypes.POINTER(_SZ), _D], ctypes.c_int),
+    ("ptyrs_kernel_names", [], ctypes.c_char_p),
+    ("ptyrs_engine_destroy", [_P], None),
+    ("ptyrs_last_error", [ctypes.c_char_p, _SZ], _SZ),
+]
+
+
+def library():
+    """The loaded libptyrs_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        path = os.environ.get("PTYRS_HIP_LIB", str(_LIB_PATH))
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: build it with `python -m ptyrs_amd.build`")
+        lib = ctypes.CDLL(path)
+        for name, args, res in _SIGNATURES:
+            f = getattr(lib, name)
+            f.argtypes, f.restype = args, res
+        _lib = lib
+    return _lib
+
+
+def _check(status):
+    if status == _OK:
+        return
+    lib = library()
+    buf = ctypes.create_string_buffer(4096)
+    lib.ptyrs_last_error(buf, len(buf))
+    msg = buf.value.decode(errors="replace")
+    if status == _EINVAL:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def _overlay(defaults, kw, what):
+    """Each keyword replaces the default of the field of the same name; an unknown keyword is a
+    TypeError (ptyrs-py/src/lib.rs:52-108)."""
+    out = dict(defaults)
+    unknown = [k for k in kw if k not in defaults]
+    if unknown:
+        raise TypeError(f"unknown parameters {unknown} for {what}")
+    out.update(kw)
+    return out
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+class Scan:
+    """A far-field scan (ptyrs-py/src/lib.rs:430-463): counts `[nframe, ny, nx]` in detector
+    order, uint16 with 65535 masked or float with negative values masked (float counts may exceed
+    65534, which the uint16 path cannot hold); positions `y`, `x` in metres; geometry in metres."""
+
+    def __init__(self, counts, y, x, wavelength, distance, detector_pixel):
+        c = np.asarray(counts)
+        if c.ndim != 3:
+            raise TypeError("counts must be a 3-D uint16 or float array")
+        if c.dtype == np.uint16:
+            self.counts = np.ascontiguousarray(c)
+        elif np.issubdtype(c.dtype, np.floating):
+            if not np.all(np.isfinite(c)):
+                raise ValueError("counts must be finite")
+            self.counts = np.ascontiguousarray(c, dtype=np.float32)
+        else:
+            raise TypeError("counts must be a 3-D uint16 or float array")
+        self.y = np.ascontiguousarray(y, dtype=np.float64)
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        if self.y.shape != (c.shape[0],) or self.x.shape != (c.shape[0],):
+            raise ValueError(f"{self.y.size} y and {self.x.size} x positions for {c.shape[0]} frames")
+        for name, v in (("wavelength", wavelength), ("distance", distance),
+                        ("detector_pixel", detector_pixel)):
+            if not (math.isfinite(v) and v > 0):
+                raise ValueError(f"{name} must be finite and > 0, got {v}")
+        self.wavelength, self.distance, self.detector_pixel = wavelength, distance, detector_pixel
+
+    @property
+    def frame(self):
+        return list(self.counts.shape[1:])
+
+    def _c(self):
+        s = _ScanC()
+        s.nframe, s.ny, s.nx = self.counts.shape
+        if self.counts.dtype == np.uint16:
+            s.counts_u16 = _ptr(self.counts, ctypes.c_uint16)
+        else:
+            s.counts_f32 = _ptr(self.counts, ctypes.c_float)
+        s.y, s.x = _ptr(self.y, ctypes.c_double), _ptr(self.x, ctypes.c_double)
+        s.wavelength, s.distance, s.detector_pixel = (self.wavelength, self.distance,
+                                                      self.detector_pixel)
+        return s
+
+
+class Wavefield:
+    """One free-space propagation (ptyrs-py/src/cpu.rs:44-57): `[ny, nx]` planes (both even),
+    `wavelength`, source-plane `pixel` `[py, px]` and signed `distance` in metres."""
+
+    def __init__(self, shape, wavelength, pixel, distance):
+        if len(shape) != 2 or any(int(s) <= 0 or int(s) % 2 for s in shape):
+            raise ValueError(f"plane shape {shape} must be even and non-zero")
+        if not (math.isfinite(wavelength) and wavelength > 0):
+            raise ValueError("wavelength must be finite and > 0")
+        if not all(math.isfinite(p) and p > 0 for p in pixel):
+            raise ValueError("pixel must be finite and > 0")
+        if math.isnan(distance) or distance == 0.0 or distance == -math.inf:
+            raise ValueError("distance must be non-zero and not -inf")
+        self.shape = [int(s) for s in shape]
+        self.wavelength, self.pixel, self.distance = wavelength, list(pixel), distance
+
+
+class Optics:
+    """The exit-wave model (ptyrs-py/src/cpu.rs:77-135): PyNX's far field or pty-chi's
+    planar free-space model of one Wavefield."""
+
+    def __init__(self, free_space, wavefield=None, far=None):
+        self._free_space, self.wavefield, self._far = free_space, wavefield, far
+
+    @staticmethod
+    def far_field(wavelength, distance, detector_pixel, frame):
+        return Optics(False, far=(wavelength, distance, detector_pixel, list(frame)))
+
+    @staticmethod
+    def free_space(w):
+        return Optics(True, wavefield=w)
+
+    def _c(self, scan):
+        o = _OpticsC()
+        if self._free_space:
+            w = self.wavefield
+            if w.shape != scan.frame:
+                raise ValueError(f"propagator plane {w.shape} differs from frame {scan.frame}")
+            o.free_space, o.wavelength = 1, w.wavelength
+            o.pixel_y, o.pixel_x, o.distance = w.pixel[0], w.pixel[1], w.distance
+            return o
+        lam, z, dpix, frame = self._far
+        if frame != scan.frame:
+            raise ValueError(f"optics frame {frame} differs from the scan's {scan.frame}")
+        # The far field of a geometry is the scan's own when they agree.
+        o.free_space = 0
+        return o, (lam, z, dpix)
+
+
+_GPU_OPTIONS = dict(stack_size=None, graph=True, llk_slots=8, fused=True, psi_f16=False,
+                    dtype="c64")
+_RPIE_PARAMS = dict(obj_step=1.0, probe_step=1.0, obj_alpha=0.1, probe_alpha=0.1,
+                    batch_size=100, calc_llk=0, seed=7)
+
+
+class GpuPtycho:
+    """The GPU reconstruction state of a scan (ptyrs-py/src/gpu.rs:24-267), on HIP device
+    `device`, under `optics` in place of the far field of the scan's geometry if given;
+    keywords are GpuOptions fields (stack_size, graph, llk_slots, fused, psi_f16) plus `dtype`,
+    "c64" (complex float, the default) or "c128" (complex double, the plain FP64 path).
+    `object` is complex `[nobj, nyo, nxo]`, `probe` complex `[nprobe, ny, nx]`."""
+
+    def __init__(self, scan, object, probe, device=0, optics=None, **options):
+        opts = _overlay(_GPU_OPTIONS, options, "GpuOptions")
+        if opts["dtype"] not in ("c64", "c128"):
+            raise ValueError(f"dtype must be \"c64\" or \"c128\", got {opts['dtype']!r}")
+        if opts["psi_f16"]:
+            raise ValueError("psi_f16 is not supported by this engine")
+        lib = library()
+        obj = np.ascontiguousarray(object, dtype=np.complex128)
+        if obj.ndim != 3:
+            raise TypeError("object must be complex128 [nmode, nyo, nxo]")
+        pr = np.ascontiguousarray(probe, dtype=np.complex128)
+        if pr.ndim != 3:
+            raise TypeError("probe must be complex128 [nmode, ny, nx]")
+        if list(pr.shape[1:]) != scan.frame:
+            raise ValueError(f"probe plane {list(pr.shape[1:])} differs from frame {scan.frame}")
+        sc = scan._c()
+        oc = None
+        if optics is not None:
+            r = optics._c(scan)
+            if isinstance(r, tuple):
+                oc, (lam, z, dpix) = r
+                sc.wavelength, sc.distance, sc.detector_pixel = lam, z, dpix
+            else:
+                oc = r
+        oo = _OptionsC()
+        oo.stack_size = opts["stack_size"] or 0
+        oo.graph = int(bool(opts["graph"]))
+        oo.llk_slots = opts["llk_slots"]
+        oo.dtype = _C64 if opts["dtype"] == "c64" else _C128
+        h = ctypes.c_void_p()
+        _check(lib.ptyrs_engine_create(
+            ctypes.byref(sc), ctypes.byref(oc) if oc is not None else None,
+            _ptr(obj.view(np.float64), ctypes.c_double), obj.shape[0], obj.shape[1], obj.shape[2],
+            _ptr(pr.view(np.float64), ctypes.c_double), pr.shape[0], ctypes.byref(oo), device,
+            ctypes.byref(h)))
+        self._h = h
+        self._obj_shape, self._probe_shape = obj.shape, pr.shape
+        self.nframe = scan.counts.shape[0]
+        self.dtype = opts["dtype"]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.ptyrs_engine_destroy(h)
+            self._h = None
+
+    def rpie(self, nepoch, **params):
+        """`nepoch` rPIE epochs; keywords are RpieParams fields (ptyrs/src/engine/ptycho.rs:145-186);
+        a step of None leaves that array fixed."""
+        p = _overlay(_RPIE_PARAMS, params, "RpieParams")
+        c = _RpieC()
+        c.has_obj_step = p["obj_step"] is not None
+        c.obj_step = p["obj_step"] if p["obj_step"] is not None else 0.0
+        c.has_probe_step = p["probe_step"] is not None
+        c.probe_step = p["probe_step"] if p["probe_step"] is not None else 0.0
+        c.obj_alpha, c.probe_alpha = p["obj_alpha"], p["probe_alpha"]
+        if int(p["batch_size"]) < 0 or int(p["calc_llk"]) < 0:
+            raise ValueError("batch_size and calc_llk must be >= 0")
+        c.batch_size, c.calc_llk, c.seed = int(p["batch_size"]), int(p["calc_llk"]), int(p["seed"])
+        _check(library().ptyrs_engine_rpie(self._h, ctypes.byref(c), int(nepoch)))
+
+    def llk(self):
+        """Normalised PyNX LLK of the current object and probe."""
+        r = _RecordC()
+        _check(library().ptyrs_engine_llk(self._h, ctypes.byref(r)))
+        return dict(poisson=r.poisson, gaussian=r.gaussian, euclidian=r.euclidian,
+                    nb_photons_calc=r.nb_photons_calc)
+
+    @property
+    def object(self):
+        out = np.empty(self._obj_shape, dtype=np.complex128)
+        _check(library().ptyrs_engine_object(self._h, _ptr(out.view(np.float64), ctypes.c_double),
+                                             out.size * 2))
+        return out
+
+    @property
+    def probe(self):
+        out = np.empty(self._probe_shape, dtype=np.complex128)
+        _check(library().ptyrs_engine_probe(self._h, _ptr(out.view(np.float64), ctypes.c_double),
+                                            out.size * 2))
+        return out
+
+    def set_object(self, obj):
+        a = np.ascontiguousarray(obj, dtype=np.complex128)
+        if a.shape != self._obj_shape:
+            raise ValueError(f"object {a.shape} differs from {self._obj_shape}")
+        _check(library().ptyrs_engine_set_object(self._h, _ptr(a.view(np.float64), ctypes.c_double),
+                                                 a.size * 2))
+
+    def set_probe(self, probe):
+        a = np.ascontiguousarray(probe, dtype=np.complex128)
+        if a.shape != self._probe_shape:
+            raise ValueError(f"probe {a.shape} differs from {self._probe_shape}")
+        _check(library().ptyrs_engine_set_probe(self._h, _ptr(a.view(np.float64), ctypes.c_double),
+                                                a.size * 2))
+
+    @property
+    def history(self):
+        """LLK records as columns: cycle, algorithm and the Llk fields (ptyrs-py/src/lib.rs:395-411)."""
+        lib = library()
+        n = ctypes.c_size_t()
+        _check(lib.ptyrs_engine_history(self._h, None, 0, ctypes.byref(n)))
+        recs = (_RecordC * max(n.value, 1))()
+        _check(lib.ptyrs_engine_history(self._h, recs, n.value, ctypes.byref(n)))
+        rs = recs[:n.value]
+        return dict(cycle=np.array([r.cycle for r in rs], dtype=np.int64),
+                    algorithm=[_ALGORITHMS[r.algorithm] for r in rs],
+                    poisson=np.array([r.poisson for r in rs]),
+                    gaussian=np.array([r.gaussian for r in rs]),
+                    euclidian=np.array([r.euclidian for r in rs]),
+                    nb_photons_calc=np.array([r.nb_photons_calc for r in rs]))
+
+    @property
+    def cycle(self):
+        c = ctypes.c_size_t()
+        _check(library().ptyrs_engine_cycle(self._h, ctypes.byref(c)))
+        return c.value
+
+    @property
+    def corners(self):
+        out = np.empty((self.nframe, 2), dtype=np.int64)
+        _check(library().ptyrs_engine_corners(self._h, _ptr(out, ctypes.c_int64), out.size))
+        return out
+
+    def synchronize(self):
+        _check(library().ptyrs_engine_synchronize(self._h))
+
+    def profile(self, enable=True):
+        """Times every launch with HIP events on the engine's stream from now on."""
+        _check(library().ptyrs_engine_profile(self._h, int(bool(enable))))
+
+    def kernel_stats(self):
+        """{kernel: (launches, total ms)} of the launches timed since `profile()`."""
+        lib = library()
+        out = {}
+        for name in lib.ptyrs_kernel_names().decode().split(","):
+            n, ms = ctypes.c_size_t(), ctypes.c_double()
+            _check(lib.ptyrs_engine_kernel_stats(self._h, name.encode(), ctypes.byref(n),
+                                                 ctypes.byref(ms)))
+            out[name] = (n.value, ms.value)
+        return out

@@ -1,0 +1,278 @@
+// The C ABI of include/ptyrs_hip.h: scan preparation on the host, engine construction for the
+// frame side, status codes, and the abort gate.
+#include <cstring>
+#include <string>
+
+#include "engine.hpp"
+
+namespace ptyrs {
+
+// One instantiation per supported frame side (engine_inst.hip, built once per side).
+#define PTYRS_SIDES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(256) X(512)
+#define PTYRS_DECL(n)                                                                       \
+  EngineBase* make_engine_##n(int dtype, const Prepared& pre, const double* obj, size_t nobj, \
+                              size_t nyo, size_t nxo, const double* probe, size_t nprobe,     \
+                              int device);
+PTYRS_SIDES(PTYRS_DECL)
+
+EngineBase* make_engine(int dtype, const Prepared& pre, const double* obj, size_t nobj,
+                        size_t nyo, size_t nxo, const double* probe, size_t nprobe, int device) {
+#define PTYRS_CASE(n) \
+  case n: return make_engine_##n(dtype, pre, obj, nobj, nyo, nxo, probe, nprobe, device);
+  switch (pre.ny) {
+    PTYRS_SIDES(PTYRS_CASE)
+    default:
+      throw Invalid("frame side " + std::to_string(pre.ny) +
+                    " not supported on the GPU (16, 24, 32, 48, 64, 96, 128, 256, 512)");
+  }
+#undef PTYRS_CASE
+}
+
+thread_local std::string last_error;
+
+// Counts to the device's DFT index order, positions to view corners, and the exit model.
+// Follows ptyrs/src/engine/gpu/mod.rs:635-760 (shape checks, pynx_preprocess, AddrTable) and
+// ptyrs/src/engine/obs.rs:13-35 (the shifted observed intensities).
+Prepared prepare(const ptyrs_scan& s, const ptyrs_optics* optics, size_t nprobe, size_t nobj,
+                 size_t nyo, size_t nxo) {
+  if (s.ny != s.nx) throw Invalid("frames must be square");
+  if (s.ny == 0 || s.ny % 2) throw Invalid("frame sides must be even and non-zero");
+  if ((s.counts_u16 == nullptr) == (s.counts_f32 == nullptr))
+    throw Invalid("give exactly one of counts_u16 and counts_f32");
+  if (s.nframe && (s.y == nullptr || s.x == nullptr)) throw Invalid("positions missing");
+  if (nprobe == 0 || nobj == 0) throw Invalid("nprobe and nobj must be >= 1");
+  if (nyo < s.ny || nxo < s.nx) throw Invalid("object smaller than frame");
+  Prepared p;
+  p.nframe = s.nframe;
+  p.ny = s.ny;
+  p.nx = s.nx;
+  const int N = int(s.ny);
+  if (optics == nullptr || !optics->free_space) {
+    if (!(s.wavelength > 0 && s.distance > 0 && s.detector_pixel > 0) ||
+        !std::isfinite(s.wavelength * s.distance / s.detector_pixel))
+      throw Invalid("wavelength, distance and detector pixel must be finite and > 0");
+    p.exit = far_field(N, N, s.wavelength, s.distance, s.detector_pixel);
+  } else {
+    p.exit = free_space(N, N, optics->wavelength, optics->pixel_y, optics->pixel_x,
+                        optics->distance);
+    if (p.exit.kind == PTYRS_ANGULAR_SPECTRUM)
+      throw Invalid("an angular-spectrum exit transform is not run on the GPU yet");
+  }
+  std::vector<double> y(s.y, s.y + s.nframe), x(s.x, s.x + s.nframe);
+  for (size_t f = 0; f < s.nframe; ++f)
+    if (!(std::isfinite(y[f]) && std::isfinite(x[f]) && std::fabs(y[f]) <= 1e10 &&
+          std::fabs(x[f]) <= 1e10))
+      throw Invalid("positions must be finite with |pos| <= 1e10");
+  y = pynx_axis(y, p.exit.pixel[0]);
+  x = pynx_axis(x, p.exit.pixel[1]);
+  p.corner.resize(2 * s.nframe);
+  for (size_t f = 0; f < s.nframe; ++f) {
+    const long oy = view_corner(long(nyo), N, y[f] / p.exit.pixel[0]);
+    const long ox = view_corner(long(nxo), N, x[f] / p.exit.pixel[1]);
+    if (oy < 0 || ox < 0)
+      throw Invalid("frame " + std::to_string(f) + " leaves the object; size it with required_obj_shape");
+    p.corner[2 * f] = int(oy);
+    p.corner[2 * f + 1] = int(ox);
+  }
+  const size_t n = size_t(N) * N;
+  p.counts.resize(s.nframe * n);
+  for (size_t f = 0; f < s.nframe; ++f) {
+    for (int ky = 0; ky < N; ++ky)
+      for (int kx = 0; kx < N; ++kx) {
+        // DFT index (ky, kx) sees detector pixel k; the inverse transform flips it.
+        int dy = ky, dx = kx;
+        if (p.exit.reverse) {
+          dy = (N - ky) % N;
+          dx = (N - kx) % N;
+        }
+        // Shifted (DFT-order) index to detector order: (d + N/2) mod N.
+        const size_t src = f * n + size_t((dy + N / 2) % N) * N + (dx + N / 2) % N;
+        float v;
+        if (s.counts_u16) {
+          const uint16_t c = s.counts_u16[src];
+          v = c == 65535 ? -1.0f : float(c);
+        } else {
+          const float c = s.counts_f32[src];
+          if (std::isnan(c)) throw Invalid("counts must not be NaN");
+          v = c < 0 ? -1.0f : c;
+        }
+        if (v >= 0) {
+          ++p.nb_obs;
+          p.total += v;
+        }
+        p.counts[f * n + size_t(ky) * N + kx] = v;
+      }
+  }
+  return p;
+}
+
+}  // namespace ptyrs
+
+using namespace ptyrs;
+
+struct ptyrs_engine {
+  std::unique_ptr<EngineBase> e;
+};
+
+namespace {
+
+template <typename F>
+int guarded(ptyrs_engine* h, bool mutating, F&& f) {
+  try {
+    if (h == nullptr || !h->e) throw Invalid("null engine");
+    if (mutating && h->e->aborted)
+      throw Aborted("aborted: an earlier call failed part-way; rebuild the engine");
+    f(*h->e);
+    return PTYRS_OK;
+  } catch (const Invalid& x) {
+    last_error = x.what();
+    return PTYRS_EINVAL;
+  } catch (const Aborted& x) {
+    last_error = x.what();
+    return PTYRS_EABORTED;
+  } catch (const std::exception& x) {
+    last_error = x.what();
+    if (h && h->e && mutating) h->e->aborted = true;
+    return PTYRS_EDEVICE;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ptyrs_engine_create(const ptyrs_scan* scan, const ptyrs_optics* optics, const double* obj,
+                        size_t nobj, size_t nyo, size_t nxo, const double* probe, size_t nprobe,
+                        const ptyrs_gpu_options* options, int device, ptyrs_engine** out) {
+  try {
+    if (!scan || !obj || !probe || !out) throw Invalid("null argument");
+    const int dtype = options ? options->dtype : PTYRS_C64;
+    if (dtype != PTYRS_C64 && dtype != PTYRS_C128) throw Invalid("dtype must be c64 or c128");
+    Prepared pre = prepare(*scan, optics, nprobe, nobj, nyo, nxo);
+    auto h = std::make_unique<ptyrs_engine>();
+    h->e.reset(make_engine(dtype, pre, obj, nobj, nyo, nxo, probe, nprobe, device));
+    *out = h.release();
+    return PTYRS_OK;
+  } catch (const Invalid& x) {
+    last_error = x.what();
+    return PTYRS_EINVAL;
+  } catch (const std::exception& x) {
+    last_error = x.what();
+    return PTYRS_EDEVICE;
+  }
+}
+
+int ptyrs_engine_rpie(ptyrs_engine* h, const ptyrs_rpie_params* p, size_t nepoch) {
+  if (!p) {
+    last_error = "null params";
+    return PTYRS_EINVAL;
+  }
+  return guarded(h, true, [&](EngineBase& e) { e.rpie(*p, nepoch); });
+}
+
+int ptyrs_engine_llk(ptyrs_engine* h, ptyrs_llk_record* out) {
+  return guarded(h, true, [&](EngineBase& e) {
+    if (!out) throw Invalid("null output");
+    double v[4];
+    e.llk(v);
+    out->cycle = e.cycle;
+    out->algorithm = -1;
+    out->poisson = v[0];
+    out->gaussian = v[1];
+    out->euclidian = v[2];
+    out->nb_photons_calc = v[3];
+  });
+}
+
+int ptyrs_engine_object(ptyrs_engine* h, double* out, size_t len) {
+  return guarded(h, false, [&](EngineBase& e) {
+    if (!out || len != e.object_len()) throw Invalid("object buffer has the wrong length");
+    e.get_object(out);
+  });
+}
+
+int ptyrs_engine_probe(ptyrs_engine* h, double* out, size_t len) {
+  return guarded(h, false, [&](EngineBase& e) {
+    if (!out || len != e.probe_len()) throw Invalid("probe buffer has the wrong length");
+    e.get_probe(out);
+  });
+}
+
+int ptyrs_engine_set_object(ptyrs_engine* h, const double* in, size_t len) {
+  return guarded(h, true, [&](EngineBase& e) {
+    if (!in || len != e.object_len()) throw Invalid("object buffer has the wrong length");
+    e.set_object(in);
+  });
+}
+
+int ptyrs_engine_set_probe(ptyrs_engine* h, const double* in, size_t len) {
+  return guarded(h, true, [&](EngineBase& e) {
+    if (!in || len != e.probe_len()) throw Invalid("probe buffer has the wrong length");
+    e.set_probe(in);
+  });
+}
+
+int ptyrs_engine_history(ptyrs_engine* h, ptyrs_llk_record* out, size_t cap, size_t* n) {
+  return guarded(h, false, [&](EngineBase& e) {
+    if (n) *n = e.history.size();
+    for (size_t i = 0; i < e.history.size() && i < cap && out; ++i) {
+      const Record& r = e.history[i];
+      out[i] = ptyrs_llk_record{r.cycle, r.algorithm, r.llk[0], r.llk[1], r.llk[2], r.llk[3]};
+    }
+  });
+}
+
+int ptyrs_engine_cycle(ptyrs_engine* h, size_t* cycle) {
+  return guarded(h, false, [&](EngineBase& e) {
+    if (cycle) *cycle = e.cycle;
+  });
+}
+
+int ptyrs_engine_corners(ptyrs_engine* h, int64_t* out, size_t len) {
+  return guarded(h, false, [&](EngineBase& e) {
+    if (!out || len != e.corners.size()) throw Invalid("corner buffer has the wrong length");
+    for (size_t i = 0; i < len; ++i) out[i] = e.corners[i];
+  });
+}
+
+int ptyrs_engine_synchronize(ptyrs_engine* h) {
+  return guarded(h, false, [&](EngineBase& e) { e.synchronize(); });
+}
+
+int ptyrs_engine_profile(ptyrs_engine* h, int enable) {
+  return guarded(h, false, [&](EngineBase& e) { e.profile(enable != 0); });
+}
+
+int ptyrs_engine_kernel_stats(ptyrs_engine* h, const char* name, size_t* launches,
+                              double* total_ms) {
+  return guarded(h, false, [&](EngineBase& e) {
+    if (!name || !launches || !total_ms) throw Invalid("null argument");
+    std::string names = kind_names();
+    int kind = 0;
+    size_t start = 0;
+    while (true) {
+      size_t end = names.find(',', start);
+      std::string k = names.substr(start, end == std::string::npos ? std::string::npos : end - start);
+      if (k == name) break;
+      if (end == std::string::npos) throw Invalid(std::string("unknown kernel name ") + name);
+      start = end + 1;
+      ++kind;
+    }
+    e.kernel_stats(kind, launches, total_ms);
+  });
+}
+
+const char* ptyrs_kernel_names(void) { return kind_names(); }
+
+void ptyrs_engine_destroy(ptyrs_engine* h) { delete h; }
+
+size_t ptyrs_last_error(char* buf, size_t len) {
+  if (buf && len) {
+    size_t k = std::min(len - 1, last_error.size());
+    std::memcpy(buf, last_error.data(), k);
+    buf[k] = 0;
+  }
+  return last_error.size();
+}
+
+}  // extern "C"

@@ -1,0 +1,68 @@
+// Complex arithmetic for the device kernels, in the precision of the engine (float or double).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace ptyrs {
+
+template <typename T>
+struct cplx {
+  T x, y;
+};
+
+template <typename T> struct vec2;
+template <> struct vec2<float> { using type = float2; };
+template <> struct vec2<double> { using type = double2; };
+
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> mk(T x, T y) { return cplx<T>{x, y}; }
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> operator+(cplx<T> a, cplx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> operator-(cplx<T> a, cplx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> operator*(cplx<T> a, cplx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> operator*(cplx<T> a, T s) { return {a.x * s, a.y * s}; }
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> conj(cplx<T> a) { return {a.x, -a.y}; }
+template <typename T>
+__host__ __device__ __forceinline__ T norm2(cplx<T> a) { return a.x * a.x + a.y * a.y; }
+// conj(a) * b
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b) {
+  return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
+}
+// a * (-i)
+template <typename T>
+__host__ __device__ __forceinline__ cplx<T> mul_mi(cplx<T> a) { return {a.y, -a.x}; }
+
+// 8- or 16-byte vector load/store of one complex value.
+template <typename T>
+__device__ __forceinline__ cplx<T> ld(const cplx<T>* p) {
+  auto v = *reinterpret_cast<const typename vec2<T>::type*>(p);
+  return {v.x, v.y};
+}
+template <typename T>
+__device__ __forceinline__ void st(cplx<T>* p, cplx<T> a) {
+  typename vec2<T>::type v;
+  v.x = a.x;
+  v.y = a.y;
+  *reinterpret_cast<typename vec2<T>::type*>(p) = v;
+}
+
+// Kahan-compensated accumulator: the running sum and the lost low-order part.
+template <typename T>
+struct kahan {
+  T s = 0, c = 0;
+  __device__ __forceinline__ void add(T v) {
+    T y = v - c;
+    T t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
+}  // namespace ptyrs

@@ -1,0 +1,435 @@
+// The engine: device state of one scan and the algorithm loops that drive the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+#include "kernels.hpp"
+
+namespace ptyrs {
+
+#define PTYRS_HIP(expr)                                                                   \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      throw DeviceError(std::string(#expr) + ": " + hipGetErrorString(e_));               \
+  } while (0)
+
+// Kernel names for timing; the index is the `kind` of a timed launch.
+enum Kind { K_PASS_A = 0, K_PASS_B, K_PASS_C, K_BIN, K_OVERLAP, K_OBJ_UPDATE, K_PROBE, K_LLK,
+            K_OTHER, K_COUNT };
+inline const char* kind_names() {
+  return "pass_a,pass_b,pass_c,bin_tiles,obj_overlap,obj_update,probe_update,llk,other";
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void ensure(size_t want) {
+    if (want <= bytes) return;
+    if (p) PTYRS_HIP(hipFree(p));
+    p = nullptr;
+    bytes = 0;
+    PTYRS_HIP(hipMalloc(&p, want));
+    bytes = want;
+  }
+  template <typename U>
+  U* as() const { return static_cast<U*>(p); }
+};
+
+struct Record {
+  size_t cycle;
+  int algorithm;
+  double llk[4];
+};
+
+// Everything of a scan the host prepares once (ptyrs/src/engine/gpu/mod.rs:635-760).
+struct Prepared {
+  size_t nframe = 0, ny = 0, nx = 0;
+  std::vector<float> counts;  // DFT index order, negative = masked
+  std::vector<int> corner;    // [nframe][2] oy, ox
+  size_t nb_obs = 0;
+  double total = 0;
+  ExitModel exit;
+};
+
+class EngineBase {
+ public:
+  virtual ~EngineBase() {
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+  virtual void rpie(const ptyrs_rpie_params& p, size_t nepoch) = 0;
+  virtual void llk(double out[4]) = 0;
+  virtual void get_object(double* out) = 0;
+  virtual void get_probe(double* out) = 0;
+  virtual void set_object(const double* in) = 0;
+  virtual void set_probe(const double* in) = 0;
+
+  void synchronize() { PTYRS_HIP(hipStreamSynchronize(stream_)); }
+
+  size_t object_len() const { return 2 * nobj_ * nyo_ * nxo_; }
+  size_t probe_len() const { return 2 * nprobe_ * ny_ * nx_; }
+
+  std::vector<Record> history;
+  size_t cycle = 0;
+  bool aborted = false;
+  std::vector<int> corners;
+
+  void profile(bool on) {
+    profiling_ = on;
+    if (on) {
+      for (auto& s : stats_) s = {0, 0.0};
+    }
+  }
+  void kernel_stats(int kind, size_t* launches, double* ms) {
+    flush_timing();
+    *launches = stats_[kind].first;
+    *ms = stats_[kind].second;
+  }
+
+ protected:
+  int device_ = 0;
+  hipStream_t stream_ = nullptr;
+  size_t nframe_ = 0, ny_ = 0, nx_ = 0, nprobe_ = 0, nobj_ = 0, nyo_ = 0, nxo_ = 0;
+  size_t nb_obs_ = 0;
+
+  bool profiling_ = false;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending_;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pool_;
+  std::pair<size_t, double> stats_[K_COUNT] = {};
+
+  // Runs `launch` on the engine's stream, timed with events when profiling.
+  template <typename F>
+  void run(int kind, F&& launch) {
+    if (!profiling_) {
+      launch();
+      PTYRS_HIP(hipGetLastError());
+      return;
+    }
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    if (!pool_.empty()) {
+      ev = pool_.back();
+      pool_.pop_back();
+    } else {
+      PTYRS_HIP(hipEventCreate(&ev.first));
+      PTYRS_HIP(hipEventCreate(&ev.second));
+    }
+    PTYRS_HIP(hipEventRecord(ev.first, stream_));
+    launch();
+    PTYRS_HIP(hipGetLastError());
+    PTYRS_HIP(hipEventRecord(ev.second, stream_));
+    pending_.push_back({kind, ev});
+  }
+  void flush_timing() {
+    if (pending_.empty()) return;
+    PTYRS_HIP(hipStreamSynchronize(stream_));
+    for (auto& p : pending_) {
+      float ms = 0;
+      PTYRS_HIP(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+      stats_[p.first].first += 1;
+      stats_[p.first].second += ms;
+      pool_.push_back(p.second);
+    }
+    pending_.clear();
+  }
+};
+
+template <typename T, int N>
+class Engine : public EngineBase {
+  using S = Shape<T, N>;
+  static constexpr int n = N * N;
+
+ public:
+  Engine(const Prepared& pre, const double* obj, size_t nobj, size_t nyo, size_t nxo,
+         const double* probe, size_t nprobe, int device) {
+    device_ = device;
+    PTYRS_HIP(hipSetDevice(device));
+    PTYRS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    nframe_ = pre.nframe;
+    ny_ = nx_ = N;
+    nprobe_ = nprobe;
+    nobj_ = nobj;
+    nyo_ = nyo;
+    nxo_ = nxo;
+    nb_obs_ = pre.nb_obs;
+    corners = pre.corner;
+    M_ = int(nprobe * nobj);
+
+    O_.ensure(sizeof(cplx<T>) * nobj * nyo * nxo);
+    P_.ensure(sizeof(cplx<T>) * nprobe * n);
+    Wt_.ensure(sizeof(cplx<T>) * nprobe * n);
+    q_.ensure(sizeof(cplx<T>) * n);
+    tw_.ensure(sizeof(cplx<T>) * N);
+    counts_.ensure(sizeof(float) * std::max<size_t>(1, nframe_ * n));
+    corner_.ensure(sizeof(int2) * std::max<size_t>(1, nframe_));
+    iota_.ensure(sizeof(int) * std::max<size_t>(1, nframe_));
+    hist_.ensure(sizeof(double) * 4 * kHistSlots);
+
+    upload_complex(O_, obj, nobj * nyo * nxo);
+    upload_complex(P_, probe, nprobe * n);
+    upload_complex(q_, pre.exit.quad.data(), n);
+    std::vector<double> tw(2 * N);
+    for (int t = 0; t < N; ++t) {
+      // exp(−2πi t/N) from the exact angle of the reduced fraction.
+      const long double a = -2.0L * 3.14159265358979323846264338327950288L * t / N;
+      tw[2 * t] = double(cosl(a));
+      tw[2 * t + 1] = double(sinl(a));
+    }
+    upload_complex(tw_, tw.data(), N);
+    if (nframe_) {
+      PTYRS_HIP(hipMemcpy(counts_.p, pre.counts.data(), sizeof(float) * nframe_ * n,
+                          hipMemcpyHostToDevice));
+      std::vector<int2> c(nframe_);
+      std::vector<int> io(nframe_);
+      for (size_t f = 0; f < nframe_; ++f) {
+        c[f] = int2{pre.corner[2 * f], pre.corner[2 * f + 1]};
+        io[f] = int(f);
+      }
+      PTYRS_HIP(hipMemcpy(corner_.p, c.data(), sizeof(int2) * nframe_, hipMemcpyHostToDevice));
+      PTYRS_HIP(hipMemcpy(iota_.p, io.data(), sizeof(int) * nframe_, hipMemcpyHostToDevice));
+    }
+    ntx_ = int((nxo + TILE_X - 1) / TILE_X);
+    nty_ = int((nyo + TILE_Y - 1) / TILE_Y);
+  }
+
+  void get_object(double* out) override { download_complex(O_, out, nobj_ * nyo_ * nxo_); }
+  void get_probe(double* out) override { download_complex(P_, out, nprobe_ * n); }
+  void set_object(const double* in) override { upload_complex(O_, in, nobj_ * nyo_ * nxo_); }
+  void set_probe(const double* in) override { upload_complex(P_, in, nprobe_ * n); }
+
+  void llk(double out[4]) override {
+    ensure_workspace(std::min<size_t>(nframe_, kDefaultStack), 1);
+    probe_terms();
+    llk_pass(0);
+    PTYRS_HIP(hipMemcpyAsync(out, hist_.p, sizeof(double) * 4, hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    flush_timing();
+    // Llk::normalised (ptyrs/src/engine/ops.rs:268-276): nb_photons_calc stays a total.
+    for (int i = 0; i < 3; ++i) out[i] /= double(nb_obs_);
+  }
+
+  void rpie(const ptyrs_rpie_params& p, size_t nepoch) override {
+    if (nobj_ != 1) throw Invalid("rPIE needs exactly one object mode");
+    if (p.batch_size == 0) throw Invalid("rPIE batch_size must be >= 1");
+    if (M_ > 4) throw Invalid("the GPU engine runs at most 4 modes");
+    if (nepoch == 0 || nframe_ == 0) {
+      cycle += nepoch;
+      return;
+    }
+    const size_t cap = std::min(p.batch_size, nframe_);
+    ensure_workspace(cap, nepoch);
+    // Every epoch's batches, uploaded once: the host never waits inside the call.
+    std::vector<int> order(nepoch * nframe_);
+    std::vector<std::vector<std::pair<int, int>>> spans(nepoch);
+    for (size_t ic = 0; ic < nepoch; ++ic) {
+      auto batches = random_batches(nframe_, p.batch_size, p.seed, uint64_t(cycle + ic));
+      size_t off = 0;
+      for (auto& b : batches) {
+        std::copy(b.begin(), b.end(), order.begin() + ic * nframe_ + off);
+        spans[ic].push_back({int(off), int(b.size())});
+        off += b.size();
+      }
+    }
+    PTYRS_HIP(hipMemcpyAsync(order_.p, order.data(), sizeof(int) * order.size(),
+                             hipMemcpyHostToDevice, stream_));
+    probe_terms();
+    std::vector<std::pair<size_t, int>> due;  // (cycle, slot)
+    for (size_t ic = 0; ic < nepoch; ++ic) {
+      const int* ord = order_.as<int>() + ic * nframe_;
+      for (auto& s : spans[ic]) rpie_batch(p, ord + s.first, s.second);
+      const bool rec = p.calc_llk > 0 && (ic % p.calc_llk == 0 || ic + 1 == nepoch);
+      if (rec) {
+        if (due.size() == kHistSlots) drain(due, PTYRS_RPIE);
+        const int slot = int(due.size());
+        llk_pass(slot);
+        due.push_back({cycle, slot});
+      }
+      ++cycle;
+    }
+    drain(due, PTYRS_RPIE);
+    synchronize();
+    flush_timing();
+  }
+
+ private:
+  static constexpr size_t kHistSlots = 64;
+  static constexpr size_t kDefaultStack = 128;
+  int M_ = 1;
+  int ntx_ = 0, nty_ = 0;
+  size_t cap_ = 0, nchunk_cap_ = 0, order_cap_ = 0;
+  DevBuf O_, P_, Wt_, q_, tw_, counts_, corner_, iota_, hist_;
+  DevBuf psi_, omax_, dP_, list_, lcount_, tilemax_, pmax_, order_, llk_part_;
+
+  void upload_complex(DevBuf& d, const double* src, size_t count) {
+    if (stream_) PTYRS_HIP(hipStreamSynchronize(stream_));
+    std::vector<cplx<T>> h(count);
+    for (size_t i = 0; i < count; ++i) h[i] = cplx<T>{T(src[2 * i]), T(src[2 * i + 1])};
+    PTYRS_HIP(hipMemcpy(d.p, h.data(), sizeof(cplx<T>) * count, hipMemcpyHostToDevice));
+  }
+  void download_complex(const DevBuf& d, double* out, size_t count) {
+    std::vector<cplx<T>> h(count);
+    PTYRS_HIP(hipStreamSynchronize(stream_));
+    PTYRS_HIP(hipMemcpy(h.data(), d.p, sizeof(cplx<T>) * count, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < count; ++i) {
+      out[2 * i] = double(h[i].x);
+      out[2 * i + 1] = double(h[i].y);
+    }
+  }
+
+  int ntiles() const { return ntx_ * nty_; }
+
+  // Pass C chunking: enough workgroups to fill the chip, few partial probe sums.
+  int chunk_of(int nb) const {
+    const int per = S::NBLK * M_;
+    int nchunk = std::max(1, std::min(nb, (1024 + per - 1) / per));
+    return (nb + nchunk - 1) / nchunk;
+  }
+
+  void ensure_workspace(size_t cap, size_t nepoch) {
+    cap = std::max<size_t>(cap, 1);
+    psi_.ensure(sizeof(cplx<T>) * cap * M_ * n);
+    omax_.ensure(sizeof(T) * cap * S::NBLK);
+    // chunk_of never cuts more chunks than this, whatever the batch length.
+    const size_t per = size_t(S::NBLK) * M_;
+    const size_t nchunk = std::min(cap, (1024 + per - 1) / per);
+    dP_.ensure(sizeof(cplx<T>) * std::max<size_t>(nchunk, 1) * M_ * n);
+    list_.ensure(sizeof(int) * size_t(ntiles()) * cap);
+    lcount_.ensure(sizeof(int) * size_t(ntiles()));
+    tilemax_.ensure(sizeof(T) * size_t(ntiles()));
+    pmax_.ensure(sizeof(T) * 1);
+    order_.ensure(sizeof(int) * std::max<size_t>(1, nepoch * nframe_));
+    llk_part_.ensure(sizeof(double) * 4 * nframe_ * S::NBLK);
+    cap_ = std::max(cap_, cap);
+  }
+
+  void probe_terms() {
+    const int total = int(nprobe_ * n);
+    run(K_OTHER, [&] {
+      hipLaunchKernelGGL(k_probe_terms<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,
+                         P_.as<cplx<T>>(), q_.as<cplx<T>>(), Wt_.as<cplx<T>>(), n, total);
+    });
+  }
+
+  void pass_a(const int* ord, int nb, bool omax) {
+    run(K_PASS_A, [&] {
+      hipLaunchKernelGGL((k_pass_a<T, N>), dim3(S::NBLK, nb * M_), dim3(S::NT), 0, stream_,
+                         O_.as<cplx<T>>(), int(nxo_), Wt_.as<cplx<T>>(), corner_.as<int2>(), ord,
+                         M_, psi_.as<cplx<T>>(), omax ? omax_.as<T>() : nullptr,
+                         tw_.as<cplx<T>>());
+    });
+  }
+
+  template <int MODE>
+  void pass_b(const int* ord, int nb) {
+    run(MODE == B_LLK ? K_LLK : K_PASS_B, [&] {
+      dim3 g(S::NBLK, nb), b(S::NT);
+      auto args = [&](auto kern) {
+        hipLaunchKernelGGL(kern, g, b, 0, stream_, psi_.as<cplx<T>>(), counts_.as<float>(), ord,
+                           llk_part_.as<double>(), tw_.as<cplx<T>>());
+      };
+      switch (M_) {
+        case 1: args(k_pass_b<T, N, 1, MODE>); break;
+        case 2: args(k_pass_b<T, N, 2, MODE>); break;
+        case 3: args(k_pass_b<T, N, 3, MODE>); break;
+        case 4: args(k_pass_b<T, N, 4, MODE>); break;
+        default: throw Invalid("the GPU engine runs at most 4 modes");
+      }
+    });
+  }
+
+  // One forward pass of every frame and the four LLK sums into history slot `slot`.
+  void llk_pass(int slot) {
+    const int step = int(std::max<size_t>(cap_, 1));
+    for (int f0 = 0; f0 < int(nframe_); f0 += step) {
+      const int nb = std::min(step, int(nframe_) - f0);
+      const int* ord = iota_.as<int>() + f0;
+      pass_a(ord, nb, false);
+      pass_b<B_LLK>(ord, nb);
+    }
+    run(K_LLK, [&] {
+      hipLaunchKernelGGL(k_llk_reduce, dim3(1), dim3(THREADS), 0, stream_,
+                         llk_part_.as<double>(), int(nframe_ * S::NBLK),
+                         hist_.as<double>() + 4 * slot);
+    });
+  }
+
+  void drain(std::vector<std::pair<size_t, int>>& due, int algorithm) {
+    if (due.empty()) return;
+    std::vector<double> h(4 * kHistSlots);
+    PTYRS_HIP(hipMemcpyAsync(h.data(), hist_.p, sizeof(double) * 4 * kHistSlots,
+                             hipMemcpyDeviceToHost, stream_));
+    PTYRS_HIP(hipStreamSynchronize(stream_));
+    const double nb = double(nb_obs_);
+    for (auto& d : due) {
+      const double* v = h.data() + 4 * d.second;
+      history.push_back(Record{d.first, algorithm, {v[0] / nb, v[1] / nb, v[2] / nb, v[3]}});
+    }
+    due.clear();
+  }
+
+  // One rPIE batch (ptyrs/src/engine/ptycho.rs:501-687) from the batch-start object and probe.
+  void rpie_batch(const ptyrs_rpie_params& p, const int* ord, int nb) {
+    const bool upd_obj = p.has_obj_step != 0, upd_probe = p.has_probe_step != 0;
+    const int nt = ntiles();
+    if (upd_obj) {
+      run(K_BIN, [&] {
+        hipLaunchKernelGGL(k_bin_tiles, dim3((nt + 255) / 256), dim3(256), 0, stream_,
+                           corner_.as<int2>(), ord, nb, N, ntx_, nt, int(cap_), list_.as<int>(),
+                           lcount_.as<int>());
+      });
+      run(K_OVERLAP, [&] {
+        hipLaunchKernelGGL(k_obj_overlap<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                           P_.as<cplx<T>>(), N, M_, corner_.as<int2>(), ord, list_.as<int>(),
+                           lcount_.as<int>(), int(cap_), int(nyo_), int(nxo_), ntx_,
+                           tilemax_.as<T>());
+        hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(THREADS), 0, stream_,
+                           tilemax_.as<T>(), nt, pmax_.as<T>());
+      });
+    }
+    pass_a(ord, nb, upd_probe);
+    pass_b<B_RPIE>(ord, nb);
+    const int chunk = chunk_of(nb);
+    const int nchunk = (nb + chunk - 1) / chunk;
+    run(K_PASS_C, [&] {
+      hipLaunchKernelGGL((k_pass_c_rpie<T, N>), dim3(S::NBLK, M_, nchunk), dim3(S::NT), 0,
+                         stream_, psi_.as<cplx<T>>(), O_.as<cplx<T>>(), int(nxo_),
+                         Wt_.as<cplx<T>>(), q_.as<cplx<T>>(), corner_.as<int2>(), ord, nb, M_,
+                         omax_.as<T>(), chunk, T(p.probe_alpha), int(upd_probe),
+                         dP_.as<cplx<T>>(), tw_.as<cplx<T>>());
+    });
+    if (upd_obj) {
+      run(K_OBJ_UPDATE, [&] {
+        hipLaunchKernelGGL(k_obj_update_rpie<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                           O_.as<cplx<T>>(), Wt_.as<cplx<T>>(), P_.as<cplx<T>>(),
+                           psi_.as<cplx<T>>(), N, M_, corner_.as<int2>(), ord, list_.as<int>(),
+                           lcount_.as<int>(), int(cap_), int(nyo_), int(nxo_), ntx_,
+                           pmax_.as<T>(), T(p.obj_alpha), T(p.obj_step));
+      });
+    }
+    if (upd_probe) {
+      const int total = int(nprobe_ * n);
+      run(K_PROBE, [&] {
+        hipLaunchKernelGGL(k_probe_apply<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,
+                           P_.as<cplx<T>>(), Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
+                           dP_.as<cplx<T>>(), nchunk, n, total, T(p.probe_step / nb));
+      });
+    }
+  }
+};
+
+EngineBase* make_engine(int dtype, const Prepared& pre, const double* obj, size_t nobj,
+                        size_t nyo, size_t nxo, const double* probe, size_t nprobe, int device);
+
+}  // namespace ptyrs

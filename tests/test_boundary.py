@@ -1,0 +1,62 @@
+"""The C-ABI library: it loads, exports every entry point include/ptyrs_hip.h declares, and
+refuses bad input before it touches a device (no compute calls: this runs without a GPU)."""
+
+import re
+
+import numpy as np
+import pytest
+
+import ptyrs_amd as pa
+from conftest import ROOT
+
+
+def declared():
+    text = (ROOT / "include" / "ptyrs_hip.h").read_text()
+    return sorted(set(re.findall(r"\b(ptyrs_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_entry_point():
+    lib = pa.library()
+    names = declared()
+    assert "ptyrs_engine_create" in names and "ptyrs_engine_rpie" in names
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(pa.library().ptyrs_kernel_names().decode().split(",")) >= {"pass_a", "pass_b", "pass_c"}
+
+
+def _scan(n=32, nframe=3):
+    counts = np.zeros((nframe, n, n), dtype=np.uint16)
+    y = np.array([0.0, 1e-8, -1e-8])[:nframe]
+    return pa.Scan(counts, y, y, 1.5e-10, 1.0, 75e-6)
+
+
+def test_bad_input_is_a_value_error_before_any_device_call():
+    obj = np.ones((1, 64, 64), complex)
+    probe = np.ones((1, 32, 32), complex)
+    with pytest.raises(ValueError, match="not supported"):
+        counts = np.zeros((3, 34, 34), np.uint16)
+        pa.GpuPtycho(pa.Scan(counts, [0, 0, 0], [0, 0, 0], 1.5e-10, 1.0, 75e-6), obj,
+                     np.ones((1, 34, 34), complex))
+    far = pa.Scan(np.zeros((3, 32, 32), np.uint16), [0.0, 3e-7, -3e-7], [0.0, 0.0, 0.0],
+                  1.5e-10, 1.0, 75e-6)  # ±4.8 object pixels
+    with pytest.raises(ValueError, match="leaves the object"):
+        pa.GpuPtycho(far, np.ones((1, 36, 36), complex), probe)
+    with pytest.raises(ValueError):
+        pa.GpuPtycho(_scan(), obj, np.ones((1, 16, 16), complex))
+    with pytest.raises(ValueError, match="dtype"):
+        pa.GpuPtycho(_scan(), obj, probe, dtype="f16")
+
+
+def test_unknown_keywords_are_type_errors():
+    with pytest.raises(TypeError, match="unknown"):
+        pa.GpuPtycho(_scan(), np.ones((1, 64, 64), complex), np.ones((1, 32, 32), complex),
+                     nonsense=1)
+
+
+def test_scan_rejects_non_finite_counts_and_bad_geometry():
+    with pytest.raises(ValueError):
+        pa.Scan(np.full((1, 4, 4), np.nan), [0.0], [0.0], 1e-10, 1.0, 1e-4)
+    with pytest.raises(ValueError):
+        pa.Scan(np.zeros((1, 4, 4)), [0.0], [0.0], -1e-10, 1.0, 1e-4)
+    with pytest.raises(TypeError):
+        pa.Scan(np.zeros((4, 4)), [0.0], [0.0], 1e-10, 1.0, 1e-4)
