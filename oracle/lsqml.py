@@ -1,0 +1,211 @@
+"""pty-chi LSQML as the reference's CPU engine restates it, single slice, no position or OPR
+features: one batch's directions (`LsqmlBatch`), the per-frame 2×2 step sizes, the step and
+the epochs with the preconditioner schedule.
+
+Test infrastructure (see oracle/__init__.py). Follows ptyrs/src/engine/lsqml.rs.
+"""
+
+import math
+
+import numpy as np
+
+from .base import random_batches, to_natural, to_shifted
+from .ptycho import obj_probe_to_psi, llk
+
+EPS_AMPLITUDE = 1e-9   # ptyrs/src/engine/lsqml.rs:25-31
+EPS_POISSON = 1e-6
+EPS_XI = 1e-5
+LAMBDA0 = 1.2e-7
+
+AMPLITUDE, POISSON = "amplitude", "poisson"
+
+
+def illumination(views, obj_shape, probe):
+    """Σ_views |P[probe_k]|² on each view's window. ptyrs/src/engine/lsqml.rs:399-416"""
+    ny, nx = probe.shape[-2:]
+    out = np.zeros(obj_shape)
+    p2 = probe.real ** 2 + probe.imag ** 2
+    for (_, _, p, o, oy, ox) in views:
+        out[o, oy:oy + ny, ox:ox + nx] += p2[p]
+    return out
+
+
+def trim_mean(x):
+    """MATLAB trimmean(x, 10). ptyrs/src/engine/lsqml.rs:418-426"""
+    s = sorted(x)
+    k = int(math.ceil(len(s) * 0.1 / 2.0 - 0.5))
+    kept = s[k:len(s) - k]
+    return sum(kept) / len(kept)
+
+
+def frame_mean(d, np_):
+    """Mean over frames of planes b·np + m per mode. ptyrs/src/engine/lsqml.rs:381-397"""
+    count = d.shape[0] // np_
+    out = np.zeros((np_,) + d.shape[1:], dtype=d.dtype)
+    for b in range(count):
+        for m in range(np_):
+            out[m] += d[b * np_ + m]
+    return out / count
+
+
+class LsqmlBatch:
+    """One batch's directions at one state. ptyrs/src/engine/lsqml.rs:97-181"""
+
+    def __init__(self, run, views, iobs, optics, obj, probe):
+        cost = run.cost
+        ny, nx = probe.shape[-2:]
+        n = ny * nx
+        nmode = 0
+        while nmode < len(views) and views[nmode][0] == views[0][0]:
+            nmode += 1
+        self.views, self.nmode = views, nmode
+        self.frames = [views[k][0] for k in range(0, len(views), nmode)]
+        self.obj, self.probe, self.optics = obj, probe, optics
+        psi = obj_probe_to_psi(None, views, obj, probe, optics.quad)
+        self.psi = psi
+        far = optics.exit_forward(psi)
+        nb = len(self.frames)
+        fa = far.reshape(nb, nmode, ny, nx)
+        icalc = (fa.real ** 2 + fa.imag ** 2).sum(axis=1)
+        io = np.stack([iobs[f] for f in self.frames])
+        if cost == POISSON:
+            a_far = [run.a_far_of(f) for f in self.frames]
+            for _ in range(2):
+                prev = sum(a_far) / len(a_far)
+                new = []
+                for k in range(nb):
+                    yt, yp = io[k], icalc[k]
+                    keep = ~np.isnan(yt)
+                    yt, yp = yt[keep], yp[keep]
+                    xi = 1.0 - yt / (yp + EPS_XI)
+                    num = float((xi * (yp - yt / (1.0 - prev * xi))).sum())
+                    den = float((xi * xi * yp).sum())
+                    ratio = num / den if den > 0.0 else 0.0
+                    new.append(min(max(0.5 * prev + 0.5 * ratio, 0.0), 1.0))
+                a_far = new
+        else:
+            a_far = [0.5] * nb
+        self.a_far = a_far
+        masked = np.isnan(io)
+        if cost == AMPLITUDE:
+            c = 2.0 * (1.0 - np.sqrt(np.maximum(np.where(masked, 0.0, io), 0.0))
+                       / (np.sqrt(icalc) + EPS_AMPLITUDE))
+        else:
+            c = 1.0 - np.where(masked, 0.0, io) / (icalc + EPS_POISSON)
+        c = np.where(masked, 0.0, c)
+        scale = 1.0 - np.asarray(a_far)[:, None, None] * c
+        chi = (fa * scale[:, None]).reshape(far.shape)
+        self.chi = optics.exit_adjoint(chi) - psi
+
+    def obj_direction(self):
+        """Σ_views conj(P·q)·chi on each window. ptyrs/src/engine/lsqml.rs:183-188"""
+        ny, nx = self.probe.shape[-2:]
+        out = np.zeros(self.obj.shape, dtype=np.complex128)
+        for k, (_, _, p, o, oy, ox) in enumerate(self.views):
+            out[o, oy:oy + ny, ox:ox + nx] += to_natural(self.chi[k]) * np.conj(
+                self.probe[p] * self.optics.quad)
+        return out
+
+    def frame_probe_directions(self):
+        """Plane b·np + m: conj(q·O_view)·chi_m. ptyrs/src/engine/lsqml.rs:190-196"""
+        ny, nx = self.probe.shape[-2:]
+        np_ = self.probe.shape[0]
+        nb = len(self.frames)
+        out = np.zeros((nb * np_, ny, nx), dtype=np.complex128)
+        for k, (f, m, p, o, oy, ox) in enumerate(self.views):
+            b = k // self.nmode
+            win = self.obj[o, oy:oy + ny, ox:ox + nx]
+            out[b * np_ + p] += np.conj(self.optics.quad * win) * to_natural(self.chi[k])
+        return out
+
+    def probe_direction(self):
+        return frame_mean(self.frame_probe_directions(), self.probe.shape[0])
+
+    def step_sizes(self, d_obj, d_probe):
+        """Per-frame a_obj, a_probe. ptyrs/src/engine/lsqml.rs:272-350 (one slice)"""
+        ny, nx = self.probe.shape[-2:]
+        n = ny * nx
+        q = self.optics.quad
+        dpo = obj_probe_to_psi(None, self.views, self.obj, d_probe, q)
+        dop = obj_probe_to_psi(None, self.views, d_obj, self.probe, q)
+        lam = LAMBDA0 * self.optics.propagator.intensity_scale() / n
+        nb = len(self.frames)
+        sh = (nb, self.nmode * n)
+        u, v, c = dop.reshape(sh), dpo.reshape(sh), self.chi.reshape(sh)
+        a12 = u * np.conj(v)
+        sums = np.stack([(u.real ** 2 + u.imag ** 2 + lam).sum(axis=1),
+                         (v.real ** 2 + v.imag ** 2 + lam).sum(axis=1),
+                         a12.real.sum(axis=1), a12.imag.sum(axis=1),
+                         (np.conj(u) * c).real.sum(axis=1),
+                         (np.conj(v) * c).real.sum(axis=1)], axis=1)
+        reg11, reg22 = 0.1 * sums[:, 0].mean(), 0.1 * sums[:, 1].mean()
+        scale = float(self.nmode)
+        a_obj, a_probe = [], []
+        for s in sums:
+            a11, a22 = s[0] + reg11, s[1] + reg22
+            det = a11 * a22 - (s[2] ** 2 + s[3] ** 2)
+            x0 = (a22 * s[4] - s[2] * s[5]) / det
+            x1 = (a11 * s[5] - s[2] * s[4]) / det
+            a_obj.append(0.9 * max(x0, 0.0) / scale)
+            a_probe.append(0.9 * max(x1, 0.0))
+        return a_obj, a_probe, sums
+
+
+class Lsqml:
+    """LSQML state across epochs. ptyrs/src/engine/lsqml.rs:428-715"""
+
+    def __init__(self, batch_size, cost=POISSON, seed=7):
+        if batch_size <= 0:
+            raise ValueError("LSQML batch_size must be >= 1")
+        if cost not in (AMPLITUDE, POISSON):
+            raise ValueError(cost)
+        self.batch_size, self.cost, self.seed = batch_size, cost, seed
+        self.epochs = 0
+        self.preconditioner = None
+        self.a_far = {}
+
+    def a_far_of(self, f):
+        return self.a_far.get(f, 0.5)
+
+    def step(self, views, iobs, optics, obj, probe):
+        """`run_minibatch`: directions, step sizes, updates. ptyrs/src/engine/lsqml.rs:563-643"""
+        pre = self.preconditioner
+        floor = (0.1 * pre.max()) ** 2
+        b = LsqmlBatch(self, views, iobs, optics, obj, probe)
+        d_obj = b.obj_direction() / np.sqrt(pre * pre + floor)
+        d_probe = b.probe_direction()
+        a_obj, a_probe, _ = b.step_sizes(d_obj, d_probe)
+        for f, a in zip(b.frames, b.a_far):
+            self.a_far[f] = a
+        obj = obj + trim_mean(a_obj) * d_obj
+        probe = probe + (sum(a_probe) / len(a_probe)) * d_probe
+        return obj, probe, dict(frames=b.frames, a_far=b.a_far, a_obj=a_obj, a_probe=a_probe)
+
+    def epoch(self, addr, iobs, optics, obj, probe):
+        """One epoch with pty-chi's preconditioner schedule. ptyrs/src/engine/lsqml.rs:645-714"""
+        nframe, nmode = addr.nframe, addr.nmode
+        views_all = addr.views()
+        if self.epochs < 10 or self.epochs % 10 == 0:
+            illum = illumination(views_all, obj.shape, probe)
+            pre = self.preconditioner
+            if pre is None:
+                n = probe.shape[-1] * probe.shape[-2]
+                k = optics.propagator.intensity_scale()
+                power = max(float(np.nansum(iobs[f])) for f in range(nframe))
+                pre = illum * (power / (4.0 * n * k))
+            self.preconditioner = (pre + illum) / 2.0
+        epoch = self.epochs
+        self.epochs += 1
+        size = min(self.batch_size, nframe)
+        steps = []
+        for frames in random_batches(nframe, size, self.seed, epoch):
+            views = [v for f in frames for v in views_all[f * nmode:(f + 1) * nmode]]
+            obj, probe, st = self.step(views, iobs, optics, obj, probe)
+            steps.append(st)
+        return obj, probe, steps
+
+    def llk(self, ptycho, obj, probe):
+        """Normalised LLK of obj, probe over every frame. ptyrs/src/engine/lsqml.rs:532-546"""
+        psi = obj_probe_to_psi(None, ptycho.views_all, obj, probe, ptycho.quad)
+        psi = ptycho.optics.exit_forward(psi)
+        return llk(ptycho.iobs, ptycho.addr.nmode, psi).normalised(ptycho.nb_obs)

@@ -148,7 +148,9 @@ class EngineBase {
 
 template <typename T, int N>
 class Engine : public EngineBase {
-  using S = Shape<T, N>;
+  using S = RowShape<T, N>;    // pass A
+  using SR = RowShapeC<T, N>;  // pass C
+  using SC = ColShape<T, N>;   // pass B
   static constexpr int n = N * N;
 
  public:
@@ -292,7 +294,7 @@ class Engine : public EngineBase {
 
   // Pass C chunking: enough workgroups to fill the chip, few partial probe sums.
   int chunk_of(int nb) const {
-    const int per = S::NBLK * M_;
+    const int per = SR::NBLK * M_;
     int nchunk = std::max(1, std::min(nb, (1024 + per - 1) / per));
     return (nb + nchunk - 1) / nchunk;
   }
@@ -302,7 +304,7 @@ class Engine : public EngineBase {
     psi_.ensure(sizeof(cplx<T>) * cap * M_ * n);
     omax_.ensure(sizeof(T) * cap * S::NBLK);
     // chunk_of never cuts more chunks than this, whatever the batch length.
-    const size_t per = size_t(S::NBLK) * M_;
+    const size_t per = size_t(SR::NBLK) * M_;
     const size_t nchunk = std::min(cap, (1024 + per - 1) / per);
     dP_.ensure(sizeof(cplx<T>) * std::max<size_t>(nchunk, 1) * M_ * n);
     list_.ensure(sizeof(int) * size_t(ntiles()) * cap);
@@ -310,7 +312,7 @@ class Engine : public EngineBase {
     tilemax_.ensure(sizeof(T) * size_t(ntiles()));
     pmax_.ensure(sizeof(T) * 1);
     order_.ensure(sizeof(int) * std::max<size_t>(1, nepoch * nframe_));
-    llk_part_.ensure(sizeof(double) * 4 * nframe_ * S::NBLK);
+    llk_part_.ensure(sizeof(double) * 4 * nframe_ * SC::NBLK);
     cap_ = std::max(cap_, cap);
   }
 
@@ -334,7 +336,7 @@ class Engine : public EngineBase {
   template <int MODE>
   void pass_b(const int* ord, int nb) {
     run(MODE == B_LLK ? K_LLK : K_PASS_B, [&] {
-      dim3 g(S::NBLK, nb), b(S::NT);
+      dim3 g(SC::NBLK, nb), b(SC::NT);
       auto args = [&](auto kern) {
         hipLaunchKernelGGL(kern, g, b, 0, stream_, psi_.as<cplx<T>>(), counts_.as<float>(), ord,
                            llk_part_.as<double>(), tw_.as<cplx<T>>());
@@ -360,7 +362,7 @@ class Engine : public EngineBase {
     }
     run(K_LLK, [&] {
       hipLaunchKernelGGL(k_llk_reduce, dim3(1), dim3(THREADS), 0, stream_,
-                         llk_part_.as<double>(), int(nframe_ * S::NBLK),
+                         llk_part_.as<double>(), int(nframe_ * SC::NBLK),
                          hist_.as<double>() + 4 * slot);
     });
   }
@@ -385,15 +387,12 @@ class Engine : public EngineBase {
     const int nt = ntiles();
     if (upd_obj) {
       run(K_BIN, [&] {
-        hipLaunchKernelGGL(k_bin_tiles, dim3((nt + 255) / 256), dim3(256), 0, stream_,
-                           corner_.as<int2>(), ord, nb, N, ntx_, nt, int(cap_), list_.as<int>(),
-                           lcount_.as<int>());
+        hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                           corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
+                           lcount_.as<int>(), P_.as<cplx<T>>(), M_, int(nyo_), int(nxo_),
+                           tilemax_.as<T>());
       });
       run(K_OVERLAP, [&] {
-        hipLaunchKernelGGL(k_obj_overlap<T>, dim3(nt), dim3(THREADS), 0, stream_,
-                           P_.as<cplx<T>>(), N, M_, corner_.as<int2>(), ord, list_.as<int>(),
-                           lcount_.as<int>(), int(cap_), int(nyo_), int(nxo_), ntx_,
-                           tilemax_.as<T>());
         hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(THREADS), 0, stream_,
                            tilemax_.as<T>(), nt, pmax_.as<T>());
       });
@@ -403,10 +402,10 @@ class Engine : public EngineBase {
     const int chunk = chunk_of(nb);
     const int nchunk = (nb + chunk - 1) / chunk;
     run(K_PASS_C, [&] {
-      hipLaunchKernelGGL((k_pass_c_rpie<T, N>), dim3(S::NBLK, M_, nchunk), dim3(S::NT), 0,
+      hipLaunchKernelGGL((k_pass_c_rpie<T, N>), dim3(SR::NBLK, M_, nchunk), dim3(SR::NT), 0,
                          stream_, psi_.as<cplx<T>>(), O_.as<cplx<T>>(), int(nxo_),
                          Wt_.as<cplx<T>>(), q_.as<cplx<T>>(), corner_.as<int2>(), ord, nb, M_,
-                         omax_.as<T>(), chunk, T(p.probe_alpha), int(upd_probe),
+                         omax_.as<T>(), int(S::NBLK), chunk, T(p.probe_alpha), int(upd_probe),
                          dP_.as<cplx<T>>(), tw_.as<cplx<T>>());
     });
     if (upd_obj) {

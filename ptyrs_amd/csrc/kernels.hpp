@@ -25,39 +25,50 @@ namespace ptyrs {
 
 constexpr int THREADS = 256;
 
-template <typename T>
-constexpr int choose_e(int n) {
-  // Elements per thread: 16 (float) or 8 (double) where the radix plan allows, else the first
-  // divisor of n that has a plan.
-  const int pref = sizeof(T) == 4 ? 16 : 8;
-  if (n % pref == 0 && n_stages(n, pref) > 0 && n / pref <= THREADS) return pref;
+// Elements per thread for a line of n: `pref` where its radix plan exists and the line fits in
+// THREADS_MAX threads, else the first divisor of n that has a plan.
+constexpr int choose_e(int n, int pref) {
+  if (n % pref == 0 && n_stages(n, pref) > 0 && n / pref <= 1024) return pref;
   const int cands[12] = {8, 16, 12, 24, 4, 6, 10, 20, 32, 40, 48, 2};
   for (int i = 0; i < 12; ++i) {
     int e = cands[i];
-    if (n % e == 0 && n_stages(n, e) > 0 && n / e <= THREADS) return e;
+    if (n % e == 0 && n_stages(n, e) > 0 && n / e <= 256) return e;
   }
   return 0;
 }
 
-// Lines per workgroup: the largest divisor of N whose lines fit in THREADS threads.
-constexpr int lines_per_block(int n, int tl) {
+// Lines per workgroup: the largest divisor of N with at most maxl lines and maxt threads.
+constexpr int lines_per_block(int n, int tl, int maxt, int maxl) {
   int best = 1;
-  for (int l = 1; l <= n; ++l)
-    if (n % l == 0 && l * tl <= THREADS) best = l;
+  for (int l = 1; l <= n && l <= maxl; ++l)
+    if (n % l == 0 && l * tl <= maxt) best = l;
   return best;
 }
 
-template <typename T, int N>
-struct Shape {
-  static constexpr int E = choose_e<T>(N);
+// A workgroup of L lines of N values, TL = N/E threads per line.
+template <int N, int E_, int MAXT, int MAXL>
+struct Lines {
+  static constexpr int E = E_;
   static_assert(E > 0, "unsupported frame side");
-  static constexpr int TL = N / E;                      // threads per line
-  static constexpr int L = lines_per_block(N, TL);      // lines (rows or columns) per workgroup
-  static constexpr int NT = TL * L;                     // threads per workgroup
+  static constexpr int TL = N / E;                            // threads per line
+  static constexpr int L = lines_per_block(N, TL, MAXT, MAXL);
+  static constexpr int NT = TL * L;                           // threads per workgroup
   static constexpr int LD = lds_line(N);
-  static constexpr int NBLK = N / L;                    // workgroups per plane
+  static constexpr int NBLK = N / L;                          // workgroups per plane
   static constexpr int LDS_WORDS = TL > 1 ? L * LD : 1;
 };
+
+// Row passes hold one mode per workgroup: 16 values per thread (float), 8 (double).
+template <typename T, int N>
+using RowShape = Lines<N, choose_e(N, sizeof(T) == 4 ? 16 : 8), THREADS, N>;
+// Pass C loops over frames with its probe terms held: 8 values per thread (float), 4 (double).
+template <typename T, int N>
+using RowShapeC = Lines<N, choose_e(N, sizeof(T) == 4 ? 8 : 4), THREADS, N>;
+// The column pass holds every mode of a 128-byte stripe of columns (16 float or 8 double
+// columns): 8 values per thread (float), 4 (double), so that four modes stay in registers at
+// several waves per SIMD.
+template <typename T, int N>
+using ColShape = Lines<N, choose_e(N, sizeof(T) == 4 ? 8 : 4), 512, sizeof(T) == 4 ? 16 : 8>;
 
 // ---- block reductions --------------------------------------------------------------------------
 template <int NT, typename V, typename Op>
@@ -98,11 +109,11 @@ __global__ void k_probe_terms(const cplx<T>* __restrict__ P, const cplx<T>* __re
 // Grid (NBLK, nb·M). View (b, m) = (blockIdx.y / M, blockIdx.y % M). With `omax_part`, mode-0
 // workgroups also write max |O|² over their rows of the window.
 template <typename T, int N>
-__global__ __launch_bounds__((Shape<T, N>::NT)) void k_pass_a(
+__global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a(
     const cplx<T>* __restrict__ O, int nxo, const cplx<T>* __restrict__ Wt,
     const int2* __restrict__ corner, const int* __restrict__ order, int M,
     cplx<T>* __restrict__ psi, T* __restrict__ omax_part, const cplx<T>* __restrict__ tw) {
-  using S = Shape<T, N>;
+  using S = RowShape<T, N>;
   constexpr int E = S::E, TL = S::TL;
   __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
   __shared__ T red[S::NT];
@@ -141,10 +152,10 @@ enum PassB { B_RPIE = 0, B_LLK = 1, B_AMPLITUDE = 2 };
 // LLK sums of the workgroup's pixels (FP64) to llk_part[frame][blockIdx.x][4] and stores
 // nothing.
 template <typename T, int N, int M, int MODE>
-__global__ __launch_bounds__((Shape<T, N>::NT)) void k_pass_b(
+__global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
     cplx<T>* __restrict__ psi, const float* __restrict__ counts, const int* __restrict__ order,
     double* __restrict__ llk_part, const cplx<T>* __restrict__ tw) {
-  using S = Shape<T, N>;
+  using S = ColShape<T, N>;
   constexpr int E = S::E, TL = S::TL, CG = S::L;
   constexpr int n = N * N;
   __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
@@ -218,15 +229,16 @@ __global__ __launch_bounds__((Shape<T, N>::NT)) void k_pass_b(
 // ---- pass C for rPIE: inverse rows, Δ = ψ' − W·O, probe direction ------------------------------
 // Grid (NBLK, M, nchunk). Each workgroup owns rows of mode m and runs over the batch slots of
 // its chunk in order: it transforms the rows back, stores Δ over the projected field, and adds
-// conj(O·q)·Δ / ((1 − α)|O|² + α·max_window |O|²) into a per-chunk probe direction.
+// conj(O·q)·Δ / ((1 − α)|O|² + α·max_window |O|²) into a per-chunk probe direction. The window
+// max comes from pass A's `na` partial maxima per slot.
 template <typename T, int N>
-__global__ __launch_bounds__((Shape<T, N>::NT)) void k_pass_c_rpie(
+__global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie(
     cplx<T>* __restrict__ psi, const cplx<T>* __restrict__ O, int nxo,
     const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ q,
     const int2* __restrict__ corner, const int* __restrict__ order, int nb, int M,
-    const T* __restrict__ omax_part, int chunk, T alpha, int probe_dir,
+    const T* __restrict__ omax_part, int na, int chunk, T alpha, int probe_dir,
     cplx<T>* __restrict__ dP_part, const cplx<T>* __restrict__ tw) {
-  using S = Shape<T, N>;
+  using S = RowShapeC<T, N>;
   constexpr int E = S::E, TL = S::TL;
   constexpr int n = N * N;
   __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
@@ -235,25 +247,26 @@ __global__ __launch_bounds__((Shape<T, N>::NT)) void k_pass_c_rpie(
   const int m = blockIdx.y, ch = blockIdx.z;
   const int b0 = ch * chunk;
   const int b1 = b0 + chunk < nb ? b0 + chunk : nb;
-  cplx<T> w[E], qq[E];
-  kahan<T> ar[E], ai[E];
+  cplx<T> w[E], qq[E], acc[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int x = jl + TL * e;
-    w[e] = ld(Wt + (size_t)m * n + y * N + x);
-    qq[e] = ld(q + y * N + x);
+    w[e] = ld(Wt + (size_t)m * n + (size_t)y * N + x);
+    qq[e] = ld(q + (size_t)y * N + x);
+    acc[e] = cplx<T>{0, 0};
   }
+  const T inv = T(1) - alpha;
   for (int b = b0; b < b1; ++b) {
     const int2 c = corner[order[b]];
-    T omax = 0;
-    for (int i = 0; i < S::NBLK; ++i) {
-      T t = omax_part[(size_t)b * S::NBLK + i];
-      omax = t > omax ? t : omax;
-    }
     cplx<T>* prow = psi + ((size_t)(b * M + m) * N + y) * N;
     cplx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = ld(prow + jl + TL * e);
+    T omax = 0;
+    for (int i = 0; i < na; ++i) {
+      T t = omax_part[(size_t)b * na + i];
+      omax = t > omax ? t : omax;
+    }
     line_fft<T, N, E, true>(v, jl, lre + l * S::LD, lim + l * S::LD, tw);
     const cplx<T>* orow = O + (size_t)(c.x + y) * nxo + c.y;
 #pragma unroll
@@ -263,17 +276,15 @@ __global__ __launch_bounds__((Shape<T, N>::NT)) void k_pass_c_rpie(
       const cplx<T> d = v[e] - w[e] * o;
       st(prow + x, d);
       if (probe_dir) {
-        const T den = (T(1) - alpha) * norm2(o) + alpha * omax;
-        const cplx<T> g = cmulc(o * qq[e], d) * (T(1) / den);
-        ar[e].add(g.x);
-        ai[e].add(g.y);
+        const T den = inv * norm2(o) + alpha * omax;
+        acc[e] = acc[e] + cmulc(o * qq[e], d) * (T(1) / den);
       }
     }
   }
   if (probe_dir) {
     cplx<T>* out = dP_part + ((size_t)ch * M + m) * n + (size_t)y * N;
 #pragma unroll
-    for (int e = 0; e < E; ++e) st(out + jl + TL * e, cplx<T>{ar[e].s, ai[e].s});
+    for (int e = 0; e < E; ++e) st(out + jl + TL * e, acc[e]);
   }
 }
 
@@ -299,47 +310,68 @@ __global__ void k_probe_apply(cplx<T>* __restrict__ P, cplx<T>* __restrict__ Wt,
 // ---- object tiles --------------------------------------------------------------------------------
 constexpr int TILE_X = 64, TILE_Y = 4;
 
-// For every object tile, the batch slots whose N×N window meets it, in slot order.
-static __global__ void k_bin_tiles(const int2* __restrict__ corner, const int* __restrict__ order, int nb,
-                            int N, int ntx, int ntiles, int cap, int* __restrict__ list,
-                            int* __restrict__ count) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntiles) return;
-  const int ty0 = (t / ntx) * TILE_Y, tx0 = (t % ntx) * TILE_X;
-  int k = 0;
-  for (int b = 0; b < nb; ++b) {
-    const int2 c = corner[order[b]];
-    if (c.x < ty0 + TILE_Y && c.x + N > ty0 && c.y < tx0 + TILE_X && c.y + N > tx0)
-      list[(size_t)t * cap + k++] = b;
-  }
-  count[t] = k;
-}
-
-// Per tile, max over its pixels of the batch overlap Σ_m Σ_views |P_m|² (pass 1 of rPIE's
-// object step: the regulariser's max).
+// One workgroup per object tile: (1) the batch slots whose N×N window meets the tile, in slot
+// order, compacted with wave ballots into list[tile][·] and count[tile]; (2) with `P`, the max
+// over the tile's pixels of the batch overlap Σ_m Σ_views |P_m|² (rPIE's regulariser max).
 template <typename T>
-__global__ __launch_bounds__(THREADS) void k_obj_overlap(
-    const cplx<T>* __restrict__ P, int N, int M, const int2* __restrict__ corner,
-    const int* __restrict__ order, const int* __restrict__ list, const int* __restrict__ count,
-    int cap, int nyo, int nxo, int ntx, T* __restrict__ tilemax) {
+__global__ __launch_bounds__(THREADS) void k_bin_tiles(
+    const int2* __restrict__ corner, const int* __restrict__ order, int nb, int N, int ntx,
+    int cap, int* __restrict__ list, int* __restrict__ count, const cplx<T>* __restrict__ P,
+    int M, int nyo, int nxo, T* __restrict__ tilemax) {
+  __shared__ int wave_n[THREADS / 64];
+  __shared__ int base_s;
   __shared__ T red[THREADS];
-  const int t = blockIdx.x;
-  const int Y = (t / ntx) * TILE_Y + threadIdx.x / TILE_X;
-  const int X = (t % ntx) * TILE_X + threadIdx.x % TILE_X;
-  const int nl = count[t];
+  const int t = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ty0 = (t / ntx) * TILE_Y, tx0 = (t % ntx) * TILE_X;
+  int* out = list + (size_t)t * cap;
+  if (tid == 0) base_s = 0;
+  __syncthreads();
+  for (int s = 0; s < nb; s += THREADS) {
+    const int b = s + tid;
+    bool hit = false;
+    if (b < nb) {
+      const int2 c = corner[order[b]];
+      hit = c.x < ty0 + TILE_Y && c.x + N > ty0 && c.y < tx0 + TILE_X && c.y + N > tx0;
+    }
+    const unsigned long long mask = __ballot(hit);
+    const int before = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_n[wave] = __popcll(mask);
+    __syncthreads();
+    int off = base_s;
+    for (int w = 0; w < wave; ++w) off += wave_n[w];
+    if (hit) out[off + before] = b;
+    __syncthreads();
+    if (tid == 0) {
+      int tot = 0;
+      for (int w = 0; w < THREADS / 64; ++w) tot += wave_n[w];
+      base_s += tot;
+    }
+    __syncthreads();
+  }
+  const int nl = base_s;
+  if (tid == 0) count[t] = nl;
+  if (P == nullptr) return;
+  __shared__ int2 cs[THREADS];
+  const int Y = ty0 + tid / TILE_X, X = tx0 + tid % TILE_X;
+  const bool inside = Y < nyo && X < nxo;
   const int n = N * N;
-  T s = 0;
-  if (Y < nyo && X < nxo) {
-    for (int i = 0; i < nl; ++i) {
-      const int2 c = corner[order[list[(size_t)t * cap + i]]];
-      const int y = Y - c.x, x = X - c.y;
-      if (y >= 0 && y < N && x >= 0 && x < N) {
-        for (int m = 0; m < M; ++m) s += norm2(ld(P + (size_t)m * n + y * N + x));
+  T sum = 0;
+  for (int s = 0; s < nl; s += THREADS) {
+    const int k = nl - s < THREADS ? nl - s : THREADS;
+    if (tid < k) cs[tid] = corner[order[out[s + tid]]];
+    __syncthreads();
+    if (inside) {
+      for (int i = 0; i < k; ++i) {
+        const int y = Y - cs[i].x, x = X - cs[i].y;
+        if (y >= 0 && y < N && x >= 0 && x < N) {
+          for (int m = 0; m < M; ++m) sum += norm2(ld(P + (size_t)m * n + y * N + x));
+        }
       }
     }
+    __syncthreads();
   }
-  T r = block_reduce<THREADS>(s, red, MaxOp());
-  if (threadIdx.x == 0) tilemax[t] = r;
+  T r = block_reduce<THREADS>(sum, red, MaxOp());
+  if (tid == 0) tilemax[t] = r;
 }
 
 template <typename T>
@@ -354,44 +386,66 @@ __global__ __launch_bounds__(THREADS) void k_max_reduce(const T* __restrict__ v,
 
 // rPIE object step, pass 2: every object pixel owned by one thread adds, in slot order,
 // O += step · Σ_m Σ_views conj(W_m)·Δ_m / ((1 − α)·D_m + α·pmax) with D_m the batch overlap
-// of |P_m|² at the pixel. No atomics; the result does not depend on the launch shape.
+// of |P_m|² at the pixel, the sums Kahan-compensated. No atomics; the result does not depend on
+// the launch shape.
 template <typename T>
 __global__ __launch_bounds__(THREADS) void k_obj_update_rpie(
     cplx<T>* __restrict__ O, const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ P,
     const cplx<T>* __restrict__ delta, int N, int M, const int2* __restrict__ corner,
     const int* __restrict__ order, const int* __restrict__ list, const int* __restrict__ count,
     int cap, int nyo, int nxo, int ntx, const T* __restrict__ pmax, T alpha, T step) {
-  const int t = blockIdx.x;
-  const int Y = (t / ntx) * TILE_Y + threadIdx.x / TILE_X;
-  const int X = (t % ntx) * TILE_X + threadIdx.x % TILE_X;
+  constexpr int MM = 4;
+  __shared__ int3 cs[THREADS];
+  const int t = blockIdx.x, tid = threadIdx.x;
+  const int Y = (t / ntx) * TILE_Y + tid / TILE_X;
+  const int X = (t % ntx) * TILE_X + tid % TILE_X;
   const int nl = count[t];
-  if (nl == 0 || Y >= nyo || X >= nxo) return;
+  if (nl == 0) return;
+  const bool inside = Y < nyo && X < nxo;
   const int n = N * N;
+  kahan<T> nr[MM], ni[MM], dd[MM];
+  bool covered = false;
+  const int* lst = list + (size_t)t * cap;
+  for (int s = 0; s < nl; s += THREADS) {
+   const int k = nl - s < THREADS ? nl - s : THREADS;
+   if (tid < k) {
+     const int b = lst[s + tid];
+     const int2 c = corner[order[b]];
+     cs[tid] = int3{b, c.x, c.y};
+   }
+   __syncthreads();
+   for (int i = 0; i < k && inside; ++i) {
+    const int b = cs[i].x;
+    const int y = Y - cs[i].y, x = X - cs[i].z;
+    if (y >= 0 && y < N && x >= 0 && x < N) {
+      const int j = y * N + x;
+      const cplx<T>* dv = delta + (size_t)b * M * n + j;
+      cplx<T> dl[MM];
+#pragma unroll
+      for (int m = 0; m < MM; ++m)
+        if (m < M) dl[m] = ld(dv + (size_t)m * n);
+#pragma unroll
+      for (int m = 0; m < MM; ++m) {
+        if (m < M) {
+          const cplx<T> g = cmulc(ld(Wt + (size_t)m * n + j), dl[m]);
+          nr[m].add(g.x);
+          ni[m].add(g.y);
+          dd[m].add(norm2(ld(P + (size_t)m * n + j)));
+        }
+      }
+      covered = true;
+    }
+   }
+   __syncthreads();
+  }
+  if (!covered) return;
   const T reg = alpha * pmax[0];
   cplx<T> upd = {0, 0};
-  bool covered = false;
-  for (int m = 0; m < M; ++m) {
-    kahan<T> nr, ni, dd;
-    for (int i = 0; i < nl; ++i) {
-      const int b = list[(size_t)t * cap + i];
-      const int2 c = corner[order[b]];
-      const int y = Y - c.x, x = X - c.y;
-      if (y >= 0 && y < N && x >= 0 && x < N) {
-        const int j = y * N + x;
-        const cplx<T> g = cmulc(ld(Wt + (size_t)m * n + j), ld(delta + ((size_t)b * M + m) * n + j));
-        nr.add(g.x);
-        ni.add(g.y);
-        dd.add(norm2(ld(P + (size_t)m * n + j)));
-        covered = true;
-      }
-    }
-    const T den = (T(1) - alpha) * dd.s + reg;
-    upd = upd + cplx<T>{nr.s, ni.s} * (T(1) / den);
-  }
-  if (covered) {
-    cplx<T>* o = O + (size_t)Y * nxo + X;
-    st(o, ld(o) + upd * step);
-  }
+#pragma unroll
+  for (int m = 0; m < MM; ++m)
+    if (m < M) upd = upd + cplx<T>{nr[m].s, ni[m].s} * (T(1) / ((T(1) - alpha) * dd[m].s + reg));
+  cplx<T>* o = O + (size_t)Y * nxo + X;
+  st(o, ld(o) + upd * step);
 }
 
 // ---- LLK sums -------------------------------------------------------------------------------------
