@@ -46,7 +46,9 @@ enum { PTYRS_AP = 0, PTYRS_DM = 1, PTYRS_RAAR = 2, PTYRS_RPIE = 3, PTYRS_ML = 4,
 /* A far-field scan, as ptyrs::base::Scan (ptyrs/src/base/scan.rs, ptyrs-py/src/lib.rs:433-463).
  * Exactly one of counts_u16 (65535 = masked, ptyrs/src/base/counts.rs:3-7) and counts_f32
  * (a negative value is masked; values above 65534 are kept, which the u16 path refuses) is
- * non-null. y, x are the sample positions in metres. */
+ * non-null. y, x are the sample positions in metres. corners, if non-null, gives the view
+ * corners [nframe][2] (oy, ox) directly, an address table as ptyrs::base::AddrTable holds it
+ * (ptyrs/src/base/addr.rs:20-81), and y, x are then not read. */
 typedef struct {
   size_t nframe, ny, nx;
   const uint16_t* counts_u16;
@@ -54,6 +56,7 @@ typedef struct {
   const double* y;
   const double* x;
   double wavelength, distance, detector_pixel;
+  const int64_t* corners;
 } ptyrs_scan;
 
 /* The exit-wave model (ptyrs::engine::Optics, ptyrs/src/engine/propagate.rs:24-186):
@@ -87,6 +90,22 @@ typedef struct {
   uint64_t seed;
 } ptyrs_rpie_params;
 
+/* LsqmlParams (ptyrs/src/engine/lsqml.rs:34-53) and Cost (ptyrs/src/engine/features.rs). */
+enum { PTYRS_AMPLITUDE = 0, PTYRS_POISSON = 1 };
+typedef struct {
+  size_t batch_size;
+  int cost;
+  uint64_t seed;
+} ptyrs_lsqml_params;
+
+/* One frame's LSQML step sizes of the last epoch (LsqmlStep, ptyrs/src/engine/lsqml.rs:55-66),
+ * one object slice: the batch index within the epoch, the frame, a_far, a_obj and a_probe. */
+typedef struct {
+  size_t batch;
+  size_t frame;
+  double a_far, a_obj, a_probe;
+} ptyrs_lsqml_step;
+
 /* One LLK record (LlkRecord, ptyrs/src/engine/ptycho.rs:33-41), normalised by nb_obs
  * (Llk::normalised, ptyrs/src/engine/ops.rs:268-276). */
 typedef struct {
@@ -106,6 +125,16 @@ int ptyrs_engine_create(const ptyrs_scan* scan, const ptyrs_optics* optics, cons
 
 /* GpuPtycho::rpie (ptyrs/src/engine/gpu/rpie.rs:31-66): nepoch epochs of rPIE. */
 int ptyrs_engine_rpie(ptyrs_engine* e, const ptyrs_rpie_params* params, size_t nepoch);
+
+/* GpuPtycho::lsqml (ptyrs/src/engine/gpu/lsqml.rs:75-120) without a probe variation or
+ * features: nepoch epochs from a fresh LSQML, an LLK record every calc_llk epochs and after
+ * the last. */
+int ptyrs_engine_lsqml(ptyrs_engine* e, const ptyrs_lsqml_params* params, size_t nepoch,
+                       size_t calc_llk);
+
+/* GpuPtycho::lsqml_steps (ptyrs/src/engine/gpu/lsqml.rs:122-127): the last LSQML epoch's steps,
+ * one per frame in batch order; copies up to cap and sets *n to the number held. */
+int ptyrs_engine_lsqml_steps(ptyrs_engine* e, ptyrs_lsqml_step* out, size_t cap, size_t* n);
 
 /* GpuPtycho::llk (ptyrs/src/engine/gpu/mod.rs:853-868): normalised LLK of the current state. */
 int ptyrs_engine_llk(ptyrs_engine* e, ptyrs_llk_record* out);

@@ -38,7 +38,8 @@ class _ScanC(ctypes.Structure):
                 ("counts_f32", ctypes.POINTER(ctypes.c_float)),
                 ("y", ctypes.POINTER(ctypes.c_double)), ("x", ctypes.POINTER(ctypes.c_double)),
                 ("wavelength", ctypes.c_double), ("distance", ctypes.c_double),
-                ("detector_pixel", ctypes.c_double)]
+                ("detector_pixel", ctypes.c_double),
+                ("corners", ctypes.POINTER(ctypes.c_int64))]
 
 
 class _OpticsC(ctypes.Structure):
@@ -60,6 +61,15 @@ class _RpieC(ctypes.Structure):
                 ("seed", ctypes.c_uint64)]
 
 
+class _LsqmlC(ctypes.Structure):
+    _fields_ = [("batch_size", ctypes.c_size_t), ("cost", ctypes.c_int), ("seed", ctypes.c_uint64)]
+
+
+class _StepC(ctypes.Structure):
+    _fields_ = [("batch", ctypes.c_size_t), ("frame", ctypes.c_size_t), ("a_far", ctypes.c_double),
+                ("a_obj", ctypes.c_double), ("a_probe", ctypes.c_double)]
+
+
 class _RecordC(ctypes.Structure):
     _fields_ = [("cycle", ctypes.c_size_t), ("algorithm", ctypes.c_int),
                 ("poisson", ctypes.c_double), ("gaussian", ctypes.c_double),
@@ -77,6 +87,9 @@ _SIGNATURES = [
                              _D, _SZ, ctypes.POINTER(_OptionsC), ctypes.c_int,
                              ctypes.POINTER(_P)], ctypes.c_int),
     ("ptyrs_engine_rpie", [_P, ctypes.POINTER(_RpieC), _SZ], ctypes.c_int),
+    ("ptyrs_engine_lsqml", [_P, ctypes.POINTER(_LsqmlC), _SZ, _SZ], ctypes.c_int),
+    ("ptyrs_engine_lsqml_steps", [_P, ctypes.POINTER(_StepC), _SZ, ctypes.POINTER(_SZ)],
+     ctypes.c_int),
     ("ptyrs_engine_llk", [_P, ctypes.POINTER(_RecordC)], ctypes.c_int),
     ("ptyrs_engine_object", [_P, _D, _SZ], ctypes.c_int),
     ("ptyrs_engine_probe", [_P, _D, _SZ], ctypes.c_int),
@@ -140,9 +153,11 @@ def _ptr(a, ctype):
 class Scan:
     """A far-field scan (ptyrs-py/src/lib.rs:430-463): counts `[nframe, ny, nx]` in detector
     order, uint16 with 65535 masked or float with negative values masked (float counts may exceed
-    65534, which the uint16 path cannot hold); positions `y`, `x` in metres; geometry in metres."""
+    65534, which the uint16 path cannot hold); positions `y`, `x` in metres; geometry in metres.
+    `corners` (`[nframe, 2]` integer (oy, ox)) gives the view corners directly, an address table
+    (ptyrs/src/base/addr.rs:20-81) in place of the positions' PyNX preprocessing."""
 
-    def __init__(self, counts, y, x, wavelength, distance, detector_pixel):
+    def __init__(self, counts, y, x, wavelength, distance, detector_pixel, corners=None):
         c = np.asarray(counts)
         if c.ndim != 3:
             raise TypeError("counts must be a 3-D uint16 or float array")
@@ -163,6 +178,11 @@ class Scan:
             if not (math.isfinite(v) and v > 0):
                 raise ValueError(f"{name} must be finite and > 0, got {v}")
         self.wavelength, self.distance, self.detector_pixel = wavelength, distance, detector_pixel
+        self.corners = None
+        if corners is not None:
+            self.corners = np.ascontiguousarray(corners, dtype=np.int64)
+            if self.corners.shape != (c.shape[0], 2):
+                raise ValueError(f"corners must be [{c.shape[0]}, 2], got {self.corners.shape}")
 
     @property
     def frame(self):
@@ -178,6 +198,8 @@ class Scan:
         s.y, s.x = _ptr(self.y, ctypes.c_double), _ptr(self.x, ctypes.c_double)
         s.wavelength, s.distance, s.detector_pixel = (self.wavelength, self.distance,
                                                       self.detector_pixel)
+        if self.corners is not None:
+            s.corners = _ptr(self.corners, ctypes.c_int64)
         return s
 
 
@@ -303,6 +325,40 @@ class GpuPtycho:
             raise ValueError("batch_size and calc_llk must be >= 0")
         c.batch_size, c.calc_llk, c.seed = int(p["batch_size"]), int(p["calc_llk"]), int(p["seed"])
         _check(library().ptyrs_engine_rpie(self._h, ctypes.byref(c), int(nepoch)))
+
+    def lsqml(self, nepoch, calc_llk, batch_size, cost, seed, **kw):
+        """`nepoch` LSQML epochs from a fresh LSQML (`cost` "amplitude" or "poisson"), an LLK
+        record every `calc_llk` epochs and after the last (ptyrs-py/src/gpu.rs:150-192). The
+        probe variation (`opr`, `hold`) is not supported."""
+        if kw:
+            raise TypeError(f"unsupported parameters {sorted(kw)}")
+        if cost not in ("amplitude", "poisson"):
+            raise ValueError(f'cost must be "amplitude" or "poisson", got {cost!r}')
+        c = _LsqmlC()
+        c.batch_size, c.cost, c.seed = int(batch_size), 0 if cost == "amplitude" else 1, int(seed)
+        _check(library().ptyrs_engine_lsqml(self._h, ctypes.byref(c), int(nepoch), int(calc_llk)))
+        return None
+
+    def lsqml_steps(self):
+        """The last LSQML epoch's steps as dicts of `frames`, `a_far`, `a_obj` (`[1, frames]`)
+        and `a_probe`, one per batch (ptyrs-py/src/lib.rs:236-248)."""
+        lib = library()
+        n = ctypes.c_size_t()
+        _check(lib.ptyrs_engine_lsqml_steps(self._h, None, 0, ctypes.byref(n)))
+        recs = (_StepC * max(n.value, 1))()
+        _check(lib.ptyrs_engine_lsqml_steps(self._h, recs, n.value, ctypes.byref(n)))
+        out = []
+        for r in recs[:n.value]:
+            if not out or out[-1]["batch"] != r.batch:
+                out.append(dict(batch=r.batch, frames=[], a_far=[], a_obj=[], a_probe=[]))
+            d = out[-1]
+            d["frames"].append(r.frame)
+            d["a_far"].append(r.a_far)
+            d["a_obj"].append(r.a_obj)
+            d["a_probe"].append(r.a_probe)
+        return [dict(frames=np.array(d["frames"]), a_far=np.array(d["a_far"]),
+                     a_obj=np.array(d["a_obj"])[None], a_probe=np.array(d["a_probe"]))
+                for d in out]
 
     def llk(self):
         """Normalised PyNX LLK of the current object and probe."""

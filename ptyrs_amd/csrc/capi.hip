@@ -39,7 +39,8 @@ Prepared prepare(const ptyrs_scan& s, const ptyrs_optics* optics, size_t nprobe,
   if (s.ny == 0 || s.ny % 2) throw Invalid("frame sides must be even and non-zero");
   if ((s.counts_u16 == nullptr) == (s.counts_f32 == nullptr))
     throw Invalid("give exactly one of counts_u16 and counts_f32");
-  if (s.nframe && (s.y == nullptr || s.x == nullptr)) throw Invalid("positions missing");
+  if (s.nframe && s.corners == nullptr && (s.y == nullptr || s.x == nullptr))
+    throw Invalid("positions missing");
   if (nprobe == 0 || nobj == 0) throw Invalid("nprobe and nobj must be >= 1");
   if (nyo < s.ny || nxo < s.nx) throw Invalid("object smaller than frame");
   Prepared p;
@@ -58,24 +59,36 @@ Prepared prepare(const ptyrs_scan& s, const ptyrs_optics* optics, size_t nprobe,
     if (p.exit.kind == PTYRS_ANGULAR_SPECTRUM)
       throw Invalid("an angular-spectrum exit transform is not run on the GPU yet");
   }
-  std::vector<double> y(s.y, s.y + s.nframe), x(s.x, s.x + s.nframe);
-  for (size_t f = 0; f < s.nframe; ++f)
-    if (!(std::isfinite(y[f]) && std::isfinite(x[f]) && std::fabs(y[f]) <= 1e10 &&
-          std::fabs(x[f]) <= 1e10))
-      throw Invalid("positions must be finite with |pos| <= 1e10");
-  y = pynx_axis(y, p.exit.pixel[0]);
-  x = pynx_axis(x, p.exit.pixel[1]);
   p.corner.resize(2 * s.nframe);
-  for (size_t f = 0; f < s.nframe; ++f) {
-    const long oy = view_corner(long(nyo), N, y[f] / p.exit.pixel[0]);
-    const long ox = view_corner(long(nxo), N, x[f] / p.exit.pixel[1]);
-    if (oy < 0 || ox < 0)
-      throw Invalid("frame " + std::to_string(f) + " leaves the object; size it with required_obj_shape");
-    p.corner[2 * f] = int(oy);
-    p.corner[2 * f + 1] = int(ox);
+  if (s.corners) {
+    for (size_t f = 0; f < s.nframe; ++f) {
+      const int64_t oy = s.corners[2 * f], ox = s.corners[2 * f + 1];
+      if (oy < 0 || ox < 0 || oy + N > int64_t(nyo) || ox + N > int64_t(nxo))
+        throw Invalid("corner of frame " + std::to_string(f) + " leaves the object");
+      p.corner[2 * f] = int(oy);
+      p.corner[2 * f + 1] = int(ox);
+    }
+  } else {
+    std::vector<double> y(s.y, s.y + s.nframe), x(s.x, s.x + s.nframe);
+    for (size_t f = 0; f < s.nframe; ++f)
+      if (!(std::isfinite(y[f]) && std::isfinite(x[f]) && std::fabs(y[f]) <= 1e10 &&
+            std::fabs(x[f]) <= 1e10))
+        throw Invalid("positions must be finite with |pos| <= 1e10");
+    y = pynx_axis(y, p.exit.pixel[0]);
+    x = pynx_axis(x, p.exit.pixel[1]);
+    for (size_t f = 0; f < s.nframe; ++f) {
+      const long oy = view_corner(long(nyo), N, y[f] / p.exit.pixel[0]);
+      const long ox = view_corner(long(nxo), N, x[f] / p.exit.pixel[1]);
+      if (oy < 0 || ox < 0)
+        throw Invalid("frame " + std::to_string(f) +
+                      " leaves the object; size it with required_obj_shape");
+      p.corner[2 * f] = int(oy);
+      p.corner[2 * f + 1] = int(ox);
+    }
   }
   const size_t n = size_t(N) * N;
   p.counts.resize(s.nframe * n);
+  double frame_total = 0;
   for (size_t f = 0; f < s.nframe; ++f) {
     for (int ky = 0; ky < N; ++ky)
       for (int kx = 0; kx < N; ++kx) {
@@ -99,9 +112,12 @@ Prepared prepare(const ptyrs_scan& s, const ptyrs_optics* optics, size_t nprobe,
         if (v >= 0) {
           ++p.nb_obs;
           p.total += v;
+          frame_total += v;
         }
         p.counts[f * n + size_t(ky) * N + kx] = v;
       }
+    p.max_frame_total = std::max(p.max_frame_total, frame_total);
+    frame_total = 0;
   }
   return p;
 }
@@ -168,6 +184,25 @@ int ptyrs_engine_rpie(ptyrs_engine* h, const ptyrs_rpie_params* p, size_t nepoch
     return PTYRS_EINVAL;
   }
   return guarded(h, true, [&](EngineBase& e) { e.rpie(*p, nepoch); });
+}
+
+int ptyrs_engine_lsqml(ptyrs_engine* h, const ptyrs_lsqml_params* p, size_t nepoch,
+                       size_t calc_llk) {
+  if (!p) {
+    last_error = "null params";
+    return PTYRS_EINVAL;
+  }
+  return guarded(h, true, [&](EngineBase& e) { e.lsqml(*p, nepoch, calc_llk); });
+}
+
+int ptyrs_engine_lsqml_steps(ptyrs_engine* h, ptyrs_lsqml_step* out, size_t cap, size_t* n) {
+  return guarded(h, false, [&](EngineBase& e) {
+    if (n) *n = e.lsqml_steps.size();
+    for (size_t i = 0; i < e.lsqml_steps.size() && i < cap && out; ++i) {
+      const StepRecord& r = e.lsqml_steps[i];
+      out[i] = ptyrs_lsqml_step{size_t(r.batch), size_t(r.frame), r.a_far, r.a_obj, r.a_probe};
+    }
+  });
 }
 
 int ptyrs_engine_llk(ptyrs_engine* h, ptyrs_llk_record* out) {

@@ -62,7 +62,15 @@ struct Prepared {
   std::vector<int> corner;    // [nframe][2] oy, ox
   size_t nb_obs = 0;
   double total = 0;
+  double max_frame_total = 0;  // max over frames of the unmasked counts' sum
   ExitModel exit;
+};
+
+// One LSQML slot's step sizes (LsqmlStep, ptyrs/src/engine/lsqml.rs:55-66), single slice.
+struct StepRecord {
+  int batch;
+  int frame;
+  double a_far, a_obj, a_probe;
 };
 
 class EngineBase {
@@ -71,6 +79,7 @@ class EngineBase {
     if (stream_) (void)hipStreamDestroy(stream_);
   }
   virtual void rpie(const ptyrs_rpie_params& p, size_t nepoch) = 0;
+  virtual void lsqml(const ptyrs_lsqml_params& p, size_t nepoch, size_t calc_llk) = 0;
   virtual void llk(double out[4]) = 0;
   virtual void get_object(double* out) = 0;
   virtual void get_probe(double* out) = 0;
@@ -83,6 +92,7 @@ class EngineBase {
   size_t probe_len() const { return 2 * nprobe_ * ny_ * nx_; }
 
   std::vector<Record> history;
+  std::vector<StepRecord> lsqml_steps;  // the last LSQML epoch's
   size_t cycle = 0;
   bool aborted = false;
   std::vector<int> corners;
@@ -167,6 +177,8 @@ class Engine : public EngineBase {
     nxo_ = nxo;
     nb_obs_ = pre.nb_obs;
     corners = pre.corner;
+    scale_ = pre.exit.scale;
+    max_frame_total_ = pre.max_frame_total;
     M_ = int(nprobe * nobj);
 
     O_.ensure(sizeof(cplx<T>) * nobj * nyo * nxo);
@@ -265,14 +277,87 @@ class Engine : public EngineBase {
     flush_timing();
   }
 
+  // GpuPtycho::lsqml (ptyrs/src/engine/gpu/lsqml.rs:75-120) without OPR or positions: nepoch
+  // epochs from a fresh LSQML (preconditioner, a_far = 1/2), batches random_batches(nframe,
+  // min(batch, nframe), seed, epoch) with the epoch counted from 0 in every call.
+  void lsqml(const ptyrs_lsqml_params& p, size_t nepoch, size_t calc_llk) override {
+    if (nobj_ != 1) throw Invalid("the GPU LSQML runs one object mode");
+    if (p.batch_size == 0) throw Invalid("LSQML batch_size must be >= 1");
+    if (p.cost != PTYRS_AMPLITUDE && p.cost != PTYRS_POISSON)
+      throw Invalid("cost must be amplitude or poisson");
+    if (M_ > 4) throw Invalid("the GPU engine runs at most 4 modes");
+    if (nepoch == 0 || nframe_ == 0) {
+      cycle += nepoch;
+      return;
+    }
+    const size_t size = std::min(p.batch_size, nframe_);
+    ensure_workspace(size, nepoch);
+    ensure_lsqml(size);
+    std::vector<int> order(nepoch * nframe_);
+    std::vector<std::vector<std::pair<int, int>>> spans(nepoch);
+    for (size_t ic = 0; ic < nepoch; ++ic) {
+      auto batches = random_batches(nframe_, size, p.seed, uint64_t(ic));
+      size_t off = 0;
+      for (auto& b : batches) {
+        std::copy(b.begin(), b.end(), order.begin() + ic * nframe_ + off);
+        spans[ic].push_back({int(off), int(b.size())});
+        off += b.size();
+      }
+    }
+    PTYRS_HIP(hipMemcpyAsync(order_.p, order.data(), sizeof(int) * order.size(),
+                             hipMemcpyHostToDevice, stream_));
+    {
+      std::vector<T> half(nframe_, T(0.5));
+      PTYRS_HIP(hipMemcpyAsync(afar_all_.p, half.data(), sizeof(T) * nframe_,
+                               hipMemcpyHostToDevice, stream_));
+      PTYRS_HIP(hipStreamSynchronize(stream_));
+    }
+    probe_terms();
+    std::vector<std::pair<size_t, int>> due;
+    for (size_t ic = 0; ic < nepoch; ++ic) {
+      if (ic < 10 || ic % 10 == 0) precondition(ic == 0);
+      const int* ord = order_.as<int>() + ic * nframe_;
+      for (auto& s : spans[ic]) lsqml_batch(p, ord + s.first, s.second, s.first);
+      const bool rec = calc_llk > 0 && (ic % calc_llk == 0 || ic + 1 == nepoch);
+      if (rec) {
+        if (due.size() == kHistSlots) drain(due, PTYRS_LSQML);
+        const int slot = int(due.size());
+        llk_pass(slot);
+        due.push_back({cycle, slot});
+      }
+      ++cycle;
+    }
+    drain(due, PTYRS_LSQML);
+    // The last epoch's steps.
+    std::vector<double> lg(3 * nframe_);
+    PTYRS_HIP(hipMemcpyAsync(lg.data(), log_.p, sizeof(double) * 3 * nframe_,
+                             hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    flush_timing();
+    lsqml_steps.clear();
+    const int* last = order.data() + (nepoch - 1) * nframe_;
+    int bi = 0;
+    for (auto& s : spans[nepoch - 1]) {
+      for (int k = 0; k < s.second; ++k) {
+        const int slot = s.first + k;
+        lsqml_steps.push_back(StepRecord{bi, last[slot], lg[3 * slot], lg[3 * slot + 1],
+                                         lg[3 * slot + 2]});
+      }
+      ++bi;
+    }
+  }
+
  private:
   static constexpr size_t kHistSlots = 64;
   static constexpr size_t kDefaultStack = 128;
   int M_ = 1;
+  double scale_ = 1.0, max_frame_total_ = 0.0;
+  DevBuf dO_, pre_, illum_, afar_all_, afar_tmp_, prev_, icalc_, xi_part_, xi_part2_, dots_part_,
+      sums_, log_, step_, pre_part_, floor_, dPm_;
   int ntx_ = 0, nty_ = 0;
   size_t cap_ = 0, nchunk_cap_ = 0, order_cap_ = 0;
   DevBuf O_, P_, Wt_, q_, tw_, counts_, corner_, iota_, hist_;
-  DevBuf psi_, omax_, dP_, list_, lcount_, tilemax_, pmax_, order_, llk_part_;
+  DevBuf psi_, omax_, dP_, list_, lcount_, tilemax_, pmax_, order_, llk_part_, D_;
 
   void upload_complex(DevBuf& d, const double* src, size_t count) {
     if (stream_) PTYRS_HIP(hipStreamSynchronize(stream_));
@@ -288,6 +373,146 @@ class Engine : public EngineBase {
       out[2 * i] = double(h[i].x);
       out[2 * i + 1] = double(h[i].y);
     }
+  }
+
+  void ensure_lsqml(size_t cap) {
+    const size_t no = nyo_ * nxo_;
+    dO_.ensure(sizeof(cplx<T>) * no);
+    pre_.ensure(sizeof(T) * no);
+    illum_.ensure(sizeof(T) * no);
+    afar_all_.ensure(sizeof(T) * std::max<size_t>(1, nframe_));
+    afar_tmp_.ensure(sizeof(T) * cap);
+    prev_.ensure(sizeof(T) * 2);
+    icalc_.ensure(sizeof(T) * cap * n);
+    xi_part_.ensure(sizeof(double) * 2 * cap * SC::NBLK);
+    xi_part2_.ensure(sizeof(double) * 2 * cap * aux_blocks());
+    dots_part_.ensure(sizeof(double) * 6 * cap * aux_blocks());
+    sums_.ensure(sizeof(double) * 6 * cap);
+    log_.ensure(sizeof(double) * 3 * std::max<size_t>(1, nframe_));
+    step_.ensure(sizeof(double) * 2);
+    pre_part_.ensure(sizeof(T) * kPreBlocks);
+    floor_.ensure(sizeof(T) * 1);
+    dPm_.ensure(sizeof(cplx<T>) * nprobe_ * n);
+  }
+
+  static constexpr int kPreBlocks = 1024;
+  // Workgroups per frame of the per-frame reductions.
+  static int aux_blocks() { return std::max(1, n / 4096); }
+
+  // Preconditioner update at an epoch's start (ptyrs/src/engine/lsqml.rs:671-699):
+  // illumination of every view at the current probe, then pre = (pre + illum)/2, the first
+  // from illum·max_f Σ iobs_f/(4·n·k); floor = (0.1·max pre)².
+  void precondition(bool first) {
+    const size_t no = nyo_ * nxo_;
+    PTYRS_HIP(hipMemsetAsync(illum_.p, 0, sizeof(T) * no, stream_));
+    const int nt = ntiles();
+    const int step = int(cap_);
+    for (int f0 = 0; f0 < int(nframe_); f0 += step) {
+      const int nb = std::min(step, int(nframe_) - f0);
+      const int* ord = iota_.as<int>() + f0;
+      run(K_BIN, [&] {
+        hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                           corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
+                           lcount_.as<int>(), (const cplx<T>*)nullptr, M_, int(nyo_),
+                           int(nxo_), (T*)nullptr, tilemax_.as<T>());
+      });
+      run(K_OTHER, [&] {
+        hipLaunchKernelGGL(k_illumination<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                           illum_.as<T>(), P_.as<cplx<T>>(), N, M_, corner_.as<int2>(), ord,
+                           list_.as<int>(), lcount_.as<int>(), int(cap_), int(nyo_), int(nxo_),
+                           ntx_);
+      });
+    }
+    const double init = max_frame_total_ / (4.0 * double(n) * scale_);
+    run(K_OTHER, [&] {
+      hipLaunchKernelGGL(k_pre_update<T>, dim3(kPreBlocks), dim3(THREADS), 0, stream_,
+                         pre_.as<T>(), illum_.as<T>(), no, int(first), init, pre_part_.as<T>());
+      hipLaunchKernelGGL(k_pre_floor<T>, dim3(1), dim3(THREADS), 0, stream_, pre_part_.as<T>(),
+                         kPreBlocks, floor_.as<T>());
+    });
+  }
+
+  // One LSQML batch (ptyrs/src/engine/lsqml.rs:563-643), single slice, no features.
+  void lsqml_batch(const ptyrs_lsqml_params& p, const int* ord, int nb, int off) {
+    const int nt = ntiles();
+    const bool gauss = p.cost == PTYRS_AMPLITUDE;
+    run(K_BIN, [&] {
+      hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                         corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
+                         lcount_.as<int>(), (const cplx<T>*)nullptr, M_, int(nyo_), int(nxo_),
+                         (T*)nullptr, tilemax_.as<T>());
+    });
+    pass_a(ord, nb, false);
+    BArgs<T> ba{afar_tmp_.as<T>(), prev_.as<T>(), xi_part_.as<double>(), icalc_.as<T>()};
+    if (gauss) {
+      pass_b<B_GAUSS>(ord, nb, ba);
+    } else {
+      const int xb = aux_blocks();
+      run(K_OTHER, [&] {
+        hipLaunchKernelGGL(k_lsqml_prev<T>, dim3(1), dim3(THREADS), 0, stream_,
+                           afar_all_.as<T>(), ord, nb, prev_.as<T>());
+      });
+      pass_b<B_ICALC>(ord, nb, ba);
+      run(K_OTHER, [&] {
+        hipLaunchKernelGGL(k_lsqml_xi<T>, dim3(1), dim3(THREADS), 0, stream_,
+                           xi_part_.as<double>(), int(SC::NBLK), nb, prev_.as<T>(),
+                           afar_tmp_.as<T>(), prev_.as<T>() + 1);
+        hipLaunchKernelGGL(k_lsqml_xi_sums<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
+                           icalc_.as<T>(), counts_.as<float>(), ord, n, prev_.as<T>() + 1,
+                           xi_part2_.as<double>());
+        hipLaunchKernelGGL(k_lsqml_xi<T>, dim3(1), dim3(THREADS), 0, stream_,
+                           xi_part2_.as<double>(), xb, nb, prev_.as<T>() + 1, afar_tmp_.as<T>(),
+                           (T*)nullptr);
+      });
+      pass_b<B_POISSON>(ord, nb, ba);
+    }
+    run(K_OTHER, [&] {
+      hipLaunchKernelGGL(k_lsqml_log_afar<T>, dim3((nb + 255) / 256), dim3(256), 0, stream_,
+                         afar_tmp_.as<T>(), ord, nb, afar_all_.as<T>(),
+                         log_.as<double>() + 3 * size_t(off), int(gauss));
+    });
+    const int chunk = chunk_of(nb);
+    const int nchunk = (nb + chunk - 1) / chunk;
+    run(K_PASS_C, [&] {
+      hipLaunchKernelGGL((k_pass_c_rpie<T, N>), dim3(SR::NBLK, M_, nchunk), dim3(SR::NT), 0,
+                         stream_, psi_.as<cplx<T>>(), O_.as<cplx<T>>(), int(nxo_),
+                         Wt_.as<cplx<T>>(), q_.as<cplx<T>>(), corner_.as<int2>(), ord, nb, M_,
+                         omax_.as<T>(), int(S::NBLK), chunk, T(0), 1, dP_.as<cplx<T>>(),
+                         tw_.as<cplx<T>>(), 0);
+    });
+    run(K_OBJ_UPDATE, [&] {
+      hipLaunchKernelGGL(k_lsqml_obj_dir<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                         dO_.as<cplx<T>>(), Wt_.as<cplx<T>>(), psi_.as<cplx<T>>(), N, M_,
+                         corner_.as<int2>(), ord, list_.as<int>(), lcount_.as<int>(), int(cap_),
+                         int(nyo_), int(nxo_), ntx_, pre_.as<T>(), floor_.as<T>());
+    });
+    const int total = int(nprobe_ * n);
+    const int xb = aux_blocks();
+    // λ0 = 1.2e-7·k/n per pixel and mode (ptyrs/src/engine/lsqml.rs:31-33), summed over both.
+    const double lam = 1.2e-7 * scale_ * double(M_);
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_probe_mean<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,
+                         dPm_.as<cplx<T>>(), dP_.as<cplx<T>>(), nchunk, total, nb);
+    });
+    run(K_OTHER, [&] {
+      hipLaunchKernelGGL(k_lsqml_dots<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
+                         dO_.as<cplx<T>>(), O_.as<cplx<T>>(), int(nxo_), Wt_.as<cplx<T>>(),
+                         dPm_.as<cplx<T>>(), q_.as<cplx<T>>(), psi_.as<cplx<T>>(), N, M_,
+                         corner_.as<int2>(), ord, dots_part_.as<double>());
+      hipLaunchKernelGGL(k_lsqml_step<T>, dim3(1), dim3(1024), 0, stream_,
+                         dots_part_.as<double>(), xb, nb, lam, M_, sums_.as<double>(),
+                         log_.as<double>() + 3 * size_t(off), step_.as<double>());
+    });
+    const size_t no = nyo_ * nxo_;
+    run(K_OBJ_UPDATE, [&] {
+      hipLaunchKernelGGL(k_lsqml_apply_obj<T>, dim3((no + 255) / 256), dim3(256), 0, stream_,
+                         O_.as<cplx<T>>(), dO_.as<cplx<T>>(), step_.as<double>(), no);
+    });
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_lsqml_apply_probe<T>, dim3((total + 255) / 256), dim3(256), 0,
+                         stream_, P_.as<cplx<T>>(), Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
+                         dPm_.as<cplx<T>>(), step_.as<double>(), n, total);
+    });
   }
 
   int ntiles() const { return ntx_ * nty_; }
@@ -310,6 +535,7 @@ class Engine : public EngineBase {
     list_.ensure(sizeof(int) * size_t(ntiles()) * cap);
     lcount_.ensure(sizeof(int) * size_t(ntiles()));
     tilemax_.ensure(sizeof(T) * size_t(ntiles()));
+    D_.ensure(sizeof(T) * size_t(M_) * nyo_ * nxo_);
     pmax_.ensure(sizeof(T) * 1);
     order_.ensure(sizeof(int) * std::max<size_t>(1, nepoch * nframe_));
     llk_part_.ensure(sizeof(double) * 4 * nframe_ * SC::NBLK);
@@ -334,12 +560,12 @@ class Engine : public EngineBase {
   }
 
   template <int MODE>
-  void pass_b(const int* ord, int nb) {
+  void pass_b(const int* ord, int nb, BArgs<T> ba = BArgs<T>{nullptr, nullptr, nullptr, nullptr}) {
     run(MODE == B_LLK ? K_LLK : K_PASS_B, [&] {
       dim3 g(SC::NBLK, nb), b(SC::NT);
       auto args = [&](auto kern) {
         hipLaunchKernelGGL(kern, g, b, 0, stream_, psi_.as<cplx<T>>(), counts_.as<float>(), ord,
-                           llk_part_.as<double>(), tw_.as<cplx<T>>());
+                           llk_part_.as<double>(), tw_.as<cplx<T>>(), ba);
       };
       switch (M_) {
         case 1: args(k_pass_b<T, N, 1, MODE>); break;
@@ -389,8 +615,8 @@ class Engine : public EngineBase {
       run(K_BIN, [&] {
         hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
                            corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
-                           lcount_.as<int>(), P_.as<cplx<T>>(), M_, int(nyo_), int(nxo_),
-                           tilemax_.as<T>());
+                           lcount_.as<int>(), Wt_.as<cplx<T>>(), M_, int(nyo_), int(nxo_),
+                           D_.as<T>(), tilemax_.as<T>());
       });
       run(K_OVERLAP, [&] {
         hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(THREADS), 0, stream_,
@@ -406,12 +632,12 @@ class Engine : public EngineBase {
                          stream_, psi_.as<cplx<T>>(), O_.as<cplx<T>>(), int(nxo_),
                          Wt_.as<cplx<T>>(), q_.as<cplx<T>>(), corner_.as<int2>(), ord, nb, M_,
                          omax_.as<T>(), int(S::NBLK), chunk, T(p.probe_alpha), int(upd_probe),
-                         dP_.as<cplx<T>>(), tw_.as<cplx<T>>());
+                         dP_.as<cplx<T>>(), tw_.as<cplx<T>>(), 1);
     });
     if (upd_obj) {
       run(K_OBJ_UPDATE, [&] {
         hipLaunchKernelGGL(k_obj_update_rpie<T>, dim3(nt), dim3(THREADS), 0, stream_,
-                           O_.as<cplx<T>>(), Wt_.as<cplx<T>>(), P_.as<cplx<T>>(),
+                           O_.as<cplx<T>>(), Wt_.as<cplx<T>>(), D_.as<T>(),
                            psi_.as<cplx<T>>(), N, M_, corner_.as<int2>(), ord, list_.as<int>(),
                            lcount_.as<int>(), int(cap_), int(nyo_), int(nxo_), ntx_,
                            pmax_.as<T>(), T(p.obj_alpha), T(p.obj_step));

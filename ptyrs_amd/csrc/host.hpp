@@ -103,6 +103,9 @@ struct ExitModel {
   bool reverse = false;      // the detector sees the inverse DFT (Fresnel with z < 0)
   double pixel[2] = {0, 0};  // object-plane pixel [py, px], metres
   int kind = 0;              // PTYRS_FOURIER / PTYRS_FRESNEL / PTYRS_ANGULAR_SPECTRUM
+  // pty-chi's intensity scale k of the propagator (ptyrs/src/engine/propagate.rs:373-385):
+  // the engine's probe is pty-chi's times √k.
+  double scale = 1.0;
 };
 
 inline void set_phase(std::vector<double>& q, size_t i, double ph) {
@@ -115,6 +118,7 @@ inline void set_phase(std::vector<double>& q, size_t i, double ph) {
 inline ExitModel far_field(int ny, int nx, double wavelength, double distance, double dpix) {
   ExitModel m;
   m.kind = PTYRS_FOURIER;
+  m.scale = double(ny) * nx;
   const double lz = wavelength * distance;
   m.pixel[0] = lz / (ny * dpix);
   m.pixel[1] = lz / (nx * dpix);
@@ -149,6 +153,7 @@ inline ExitModel free_space(int ny, int nx, double wavelength, double py, double
   for (size_t i = 0; i < size_t(ny) * nx; ++i) m.quad[2 * i] = 1.0;
   if (std::isinf(distance)) {
     m.kind = PTYRS_FOURIER;
+    m.scale = double(ny) * nx;
     return m;
   }
   const double w = px / wavelength, ar = px / py, d = distance / wavelength;
@@ -161,6 +166,7 @@ inline ExitModel free_space(int ny, int nx, double wavelength, double py, double
   m.kind = PTYRS_FRESNEL;
   const double fr = w * w / d;
   m.reverse = distance < 0.0;
+  m.scale = double(ny) * nx * (fr / ar) * (fr / ar);
   for (int y = 0; y < ny; ++y)
     for (int x = 0; x < nx; ++x) {
       const double xx = double(x) - double(nx / 2), yy = double(y) - double(ny / 2);

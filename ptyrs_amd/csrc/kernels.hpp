@@ -144,17 +144,30 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a(
 }
 
 // ---- pass B: column transforms, the detector-plane operator, inverse column transforms --------
-enum PassB { B_RPIE = 0, B_LLK = 1, B_AMPLITUDE = 2 };
+enum PassB { B_RPIE = 0, B_LLK = 1, B_AMPLITUDE = 2, B_GAUSS = 3, B_ICALC = 4, B_POISSON = 5 };
+
+// Per-slot values some pass-B modes read or write.
+template <typename T>
+struct BArgs {
+  const T* a_far;   // B_POISSON: LSQML far-field step per batch slot
+  const T* prev;    // B_ICALC: batch mean of the slots' last a_far
+  double* part;     // B_ICALC: per-workgroup (num, den) of the Poisson xi solver
+  T* icalc;         // B_ICALC: model intensity per slot and pixel, DFT index order
+};
 
 // Grid (NBLK, nb). Each workgroup holds L columns of every mode of one frame. B_RPIE scales
 // the field by the pty-chi modulus sqrt(iobs + 1e-7)/(sqrt(icalc) + 1e-7) (masked pixels keep
 // their value), with the 1/n of the unnormalised round trip folded in; B_LLK writes the four
 // LLK sums of the workgroup's pixels (FP64) to llk_part[frame][blockIdx.x][4] and stores
 // nothing.
+// B_GAUSS and B_POISSON scale the field by (1 − a_far·c)/n with LSQML's residual c (Gaussian
+// 2(1 − √iobs/(√icalc + 1e-9)) with a_far = 1/2, Poisson 1 − iobs/(icalc + 1e-6)), 0 where
+// masked (ptyrs/src/engine/lsqml.rs:97-170). B_ICALC stores icalc and the first pass of the
+// Poisson a_far solver's sums and stores no field.
 template <typename T, int N, int M, int MODE>
 __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
     cplx<T>* __restrict__ psi, const float* __restrict__ counts, const int* __restrict__ order,
-    double* __restrict__ llk_part, const cplx<T>* __restrict__ tw) {
+    double* __restrict__ llk_part, const cplx<T>* __restrict__ tw, BArgs<T> ba) {
   using S = ColShape<T, N>;
   constexpr int E = S::E, TL = S::TL, CG = S::L;
   constexpr int n = N * N;
@@ -176,6 +189,10 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
   const float* cnt = counts + (size_t)f * n;
   const T inv_n = T(1) / T(n);
   double acc[4] = {0, 0, 0, 0};
+  T afar = 0;
+  double prev = 0;
+  if constexpr (MODE == B_POISSON) afar = ba.a_far[b];
+  if constexpr (MODE == B_ICALC) prev = double(ba.prev[0]);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const float io = cnt[(jl + TL * e) * N + x];
@@ -196,6 +213,25 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
       }
 #pragma unroll
       for (int m = 0; m < M; ++m) v[m][e] = v[m][e] * r;
+    } else if constexpr (MODE == B_GAUSS || MODE == B_POISSON) {
+      const T ic = I * inv_n;
+      T c = 0;
+      if (io >= 0.f) {
+        if constexpr (MODE == B_GAUSS) c = T(2) * (T(1) - sqrt(T(io)) / (sqrt(ic) + T(1e-9)));
+        else c = T(1) - T(io) / (ic + T(1e-6));
+      }
+      const T r = (T(1) - (MODE == B_GAUSS ? T(0.5) : afar) * c) * inv_n;
+#pragma unroll
+      for (int m = 0; m < M; ++m) v[m][e] = v[m][e] * r;
+    } else if constexpr (MODE == B_ICALC) {
+      const T ic = I * inv_n;
+      ba.icalc[(size_t)b * n + (jl + TL * e) * N + x] = ic;
+      if (io >= 0.f) {
+        const double yp = double(ic), yt = double(io);
+        const double xi = 1.0 - yt / (yp + 1e-5);
+        acc[0] += xi * (yp - yt / (1.0 - prev * xi));
+        acc[1] += xi * xi * yp;
+      }
     } else {
       if (io >= 0.f) {
         const double ic = double(I) * (1.0 / double(n));
@@ -215,6 +251,13 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
       double s = block_reduce<S::NT>(acc[i], red, AddOp());
       if (tid == 0) llk_part[((size_t)f * S::NBLK + blockIdx.x) * 4 + i] = s;
     }
+  } else if constexpr (MODE == B_ICALC) {
+    __shared__ double red[S::NT];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      double s = block_reduce<S::NT>(acc[i], red, AddOp());
+      if (tid == 0) ba.part[((size_t)b * S::NBLK + blockIdx.x) * 2 + i] = s;
+    }
   } else {
 #pragma unroll
     for (int m = 0; m < M; ++m) {
@@ -226,18 +269,19 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
   }
 }
 
-// ---- pass C for rPIE: inverse rows, Δ = ψ' − W·O, probe direction ------------------------------
+// ---- pass C: inverse rows, Δ = ψ' − W·O, probe direction ----------------------------------------
 // Grid (NBLK, M, nchunk). Each workgroup owns rows of mode m and runs over the batch slots of
-// its chunk in order: it transforms the rows back, stores Δ over the projected field, and adds
-// conj(O·q)·Δ / ((1 − α)|O|² + α·max_window |O|²) into a per-chunk probe direction. The window
-// max comes from pass A's `na` partial maxima per slot.
+// its chunk in order: it transforms the rows back, stores Δ (rPIE) or χ (LSQML) over the
+// projected field, and adds conj(O·q)·Δ, divided by (1 − α)|O|² + α·max_window |O|² when
+// `weighted` (rPIE), into a per-chunk probe direction. The window max comes from pass A's
+// `na` partial maxima per slot.
 template <typename T, int N>
 __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie(
     cplx<T>* __restrict__ psi, const cplx<T>* __restrict__ O, int nxo,
     const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ q,
     const int2* __restrict__ corner, const int* __restrict__ order, int nb, int M,
     const T* __restrict__ omax_part, int na, int chunk, T alpha, int probe_dir,
-    cplx<T>* __restrict__ dP_part, const cplx<T>* __restrict__ tw) {
+    cplx<T>* __restrict__ dP_part, const cplx<T>* __restrict__ tw, int weighted) {
   using S = RowShapeC<T, N>;
   constexpr int E = S::E, TL = S::TL;
   constexpr int n = N * N;
@@ -263,9 +307,11 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie(
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = ld(prow + jl + TL * e);
     T omax = 0;
-    for (int i = 0; i < na; ++i) {
-      T t = omax_part[(size_t)b * na + i];
-      omax = t > omax ? t : omax;
+    if (weighted) {
+      for (int i = 0; i < na; ++i) {
+        T t = omax_part[(size_t)b * na + i];
+        omax = t > omax ? t : omax;
+      }
     }
     line_fft<T, N, E, true>(v, jl, lre + l * S::LD, lim + l * S::LD, tw);
     const cplx<T>* orow = O + (size_t)(c.x + y) * nxo + c.y;
@@ -276,8 +322,8 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie(
       const cplx<T> d = v[e] - w[e] * o;
       st(prow + x, d);
       if (probe_dir) {
-        const T den = inv * norm2(o) + alpha * omax;
-        acc[e] = acc[e] + cmulc(o * qq[e], d) * (T(1) / den);
+        const cplx<T> g = cmulc(o * qq[e], d);
+        acc[e] = acc[e] + (weighted ? g * (T(1) / (inv * norm2(o) + alpha * omax)) : g);
       }
     }
   }
@@ -311,13 +357,14 @@ __global__ void k_probe_apply(cplx<T>* __restrict__ P, cplx<T>* __restrict__ Wt,
 constexpr int TILE_X = 64, TILE_Y = 4;
 
 // One workgroup per object tile: (1) the batch slots whose N×N window meets the tile, in slot
-// order, compacted with wave ballots into list[tile][·] and count[tile]; (2) with `P`, the max
-// over the tile's pixels of the batch overlap Σ_m Σ_views |P_m|² (rPIE's regulariser max).
+// order, compacted with wave ballots into list[tile][·] and count[tile]; (2) with `Wt`, each
+// pixel's batch overlap D_m = Σ_views |W_m|² per mode into D[m][pixel] (W = P·q, |q| = 1) and
+// the tile max of Σ_m D_m (rPIE's regulariser max).
 template <typename T>
 __global__ __launch_bounds__(THREADS) void k_bin_tiles(
     const int2* __restrict__ corner, const int* __restrict__ order, int nb, int N, int ntx,
-    int cap, int* __restrict__ list, int* __restrict__ count, const cplx<T>* __restrict__ P,
-    int M, int nyo, int nxo, T* __restrict__ tilemax) {
+    int cap, int* __restrict__ list, int* __restrict__ count, const cplx<T>* __restrict__ Wt,
+    int M, int nyo, int nxo, T* __restrict__ D, T* __restrict__ tilemax) {
   __shared__ int wave_n[THREADS / 64];
   __shared__ int base_s;
   __shared__ T red[THREADS];
@@ -350,12 +397,13 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
   }
   const int nl = base_s;
   if (tid == 0) count[t] = nl;
-  if (P == nullptr) return;
+  if (Wt == nullptr) return;
   __shared__ int2 cs[THREADS];
+  constexpr int MM = 4;
   const int Y = ty0 + tid / TILE_X, X = tx0 + tid % TILE_X;
   const bool inside = Y < nyo && X < nxo;
   const int n = N * N;
-  T sum = 0;
+  T dm[MM] = {0, 0, 0, 0};
   for (int s = 0; s < nl; s += THREADS) {
     const int k = nl - s < THREADS ? nl - s : THREADS;
     if (tid < k) cs[tid] = corner[order[out[s + tid]]];
@@ -364,11 +412,23 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
       for (int i = 0; i < k; ++i) {
         const int y = Y - cs[i].x, x = X - cs[i].y;
         if (y >= 0 && y < N && x >= 0 && x < N) {
-          for (int m = 0; m < M; ++m) sum += norm2(ld(P + (size_t)m * n + y * N + x));
+#pragma unroll
+          for (int m = 0; m < MM; ++m)
+            if (m < M) dm[m] += norm2(ld(Wt + (size_t)m * n + y * N + x));
         }
       }
     }
     __syncthreads();
+  }
+  T sum = 0;
+  if (inside && nl > 0) {
+    const size_t no = (size_t)nyo * nxo, p = (size_t)Y * nxo + X;
+#pragma unroll
+    for (int m = 0; m < MM; ++m)
+      if (m < M) {
+        D[m * no + p] = dm[m];
+        sum += dm[m];
+      }
   }
   T r = block_reduce<THREADS>(sum, red, MaxOp());
   if (tid == 0) tilemax[t] = r;
@@ -386,11 +446,11 @@ __global__ __launch_bounds__(THREADS) void k_max_reduce(const T* __restrict__ v,
 
 // rPIE object step, pass 2: every object pixel owned by one thread adds, in slot order,
 // O += step · Σ_m Σ_views conj(W_m)·Δ_m / ((1 − α)·D_m + α·pmax) with D_m the batch overlap
-// of |P_m|² at the pixel, the sums Kahan-compensated. No atomics; the result does not depend on
-// the launch shape.
+// of |W_m|² at the pixel (from k_bin_tiles), the sums Kahan-compensated. No atomics; the result
+// does not depend on the launch shape.
 template <typename T>
 __global__ __launch_bounds__(THREADS) void k_obj_update_rpie(
-    cplx<T>* __restrict__ O, const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ P,
+    cplx<T>* __restrict__ O, const cplx<T>* __restrict__ Wt, const T* __restrict__ D,
     const cplx<T>* __restrict__ delta, int N, int M, const int2* __restrict__ corner,
     const int* __restrict__ order, const int* __restrict__ list, const int* __restrict__ count,
     int cap, int nyo, int nxo, int ntx, const T* __restrict__ pmax, T alpha, T step) {
@@ -403,49 +463,365 @@ __global__ __launch_bounds__(THREADS) void k_obj_update_rpie(
   if (nl == 0) return;
   const bool inside = Y < nyo && X < nxo;
   const int n = N * N;
-  kahan<T> nr[MM], ni[MM], dd[MM];
+  kahan<T> nr[MM], ni[MM];
   bool covered = false;
   const int* lst = list + (size_t)t * cap;
   for (int s = 0; s < nl; s += THREADS) {
-   const int k = nl - s < THREADS ? nl - s : THREADS;
-   if (tid < k) {
-     const int b = lst[s + tid];
-     const int2 c = corner[order[b]];
-     cs[tid] = int3{b, c.x, c.y};
-   }
-   __syncthreads();
-   for (int i = 0; i < k && inside; ++i) {
-    const int b = cs[i].x;
-    const int y = Y - cs[i].y, x = X - cs[i].z;
-    if (y >= 0 && y < N && x >= 0 && x < N) {
-      const int j = y * N + x;
-      const cplx<T>* dv = delta + (size_t)b * M * n + j;
-      cplx<T> dl[MM];
-#pragma unroll
-      for (int m = 0; m < MM; ++m)
-        if (m < M) dl[m] = ld(dv + (size_t)m * n);
-#pragma unroll
-      for (int m = 0; m < MM; ++m) {
-        if (m < M) {
-          const cplx<T> g = cmulc(ld(Wt + (size_t)m * n + j), dl[m]);
-          nr[m].add(g.x);
-          ni[m].add(g.y);
-          dd[m].add(norm2(ld(P + (size_t)m * n + j)));
-        }
-      }
-      covered = true;
+    const int k = nl - s < THREADS ? nl - s : THREADS;
+    if (tid < k) {
+      const int b = lst[s + tid];
+      const int2 c = corner[order[b]];
+      cs[tid] = int3{b, c.x, c.y};
     }
-   }
-   __syncthreads();
+    __syncthreads();
+    for (int i = 0; i < k && inside; ++i) {
+      const int y = Y - cs[i].y, x = X - cs[i].z;
+      if (y >= 0 && y < N && x >= 0 && x < N) {
+        const int j = y * N + x;
+        const cplx<T>* dv = delta + (size_t)cs[i].x * M * n + j;
+        cplx<T> dl[MM], w[MM];
+#pragma unroll
+        for (int m = 0; m < MM; ++m)
+          if (m < M) {
+            dl[m] = ld(dv + (size_t)m * n);
+            w[m] = ld(Wt + (size_t)m * n + j);
+          }
+#pragma unroll
+        for (int m = 0; m < MM; ++m)
+          if (m < M) {
+            const cplx<T> g = cmulc(w[m], dl[m]);
+            nr[m].add(g.x);
+            ni[m].add(g.y);
+          }
+        covered = true;
+      }
+    }
+    __syncthreads();
   }
   if (!covered) return;
   const T reg = alpha * pmax[0];
+  const size_t no = (size_t)nyo * nxo, p = (size_t)Y * nxo + X;
   cplx<T> upd = {0, 0};
 #pragma unroll
   for (int m = 0; m < MM; ++m)
-    if (m < M) upd = upd + cplx<T>{nr[m].s, ni[m].s} * (T(1) / ((T(1) - alpha) * dd[m].s + reg));
-  cplx<T>* o = O + (size_t)Y * nxo + X;
+    if (m < M) upd = upd + cplx<T>{nr[m].s, ni[m].s} * (T(1) / ((T(1) - alpha) * D[m * no + p] + reg));
+  cplx<T>* o = O + p;
   st(o, ld(o) + upd * step);
+}
+
+// ---- LSQML ------------------------------------------------------------------------------------------
+// Batch mean of the slots' last a_far (the Poisson solver's first `prev`).
+template <typename T>
+__global__ void k_lsqml_prev(const T* __restrict__ a_far_all, const int* __restrict__ order, int nb,
+                             T* __restrict__ prev) {
+  __shared__ double red[THREADS];
+  double s = 0;
+  for (int b = threadIdx.x; b < nb; b += THREADS) s += double(a_far_all[order[b]]);
+  double r = block_reduce<THREADS>(s, red, AddOp());
+  if (threadIdx.x == 0) prev[0] = T(r / nb);
+}
+
+// One pass of the Poisson a_far solver per slot: a = clip(prev/2 + ratio/2, 0, 1) with ratio
+// num/den of the slot's sums (0 where den ≤ 0); writes a[b] and, with `prev_out`, their batch
+// mean. ptyrs/src/engine/lsqml.rs:137-155
+template <typename T>
+__global__ void k_lsqml_xi(const double* __restrict__ part, int nblk, int nb,
+                           const T* __restrict__ prev, T* __restrict__ a, T* __restrict__ prev_out) {
+  __shared__ double red[THREADS];
+  double mine = 0;
+  for (int b = threadIdx.x; b < nb; b += THREADS) {
+    double num = 0, den = 0;
+    for (int i = 0; i < nblk; ++i) {
+      num += part[((size_t)b * nblk + i) * 2];
+      den += part[((size_t)b * nblk + i) * 2 + 1];
+    }
+    const double ratio = den > 0.0 ? num / den : 0.0;
+    double v = 0.5 * double(prev[0]) + 0.5 * ratio;
+    v = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+    a[b] = T(v);
+    mine += v;
+  }
+  double r = block_reduce<THREADS>(mine, red, AddOp());
+  if (threadIdx.x == 0 && prev_out) prev_out[0] = T(r / nb);
+}
+
+// Second pass sums of the Poisson solver from stored icalc: grid (nblk, nb), per workgroup
+// (num, den) with `prev` the first pass's batch mean.
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_lsqml_xi_sums(
+    const T* __restrict__ icalc, const float* __restrict__ counts, const int* __restrict__ order,
+    int n, const T* __restrict__ prev, double* __restrict__ part) {
+  __shared__ double red[THREADS];
+  const int b = blockIdx.y, f = order[b];
+  const double pv = double(prev[0]);
+  double num = 0, den = 0;
+  for (int j = blockIdx.x * THREADS + threadIdx.x; j < n; j += gridDim.x * THREADS) {
+    const float io = counts[(size_t)f * n + j];
+    if (io >= 0.f) {
+      const double yp = double(icalc[(size_t)b * n + j]), yt = double(io);
+      const double xi = 1.0 - yt / (yp + 1e-5);
+      num += xi * (yp - yt / (1.0 - pv * xi));
+      den += xi * xi * yp;
+    }
+  }
+  double rn = block_reduce<THREADS>(num, red, AddOp());
+  double rd = block_reduce<THREADS>(den, red, AddOp());
+  if (threadIdx.x == 0) {
+    part[((size_t)b * gridDim.x + blockIdx.x) * 2] = rn;
+    part[((size_t)b * gridDim.x + blockIdx.x) * 2 + 1] = rd;
+  }
+}
+
+// LSQML object direction: every object pixel owned by one thread sums, in slot order,
+// Σ_m Σ_views conj(W_m)·χ_m (Kahan-compensated) and divides by √(pre² + floor); 0 where no view
+// covers it. ptyrs/src/engine/lsqml.rs:183-188, :587-598
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_lsqml_obj_dir(
+    cplx<T>* __restrict__ dO, const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ chi,
+    int N, int M, const int2* __restrict__ corner, const int* __restrict__ order,
+    const int* __restrict__ list, const int* __restrict__ count, int cap, int nyo, int nxo,
+    int ntx, const T* __restrict__ pre, const T* __restrict__ floor) {
+  __shared__ int3 cs[THREADS];
+  const int t = blockIdx.x, tid = threadIdx.x;
+  const int Y = (t / ntx) * TILE_Y + tid / TILE_X;
+  const int X = (t % ntx) * TILE_X + tid % TILE_X;
+  const bool inside = Y < nyo && X < nxo;
+  const int nl = count[t];
+  const int n = N * N;
+  kahan<T> sr, si;
+  const int* lst = list + (size_t)t * cap;
+  for (int s = 0; s < nl; s += THREADS) {
+    const int k = nl - s < THREADS ? nl - s : THREADS;
+    if (tid < k) {
+      const int b = lst[s + tid];
+      const int2 c = corner[order[b]];
+      cs[tid] = int3{b, c.x, c.y};
+    }
+    __syncthreads();
+    for (int i = 0; i < k && inside; ++i) {
+      const int y = Y - cs[i].y, x = X - cs[i].z;
+      if (y >= 0 && y < N && x >= 0 && x < N) {
+        const int j = y * N + x;
+        const cplx<T>* cv = chi + (size_t)cs[i].x * M * n + j;
+        for (int m = 0; m < M; ++m) {
+          const cplx<T> g = cmulc(ld(Wt + (size_t)m * n + j), ld(cv + (size_t)m * n));
+          sr.add(g.x);
+          si.add(g.y);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!inside) return;
+  const size_t p = (size_t)Y * nxo + X;
+  const T pr = pre[p];
+  const T inv = T(1) / sqrt(pr * pr + floor[0]);
+  st(dO + p, cplx<T>{sr.s * inv, si.s * inv});
+}
+
+// dP = Σ_chunks partials / nb (the frame mean of the probe directions).
+template <typename T>
+__global__ void k_probe_mean(cplx<T>* __restrict__ dP, const cplx<T>* __restrict__ dP_part,
+                             int nchunk, int total, int nb) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  kahan<T> sr, si;
+  for (int c = 0; c < nchunk; ++c) {
+    cplx<T> d = ld(dP_part + (size_t)c * total + i);
+    sr.add(d.x);
+    si.add(d.y);
+  }
+  st(dP + i, cplx<T>{sr.s, si.s} * (T(1) / T(nb)));
+}
+
+// Per slot and workgroup the six sums of the step-size system (FP64):
+// Σ|dop|², Σ|dpo|², Re Σ dop·conj(dpo), Im Σ dop·conj(dpo), Re Σ conj(dop)·χ, Re Σ conj(dpo)·χ
+// over modes and the workgroup's pixels, dop = dO_window·W_m, dpo = O_window·dP_m·q.
+// Grid (nblk, nb). ptyrs/src/engine/lsqml.rs:300-318
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_lsqml_dots(
+    const cplx<T>* __restrict__ dO, const cplx<T>* __restrict__ O, int nxo,
+    const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ dP, const cplx<T>* __restrict__ q,
+    const cplx<T>* __restrict__ chi, int N, int M, const int2* __restrict__ corner,
+    const int* __restrict__ order, double* __restrict__ part) {
+  __shared__ double red[THREADS];
+  const int b = blockIdx.y;
+  const int2 c = corner[order[b]];
+  const int n = N * N;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (int j = blockIdx.x * THREADS + threadIdx.x; j < n; j += gridDim.x * THREADS) {
+    const int y = j / N, x = j % N;
+    const size_t p = (size_t)(c.x + y) * nxo + c.y + x;
+    const cplx<T> dow = ld(dO + p), ow = ld(O + p), qj = ld(q + j);
+    for (int m = 0; m < M; ++m) {
+      const cplx<T> u = dow * ld(Wt + (size_t)m * n + j);
+      const cplx<T> v = ow * ld(dP + (size_t)m * n + j) * qj;
+      const cplx<T> ch = ld(chi + ((size_t)b * M + m) * n + j);
+      const cplx<T> a12 = cmulc(v, u);  // u·conj(v)
+      s[0] += double(norm2(u));
+      s[1] += double(norm2(v));
+      s[2] += double(a12.x);
+      s[3] += double(a12.y);
+      s[4] += double(cmulc(u, ch).x);
+      s[5] += double(cmulc(v, ch).x);
+    }
+  }
+  for (int i = 0; i < 6; ++i) {
+    double r = block_reduce<THREADS>(s[i], red, AddOp());
+    if (threadIdx.x == 0) part[((size_t)b * gridDim.x + blockIdx.x) * 6 + i] = r;
+  }
+}
+
+// One workgroup: per slot the 2×2 step-size system in FP64 (ptyrs/src/engine/lsqml.rs:319-335),
+// a_obj = 0.9·max(x0, 0)/M and a_probe = 0.9·max(x1, 0); then the object step, the trimmed mean
+// of a_obj (MATLAB trimmean 10 %, ranks with ties broken by slot), and the probe step, the mean
+// of a_probe, into step[0], step[1]. Per slot a_obj, a_probe go to log[b][1..2] (log[b][0] is
+// a_far, written before).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_lsqml_step(const double* __restrict__ part, int nblk,
+                                                     int nb, double lam, int M,
+                                                     double* __restrict__ sums,
+                                                     double* __restrict__ log,
+                                                     double* __restrict__ step) {
+  __shared__ double red[1024];
+  const int tid = threadIdx.x;
+  double m0 = 0, m1 = 0;
+  for (int b = tid; b < nb; b += 1024) {
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < nblk; ++i)
+      for (int k = 0; k < 6; ++k) s[k] += part[((size_t)b * nblk + i) * 6 + k];
+    s[0] += lam;
+    s[1] += lam;
+    for (int k = 0; k < 6; ++k) sums[(size_t)b * 6 + k] = s[k];
+    m0 += s[0];
+    m1 += s[1];
+  }
+  const double reg11 = 0.1 * block_reduce<1024>(m0, red, AddOp()) / nb;
+  const double reg22 = 0.1 * block_reduce<1024>(m1, red, AddOp()) / nb;
+  double pmean = 0;
+  for (int b = tid; b < nb; b += 1024) {
+    const double* s = sums + (size_t)b * 6;
+    const double a11 = s[0] + reg11, a22 = s[1] + reg22;
+    const double det = a11 * a22 - (s[2] * s[2] + s[3] * s[3]);
+    const double x0 = (a22 * s[4] - s[2] * s[5]) / det;
+    const double x1 = (a11 * s[5] - s[2] * s[4]) / det;
+    const double ao = 0.9 * (x0 > 0.0 ? x0 : 0.0) / double(M);
+    const double ap = 0.9 * (x1 > 0.0 ? x1 : 0.0);
+    log[(size_t)b * 3 + 1] = ao;
+    log[(size_t)b * 3 + 2] = ap;
+    pmean += ap;
+  }
+  const double psum = block_reduce<1024>(pmean, red, AddOp());
+  __syncthreads();
+  // Trimmed mean: slot b's rank among a_obj (ties by slot), kept ranks [k, nb − k).
+  const int kcut = int(ceil(double(nb) * 0.1 / 2.0 - 0.5));
+  double kept = 0;
+  for (int b = tid; b < nb; b += 1024) {
+    const double v = log[(size_t)b * 3 + 1];
+    int rank = 0;
+    for (int o = 0; o < nb; ++o) {
+      const double w = log[(size_t)o * 3 + 1];
+      rank += (w < v) || (w == v && o < b);
+    }
+    if (rank >= kcut && rank < nb - kcut) kept += v;
+  }
+  const double ksum = block_reduce<1024>(kept, red, AddOp());
+  if (tid == 0) {
+    step[0] = ksum / double(nb - 2 * kcut);
+    step[1] = psum / double(nb);
+  }
+}
+
+// O += step[0]·dO over the object.
+template <typename T>
+__global__ void k_lsqml_apply_obj(cplx<T>* __restrict__ O, const cplx<T>* __restrict__ dO,
+                                  const double* __restrict__ step, size_t total) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  st(O + i, ld(O + i) + ld(dO + i) * T(step[0]));
+}
+
+// P += step[1]·dP, then Wt = P·q.
+template <typename T>
+__global__ void k_lsqml_apply_probe(cplx<T>* __restrict__ P, cplx<T>* __restrict__ Wt,
+                                    const cplx<T>* __restrict__ q, const cplx<T>* __restrict__ dP,
+                                    const double* __restrict__ step, int n, int total) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  cplx<T> p = ld(P + i) + ld(dP + i) * T(step[1]);
+  st(P + i, p);
+  st(Wt + i, p * ld(q + i % n));
+}
+
+// Illumination Σ_m Σ_views |P_m|² over a tile list's views, added into illum (each pixel owned
+// by one thread).
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_illumination(
+    T* __restrict__ illum, const cplx<T>* __restrict__ P, int N, int M,
+    const int2* __restrict__ corner, const int* __restrict__ order, const int* __restrict__ list,
+    const int* __restrict__ count, int cap, int nyo, int nxo, int ntx) {
+  __shared__ int2 cs[THREADS];
+  const int t = blockIdx.x, tid = threadIdx.x;
+  const int Y = (t / ntx) * TILE_Y + tid / TILE_X;
+  const int X = (t % ntx) * TILE_X + tid % TILE_X;
+  const bool inside = Y < nyo && X < nxo;
+  const int nl = count[t];
+  const int n = N * N;
+  T sum = 0;
+  for (int s = 0; s < nl; s += THREADS) {
+    const int k = nl - s < THREADS ? nl - s : THREADS;
+    if (tid < k) cs[tid] = corner[order[list[(size_t)t * cap + s + tid]]];
+    __syncthreads();
+    for (int i = 0; i < k && inside; ++i) {
+      const int y = Y - cs[i].x, x = X - cs[i].y;
+      if (y >= 0 && y < N && x >= 0 && x < N)
+        for (int m = 0; m < M; ++m) sum += norm2(ld(P + (size_t)m * n + y * N + x));
+    }
+    __syncthreads();
+  }
+  if (inside && nl > 0) illum[(size_t)Y * nxo + X] += sum;
+}
+
+// pre = (pre + illum)/2, or on the first update (illum·init + illum)/2; pre's max goes to
+// tilemax-style partials for the floor (0.1·max pre)². ptyrs/src/engine/lsqml.rs:671-699
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_pre_update(T* __restrict__ pre, const T* __restrict__ illum,
+                                                        size_t total, int first, double init,
+                                                        T* __restrict__ part) {
+  __shared__ T red[THREADS];
+  T mx = 0;
+  for (size_t i = (size_t)blockIdx.x * THREADS + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * THREADS) {
+    const T il = illum[i];
+    const T p = first ? T(double(il) * init) : pre[i];
+    const T v = (p + il) / T(2);
+    pre[i] = v;
+    mx = v > mx ? v : mx;
+  }
+  T r = block_reduce<THREADS>(mx, red, MaxOp());
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// floor = (0.1·max)² from partial maxima.
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_pre_floor(const T* __restrict__ part, int count,
+                                                       T* __restrict__ floor) {
+  __shared__ T red[THREADS];
+  T m = 0;
+  for (int i = threadIdx.x; i < count; i += THREADS) m = part[i] > m ? part[i] : m;
+  T r = block_reduce<THREADS>(m, red, MaxOp());
+  if (threadIdx.x == 0) floor[0] = (T(0.1) * r) * (T(0.1) * r);
+}
+
+// The slot's a_far into the step log and the per-frame store.
+template <typename T>
+__global__ void k_lsqml_log_afar(const T* __restrict__ a, const int* __restrict__ order, int nb,
+                                 T* __restrict__ a_far_all, double* __restrict__ log, int gauss) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const T v = gauss ? T(0.5) : a[b];
+  a_far_all[order[b]] = v;
+  log[(size_t)b * 3] = double(v);
 }
 
 // ---- LLK sums -------------------------------------------------------------------------------------
