@@ -34,23 +34,26 @@ def _compile(src, obj, defines=()):
     return obj
 
 
-def build(jobs=None, verbose=False):
-    OUT.mkdir(exist_ok=True)
+def build(jobs=None, verbose=False, lib=LIB, out=OUT, defines=(), sides=SIDES):
+    """Compiles every unit (with extra `defines`) into `out` and links `lib`."""
+    out = pathlib.Path(out)
+    lib = pathlib.Path(lib)
+    out.mkdir(parents=True, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 4)
-    units = [(CSRC / "capi.hip", OUT / "capi.o", ())]
-    units += [(CSRC / "engine_inst.hip", OUT / f"engine_{n}.o", (f"-DPTYRS_N={n}",))
-              for n in SIDES]
+    units = [(CSRC / "capi.hip", out / "capi.o", tuple(defines))]
+    units += [(CSRC / "engine_inst.hip", out / f"engine_{n}.o", (f"-DPTYRS_N={n}", *defines))
+              for n in sides]
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda u: _compile(*u), units))
-    tmp = LIB.with_suffix(".so.tmp")
+    tmp = lib.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print(f"built {LIB}")
-    return LIB
+        print(f"built {lib}")
+    return lib
 
 
 if __name__ == "__main__":

@@ -160,58 +160,61 @@ __device__ __forceinline__ void fft_stage(cplx<T> (&v)[E], int jl, T* re, T* im,
   constexpr int TL = N / E;
   constexpr int U = E / R;
   constexpr bool LAST = (NS * R == N);
-  cplx<T> out[E];
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    cplx<T> a[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) a[r] = v[u + U * r];
-    const int jb = jl + TL * u;
-    if constexpr (NS > 1) {
-      const int k = jb % NS;
-#pragma unroll
-      for (int r = 1; r < R; ++r) a[r] = a[r] * tw[k * r * (N / (NS * R))];
-    }
-    dft<T, R>(a);
-#pragma unroll
-    for (int r = 0; r < R; ++r) out[u + U * r] = a[r];
-  }
-  if constexpr (LAST) {
-#pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = out[e];
-  } else if constexpr (TL == 1) {
+  if constexpr (!LAST && TL == 1) {
     // One thread per line: the exchange is a register permutation known at compile time.
     cplx<T> t[N];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
+      cplx<T> a[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int jb = u;
-        const int idx = (jb / NS) * NS * R + jb % NS + r * NS;
-        t[idx] = out[u + U * r];
+      for (int r = 0; r < R; ++r) a[r] = v[u + U * r];
+      if constexpr (NS > 1) {
+        const int k = u % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) a[r] = a[r] * tw[k * r * (N / (NS * R))];
       }
+      dft<T, R>(a);
+#pragma unroll
+      for (int r = 0; r < R; ++r) t[(u / NS) * NS * R + u % NS + r * NS] = a[r];
     }
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = t[e];
   } else {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int jb = jl + TL * u;
-      const int base = (jb / NS) * NS * R + jb % NS;
+      cplx<T> a[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int idx = lds_pad(base + r * NS);
-        re[idx] = out[u + U * r].x;
-        im[idx] = out[u + U * r].y;
+      for (int r = 0; r < R; ++r) a[r] = v[u + U * r];
+      const int jb = jl + TL * u;
+      if constexpr (NS > 1) {
+        const int k = jb % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) a[r] = a[r] * tw[k * r * (N / (NS * R))];
+      }
+      dft<T, R>(a);
+      if constexpr (LAST) {
+        // The last stage writes to index jb + r·N/R, the element it read: in place.
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[u + U * r] = a[r];
+      } else {
+        const int base = (jb / NS) * NS * R + jb % NS;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int idx = lds_pad(base + r * NS);
+          re[idx] = a[r].x;
+          im[idx] = a[r].y;
+        }
       }
     }
-    __syncthreads();
+    if constexpr (!LAST) {
+      __syncthreads();
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int idx = lds_pad(jl + TL * e);
-      v[e] = cplx<T>{re[idx], im[idx]};
+      for (int e = 0; e < E; ++e) {
+        const int idx = lds_pad(jl + TL * e);
+        v[e] = cplx<T>{re[idx], im[idx]};
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 

