@@ -103,6 +103,14 @@ struct AddOp {
   __device__ V operator()(V a, V b) const { return a + b; }
 };
 
+// The DFT twiddle table exp(−2πi·t/N), t < N, copied to LDS once per workgroup: every butterfly
+// of every line then reads it from LDS instead of global memory.
+template <typename T, int N, int NT>
+__device__ __forceinline__ void stage_twiddles(cplx<T>* stw, const cplx<T>* __restrict__ tw) {
+  for (int i = threadIdx.x; i < N; i += NT) stw[i] = ld(tw + i);
+  __syncthreads();
+}
+
 // ---- probe terms ------------------------------------------------------------------------------
 // Wt[m][i] = P[m][i]·q[i].
 template <typename T>
@@ -124,9 +132,11 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a(
   constexpr int E = S::E, TL = S::TL;
   __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
   __shared__ T red[S::NT];
+  __shared__ cplx<T> stw[N];
   const int tid = threadIdx.x, l = tid / TL, jl = tid % TL;
   const int y = blockIdx.x * S::L + l;
   const int b = blockIdx.y / M, m = blockIdx.y % M;
+  stage_twiddles<T, N, S::NT>(stw, tw);
   const int2 c = corner[order[b]];
   const cplx<T>* orow = O + (size_t)(c.x + y) * nxo + c.y;
   const cplx<T>* wrow = Wt + ((size_t)m * N + y) * N;
@@ -140,7 +150,7 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a(
     T o2 = norm2(o);
     mx = o2 > mx ? o2 : mx;
   }
-  line_fft<T, N, E, false>(v, jl, lre + l * S::LD, lim + l * S::LD, tw);
+  line_fft<T, N, E, false>(v, jl, lre + l * S::LD, lim + l * S::LD, stw);
   cplx<T>* prow = psi + ((size_t)(b * M + m) * N + y) * N;
 #pragma unroll
   for (int e = 0; e < E; ++e) st(prow + jl + TL * e, v[e]);
@@ -179,21 +189,27 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
   constexpr int E = S::E, TL = S::TL, CG = S::L;
   constexpr int n = N * N;
   __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
+  __shared__ cplx<T> stw[N];
   const int tid = threadIdx.x, col = tid % CG, jl = tid / CG;
   const int x = blockIdx.x * CG + col;
   const int b = blockIdx.y;
   const int f = order[b];
   T* re = lre + col * S::LD;
   T* im = lim + col * S::LD;
+  const float* cnt = counts + (size_t)f * n;
+  float cv[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) cv[e] = cnt[(jl + TL * e) * N + x];
   cplx<T> v[M][E];
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const cplx<T>* plane = psi + (size_t)(b * M + m) * n;
 #pragma unroll
     for (int e = 0; e < E; ++e) v[m][e] = ld(plane + (jl + TL * e) * N + x);
-    line_fft<T, N, E, false>(v[m], jl, re, im, tw);
   }
-  const float* cnt = counts + (size_t)f * n;
+  stage_twiddles<T, N, S::NT>(stw, tw);
+#pragma unroll
+  for (int m = 0; m < M; ++m) line_fft<T, N, E, false>(v[m], jl, re, im, stw);
   const T inv_n = T(1) / T(n);
   double acc[4] = {0, 0, 0, 0};
   T afar = 0;
@@ -202,7 +218,7 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
   if constexpr (MODE == B_ICALC) prev = double(ba.prev[0]);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const float io = cnt[(jl + TL * e) * N + x];
+    const float io = cv[e];
     T I = 0;
 #pragma unroll
     for (int m = 0; m < M; ++m) I += norm2(v[m][e]);
@@ -268,7 +284,7 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
   } else {
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-      line_fft<T, N, E, true>(v[m], jl, re, im, tw);
+      line_fft<T, N, E, true>(v[m], jl, re, im, stw);
       cplx<T>* plane = psi + (size_t)(b * M + m) * n;
 #pragma unroll
       for (int e = 0; e < E; ++e) st(plane + (jl + TL * e) * N + x, v[m][e]);
@@ -293,11 +309,13 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie(
   constexpr int E = S::E, TL = S::TL;
   constexpr int n = N * N;
   __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
+  __shared__ cplx<T> stw[N];
   const int tid = threadIdx.x, l = tid / TL, jl = tid % TL;
   const int y = blockIdx.x * S::L + l;
   const int m = blockIdx.y, ch = blockIdx.z;
   const int b0 = ch * chunk;
   const int b1 = b0 + chunk < nb ? b0 + chunk : nb;
+  stage_twiddles<T, N, S::NT>(stw, tw);
   cplx<T> w[E], qq[E], acc[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -308,22 +326,34 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie(
   }
   const T inv = T(1) - alpha;
   // The next slot's rows are loaded before this slot's transform.
-  cplx<T> nxt[E];
+  cplx<T> nxt[E], onx[E];
   if (b0 < b1) {
     const cplx<T>* p0 = psi + ((size_t)(b0 * M + m) * N + y) * N;
+    const int2 c0 = corner[order[b0]];
+    const cplx<T>* o0 = O + (size_t)(c0.x + y) * nxo + c0.y;
 #pragma unroll
-    for (int e = 0; e < E; ++e) nxt[e] = ld(p0 + jl + TL * e);
+    for (int e = 0; e < E; ++e) {
+      nxt[e] = ld(p0 + jl + TL * e);
+      onx[e] = ld(o0 + jl + TL * e);
+    }
   }
   for (int b = b0; b < b1; ++b) {
-    const int2 c = corner[order[b]];
     cplx<T>* prow = psi + ((size_t)(b * M + m) * N + y) * N;
-    cplx<T> v[E];
+    cplx<T> v[E], ow[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = nxt[e];
+    for (int e = 0; e < E; ++e) {
+      v[e] = nxt[e];
+      ow[e] = onx[e];
+    }
     if (b + 1 < b1) {
       const cplx<T>* pn = psi + ((size_t)((b + 1) * M + m) * N + y) * N;
+      const int2 cn = corner[order[b + 1]];
+      const cplx<T>* on = O + (size_t)(cn.x + y) * nxo + cn.y;
 #pragma unroll
-      for (int e = 0; e < E; ++e) nxt[e] = ld(pn + jl + TL * e);
+      for (int e = 0; e < E; ++e) {
+        nxt[e] = ld(pn + jl + TL * e);
+        onx[e] = ld(on + jl + TL * e);
+      }
     }
     T omax = 0;
     if (weighted) {
@@ -332,12 +362,11 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie(
         omax = t > omax ? t : omax;
       }
     }
-    line_fft<T, N, E, true>(v, jl, lre + l * S::LD, lim + l * S::LD, tw);
-    const cplx<T>* orow = O + (size_t)(c.x + y) * nxo + c.y;
+    line_fft<T, N, E, true>(v, jl, lre + l * S::LD, lim + l * S::LD, stw);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int x = jl + TL * e;
-      const cplx<T> o = ld(orow + x);
+      const cplx<T> o = ow[e];
       const cplx<T> d = v[e] - w[e] * o;
       st(prow + x, d);
       if (probe_dir) {
