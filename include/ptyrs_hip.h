@@ -90,6 +90,41 @@ typedef struct {
   uint64_t seed;
 } ptyrs_rpie_params;
 
+/* ApParams, DmParams, RaarParams (ptyrs/src/engine/ptycho.rs:48-121); an inertia with has_* = 0
+ * leaves that array fixed. Defaults: AP obj_inertia 0.01, no probe update; DM and RAAR
+ * inertias 0.01 / 0.001, alpha 0.02, beta 0.9, loop_obj_probe 1; calc_llk 0;
+ * zero_phase_ramp 1. */
+typedef struct {
+  int has_obj_inertia;
+  double obj_inertia;
+  int has_probe_inertia;
+  double probe_inertia;
+  size_t calc_llk;
+  int zero_phase_ramp;
+} ptyrs_ap_params;
+
+typedef struct {
+  int has_obj_inertia;
+  double obj_inertia;
+  int has_probe_inertia;
+  double probe_inertia;
+  double alpha;
+  size_t loop_obj_probe;
+  size_t calc_llk;
+  int zero_phase_ramp;
+} ptyrs_dm_params;
+
+typedef struct {
+  int has_obj_inertia;
+  double obj_inertia;
+  int has_probe_inertia;
+  double probe_inertia;
+  double beta;
+  size_t loop_obj_probe;
+  size_t calc_llk;
+  int zero_phase_ramp;
+} ptyrs_raar_params;
+
 /* LsqmlParams (ptyrs/src/engine/lsqml.rs:34-53) and Cost (ptyrs/src/engine/features.rs). */
 enum { PTYRS_AMPLITUDE = 0, PTYRS_POISSON = 1 };
 typedef struct {
@@ -125,6 +160,12 @@ int ptyrs_engine_create(const ptyrs_scan* scan, const ptyrs_optics* optics, cons
 
 /* GpuPtycho::rpie (ptyrs/src/engine/gpu/rpie.rs:31-66): nepoch epochs of rPIE. */
 int ptyrs_engine_rpie(ptyrs_engine* e, const ptyrs_rpie_params* params, size_t nepoch);
+
+/* GpuPtycho::ap, ::dm, ::raar (ptyrs/src/engine/gpu/ap.rs:28, dm.rs:32, raar.rs:17): ncycle
+ * cycles of PyNX AP, DM or RAAR, the exit waves of DM and RAAR carried across its cycles. */
+int ptyrs_engine_ap(ptyrs_engine* e, const ptyrs_ap_params* params, size_t ncycle);
+int ptyrs_engine_dm(ptyrs_engine* e, const ptyrs_dm_params* params, size_t ncycle);
+int ptyrs_engine_raar(ptyrs_engine* e, const ptyrs_raar_params* params, size_t ncycle);
 
 /* GpuPtycho::lsqml (ptyrs/src/engine/gpu/lsqml.rs:75-120) without a probe variation or
  * features: nepoch epochs from a fresh LSQML, an LLK record every calc_llk epochs and after

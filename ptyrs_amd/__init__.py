@@ -61,6 +61,26 @@ class _RpieC(ctypes.Structure):
                 ("seed", ctypes.c_uint64)]
 
 
+class _ApC(ctypes.Structure):
+    _fields_ = [("has_obj_inertia", ctypes.c_int), ("obj_inertia", ctypes.c_double),
+                ("has_probe_inertia", ctypes.c_int), ("probe_inertia", ctypes.c_double),
+                ("calc_llk", ctypes.c_size_t), ("zero_phase_ramp", ctypes.c_int)]
+
+
+class _DmC(ctypes.Structure):
+    _fields_ = [("has_obj_inertia", ctypes.c_int), ("obj_inertia", ctypes.c_double),
+                ("has_probe_inertia", ctypes.c_int), ("probe_inertia", ctypes.c_double),
+                ("alpha", ctypes.c_double), ("loop_obj_probe", ctypes.c_size_t),
+                ("calc_llk", ctypes.c_size_t), ("zero_phase_ramp", ctypes.c_int)]
+
+
+class _RaarC(ctypes.Structure):
+    _fields_ = [("has_obj_inertia", ctypes.c_int), ("obj_inertia", ctypes.c_double),
+                ("has_probe_inertia", ctypes.c_int), ("probe_inertia", ctypes.c_double),
+                ("beta", ctypes.c_double), ("loop_obj_probe", ctypes.c_size_t),
+                ("calc_llk", ctypes.c_size_t), ("zero_phase_ramp", ctypes.c_int)]
+
+
 class _LsqmlC(ctypes.Structure):
     _fields_ = [("batch_size", ctypes.c_size_t), ("cost", ctypes.c_int), ("seed", ctypes.c_uint64)]
 
@@ -87,6 +107,9 @@ _SIGNATURES = [
                              _D, _SZ, ctypes.POINTER(_OptionsC), ctypes.c_int,
                              ctypes.POINTER(_P)], ctypes.c_int),
     ("ptyrs_engine_rpie", [_P, ctypes.POINTER(_RpieC), _SZ], ctypes.c_int),
+    ("ptyrs_engine_ap", [_P, ctypes.POINTER(_ApC), _SZ], ctypes.c_int),
+    ("ptyrs_engine_dm", [_P, ctypes.POINTER(_DmC), _SZ], ctypes.c_int),
+    ("ptyrs_engine_raar", [_P, ctypes.POINTER(_RaarC), _SZ], ctypes.c_int),
     ("ptyrs_engine_lsqml", [_P, ctypes.POINTER(_LsqmlC), _SZ, _SZ], ctypes.c_int),
     ("ptyrs_engine_lsqml_steps", [_P, ctypes.POINTER(_StepC), _SZ, ctypes.POINTER(_SZ)],
      ctypes.c_int),
@@ -254,6 +277,11 @@ class Optics:
 
 _GPU_OPTIONS = dict(stack_size=None, graph=True, llk_slots=8, fused=True, psi_f16=False,
                     dtype="c64")
+_AP_PARAMS = dict(obj_inertia=0.01, probe_inertia=None, calc_llk=0, zero_phase_ramp=True)
+_DM_PARAMS = dict(obj_inertia=0.01, probe_inertia=0.001, alpha=0.02, loop_obj_probe=1, calc_llk=0,
+                  zero_phase_ramp=True)
+_RAAR_PARAMS = dict(obj_inertia=0.01, probe_inertia=0.001, beta=0.9, loop_obj_probe=1, calc_llk=0,
+                    zero_phase_ramp=True)
 _RPIE_PARAMS = dict(obj_step=1.0, probe_step=1.0, obj_alpha=0.1, probe_alpha=0.1,
                     batch_size=100, calc_llk=0, seed=7)
 
@@ -310,6 +338,39 @@ class GpuPtycho:
         if h and _lib is not None:
             _lib.ptyrs_engine_destroy(h)
             self._h = None
+
+    @staticmethod
+    def _inertias(c, p):
+        c.has_obj_inertia = p["obj_inertia"] is not None
+        c.obj_inertia = p["obj_inertia"] if p["obj_inertia"] is not None else 0.0
+        c.has_probe_inertia = p["probe_inertia"] is not None
+        c.probe_inertia = p["probe_inertia"] if p["probe_inertia"] is not None else 0.0
+        if int(p["calc_llk"]) < 0:
+            raise ValueError("calc_llk must be >= 0")
+        c.calc_llk, c.zero_phase_ramp = int(p["calc_llk"]), int(bool(p["zero_phase_ramp"]))
+
+    def ap(self, ncycle, **params):
+        """`ncycle` AP cycles; keywords are ApParams fields (ptyrs/src/engine/ptycho.rs:48-69)."""
+        p = _overlay(_AP_PARAMS, params, "ApParams")
+        c = _ApC()
+        self._inertias(c, p)
+        _check(library().ptyrs_engine_ap(self._h, ctypes.byref(c), int(ncycle)))
+
+    def dm(self, ncycle, **params):
+        """`ncycle` DM cycles; keywords are DmParams fields (ptyrs/src/engine/ptycho.rs:71-95)."""
+        p = _overlay(_DM_PARAMS, params, "DmParams")
+        c = _DmC()
+        self._inertias(c, p)
+        c.alpha, c.loop_obj_probe = float(p["alpha"]), int(p["loop_obj_probe"])
+        _check(library().ptyrs_engine_dm(self._h, ctypes.byref(c), int(ncycle)))
+
+    def raar(self, ncycle, **params):
+        """`ncycle` RAAR cycles; keywords are RaarParams fields (ptyrs/src/engine/ptycho.rs:97-121)."""
+        p = _overlay(_RAAR_PARAMS, params, "RaarParams")
+        c = _RaarC()
+        self._inertias(c, p)
+        c.beta, c.loop_obj_probe = float(p["beta"]), int(p["loop_obj_probe"])
+        _check(library().ptyrs_engine_raar(self._h, ctypes.byref(c), int(ncycle)))
 
     def rpie(self, nepoch, **params):
         """`nepoch` rPIE epochs; keywords are RpieParams fields (ptyrs/src/engine/ptycho.rs:145-186);

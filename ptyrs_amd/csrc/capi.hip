@@ -186,6 +186,51 @@ int ptyrs_engine_rpie(ptyrs_engine* h, const ptyrs_rpie_params* p, size_t nepoch
   return guarded(h, true, [&](EngineBase& e) { e.rpie(*p, nepoch); });
 }
 
+namespace {
+ProjParams proj(int algorithm, int ho, double oi, int hp, double pi, double mix, size_t loops,
+                size_t calc_llk, int zpr) {
+  return ProjParams{algorithm, ho != 0, hp != 0, oi, pi, mix, loops, calc_llk, zpr != 0};
+}
+}  // namespace
+
+int ptyrs_engine_ap(ptyrs_engine* h, const ptyrs_ap_params* p, size_t ncycle) {
+  if (!p) {
+    last_error = "null params";
+    return PTYRS_EINVAL;
+  }
+  return guarded(h, true, [&](EngineBase& e) {
+    e.project(proj(PTYRS_AP, p->has_obj_inertia, p->obj_inertia, p->has_probe_inertia,
+                   p->probe_inertia, 0.0, 1, p->calc_llk, p->zero_phase_ramp),
+              ncycle);
+  });
+}
+
+int ptyrs_engine_dm(ptyrs_engine* h, const ptyrs_dm_params* p, size_t ncycle) {
+  if (!p) {
+    last_error = "null params";
+    return PTYRS_EINVAL;
+  }
+  return guarded(h, true, [&](EngineBase& e) {
+    e.project(proj(PTYRS_DM, p->has_obj_inertia, p->obj_inertia, p->has_probe_inertia,
+                   p->probe_inertia, p->alpha, p->loop_obj_probe, p->calc_llk,
+                   p->zero_phase_ramp),
+              ncycle);
+  });
+}
+
+int ptyrs_engine_raar(ptyrs_engine* h, const ptyrs_raar_params* p, size_t ncycle) {
+  if (!p) {
+    last_error = "null params";
+    return PTYRS_EINVAL;
+  }
+  return guarded(h, true, [&](EngineBase& e) {
+    e.project(proj(PTYRS_RAAR, p->has_obj_inertia, p->obj_inertia, p->has_probe_inertia,
+                   p->probe_inertia, p->beta, p->loop_obj_probe, p->calc_llk,
+                   p->zero_phase_ramp),
+              ncycle);
+  });
+}
+
 int ptyrs_engine_lsqml(ptyrs_engine* h, const ptyrs_lsqml_params* p, size_t nepoch,
                        size_t calc_llk) {
   if (!p) {

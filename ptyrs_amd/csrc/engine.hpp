@@ -66,6 +66,16 @@ struct Prepared {
   ExitModel exit;
 };
 
+// AP, DM and RAAR in one form (ptyrs/src/engine/ptycho.rs:206-356).
+struct ProjParams {
+  int algorithm;  // PTYRS_AP, PTYRS_DM, PTYRS_RAAR
+  bool upd_obj, upd_probe;
+  double obj_inertia, probe_inertia;
+  double mix;     // DM alpha, RAAR beta
+  size_t loops, calc_llk;
+  bool zero_phase_ramp;
+};
+
 // One LSQML slot's step sizes (LsqmlStep, ptyrs/src/engine/lsqml.rs:55-66), single slice.
 struct StepRecord {
   int batch;
@@ -80,6 +90,7 @@ class EngineBase {
   }
   virtual void rpie(const ptyrs_rpie_params& p, size_t nepoch) = 0;
   virtual void lsqml(const ptyrs_lsqml_params& p, size_t nepoch, size_t calc_llk) = 0;
+  virtual void project(const ProjParams& p, size_t ncycle) = 0;
   virtual void llk(double out[4]) = 0;
   virtual void get_object(double* out) = 0;
   virtual void get_probe(double* out) = 0;
@@ -347,11 +358,108 @@ class Engine : public EngineBase {
     }
   }
 
+  // PyNX AP, DM, RAAR (ptyrs/src/engine/ptycho.rs:206-356), stacks of the whole scan; DM and
+  // RAAR carry their exit waves across the cycles of a call.
+  void project(const ProjParams& p, size_t ncycle) override {
+    if (nobj_ != 1) throw Invalid("the GPU projection algorithms run one object mode");
+    if (M_ > 4) throw Invalid("the GPU engine runs at most 4 modes");
+    if (p.algorithm != PTYRS_AP && p.loops == 0)
+      throw Invalid("loop_obj_probe must be >= 1");
+    if (ncycle == 0 || nframe_ == 0) {
+      cycle += ncycle;
+      if (p.zero_phase_ramp && ncycle > 0) host_zero_phase_ramp();
+      return;
+    }
+    const size_t stack = std::min<size_t>(nframe_, stack_size_ ? stack_size_ : kDefaultStack);
+    ensure_workspace(stack, 1);
+    const size_t nb_all = nframe_;
+    carried_.ensure(sizeof(cplx<T>) * nb_all * M_ * n);
+    if (p.algorithm == PTYRS_RAAR) iter_.ensure(sizeof(cplx<T>) * nb_all * M_ * n);
+    const int nt = ntiles();
+    list_all_.ensure(sizeof(int) * size_t(nt) * nb_all);
+    count_all_.ensure(sizeof(int) * size_t(nt));
+    const int nstack = int((nb_all + stack - 1) / stack);
+    const int chunk = chunk_of(int(stack));
+    const int nchunk = int((stack + chunk - 1) / chunk);
+    pnum_part_.ensure(sizeof(cplx<T>) * size_t(nstack) * nchunk * M_ * n);
+    pden_part_.ensure(sizeof(T) * size_t(nstack) * nchunk * n);
+    pnum_.ensure(sizeof(cplx<T>) * M_ * n);
+    pden_.ensure(sizeof(T) * n);
+    pmaxpart_.ensure(sizeof(T) * ((n + THREADS - 1) / THREADS));
+    pmax2_.ensure(sizeof(T) * 2);
+    abs_part_.ensure(sizeof(double) * 2 * kAbsBlocks);
+    probe_terms();
+    std::vector<std::pair<size_t, int>> due;
+    for (size_t ic = 0; ic < ncycle; ++ic) {
+      const bool rec = p.calc_llk > 0 && (ic % p.calc_llk == 0 || ic + 1 == ncycle);
+      if (rec) {
+        if (due.size() == kHistSlots) drain(due, p.algorithm);
+        const int slot = int(due.size());
+        llk_pass(slot);
+        due.push_back({cycle, slot});
+      }
+      // Every call's first cycle starts from ψ = PO (ptyrs/src/engine/ptycho.rs:264-266).
+      const bool first = ic == 0;
+      MixArgs<T> mix{MIX_NONE, carried_.as<cplx<T>>(), iter_.as<cplx<T>>(), T(0), T(0)};
+      if (p.algorithm == PTYRS_DM && !first) {
+        mix.kind = MIX_DM;
+        mix.c0 = T(2.0 - p.mix);
+        mix.c1 = T(1.0 - p.mix);
+      } else if (p.algorithm == PTYRS_RAAR) {
+        mix.kind = first ? MIX_RAAR0 : MIX_RAAR;
+        mix.c0 = T(1.0 - 0.5 * p.mix);
+        mix.c1 = T(0.5 * p.mix);
+      }
+      const bool ap_probe = p.algorithm == PTYRS_AP && p.upd_probe;
+      for (int s = 0; s < nstack; ++s) {
+        const int f0 = int(s * stack);
+        const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+        const int* ord = iota_.as<int>() + f0;
+        pass_a(ord, nb, false, mix);
+        pass_b<B_AMPLITUDE>(ord, nb);
+        const int ck = chunk_of(nb);
+        const int nck = (nb + ck - 1) / ck;
+        const bool dm = p.algorithm == PTYRS_DM && !first;
+        run(K_PASS_C, [&] {
+          hipLaunchKernelGGL((k_pass_c_proj<T, N>), dim3(SR::NBLK, M_, nck), dim3(SR::NT), 0,
+                             stream_, psi_.as<cplx<T>>(), carried_.as<cplx<T>>(),
+                             O_.as<cplx<T>>(), int(nxo_), Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
+                             corner_.as<int2>(), ord, nb, M_, ck, int(dm), T(2.0 - p.mix),
+                             T(1.0 - p.mix), int(ap_probe),
+                             pnum_part_.as<cplx<T>>() + size_t(s) * nchunk * M_ * n,
+                             pden_part_.as<T>() + size_t(s) * nchunk * n, tw_.as<cplx<T>>());
+        });
+        if (ap_probe && nck < nchunk) zero_partials(s, nck, nchunk);
+      }
+      const int tot_chunks = nstack * nchunk;
+      if (p.algorithm == PTYRS_AP) {
+        if (p.upd_obj) object_blend(p.obj_inertia);
+        if (p.upd_probe) probe_blend(tot_chunks, p.probe_inertia);
+      } else {
+        for (size_t l = 0; l < p.loops; ++l) {
+          if (p.upd_obj) object_blend(p.obj_inertia);
+          if (p.upd_probe) {
+            probe_sums_all(stack, nstack, nchunk);
+            probe_blend(tot_chunks, p.probe_inertia);
+          }
+        }
+      }
+      if (rec && p.algorithm != PTYRS_AP) balance(1.0);
+      ++cycle;
+    }
+    drain(due, p.algorithm);
+    synchronize();
+    flush_timing();
+    if (p.zero_phase_ramp) host_zero_phase_ramp();
+  }
+
  private:
   static constexpr size_t kHistSlots = 64;
   static constexpr size_t kDefaultStack = 128;
   int M_ = 1;
   double scale_ = 1.0, max_frame_total_ = 0.0;
+  DevBuf carried_, iter_, list_all_, count_all_, pnum_part_, pden_part_, pnum_, pden_,
+      pmaxpart_, pmax2_, abs_part_;
   DevBuf dO_, pre_, illum_, afar_all_, afar_tmp_, prev_, icalc_, xi_part_, xi_part2_, dots_part_,
       sums_, log_, step_, pre_part_, floor_, dPm_;
   int ntx_ = 0, nty_ = 0;
@@ -515,6 +623,98 @@ class Engine : public EngineBase {
     });
   }
 
+  static constexpr int kAbsBlocks = 256;
+  size_t stack_size_ = 0;
+
+  // Chunks of a short last stack hold no partials: zero them so every stack's slab sums.
+  void zero_partials(int s, int from, int nchunk) {
+    const size_t cn = size_t(M_) * n;
+    PTYRS_HIP(hipMemsetAsync(pnum_part_.as<cplx<T>>() + (size_t(s) * nchunk + from) * cn, 0,
+                             sizeof(cplx<T>) * (nchunk - from) * cn, stream_));
+    PTYRS_HIP(hipMemsetAsync(pden_part_.as<T>() + (size_t(s) * nchunk + from) * n, 0,
+                             sizeof(T) * (nchunk - from) * n, stream_));
+  }
+
+  // Object update of AP/DM/RAAR: tile lists over every frame, norm = Σ_m D_m, its max, blend.
+  void object_blend(double inertia) {
+    const int nt = ntiles();
+    run(K_BIN, [&] {
+      hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                         corner_.as<int2>(), iota_.as<int>(), int(nframe_), N, ntx_,
+                         int(nframe_), list_all_.as<int>(), count_all_.as<int>(),
+                         Wt_.as<cplx<T>>(), M_, int(nyo_), int(nxo_), D_.as<T>(),
+                         tilemax_.as<T>());
+    });
+    run(K_OVERLAP, [&] {
+      hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(THREADS), 0, stream_,
+                         tilemax_.as<T>(), nt, pmax_.as<T>());
+    });
+    run(K_OBJ_UPDATE, [&] {
+      hipLaunchKernelGGL(k_obj_blend<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                         O_.as<cplx<T>>(), Wt_.as<cplx<T>>(), D_.as<T>(),
+                         carried_.as<cplx<T>>(), N, M_, corner_.as<int2>(), iota_.as<int>(),
+                         list_all_.as<int>(), count_all_.as<int>(), int(nframe_), int(nyo_),
+                         int(nxo_), ntx_, pmax_.as<T>(), T(inertia));
+    });
+  }
+
+  // Probe sums of the carried waves against the current object (DM, RAAR).
+  void probe_sums_all(size_t stack, int nstack, int nchunk) {
+    for (int s = 0; s < nstack; ++s) {
+      const int f0 = int(s * stack);
+      const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+      const int ck = chunk_of(nb);
+      const int nck = (nb + ck - 1) / ck;
+      run(K_PROBE, [&] {
+        hipLaunchKernelGGL(k_probe_sums<T>, dim3((n + THREADS - 1) / THREADS, M_, nck),
+                           dim3(THREADS), 0, stream_, carried_.as<cplx<T>>(), O_.as<cplx<T>>(),
+                           int(nxo_), q_.as<cplx<T>>(), corner_.as<int2>(),
+                           iota_.as<int>() + f0, nb, N, M_, ck,
+                           pnum_part_.as<cplx<T>>() + size_t(s) * nchunk * M_ * n,
+                           pden_part_.as<T>() + size_t(s) * nchunk * n);
+      });
+      if (nck < nchunk) zero_partials(s, nck, nchunk);
+    }
+  }
+
+  void probe_blend(int tot_chunks, double inertia) {
+    const int nblk = (n + THREADS - 1) / THREADS;
+    const int total = int(nprobe_ * n);
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_probe_reduce<T>, dim3(nblk), dim3(THREADS), 0, stream_,
+                         pnum_part_.as<cplx<T>>(), pden_part_.as<T>(), tot_chunks, n, M_,
+                         pnum_.as<cplx<T>>(), pden_.as<T>(), pmaxpart_.as<T>());
+      hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(THREADS), 0, stream_,
+                         pmaxpart_.as<T>(), nblk, pmax2_.as<T>());
+      hipLaunchKernelGGL(k_probe_blend<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,
+                         P_.as<cplx<T>>(), Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
+                         pnum_.as<cplx<T>>(), pden_.as<T>(), pmax2_.as<T>(), T(inertia), n,
+                         total);
+    });
+  }
+
+  // split_scale(s) (ptyrs/src/engine/ptycho.rs:184-192).
+  void balance(double s) {
+    run(K_OTHER, [&] {
+      hipLaunchKernelGGL(k_abs_sums<T>, dim3(kAbsBlocks), dim3(THREADS), 0, stream_,
+                         O_.as<cplx<T>>(), nobj_ * nyo_ * nxo_, P_.as<cplx<T>>(), nprobe_ * n,
+                         abs_part_.as<double>());
+      hipLaunchKernelGGL(k_balance<T>, dim3(kAbsBlocks), dim3(THREADS), 0, stream_,
+                         O_.as<cplx<T>>(), nobj_ * nyo_ * nxo_, P_.as<cplx<T>>(),
+                         Wt_.as<cplx<T>>(), q_.as<cplx<T>>(), n, nprobe_ * n,
+                         abs_part_.as<double>(), kAbsBlocks, s);
+    });
+  }
+
+  void host_zero_phase_ramp() {
+    std::vector<double> o(object_len()), pr(probe_len());
+    get_object(o.data());
+    get_probe(pr.data());
+    zero_phase_ramp(o.data(), nobj_, nyo_, nxo_, pr.data(), nprobe_, N, N);
+    set_object(o.data());
+    set_probe(pr.data());
+  }
+
   int ntiles() const { return ntx_ * nty_; }
 
   // Pass C chunking: enough workgroups to fill the chip, few partial probe sums.
@@ -550,12 +750,21 @@ class Engine : public EngineBase {
     });
   }
 
-  void pass_a(const int* ord, int nb, bool omax) {
-    run(K_PASS_A, [&] {
-      hipLaunchKernelGGL((k_pass_a<T, N>), dim3(S::NBLK, nb * M_), dim3(S::NT), 0, stream_,
+  void pass_a(const int* ord, int nb, bool omax,
+              MixArgs<T> mix = MixArgs<T>{MIX_NONE, nullptr, nullptr, T(0), T(0)}) {
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(S::NBLK, nb * M_), dim3(S::NT), 0, stream_,
                          O_.as<cplx<T>>(), int(nxo_), Wt_.as<cplx<T>>(), corner_.as<int2>(), ord,
                          M_, psi_.as<cplx<T>>(), omax ? omax_.as<T>() : nullptr,
-                         tw_.as<cplx<T>>());
+                         tw_.as<cplx<T>>(), mix);
+    };
+    run(K_PASS_A, [&] {
+      switch (mix.kind) {
+        case MIX_DM: go(k_pass_a<T, N, MIX_DM>); break;
+        case MIX_RAAR: go(k_pass_a<T, N, MIX_RAAR>); break;
+        case MIX_RAAR0: go(k_pass_a<T, N, MIX_RAAR0>); break;
+        default: go(k_pass_a<T, N, MIX_NONE>); break;
+      }
     });
   }
 

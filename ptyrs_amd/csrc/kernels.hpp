@@ -123,11 +123,23 @@ __global__ void k_probe_terms(const cplx<T>* __restrict__ P, const cplx<T>* __re
 // ---- pass A: gather O·P·q and transform rows ---------------------------------------------------
 // Grid (NBLK, nb·M). View (b, m) = (blockIdx.y / M, blockIdx.y % M). With `omax_part`, mode-0
 // workgroups also write max |O|² over their rows of the window.
-template <typename T, int N>
+// The exit wave pass A transforms: PO itself, or DM's and RAAR's mix of PO with the waves
+// they carry across cycles (ptyrs/src/engine/ptycho.rs:254-332).
+enum Mix { MIX_NONE = 0, MIX_DM = 1, MIX_RAAR = 2, MIX_RAAR0 = 3 };
+template <typename T>
+struct MixArgs {
+  int kind;
+  const cplx<T>* carried;  // [nframe][M][N][N]: DM's and RAAR's ψ
+  cplx<T>* iterate;        // [nframe][M][N][N]: RAAR's iterate, rewritten with the mix
+  T c0, c1;                // DM: (2 − a, 1 − a); RAAR: (1 − β/2, β/2)
+};
+
+template <typename T, int N, int MIX>
 __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a(
     const cplx<T>* __restrict__ O, int nxo, const cplx<T>* __restrict__ Wt,
     const int2* __restrict__ corner, const int* __restrict__ order, int M,
-    cplx<T>* __restrict__ psi, T* __restrict__ omax_part, const cplx<T>* __restrict__ tw) {
+    cplx<T>* __restrict__ psi, T* __restrict__ omax_part, const cplx<T>* __restrict__ tw,
+    MixArgs<T> mix) {
   using S = RowShape<T, N>;
   constexpr int E = S::E, TL = S::TL;
   __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
@@ -137,9 +149,11 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a(
   const int y = blockIdx.x * S::L + l;
   const int b = blockIdx.y / M, m = blockIdx.y % M;
   stage_twiddles<T, N, S::NT>(stw, tw);
-  const int2 c = corner[order[b]];
+  const int f = order[b];
+  const int2 c = corner[f];
   const cplx<T>* orow = O + (size_t)(c.x + y) * nxo + c.y;
   const cplx<T>* wrow = Wt + ((size_t)m * N + y) * N;
+  const size_t crow = (((size_t)f * M + m) * N + y) * N;
   cplx<T> v[E];
   T mx = 0;
 #pragma unroll
@@ -147,6 +161,15 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a(
     const int x = jl + TL * e;
     cplx<T> o = ld(orow + x);
     v[e] = ld(wrow + x) * o;
+    if constexpr (MIX == MIX_DM) {
+      v[e] = v[e] * mix.c0 - ld(mix.carried + crow + x) * mix.c1;
+    } else if constexpr (MIX == MIX_RAAR) {
+      const cplx<T> it = ld(mix.iterate + crow + x);
+      v[e] = ld(mix.carried + crow + x) * mix.c0 + (v[e] * T(2) - it) * mix.c1;
+      st(mix.iterate + crow + x, v[e]);
+    } else if constexpr (MIX == MIX_RAAR0) {
+      st(mix.iterate + crow + x, v[e]);  // RAAR's first cycle: ψ = PO, iterate = ψ
+    }
     T o2 = norm2(o);
     mx = o2 > mx ? o2 : mx;
   }
@@ -911,6 +934,250 @@ __global__ void k_lsqml_log_afar(const T* __restrict__ a, const int* __restrict_
   const T v = gauss ? T(0.5) : a[b];
   a_far_all[order[b]] = v;
   log[(size_t)b * 3] = double(v);
+}
+
+// ---- AP, DM, RAAR ----------------------------------------------------------------------------------
+// Pass C of the projection algorithms. Grid (NBLK, M, nchunk); each workgroup runs over the
+// slots of its chunk: inverse row transform of the projected field Π, then the new carried wave
+// ψ' = Π (AP, RAAR) or Π + (1 − a)·(PO − R) with R = (2 − a)·PO − (1 − a)·ψ (DM), written to
+// carried[frame]. With `probe_sums` (AP: both updates read the old arrays) it adds
+// conj(q·O)·ψ' and |O|² of the slot's window into per-chunk partials.
+template <typename T, int N>
+__global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_proj(
+    const cplx<T>* __restrict__ psi, cplx<T>* __restrict__ carried, const cplx<T>* __restrict__ O,
+    int nxo, const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ q,
+    const int2* __restrict__ corner, const int* __restrict__ order, int nb, int M, int chunk,
+    int dm, T c0, T c1, int probe_sums, cplx<T>* __restrict__ num_part, T* __restrict__ den_part,
+    const cplx<T>* __restrict__ tw) {
+  using S = RowShapeC<T, N>;
+  constexpr int E = S::E, TL = S::TL;
+  constexpr int n = N * N;
+  __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
+  __shared__ cplx<T> stw[N];
+  const int tid = threadIdx.x, l = tid / TL, jl = tid % TL;
+  const int y = blockIdx.x * S::L + l;
+  const int m = blockIdx.y, ch = blockIdx.z;
+  const int b0 = ch * chunk;
+  const int b1 = b0 + chunk < nb ? b0 + chunk : nb;
+  stage_twiddles<T, N, S::NT>(stw, tw);
+  cplx<T> w[E], qq[E], acc[E];
+  T den[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int x = jl + TL * e;
+    w[e] = ld(Wt + (size_t)m * n + (size_t)y * N + x);
+    qq[e] = ld(q + (size_t)y * N + x);
+    acc[e] = cplx<T>{0, 0};
+    den[e] = 0;
+  }
+  for (int b = b0; b < b1; ++b) {
+    const int f = order[b];
+    const int2 c = corner[f];
+    const cplx<T>* prow = psi + ((size_t)(b * M + m) * N + y) * N;
+    cplx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = ld(prow + jl + TL * e);
+    line_fft<T, N, E, true>(v, jl, lre + l * S::LD, lim + l * S::LD, stw);
+    const cplx<T>* orow = O + (size_t)(c.x + y) * nxo + c.y;
+    cplx<T>* crow = carried + (((size_t)f * M + m) * N + y) * N;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int x = jl + TL * e;
+      const cplx<T> o = ld(orow + x);
+      cplx<T> p = v[e];
+      if (dm) {
+        const cplx<T> po = w[e] * o;
+        const cplx<T> r = po * c0 - ld(crow + x) * c1;
+        p = p + (po - r) * c1;
+      }
+      st(crow + x, p);
+      if (probe_sums) {
+        acc[e] = acc[e] + cmulc(o * qq[e], p);
+        den[e] += norm2(o);
+      }
+    }
+  }
+  if (probe_sums) {
+    const size_t base = ((size_t)ch * M + m) * n + (size_t)y * N;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      st(num_part + base + jl + TL * e, acc[e]);
+      if (m == 0) den_part[(size_t)ch * n + (size_t)y * N + jl + TL * e] = den[e];
+    }
+  }
+}
+
+// Probe sums of carried waves against the current object (DM, RAAR: the probe update sees the
+// new object): grid (N·N/THREADS, M, nchunk), per-chunk partials of conj(q·O)·ψ and |O|².
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_probe_sums(
+    const cplx<T>* __restrict__ carried, const cplx<T>* __restrict__ O, int nxo,
+    const cplx<T>* __restrict__ q, const int2* __restrict__ corner, const int* __restrict__ order,
+    int nb, int N, int M, int chunk, cplx<T>* __restrict__ num_part, T* __restrict__ den_part) {
+  const int n = N * N;
+  const int i = blockIdx.x * THREADS + threadIdx.x;
+  const int m = blockIdx.y, ch = blockIdx.z;
+  if (i >= n) return;
+  const int y = i / N, x = i % N;
+  const int b0 = ch * chunk, b1 = b0 + chunk < nb ? b0 + chunk : nb;
+  const cplx<T> qq = ld(q + i);
+  cplx<T> acc = {0, 0};
+  T den = 0;
+  for (int b = b0; b < b1; ++b) {
+    const int f = order[b];
+    const int2 c = corner[f];
+    const cplx<T> o = ld(O + (size_t)(c.x + y) * nxo + c.y + x);
+    acc = acc + cmulc(o * qq, ld(carried + ((size_t)f * M + m) * n + i));
+    den += norm2(o);
+  }
+  st(num_part + ((size_t)ch * M + m) * n + i, acc);
+  if (m == 0) den_part[(size_t)ch * n + i] = den;
+}
+
+// Sums the probe partials over chunks (Kahan) into num[M][n] and den[n], and writes the per
+// workgroup max of den for the regulariser.
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_probe_reduce(
+    const cplx<T>* __restrict__ num_part, const T* __restrict__ den_part, int nchunk, int n, int M,
+    cplx<T>* __restrict__ num, T* __restrict__ den, T* __restrict__ maxpart) {
+  __shared__ T red[THREADS];
+  const int i = blockIdx.x * THREADS + threadIdx.x;
+  T mx = 0;
+  if (i < n) {
+    kahan<T> d;
+    for (int c = 0; c < nchunk; ++c) d.add(den_part[(size_t)c * n + i]);
+    den[i] = d.s;
+    mx = d.s;
+    for (int m = 0; m < M; ++m) {
+      kahan<T> sr, si;
+      for (int c = 0; c < nchunk; ++c) {
+        const cplx<T> v = ld(num_part + ((size_t)c * M + m) * n + i);
+        sr.add(v.x);
+        si.add(v.y);
+      }
+      st(num + (size_t)m * n + i, cplx<T>{sr.s, si.s});
+    }
+  }
+  T r = block_reduce<THREADS>(mx, red, MaxOp());
+  if (threadIdx.x == 0) maxpart[blockIdx.x] = r;
+}
+
+// PyNX blend of the probe: P = (num + reg·P)/max(den + reg, 1e-12), reg = inertia·max(den);
+// then Wt = P·q. ptyrs/src/engine/ops.rs:367-381
+template <typename T>
+__global__ void k_probe_blend(cplx<T>* __restrict__ P, cplx<T>* __restrict__ Wt,
+                              const cplx<T>* __restrict__ q, const cplx<T>* __restrict__ num,
+                              const T* __restrict__ den, const T* __restrict__ dmax, T inertia,
+                              int n, int total) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const T reg = dmax[0] * inertia;
+  const T d = den[i % n] + reg;
+  const cplx<T> p = (ld(num + i) + ld(P + i) * reg) * (T(1) / (d > T(1e-12) ? d : T(1e-12)));
+  st(P + i, p);
+  st(Wt + i, p * ld(q + i % n));
+}
+
+// PyNX blend of the object from tile-owned sums over every view: every pixel owned by one thread
+// sums Σ_m Σ_views conj(W_m)·ψ_m (Kahan) in view order, and O = (num + reg·O)/max(norm + reg,
+// 1e-12) with norm = Σ_m D_m from k_bin_tiles and reg = inertia·max(norm).
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_obj_blend(
+    cplx<T>* __restrict__ O, const cplx<T>* __restrict__ Wt, const T* __restrict__ D,
+    const cplx<T>* __restrict__ carried, int N, int M, const int2* __restrict__ corner,
+    const int* __restrict__ order, const int* __restrict__ list, const int* __restrict__ count,
+    int cap, int nyo, int nxo, int ntx, const T* __restrict__ nmax, T inertia) {
+  __shared__ int3 cs[THREADS];
+  const int t = blockIdx.x, tid = threadIdx.x;
+  const int X = (t % ntx) * TILE_X + tid % TILE_X, Y = (t / ntx) * TILE_Y + tid / TILE_X;
+  const int nl = count[t];
+  const bool inside = X < nxo && Y < nyo;
+  const int n = N * N;
+  kahan<T> sr, si;
+  const int* lst = list + (size_t)t * cap;
+  for (int s = 0; s < nl; s += THREADS) {
+    const int k = nl - s < THREADS ? nl - s : THREADS;
+    if (tid < k) {
+      const int f = order[lst[s + tid]];
+      const int2 c = corner[f];
+      cs[tid] = int3{f, c.x, c.y};
+    }
+    __syncthreads();
+    for (int i = 0; i < k && inside; ++i) {
+      const int y = Y - cs[i].y, x = X - cs[i].z;
+      if (y >= 0 && y < N && x >= 0 && x < N) {
+        const int j = y * N + x;
+        const cplx<T>* cv = carried + (size_t)cs[i].x * M * n + j;
+        for (int m = 0; m < M; ++m) {
+          const cplx<T> g = cmulc(ld(Wt + (size_t)m * n + j), ld(cv + (size_t)m * n));
+          sr.add(g.x);
+          si.add(g.y);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!inside) return;
+  const size_t no = (size_t)nyo * nxo, p = (size_t)Y * nxo + X;
+  T norm = 0;
+  if (nl > 0)
+    for (int m = 0; m < M; ++m) norm += D[m * no + p];
+  const T reg = nmax[0] * inertia;
+  const T d = norm + reg;
+  const cplx<T> o = (cplx<T>{sr.s, si.s} + ld(O + p) * reg) * (T(1) / (d > T(1e-12) ? d : T(1e-12)));
+  st(O + p, o);
+}
+
+// Σ|O| and Σ|P| in FP64: per-workgroup partials into part[2·blocks]. The host-side factors of
+// split_scale (ptyrs/src/engine/ptycho.rs:184-192) are then applied by k_balance.
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_abs_sums(const cplx<T>* __restrict__ O, size_t no,
+                                                      const cplx<T>* __restrict__ P, size_t np,
+                                                      double* __restrict__ part) {
+  __shared__ double red[THREADS];
+  double so = 0, sp = 0;
+  for (size_t i = (size_t)blockIdx.x * THREADS + threadIdx.x; i < no; i += (size_t)gridDim.x * THREADS) {
+    const cplx<T> v = ld(O + i);
+    so += sqrt(double(v.x) * v.x + double(v.y) * v.y);
+  }
+  for (size_t i = (size_t)blockIdx.x * THREADS + threadIdx.x; i < np; i += (size_t)gridDim.x * THREADS) {
+    const cplx<T> v = ld(P + i);
+    sp += sqrt(double(v.x) * v.x + double(v.y) * v.y);
+  }
+  double a = block_reduce<THREADS>(so, red, AddOp());
+  double b = block_reduce<THREADS>(sp, red, AddOp());
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// P *= sqrt(Σ|O|/Σ|P|·s), O *= sqrt(Σ|P|/Σ|O|·s), Wt = P·q; sums from k_abs_sums' partials.
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_balance(cplx<T>* __restrict__ O, size_t no,
+                                                     cplx<T>* __restrict__ P, cplx<T>* __restrict__ Wt,
+                                                     const cplx<T>* __restrict__ q, int n, size_t np,
+                                                     const double* __restrict__ part, int nparts,
+                                                     double s) {
+  __shared__ double fac[2];
+  if (threadIdx.x == 0) {
+    double so = 0, sp = 0;
+    for (int i = 0; i < nparts; ++i) {
+      so += part[2 * i];
+      sp += part[2 * i + 1];
+    }
+    fac[0] = sqrt(sp / so * s);  // object
+    fac[1] = sqrt(so / sp * s);  // probe
+  }
+  __syncthreads();
+  const T fo = T(fac[0]), fp = T(fac[1]);
+  for (size_t i = (size_t)blockIdx.x * THREADS + threadIdx.x; i < no; i += (size_t)gridDim.x * THREADS)
+    st(O + i, ld(O + i) * fo);
+  for (size_t i = (size_t)blockIdx.x * THREADS + threadIdx.x; i < np; i += (size_t)gridDim.x * THREADS) {
+    const cplx<T> p = ld(P + i) * fp;
+    st(P + i, p);
+    st(Wt + i, p * ld(q + i % n));
+  }
 }
 
 // ---- LLK sums -------------------------------------------------------------------------------------
