@@ -125,6 +125,15 @@ typedef struct {
   int zero_phase_ramp;
 } ptyrs_raar_params;
 
+/* MlParams (ptyrs/src/engine/ptycho.rs:123-143). Defaults: update_object 1, update_probe 0,
+ * calc_llk 0, zero_phase_ramp 1. */
+typedef struct {
+  int update_object;
+  int update_probe;
+  size_t calc_llk;
+  int zero_phase_ramp;
+} ptyrs_ml_params;
+
 /* LsqmlParams (ptyrs/src/engine/lsqml.rs:34-53) and Cost (ptyrs/src/engine/features.rs). */
 enum { PTYRS_AMPLITUDE = 0, PTYRS_POISSON = 1 };
 typedef struct {
@@ -166,6 +175,10 @@ int ptyrs_engine_rpie(ptyrs_engine* e, const ptyrs_rpie_params* params, size_t n
 int ptyrs_engine_ap(ptyrs_engine* e, const ptyrs_ap_params* params, size_t ncycle);
 int ptyrs_engine_dm(ptyrs_engine* e, const ptyrs_dm_params* params, size_t ncycle);
 int ptyrs_engine_raar(ptyrs_engine* e, const ptyrs_raar_params* params, size_t ncycle);
+
+/* GpuPtycho::ml (ptyrs/src/engine/gpu/ml.rs:38-56): PyNX ScaleObjProbe(absolute=True), then
+ * ncycle cycles of Poisson conjugate gradient (Polak–Ribière β, one Newton step γ). */
+int ptyrs_engine_ml(ptyrs_engine* e, const ptyrs_ml_params* params, size_t ncycle);
 
 /* GpuPtycho::lsqml (ptyrs/src/engine/gpu/lsqml.rs:75-120) without a probe variation or
  * features: nepoch epochs from a fresh LSQML, an LLK record every calc_llk epochs and after

@@ -81,6 +81,11 @@ class _RaarC(ctypes.Structure):
                 ("calc_llk", ctypes.c_size_t), ("zero_phase_ramp", ctypes.c_int)]
 
 
+class _MlC(ctypes.Structure):
+    _fields_ = [("update_object", ctypes.c_int), ("update_probe", ctypes.c_int),
+                ("calc_llk", ctypes.c_size_t), ("zero_phase_ramp", ctypes.c_int)]
+
+
 class _LsqmlC(ctypes.Structure):
     _fields_ = [("batch_size", ctypes.c_size_t), ("cost", ctypes.c_int), ("seed", ctypes.c_uint64)]
 
@@ -110,6 +115,7 @@ _SIGNATURES = [
     ("ptyrs_engine_ap", [_P, ctypes.POINTER(_ApC), _SZ], ctypes.c_int),
     ("ptyrs_engine_dm", [_P, ctypes.POINTER(_DmC), _SZ], ctypes.c_int),
     ("ptyrs_engine_raar", [_P, ctypes.POINTER(_RaarC), _SZ], ctypes.c_int),
+    ("ptyrs_engine_ml", [_P, ctypes.POINTER(_MlC), _SZ], ctypes.c_int),
     ("ptyrs_engine_lsqml", [_P, ctypes.POINTER(_LsqmlC), _SZ, _SZ], ctypes.c_int),
     ("ptyrs_engine_lsqml_steps", [_P, ctypes.POINTER(_StepC), _SZ, ctypes.POINTER(_SZ)],
      ctypes.c_int),
@@ -282,6 +288,7 @@ _DM_PARAMS = dict(obj_inertia=0.01, probe_inertia=0.001, alpha=0.02, loop_obj_pr
                   zero_phase_ramp=True)
 _RAAR_PARAMS = dict(obj_inertia=0.01, probe_inertia=0.001, beta=0.9, loop_obj_probe=1, calc_llk=0,
                     zero_phase_ramp=True)
+_ML_PARAMS = dict(update_object=True, update_probe=False, calc_llk=0, zero_phase_ramp=True)
 _RPIE_PARAMS = dict(obj_step=1.0, probe_step=1.0, obj_alpha=0.1, probe_alpha=0.1,
                     batch_size=100, calc_llk=0, seed=7)
 
@@ -371,6 +378,16 @@ class GpuPtycho:
         self._inertias(c, p)
         c.beta, c.loop_obj_probe = float(p["beta"]), int(p["loop_obj_probe"])
         _check(library().ptyrs_engine_raar(self._h, ctypes.byref(c), int(ncycle)))
+
+    def ml(self, ncycle, **params):
+        """`ncycle` ML cycles; keywords are MlParams fields (ptyrs/src/engine/ptycho.rs:123-143)."""
+        p = _overlay(_ML_PARAMS, params, "MlParams")
+        c = _MlC()
+        c.update_object, c.update_probe = int(bool(p["update_object"])), int(bool(p["update_probe"]))
+        if int(p["calc_llk"]) < 0:
+            raise ValueError("calc_llk must be >= 0")
+        c.calc_llk, c.zero_phase_ramp = int(p["calc_llk"]), int(bool(p["zero_phase_ramp"]))
+        _check(library().ptyrs_engine_ml(self._h, ctypes.byref(c), int(ncycle)))
 
     def rpie(self, nepoch, **params):
         """`nepoch` rPIE epochs; keywords are RpieParams fields (ptyrs/src/engine/ptycho.rs:145-186);

@@ -231,6 +231,14 @@ int ptyrs_engine_raar(ptyrs_engine* h, const ptyrs_raar_params* p, size_t ncycle
   });
 }
 
+int ptyrs_engine_ml(ptyrs_engine* h, const ptyrs_ml_params* p, size_t ncycle) {
+  if (!p) {
+    last_error = "null params";
+    return PTYRS_EINVAL;
+  }
+  return guarded(h, true, [&](EngineBase& e) { e.ml(*p, ncycle); });
+}
+
 int ptyrs_engine_lsqml(ptyrs_engine* h, const ptyrs_lsqml_params* p, size_t nepoch,
                        size_t calc_llk) {
   if (!p) {

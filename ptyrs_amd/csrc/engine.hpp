@@ -91,6 +91,7 @@ class EngineBase {
   virtual void rpie(const ptyrs_rpie_params& p, size_t nepoch) = 0;
   virtual void lsqml(const ptyrs_lsqml_params& p, size_t nepoch, size_t calc_llk) = 0;
   virtual void project(const ProjParams& p, size_t ncycle) = 0;
+  virtual void ml(const ptyrs_ml_params& p, size_t ncycle) = 0;
   virtual void llk(double out[4]) = 0;
   virtual void get_object(double* out) = 0;
   virtual void get_probe(double* out) = 0;
@@ -190,6 +191,7 @@ class Engine : public EngineBase {
     corners = pre.corner;
     scale_ = pre.exit.scale;
     max_frame_total_ = pre.max_frame_total;
+    total_ = pre.total;
     M_ = int(nprobe * nobj);
 
     O_.ensure(sizeof(cplx<T>) * nobj * nyo * nxo);
@@ -453,11 +455,187 @@ class Engine : public EngineBase {
     if (p.zero_phase_ramp) host_zero_phase_ramp();
   }
 
+  // PyNX ML, Poisson conjugate gradient (ptyrs/src/engine/ptycho.rs:358-443): scale_obj_probe,
+  // then per cycle the gradients, Polak–Ribière β, d = β·d − g, the Newton step γ along d and
+  // O += γ·d_O, P += γ·d_P (or d = −g where γ is not finite).
+  void ml(const ptyrs_ml_params& p, size_t ncycle) override {
+    if (nobj_ != 1) throw Invalid("the GPU ML runs one object mode");
+    if (M_ > 4) throw Invalid("the GPU engine runs at most 4 modes");
+    if (nframe_ == 0) {
+      cycle += ncycle;
+      return;
+    }
+    const bool uo = p.update_object != 0, up = p.update_probe != 0;
+    const size_t stack = std::min<size_t>(nframe_, stack_size_ ? stack_size_ : kDefaultStack);
+    ensure_workspace(stack, 1);
+    const size_t no = nobj_ * nyo_ * nxo_, np = nprobe_ * n;
+    const int nt = ntiles();
+    carried_.ensure(sizeof(cplx<T>) * nframe_ * M_ * n);
+    psi3_.ensure(sizeof(cplx<T>) * 3 * stack * M_ * n);
+    for (DevBuf* b : {&gO_, &hO_, &dOc_}) b->ensure(sizeof(cplx<T>) * no);
+    for (DevBuf* b : {&gP_, &hP_, &dPc_, &dWt_}) b->ensure(sizeof(cplx<T>) * np);
+    list_all_.ensure(sizeof(int) * size_t(nt) * nframe_);
+    count_all_.ensure(sizeof(int) * size_t(nt));
+    const int nstack = int((nframe_ + stack - 1) / stack);
+    const int chunk = chunk_of(int(stack));
+    const int nchunk = int((stack + chunk - 1) / chunk);
+    pnum_part_.ensure(sizeof(cplx<T>) * size_t(nstack) * nchunk * M_ * n);
+    pden_part_.ensure(sizeof(T) * size_t(nstack) * nchunk * n);
+    pnum_.ensure(sizeof(cplx<T>) * M_ * n);
+    pden_.ensure(sizeof(T) * n);
+    pmaxpart_.ensure(sizeof(T) * ((n + THREADS - 1) / THREADS));
+    abs_part_.ensure(sizeof(double) * 2 * kAbsBlocks);
+    cg_part_.ensure(sizeof(double) * 2 * kAbsBlocks);
+    cg_sums_.ensure(sizeof(double) * 2);
+    gam_part_.ensure(sizeof(double) * 2 * nframe_ * SC::NBLK);
+    gam_.ensure(sizeof(double) * 2);
+    PTYRS_HIP(hipMemsetAsync(dOc_.p, 0, sizeof(cplx<T>) * no, stream_));
+    PTYRS_HIP(hipMemsetAsync(dPc_.p, 0, sizeof(cplx<T>) * np, stream_));
+    PTYRS_HIP(hipMemsetAsync(gO_.p, 0, sizeof(cplx<T>) * no, stream_));
+    PTYRS_HIP(hipMemsetAsync(gP_.p, 0, sizeof(cplx<T>) * np, stream_));
+    probe_terms();
+    // ScaleObjProbe(absolute=True) from the model photons of the start.
+    llk_pass(0);
+    balance(1.0, hist_.as<double>() + 3);
+    if (uo) {
+      run(K_BIN, [&] {
+        hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                           corner_.as<int2>(), iota_.as<int>(), int(nframe_), N, ntx_,
+                           int(nframe_), list_all_.as<int>(), count_all_.as<int>(),
+                           (const cplx<T>*)nullptr, M_, int(nyo_), int(nxo_), (T*)nullptr,
+                           tilemax_.as<T>());
+      });
+    }
+    const int grid_o = int(std::min<size_t>(kAbsBlocks, (no + THREADS - 1) / THREADS));
+    const int grid_p = int(std::min<size_t>(kAbsBlocks, (np + THREADS - 1) / THREADS));
+    std::vector<std::pair<size_t, int>> due;
+    for (size_t ic = 0; ic < ncycle; ++ic) {
+      const bool rec = p.calc_llk > 0 && (ic % p.calc_llk == 0 || ic + 1 == ncycle);
+      if (rec) {
+        if (due.size() == kHistSlots) drain(due, PTYRS_ML);
+        const int slot = int(due.size());
+        llk_pass(slot);
+        due.push_back({cycle, slot});
+      }
+      // Gradients: 2·∂L/∂ψ* back-projected (ptyrs/src/engine/ptycho.rs:93-162).
+      for (int s = 0; s < nstack; ++s) {
+        const int f0 = int(s * stack);
+        const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+        const int* ord = iota_.as<int>() + f0;
+        pass_a(ord, nb, false);
+        pass_b<B_GRAD>(ord, nb);
+        const int ck = chunk_of(nb);
+        const int nck = (nb + ck - 1) / ck;
+        run(K_PASS_C, [&] {
+          hipLaunchKernelGGL((k_pass_c_proj<T, N>), dim3(SR::NBLK, M_, nck), dim3(SR::NT), 0,
+                             stream_, psi_.as<cplx<T>>(), carried_.as<cplx<T>>(),
+                             O_.as<cplx<T>>(), int(nxo_), Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
+                             corner_.as<int2>(), ord, nb, M_, ck, 0, T(0), T(0), int(up),
+                             pnum_part_.as<cplx<T>>() + size_t(s) * nchunk * M_ * n,
+                             pden_part_.as<T>() + size_t(s) * nchunk * n, tw_.as<cplx<T>>());
+        });
+        if (up && nck < nchunk) zero_partials(s, nck, nchunk);
+      }
+      if (uo) {
+        run(K_OBJ_UPDATE, [&] {
+          hipLaunchKernelGGL(k_lsqml_obj_dir<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                             gO_.as<cplx<T>>(), Wt_.as<cplx<T>>(), carried_.as<cplx<T>>(), N,
+                             M_, corner_.as<int2>(), iota_.as<int>(), list_all_.as<int>(),
+                             count_all_.as<int>(), int(nframe_), int(nyo_), int(nxo_), ntx_,
+                             (const T*)nullptr, (const T*)nullptr);
+        });
+      }
+      if (up) {
+        const int nblk = (n + THREADS - 1) / THREADS;
+        run(K_PROBE, [&] {
+          hipLaunchKernelGGL(k_probe_reduce<T>, dim3(nblk), dim3(THREADS), 0, stream_,
+                             pnum_part_.as<cplx<T>>(), pden_part_.as<T>(), nstack * nchunk, n,
+                             M_, gP_.as<cplx<T>>(), pden_.as<T>(), pmaxpart_.as<T>());
+        });
+      }
+      // Polak–Ribière β over both arrays, then the direction; h keeps this cycle's g.
+      run(K_OTHER, [&] {
+        if (ic > 0) {
+          PTYRS_HIP(hipMemsetAsync(cg_sums_.p, 0, sizeof(double) * 2, stream_));
+          if (uo) {
+            hipLaunchKernelGGL(k_cg_dots<T>, dim3(grid_o), dim3(THREADS), 0, stream_,
+                               gO_.as<cplx<T>>(), hO_.as<cplx<T>>(), no, cg_part_.as<double>());
+            hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(THREADS), 0, stream_,
+                               cg_part_.as<double>(), grid_o, cg_sums_.as<double>(), 1);
+          }
+          if (up) {
+            hipLaunchKernelGGL(k_cg_dots<T>, dim3(grid_p), dim3(THREADS), 0, stream_,
+                               gP_.as<cplx<T>>(), hP_.as<cplx<T>>(), np, cg_part_.as<double>());
+            hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(THREADS), 0, stream_,
+                               cg_part_.as<double>(), grid_p, cg_sums_.as<double>(), 1);
+          }
+        }
+        if (uo)
+          hipLaunchKernelGGL(k_cg_direction<T>, dim3((no + 255) / 256), dim3(256), 0, stream_,
+                             dOc_.as<cplx<T>>(), hO_.as<cplx<T>>(), gO_.as<cplx<T>>(), no,
+                             cg_sums_.as<double>(), int(ic == 0));
+        if (up)
+          hipLaunchKernelGGL(k_cg_direction<T>, dim3((np + 255) / 256), dim3(256), 0, stream_,
+                             dPc_.as<cplx<T>>(), hP_.as<cplx<T>>(), gP_.as<cplx<T>>(), np,
+                             cg_sums_.as<double>(), int(ic == 0));
+        hipLaunchKernelGGL(k_probe_terms<T>, dim3((np + 255) / 256), dim3(256), 0, stream_,
+                           dPc_.as<cplx<T>>(), q_.as<cplx<T>>(), dWt_.as<cplx<T>>(), n, int(np));
+      });
+      // Newton step γ = num/den along d (ptyrs/src/engine/ops.rs:197-253).
+      const size_t field = stack * M_ * n;
+      for (int s = 0; s < nstack; ++s) {
+        const int f0 = int(s * stack);
+        const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+        const int* ord = iota_.as<int>() + f0;
+        run(K_PASS_A, [&] {
+          hipLaunchKernelGGL((k_pass_a3<T, N>), dim3(S::NBLK, nb * M_), dim3(S::NT), 0, stream_,
+                             O_.as<cplx<T>>(), dOc_.as<cplx<T>>(), int(nxo_), Wt_.as<cplx<T>>(),
+                             dWt_.as<cplx<T>>(), corner_.as<int2>(), ord, M_, field,
+                             psi3_.as<cplx<T>>(), tw_.as<cplx<T>>());
+        });
+        run(K_PASS_B, [&] {
+          auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(SC::NBLK, nb), dim3(SC::NT), 0, stream_,
+                               psi3_.as<cplx<T>>(), field, counts_.as<float>(), ord,
+                               gam_part_.as<double>() + size_t(f0) * SC::NBLK * 2,
+                               tw_.as<cplx<T>>());
+          };
+          switch (M_) {
+            case 1: go(k_pass_b_gamma<T, N, 1>); break;
+            case 2: go(k_pass_b_gamma<T, N, 2>); break;
+            case 3: go(k_pass_b_gamma<T, N, 3>); break;
+            default: go(k_pass_b_gamma<T, N, 4>); break;
+          }
+        });
+      }
+      run(K_OTHER, [&] {
+        hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(THREADS), 0, stream_,
+                           gam_part_.as<double>(), int(nframe_ * SC::NBLK), gam_.as<double>(), 0);
+        if (uo)
+          hipLaunchKernelGGL(k_cg_step<T>, dim3((no + 255) / 256), dim3(256), 0, stream_,
+                             O_.as<cplx<T>>(), dOc_.as<cplx<T>>(), gO_.as<cplx<T>>(), no,
+                             gam_.as<double>());
+        if (up)
+          hipLaunchKernelGGL(k_cg_step<T>, dim3((np + 255) / 256), dim3(256), 0, stream_,
+                             P_.as<cplx<T>>(), dPc_.as<cplx<T>>(), gP_.as<cplx<T>>(), np,
+                             gam_.as<double>());
+        hipLaunchKernelGGL(k_probe_terms<T>, dim3((np + 255) / 256), dim3(256), 0, stream_,
+                           P_.as<cplx<T>>(), q_.as<cplx<T>>(), Wt_.as<cplx<T>>(), n, int(np));
+      });
+      ++cycle;
+    }
+    drain(due, PTYRS_ML);
+    synchronize();
+    flush_timing();
+    if (p.zero_phase_ramp) host_zero_phase_ramp();
+  }
+
  private:
   static constexpr size_t kHistSlots = 64;
   static constexpr size_t kDefaultStack = 128;
   int M_ = 1;
-  double scale_ = 1.0, max_frame_total_ = 0.0;
+  double scale_ = 1.0, max_frame_total_ = 0.0, total_ = 0.0;
+  DevBuf psi3_, gO_, hO_, dOc_, gP_, hP_, dPc_, dWt_, cg_part_, cg_sums_, gam_part_, gam_;
   DevBuf carried_, iter_, list_all_, count_all_, pnum_part_, pden_part_, pnum_, pden_,
       pmaxpart_, pmax2_, abs_part_;
   DevBuf dO_, pre_, illum_, afar_all_, afar_tmp_, prev_, icalc_, xi_part_, xi_part2_, dots_part_,
@@ -693,8 +871,9 @@ class Engine : public EngineBase {
     });
   }
 
-  // split_scale(s) (ptyrs/src/engine/ptycho.rs:184-192).
-  void balance(double s) {
+  // split_scale(s) (ptyrs/src/engine/ptycho.rs:184-192); with `calc` (device, the model photons
+  // over unmasked pixels) s = √(Σ iobs / calc), scale_obj_probe (:172-182).
+  void balance(double s, const double* calc = nullptr) {
     run(K_OTHER, [&] {
       hipLaunchKernelGGL(k_abs_sums<T>, dim3(kAbsBlocks), dim3(THREADS), 0, stream_,
                          O_.as<cplx<T>>(), nobj_ * nyo_ * nxo_, P_.as<cplx<T>>(), nprobe_ * n,
@@ -702,7 +881,7 @@ class Engine : public EngineBase {
       hipLaunchKernelGGL(k_balance<T>, dim3(kAbsBlocks), dim3(THREADS), 0, stream_,
                          O_.as<cplx<T>>(), nobj_ * nyo_ * nxo_, P_.as<cplx<T>>(),
                          Wt_.as<cplx<T>>(), q_.as<cplx<T>>(), n, nprobe_ * n,
-                         abs_part_.as<double>(), kAbsBlocks, s);
+                         abs_part_.as<double>(), kAbsBlocks, s, calc, total_);
     });
   }
 

@@ -184,7 +184,8 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a(
 }
 
 // ---- pass B: column transforms, the detector-plane operator, inverse column transforms --------
-enum PassB { B_RPIE = 0, B_LLK = 1, B_AMPLITUDE = 2, B_GAUSS = 3, B_ICALC = 4, B_POISSON = 5 };
+enum PassB { B_RPIE = 0, B_LLK = 1, B_AMPLITUDE = 2, B_GAUSS = 3, B_ICALC = 4, B_POISSON = 5,
+             B_GRAD = 6 };
 
 // Per-slot values some pass-B modes read or write.
 template <typename T>
@@ -256,6 +257,16 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
       if (io >= 0.f) {
         T calc = sqrt(I * inv_n);
         r = sqrt(T(io)) / (calc > T(1e-12) ? calc : T(1e-12)) * inv_n;
+      }
+#pragma unroll
+      for (int m = 0; m < M; ++m) v[m][e] = v[m][e] * r;
+    } else if constexpr (MODE == B_GRAD) {
+      // PyNX GradPoissonFourier: 2(1 − iobs/max(icalc, 1e-12)), 0 where masked
+      // (ptyrs/src/engine/ops.rs:156-195).
+      T r = 0;
+      if (io >= 0.f) {
+        const T ic = I * inv_n;
+        r = T(2) * (T(1) - T(io) / (ic > T(1e-12) ? ic : T(1e-12))) * inv_n;
       }
 #pragma unroll
       for (int m = 0; m < M; ++m) v[m][e] = v[m][e] * r;
@@ -666,9 +677,10 @@ __global__ __launch_bounds__(THREADS) void k_lsqml_xi_sums(
   }
 }
 
-// LSQML object direction: every object pixel owned by one thread sums, in slot order,
+// LSQML object direction (and, with pre = null, the object back-projection numerator of ML's
+// gradient): every object pixel owned by one thread sums, in slot order,
 // Σ_m Σ_views conj(W_m)·χ_m (Kahan-compensated) and divides by √(pre² + floor); 0 where no view
-// covers it. ptyrs/src/engine/lsqml.rs:183-188, :587-598
+// covers it. ptyrs/src/engine/lsqml.rs:183-188, :587-598; ptyrs/src/engine/ops.rs:279-310
 template <typename T>
 __global__ __launch_bounds__(THREADS) void k_lsqml_obj_dir(
     cplx<T>* __restrict__ dO, const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ chi,
@@ -710,14 +722,17 @@ __global__ __launch_bounds__(THREADS) void k_lsqml_obj_dir(
     __syncthreads();
   }
   if (X >= nxo) return;
-  const T fl = floor[0];
+  const T fl = pre ? floor[0] : T(0);
 #pragma unroll
   for (int r = 0; r < TROWS; ++r) {
     const int Y = Y0 + RS * r;
     if (Y >= nyo) continue;
     const size_t p = (size_t)Y * nxo + X;
-    const T pr = pre[p];
-    const T inv = T(1) / sqrt(pr * pr + fl);
+    T inv = T(1);
+    if (pre) {
+      const T pr = pre[p];
+      inv = T(1) / sqrt(pr * pr + fl);
+    }
     st(dO + p, cplx<T>{sr[r].s * inv, si[r].s * inv});
   }
 }
@@ -1158,9 +1173,13 @@ __global__ __launch_bounds__(THREADS) void k_balance(cplx<T>* __restrict__ O, si
                                                      cplx<T>* __restrict__ P, cplx<T>* __restrict__ Wt,
                                                      const cplx<T>* __restrict__ q, int n, size_t np,
                                                      const double* __restrict__ part, int nparts,
-                                                     double s) {
+                                                     double s, const double* __restrict__ calc,
+                                                     double total) {
   __shared__ double fac[2];
   if (threadIdx.x == 0) {
+    // With `calc` (the model photons over unmasked pixels), s = √(total/calc): PyNX
+    // ScaleObjProbe(absolute=True), ptyrs/src/engine/ptycho.rs:172-182.
+    if (calc) s = sqrt(total / calc[0]);
     double so = 0, sp = 0;
     for (int i = 0; i < nparts; ++i) {
       so += part[2 * i];
@@ -1177,6 +1196,191 @@ __global__ __launch_bounds__(THREADS) void k_balance(cplx<T>* __restrict__ O, si
     const cplx<T> p = ld(P + i) * fp;
     st(P + i, p);
     st(Wt + i, p * ld(q + i % n));
+  }
+}
+
+// ---- ML (PyNX Poisson conjugate gradient) ------------------------------------------------------------
+// Pass A of the Newton step: per view the rows of po = W·O, a = W·dO + dW·O and b = dW·dO
+// (dW = dP·q), each row-transformed, into psi3[field][slot][mode][N][N].
+template <typename T, int N>
+__global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a3(
+    const cplx<T>* __restrict__ O, const cplx<T>* __restrict__ dO, int nxo,
+    const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ dWt,
+    const int2* __restrict__ corner, const int* __restrict__ order, int M, size_t field,
+    cplx<T>* __restrict__ psi3, const cplx<T>* __restrict__ tw) {
+  using S = RowShape<T, N>;
+  constexpr int E = S::E, TL = S::TL;
+  __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
+  __shared__ cplx<T> stw[N];
+  const int tid = threadIdx.x, l = tid / TL, jl = tid % TL;
+  const int y = blockIdx.x * S::L + l;
+  const int b = blockIdx.y / M, m = blockIdx.y % M;
+  stage_twiddles<T, N, S::NT>(stw, tw);
+  const int2 c = corner[order[b]];
+  const size_t op = (size_t)(c.x + y) * nxo + c.y;
+  const size_t wp = ((size_t)m * N + y) * N;
+  cplx<T> po[E], a[E], bb[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int x = jl + TL * e;
+    const cplx<T> o = ld(O + op + x), d = ld(dO + op + x);
+    const cplx<T> w = ld(Wt + wp + x), dw = ld(dWt + wp + x);
+    po[e] = w * o;
+    a[e] = w * d + dw * o;
+    bb[e] = dw * d;
+  }
+  T* re = lre + l * S::LD;
+  T* im = lim + l * S::LD;
+  line_fft<T, N, E, false>(po, jl, re, im, stw);
+  line_fft<T, N, E, false>(a, jl, re, im, stw);
+  line_fft<T, N, E, false>(bb, jl, re, im, stw);
+  const size_t row = ((size_t)(b * M + m) * N + y) * N;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    st(psi3 + row + jl + TL * e, po[e]);
+    st(psi3 + field + row + jl + TL * e, a[e]);
+    st(psi3 + 2 * field + row + jl + TL * e, bb[e]);
+  }
+}
+
+// Pass B of the Newton step: column transforms of po, a, b per mode, then over unmasked pixels
+// with calc = max(Σ|po|², 1e-32), r = Σ Re(conj(po)·a), s = Σ |a|² + 2·Re(conj(po)·b),
+// d = 1 − iobs/calc: num −= 2·r·d, den += 2·s·d + iobs·(2r/calc)² (FP64, per workgroup into
+// part[slot][blockIdx.x][2]). ptyrs/src/engine/ops.rs:197-253
+template <typename T, int N, int M>
+__global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b_gamma(
+    const cplx<T>* __restrict__ psi3, size_t field, const float* __restrict__ counts,
+    const int* __restrict__ order, double* __restrict__ part, const cplx<T>* __restrict__ tw) {
+  using S = ColShape<T, N>;
+  constexpr int E = S::E, TL = S::TL, CG = S::L;
+  constexpr int n = N * N;
+  __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
+  __shared__ cplx<T> stw[N];
+  __shared__ double red[S::NT];
+  const int tid = threadIdx.x, col = tid % CG, jl = tid / CG;
+  const int x = blockIdx.x * CG + col;
+  const int b = blockIdx.y;
+  const int f = order[b];
+  T* re = lre + col * S::LD;
+  T* im = lim + col * S::LD;
+  stage_twiddles<T, N, S::NT>(stw, tw);
+  T calc[E], r[E], s[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) calc[e] = r[e] = s[e] = 0;
+  for (int m = 0; m < M; ++m) {
+    const size_t plane = (size_t)(b * M + m) * n;
+    cplx<T> po[E], a[E], bb[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const size_t i = plane + (jl + TL * e) * N + x;
+      po[e] = ld(psi3 + i);
+      a[e] = ld(psi3 + field + i);
+      bb[e] = ld(psi3 + 2 * field + i);
+    }
+    line_fft<T, N, E, false>(po, jl, re, im, stw);
+    line_fft<T, N, E, false>(a, jl, re, im, stw);
+    line_fft<T, N, E, false>(bb, jl, re, im, stw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      calc[e] += norm2(po[e]);
+      r[e] += cmulc(po[e], a[e]).x;
+      s[e] += norm2(a[e]) + T(2) * cmulc(po[e], bb[e]).x;
+    }
+  }
+  const double inv_n = 1.0 / double(n);
+  double num = 0, den = 0;
+  const float* cnt = counts + (size_t)f * n;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const float io = cnt[(jl + TL * e) * N + x];
+    if (io >= 0.f) {
+      double cl = double(calc[e]) * inv_n;
+      cl = cl > 1e-32 ? cl : 1e-32;
+      const double rr = double(r[e]) * inv_n, ss = double(s[e]) * inv_n, ob = double(io);
+      const double d = 1.0 - ob / cl;
+      num -= 2.0 * rr * d;
+      const double t = 2.0 * rr / cl;
+      den += 2.0 * ss * d + ob * t * t;
+    }
+  }
+  const double rn = block_reduce<S::NT>(num, red, AddOp());
+  const double rd = block_reduce<S::NT>(den, red, AddOp());
+  if (tid == 0) {
+    part[((size_t)b * gridDim.x + blockIdx.x) * 2] = rn;
+    part[((size_t)b * gridDim.x + blockIdx.x) * 2 + 1] = rd;
+  }
+}
+
+// Sums of pairs in a fixed order: out[0] = Σ part[2i], out[1] = Σ part[2i + 1] (add = 1 adds to
+// out instead of writing it).
+static __global__ void k_sum_pairs(const double* __restrict__ part, int count, double* __restrict__ out,
+                            int add) {
+  __shared__ double red[THREADS];
+  double a = 0, b = 0;
+  for (int i = threadIdx.x; i < count; i += THREADS) {
+    a += part[2 * (size_t)i];
+    b += part[2 * (size_t)i + 1];
+  }
+  a = block_reduce<THREADS>(a, red, AddOp());
+  b = block_reduce<THREADS>(b, red, AddOp());
+  if (threadIdx.x == 0) {
+    out[0] = add ? out[0] + a : a;
+    out[1] = add ? out[1] + b : b;
+  }
+}
+
+// Polak–Ribière sums over one array in FP64: part[2·block] = Σ Re(conj(g)·(g − h)),
+// part[2·block + 1] = Σ |h|². ptyrs/src/engine/ptycho.rs:390-403
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_cg_dots(const cplx<T>* __restrict__ g,
+                                                     const cplx<T>* __restrict__ h, size_t total,
+                                                     double* __restrict__ part) {
+  __shared__ double red[THREADS];
+  double a = 0, b = 0;
+  for (size_t i = (size_t)blockIdx.x * THREADS + threadIdx.x; i < total; i += (size_t)gridDim.x * THREADS) {
+    const cplx<T> gv = ld(g + i), hv = ld(h + i);
+    a += double(gv.x) * (double(gv.x) - hv.x) + double(gv.y) * (double(gv.y) - hv.y);
+    b += double(hv.x) * hv.x + double(hv.y) * hv.y;
+  }
+  a = block_reduce<THREADS>(a, red, AddOp());
+  b = block_reduce<THREADS>(b, red, AddOp());
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// d = β·d − g with β = num/den from sums[0..1] (0 when negative, not finite, or first);
+// then h = g.
+template <typename T>
+__global__ void k_cg_direction(cplx<T>* __restrict__ d, cplx<T>* __restrict__ h,
+                               const cplx<T>* __restrict__ g, size_t total,
+                               const double* __restrict__ sums, int first) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  double beta = 0;
+  if (!first) {
+    const double b = sums[0] / sums[1];
+    beta = (isfinite(b) && b >= 0.0) ? b : 0.0;
+  }
+  const cplx<T> gv = ld(g + i);
+  st(d + i, ld(d + i) * T(beta) - gv);
+  st(h + i, gv);
+}
+
+// x += γ·d with γ = nd[0]/nd[1]; where γ is not finite, d = −g instead.
+template <typename T>
+__global__ void k_cg_step(cplx<T>* __restrict__ x, cplx<T>* __restrict__ d,
+                          const cplx<T>* __restrict__ g, size_t total,
+                          const double* __restrict__ nd) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const double gamma = nd[0] / nd[1];
+  if (isfinite(gamma)) {
+    st(x + i, ld(x + i) + ld(d + i) * T(gamma));
+  } else {
+    const cplx<T> gv = ld(g + i);
+    st(d + i, cplx<T>{-gv.x, -gv.y});
   }
 }
 

@@ -64,3 +64,20 @@ def test_projection_fixed_arrays_and_repeat_calls():
         g.dm(1, loop_obj_probe=0)
     with pytest.raises(TypeError):
         g.ap(1, beta=0.5)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("probe", [False, True])
+def test_ml_matches_oracle(dtype, probe):
+    d = small_scan(n=64, nprobe=2, npos=30)
+    g, o = gpu(d, dtype), oracle(d)
+    o.ap(5, zero_phase_ramp=False)
+    g.ap(5, zero_phase_ramp=False)
+    g.ml(6, update_probe=probe, calc_llk=1)
+    o.ml(6, update_probe=probe, calc_llk=1)
+    want = [r[2].poisson for r in o.history if r[1] == "ml"]
+    got = [v for v, a in zip(g.history["poisson"], g.history["algorithm"]) if a == "ml"]
+    tol = {"c128": 1e-9, "c64": 1e-4}[dtype]
+    assert max_rel(got, want) < tol
+    assert rel_l2(g.object, o.obj) < tol * 10
+    assert rel_l2(g.probe, o.probe) < tol * 10
