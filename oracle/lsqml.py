@@ -1,6 +1,7 @@
-"""pty-chi LSQML as the reference's CPU engine restates it, single slice, no position or OPR
-features: one batch's directions (`LsqmlBatch`), the per-frame 2×2 step sizes, the step and
-the epochs with the preconditioner schedule.
+"""pty-chi LSQML as the reference's CPU engine restates it, one object slice or several
+(`SliceWaves`), no position or OPR features: one batch's directions (`LsqmlBatch`), the
+per-frame step sizes, the step and the epochs with the preconditioner schedule. An object is
+one `[nobj, nyo, nxo]` array, or a list of them, one per slice of the optics.
 
 Test infrastructure (see oracle/__init__.py). Follows ptyrs/src/engine/lsqml.rs.
 """
@@ -10,6 +11,7 @@ import math
 import numpy as np
 
 from .base import random_batches, to_natural, to_shifted
+from .multislice import SliceWaves
 from .ptycho import obj_probe_to_psi, llk
 
 EPS_AMPLITUDE = 1e-9   # ptyrs/src/engine/lsqml.rs:25-31
@@ -48,20 +50,25 @@ def frame_mean(d, np_):
     return out / count
 
 
+def as_slices(obj):
+    """An object as its list of slices."""
+    return list(obj) if isinstance(obj, (list, tuple)) else [obj]
+
+
 class LsqmlBatch:
     """One batch's directions at one state. ptyrs/src/engine/lsqml.rs:97-181"""
 
     def __init__(self, run, views, iobs, optics, obj, probe):
         cost = run.cost
         ny, nx = probe.shape[-2:]
-        n = ny * nx
         nmode = 0
         while nmode < len(views) and views[nmode][0] == views[0][0]:
             nmode += 1
         self.views, self.nmode = views, nmode
         self.frames = [views[k][0] for k in range(0, len(views), nmode)]
-        self.obj, self.probe, self.optics = obj, probe, optics
-        psi = obj_probe_to_psi(None, views, obj, probe, optics.quad)
+        self.objects, self.probe, self.optics = as_slices(obj), probe, optics
+        self.waves = SliceWaves(optics, views, self.objects, probe)
+        psi = self.waves.psi
         self.psi = psi
         far = optics.exit_forward(psi)
         nb = len(self.frames)
@@ -95,43 +102,37 @@ class LsqmlBatch:
         c = np.where(masked, 0.0, c)
         scale = 1.0 - np.asarray(a_far)[:, None, None] * c
         chi = (fa * scale[:, None]).reshape(far.shape)
-        self.chi = optics.exit_adjoint(chi) - psi
+        # chi_s per slice, the exit's on the last (ptyrs/src/engine/lsqml.rs:166-169)
+        self.chis = self.waves.back_propagate(optics.exit_adjoint(chi) - psi)
+        self.chi = self.chis[0]
+
+    def obj_directions(self):
+        """pty-chi `delta_o_comb` per slice. ptyrs/src/engine/lsqml.rs:191-198"""
+        return [self.waves.obj_adjoint(s, c) for s, c in enumerate(self.chis)]
 
     def obj_direction(self):
-        """Σ_views conj(P·q)·chi on each window. ptyrs/src/engine/lsqml.rs:183-188"""
-        ny, nx = self.probe.shape[-2:]
-        out = np.zeros(self.obj.shape, dtype=np.complex128)
-        for k, (_, _, p, o, oy, ox) in enumerate(self.views):
-            out[o, oy:oy + ny, ox:ox + nx] += to_natural(self.chi[k]) * np.conj(
-                self.probe[p] * self.optics.quad)
-        return out
+        """The first slice's object direction."""
+        return self.obj_directions()[0]
 
     def frame_probe_directions(self):
-        """Plane b·np + m: conj(q·O_view)·chi_m. ptyrs/src/engine/lsqml.rs:190-196"""
-        ny, nx = self.probe.shape[-2:]
-        np_ = self.probe.shape[0]
-        nb = len(self.frames)
-        out = np.zeros((nb * np_, ny, nx), dtype=np.complex128)
-        for k, (f, m, p, o, oy, ox) in enumerate(self.views):
-            b = k // self.nmode
-            win = self.obj[o, oy:oy + ny, ox:ox + nx]
-            out[b * np_ + p] += np.conj(self.optics.quad * win) * to_natural(self.chi[k])
-        return out
+        """Plane b·np + m: conj(q·O_view)·chi_m on slice 0. ptyrs/src/engine/lsqml.rs:200-205"""
+        return self.waves.probe_adjoint(self.chis[0])
 
     def probe_direction(self):
         return frame_mean(self.frame_probe_directions(), self.probe.shape[0])
 
     def step_sizes(self, d_obj, d_probe):
-        """Per-frame a_obj, a_probe. ptyrs/src/engine/lsqml.rs:272-350 (one slice)"""
+        """Per slice and frame a_obj, per frame a_probe: slice 0 jointly with the probe by the
+        2×2 system, every other slice alone (Eq. 23b). ptyrs/src/engine/lsqml.rs:272-350"""
+        d_obj = as_slices(d_obj)
         ny, nx = self.probe.shape[-2:]
         n = ny * nx
-        q = self.optics.quad
-        dpo = obj_probe_to_psi(None, self.views, self.obj, d_probe, q)
-        dop = obj_probe_to_psi(None, self.views, d_obj, self.probe, q)
+        dpo = obj_probe_to_psi(None, self.views, self.objects[0], d_probe, self.waves.quad(0))
+        dop = self.waves.obj_tangent(0, d_obj[0])
         lam = LAMBDA0 * self.optics.propagator.intensity_scale() / n
         nb = len(self.frames)
         sh = (nb, self.nmode * n)
-        u, v, c = dop.reshape(sh), dpo.reshape(sh), self.chi.reshape(sh)
+        u, v, c = dop.reshape(sh), dpo.reshape(sh), self.chis[0].reshape(sh)
         a12 = u * np.conj(v)
         sums = np.stack([(u.real ** 2 + u.imag ** 2 + lam).sum(axis=1),
                          (v.real ** 2 + v.imag ** 2 + lam).sum(axis=1),
@@ -139,7 +140,7 @@ class LsqmlBatch:
                          (np.conj(u) * c).real.sum(axis=1),
                          (np.conj(v) * c).real.sum(axis=1)], axis=1)
         reg11, reg22 = 0.1 * sums[:, 0].mean(), 0.1 * sums[:, 1].mean()
-        scale = float(self.nmode)
+        scale = float(len(self.objects) * self.nmode)
         a_obj, a_probe = [], []
         for s in sums:
             a11, a22 = s[0] + reg11, s[1] + reg22
@@ -148,7 +149,16 @@ class LsqmlBatch:
             x1 = (a11 * s[5] - s[2] * s[4]) / det
             a_obj.append(0.9 * max(x0, 0.0) / scale)
             a_probe.append(0.9 * max(x1, 0.0))
-        return a_obj, a_probe, sums
+        a_objs = [a_obj]
+        for s in range(1, len(self.objects)):
+            u = self.waves.obj_tangent(s, d_obj[s]).reshape(sh)
+            c = self.chis[s].reshape(sh)
+            num = (np.conj(u) * c).real.sum(axis=1)
+            den = (u.real ** 2 + u.imag ** 2).sum(axis=1)
+            a_objs.append([max(0.5 * a / b * 0.9 / scale, 0.0) for a, b in zip(num, den)])
+        if len(self.objects) == 1:
+            return a_obj, a_probe, sums
+        return a_objs, a_probe, sums
 
 
 class Lsqml:
@@ -168,25 +178,28 @@ class Lsqml:
         return self.a_far.get(f, 0.5)
 
     def step(self, views, iobs, optics, obj, probe):
-        """`run_minibatch`: directions, step sizes, updates. ptyrs/src/engine/lsqml.rs:563-643"""
+        """`run_minibatch`: directions, step sizes, updates. ptyrs/src/engine/lsqml.rs:563-643
+        Returns the object in the form given (one array, or a list of slices)."""
         pre = self.preconditioner
         floor = (0.1 * pre.max()) ** 2
         b = LsqmlBatch(self, views, iobs, optics, obj, probe)
-        d_obj = b.obj_direction() / np.sqrt(pre * pre + floor)
+        d_obj = [d / np.sqrt(pre * pre + floor) for d in b.obj_directions()]
         d_probe = b.probe_direction()
         a_obj, a_probe, _ = b.step_sizes(d_obj, d_probe)
         for f, a in zip(b.frames, b.a_far):
             self.a_far[f] = a
-        obj = obj + trim_mean(a_obj) * d_obj
+        per_slice = a_obj if len(b.objects) > 1 else [a_obj]
+        new = [o + trim_mean(a) * d for o, a, d in zip(as_slices(obj), per_slice, d_obj)]
         probe = probe + (sum(a_probe) / len(a_probe)) * d_probe
-        return obj, probe, dict(frames=b.frames, a_far=b.a_far, a_obj=a_obj, a_probe=a_probe)
+        out = new if isinstance(obj, (list, tuple)) else new[0]
+        return out, probe, dict(frames=b.frames, a_far=b.a_far, a_obj=a_obj, a_probe=a_probe)
 
     def epoch(self, addr, iobs, optics, obj, probe):
         """One epoch with pty-chi's preconditioner schedule. ptyrs/src/engine/lsqml.rs:645-714"""
         nframe, nmode = addr.nframe, addr.nmode
         views_all = addr.views()
         if self.epochs < 10 or self.epochs % 10 == 0:
-            illum = illumination(views_all, obj.shape, probe)
+            illum = illumination(views_all, as_slices(obj)[0].shape, probe)
             pre = self.preconditioner
             if pre is None:
                 n = probe.shape[-1] * probe.shape[-2]
@@ -206,6 +219,12 @@ class Lsqml:
 
     def llk(self, ptycho, obj, probe):
         """Normalised LLK of obj, probe over every frame. ptyrs/src/engine/lsqml.rs:532-546"""
-        psi = obj_probe_to_psi(None, ptycho.views_all, obj, probe, ptycho.quad)
-        psi = ptycho.optics.exit_forward(psi)
-        return llk(ptycho.iobs, ptycho.addr.nmode, psi).normalised(ptycho.nb_obs)
+        return slice_llk(ptycho.optics, ptycho.views_all, ptycho.iobs, ptycho.addr.nmode,
+                         ptycho.nb_obs, obj, probe)
+
+
+def slice_llk(optics, views, iobs, nmode, nb_obs, obj, probe):
+    """Normalised LLK of the slice model's exit waves over `views`.
+    ptyrs/src/engine/lsqml.rs:538-555"""
+    psi = optics.exit_forward(SliceWaves(optics, views, as_slices(obj), probe).psi)
+    return llk(iobs, nmode, psi).normalised(nb_obs)
