@@ -9,6 +9,7 @@ same epochs from the same start; `nrmse` compares the two objects after phase-ra
 global-phase alignment on the illuminated region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --copy-probe     # a plain 2 GiB device copy: the PMC calibration kernel
 
 Prints one JSON line.
 """
@@ -28,6 +29,41 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 BATCH = 128
 N, M, NPOS = 256, 4, 1024
+# HBM bytes per dispatch of the dominant kernels at this workload, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes scaled by the copy probe's readings (the file gives the commands).
+TRAFFIC = os.path.join(ROOT, "profiles", "traffic_headline.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` (a kernel kind) from the committed counter summary."""
+    try:
+        with open(TRAFFIC) as f:
+            for line in f:
+                p = line.split()
+                if len(p) == 3 and p[0] == kernel:
+                    return float(p[1]) + float(p[2])
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe(gib=2, reps=20):
+    """A plain device copy of `gib` GiB (float32), the kernel whose FETCH_SIZE and WRITE_SIZE
+    calibrate the counters: its bytes are known. Prints its bandwidth."""
+    import torch
+    n = gib * (1 << 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / reps
+    print(json.dumps({"copy_gib": gib, "read_bytes": n * 4, "write_bytes": n * 4,
+                      "ms": round(dt * 1e3, 4), "gb_per_s": round(2 * n * 4 / dt / 1e9, 1)}),
+          flush=True)
 
 
 def aligned_nrmse(a, b, weight):
@@ -77,11 +113,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--steps64", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every GPU box of this build has one, and the "
               "reconstruction step is not sharded here (see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if a.copy_probe:
+        copy_probe()
+        return
 
     import ptyrs_amd as pa
     from ptyrs_amd import sim
@@ -170,7 +210,7 @@ def main():
                      "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved * 1e9 / HBM_PEAK, 4) if achieved else None,
-                     "traffic": None,
+                     "traffic": measured_traffic(kern),
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_s * 1e3, 4)},
     }
     if not a.no_cpu:

@@ -62,10 +62,15 @@ typedef struct {
 /* The exit-wave model (ptyrs::engine::Optics, ptyrs/src/engine/propagate.rs:24-186):
  * free_space = 0 takes the far field of the scan geometry (Optics::far_field); free_space = 1
  * takes pty-chi's planar model of one Wavefield (Optics::free_space) of wavelength, object
- * pixel [pixel_y, pixel_x] and signed distance (INFINITY for the far field). */
+ * pixel [pixel_y, pixel_x] and signed distance (INFINITY for the far field).
+ * ngap > 0 makes it multislice (Optics::with_slices, ptyrs/src/engine/propagate.rs:84-98):
+ * gaps[4·s ..] = {wavelength, pixel_y, pixel_x, distance} of the angular-spectrum gap from
+ * slice s to s + 1; the object then holds ngap + 1 slices. */
 typedef struct {
   int free_space;
   double wavelength, pixel_y, pixel_x, distance;
+  size_t ngap;
+  const double* gaps;
 } ptyrs_optics;
 
 /* GpuOptions (ptyrs/src/engine/gpu/mod.rs:61-108) plus the element type. stack_size = 0
@@ -160,9 +165,14 @@ typedef struct {
 
 typedef struct ptyrs_engine ptyrs_engine;
 
-/* GpuPtycho::with_optics (ptyrs/src/engine/gpu/mod.rs:619-760): object [nobj, nyo, nxo] and
- * probe [nprobe, ny, nx] complex128; positions go through PyNX preprocessing and become view
- * corners. optics may be NULL (far field of the scan). device is the HIP device ordinal. */
+/* GpuPtycho::with_slices (ptyrs/src/engine/gpu/mod.rs:619-760): object [nslice, nobj, nyo, nxo]
+ * (nslice = optics->ngap + 1, else 1) and probe [nprobe, ny, nx] complex128; positions go
+ * through PyNX preprocessing and become view corners. optics may be NULL (far field of the
+ * scan). device is the HIP device ordinal. More than one slice runs LSQML and the LLK pass
+ * only, of one object mode (the others fail with PTYRS_EINVAL).
+ * An angular-spectrum exit whose transfer function drops an evanescent band is not unitary:
+ * rPIE, AP, DM, RAAR and ML then fail with PTYRS_EINVAL, and LSQML and the LLK pass run
+ * (ptyrs/src/engine/gpu/device.rs:3087-3105). */
 int ptyrs_engine_create(const ptyrs_scan* scan, const ptyrs_optics* optics, const double* obj,
                         size_t nobj, size_t nyo, size_t nxo, const double* probe, size_t nprobe,
                         const ptyrs_gpu_options* options, int device, ptyrs_engine** out);
@@ -187,14 +197,20 @@ int ptyrs_engine_lsqml(ptyrs_engine* e, const ptyrs_lsqml_params* params, size_t
                        size_t calc_llk);
 
 /* GpuPtycho::lsqml_steps (ptyrs/src/engine/gpu/lsqml.rs:122-127): the last LSQML epoch's steps,
- * one per frame in batch order; copies up to cap and sets *n to the number held. */
+ * one per frame in batch order (a_obj of slice 0); copies up to cap and sets *n to the number
+ * held. */
 int ptyrs_engine_lsqml_steps(ptyrs_engine* e, ptyrs_lsqml_step* out, size_t cap, size_t* n);
+
+/* LsqmlStep::a_obj of every slice (ptyrs/src/engine/lsqml.rs:55-66): out[i·nslice + s] is a_obj
+ * of slice s at step i of ptyrs_engine_lsqml_steps; len must be n·nslice. */
+int ptyrs_engine_lsqml_slice_steps(ptyrs_engine* e, double* out, size_t len);
 
 /* GpuPtycho::llk (ptyrs/src/engine/gpu/mod.rs:853-868): normalised LLK of the current state. */
 int ptyrs_engine_llk(ptyrs_engine* e, ptyrs_llk_record* out);
 
-/* GpuPtycho::object / probe (ptyrs/src/engine/gpu/mod.rs:762-815): complex128 downloads of
- * len doubles (2·nobj·nyo·nxo, 2·nprobe·ny·nx). Allowed on an aborted engine. */
+/* GpuPtycho::objects / probe (ptyrs/src/engine/gpu/mod.rs:762-815): complex128 downloads of
+ * len doubles (2·nslice·nobj·nyo·nxo, every slice; 2·nprobe·ny·nx). Allowed on an aborted
+ * engine. */
 int ptyrs_engine_object(ptyrs_engine* e, double* out, size_t len);
 int ptyrs_engine_probe(ptyrs_engine* e, double* out, size_t len);
 

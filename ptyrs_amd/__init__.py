@@ -45,7 +45,8 @@ class _ScanC(ctypes.Structure):
 class _OpticsC(ctypes.Structure):
     _fields_ = [("free_space", ctypes.c_int), ("wavelength", ctypes.c_double),
                 ("pixel_y", ctypes.c_double), ("pixel_x", ctypes.c_double),
-                ("distance", ctypes.c_double)]
+                ("distance", ctypes.c_double), ("ngap", ctypes.c_size_t),
+                ("gaps", ctypes.POINTER(ctypes.c_double))]
 
 
 class _OptionsC(ctypes.Structure):
@@ -119,6 +120,7 @@ _SIGNATURES = [
     ("ptyrs_engine_lsqml", [_P, ctypes.POINTER(_LsqmlC), _SZ, _SZ], ctypes.c_int),
     ("ptyrs_engine_lsqml_steps", [_P, ctypes.POINTER(_StepC), _SZ, ctypes.POINTER(_SZ)],
      ctypes.c_int),
+    ("ptyrs_engine_lsqml_slice_steps", [_P, _D, _SZ], ctypes.c_int),
     ("ptyrs_engine_llk", [_P, ctypes.POINTER(_RecordC)], ctypes.c_int),
     ("ptyrs_engine_object", [_P, _D, _SZ], ctypes.c_int),
     ("ptyrs_engine_probe", [_P, _D, _SZ], ctypes.c_int),
@@ -251,10 +253,11 @@ class Wavefield:
 
 class Optics:
     """The exit-wave model (ptyrs-py/src/cpu.rs:77-135): PyNX's far field or pty-chi's
-    planar free-space model of one Wavefield."""
+    planar free-space model of one Wavefield, through `len(gaps) + 1` object slices."""
 
-    def __init__(self, free_space, wavefield=None, far=None):
+    def __init__(self, free_space, wavefield=None, far=None, gaps=()):
         self._free_space, self.wavefield, self._far = free_space, wavefield, far
+        self.gaps = list(gaps)
 
     @staticmethod
     def far_field(wavelength, distance, detector_pixel, frame):
@@ -264,8 +267,29 @@ class Optics:
     def free_space(w):
         return Optics(True, wavefield=w)
 
+    def with_slices(self, gaps):
+        """This exit model through `len(gaps) + 1` object slices, gap `s` an angular-spectrum
+        Wavefield from slice `s` to `s + 1` (ptyrs-py/src/cpu.rs:122-133)."""
+        for w in gaps:
+            if not isinstance(w, Wavefield):
+                raise TypeError("gaps must be Wavefields")
+            if not math.isfinite(w.distance):
+                raise ValueError("angular spectrum needs a finite distance")
+        return Optics(self._free_space, self.wavefield, self._far, gaps)
+
+    def nslice(self):
+        return len(self.gaps) + 1
+
     def _c(self, scan):
         o = _OpticsC()
+        if self.gaps:
+            for w in self.gaps:
+                if w.shape != scan.frame:
+                    raise ValueError(f"slice gap plane {w.shape} is not the frame {scan.frame}")
+            self._gaps_c = np.ascontiguousarray(
+                [[w.wavelength, w.pixel[0], w.pixel[1], w.distance] for w in self.gaps],
+                dtype=np.float64)
+            o.ngap, o.gaps = len(self.gaps), _ptr(self._gaps_c, ctypes.c_double)
         if self._free_space:
             w = self.wavefield
             if w.shape != scan.frame:
@@ -298,7 +322,9 @@ class GpuPtycho:
     `device`, under `optics` in place of the far field of the scan's geometry if given;
     keywords are GpuOptions fields (stack_size, graph, llk_slots, fused, psi_f16) plus `dtype`,
     "c64" (complex float, the default) or "c128" (complex double, the plain FP64 path).
-    `object` is complex `[nobj, nyo, nxo]`, `probe` complex `[nprobe, ny, nx]`."""
+    `object` is complex `[nobj, nyo, nxo]`, or `[nslice, nobj, nyo, nxo]` with a slice per slice
+    of `optics` (more than one runs LSQML and the LLK alone, of one mode), `probe` complex
+    `[nprobe, ny, nx]`."""
 
     def __init__(self, scan, object, probe, device=0, optics=None, **options):
         opts = _overlay(_GPU_OPTIONS, options, "GpuOptions")
@@ -308,8 +334,14 @@ class GpuPtycho:
             raise ValueError("psi_f16 is not supported by this engine")
         lib = library()
         obj = np.ascontiguousarray(object, dtype=np.complex128)
-        if obj.ndim != 3:
-            raise TypeError("object must be complex128 [nmode, nyo, nxo]")
+        nslice = optics.nslice() if optics is not None else 1
+        if obj.ndim == 3:
+            obj = obj[None]
+        if obj.ndim != 4:
+            raise TypeError("object must be complex128 [nmode, nyo, nxo] or "
+                            "[nslice, nmode, nyo, nxo]")
+        if obj.shape[0] != nslice:
+            raise ValueError(f"{obj.shape[0]} object slices for {nslice} slices of the optics")
         pr = np.ascontiguousarray(probe, dtype=np.complex128)
         if pr.ndim != 3:
             raise TypeError("probe must be complex128 [nmode, ny, nx]")
@@ -332,11 +364,13 @@ class GpuPtycho:
         h = ctypes.c_void_p()
         _check(lib.ptyrs_engine_create(
             ctypes.byref(sc), ctypes.byref(oc) if oc is not None else None,
-            _ptr(obj.view(np.float64), ctypes.c_double), obj.shape[0], obj.shape[1], obj.shape[2],
+            _ptr(obj.view(np.float64), ctypes.c_double), obj.shape[1], obj.shape[2], obj.shape[3],
             _ptr(pr.view(np.float64), ctypes.c_double), pr.shape[0], ctypes.byref(oo), device,
             ctypes.byref(h)))
         self._h = h
-        self._obj_shape, self._probe_shape = obj.shape, pr.shape
+        self._objs_shape, self._probe_shape = obj.shape, pr.shape
+        self._obj_shape = obj.shape if nslice > 1 else obj.shape[1:]
+        self.nslice = nslice
         self.nframe = scan.counts.shape[0]
         self.dtype = opts["dtype"]
 
@@ -418,24 +452,27 @@ class GpuPtycho:
         return None
 
     def lsqml_steps(self):
-        """The last LSQML epoch's steps as dicts of `frames`, `a_far`, `a_obj` (`[1, frames]`)
-        and `a_probe`, one per batch (ptyrs-py/src/lib.rs:236-248)."""
+        """The last LSQML epoch's steps as dicts of `frames`, `a_far`, `a_obj`
+        (`[nslice, frames]`) and `a_probe`, one per batch (ptyrs-py/src/lib.rs:236-248)."""
         lib = library()
         n = ctypes.c_size_t()
         _check(lib.ptyrs_engine_lsqml_steps(self._h, None, 0, ctypes.byref(n)))
         recs = (_StepC * max(n.value, 1))()
         _check(lib.ptyrs_engine_lsqml_steps(self._h, recs, n.value, ctypes.byref(n)))
+        a_obj = np.empty((n.value, self.nslice))
+        _check(lib.ptyrs_engine_lsqml_slice_steps(
+            self._h, _ptr(a_obj, ctypes.c_double), a_obj.size))
         out = []
-        for r in recs[:n.value]:
+        for i, r in enumerate(recs[:n.value]):
             if not out or out[-1]["batch"] != r.batch:
                 out.append(dict(batch=r.batch, frames=[], a_far=[], a_obj=[], a_probe=[]))
             d = out[-1]
             d["frames"].append(r.frame)
             d["a_far"].append(r.a_far)
-            d["a_obj"].append(r.a_obj)
+            d["a_obj"].append(a_obj[i])
             d["a_probe"].append(r.a_probe)
         return [dict(frames=np.array(d["frames"]), a_far=np.array(d["a_far"]),
-                     a_obj=np.array(d["a_obj"])[None], a_probe=np.array(d["a_probe"]))
+                     a_obj=np.array(d["a_obj"]).T, a_probe=np.array(d["a_probe"]))
                 for d in out]
 
     def llk(self):
@@ -446,11 +483,17 @@ class GpuPtycho:
                     nb_photons_calc=r.nb_photons_calc)
 
     @property
-    def object(self):
-        out = np.empty(self._obj_shape, dtype=np.complex128)
+    def objects(self):
+        """The object's slices `[nslice, nobj, nyo, nxo]` in the order the wave crosses them."""
+        out = np.empty(self._objs_shape, dtype=np.complex128)
         _check(library().ptyrs_engine_object(self._h, _ptr(out.view(np.float64), ctypes.c_double),
                                              out.size * 2))
         return out
+
+    @property
+    def object(self):
+        """The object; of a multislice object its first slice."""
+        return self.objects[0]
 
     @property
     def probe(self):
@@ -460,6 +503,8 @@ class GpuPtycho:
         return out
 
     def set_object(self, obj):
+        """Replaces the object: `[nobj, nyo, nxo]`, of a multislice object every slice
+        `[nslice, nobj, nyo, nxo]`."""
         a = np.ascontiguousarray(obj, dtype=np.complex128)
         if a.shape != self._obj_shape:
             raise ValueError(f"object {a.shape} differs from {self._obj_shape}")

@@ -56,8 +56,22 @@ Prepared prepare(const ptyrs_scan& s, const ptyrs_optics* optics, size_t nprobe,
   } else {
     p.exit = free_space(N, N, optics->wavelength, optics->pixel_y, optics->pixel_x,
                         optics->distance);
-    if (p.exit.kind == PTYRS_ANGULAR_SPECTRUM)
-      throw Invalid("an angular-spectrum exit transform is not run on the GPU yet");
+  }
+  if (optics && optics->ngap) {
+    // Optics::with_slices: every gap an angular-spectrum propagator over the frame
+    // (ptyrs/src/engine/propagate.rs:84-98).
+    if (!optics->gaps) throw Invalid("ngap > 0 needs gaps");
+    p.nslice = optics->ngap + 1;
+    for (size_t g = 0; g < optics->ngap; ++g) {
+      const double* w = optics->gaps + 4 * g;
+      if (!(std::isfinite(w[0]) && w[0] > 0 && std::isfinite(w[1]) && w[1] > 0 &&
+            std::isfinite(w[2]) && w[2] > 0))
+        throw Invalid("gap wavelength and pixel must be finite and > 0");
+      if (!std::isfinite(w[3]) || w[3] == 0.0)
+        throw Invalid("gap distance must be finite and non-zero");
+      p.gaps.push_back(angular_spectrum_tf(N, N, w[0], w[1], w[2], w[3], nullptr));
+    }
+    if (nobj != 1) throw Invalid("a multislice object has one mode");
   }
   p.corner.resize(2 * s.nframe);
   if (s.corners) {
@@ -255,6 +269,15 @@ int ptyrs_engine_lsqml_steps(ptyrs_engine* h, ptyrs_lsqml_step* out, size_t cap,
       const StepRecord& r = e.lsqml_steps[i];
       out[i] = ptyrs_lsqml_step{size_t(r.batch), size_t(r.frame), r.a_far, r.a_obj, r.a_probe};
     }
+  });
+}
+
+int ptyrs_engine_lsqml_slice_steps(ptyrs_engine* h, double* out, size_t len) {
+  return guarded(h, false, [&](EngineBase& e) {
+    const size_t S = e.nslice();
+    if (!out || len != e.lsqml_steps.size() * S) throw Invalid("buffer has the wrong length");
+    for (size_t i = 0; i < e.lsqml_steps.size(); ++i)
+      for (size_t s = 0; s < S; ++s) out[i * S + s] = e.lsqml_steps[i].a_objs[s];
   });
 }
 

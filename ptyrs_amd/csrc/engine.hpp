@@ -64,6 +64,9 @@ struct Prepared {
   double total = 0;
   double max_frame_total = 0;  // max over frames of the unmasked counts' sum
   ExitModel exit;
+  // Multislice: object slices and, per gap, its angular-spectrum transfer function.
+  size_t nslice = 1;
+  std::vector<std::vector<double>> gaps;
 };
 
 // AP, DM and RAAR in one form (ptyrs/src/engine/ptycho.rs:206-356).
@@ -81,6 +84,7 @@ struct StepRecord {
   int batch;
   int frame;
   double a_far, a_obj, a_probe;
+  std::vector<double> a_objs;  // every slice's a_obj, slice 0's first
 };
 
 class EngineBase {
@@ -100,7 +104,8 @@ class EngineBase {
 
   void synchronize() { PTYRS_HIP(hipStreamSynchronize(stream_)); }
 
-  size_t object_len() const { return 2 * nobj_ * nyo_ * nxo_; }
+  size_t object_len() const { return 2 * nslice_ * nobj_ * nyo_ * nxo_; }
+  size_t nslice() const { return nslice_; }
   size_t probe_len() const { return 2 * nprobe_ * ny_ * nx_; }
 
   std::vector<Record> history;
@@ -125,6 +130,7 @@ class EngineBase {
   int device_ = 0;
   hipStream_t stream_ = nullptr;
   size_t nframe_ = 0, ny_ = 0, nx_ = 0, nprobe_ = 0, nobj_ = 0, nyo_ = 0, nxo_ = 0;
+  size_t nslice_ = 1;
   size_t nb_obs_ = 0;
 
   bool profiling_ = false;
@@ -193,8 +199,9 @@ class Engine : public EngineBase {
     max_frame_total_ = pre.max_frame_total;
     total_ = pre.total;
     M_ = int(nprobe * nobj);
+    nslice_ = pre.nslice;
 
-    O_.ensure(sizeof(cplx<T>) * nobj * nyo * nxo);
+    O_.ensure(sizeof(cplx<T>) * nslice_ * nobj * nyo * nxo);
     P_.ensure(sizeof(cplx<T>) * nprobe * n);
     Wt_.ensure(sizeof(cplx<T>) * nprobe * n);
     q_.ensure(sizeof(cplx<T>) * n);
@@ -204,9 +211,25 @@ class Engine : public EngineBase {
     iota_.ensure(sizeof(int) * std::max<size_t>(1, nframe_));
     hist_.ensure(sizeof(double) * 4 * kHistSlots);
 
-    upload_complex(O_, obj, nobj * nyo * nxo);
+    upload_complex(O_, obj, nslice_ * nobj * nyo * nxo);
+    if (nslice_ > 1) {
+      gaptf_.ensure(sizeof(cplx<T>) * (nslice_ - 1) * n);
+      std::vector<double> all;
+      for (auto& g : pre.gaps) all.insert(all.end(), g.begin(), g.end());
+      upload_complex(gaptf_, all.data(), (nslice_ - 1) * n);
+      std::vector<double> one(2 * n, 0.0);
+      for (int i = 0; i < n; ++i) one[2 * i] = 1.0;
+      ones_.ensure(sizeof(cplx<T>) * n);
+      upload_complex(ones_, one.data(), n);
+    }
     upload_complex(P_, probe, nprobe * n);
     upload_complex(q_, pre.exit.quad.data(), n);
+    if (pre.exit.kind == PTYRS_ANGULAR_SPECTRUM) {
+      angular_ = true;
+      banded_ = pre.exit.banded;
+      tf_.ensure(sizeof(cplx<T>) * n);
+      upload_complex(tf_, pre.exit.tf.data(), n);
+    }
     std::vector<double> tw(2 * N);
     for (int t = 0; t < N; ++t) {
       // exp(−2πi t/N) from the exact angle of the reduced fraction.
@@ -231,9 +254,13 @@ class Engine : public EngineBase {
     nty_ = int((nyo + TILE_Y - 1) / TILE_Y);
   }
 
-  void get_object(double* out) override { download_complex(O_, out, nobj_ * nyo_ * nxo_); }
+  void get_object(double* out) override {
+    download_complex(O_, out, nslice_ * nobj_ * nyo_ * nxo_);
+  }
   void get_probe(double* out) override { download_complex(P_, out, nprobe_ * n); }
-  void set_object(const double* in) override { upload_complex(O_, in, nobj_ * nyo_ * nxo_); }
+  void set_object(const double* in) override {
+    upload_complex(O_, in, nslice_ * nobj_ * nyo_ * nxo_);
+  }
   void set_probe(const double* in) override { upload_complex(P_, in, nprobe_ * n); }
 
   void llk(double out[4]) override {
@@ -248,6 +275,7 @@ class Engine : public EngineBase {
   }
 
   void rpie(const ptyrs_rpie_params& p, size_t nepoch) override {
+    one_unitary("rPIE");
     if (nobj_ != 1) throw Invalid("rPIE needs exactly one object mode");
     if (p.batch_size == 0) throw Invalid("rPIE batch_size must be >= 1");
     if (M_ > 4) throw Invalid("the GPU engine runs at most 4 modes");
@@ -330,7 +358,12 @@ class Engine : public EngineBase {
     for (size_t ic = 0; ic < nepoch; ++ic) {
       if (ic < 10 || ic % 10 == 0) precondition(ic == 0);
       const int* ord = order_.as<int>() + ic * nframe_;
-      for (auto& s : spans[ic]) lsqml_batch(p, ord + s.first, s.second, s.first);
+      for (auto& s : spans[ic]) {
+        if (nslice_ > 1)
+          lsqml_batch_ms(p, ord + s.first, s.second, s.first);
+        else
+          lsqml_batch(p, ord + s.first, s.second, s.first);
+      }
       const bool rec = calc_llk > 0 && (ic % calc_llk == 0 || ic + 1 == nepoch);
       if (rec) {
         if (due.size() == kHistSlots) drain(due, PTYRS_LSQML);
@@ -342,8 +375,9 @@ class Engine : public EngineBase {
     }
     drain(due, PTYRS_LSQML);
     // The last epoch's steps.
-    std::vector<double> lg(3 * nframe_);
-    PTYRS_HIP(hipMemcpyAsync(lg.data(), log_.p, sizeof(double) * 3 * nframe_,
+    const size_t ls = 2 + nslice_;
+    std::vector<double> lg(ls * nframe_);
+    PTYRS_HIP(hipMemcpyAsync(lg.data(), log_.p, sizeof(double) * ls * nframe_,
                              hipMemcpyDeviceToHost, stream_));
     synchronize();
     flush_timing();
@@ -352,9 +386,10 @@ class Engine : public EngineBase {
     int bi = 0;
     for (auto& s : spans[nepoch - 1]) {
       for (int k = 0; k < s.second; ++k) {
-        const int slot = s.first + k;
-        lsqml_steps.push_back(StepRecord{bi, last[slot], lg[3 * slot], lg[3 * slot + 1],
-                                         lg[3 * slot + 2]});
+        const size_t slot = size_t(s.first + k);
+        const double* r = lg.data() + ls * slot;
+        lsqml_steps.push_back(StepRecord{bi, last[slot], r[0], r[1], r[ls - 1],
+                                         std::vector<double>(r + 1, r + 1 + nslice_)});
       }
       ++bi;
     }
@@ -363,6 +398,7 @@ class Engine : public EngineBase {
   // PyNX AP, DM, RAAR (ptyrs/src/engine/ptycho.rs:206-356), stacks of the whole scan; DM and
   // RAAR carry their exit waves across the cycles of a call.
   void project(const ProjParams& p, size_t ncycle) override {
+    one_unitary(p.algorithm == PTYRS_AP ? "AP" : p.algorithm == PTYRS_DM ? "DM" : "RAAR");
     if (nobj_ != 1) throw Invalid("the GPU projection algorithms run one object mode");
     if (M_ > 4) throw Invalid("the GPU engine runs at most 4 modes");
     if (p.algorithm != PTYRS_AP && p.loops == 0)
@@ -459,6 +495,7 @@ class Engine : public EngineBase {
   // then per cycle the gradients, Polak–Ribière β, d = β·d − g, the Newton step γ along d and
   // O += γ·d_O, P += γ·d_P (or d = −g where γ is not finite).
   void ml(const ptyrs_ml_params& p, size_t ncycle) override {
+    one_unitary("ML");
     if (nobj_ != 1) throw Invalid("the GPU ML runs one object mode");
     if (M_ > 4) throw Invalid("the GPU engine runs at most 4 modes");
     if (nframe_ == 0) {
@@ -594,17 +631,19 @@ class Engine : public EngineBase {
                              psi3_.as<cplx<T>>(), tw_.as<cplx<T>>());
         });
         run(K_PASS_B, [&] {
-          auto go = [&](auto kern) {
-            hipLaunchKernelGGL(kern, dim3(SC::NBLK, nb), dim3(SC::NT), 0, stream_,
-                               psi3_.as<cplx<T>>(), field, counts_.as<float>(), ord,
+          if (angular_) pass_t<false>(psi3_.as<cplx<T>>(), nb * M_, 3, field);
+          const T norm = T(1) / (angular_ ? T(n) * T(n) : T(n));
+          auto go = [&](auto col, auto row) {
+            hipLaunchKernelGGL(angular_ ? row : col, dim3(SC::NBLK, nb), dim3(SC::NT), 0,
+                               stream_, psi3_.as<cplx<T>>(), field, counts_.as<float>(), ord,
                                gam_part_.as<double>() + size_t(f0) * SC::NBLK * 2,
-                               tw_.as<cplx<T>>());
+                               tw_.as<cplx<T>>(), norm);
           };
           switch (M_) {
-            case 1: go(k_pass_b_gamma<T, N, 1>); break;
-            case 2: go(k_pass_b_gamma<T, N, 2>); break;
-            case 3: go(k_pass_b_gamma<T, N, 3>); break;
-            default: go(k_pass_b_gamma<T, N, 4>); break;
+            case 1: go(k_pass_b_gamma<T, N, 1, false>, k_pass_b_gamma<T, N, 1, true>); break;
+            case 2: go(k_pass_b_gamma<T, N, 2, false>, k_pass_b_gamma<T, N, 2, true>); break;
+            case 3: go(k_pass_b_gamma<T, N, 3, false>, k_pass_b_gamma<T, N, 3, true>); break;
+            default: go(k_pass_b_gamma<T, N, 4, false>, k_pass_b_gamma<T, N, 4, true>); break;
           }
         });
       }
@@ -635,6 +674,90 @@ class Engine : public EngineBase {
   static constexpr size_t kDefaultStack = 128;
   int M_ = 1;
   double scale_ = 1.0, max_frame_total_ = 0.0, total_ = 0.0;
+  // Angular-spectrum exit: the transfer function T on the unshifted grid, and whether it drops
+  // an evanescent band (then its adjoint does not invert it).
+  bool angular_ = false, banded_ = false;
+  DevBuf tf_;
+
+  // The algorithms that take the exit adjoint for its inverse refuse a banded transfer
+  // function; LSQML and the LLK pass run it (ptyrs/src/engine/gpu/device.rs:3087-3105).
+  void unitary(const char* what) const {
+    if (banded_)
+      throw Invalid(std::string(what) +
+                    " needs a unitary propagator; this one drops an evanescent band: only "
+                    "LSQML and the LLK pass run it");
+  }
+  // ... and one object slice (ptyrs/src/engine/gpu/device.rs:3100-3105).
+  void one_unitary(const char* what) const {
+    if (nslice_ > 1)
+      throw Invalid(std::string(what) +
+                    " runs one object slice; only LSQML and the LLK pass run several");
+    unitary(what);
+  }
+
+  // Multislice state: the gaps' transfer functions, a plane of ones, the incident waves of
+  // slices 1.. of a stack ([nslice − 1][cap·M][n], natural layout).
+  DevBuf gaptf_, ones_, inc_;
+  cplx<T>* slice_obj(size_t s) { return O_.as<cplx<T>>() + s * nobj_ * nyo_ * nxo_; }
+  cplx<T>* slice_dir(size_t s) { return dO_.as<cplx<T>>() + s * nyo_ * nxo_; }
+  cplx<T>* incident(size_t s) { return inc_.as<cplx<T>>() + (s - 1) * cap_ * M_ * n; }
+
+  // An angular-spectrum gap's propagation of `nplane` planes in place, or its adjoint.
+  void gap(cplx<T>* planes, int nplane, size_t g, bool adjoint) {
+    const cplx<T>* tf = gaptf_.as<cplx<T>>() + g * n;
+    const T inv_n = T(1) / T(n);
+    hipLaunchKernelGGL((k_rows<T, N, false>), dim3(S::NBLK, nplane), dim3(S::NT), 0, stream_,
+                       planes, tw_.as<cplx<T>>());
+    if (adjoint)
+      hipLaunchKernelGGL((k_gap_cols<T, N, true>), dim3(SC::NBLK, nplane), dim3(SC::NT), 0,
+                         stream_, planes, tf, tw_.as<cplx<T>>(), inv_n);
+    else
+      hipLaunchKernelGGL((k_gap_cols<T, N, false>), dim3(SC::NBLK, nplane), dim3(SC::NT), 0,
+                         stream_, planes, tf, tw_.as<cplx<T>>(), inv_n);
+    hipLaunchKernelGGL((k_rows<T, N, true>), dim3(S::NBLK, nplane), dim3(S::NT), 0, stream_,
+                       planes, tw_.as<cplx<T>>());
+  }
+
+  template <int OP>
+  void window(cplx<T>* out, size_t s, const cplx<T>* w, int w_view, const cplx<T>* q,
+              const int* ord, int nb) {
+    hipLaunchKernelGGL((k_ms_window<T, OP>), dim3((n + 255) / 256, nb * M_), dim3(256), 0,
+                       stream_, out, slice_obj(s), int(nxo_), w, w_view, q, corner_.as<int2>(),
+                       ord, N, M_);
+  }
+
+  // The exit waves of a stack through the slices (ptyrs/src/engine/multislice.rs:86-171):
+  // the incident waves of slices 1.. kept, psi = q·O_last[win]·w_last, natural layout.
+  void ms_forward(const int* ord, int nb) {
+    const size_t S = nslice_;
+    run(K_PASS_A, [&] {
+      for (size_t s = 0; s + 1 < S; ++s) {
+        cplx<T>* dst = incident(s + 1);
+        if (s == 0)
+          window<MS_MUL>(dst, 0, P_.as<cplx<T>>(), 0, nullptr, ord, nb);
+        else
+          window<MS_MUL>(dst, s, incident(s), 1, nullptr, ord, nb);
+        gap(dst, nb * M_, s, false);
+      }
+      window<MS_MUL>(psi_.as<cplx<T>>(), S - 1, incident(S - 1), 1, q_.as<cplx<T>>(), ord, nb);
+    });
+  }
+
+  void rows(bool inverse, int nplane) {
+    if (inverse)
+      hipLaunchKernelGGL((k_rows<T, N, true>), dim3(S::NBLK, nplane), dim3(S::NT), 0, stream_,
+                         psi_.as<cplx<T>>(), tw_.as<cplx<T>>());
+    else
+      hipLaunchKernelGGL((k_rows<T, N, false>), dim3(S::NBLK, nplane), dim3(S::NT), 0, stream_,
+                         psi_.as<cplx<T>>(), tw_.as<cplx<T>>());
+  }
+
+  // The angular spectrum's column passes over `nplane` planes of `nfield` fields `field` apart.
+  template <bool INV>
+  void pass_t(cplx<T>* psi, int nplane, int nfield = 1, size_t field = 0) {
+    hipLaunchKernelGGL((k_pass_t<T, N, INV>), dim3(SC::NBLK, nplane, nfield), dim3(SC::NT), 0,
+                       stream_, psi, field, tf_.as<cplx<T>>(), tw_.as<cplx<T>>());
+  }
   DevBuf psi3_, gO_, hO_, dOc_, gP_, hP_, dPc_, dWt_, cg_part_, cg_sums_, gam_part_, gam_;
   DevBuf carried_, iter_, list_all_, count_all_, pnum_part_, pden_part_, pnum_, pden_,
       pmaxpart_, pmax2_, abs_part_;
@@ -663,7 +786,7 @@ class Engine : public EngineBase {
 
   void ensure_lsqml(size_t cap) {
     const size_t no = nyo_ * nxo_;
-    dO_.ensure(sizeof(cplx<T>) * no);
+    dO_.ensure(sizeof(cplx<T>) * nslice_ * no);
     pre_.ensure(sizeof(T) * no);
     illum_.ensure(sizeof(T) * no);
     afar_all_.ensure(sizeof(T) * std::max<size_t>(1, nframe_));
@@ -674,8 +797,8 @@ class Engine : public EngineBase {
     xi_part2_.ensure(sizeof(double) * 2 * cap * aux_blocks());
     dots_part_.ensure(sizeof(double) * 6 * cap * aux_blocks());
     sums_.ensure(sizeof(double) * 6 * cap);
-    log_.ensure(sizeof(double) * 3 * std::max<size_t>(1, nframe_));
-    step_.ensure(sizeof(double) * 2);
+    log_.ensure(sizeof(double) * (2 + nslice_) * std::max<size_t>(1, nframe_));
+    step_.ensure(sizeof(double) * (2 + nslice_));
     pre_part_.ensure(sizeof(T) * kPreBlocks);
     floor_.ensure(sizeof(T) * 1);
     dPm_.ensure(sizeof(cplx<T>) * nprobe_ * n);
@@ -755,7 +878,7 @@ class Engine : public EngineBase {
     run(K_OTHER, [&] {
       hipLaunchKernelGGL(k_lsqml_log_afar<T>, dim3((nb + 255) / 256), dim3(256), 0, stream_,
                          afar_tmp_.as<T>(), ord, nb, afar_all_.as<T>(),
-                         log_.as<double>() + 3 * size_t(off), int(gauss));
+                         log_.as<double>() + 3 * size_t(off), int(gauss), 3);
     });
     const int chunk = chunk_of(nb);
     const int nchunk = (nb + chunk - 1) / chunk;
@@ -787,7 +910,7 @@ class Engine : public EngineBase {
                          corner_.as<int2>(), ord, dots_part_.as<double>());
       hipLaunchKernelGGL(k_lsqml_step<T>, dim3(1), dim3(1024), 0, stream_,
                          dots_part_.as<double>(), xb, nb, lam, M_, sums_.as<double>(),
-                         log_.as<double>() + 3 * size_t(off), step_.as<double>());
+                         log_.as<double>() + 3 * size_t(off), step_.as<double>(), 3);
     });
     const size_t no = nyo_ * nxo_;
     run(K_OBJ_UPDATE, [&] {
@@ -798,6 +921,121 @@ class Engine : public EngineBase {
       hipLaunchKernelGGL(k_lsqml_apply_probe<T>, dim3((total + 255) / 256), dim3(256), 0,
                          stream_, P_.as<cplx<T>>(), Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
                          dPm_.as<cplx<T>>(), step_.as<double>(), n, total);
+    });
+  }
+
+  // One multislice LSQML batch (ptyrs/src/engine/lsqml.rs:563-643 over
+  // ptyrs/src/engine/multislice.rs): the exit waves through the slices, the detector operator
+  // as for one slice, χ back through the slices with each slice's object direction and step
+  // (Eq. 23b) on the way, slice 0 jointly with the probe (Eq. 22), then every update from the
+  // state on entry. One object mode.
+  void lsqml_batch_ms(const ptyrs_lsqml_params& p, const int* ord, int nb, int off) {
+    const int nt = ntiles();
+    const bool gauss = p.cost == PTYRS_AMPLITUDE;
+    const size_t S = nslice_;
+    const int ls = int(2 + S);
+    const int K = nb * M_;
+    run(K_BIN, [&] {
+      hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                         corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
+                         lcount_.as<int>(), (const cplx<T>*)nullptr, M_, int(nyo_), int(nxo_),
+                         (T*)nullptr, tilemax_.as<T>());
+    });
+    ms_forward(ord, nb);
+    run(K_PASS_A, [&] { rows(false, K); });
+    BArgs<T> ba{afar_tmp_.as<T>(), prev_.as<T>(), xi_part_.as<double>(), icalc_.as<T>()};
+    const int xb = aux_blocks();
+    if (gauss) {
+      pass_b<B_GAUSS>(ord, nb, ba);
+    } else {
+      run(K_OTHER, [&] {
+        hipLaunchKernelGGL(k_lsqml_prev<T>, dim3(1), dim3(THREADS), 0, stream_,
+                           afar_all_.as<T>(), ord, nb, prev_.as<T>());
+      });
+      pass_b<B_ICALC>(ord, nb, ba);
+      run(K_OTHER, [&] {
+        hipLaunchKernelGGL(k_lsqml_xi<T>, dim3(1), dim3(THREADS), 0, stream_,
+                           xi_part_.as<double>(), int(SC::NBLK), nb, prev_.as<T>(),
+                           afar_tmp_.as<T>(), prev_.as<T>() + 1);
+        hipLaunchKernelGGL(k_lsqml_xi_sums<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
+                           icalc_.as<T>(), counts_.as<float>(), ord, n, prev_.as<T>() + 1,
+                           xi_part2_.as<double>());
+        hipLaunchKernelGGL(k_lsqml_xi<T>, dim3(1), dim3(THREADS), 0, stream_,
+                           xi_part2_.as<double>(), xb, nb, prev_.as<T>() + 1, afar_tmp_.as<T>(),
+                           (T*)nullptr);
+      });
+      pass_b<B_POISSON>(ord, nb, ba);
+    }
+    double* log = log_.as<double>() + size_t(ls) * off;
+    double* step = step_.as<double>();
+    run(K_OTHER, [&] {
+      hipLaunchKernelGGL(k_lsqml_log_afar<T>, dim3((nb + 255) / 256), dim3(256), 0, stream_,
+                         afar_tmp_.as<T>(), ord, nb, afar_all_.as<T>(), log, int(gauss), ls);
+    });
+    cplx<T>* chi = psi_.as<cplx<T>>();
+    run(K_PASS_C, [&] {
+      rows(true, K);
+      window<MS_SUB>(chi, S - 1, incident(S - 1), 1, q_.as<cplx<T>>(), ord, nb);
+    });
+    for (size_t s = S - 1; s >= 1; --s) {
+      const cplx<T>* qs = s + 1 == S ? q_.as<cplx<T>>() : nullptr;
+      run(K_OBJ_UPDATE, [&] {
+        hipLaunchKernelGGL(k_lsqml_obj_dir<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                           slice_dir(s), (const cplx<T>*)incident(s), (const cplx<T>*)chi, N, M_,
+                           corner_.as<int2>(), ord, list_.as<int>(), lcount_.as<int>(),
+                           int(cap_), int(nyo_), int(nxo_), ntx_, pre_.as<T>(), floor_.as<T>(),
+                           1, qs);
+      });
+      run(K_OTHER, [&] {
+        hipLaunchKernelGGL(k_ms_dots<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
+                           (const cplx<T>*)slice_dir(s), int(nxo_), (const cplx<T>*)incident(s),
+                           qs, (const cplx<T>*)chi, N, M_, corner_.as<int2>(), ord,
+                           xi_part2_.as<double>());
+        hipLaunchKernelGGL(k_ms_slice_step, dim3(1), dim3(1024), 0, stream_,
+                           xi_part2_.as<double>(), xb, nb, int(S) * M_, log, ls, int(1 + s),
+                           step + 1 + s);
+      });
+      run(K_PASS_C, [&] {
+        window<MS_ADJ>(chi, s, nullptr, 0, qs, ord, nb);
+        gap(chi, K, s - 1, true);
+      });
+    }
+    const int total = int(nprobe_ * n);
+    const double lam = 1.2e-7 * scale_ * double(M_);
+    run(K_OBJ_UPDATE, [&] {
+      hipLaunchKernelGGL(k_lsqml_obj_dir<T>, dim3(nt), dim3(THREADS), 0, stream_, slice_dir(0),
+                         (const cplx<T>*)P_.as<cplx<T>>(), (const cplx<T>*)chi, N, M_,
+                         corner_.as<int2>(), ord, list_.as<int>(), lcount_.as<int>(), int(cap_),
+                         int(nyo_), int(nxo_), ntx_, pre_.as<T>(), floor_.as<T>(), 0,
+                         (const cplx<T>*)nullptr);
+    });
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_ms_probe_dir<T>, dim3((n + 255) / 256, M_), dim3(256), 0, stream_,
+                         dPm_.as<cplx<T>>(), (const cplx<T>*)slice_obj(0), int(nxo_),
+                         (const cplx<T>*)nullptr, (const cplx<T>*)chi, corner_.as<int2>(), ord,
+                         nb, N, M_);
+    });
+    run(K_OTHER, [&] {
+      hipLaunchKernelGGL(k_lsqml_dots<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
+                         (const cplx<T>*)slice_dir(0), (const cplx<T>*)slice_obj(0), int(nxo_),
+                         (const cplx<T>*)P_.as<cplx<T>>(), (const cplx<T>*)dPm_.as<cplx<T>>(),
+                         (const cplx<T>*)ones_.as<cplx<T>>(), (const cplx<T>*)chi, N, M_,
+                         corner_.as<int2>(), ord, dots_part_.as<double>());
+      hipLaunchKernelGGL(k_lsqml_step<T>, dim3(1), dim3(1024), 0, stream_,
+                         dots_part_.as<double>(), xb, nb, lam, int(S) * M_, sums_.as<double>(),
+                         log, step, ls);
+    });
+    const size_t no = nyo_ * nxo_;
+    run(K_OBJ_UPDATE, [&] {
+      for (size_t s = 0; s < S; ++s)
+        hipLaunchKernelGGL(k_lsqml_apply_obj<T>, dim3((no + 255) / 256), dim3(256), 0, stream_,
+                           slice_obj(s), (const cplx<T>*)slice_dir(s),
+                           (const double*)(s == 0 ? step : step + 1 + s), no);
+    });
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_lsqml_apply_probe<T>, dim3((total + 255) / 256), dim3(256), 0,
+                         stream_, P_.as<cplx<T>>(), Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
+                         dPm_.as<cplx<T>>(), (const double*)step, n, total);
     });
   }
 
@@ -919,6 +1157,7 @@ class Engine : public EngineBase {
     order_.ensure(sizeof(int) * std::max<size_t>(1, nepoch * nframe_));
     llk_part_.ensure(sizeof(double) * 4 * nframe_ * SC::NBLK);
     cap_ = std::max(cap_, cap);
+    if (nslice_ > 1) inc_.ensure(sizeof(cplx<T>) * (nslice_ - 1) * cap_ * M_ * n);
   }
 
   void probe_terms() {
@@ -949,19 +1188,30 @@ class Engine : public EngineBase {
 
   template <int MODE>
   void pass_b(const int* ord, int nb, BArgs<T> ba = BArgs<T>{nullptr, nullptr, nullptr, nullptr}) {
+    // Far field and Fresnel: the column DFTs around the detector operator, 1/n for the round
+    // trip. Angular spectrum: columns, T, columns; then the row DFTs around the detector
+    // operator, 1/n² for two round trips; then the inverse column passes with conj T
+    // (nothing to carry back for B_LLK and B_ICALC). B_POISSON runs on the stack B_ICALC left
+    // behind, whose forward column passes are done.
+    constexpr bool back = MODE != B_LLK && MODE != B_ICALC;
+    constexpr bool fwd = MODE != B_POISSON;
     run(MODE == B_LLK ? K_LLK : K_PASS_B, [&] {
+      if (angular_ && fwd) pass_t<false>(psi_.as<cplx<T>>(), nb * M_);
       dim3 g(SC::NBLK, nb), b(SC::NT);
-      auto args = [&](auto kern) {
-        hipLaunchKernelGGL(kern, g, b, 0, stream_, psi_.as<cplx<T>>(), counts_.as<float>(), ord,
-                           llk_part_.as<double>(), tw_.as<cplx<T>>(), ba);
+      const T norm = T(1) / (angular_ ? T(n) * T(n) : T(n));
+      auto args = [&](auto col, auto row) {
+        hipLaunchKernelGGL(angular_ ? row : col, g, b, 0, stream_, psi_.as<cplx<T>>(),
+                           counts_.as<float>(), ord, llk_part_.as<double>(), tw_.as<cplx<T>>(),
+                           ba, norm);
       };
       switch (M_) {
-        case 1: args(k_pass_b<T, N, 1, MODE>); break;
-        case 2: args(k_pass_b<T, N, 2, MODE>); break;
-        case 3: args(k_pass_b<T, N, 3, MODE>); break;
-        case 4: args(k_pass_b<T, N, 4, MODE>); break;
+        case 1: args(k_pass_b<T, N, 1, MODE, false>, k_pass_b<T, N, 1, MODE, true>); break;
+        case 2: args(k_pass_b<T, N, 2, MODE, false>, k_pass_b<T, N, 2, MODE, true>); break;
+        case 3: args(k_pass_b<T, N, 3, MODE, false>, k_pass_b<T, N, 3, MODE, true>); break;
+        case 4: args(k_pass_b<T, N, 4, MODE, false>, k_pass_b<T, N, 4, MODE, true>); break;
         default: throw Invalid("the GPU engine runs at most 4 modes");
       }
+      if (angular_ && back) pass_t<true>(psi_.as<cplx<T>>(), nb * M_);
     });
   }
 
@@ -971,7 +1221,12 @@ class Engine : public EngineBase {
     for (int f0 = 0; f0 < int(nframe_); f0 += step) {
       const int nb = std::min(step, int(nframe_) - f0);
       const int* ord = iota_.as<int>() + f0;
-      pass_a(ord, nb, false);
+      if (nslice_ > 1) {
+        ms_forward(ord, nb);
+        run(K_PASS_A, [&] { rows(false, nb * M_); });
+      } else {
+        pass_a(ord, nb, false);
+      }
       pass_b<B_LLK>(ord, nb);
     }
     run(K_LLK, [&] {

@@ -107,6 +107,10 @@ struct ExitModel {
   // pty-chi's intensity scale k of the propagator (ptyrs/src/engine/propagate.rs:373-385):
   // the engine's probe is pty-chi's times √k.
   double scale = 1.0;
+  // Angular spectrum: the transfer function on the unshifted fftfreq grid (interleaved re, im)
+  // and whether it has an evanescent band (zeros), where its adjoint does not invert it.
+  std::vector<double> tf;
+  bool banded = false;
 };
 
 inline void set_phase(std::vector<double>& q, size_t i, double ph) {
@@ -139,6 +143,36 @@ inline ExitModel far_field(int ny, int nx, double wavelength, double distance, d
 // output pitch is finer than the source pitch, else angular spectrum. Under the Fresnel
 // transform the plane phase is b and the unit-modulus a drops out of every detector
 // intensity and projection.
+// The angular-spectrum transfer function T = exp(2πi z √(1 − r)) for
+// r = (fx² + (a fy)²)/px² < 1, else 0 (the evanescent band, which sets *banded), lengths in
+// wavelengths, on the unshifted fftfreq grid, interleaved re, im.
+// ptyrs/src/engine/propagate.rs:505-531
+inline std::vector<double> angular_spectrum_tf(int ny, int nx, double wavelength, double py,
+                                               double px, double distance, bool* banded) {
+  const double w = px / wavelength, ar = px / py, d = distance / wavelength;
+  std::vector<double> tf(2 * size_t(ny) * nx, 0.0);
+  const double i2piz = 2.0 * M_PI * d;
+  auto freq = [](int k, int len) {
+    const double kk = k < (len + 1) / 2 ? double(k) : double(k) - double(len);
+    return kk * (1.0 / double(len));
+  };
+  bool band = false;
+  for (int y = 0; y < ny; ++y) {
+    const double fy = freq(y, ny);
+    for (int x = 0; x < nx; ++x) {
+      const double fx = freq(x, nx);
+      const double ratio = (fx * fx + (ar * fy) * (ar * fy)) / (w * w);
+      if (ratio < 1.0) {
+        set_phase(tf, size_t(y) * nx + x, i2piz * std::sqrt(1.0 - ratio));
+      } else {
+        band = true;
+      }
+    }
+  }
+  if (banded) *banded = band;
+  return tf;
+}
+
 inline ExitModel free_space(int ny, int nx, double wavelength, double py, double px,
                             double distance) {
   if (ny <= 0 || nx <= 0 || ny % 2 || nx % 2) throw Invalid("plane shape must be even and non-zero");
@@ -161,7 +195,12 @@ inline ExitModel free_space(int ny, int nx, double wavelength, double py, double
   const double h = w / ar;
   const bool fresnel = w * w / std::fabs(d) < 1.0 / nx || h * h / std::fabs(d) < 1.0 / ny;
   if (!fresnel) {
+    // The detector sees IDFT(T·DFT(ψ)), which at index k is DFT(T·DFT(ψ)) at −k over n: the
+    // counts are stored index-reversed.
     m.kind = PTYRS_ANGULAR_SPECTRUM;
+    m.reverse = true;
+    m.scale = 1.0;
+    m.tf = angular_spectrum_tf(ny, nx, wavelength, py, px, distance, &m.banded);
     return m;
   }
   m.kind = PTYRS_FRESNEL;

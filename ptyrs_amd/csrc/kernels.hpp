@@ -205,36 +205,42 @@ struct BArgs {
 // 2(1 − √iobs/(√icalc + 1e-9)) with a_far = 1/2, Poisson 1 − iobs/(icalc + 1e-6)), 0 where
 // masked (ptyrs/src/engine/lsqml.rs:97-170). B_ICALC stores icalc and the first pass of the
 // Poisson a_far solver's sums and stores no field.
-template <typename T, int N, int M, int MODE>
+// ROW = false: lines are columns (the far field's column pass); ROW = true: lines are rows (the
+// angular spectrum's second row pass). `norm` is 1/n (one unnormalised DFT round trip) or 1/n²
+// (two).
+template <typename T, int N, int M, int MODE, bool ROW>
 __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
     cplx<T>* __restrict__ psi, const float* __restrict__ counts, const int* __restrict__ order,
-    double* __restrict__ llk_part, const cplx<T>* __restrict__ tw, BArgs<T> ba) {
+    double* __restrict__ llk_part, const cplx<T>* __restrict__ tw, BArgs<T> ba, T norm) {
   using S = ColShape<T, N>;
   constexpr int E = S::E, TL = S::TL, CG = S::L;
   constexpr int n = N * N;
   __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
   __shared__ cplx<T> stw[N];
-  const int tid = threadIdx.x, col = tid % CG, jl = tid / CG;
-  const int x = blockIdx.x * CG + col;
+  // Columns: lanes run along the 128-byte stripe of lines; rows: along each line.
+  const int tid = threadIdx.x;
+  const int line = ROW ? tid / TL : tid % CG, jl = ROW ? tid % TL : tid / CG;
+  const int x = blockIdx.x * CG + line;
   const int b = blockIdx.y;
   const int f = order[b];
-  T* re = lre + col * S::LD;
-  T* im = lim + col * S::LD;
+  T* re = lre + line * S::LD;
+  T* im = lim + line * S::LD;
+  auto at = [&](int e) { return ROW ? x * N + jl + TL * e : (jl + TL * e) * N + x; };
   const float* cnt = counts + (size_t)f * n;
   float cv[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) cv[e] = cnt[(jl + TL * e) * N + x];
+  for (int e = 0; e < E; ++e) cv[e] = cnt[at(e)];
   cplx<T> v[M][E];
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const cplx<T>* plane = psi + (size_t)(b * M + m) * n;
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[m][e] = ld(plane + (jl + TL * e) * N + x);
+    for (int e = 0; e < E; ++e) v[m][e] = ld(plane + at(e));
   }
   stage_twiddles<T, N, S::NT>(stw, tw);
 #pragma unroll
   for (int m = 0; m < M; ++m) line_fft<T, N, E, false>(v[m], jl, re, im, stw);
-  const T inv_n = T(1) / T(n);
+  const T inv_n = norm;
   double acc[4] = {0, 0, 0, 0};
   T afar = 0;
   double prev = 0;
@@ -282,7 +288,7 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
       for (int m = 0; m < M; ++m) v[m][e] = v[m][e] * r;
     } else if constexpr (MODE == B_ICALC) {
       const T ic = I * inv_n;
-      ba.icalc[(size_t)b * n + (jl + TL * e) * N + x] = ic;
+      ba.icalc[(size_t)b * n + at(e)] = ic;
       if (io >= 0.f) {
         const double yp = double(ic), yt = double(io);
         const double xi = 1.0 - yt / (yp + 1e-5);
@@ -291,7 +297,7 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
       }
     } else {
       if (io >= 0.f) {
-        const double ic = double(I) * (1.0 / double(n));
+        const double ic = double(I) * double(norm);
         const double ob = double(io);
         acc[0] += ob > 0.0 ? ic - ob + ob * log(ob / ic) : ic;
         acc[1] += (ob - ic) * (ob - ic) / (ob + 1.0);
@@ -321,9 +327,42 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
       line_fft<T, N, E, true>(v[m], jl, re, im, stw);
       cplx<T>* plane = psi + (size_t)(b * M + m) * n;
 #pragma unroll
-      for (int e = 0; e < E; ++e) st(plane + (jl + TL * e) * N + x, v[m][e]);
+      for (int e = 0; e < E; ++e) st(plane + at(e), v[m][e]);
     }
   }
+}
+
+// ---- the angular spectrum's column passes -------------------------------------------------------------
+// Per plane and stripe of columns: DFT along columns, times T, DFT again (INV: the inverse DFTs
+// and conj T), unnormalised. Grid (NBLK, nplanes, nfield), planes of field z at psi + z·field;
+// linear, one stripe of one plane per workgroup. ptyrs/src/engine/propagate.rs:380-428
+template <typename T, int N, bool INV>
+__global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_t(cplx<T>* __restrict__ psi,
+                                                                  size_t field,
+                                                                  const cplx<T>* __restrict__ tf,
+                                                                  const cplx<T>* __restrict__ tw) {
+  using S = ColShape<T, N>;
+  constexpr int E = S::E, TL = S::TL, CG = S::L;
+  __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
+  __shared__ cplx<T> stw[N];
+  const int tid = threadIdx.x, col = tid % CG, jl = tid / CG;
+  const int x = blockIdx.x * CG + col;
+  cplx<T>* plane = psi + blockIdx.z * field + (size_t)blockIdx.y * N * N;
+  cplx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = ld(plane + (jl + TL * e) * N + x);
+  stage_twiddles<T, N, S::NT>(stw, tw);
+  T* re = lre + col * S::LD;
+  T* im = lim + col * S::LD;
+  line_fft<T, N, E, INV>(v, jl, re, im, stw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const cplx<T> t = ld(tf + (jl + TL * e) * N + x);
+    v[e] = INV ? v[e] * conj(t) : v[e] * t;
+  }
+  line_fft<T, N, E, INV>(v, jl, re, im, stw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) st(plane + (jl + TL * e) * N + x, v[e]);
 }
 
 // ---- pass C: inverse rows, Δ = ψ' − W·O, probe direction ----------------------------------------
@@ -679,14 +718,18 @@ __global__ __launch_bounds__(THREADS) void k_lsqml_xi_sums(
 
 // LSQML object direction (and, with pre = null, the object back-projection numerator of ML's
 // gradient): every object pixel owned by one thread sums, in slot order,
-// Σ_m Σ_views conj(W_m)·χ_m (Kahan-compensated) and divides by √(pre² + floor); 0 where no view
-// covers it. ptyrs/src/engine/lsqml.rs:183-188, :587-598; ptyrs/src/engine/ops.rs:279-310
+// Σ_m Σ_views conj(W·q)·χ_m (Kahan-compensated) and divides by √(pre² + floor); 0 where no view
+// covers it. W is W_m ([M][n]) or, with per_view, the slot's own plane ([slot·M + m][n], a
+// multislice slice's incident waves); q (nullable) a plane phase.
+// ptyrs/src/engine/lsqml.rs:183-188, :587-598; ptyrs/src/engine/ops.rs:279-310;
+// ptyrs/src/engine/multislice.rs:402-430
 template <typename T>
 __global__ __launch_bounds__(THREADS) void k_lsqml_obj_dir(
     cplx<T>* __restrict__ dO, const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ chi,
     int N, int M, const int2* __restrict__ corner, const int* __restrict__ order,
     const int* __restrict__ list, const int* __restrict__ count, int cap, int nyo, int nxo,
-    int ntx, const T* __restrict__ pre, const T* __restrict__ floor) {
+    int ntx, const T* __restrict__ pre, const T* __restrict__ floor, int per_view = 0,
+    const cplx<T>* __restrict__ q = nullptr) {
   constexpr int RS = THREADS / TILE_X;
   __shared__ int3 cs[THREADS];
   const int t = blockIdx.x, tid = threadIdx.x;
@@ -707,13 +750,16 @@ __global__ __launch_bounds__(THREADS) void k_lsqml_obj_dir(
       const int x = X - cs[i].z;
       if (x < 0 || x >= N) continue;
       const cplx<T>* cv = chi + (size_t)cs[i].x * M * n;
+      const cplx<T>* wv = per_view ? Wt + (size_t)cs[i].x * M * n : Wt;
 #pragma unroll
       for (int r = 0; r < TROWS; ++r) {
         const int y = Y0 + RS * r - cs[i].y;
         if (y < 0 || y >= N) continue;
         const int j = y * N + x;
         for (int m = 0; m < M; ++m) {
-          const cplx<T> g = cmulc(ld(Wt + (size_t)m * n + j), ld(cv + (size_t)m * n + j));
+          cplx<T> w = ld(wv + (size_t)m * n + j);
+          if (q) w = w * ld(q + j);
+          const cplx<T> g = cmulc(w, ld(cv + (size_t)m * n + j));
           sr[r].add(g.x);
           si[r].add(g.y);
         }
@@ -791,16 +837,16 @@ __global__ __launch_bounds__(THREADS) void k_lsqml_dots(
 }
 
 // One workgroup: per slot the 2×2 step-size system in FP64 (ptyrs/src/engine/lsqml.rs:319-335),
-// a_obj = 0.9·max(x0, 0)/M and a_probe = 0.9·max(x1, 0); then the object step, the trimmed mean
-// of a_obj (MATLAB trimmean 10 %, ranks with ties broken by slot), and the probe step, the mean
-// of a_probe, into step[0], step[1]. Per slot a_obj, a_probe go to log[b][1..2] (log[b][0] is
-// a_far, written before).
+// a_obj = 0.9·max(x0, 0)/div (div = slices × modes) and a_probe = 0.9·max(x1, 0); then the
+// object step, the trimmed mean of a_obj (MATLAB trimmean 10 %, ranks with ties broken by slot),
+// and the probe step, the mean of a_probe, into step[0], step[1]. Per slot, log[b·ls + 1] gets
+// a_obj and log[b·ls + ls − 1] a_probe (log[b·ls] is a_far, written before).
 template <typename T>
 __global__ __launch_bounds__(1024) void k_lsqml_step(const double* __restrict__ part, int nblk,
-                                                     int nb, double lam, int M,
+                                                     int nb, double lam, int div,
                                                      double* __restrict__ sums,
                                                      double* __restrict__ log,
-                                                     double* __restrict__ step) {
+                                                     double* __restrict__ step, int ls = 3) {
   __shared__ double red[1024];
   const int tid = threadIdx.x;
   double m0 = 0, m1 = 0;
@@ -823,10 +869,10 @@ __global__ __launch_bounds__(1024) void k_lsqml_step(const double* __restrict__ 
     const double det = a11 * a22 - (s[2] * s[2] + s[3] * s[3]);
     const double x0 = (a22 * s[4] - s[2] * s[5]) / det;
     const double x1 = (a11 * s[5] - s[2] * s[4]) / det;
-    const double ao = 0.9 * (x0 > 0.0 ? x0 : 0.0) / double(M);
+    const double ao = 0.9 * (x0 > 0.0 ? x0 : 0.0) / double(div);
     const double ap = 0.9 * (x1 > 0.0 ? x1 : 0.0);
-    log[(size_t)b * 3 + 1] = ao;
-    log[(size_t)b * 3 + 2] = ap;
+    log[(size_t)b * ls + 1] = ao;
+    log[(size_t)b * ls + ls - 1] = ap;
     pmean += ap;
   }
   const double psum = block_reduce<1024>(pmean, red, AddOp());
@@ -835,10 +881,10 @@ __global__ __launch_bounds__(1024) void k_lsqml_step(const double* __restrict__ 
   const int kcut = int(ceil(double(nb) * 0.1 / 2.0 - 0.5));
   double kept = 0;
   for (int b = tid; b < nb; b += 1024) {
-    const double v = log[(size_t)b * 3 + 1];
+    const double v = log[(size_t)b * ls + 1];
     int rank = 0;
     for (int o = 0; o < nb; ++o) {
-      const double w = log[(size_t)o * 3 + 1];
+      const double w = log[(size_t)o * ls + 1];
       rank += (w < v) || (w == v && o < b);
     }
     if (rank >= kcut && rank < nb - kcut) kept += v;
@@ -943,12 +989,180 @@ __global__ __launch_bounds__(THREADS) void k_pre_floor(const T* __restrict__ par
 // The slot's a_far into the step log and the per-frame store.
 template <typename T>
 __global__ void k_lsqml_log_afar(const T* __restrict__ a, const int* __restrict__ order, int nb,
-                                 T* __restrict__ a_far_all, double* __restrict__ log, int gauss) {
+                                 T* __restrict__ a_far_all, double* __restrict__ log, int gauss,
+                                 int ls = 3) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   const T v = gauss ? T(0.5) : a[b];
   a_far_all[order[b]] = v;
-  log[(size_t)b * 3] = double(v);
+  log[(size_t)b * ls] = double(v);
+}
+
+// ---- multislice (ptyrs/src/engine/multislice.rs) ----------------------------------------------------
+// Row DFTs of `nplane` planes in place (INV: inverse), unnormalised. Grid (NBLK, nplane).
+template <typename T, int N, bool INV>
+__global__ __launch_bounds__((RowShape<T, N>::NT)) void k_rows(cplx<T>* __restrict__ psi,
+                                                                const cplx<T>* __restrict__ tw) {
+  using S = RowShape<T, N>;
+  constexpr int E = S::E, TL = S::TL;
+  __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
+  __shared__ cplx<T> stw[N];
+  const int tid = threadIdx.x, line = tid / TL, jl = tid % TL;
+  const int y = blockIdx.x * S::L + line;
+  cplx<T>* row = psi + (size_t)blockIdx.y * N * N + (size_t)y * N;
+  cplx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = ld(row + jl + TL * e);
+  stage_twiddles<T, N, S::NT>(stw, tw);
+  line_fft<T, N, E, INV>(v, jl, lre + line * S::LD, lim + line * S::LD, stw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) st(row + jl + TL * e, v[e]);
+}
+
+// An angular-spectrum gap's column passes over row-transformed planes: forward column DFT,
+// times T (CONJ: conj T, the adjoint), inverse column DFT, times `scale` (1/n for the unitary
+// round trip). Grid (NBLK, nplane). ptyrs/src/engine/propagate.rs:380-428
+template <typename T, int N, bool CONJ>
+__global__ __launch_bounds__((ColShape<T, N>::NT)) void k_gap_cols(cplx<T>* __restrict__ psi,
+                                                                    const cplx<T>* __restrict__ tf,
+                                                                    const cplx<T>* __restrict__ tw,
+                                                                    T scale) {
+  using S = ColShape<T, N>;
+  constexpr int E = S::E, TL = S::TL, CG = S::L;
+  __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
+  __shared__ cplx<T> stw[N];
+  const int tid = threadIdx.x, col = tid % CG, jl = tid / CG;
+  const int x = blockIdx.x * CG + col;
+  cplx<T>* plane = psi + (size_t)blockIdx.y * N * N;
+  cplx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = ld(plane + (jl + TL * e) * N + x);
+  stage_twiddles<T, N, S::NT>(stw, tw);
+  T* re = lre + col * S::LD;
+  T* im = lim + col * S::LD;
+  line_fft<T, N, E, false>(v, jl, re, im, stw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const cplx<T> t = ld(tf + (jl + TL * e) * N + x);
+    v[e] = (CONJ ? v[e] * conj(t) : v[e] * t) * scale;
+  }
+  line_fft<T, N, E, true>(v, jl, re, im, stw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) st(plane + (jl + TL * e) * N + x, v[e]);
+}
+
+// A slice's window product over the views of a stack (natural layout, one plane per slot and
+// mode): MS_MUL out = O_s[win]·w·q, MS_SUB out −= O_s[win]·w·q, MS_ADJ out = conj(O_s[win]·q)·out.
+// w is P_m (w_view = 0) or the slot's own plane (w_view = 1); q is nullable (1).
+// Grid (ceil(n/256), nb·M). ptyrs/src/engine/multislice.rs:173-185, :266-291
+enum MsOp { MS_MUL = 0, MS_SUB, MS_ADJ };
+template <typename T, int OP>
+__global__ void k_ms_window(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ O, int nxo,
+                            const cplx<T>* __restrict__ w, int w_view,
+                            const cplx<T>* __restrict__ q, const int2* __restrict__ corner,
+                            const int* __restrict__ order, int N, int M) {
+  const int n = N * N;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int k = blockIdx.y, b = k / M, m = k % M;
+  const int2 c = corner[order[b]];
+  const int y = j / N, x = j % N;
+  cplx<T> o = ld(O + (size_t)(c.x + y) * nxo + c.y + x);
+  if (q) o = o * ld(q + j);
+  cplx<T>* p = out + (size_t)k * n + j;
+  if (OP == MS_ADJ) {
+    st(p, cmulc(o, ld(p)));
+  } else {
+    const cplx<T> v = o * ld(w + (size_t)(w_view ? k : m) * n + j);
+    st(p, OP == MS_SUB ? ld(p) - v : v);
+  }
+}
+
+// The probe direction of slice 0: dP_m = (1/nb)·Σ_b conj(O_0[win_b]·q)·χ_0[b·M + m], summed in
+// slot order (Kahan). Grid (ceil(n/256), M). ptyrs/src/engine/lsqml.rs:200-229
+template <typename T>
+__global__ void k_ms_probe_dir(cplx<T>* __restrict__ dP, const cplx<T>* __restrict__ O, int nxo,
+                               const cplx<T>* __restrict__ q, const cplx<T>* __restrict__ chi,
+                               const int2* __restrict__ corner, const int* __restrict__ order,
+                               int nb, int N, int M) {
+  const int n = N * N;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int m = blockIdx.y, y = j / N, x = j % N;
+  const cplx<T> qj = q ? ld(q + j) : cplx<T>{T(1), T(0)};
+  kahan<T> sr, si;
+  for (int b = 0; b < nb; ++b) {
+    const int2 c = corner[order[b]];
+    const cplx<T> o = ld(O + (size_t)(c.x + y) * nxo + c.y + x) * qj;
+    const cplx<T> g = cmulc(o, ld(chi + ((size_t)b * M + m) * n + j));
+    sr.add(g.x);
+    si.add(g.y);
+  }
+  st(dP + (size_t)m * n + j, cplx<T>{sr.s, si.s} * (T(1) / T(nb)));
+}
+
+// Per slot and workgroup of a slice past the first: Re Σ conj(dop)·χ_s and Σ|dop|² over modes and
+// the workgroup's pixels, dop = dO_s[win]·w_s·q. Grid (nblk, nb). ptyrs/src/engine/lsqml.rs:336-349
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_ms_dots(
+    const cplx<T>* __restrict__ dO, int nxo, const cplx<T>* __restrict__ w,
+    const cplx<T>* __restrict__ q, const cplx<T>* __restrict__ chi, int N, int M,
+    const int2* __restrict__ corner, const int* __restrict__ order, double* __restrict__ part) {
+  __shared__ double red[THREADS];
+  const int b = blockIdx.y;
+  const int2 c = corner[order[b]];
+  const int n = N * N;
+  double s[2] = {0, 0};
+  for (int j = blockIdx.x * THREADS + threadIdx.x; j < n; j += gridDim.x * THREADS) {
+    const int y = j / N, x = j % N;
+    cplx<T> d = ld(dO + (size_t)(c.x + y) * nxo + c.y + x);
+    if (q) d = d * ld(q + j);
+    for (int m = 0; m < M; ++m) {
+      const size_t i = ((size_t)b * M + m) * n + j;
+      const cplx<T> u = d * ld(w + i);
+      s[0] += double(cmulc(u, ld(chi + i)).x);
+      s[1] += double(norm2(u));
+    }
+  }
+  for (int i = 0; i < 2; ++i) {
+    double r = block_reduce<THREADS>(s[i], red, AddOp());
+    if (threadIdx.x == 0) part[((size_t)b * gridDim.x + blockIdx.x) * 2 + i] = r;
+  }
+}
+
+// One workgroup: a slice past the first takes a_obj = max(0.5·num/den·0.9/div, 0) per slot
+// (Eq. 23b), log[b·ls + col], and its step, the trimmed mean, into step[0].
+// ptyrs/src/engine/lsqml.rs:336-349, :418-426
+__global__ __launch_bounds__(1024) static void k_ms_slice_step(const double* __restrict__ part,
+                                                               int nblk, int nb, int div,
+                                                               double* __restrict__ log, int ls,
+                                                               int col,
+                                                               double* __restrict__ step) {
+  __shared__ double red[1024];
+  const int tid = threadIdx.x;
+  for (int b = tid; b < nb; b += 1024) {
+    double num = 0, den = 0;
+    for (int i = 0; i < nblk; ++i) {
+      num += part[((size_t)b * nblk + i) * 2];
+      den += part[((size_t)b * nblk + i) * 2 + 1];
+    }
+    const double a = 0.5 * num / den * 0.9 / double(div);
+    log[(size_t)b * ls + col] = a > 0.0 ? a : 0.0;
+  }
+  __syncthreads();
+  const int kcut = int(ceil(double(nb) * 0.1 / 2.0 - 0.5));
+  double kept = 0;
+  for (int b = tid; b < nb; b += 1024) {
+    const double v = log[(size_t)b * ls + col];
+    int rank = 0;
+    for (int o = 0; o < nb; ++o) {
+      const double w = log[(size_t)o * ls + col];
+      rank += (w < v) || (w == v && o < b);
+    }
+    if (rank >= kcut && rank < nb - kcut) kept += v;
+  }
+  const double ksum = block_reduce<1024>(kept, red, AddOp());
+  if (tid == 0) step[0] = ksum / double(nb - 2 * kcut);
 }
 
 // ---- AP, DM, RAAR ----------------------------------------------------------------------------------
@@ -1247,22 +1461,25 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a3(
 // with calc = max(Σ|po|², 1e-32), r = Σ Re(conj(po)·a), s = Σ |a|² + 2·Re(conj(po)·b),
 // d = 1 − iobs/calc: num −= 2·r·d, den += 2·s·d + iobs·(2r/calc)² (FP64, per workgroup into
 // part[slot][blockIdx.x][2]). ptyrs/src/engine/ops.rs:197-253
-template <typename T, int N, int M>
+template <typename T, int N, int M, bool ROW>
 __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b_gamma(
     const cplx<T>* __restrict__ psi3, size_t field, const float* __restrict__ counts,
-    const int* __restrict__ order, double* __restrict__ part, const cplx<T>* __restrict__ tw) {
+    const int* __restrict__ order, double* __restrict__ part, const cplx<T>* __restrict__ tw,
+    T norm) {
   using S = ColShape<T, N>;
   constexpr int E = S::E, TL = S::TL, CG = S::L;
   constexpr int n = N * N;
   __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
   __shared__ cplx<T> stw[N];
   __shared__ double red[S::NT];
-  const int tid = threadIdx.x, col = tid % CG, jl = tid / CG;
-  const int x = blockIdx.x * CG + col;
+  const int tid = threadIdx.x;
+  const int line = ROW ? tid / TL : tid % CG, jl = ROW ? tid % TL : tid / CG;
+  const int x = blockIdx.x * CG + line;
   const int b = blockIdx.y;
   const int f = order[b];
-  T* re = lre + col * S::LD;
-  T* im = lim + col * S::LD;
+  T* re = lre + line * S::LD;
+  T* im = lim + line * S::LD;
+  auto at = [&](int e) { return ROW ? x * N + jl + TL * e : (jl + TL * e) * N + x; };
   stage_twiddles<T, N, S::NT>(stw, tw);
   T calc[E], r[E], s[E];
 #pragma unroll
@@ -1272,7 +1489,7 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b_gamma(
     cplx<T> po[E], a[E], bb[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const size_t i = plane + (jl + TL * e) * N + x;
+      const size_t i = plane + at(e);
       po[e] = ld(psi3 + i);
       a[e] = ld(psi3 + field + i);
       bb[e] = ld(psi3 + 2 * field + i);
@@ -1287,12 +1504,12 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b_gamma(
       s[e] += norm2(a[e]) + T(2) * cmulc(po[e], bb[e]).x;
     }
   }
-  const double inv_n = 1.0 / double(n);
+  const double inv_n = double(norm);
   double num = 0, den = 0;
   const float* cnt = counts + (size_t)f * n;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const float io = cnt[(jl + TL * e) * N + x];
+    const float io = cnt[at(e)];
     if (io >= 0.f) {
       double cl = double(calc[e]) * inv_n;
       cl = cl > 1e-32 ? cl : 1e-32;

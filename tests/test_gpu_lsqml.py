@@ -79,10 +79,12 @@ def _near_state():
 
 
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
-@pytest.mark.parametrize("case", [1, 3])
-def test_lsqml_fresnel_matches_reference_fixtures(dtype, case):
-    """Poisson LSQML, batches of 8 (seed 7), Fresnel forward (case 1) and back (case 3): every
-    step of every epoch, the LLK after every epoch and the changes over the run."""
+@pytest.mark.parametrize("case", [0, 1, 2, 3])
+def test_lsqml_near_field_matches_reference_fixtures(dtype, case):
+    """Poisson LSQML, batches of 8 (seed 7), under an angular-spectrum exit (case 0; case 2 with
+    an evanescent band the transfer function zeroes) and a Fresnel one forward (case 1) and back
+    (case 3): every step of the last epoch, the LLK after every epoch and the changes over the
+    run."""
     corners, obj, probe, counts = _near_state()
     c = table("lsqml_near_cases_f64.txt")[case]
     vals = np.array([r[2:] for r in counts if r[0] == case]).reshape(len(corners), 24, 24)

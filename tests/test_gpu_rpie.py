@@ -122,10 +122,11 @@ def _near_start(c):
 
 
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
-@pytest.mark.parametrize("run", [3, 4, 5, 6, 7, 8])
-def test_rpie_fresnel_matches_reference_fixtures(dtype, run):
-    """The reference's rPIE under a Fresnel transform forward (case 1) and back (case 2),
-    batches 1, 4 and 16, 30 epochs (ptyrs/tests/engine_rpie_near.rs:102-169)."""
+@pytest.mark.parametrize("run", range(9))
+def test_rpie_near_field_matches_reference_fixtures(dtype, run):
+    """The reference's rPIE under an angular-spectrum exit (case 0) and a Fresnel one forward
+    (case 1) and back (case 2), batches 1, 4 and 16, 30 epochs
+    (ptyrs/tests/engine_rpie_near.rs:102-169)."""
     r = table("rpie_near_runs_f64.txt")[run]
     c = table("rpie_near_cases_f64.txt")[int(r[1])]
     counts, at, obj, probe, corners = _near_start(c)
@@ -147,10 +148,25 @@ def test_rpie_fresnel_matches_reference_fixtures(dtype, run):
         assert rel_l2(got.ravel(), want) < tol * 10, name
 
 
-def test_angular_spectrum_exit_is_refused():
-    c = table("rpie_near_cases_f64.txt")[0]
-    counts, at, obj, probe, _ = _near_start(c)
+def test_banded_angular_spectrum_runs_lsqml_and_llk_only():
+    """A transfer function with an evanescent band is not inverted by its adjoint: rPIE, AP, DM,
+    RAAR and ML refuse it, LSQML and the LLK pass run it (ptyrs/src/engine/gpu/device.rs:
+    3087-3105). The LSQML fixtures' case 2 has such a band."""
+    c = table("lsqml_near_cases_f64.txt")[2]
+    corners = np.array(table("lsqml_near_corners_f64.txt"))
+    counts = np.array([r[2:] for r in table("lsqml_near_counts_f64.txt") if r[0] == 2])
+    counts = counts.reshape(len(corners), 24, 24).astype(np.float32)
+    obj = np.array([r[2] + 1j * r[3] for r in table("lsqml_near_obj_f64.txt")]).reshape(1, 40, 40)
+    probe = np.array([r[3] + 1j * r[4] for r in table("lsqml_near_probe_f64.txt")])
+    nf = len(corners)
+    scan = pa.Scan(counts, np.zeros(nf), np.zeros(nf), c[1], 1.0, 75e-6,
+                   corners=corners[:, 1:3].astype(np.int64))
     w = pa.Wavefield([24, 24], c[1], [c[2], c[2]], c[3])
-    scan = pa.Scan(counts.astype(np.float32), at[0], at[1], c[1], 1.0, 75e-6)
-    with pytest.raises(ValueError, match="angular"):
-        pa.GpuPtycho(scan, obj, probe, optics=pa.Optics.free_space(w))
+    g = pa.GpuPtycho(scan, obj, probe.reshape(-1, 24, 24), optics=pa.Optics.free_space(w))
+    for name in ("rpie", "ap", "dm", "raar", "ml"):
+        with pytest.raises(ValueError, match="unitary"):
+            getattr(g, name)(1)
+    assert np.array_equal(g.object, obj)
+    assert np.isfinite(g.llk()["poisson"])
+    g.lsqml(1, 1, 8, "poisson", 7)
+    assert len(g.history["poisson"]) == 1
