@@ -1,7 +1,8 @@
 """pty-chi LSQML as the reference's CPU engine restates it, one object slice or several
-(`SliceWaves`), no position or OPR features: one batch's directions (`LsqmlBatch`), the
-per-frame step sizes, the step and the epochs with the preconditioner schedule. An object is
-one `[nobj, nyo, nxo]` array, or a list of them, one per slice of the optics.
+(`SliceWaves`), with position offsets under the PATCH model (refined or held) and no OPR:
+one batch's directions (`LsqmlBatch`), the per-frame step sizes, the step and the epochs with
+the preconditioner schedule. An object is one `[nobj, nyo, nxo]` array, or a list of them, one
+per slice of the optics.
 
 Test infrastructure (see oracle/__init__.py). Follows ptyrs/src/engine/lsqml.rs.
 """
@@ -12,6 +13,8 @@ import numpy as np
 
 from .base import random_batches, to_natural, to_shifted
 from .multislice import SliceWaves
+from .positions import (EPS_POSITION, batch_position_step, gradient_directions, gradient_sums,
+                        move_planes, moved_quad)
 from .ptycho import obj_probe_to_psi, llk
 
 EPS_AMPLITUDE = 1e-9   # ptyrs/src/engine/lsqml.rs:25-31
@@ -55,19 +58,39 @@ def as_slices(obj):
     return list(obj) if isinstance(obj, (list, tuple)) else [obj]
 
 
+def unique_waves(run, views, optics, objects, probe):
+    """The slice waves of `views` (whole frames) at each frame's own probe planes: plane
+    b·np + m of the b-th frame, moved by its offset under PATCH (q = the first slice's plane
+    phase), and past one slice the last slice's phase displaced with it.
+    ptyrs/src/engine/lsqml.rs:474-530, ptyrs/src/engine/opr.rs:88-124"""
+    nmode = 0
+    while nmode < len(views) and views[nmode][0] == views[0][0]:
+        nmode += 1
+    frames = [views[k][0] for k in range(0, len(views), nmode)]
+    if run.offsets is None:
+        return frames, nmode, None, SliceWaves(optics, views, objects, probe)
+    np_ = probe.shape[0]
+    q0 = optics.quad if len(objects) == 1 else None
+    moves = [run.offsets[f].copy() for f in frames]
+    planes = np.concatenate([move_planes(probe, q0, d) for d in moves])
+    uviews = [(f, m, (k // nmode) * np_ + p, o, oy, ox)
+              for k, (f, m, p, o, oy, ox) in enumerate(views)]
+    quads = None
+    if len(objects) > 1:
+        quads = np.stack([moved_quad(optics, moves[k // nmode]) for k in range(len(views))])
+    return frames, nmode, moves, SliceWaves(optics, uviews, objects, planes, quads)
+
+
 class LsqmlBatch:
     """One batch's directions at one state. ptyrs/src/engine/lsqml.rs:97-181"""
 
     def __init__(self, run, views, iobs, optics, obj, probe):
         cost = run.cost
         ny, nx = probe.shape[-2:]
-        nmode = 0
-        while nmode < len(views) and views[nmode][0] == views[0][0]:
-            nmode += 1
-        self.views, self.nmode = views, nmode
-        self.frames = [views[k][0] for k in range(0, len(views), nmode)]
         self.objects, self.probe, self.optics = as_slices(obj), probe, optics
-        self.waves = SliceWaves(optics, views, self.objects, probe)
+        self.frames, nmode, self.moves, self.waves = unique_waves(run, views, optics,
+                                                                  self.objects, probe)
+        self.views, self.nmode = views, nmode
         psi = self.waves.psi
         self.psi = psi
         far = optics.exit_forward(psi)
@@ -116,10 +139,38 @@ class LsqmlBatch:
 
     def frame_probe_directions(self):
         """Plane b·np + m: conj(q·O_view)·chi_m on slice 0. ptyrs/src/engine/lsqml.rs:200-205"""
-        return self.waves.probe_adjoint(self.chis[0])
+        return self.waves.probe_adjoint(self.chis[0], self.probe.shape[0])
+
+    def adjoint_move(self, d):
+        """Frame b's planes of `d` moved back by −offset (the adjoint of the probe move).
+        ptyrs/src/engine/lsqml.rs:207-222"""
+        if self.moves is None:
+            return d
+        np_ = self.probe.shape[0]
+        q0 = self.optics.quad if len(self.objects) == 1 else None
+        out = d.copy()
+        for b, off in enumerate(self.moves):
+            out[b * np_:(b + 1) * np_] = move_planes(d[b * np_:(b + 1) * np_], q0, -off)
+        return out
 
     def probe_direction(self):
-        return frame_mean(self.frame_probe_directions(), self.probe.shape[0])
+        return frame_mean(self.adjoint_move(self.frame_probe_directions()), self.probe.shape[0])
+
+    def position_directions(self):
+        """Per frame [Δy, Δx] on slice S//2 from its first view (probe and object mode 0):
+        gradient sums of the Fourier derivative of the object window against w·q and chi_s.
+        ptyrs/src/engine/lsqml.rs:230-270"""
+        s = len(self.objects) // 2
+        ny, nx = self.probe.shape[-2:]
+        sums = []
+        for b in range(len(self.frames)):
+            k = b * self.nmode
+            _, _, _, o, oy, ox = self.views[k]
+            wq = self.waves.incident[s][k] * self.waves.quad(s, k)
+            patch = self.objects[s][o, oy:oy + ny, ox:ox + nx]
+            sums.append(gradient_sums(patch, wq, to_natural(self.chis[s][k])))
+        eps = EPS_POSITION * self.optics.propagator.intensity_scale()
+        return gradient_directions(sums, eps)
 
     def step_sizes(self, d_obj, d_probe):
         """Per slice and frame a_obj, per frame a_probe: slice 0 jointly with the probe by the
@@ -127,7 +178,7 @@ class LsqmlBatch:
         d_obj = as_slices(d_obj)
         ny, nx = self.probe.shape[-2:]
         n = ny * nx
-        dpo = obj_probe_to_psi(None, self.views, self.objects[0], d_probe, self.waves.quad(0))
+        dpo = obj_probe_to_psi(None, self.views, self.objects[0], d_probe, self.waves.quad(0, 0))
         dop = self.waves.obj_tangent(0, d_obj[0])
         lam = LAMBDA0 * self.optics.propagator.intensity_scale() / n
         nb = len(self.frames)
@@ -164,7 +215,9 @@ class LsqmlBatch:
 class Lsqml:
     """LSQML state across epochs. ptyrs/src/engine/lsqml.rs:428-715"""
 
-    def __init__(self, batch_size, cost=POISSON, seed=7):
+    def __init__(self, batch_size, cost=POISSON, seed=7, offsets=None, refine=False):
+        """`offsets` ([nframe, 2] object pixels, PATCH model) are held, or refined after every
+        step with `refine` (Features::position, ptyrs/src/engine/features.rs:304-330)."""
         if batch_size <= 0:
             raise ValueError("LSQML batch_size must be >= 1")
         if cost not in (AMPLITUDE, POISSON):
@@ -173,6 +226,8 @@ class Lsqml:
         self.epochs = 0
         self.preconditioner = None
         self.a_far = {}
+        self.offsets = None if offsets is None else np.array(offsets, dtype=np.float64)
+        self.refine = refine and offsets is not None
 
     def a_far_of(self, f):
         return self.a_far.get(f, 0.5)
@@ -186,11 +241,14 @@ class Lsqml:
         d_obj = [d / np.sqrt(pre * pre + floor) for d in b.obj_directions()]
         d_probe = b.probe_direction()
         a_obj, a_probe, _ = b.step_sizes(d_obj, d_probe)
+        d_pos = b.position_directions() if self.refine else None
         for f, a in zip(b.frames, b.a_far):
             self.a_far[f] = a
         per_slice = a_obj if len(b.objects) > 1 else [a_obj]
         new = [o + trim_mean(a) * d for o, a, d in zip(as_slices(obj), per_slice, d_obj)]
         probe = probe + (sum(a_probe) / len(a_probe)) * d_probe
+        if d_pos is not None:
+            batch_position_step(self.offsets, b.frames, d_pos)
         out = new if isinstance(obj, (list, tuple)) else new[0]
         return out, probe, dict(frames=b.frames, a_far=b.a_far, a_obj=a_obj, a_probe=a_probe)
 
@@ -220,11 +278,15 @@ class Lsqml:
     def llk(self, ptycho, obj, probe):
         """Normalised LLK of obj, probe over every frame. ptyrs/src/engine/lsqml.rs:532-546"""
         return slice_llk(ptycho.optics, ptycho.views_all, ptycho.iobs, ptycho.addr.nmode,
-                         ptycho.nb_obs, obj, probe)
+                         ptycho.nb_obs, obj, probe, self)
 
 
-def slice_llk(optics, views, iobs, nmode, nb_obs, obj, probe):
-    """Normalised LLK of the slice model's exit waves over `views`.
+def slice_llk(optics, views, iobs, nmode, nb_obs, obj, probe, run=None):
+    """Normalised LLK of the slice model's exit waves over `views`, at `run`'s offsets if given.
     ptyrs/src/engine/lsqml.rs:538-555"""
-    psi = optics.exit_forward(SliceWaves(optics, views, as_slices(obj), probe).psi)
+    if run is not None and run.offsets is not None:
+        waves = unique_waves(run, views, optics, as_slices(obj), probe)[3]
+    else:
+        waves = SliceWaves(optics, views, as_slices(obj), probe)
+    psi = optics.exit_forward(waves.psi)
     return llk(iobs, nmode, psi).normalised(nb_obs)

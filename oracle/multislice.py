@@ -1,8 +1,9 @@
 """pty-chi's multislice exit wave over the oracle's arrays: the wave entering slice 0 is the
 view's probe plane; slice s multiplies the wave entering it by its object window, giving psi_s,
 which reaches slice s + 1 through gap s (an angular-spectrum propagator); the last slice's psi,
-times the plane phase q, is the exit wave. One slice is `obj_probe_to_psi` itself. No position
-offsets and no FullObject sampling.
+times the plane phase q, is the exit wave. One slice is `obj_probe_to_psi` itself. No
+FullObject sampling: a frame's position offset enters through its moved probe planes (the
+caller's) and, past one slice, the last slice's plane phase displaced with them.
 
 Test infrastructure (see oracle/__init__.py). Follows ptyrs/src/engine/multislice.rs:35-430.
 """
@@ -16,10 +17,12 @@ class SliceWaves:
     """The waves of `views` through the slices of `optics` at `objects` (one `[nobj, nyo, nxo]`
     array per slice) and `probe`. ptyrs/src/engine/multislice.rs:70-171"""
 
-    def __init__(self, optics, views, objects, probe):
+    def __init__(self, optics, views, objects, probe, view_quads=None):
         if len(objects) != optics.nslice():
             raise ValueError("one object per slice of the optics")
         self.optics, self.views, self.objects = optics, list(views), list(objects)
+        # per view, the last slice's phase displaced by its frame's offset (more than one slice)
+        self.view_quads = view_quads if len(objects) > 1 else None
         ny, nx = probe.shape[-2:]
         self.shape = (ny, nx)
         self.probe_shape = probe.shape
@@ -33,20 +36,19 @@ class SliceWaves:
         nat = self._times_window(last, self.objects[last], self.incident[last])
         self.psi = np.stack([to_shifted(p) for p in nat])
 
-    def quad(self, s):
-        """The plane phase slice s applies: q on the last slice, 1 elsewhere.
-        ptyrs/src/engine/propagate.rs:127-129"""
+    def quad(self, s, k=0):
+        """The plane phase slice s applies to view k: q on the last slice (displaced per frame
+        where offsets move it), 1 elsewhere. ptyrs/src/engine/multislice.rs:187-241"""
         if s + 1 == len(self.objects):
-            return self.optics.quad
+            return self.optics.quad if self.view_quads is None else self.view_quads[k]
         return np.ones(self.shape, dtype=np.complex128)
 
     def _times_window(self, s, obj, w):
         """`q_s·O[win_k]·w_k` per view, natural layout. ptyrs/src/engine/multislice.rs:173-185"""
         ny, nx = self.shape
-        q = self.quad(s)
         out = np.empty_like(w)
         for k, (_, _, _, o, oy, ox) in enumerate(self.views):
-            out[k] = w[k] * obj[o, oy:oy + ny, ox:ox + nx] * q
+            out[k] = w[k] * obj[o, oy:oy + ny, ox:ox + nx] * self.quad(s, k)
         return out
 
     def back_propagate(self, chi):
@@ -57,9 +59,10 @@ class SliceWaves:
         out = [chi]
         for s in range(len(self.objects) - 2, -1, -1):
             up = out[-1]
-            q, obj = self.quad(s + 1), self.objects[s + 1]
+            obj = self.objects[s + 1]
             g = np.empty((len(self.views), ny, nx), dtype=np.complex128)
             for k, (_, _, _, o, oy, ox) in enumerate(self.views):
+                q = self.quad(s + 1, k)
                 g[k] = to_natural(up[k]) * np.conj(q * obj[o, oy:oy + ny, ox:ox + nx])
             g = self.optics.gaps[s].adjoint(g)
             out.append(np.stack([to_shifted(p) for p in g]))
@@ -70,11 +73,10 @@ class SliceWaves:
         """`Σ_views conj(q_s·in_s)·chi_s` scattered onto slice s.
         ptyrs/src/engine/multislice.rs:402-430"""
         ny, nx = self.shape
-        q = self.quad(s)
         out = np.zeros(self.objects[s].shape, dtype=np.complex128)
         for k, (_, _, _, o, oy, ox) in enumerate(self.views):
             out[o, oy:oy + ny, ox:ox + nx] += to_natural(chi_s[k]) * np.conj(
-                q * self.incident[s][k])
+                self.quad(s, k) * self.incident[s][k])
         return out
 
     def obj_tangent(self, s, d):
@@ -83,20 +85,22 @@ class SliceWaves:
         nat = self._times_window(s, d, self.incident[s])
         return np.stack([to_shifted(p) for p in nat])
 
-    def probe_adjoint(self, chi_0):
-        """Plane b·np + m: `conj(q_0·O_0[win])·chi_0` of the b-th frame and probe mode m, and
-        |O_0[win]|². ptyrs/src/engine/multislice.rs:432-442, lsqml.rs:200-205"""
+    def probe_adjoint(self, chi_0, np_):
+        """Plane b·np + m: `conj(q_0·O_0[win])·chi_0` of the b-th frame and probe mode m
+        (a view's probe index taken modulo `np_`, the shared probe's modes).
+        ptyrs/src/engine/multislice.rs:432-442, lsqml.rs:200-205"""
         ny, nx = self.shape
-        np_ = self.probe_shape[0]
         nmode = 0
         while nmode < len(self.views) and self.views[nmode][0] == self.views[0][0]:
             nmode += 1
         nb = len(self.views) // nmode
         out = np.zeros((nb * np_, ny, nx), dtype=np.complex128)
-        q, obj = self.quad(0), self.objects[0]
+        obj = self.objects[0]
         for k, (_, _, p, o, oy, ox) in enumerate(self.views):
             b = k // nmode
-            out[b * np_ + p] += np.conj(q * obj[o, oy:oy + ny, ox:ox + nx]) * to_natural(chi_0[k])
+            q = self.quad(0, k)
+            out[b * np_ + p % np_] += np.conj(q * obj[o, oy:oy + ny, ox:ox + nx]) * to_natural(
+                chi_0[k])
         return out
 
     def tangent(self, d_objects, d_probe):

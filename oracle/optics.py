@@ -188,11 +188,13 @@ class Optics:
     propagator, the object pixel, and in-object gaps for multislice.
     ptyrs/src/engine/propagate.rs:24-186"""
 
-    def __init__(self, pixel, quad, propagator, gaps=()):
+    def __init__(self, pixel, quad, propagator, gaps=(), plane_quad=(0.0, 0.0)):
         self.pixel = list(pixel)
         self.quad = quad
         self.propagator = propagator
         self.gaps = list(gaps)
+        # quad = exp(i·(c[0]·Y² + c[1]·X²)) over the centred pixel (propagate.rs:29-31)
+        self.plane_quad = tuple(plane_quad)
 
     @staticmethod
     def far_field(wavelength, distance, detector_pixel, frame):
@@ -200,7 +202,9 @@ class Optics:
         from .base import pixel_size_object
         pixel = pixel_size_object(wavelength, distance, detector_pixel, frame)
         q = quad_phase(wavelength, distance, pixel, frame)
-        return Optics(pixel, q, Propagator(FOURIER, frame, scale=float(frame[0] * frame[1])))
+        lz = wavelength * distance
+        return Optics(pixel, q, Propagator(FOURIER, frame, scale=float(frame[0] * frame[1])),
+                      plane_quad=(math.pi * pixel[0] ** 2 / lz, math.pi * pixel[1] ** 2 / lz))
 
     @staticmethod
     def free_space(w):
@@ -209,13 +213,15 @@ class Optics:
         p = Propagator.free_space(w)
         ny, nx = w.shape
         quad = np.ones((ny, nx), dtype=np.complex128)
+        plane_quad = (0.0, 0.0)
         if p.kind == FRESNEL:
             quad = to_natural(p.b)
-        return Optics(w.pixel, quad, p)
+            plane_quad = p.plane_quad
+        return Optics(w.pixel, quad, p, plane_quad=plane_quad)
 
     def with_slices(self, gaps):
         return Optics(self.pixel, self.quad, self.propagator,
-                      [Propagator.angular_spectrum(w) for w in gaps])
+                      [Propagator.angular_spectrum(w) for w in gaps], self.plane_quad)
 
     def nslice(self):
         return len(self.gaps) + 1
