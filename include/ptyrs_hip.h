@@ -39,6 +39,9 @@ enum { PTYRS_FOURIER = 0, PTYRS_ANGULAR_SPECTRUM = 1, PTYRS_FRESNEL = 2 };
 /* Element types of the engine's arithmetic. */
 enum { PTYRS_C64 = 0, PTYRS_C128 = 1 };
 
+/* Sampling, how position offsets enter the exit waves (ptyrs/src/engine/sampling.rs:32-36). */
+enum { PTYRS_PATCH = 0, PTYRS_FULL_OBJECT = 1 };
+
 /* Algorithms of LLK records (ptyrs/src/engine/ptycho.rs:22-31). */
 enum { PTYRS_AP = 0, PTYRS_DM = 1, PTYRS_RAAR = 2, PTYRS_RPIE = 3, PTYRS_ML = 4, PTYRS_BH = 5,
        PTYRS_LSQML = 6 };
@@ -204,6 +207,23 @@ int ptyrs_engine_lsqml_steps(ptyrs_engine* e, ptyrs_lsqml_step* out, size_t cap,
 /* LsqmlStep::a_obj of every slice (ptyrs/src/engine/lsqml.rs:55-66): out[i·nslice + s] is a_obj
  * of slice s at step i of ptyrs_engine_lsqml_steps; len must be n·nslice. */
 int ptyrs_engine_lsqml_slice_steps(ptyrs_engine* e, double* out, size_t len);
+
+/* GpuPtycho::set_features with Features::position only (ptyrs/src/engine/gpu/mod.rs:846-857,
+ * ptyrs/src/engine/features.rs:304-330): offsets [nframe][2] (dy, dx) in object pixels,
+ * len = 2·nframe, refined by every LSQML step when refine != 0, else held; offsets = NULL
+ * stops modelling them. Offsets are kept in FP64 on the device. With offsets only LSQML and
+ * the LLK pass run (the other algorithms fail with PTYRS_EINVAL), under the PATCH sampling. */
+int ptyrs_engine_set_position(ptyrs_engine* e, const double* offsets, size_t len, int refine);
+
+/* GpuPtycho::features' position (ptyrs/src/engine/gpu/mod.rs:823-825): *state = 0 (none),
+ * 1 (held) or 2 (refined); the offsets are copied to out (len = 2·nframe) if out is non-null. */
+int ptyrs_engine_position(ptyrs_engine* e, double* out, size_t len, int* state);
+
+/* GpuPtycho::sampling / set_sampling (ptyrs/src/engine/gpu/mod.rs:827-844): PTYRS_PATCH or
+ * PTYRS_FULL_OBJECT, Sampling::auto of the optics, object and frame until set. Setting
+ * PTYRS_FULL_OBJECT fails with PTYRS_EINVAL: this engine runs the PATCH model only. */
+int ptyrs_engine_sampling(ptyrs_engine* e, int* sampling);
+int ptyrs_engine_set_sampling(ptyrs_engine* e, int sampling);
 
 /* GpuPtycho::llk (ptyrs/src/engine/gpu/mod.rs:853-868): normalised LLK of the current state. */
 int ptyrs_engine_llk(ptyrs_engine* e, ptyrs_llk_record* out);

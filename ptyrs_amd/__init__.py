@@ -121,6 +121,10 @@ _SIGNATURES = [
     ("ptyrs_engine_lsqml_steps", [_P, ctypes.POINTER(_StepC), _SZ, ctypes.POINTER(_SZ)],
      ctypes.c_int),
     ("ptyrs_engine_lsqml_slice_steps", [_P, _D, _SZ], ctypes.c_int),
+    ("ptyrs_engine_set_position", [_P, _D, _SZ, ctypes.c_int], ctypes.c_int),
+    ("ptyrs_engine_position", [_P, _D, _SZ, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    ("ptyrs_engine_sampling", [_P, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    ("ptyrs_engine_set_sampling", [_P, ctypes.c_int], ctypes.c_int),
     ("ptyrs_engine_llk", [_P, ctypes.POINTER(_RecordC)], ctypes.c_int),
     ("ptyrs_engine_object", [_P, _D, _SZ], ctypes.c_int),
     ("ptyrs_engine_probe", [_P, _D, _SZ], ctypes.c_int),
@@ -510,6 +514,56 @@ class GpuPtycho:
             raise ValueError(f"object {a.shape} differs from {self._obj_shape}")
         _check(library().ptyrs_engine_set_object(self._h, _ptr(a.view(np.float64), ctypes.c_double),
                                                  a.size * 2))
+
+    def set_features(self, *, scale=None, background=None, position=None, hold=()):
+        """As `Ptycho.set_features` (ptyrs-py/src/gpu.rs:199-213, lib.rs:313-345): `position`
+        `[nframe, 2]` offsets `[dy, dx]` in object pixels, refined by LSQML unless `hold` names
+        "position"; left out, none are modelled. This engine models no intensity scale or
+        background."""
+        hold = list(hold)
+        for h in hold:
+            if h not in ("scale", "background", "position"):
+                raise ValueError(f"hold names {h!r}, not scale, background or position")
+            if {"scale": scale, "background": background, "position": position}[h] is None:
+                raise ValueError(f"hold names {h!r}, which is not given")
+        if scale is not None or background is not None:
+            raise ValueError("this GPU engine models no intensity scale or background")
+        lib = library()
+        if position is None:
+            _check(lib.ptyrs_engine_set_position(self._h, None, 0, 0))
+            return
+        d = np.ascontiguousarray(position, dtype=np.float64)
+        if d.ndim != 2 or d.shape[1] != 2:
+            raise ValueError(f"position offsets must be [nframe, 2], got {list(d.shape)}")
+        if d.shape[0] != self.nframe:
+            raise ValueError(f"{d.shape[0]} position offsets for {self.nframe} frames")
+        _check(lib.ptyrs_engine_set_position(self._h, _ptr(d, ctypes.c_double), d.size,
+                                             int("position" not in hold)))
+
+    @property
+    def features(self):
+        """As `Ptycho.features`: `scale`, `background`, `position` (None where not modelled)
+        and `hold`, the names of those held."""
+        out = np.empty((self.nframe, 2))
+        state = ctypes.c_int()
+        _check(library().ptyrs_engine_position(self._h, _ptr(out, ctypes.c_double), out.size,
+                                               ctypes.byref(state)))
+        position = out if state.value else None
+        return dict(scale=None, background=None, position=position,
+                    hold=["position"] if state.value == 1 else [])
+
+    @property
+    def sampling(self):
+        """`"patch"` or `"full_object"` (Sampling::auto of the optics, object and frame until
+        set)."""
+        s = ctypes.c_int()
+        _check(library().ptyrs_engine_sampling(self._h, ctypes.byref(s)))
+        return "patch" if s.value == 0 else "full_object"
+
+    def set_sampling(self, sampling):
+        if sampling not in ("patch", "full_object"):
+            raise ValueError(f"sampling must be \"patch\" or \"full_object\", got {sampling!r}")
+        _check(library().ptyrs_engine_set_sampling(self._h, 0 if sampling == "patch" else 1))
 
     def set_probe(self, probe):
         a = np.ascontiguousarray(probe, dtype=np.complex128)

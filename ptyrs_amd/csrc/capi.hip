@@ -281,6 +281,37 @@ int ptyrs_engine_lsqml_slice_steps(ptyrs_engine* h, double* out, size_t len) {
   });
 }
 
+int ptyrs_engine_set_position(ptyrs_engine* h, const double* offsets, size_t len, int refine) {
+  return guarded(h, true, [&](EngineBase& e) {
+    if (offsets && len != e.corners.size())  // 2·nframe
+      throw Invalid("position offsets must be [nframe, 2]");
+    e.set_position(offsets, refine != 0);
+  });
+}
+
+int ptyrs_engine_position(ptyrs_engine* h, double* out, size_t len, int* state) {
+  return guarded(h, false, [&](EngineBase& e) {
+    if (out && len != e.corners.size()) throw Invalid("position buffer has the wrong length");
+    const int s = e.get_position(out);
+    if (state) *state = s;
+  });
+}
+
+int ptyrs_engine_sampling(ptyrs_engine* h, int* sampling) {
+  return guarded(h, false, [&](EngineBase& e) {
+    if (sampling) *sampling = e.sampling;
+  });
+}
+
+int ptyrs_engine_set_sampling(ptyrs_engine* h, int sampling) {
+  return guarded(h, true, [&](EngineBase& e) {
+    if (sampling == PTYRS_FULL_OBJECT)
+      throw Invalid("FullObject sampling is not run on this GPU engine");
+    if (sampling != PTYRS_PATCH) throw Invalid("sampling must be PATCH or FULL_OBJECT");
+    e.sampling = sampling;
+  });
+}
+
 int ptyrs_engine_llk(ptyrs_engine* h, ptyrs_llk_record* out) {
   return guarded(h, true, [&](EngineBase& e) {
     if (!out) throw Invalid("null output");

@@ -101,6 +101,11 @@ class EngineBase {
   virtual void get_probe(double* out) = 0;
   virtual void set_object(const double* in) = 0;
   virtual void set_probe(const double* in) = 0;
+  // Features::position (ptyrs/src/engine/features.rs:304-330): offsets [nframe][2] in object
+  // pixels, or null for none; refine or hold.
+  virtual void set_position(const double* offsets, bool refine) = 0;
+  virtual int get_position(double* out) = 0;  // 0 none, 1 held, 2 refined
+  int sampling = 0;                            // PTYRS_PATCH or PTYRS_FULL_OBJECT
 
   void synchronize() { PTYRS_HIP(hipStreamSynchronize(stream_)); }
 
@@ -200,6 +205,13 @@ class Engine : public EngineBase {
     total_ = pre.total;
     M_ = int(nprobe * nobj);
     nslice_ = pre.nslice;
+    plane_quad_[0] = pre.exit.plane_quad[0];
+    plane_quad_[1] = pre.exit.plane_quad[1];
+    // Sampling::auto (ptyrs/src/engine/sampling.rs:38-49): FullObject in the near field when
+    // 5·frame > 4·object on both axes.
+    sampling = (pre.exit.kind != PTYRS_FOURIER && 5 * N > 4 * int(nyo) && 5 * N > 4 * int(nxo))
+                   ? PTYRS_FULL_OBJECT
+                   : PTYRS_PATCH;
 
     O_.ensure(sizeof(cplx<T>) * nslice_ * nobj * nyo * nxo);
     P_.ensure(sizeof(cplx<T>) * nprobe * n);
@@ -264,6 +276,9 @@ class Engine : public EngineBase {
   void set_probe(const double* in) override { upload_complex(P_, in, nprobe_ * n); }
 
   void llk(double out[4]) override {
+    if (pos_on_ && sampling != PTYRS_PATCH)
+      throw Invalid("position offsets under FullObject sampling are not run on this GPU "
+                    "engine; set the PATCH sampling");
     ensure_workspace(std::min<size_t>(nframe_, kDefaultStack), 1);
     probe_terms();
     llk_pass(0);
@@ -327,6 +342,9 @@ class Engine : public EngineBase {
     if (p.cost != PTYRS_AMPLITUDE && p.cost != PTYRS_POISSON)
       throw Invalid("cost must be amplitude or poisson");
     if (M_ > 4) throw Invalid("the GPU engine runs at most 4 modes");
+    if (pos_on_ && sampling != PTYRS_PATCH)
+      throw Invalid("position offsets under FullObject sampling are not run on this GPU "
+                    "engine; set the PATCH sampling");
     if (nepoch == 0 || nframe_ == 0) {
       cycle += nepoch;
       return;
@@ -359,7 +377,7 @@ class Engine : public EngineBase {
       if (ic < 10 || ic % 10 == 0) precondition(ic == 0);
       const int* ord = order_.as<int>() + ic * nframe_;
       for (auto& s : spans[ic]) {
-        if (nslice_ > 1)
+        if (nslice_ > 1 || pos_on_)
           lsqml_batch_ms(p, ord + s.first, s.second, s.first);
         else
           lsqml_batch(p, ord + s.first, s.second, s.first);
@@ -692,7 +710,103 @@ class Engine : public EngineBase {
     if (nslice_ > 1)
       throw Invalid(std::string(what) +
                     " runs one object slice; only LSQML and the LLK pass run several");
+    if (pos_on_)
+      throw Invalid(std::string(what) +
+                    " with position offsets is not run on this GPU engine; LSQML and the LLK "
+                    "pass model them");
     unitary(what);
+  }
+
+  // Position offsets (PATCH): device [nframe] (dy, dx) in FP64; the moved probe planes of a
+  // stack, the probe spectra, the moved last-slice phases, the window derivative planes.
+  bool pos_on_ = false, pos_refine_ = false;
+  double plane_quad_[2] = {0, 0};
+  DevBuf offs_, inc0_, spec_, qv_, pgrad_, gspec_, pos_part_, pos_sums_;
+
+  void set_position(const double* offsets, bool refine) override {
+    if (offsets == nullptr) {
+      pos_on_ = pos_refine_ = false;
+      return;
+    }
+    for (size_t i = 0; i < 2 * nframe_; ++i)
+      if (!std::isfinite(offsets[i])) throw Invalid("position offsets must be finite");
+    offs_.ensure(sizeof(double) * 2 * std::max<size_t>(1, nframe_));
+    if (nframe_)
+      PTYRS_HIP(hipMemcpy(offs_.p, offsets, sizeof(double) * 2 * nframe_,
+                          hipMemcpyHostToDevice));
+    pos_on_ = true;
+    pos_refine_ = refine;
+  }
+  int get_position(double* out) override {
+    if (!pos_on_) return 0;
+    if (out && nframe_) {
+      PTYRS_HIP(hipStreamSynchronize(stream_));
+      PTYRS_HIP(hipMemcpy(out, offs_.p, sizeof(double) * 2 * nframe_, hipMemcpyDeviceToHost));
+    }
+    return pos_refine_ ? 2 : 1;
+  }
+
+  // The probe planes of a stack's views moved by their frames' offsets as pty-chi PATCH moves
+  // them, conj(q0)·DFT⁻¹(DFT(q0·P_m)·exp(−2πi f·d_b)), q0 the first slice's plane phase
+  // (ptyrs/src/engine/features.rs:386-429, lsqml.rs:474-530), into inc0_.
+  void moved_probes(const int* ord, int nb) {
+    const int K = nb * M_;
+    const size_t pn = size_t(M_) * n;
+    run(K_PROBE, [&] {
+      PTYRS_HIP(hipMemcpyAsync(spec_.p, nslice_ == 1 ? Wt_.p : P_.p, sizeof(cplx<T>) * pn,
+                               hipMemcpyDeviceToDevice, stream_));
+      hipLaunchKernelGGL((k_rows<T, N, false>), dim3(S::NBLK, M_), dim3(S::NT), 0, stream_,
+                         spec_.as<cplx<T>>(), tw_.as<cplx<T>>());
+      hipLaunchKernelGGL((k_cols<T, N, false>), dim3(SC::NBLK, M_), dim3(SC::NT), 0, stream_,
+                         spec_.as<cplx<T>>(), tw_.as<cplx<T>>());
+      hipLaunchKernelGGL(k_ramp_planes<T>, dim3((n + 255) / 256, K), dim3(256), 0, stream_,
+                         inc0_.as<cplx<T>>(), (const cplx<T>*)spec_.as<cplx<T>>(),
+                         (const double2*)offs_.as<double2>(), ord, N, M_, 1.0 / double(n));
+      hipLaunchKernelGGL((k_cols<T, N, true>), dim3(SC::NBLK, K), dim3(SC::NT), 0, stream_,
+                         inc0_.as<cplx<T>>(), tw_.as<cplx<T>>());
+      hipLaunchKernelGGL((k_rows<T, N, true>), dim3(S::NBLK, K), dim3(S::NT), 0, stream_,
+                         inc0_.as<cplx<T>>(), tw_.as<cplx<T>>());
+      if (nslice_ == 1)
+        hipLaunchKernelGGL((k_times_q<T, true>), dim3((n + 255) / 256, K), dim3(256), 0, stream_,
+                           inc0_.as<cplx<T>>(), (const cplx<T>*)q_.as<cplx<T>>(), n);
+    });
+  }
+
+  // The wave entering slice 0 (P per mode, or the moved planes per view) and the last slice's
+  // plane phase (q, or past one slice with offsets and a plane phase, q moved per frame).
+  struct Waves {
+    const cplx<T>* w0;
+    int w0_view;
+    const cplx<T>* qlast;
+    int q_frame;
+  };
+  Waves stack_waves(const int* ord, int nb) {
+    Waves w{P_.as<cplx<T>>(), 0, q_.as<cplx<T>>(), 0};
+    if (!pos_on_) return w;
+    moved_probes(ord, nb);
+    w.w0 = inc0_.as<cplx<T>>();
+    w.w0_view = 1;
+    if (nslice_ > 1 && (plane_quad_[0] != 0.0 || plane_quad_[1] != 0.0)) {
+      run(K_PROBE, [&] {
+        hipLaunchKernelGGL(k_moved_quad<T>, dim3((n + 255) / 256, nb), dim3(256), 0, stream_,
+                           qv_.as<cplx<T>>(), (const cplx<T>*)q_.as<cplx<T>>(),
+                           (const double2*)offs_.as<double2>(), ord, N, plane_quad_[0],
+                           plane_quad_[1]);
+      });
+      w.qlast = qv_.as<cplx<T>>();
+      w.q_frame = 1;
+    }
+    return w;
+  }
+  void ensure_positions(size_t cap) {
+    if (!pos_on_) return;
+    inc0_.ensure(sizeof(cplx<T>) * cap * M_ * n);
+    spec_.ensure(sizeof(cplx<T>) * M_ * n);
+    qv_.ensure(sizeof(cplx<T>) * cap * n);
+    gspec_.ensure(sizeof(cplx<T>) * cap * M_ * n);
+    pgrad_.ensure(sizeof(cplx<T>) * 3 * cap * n);
+    pos_part_.ensure(sizeof(double) * 4 * cap * aux_blocks());
+    pos_sums_.ensure(sizeof(double) * 4 * cap);
   }
 
   // Multislice state: the gaps' transfer functions, a plane of ones, the incident waves of
@@ -720,26 +834,31 @@ class Engine : public EngineBase {
 
   template <int OP>
   void window(cplx<T>* out, size_t s, const cplx<T>* w, int w_view, const cplx<T>* q,
-              const int* ord, int nb) {
+              int q_frame, const int* ord, int nb) {
     hipLaunchKernelGGL((k_ms_window<T, OP>), dim3((n + 255) / 256, nb * M_), dim3(256), 0,
                        stream_, out, slice_obj(s), int(nxo_), w, w_view, q, corner_.as<int2>(),
-                       ord, N, M_);
+                       ord, N, M_, q_frame);
+  }
+
+  // The wave entering slice s: slice 0's from `w`, the others' kept per view.
+  std::pair<const cplx<T>*, int> wave(size_t s, const Waves& w) {
+    if (s == 0) return {w.w0, w.w0_view};
+    return {incident(s), 1};
   }
 
   // The exit waves of a stack through the slices (ptyrs/src/engine/multislice.rs:86-171):
   // the incident waves of slices 1.. kept, psi = q·O_last[win]·w_last, natural layout.
-  void ms_forward(const int* ord, int nb) {
+  void ms_forward(const int* ord, int nb, const Waves& w) {
     const size_t S = nslice_;
     run(K_PASS_A, [&] {
       for (size_t s = 0; s + 1 < S; ++s) {
-        cplx<T>* dst = incident(s + 1);
-        if (s == 0)
-          window<MS_MUL>(dst, 0, P_.as<cplx<T>>(), 0, nullptr, ord, nb);
-        else
-          window<MS_MUL>(dst, s, incident(s), 1, nullptr, ord, nb);
-        gap(dst, nb * M_, s, false);
+        auto in = wave(s, w);
+        window<MS_MUL>(incident(s + 1), s, in.first, in.second, nullptr, 0, ord, nb);
+        gap(incident(s + 1), nb * M_, s, false);
       }
-      window<MS_MUL>(psi_.as<cplx<T>>(), S - 1, incident(S - 1), 1, q_.as<cplx<T>>(), ord, nb);
+      auto in = wave(S - 1, w);
+      window<MS_MUL>(psi_.as<cplx<T>>(), S - 1, in.first, in.second, w.qlast, w.q_frame, ord,
+                     nb);
     });
   }
 
@@ -924,11 +1043,13 @@ class Engine : public EngineBase {
     });
   }
 
-  // One multislice LSQML batch (ptyrs/src/engine/lsqml.rs:563-643 over
-  // ptyrs/src/engine/multislice.rs): the exit waves through the slices, the detector operator
-  // as for one slice, χ back through the slices with each slice's object direction and step
-  // (Eq. 23b) on the way, slice 0 jointly with the probe (Eq. 22), then every update from the
-  // state on entry. One object mode.
+  // One LSQML batch of the general model (ptyrs/src/engine/lsqml.rs:563-643 over
+  // ptyrs/src/engine/multislice.rs): any number of slices, the probe planes moved by the
+  // frames' offsets under PATCH. The exit waves through the slices, the detector operator as
+  // for one slice, χ back through the slices with each slice's object direction and step
+  // (Eq. 23b) on the way, slice 0 jointly with the probe (Eq. 22), the position directions on
+  // slice S/2, then every update from the state on entry and the position step. One object
+  // mode; unfused (one kernel per stage).
   void lsqml_batch_ms(const ptyrs_lsqml_params& p, const int* ord, int nb, int off) {
     const int nt = ntiles();
     const bool gauss = p.cost == PTYRS_AMPLITUDE;
@@ -941,7 +1062,8 @@ class Engine : public EngineBase {
                          lcount_.as<int>(), (const cplx<T>*)nullptr, M_, int(nyo_), int(nxo_),
                          (T*)nullptr, tilemax_.as<T>());
     });
-    ms_forward(ord, nb);
+    const Waves wv = stack_waves(ord, nb);
+    ms_forward(ord, nb, wv);
     run(K_PASS_A, [&] { rows(false, K); });
     BArgs<T> ba{afar_tmp_.as<T>(), prev_.as<T>(), xi_part_.as<double>(), icalc_.as<T>()};
     const int xb = aux_blocks();
@@ -975,52 +1097,75 @@ class Engine : public EngineBase {
     cplx<T>* chi = psi_.as<cplx<T>>();
     run(K_PASS_C, [&] {
       rows(true, K);
-      window<MS_SUB>(chi, S - 1, incident(S - 1), 1, q_.as<cplx<T>>(), ord, nb);
+      auto in = wave(S - 1, wv);
+      window<MS_SUB>(chi, S - 1, in.first, in.second, wv.qlast, wv.q_frame, ord, nb);
     });
-    for (size_t s = S - 1; s >= 1; --s) {
-      const cplx<T>* qs = s + 1 == S ? q_.as<cplx<T>>() : nullptr;
+    const size_t sp = S / 2;  // pty-chi's slice_for_correction
+    for (size_t s = S; s-- > 0;) {
+      const auto in = wave(s, wv);
+      const cplx<T>* qs = s + 1 == S ? wv.qlast : nullptr;
+      const int qf = s + 1 == S ? wv.q_frame : 0;
       run(K_OBJ_UPDATE, [&] {
         hipLaunchKernelGGL(k_lsqml_obj_dir<T>, dim3(nt), dim3(THREADS), 0, stream_,
-                           slice_dir(s), (const cplx<T>*)incident(s), (const cplx<T>*)chi, N, M_,
-                           corner_.as<int2>(), ord, list_.as<int>(), lcount_.as<int>(),
-                           int(cap_), int(nyo_), int(nxo_), ntx_, pre_.as<T>(), floor_.as<T>(),
-                           1, qs);
+                           slice_dir(s), in.first, (const cplx<T>*)chi, N, M_, corner_.as<int2>(),
+                           ord, list_.as<int>(), lcount_.as<int>(), int(cap_), int(nyo_),
+                           int(nxo_), ntx_, pre_.as<T>(), floor_.as<T>(), in.second, qs, qf);
       });
+      if (pos_refine_ && s == sp) position_sums(s, in.first, qs, qf, ord, nb);
+      if (s == 0) break;
       run(K_OTHER, [&] {
         hipLaunchKernelGGL(k_ms_dots<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
-                           (const cplx<T>*)slice_dir(s), int(nxo_), (const cplx<T>*)incident(s),
-                           qs, (const cplx<T>*)chi, N, M_, corner_.as<int2>(), ord,
-                           xi_part2_.as<double>());
+                           (const cplx<T>*)slice_dir(s), int(nxo_), in.first, qs,
+                           (const cplx<T>*)chi, N, M_, corner_.as<int2>(), ord,
+                           xi_part2_.as<double>(), qf);
         hipLaunchKernelGGL(k_ms_slice_step, dim3(1), dim3(1024), 0, stream_,
                            xi_part2_.as<double>(), xb, nb, int(S) * M_, log, ls, int(1 + s),
                            step + 1 + s);
       });
       run(K_PASS_C, [&] {
-        window<MS_ADJ>(chi, s, nullptr, 0, qs, ord, nb);
+        window<MS_ADJ>(chi, s, nullptr, 0, qs, qf, ord, nb);
         gap(chi, K, s - 1, true);
+      });
+    }
+    // Slice 0: the probe direction (moved back per frame with offsets), then the joint step.
+    const cplx<T>* q0 = S == 1 ? q_.as<cplx<T>>() : nullptr;
+    if (pos_on_) {
+      cplx<T>* g = gspec_.as<cplx<T>>();
+      run(K_PROBE, [&] {
+        PTYRS_HIP(hipMemcpyAsync(g, chi, sizeof(cplx<T>) * size_t(K) * n,
+                                 hipMemcpyDeviceToDevice, stream_));
+        window<MS_ADJ>(g, 0, nullptr, 0, nullptr, 0, ord, nb);  // conj(O_0)·χ_0 = q0·g
+        hipLaunchKernelGGL((k_rows<T, N, false>), dim3(S::NBLK, K), dim3(S::NT), 0, stream_, g,
+                           tw_.as<cplx<T>>());
+        hipLaunchKernelGGL((k_cols<T, N, false>), dim3(SC::NBLK, K), dim3(SC::NT), 0, stream_, g,
+                           tw_.as<cplx<T>>());
+        hipLaunchKernelGGL(k_unramp_mean<T>, dim3((n + 255) / 256, M_), dim3(256), 0, stream_,
+                           dPm_.as<cplx<T>>(), (const cplx<T>*)g,
+                           (const double2*)offs_.as<double2>(), ord, nb, N, M_,
+                           1.0 / (double(n) * double(nb)));
+        hipLaunchKernelGGL((k_cols<T, N, true>), dim3(SC::NBLK, M_), dim3(SC::NT), 0, stream_,
+                           dPm_.as<cplx<T>>(), tw_.as<cplx<T>>());
+        hipLaunchKernelGGL((k_rows<T, N, true>), dim3(S::NBLK, M_), dim3(S::NT), 0, stream_,
+                           dPm_.as<cplx<T>>(), tw_.as<cplx<T>>());
+        if (q0)
+          hipLaunchKernelGGL((k_times_q<T, true>), dim3((n + 255) / 256, M_), dim3(256), 0,
+                             stream_, dPm_.as<cplx<T>>(), q0, n);
+      });
+    } else {
+      run(K_PROBE, [&] {
+        hipLaunchKernelGGL(k_ms_probe_dir<T>, dim3((n + 255) / 256, M_), dim3(256), 0, stream_,
+                           dPm_.as<cplx<T>>(), (const cplx<T>*)slice_obj(0), int(nxo_), q0,
+                           (const cplx<T>*)chi, corner_.as<int2>(), ord, nb, N, M_);
       });
     }
     const int total = int(nprobe_ * n);
     const double lam = 1.2e-7 * scale_ * double(M_);
-    run(K_OBJ_UPDATE, [&] {
-      hipLaunchKernelGGL(k_lsqml_obj_dir<T>, dim3(nt), dim3(THREADS), 0, stream_, slice_dir(0),
-                         (const cplx<T>*)P_.as<cplx<T>>(), (const cplx<T>*)chi, N, M_,
-                         corner_.as<int2>(), ord, list_.as<int>(), lcount_.as<int>(), int(cap_),
-                         int(nyo_), int(nxo_), ntx_, pre_.as<T>(), floor_.as<T>(), 0,
-                         (const cplx<T>*)nullptr);
-    });
-    run(K_PROBE, [&] {
-      hipLaunchKernelGGL(k_ms_probe_dir<T>, dim3((n + 255) / 256, M_), dim3(256), 0, stream_,
-                         dPm_.as<cplx<T>>(), (const cplx<T>*)slice_obj(0), int(nxo_),
-                         (const cplx<T>*)nullptr, (const cplx<T>*)chi, corner_.as<int2>(), ord,
-                         nb, N, M_);
-    });
     run(K_OTHER, [&] {
       hipLaunchKernelGGL(k_lsqml_dots<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
                          (const cplx<T>*)slice_dir(0), (const cplx<T>*)slice_obj(0), int(nxo_),
-                         (const cplx<T>*)P_.as<cplx<T>>(), (const cplx<T>*)dPm_.as<cplx<T>>(),
-                         (const cplx<T>*)ones_.as<cplx<T>>(), (const cplx<T>*)chi, N, M_,
-                         corner_.as<int2>(), ord, dots_part_.as<double>());
+                         wv.w0, (const cplx<T>*)dPm_.as<cplx<T>>(),
+                         q0 ? q0 : (const cplx<T>*)ones_.as<cplx<T>>(), (const cplx<T>*)chi, N,
+                         M_, corner_.as<int2>(), ord, dots_part_.as<double>(), wv.w0_view, q0);
       hipLaunchKernelGGL(k_lsqml_step<T>, dim3(1), dim3(1024), 0, stream_,
                          dots_part_.as<double>(), xb, nb, lam, int(S) * M_, sums_.as<double>(),
                          log, step, ls);
@@ -1036,6 +1181,39 @@ class Engine : public EngineBase {
       hipLaunchKernelGGL(k_lsqml_apply_probe<T>, dim3((total + 255) / 256), dim3(256), 0,
                          stream_, P_.as<cplx<T>>(), Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
                          dPm_.as<cplx<T>>(), (const double*)step, n, total);
+    });
+    if (pos_refine_) {
+      const double eps = 1e-6 * scale_;  // EPS_POSITION·k (ptyrs/src/engine/features.rs:455)
+      run(K_OTHER, [&] {
+        hipLaunchKernelGGL(k_pos_step, dim3(1), dim3(1024), 0, stream_,
+                           (const double*)pos_part_.as<double>(), xb, nb, int(nframe_), eps, ord,
+                           offs_.as<double2>(), pos_sums_.as<double>());
+      });
+    }
+  }
+
+  // The position sums of the batch's frames on slice s (ptyrs/src/engine/lsqml.rs:230-270):
+  // the windows' Fourier derivatives against the wave entering the slice times its phase and χ_s.
+  void position_sums(size_t s, const cplx<T>* w, const cplx<T>* q, int q_frame, const int* ord,
+                     int nb) {
+    cplx<T>* spec = pgrad_.as<cplx<T>>();
+    cplx<T>* grad = spec + cap_ * n;
+    run(K_OTHER, [&] {
+      hipLaunchKernelGGL(k_patch<T>, dim3((n + 255) / 256, nb), dim3(256), 0, stream_, spec,
+                         (const cplx<T>*)slice_obj(s), int(nxo_), corner_.as<int2>(), ord, N);
+      hipLaunchKernelGGL((k_rows<T, N, false>), dim3(S::NBLK, nb), dim3(S::NT), 0, stream_, spec,
+                         tw_.as<cplx<T>>());
+      hipLaunchKernelGGL((k_cols<T, N, false>), dim3(SC::NBLK, nb), dim3(SC::NT), 0, stream_,
+                         spec, tw_.as<cplx<T>>());
+      hipLaunchKernelGGL(k_deriv<T>, dim3((n + 255) / 256, nb), dim3(256), 0, stream_, grad,
+                         (const cplx<T>*)spec, N, 1.0 / double(n));
+      hipLaunchKernelGGL((k_cols<T, N, true>), dim3(SC::NBLK, 2 * nb), dim3(SC::NT), 0, stream_,
+                         grad, tw_.as<cplx<T>>());
+      hipLaunchKernelGGL((k_rows<T, N, true>), dim3(S::NBLK, 2 * nb), dim3(S::NT), 0, stream_,
+                         grad, tw_.as<cplx<T>>());
+      hipLaunchKernelGGL(k_pos_sums<T>, dim3(aux_blocks(), nb), dim3(THREADS), 0, stream_,
+                         (const cplx<T>*)grad, w, q, q_frame, (const cplx<T>*)psi_.as<cplx<T>>(),
+                         N, M_, pos_part_.as<double>());
     });
   }
 
@@ -1158,6 +1336,7 @@ class Engine : public EngineBase {
     llk_part_.ensure(sizeof(double) * 4 * nframe_ * SC::NBLK);
     cap_ = std::max(cap_, cap);
     if (nslice_ > 1) inc_.ensure(sizeof(cplx<T>) * (nslice_ - 1) * cap_ * M_ * n);
+    ensure_positions(cap_);
   }
 
   void probe_terms() {
@@ -1221,8 +1400,8 @@ class Engine : public EngineBase {
     for (int f0 = 0; f0 < int(nframe_); f0 += step) {
       const int nb = std::min(step, int(nframe_) - f0);
       const int* ord = iota_.as<int>() + f0;
-      if (nslice_ > 1) {
-        ms_forward(ord, nb);
+      if (nslice_ > 1 || pos_on_) {
+        ms_forward(ord, nb, stack_waves(ord, nb));
         run(K_PASS_A, [&] { rows(false, nb * M_); });
       } else {
         pass_a(ord, nb, false);

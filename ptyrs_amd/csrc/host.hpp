@@ -111,6 +111,9 @@ struct ExitModel {
   // and whether it has an evanescent band (zeros), where its adjoint does not invert it.
   std::vector<double> tf;
   bool banded = false;
+  // The plane phase as exp(i·(c[0]·Y² + c[1]·X²)) over the centred pixel, c = plane_quad
+  // (ptyrs/src/engine/propagate.rs:29-31); 0 where there is none.
+  double plane_quad[2] = {0, 0};
 };
 
 inline void set_phase(std::vector<double>& q, size_t i, double ph) {
@@ -128,6 +131,8 @@ inline ExitModel far_field(int ny, int nx, double wavelength, double distance, d
   m.pixel[0] = lz / (ny * dpix);
   m.pixel[1] = lz / (nx * dpix);
   const double f = M_PI / lz;
+  m.plane_quad[0] = f * m.pixel[0] * m.pixel[0];
+  m.plane_quad[1] = f * m.pixel[1] * m.pixel[1];
   m.quad.resize(2 * size_t(ny) * nx);
   for (int y = 0; y < ny; ++y)
     for (int x = 0; x < nx; ++x) {
@@ -138,11 +143,6 @@ inline ExitModel far_field(int ny, int nx, double wavelength, double distance, d
   return m;
 }
 
-// pty-chi's free-space model (ptyrs/src/engine/propagate.rs:66-83, 270-298, 356-366,
-// 533-566): Fourier for an infinite distance, the Fresnel transform x ↦ a·DFT(b·x) where its
-// output pitch is finer than the source pitch, else angular spectrum. Under the Fresnel
-// transform the plane phase is b and the unit-modulus a drops out of every detector
-// intensity and projection.
 // The angular-spectrum transfer function T = exp(2πi z √(1 − r)) for
 // r = (fx² + (a fy)²)/px² < 1, else 0 (the evanescent band, which sets *banded), lengths in
 // wavelengths, on the unshifted fftfreq grid, interleaved re, im.
@@ -173,6 +173,11 @@ inline std::vector<double> angular_spectrum_tf(int ny, int nx, double wavelength
   return tf;
 }
 
+// pty-chi's free-space model (ptyrs/src/engine/propagate.rs:66-83, 270-298, 356-366,
+// 533-566): Fourier for an infinite distance, the Fresnel transform x ↦ a·DFT(b·x) where its
+// output pitch is finer than the source pitch, else angular spectrum. Under the Fresnel
+// transform the plane phase is b and the unit-modulus a drops out of every detector
+// intensity and projection.
 inline ExitModel free_space(int ny, int nx, double wavelength, double py, double px,
                             double distance) {
   if (ny <= 0 || nx <= 0 || ny % 2 || nx % 2) throw Invalid("plane shape must be even and non-zero");
@@ -207,6 +212,8 @@ inline ExitModel free_space(int ny, int nx, double wavelength, double py, double
   const double fr = w * w / d;
   m.reverse = distance < 0.0;
   m.scale = double(ny) * nx * (fr / ar) * (fr / ar);
+  m.plane_quad[0] = M_PI * fr / (ar * ar);
+  m.plane_quad[1] = M_PI * fr;
   for (int y = 0; y < ny; ++y)
     for (int x = 0; x < nx; ++x) {
       const double xx = double(x) - double(nx / 2), yy = double(y) - double(ny / 2);

@@ -720,7 +720,8 @@ __global__ __launch_bounds__(THREADS) void k_lsqml_xi_sums(
 // gradient): every object pixel owned by one thread sums, in slot order,
 // Σ_m Σ_views conj(W·q)·χ_m (Kahan-compensated) and divides by √(pre² + floor); 0 where no view
 // covers it. W is W_m ([M][n]) or, with per_view, the slot's own plane ([slot·M + m][n], a
-// multislice slice's incident waves); q (nullable) a plane phase.
+// multislice slice's incident waves or a moved probe); q (nullable) a plane phase, per slot
+// ([slot][n]) with q_frame.
 // ptyrs/src/engine/lsqml.rs:183-188, :587-598; ptyrs/src/engine/ops.rs:279-310;
 // ptyrs/src/engine/multislice.rs:402-430
 template <typename T>
@@ -729,7 +730,7 @@ __global__ __launch_bounds__(THREADS) void k_lsqml_obj_dir(
     int N, int M, const int2* __restrict__ corner, const int* __restrict__ order,
     const int* __restrict__ list, const int* __restrict__ count, int cap, int nyo, int nxo,
     int ntx, const T* __restrict__ pre, const T* __restrict__ floor, int per_view = 0,
-    const cplx<T>* __restrict__ q = nullptr) {
+    const cplx<T>* __restrict__ q = nullptr, int q_frame = 0) {
   constexpr int RS = THREADS / TILE_X;
   __shared__ int3 cs[THREADS];
   const int t = blockIdx.x, tid = threadIdx.x;
@@ -751,6 +752,7 @@ __global__ __launch_bounds__(THREADS) void k_lsqml_obj_dir(
       if (x < 0 || x >= N) continue;
       const cplx<T>* cv = chi + (size_t)cs[i].x * M * n;
       const cplx<T>* wv = per_view ? Wt + (size_t)cs[i].x * M * n : Wt;
+      const cplx<T>* qv = q && q_frame ? q + (size_t)cs[i].x * n : q;
 #pragma unroll
       for (int r = 0; r < TROWS; ++r) {
         const int y = Y0 + RS * r - cs[i].y;
@@ -758,7 +760,7 @@ __global__ __launch_bounds__(THREADS) void k_lsqml_obj_dir(
         const int j = y * N + x;
         for (int m = 0; m < M; ++m) {
           cplx<T> w = ld(wv + (size_t)m * n + j);
-          if (q) w = w * ld(q + j);
+          if (qv) w = w * ld(qv + j);
           const cplx<T> g = cmulc(w, ld(cv + (size_t)m * n + j));
           sr[r].add(g.x);
           si[r].add(g.y);
@@ -800,14 +802,16 @@ __global__ void k_probe_mean(cplx<T>* __restrict__ dP, const cplx<T>* __restrict
 
 // Per slot and workgroup the six sums of the step-size system (FP64):
 // Σ|dop|², Σ|dpo|², Re Σ dop·conj(dpo), Im Σ dop·conj(dpo), Re Σ conj(dop)·χ, Re Σ conj(dpo)·χ
-// over modes and the workgroup's pixels, dop = dO_window·W_m, dpo = O_window·dP_m·q.
+// over modes and the workgroup's pixels, dop = dO_window·W·qu, dpo = O_window·dP_m·q, W = W_m or
+// with per_view the slot's own plane (a moved probe), qu nullable (1).
 // Grid (nblk, nb). ptyrs/src/engine/lsqml.rs:300-318
 template <typename T>
 __global__ __launch_bounds__(THREADS) void k_lsqml_dots(
     const cplx<T>* __restrict__ dO, const cplx<T>* __restrict__ O, int nxo,
     const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ dP, const cplx<T>* __restrict__ q,
     const cplx<T>* __restrict__ chi, int N, int M, const int2* __restrict__ corner,
-    const int* __restrict__ order, double* __restrict__ part) {
+    const int* __restrict__ order, double* __restrict__ part, int per_view = 0,
+    const cplx<T>* __restrict__ qu = nullptr) {
   __shared__ double red[THREADS];
   const int b = blockIdx.y;
   const int2 c = corner[order[b]];
@@ -816,9 +820,12 @@ __global__ __launch_bounds__(THREADS) void k_lsqml_dots(
   for (int j = blockIdx.x * THREADS + threadIdx.x; j < n; j += gridDim.x * THREADS) {
     const int y = j / N, x = j % N;
     const size_t p = (size_t)(c.x + y) * nxo + c.y + x;
-    const cplx<T> dow = ld(dO + p), ow = ld(O + p), qj = ld(q + j);
+    const cplx<T> ow = ld(O + p), qj = ld(q + j);
+    cplx<T> dow = ld(dO + p);
+    if (qu) dow = dow * ld(qu + j);
+    const cplx<T>* wv = per_view ? Wt + (size_t)b * M * n : Wt;
     for (int m = 0; m < M; ++m) {
-      const cplx<T> u = dow * ld(Wt + (size_t)m * n + j);
+      const cplx<T> u = dow * ld(wv + (size_t)m * n + j);
       const cplx<T> v = ow * ld(dP + (size_t)m * n + j) * qj;
       const cplx<T> ch = ld(chi + ((size_t)b * M + m) * n + j);
       const cplx<T> a12 = cmulc(v, u);  // u·conj(v)
@@ -1053,14 +1060,15 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_gap_cols(cplx<T>* __re
 
 // A slice's window product over the views of a stack (natural layout, one plane per slot and
 // mode): MS_MUL out = O_s[win]·w·q, MS_SUB out −= O_s[win]·w·q, MS_ADJ out = conj(O_s[win]·q)·out.
-// w is P_m (w_view = 0) or the slot's own plane (w_view = 1); q is nullable (1).
+// w is P_m (w_view = 0) or the slot's own plane (w_view = 1); q is nullable (1), per slot
+// ([slot][n]) with q_frame.
 // Grid (ceil(n/256), nb·M). ptyrs/src/engine/multislice.rs:173-185, :266-291
 enum MsOp { MS_MUL = 0, MS_SUB, MS_ADJ };
 template <typename T, int OP>
 __global__ void k_ms_window(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ O, int nxo,
                             const cplx<T>* __restrict__ w, int w_view,
                             const cplx<T>* __restrict__ q, const int2* __restrict__ corner,
-                            const int* __restrict__ order, int N, int M) {
+                            const int* __restrict__ order, int N, int M, int q_frame) {
   const int n = N * N;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
@@ -1068,7 +1076,7 @@ __global__ void k_ms_window(cplx<T>* __restrict__ out, const cplx<T>* __restrict
   const int2 c = corner[order[b]];
   const int y = j / N, x = j % N;
   cplx<T> o = ld(O + (size_t)(c.x + y) * nxo + c.y + x);
-  if (q) o = o * ld(q + j);
+  if (q) o = o * ld(q + (q_frame ? (size_t)b * n : 0) + j);
   cplx<T>* p = out + (size_t)k * n + j;
   if (OP == MS_ADJ) {
     st(p, cmulc(o, ld(p)));
@@ -1107,7 +1115,8 @@ template <typename T>
 __global__ __launch_bounds__(THREADS) void k_ms_dots(
     const cplx<T>* __restrict__ dO, int nxo, const cplx<T>* __restrict__ w,
     const cplx<T>* __restrict__ q, const cplx<T>* __restrict__ chi, int N, int M,
-    const int2* __restrict__ corner, const int* __restrict__ order, double* __restrict__ part) {
+    const int2* __restrict__ corner, const int* __restrict__ order, double* __restrict__ part,
+    int q_frame) {
   __shared__ double red[THREADS];
   const int b = blockIdx.y;
   const int2 c = corner[order[b]];
@@ -1116,7 +1125,7 @@ __global__ __launch_bounds__(THREADS) void k_ms_dots(
   for (int j = blockIdx.x * THREADS + threadIdx.x; j < n; j += gridDim.x * THREADS) {
     const int y = j / N, x = j % N;
     cplx<T> d = ld(dO + (size_t)(c.x + y) * nxo + c.y + x);
-    if (q) d = d * ld(q + j);
+    if (q) d = d * ld(q + (q_frame ? (size_t)b * n : 0) + j);
     for (int m = 0; m < M; ++m) {
       const size_t i = ((size_t)b * M + m) * n + j;
       const cplx<T> u = d * ld(w + i);
@@ -1163,6 +1172,219 @@ __global__ __launch_bounds__(1024) static void k_ms_slice_step(const double* __r
   }
   const double ksum = block_reduce<1024>(kept, red, AddOp());
   if (tid == 0) step[0] = ksum / double(nb - 2 * kcut);
+}
+
+// ---- positions (pty-chi PATCH; ptyrs/src/engine/features.rs:334-521) ---------------------------------
+// Column DFTs of `nplane` planes in place (INV: inverse), unnormalised. Grid (NBLK, nplane).
+template <typename T, int N, bool INV>
+__global__ __launch_bounds__((ColShape<T, N>::NT)) void k_cols(cplx<T>* __restrict__ psi,
+                                                                const cplx<T>* __restrict__ tw) {
+  using S = ColShape<T, N>;
+  constexpr int E = S::E, TL = S::TL, CG = S::L;
+  __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
+  __shared__ cplx<T> stw[N];
+  const int tid = threadIdx.x, col = tid % CG, jl = tid / CG;
+  const int x = blockIdx.x * CG + col;
+  cplx<T>* plane = psi + (size_t)blockIdx.y * N * N;
+  cplx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = ld(plane + (jl + TL * e) * N + x);
+  stage_twiddles<T, N, S::NT>(stw, tw);
+  line_fft<T, N, E, INV>(v, jl, lre + col * S::LD, lim + col * S::LD, stw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) st(plane + (jl + TL * e) * N + x, v[e]);
+}
+
+// fftfreq of DFT index k on an axis of length len.
+__device__ __forceinline__ double dft_freq(int k, int len) {
+  return double(k < (len + 1) / 2 ? k : k - len) * (1.0 / double(len));
+}
+
+// Spectra moved by the slots' offsets: out[b·M + m] = spec[m]·exp(−2πi f·d_b)·scale (the probe
+// move's translation phase, ptyrs/src/engine/features.rs:362-368), d_b = offs[order[b]] in
+// object pixels [dy, dx]. Grid (ceil(n/256), nb·M).
+template <typename T>
+__global__ void k_ramp_planes(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ spec,
+                              const double2* __restrict__ offs, const int* __restrict__ order,
+                              int N, int M, double scale) {
+  const int n = N * N;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int k = blockIdx.y, b = k / M, m = k % M;
+  const double2 d = offs[order[b]];
+  const double ph = -2.0 * M_PI * (dft_freq(j / N, N) * d.x + dft_freq(j % N, N) * d.y);
+  double s, c;
+  sincos(ph, &s, &c);
+  const cplx<T> v = ld(spec + (size_t)m * n + j);
+  st(out + (size_t)k * n + j,
+     cplx<T>{T((double(v.x) * c - double(v.y) * s) * scale),
+             T((double(v.x) * s + double(v.y) * c) * scale)});
+}
+
+// The frame mean of spectra moved back: out[m] = scale·Σ_b G[b·M + m]·exp(+2πi f·d_b), slots in
+// order (Kahan). Grid (ceil(n/256), M). ptyrs/src/engine/lsqml.rs:207-229
+template <typename T>
+__global__ void k_unramp_mean(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ G,
+                              const double2* __restrict__ offs, const int* __restrict__ order,
+                              int nb, int N, int M, double scale) {
+  const int n = N * N;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int m = blockIdx.y;
+  const double fy = dft_freq(j / N, N), fx = dft_freq(j % N, N);
+  kahan<T> sr, si;
+  for (int b = 0; b < nb; ++b) {
+    const double2 d = offs[order[b]];
+    double s, c;
+    sincos(2.0 * M_PI * (fy * d.x + fx * d.y), &s, &c);
+    const cplx<T> v = ld(G + ((size_t)b * M + m) * n + j);
+    sr.add(T(double(v.x) * c - double(v.y) * s));
+    si.add(T(double(v.x) * s + double(v.y) * c));
+  }
+  st(out + (size_t)m * n + j, cplx<T>{T(double(sr.s) * scale), T(double(si.s) * scale)});
+}
+
+// planes[k] *= q (CONJ: conj q) over `nplane` planes. Grid (ceil(n/256), nplane).
+template <typename T, bool CONJ>
+__global__ void k_times_q(cplx<T>* __restrict__ planes, const cplx<T>* __restrict__ q, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  cplx<T>* p = planes + (size_t)blockIdx.y * n + j;
+  const cplx<T> qj = ld(q + j);
+  st(p, CONJ ? cmulc(qj, ld(p)) : ld(p) * qj);
+}
+
+// The last slice's plane phase displaced by each slot's offset, q(r − d) =
+// q·exp(i Σ_a c_a·d_a·(d_a − 2 r_a)) over the centred pixel r. Grid (ceil(n/256), nb).
+// ptyrs/src/engine/propagate.rs:134-163
+template <typename T>
+__global__ void k_moved_quad(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ q,
+                             const double2* __restrict__ offs, const int* __restrict__ order,
+                             int N, double cy, double cx) {
+  const int n = N * N;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int b = blockIdx.y;
+  const double2 d = offs[order[b]];
+  const double ry = double(j / N - N / 2), rx = double(j % N - N / 2);
+  const double ph = cy * d.x * (d.x - 2.0 * ry) + cx * d.y * (d.y - 2.0 * rx);
+  double s, c;
+  sincos(ph, &s, &c);
+  st(out + (size_t)b * n + j, ld(q + j) * cplx<T>{T(c), T(s)});
+}
+
+// The object windows of the slots, one plane each. Grid (ceil(n/256), nb).
+template <typename T>
+__global__ void k_patch(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ O, int nxo,
+                        const int2* __restrict__ corner, const int* __restrict__ order, int N) {
+  const int n = N * N;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int b = blockIdx.y;
+  const int2 c = corner[order[b]];
+  st(out + (size_t)b * n + j, ld(O + (size_t)(c.x + j / N) * nxo + c.y + j % N));
+}
+
+// The window's Fourier derivatives: out[2b + a] = spec[b]·(2πi f_a)·scale. Grid (ceil(n/256), nb).
+template <typename T>
+__global__ void k_deriv(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ spec, int N,
+                        double scale) {
+  const int n = N * N;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int b = blockIdx.y;
+  const cplx<T> v = ld(spec + (size_t)b * n + j);
+  const double f[2] = {dft_freq(j / N, N), dft_freq(j % N, N)};
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const double w = 2.0 * M_PI * f[a] * scale;  // times i
+    st(out + ((size_t)2 * b + a) * n + j, cplx<T>{T(-double(v.y) * w), T(double(v.x) * w)});
+  }
+}
+
+// Per slot and workgroup the position sums of its first view (mode 0):
+// num_a = Re Σ conj(g_a·w·q)·χ, den_a = Σ |g_a·w·q|² over the workgroup's pixels, g_a the window's
+// derivative along axis a; w per slot and mode ([slot·M + m][n]), q nullable, per slot with
+// q_frame. part[b][blk] = {num_y, num_x, den_y, den_x}. Grid (nblk, nb).
+// ptyrs/src/engine/features.rs:457-497
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_pos_sums(
+    const cplx<T>* __restrict__ grad, const cplx<T>* __restrict__ w, const cplx<T>* __restrict__ q,
+    int q_frame, const cplx<T>* __restrict__ chi, int N, int M, double* __restrict__ part) {
+  __shared__ double red[THREADS];
+  const int b = blockIdx.y;
+  const int n = N * N;
+  double s[4] = {0, 0, 0, 0};
+  const size_t k = (size_t)b * M * n;
+  for (int j = blockIdx.x * THREADS + threadIdx.x; j < n; j += gridDim.x * THREADS) {
+    cplx<T> wq = ld(w + k + j);
+    if (q) wq = wq * ld(q + (q_frame ? (size_t)b * n : 0) + j);
+    const cplx<T> c = ld(chi + k + j);
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const cplx<T> u = ld(grad + ((size_t)2 * b + a) * n + j) * wq;
+      s[a] += double(cmulc(u, c).x);
+      s[2 + a] += double(norm2(u));
+    }
+  }
+  for (int i = 0; i < 4; ++i) {
+    double r = block_reduce<THREADS>(s[i], red, AddOp());
+    if (threadIdx.x == 0) part[((size_t)b * gridDim.x + blockIdx.x) * 4 + i] = r;
+  }
+}
+
+// One workgroup: the batch's position directions num/(den + max(max_batch den, eps)) and
+// pty-chi's SGD step 0.3·Δ clipped per axis to min(10·mean|dx − mean dx|, 0.1) over all nframe
+// frames (dx = 0 off the batch), added to the batch's offsets (FP64).
+// ptyrs/src/engine/features.rs:499-521
+__global__ __launch_bounds__(1024) static void k_pos_step(const double* __restrict__ part,
+                                                          int nblk, int nb, int nframe,
+                                                          double eps,
+                                                          const int* __restrict__ order,
+                                                          double2* __restrict__ offs,
+                                                          double* __restrict__ sums) {
+  __shared__ double red[1024];
+  const int tid = threadIdx.x;
+  double mden[2] = {eps, eps};
+  for (int b = tid; b < nb; b += 1024) {
+    double s[4] = {0, 0, 0, 0};
+    for (int i = 0; i < nblk; ++i)
+      for (int a = 0; a < 4; ++a) s[a] += part[((size_t)b * nblk + i) * 4 + a];
+    for (int a = 0; a < 4; ++a) sums[(size_t)b * 4 + a] = s[a];
+    for (int a = 0; a < 2; ++a) mden[a] = s[2 + a] > mden[a] ? s[2 + a] : mden[a];
+  }
+  double mx[2];
+  for (int a = 0; a < 2; ++a) mx[a] = block_reduce<1024>(mden[a], red, MaxOp());
+  // dx per slot, its sum over the batch (the mean over all frames has the others at 0)
+  double dsum[2] = {0, 0};
+  for (int b = tid; b < nb; b += 1024)
+    for (int a = 0; a < 2; ++a) {
+      const double dx = 0.3 * sums[(size_t)b * 4 + a] / (sums[(size_t)b * 4 + 2 + a] + mx[a]);
+      sums[(size_t)b * 4 + a] = dx;
+      dsum[a] += dx;
+    }
+  double mean[2];
+  for (int a = 0; a < 2; ++a) mean[a] = block_reduce<1024>(dsum[a], red, AddOp()) / nframe;
+  double dev[2] = {0, 0};
+  for (int b = tid; b < nb; b += 1024)
+    for (int a = 0; a < 2; ++a) dev[a] += fabs(sums[(size_t)b * 4 + a] - mean[a]);
+  double limit[2];
+  for (int a = 0; a < 2; ++a) {
+    const double mad = (block_reduce<1024>(dev[a], red, AddOp()) +
+                        double(nframe - nb) * fabs(mean[a])) / nframe;
+    limit[a] = 10.0 * mad < 0.1 ? 10.0 * mad : 0.1;
+  }
+  for (int b = tid; b < nb; b += 1024) {
+    double2 d = offs[order[b]];
+    double v[2];
+    for (int a = 0; a < 2; ++a) {
+      const double x = sums[(size_t)b * 4 + a];
+      v[a] = x < -limit[a] ? -limit[a] : (x > limit[a] ? limit[a] : x);
+    }
+    d.x += v[0];
+    d.y += v[1];
+    offs[order[b]] = d;
+  }
 }
 
 // ---- AP, DM, RAAR ----------------------------------------------------------------------------------

@@ -1,0 +1,139 @@
+"""LSQML with position offsets (pty-chi PATCH) on the GPU through the C ABI against the FP64
+oracle, whose position model and step the reference's fixtures pin
+(tests/test_oracle_positions.py): several epochs with the offsets refined, from the fixtures'
+truth object with every frame moved by its offset, near field (angular spectrum; angular
+spectrum with an evanescent band), three slices, and the far field; the LLK after every epoch,
+the object, probe and offsets at the end. c128 to FP64 rounding, c64 to 1e-3. Plus the feature
+API: held offsets stay, other algorithms and FullObject sampling are refused."""
+
+import numpy as np
+import pytest
+
+import ptyrs_amd as pa
+from conftest import max_rel, table
+from oracle.lsqml import POISSON, Lsqml, slice_llk
+from test_gpu_rpie import gpu, oracle, rel_l2, small_scan
+from test_oracle_lsqml import state as near_state
+from test_oracle_multislice import state as multislice_state
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"c128": 1e-9, "c64": 1e-3}
+
+
+def run_oracle(addr, optics, iobs, nb_obs, obj, probe, d0, nepoch, batch, seed):
+    run = Lsqml(batch, POISSON, seed, offsets=d0, refine=True)
+    llks = []
+    for _ in range(nepoch):
+        obj, probe, _ = run.epoch(addr, iobs, optics, obj, probe)
+        llks.append(slice_llk(optics, addr.views(), iobs, addr.nmode, nb_obs, obj, probe,
+                              run).poisson)
+    return obj, probe, run.offsets, llks
+
+
+def near_gpu(case, dtype, obj):
+    corners = np.array(table("lsqml_near_corners_f64.txt"))
+    c = table("lsqml_near_cases_f64.txt")[case]
+    counts = np.array([r[2:] for r in table("lsqml_near_counts_f64.txt") if r[0] == case])
+    counts = counts.reshape(len(corners), 24, 24).astype(np.float32)
+    probe = np.array([r[3] + 1j * r[4] for r in table("lsqml_near_probe_f64.txt")])
+    nf = len(corners)
+    scan = pa.Scan(counts, np.zeros(nf), np.zeros(nf), c[1], 1.0, 75e-6,
+                   corners=corners[:, 1:3].astype(np.int64))
+    w = pa.Wavefield([24, 24], c[1], [c[2], c[2]], c[3])
+    return pa.GpuPtycho(scan, obj, probe.reshape(2, 24, 24), optics=pa.Optics.free_space(w),
+                        dtype=dtype)
+
+
+def offsets_of(file, case):
+    return np.array([[r[2], r[3]] for r in table(file) if r[0] == case])
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("case", [0, 2])
+def test_near_field_positions_match_oracle(dtype, case):
+    addr, _, probe, cases = near_state()
+    start = np.array(table("lsqml_near_pos_start_f64.txt"))
+    obj = (start[:, 2] + 1j * start[:, 3]).reshape(1, 40, 40)
+    d0 = offsets_of("lsqml_near_pos_offsets_f64.txt", case)
+    optics, iobs, nb_obs, _ = cases[case]
+    o_obj, o_probe, o_d, o_llk = run_oracle(addr, optics, iobs, nb_obs, obj, probe, d0, 4, 8, 7)
+    g = near_gpu(case, dtype, obj)
+    if g.sampling != "patch":
+        g.set_sampling("patch")
+    g.set_features(position=d0)
+    g.lsqml(4, 1, 8, "poisson", 7)
+    tol = TOL[dtype]
+    assert max_rel(g.history["poisson"], o_llk) < tol
+    assert rel_l2(g.object, o_obj) < tol * 10
+    assert rel_l2(g.probe, o_probe) < tol * 10
+    got = g.features["position"]
+    assert max_rel((got - d0).ravel(), (o_d - d0).ravel()) < tol * 100
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_multislice_positions_match_oracle(dtype):
+    addr, objects, probe, cases = multislice_state()
+    start = np.array(table("lsqml_multislice_pos_start_f64.txt"))
+    objects = [(start[start[:, 0] == s][:, 3] + 1j * start[start[:, 0] == s][:, 4])
+               .reshape(1, 40, 40) for s in range(3)]
+    d0 = offsets_of("lsqml_multislice_pos_offsets_f64.txt", 0)
+    optics, iobs, nb_obs, _ = cases[0]
+    o_obj, o_probe, o_d, o_llk = run_oracle(addr, optics, iobs, nb_obs, list(objects), probe,
+                                            d0, 3, 8, 7)
+    from test_gpu_multislice import start as ms_start
+    g, _, _ = ms_start(0, dtype)
+    g.set_object(np.stack(objects))
+    g.set_sampling("patch")
+    g.set_features(position=d0)
+    g.lsqml(3, 1, 8, "poisson", 7)
+    tol = TOL[dtype]
+    assert max_rel(g.history["poisson"], o_llk) < tol
+    assert rel_l2(g.objects, np.stack(o_obj)) < tol * 10
+    assert rel_l2(g.probe, o_probe) < tol * 10
+    got = g.features["position"]
+    assert max_rel((got - d0).ravel(), (o_d - d0).ravel()) < tol * 100
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_far_field_positions_match_oracle(dtype):
+    d = small_scan(n=32, nprobe=2, npos=20)
+    o = oracle(d)
+    rng = np.random.default_rng(11)
+    d0 = rng.uniform(-0.6, 0.6, (20, 2))
+    o_obj, o_probe, o_d, o_llk = run_oracle(o.addr, o.optics, o.iobs, o.nb_obs, o.obj, o.probe,
+                                            d0, 3, 7, 5)
+    g = gpu(d, dtype)
+    assert g.sampling == "patch"
+    g.set_features(position=d0)
+    g.lsqml(3, 1, 7, "poisson", 5)
+    tol = TOL[dtype]
+    assert max_rel(g.history["poisson"], o_llk) < tol
+    assert rel_l2(g.object, o_obj) < tol * 10
+    assert rel_l2(g.probe, o_probe) < tol * 10
+    got = g.features["position"]
+    assert max_rel((got - d0).ravel(), (o_d - d0).ravel()) < tol * 100
+
+
+def test_position_feature_api():
+    d = small_scan(n=32, nprobe=1, npos=12)
+    g = gpu(d, "c64")
+    assert g.features["position"] is None
+    d0 = np.linspace(-0.4, 0.4, 24).reshape(12, 2)
+    g.set_features(position=d0, hold=["position"])
+    f = g.features
+    assert np.array_equal(f["position"], d0) and f["hold"] == ["position"]
+    g.lsqml(1, 1, 6, "poisson", 1)
+    assert np.array_equal(g.features["position"], d0)
+    for name in ("rpie", "ap", "dm", "raar", "ml"):
+        with pytest.raises(ValueError, match="position"):
+            getattr(g, name)(1)
+    with pytest.raises(ValueError):
+        g.set_sampling("full_object")
+    with pytest.raises(ValueError):
+        g.set_features(position=d0[:5])
+    with pytest.raises(ValueError):
+        g.set_features(scale=np.ones(12))
+    g.set_features()
+    assert g.features["position"] is None
+    g.rpie(1)
