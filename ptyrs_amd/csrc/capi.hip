@@ -106,14 +106,16 @@ Prepared prepare(const ptyrs_scan& s, const ptyrs_optics* optics, size_t nprobe,
   for (size_t f = 0; f < s.nframe; ++f) {
     for (int ky = 0; ky < N; ++ky)
       for (int kx = 0; kx < N; ++kx) {
-        // DFT index (ky, kx) sees detector pixel k; the inverse transform flips it.
+        // DFT index (ky, kx) sees detector pixel k; the inverse transform flips it (the angular
+        // spectrum's two forward transforms land the field at −k).
         int dy = ky, dx = kx;
         if (p.exit.reverse) {
           dy = (N - ky) % N;
           dx = (N - kx) % N;
         }
-        // Shifted (DFT-order) index to detector order: (d + N/2) mod N.
-        const size_t src = f * n + size_t((dy + N / 2) % N) * N + (dx + N / 2) % N;
+        // A DFT plane's index to detector order, (d + N/2) mod N; a spatial plane's is its own.
+        const int h = p.exit.dft_plane ? N / 2 : 0;
+        const size_t src = f * n + size_t((dy + h) % N) * N + (dx + h) % N;
         float v;
         if (s.counts_u16) {
           const uint16_t c = s.counts_u16[src];

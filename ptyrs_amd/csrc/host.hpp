@@ -102,6 +102,10 @@ inline std::vector<std::vector<int>> random_batches(size_t nframe, size_t size, 
 struct ExitModel {
   std::vector<double> quad;  // interleaved re, im; n values
   bool reverse = false;      // the detector sees the inverse DFT (Fresnel with z < 0)
+  // The detector plane is a DFT of the exit wave (far field, Fresnel: its frame is stored
+  // fftshifted against the DFT index), or a plane in the exit wave's own layout (angular
+  // spectrum: the frame's natural index is the field's).
+  bool dft_plane = true;
   double pixel[2] = {0, 0};  // object-plane pixel [py, px], metres
   int kind = 0;              // PTYRS_FOURIER / PTYRS_FRESNEL / PTYRS_ANGULAR_SPECTRUM
   // pty-chi's intensity scale k of the propagator (ptyrs/src/engine/propagate.rs:373-385):
@@ -204,6 +208,7 @@ inline ExitModel free_space(int ny, int nx, double wavelength, double py, double
     // counts are stored index-reversed.
     m.kind = PTYRS_ANGULAR_SPECTRUM;
     m.reverse = true;
+    m.dft_plane = false;
     m.scale = 1.0;
     m.tf = angular_spectrum_tf(ny, nx, wavelength, py, px, distance, &m.banded);
     return m;
