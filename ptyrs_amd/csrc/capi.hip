@@ -181,6 +181,7 @@ int ptyrs_engine_create(const ptyrs_scan* scan, const ptyrs_optics* optics, cons
     const int dtype = options ? options->dtype : PTYRS_C64;
     if (dtype != PTYRS_C64 && dtype != PTYRS_C128) throw Invalid("dtype must be c64 or c128");
     Prepared pre = prepare(*scan, optics, nprobe, nobj, nyo, nxo);
+    pre.graph = options ? options->graph != 0 : true;
     auto h = std::make_unique<ptyrs_engine>();
     h->e.reset(make_engine(dtype, pre, obj, nobj, nyo, nxo, probe, nprobe, device));
     *out = h.release();

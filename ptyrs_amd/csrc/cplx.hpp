@@ -53,6 +53,39 @@ __device__ __forceinline__ void st(cplx<T>* p, cplx<T> a) {
   *reinterpret_cast<typename vec2<T>::type*>(p) = v;
 }
 
+// Buffer-resource access: a uniform base in SGPRs (the resource), a per-lane 32-bit byte
+// offset and a uniform byte offset, so that a kernel's many loads of one plane share one
+// address VGPR. Accesses past `bytes` read 0 and write nothing. The third resource dword is
+// the raw-buffer format word of gfx9-family GPUs (CDNA).
+struct Buf {
+  __amdgpu_buffer_rsrc_t r;
+};
+__device__ __forceinline__ Buf buf(const void* base, uint32_t bytes) {
+  return Buf{__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, int(bytes),
+                                               0x00020000)};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> bld(Buf b, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(cplx<T>, __builtin_amdgcn_raw_buffer_load_b64(b.r, voff, soff, 0));
+  } else {
+    return __builtin_bit_cast(cplx<T>, __builtin_amdgcn_raw_buffer_load_b128(b.r, voff, soff, 0));
+  }
+}
+__device__ __forceinline__ float bld1(Buf b, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, voff, soff, 0));
+}
+template <typename T>
+__device__ __forceinline__ void bst(Buf b, uint32_t voff, uint32_t soff, cplx<T> a) {
+  if constexpr (sizeof(T) == 4) {
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(b.r, 0u, 0u, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, a), b.r, voff, soff, 0);
+  } else {
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b128(b.r, 0u, 0u, 0));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, a), b.r, voff, soff, 0);
+  }
+}
+
 // Kahan-compensated accumulator: the running sum and the lost low-order part.
 template <typename T>
 struct kahan {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -31,6 +32,8 @@ inline const char* kind_names() {
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  // Bumped on every (re)allocation: captured graphs hold device addresses.
+  static inline std::atomic<uint64_t> generation{0};
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
@@ -44,6 +47,7 @@ struct DevBuf {
     bytes = 0;
     PTYRS_HIP(hipMalloc(&p, want));
     bytes = want;
+    ++generation;
   }
   template <typename U>
   U* as() const { return static_cast<U*>(p); }
@@ -63,6 +67,7 @@ struct Prepared {
   size_t nb_obs = 0;
   double total = 0;
   double max_frame_total = 0;  // max over frames of the unmasked counts' sum
+  bool graph = true;           // GpuOptions::graph: replay captured epochs
   ExitModel exit;
   // Multislice: object slices and, per gap, its angular-spectrum transfer function.
   size_t nslice = 1;
@@ -87,10 +92,23 @@ struct StepRecord {
   std::vector<double> a_objs;  // every slice's a_obj, slice 0's first
 };
 
+// A captured epoch (GpuOptions::graph, ptyrs/src/engine/gpu/mod.rs:110-179): one graph per
+// epoch kind, keyed by what its launches bake in (shapes, parameters, buffer generation).
+struct EpochGraph {
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint64_t> key;
+  ~EpochGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
+
 class EngineBase {
  public:
   virtual ~EngineBase() {
-    if (stream_) (void)hipStreamDestroy(stream_);
+    if (stream_) {
+      (void)hipStreamSynchronize(stream_);
+      (void)hipStreamDestroy(stream_);
+    }
   }
   virtual void rpie(const ptyrs_rpie_params& p, size_t nepoch) = 0;
   virtual void lsqml(const ptyrs_lsqml_params& p, size_t nepoch, size_t calc_llk) = 0;
@@ -139,6 +157,45 @@ class EngineBase {
   size_t nb_obs_ = 0;
 
   bool profiling_ = false;
+  bool graph_ = true;
+  EpochGraph rpie_graph_, lsqml_graph_;
+
+  // Replays `g` if its key matches, else captures `body` (the launches of one epoch on the
+  // engine's stream, no allocation, memset or host wait inside) into it and launches it.
+  // Profiling runs `body` eagerly, its launches timed one by one.
+  template <typename F>
+  void epoch(EpochGraph& g, std::vector<uint64_t> key, F&& body) {
+    if (!graph_ || profiling_) {
+      body();
+      return;
+    }
+    key.push_back(DevBuf::generation.load());
+    if (g.exec == nullptr || g.key != key) {
+      if (g.exec) {
+        PTYRS_HIP(hipGraphExecDestroy(g.exec));
+        g.exec = nullptr;
+      }
+      hipGraph_t graph = nullptr;
+      PTYRS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+      try {
+        body();
+      } catch (...) {
+        hipGraph_t dropped = nullptr;
+        (void)hipStreamEndCapture(stream_, &dropped);
+        if (dropped) (void)hipGraphDestroy(dropped);
+        throw;
+      }
+      PTYRS_HIP(hipStreamEndCapture(stream_, &graph));
+      const hipError_t inst = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (inst != hipSuccess) {
+        g.exec = nullptr;
+        throw DeviceError(std::string("hipGraphInstantiate: ") + hipGetErrorString(inst));
+      }
+      g.key = std::move(key);
+    }
+    PTYRS_HIP(hipGraphLaunch(g.exec, stream_));
+  }
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending_;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pool_;
   std::pair<size_t, double> stats_[K_COUNT] = {};
@@ -205,6 +262,7 @@ class Engine : public EngineBase {
     total_ = pre.total;
     M_ = int(nprobe * nobj);
     nslice_ = pre.nslice;
+    graph_ = pre.graph;
     plane_quad_[0] = pre.exit.plane_quad[0];
     plane_quad_[1] = pre.exit.plane_quad[1];
     // Sampling::auto (ptyrs/src/engine/sampling.rs:38-49): FullObject in the near field when
@@ -315,10 +373,21 @@ class Engine : public EngineBase {
     PTYRS_HIP(hipMemcpyAsync(order_.p, order.data(), sizeof(int) * order.size(),
                              hipMemcpyHostToDevice, stream_));
     probe_terms();
+    // Every epoch has the same batch lengths; its order is copied into the captured order
+    // buffer ahead of the replay.
+    const std::vector<uint64_t> base = {uint64_t(p.has_obj_step), uint64_t(p.has_probe_step),
+                                        bits(p.obj_step), bits(p.probe_step), bits(p.obj_alpha),
+                                        bits(p.probe_alpha), cap_};
     std::vector<std::pair<size_t, int>> due;  // (cycle, slot)
     for (size_t ic = 0; ic < nepoch; ++ic) {
-      const int* ord = order_.as<int>() + ic * nframe_;
-      for (auto& s : spans[ic]) rpie_batch(p, ord + s.first, s.second);
+      PTYRS_HIP(hipMemcpyAsync(eord_.p, order_.as<int>() + ic * nframe_, sizeof(int) * nframe_,
+                               hipMemcpyDeviceToDevice, stream_));
+      const int* ord = eord_.as<int>();
+      std::vector<uint64_t> key = base;
+      for (auto& s : spans[ic]) key.push_back(uint64_t(s.second));
+      epoch(rpie_graph_, key, [&] {
+        for (auto& s : spans[ic]) rpie_batch(p, ord + s.first, s.second);
+      });
       const bool rec = p.calc_llk > 0 && (ic % p.calc_llk == 0 || ic + 1 == nepoch);
       if (rec) {
         if (due.size() == kHistSlots) drain(due, PTYRS_RPIE);
@@ -375,12 +444,17 @@ class Engine : public EngineBase {
     std::vector<std::pair<size_t, int>> due;
     for (size_t ic = 0; ic < nepoch; ++ic) {
       if (ic < 10 || ic % 10 == 0) precondition(ic == 0);
-      const int* ord = order_.as<int>() + ic * nframe_;
-      for (auto& s : spans[ic]) {
-        if (nslice_ > 1 || pos_on_)
-          lsqml_batch_ms(p, ord + s.first, s.second, s.first);
-        else
-          lsqml_batch(p, ord + s.first, s.second, s.first);
+      PTYRS_HIP(hipMemcpyAsync(eord_.p, order_.as<int>() + ic * nframe_, sizeof(int) * nframe_,
+                               hipMemcpyDeviceToDevice, stream_));
+      const int* ord = eord_.as<int>();
+      if (nslice_ > 1 || pos_on_) {
+        for (auto& s : spans[ic]) lsqml_batch_ms(p, ord + s.first, s.second, s.first);
+      } else {
+        std::vector<uint64_t> key = {uint64_t(p.cost), cap_};
+        for (auto& s : spans[ic]) key.push_back(uint64_t(s.second));
+        epoch(lsqml_graph_, key, [&] {
+          for (auto& s : spans[ic]) lsqml_batch(p, ord + s.first, s.second, s.first);
+        });
       }
       const bool rec = calc_llk > 0 && (ic % calc_llk == 0 || ic + 1 == nepoch);
       if (rec) {
@@ -557,7 +631,7 @@ class Engine : public EngineBase {
         hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
                            corner_.as<int2>(), iota_.as<int>(), int(nframe_), N, ntx_,
                            int(nframe_), list_all_.as<int>(), count_all_.as<int>(),
-                           (const cplx<T>*)nullptr, M_, int(nyo_), int(nxo_), (T*)nullptr,
+                           (const T*)nullptr, M_, int(nyo_), int(nxo_), (T*)nullptr,
                            tilemax_.as<T>());
       });
     }
@@ -884,7 +958,13 @@ class Engine : public EngineBase {
       sums_, log_, step_, pre_part_, floor_, dPm_;
   int ntx_ = 0, nty_ = 0;
   size_t cap_ = 0, nchunk_cap_ = 0, order_cap_ = 0;
-  DevBuf O_, P_, Wt_, q_, tw_, counts_, corner_, iota_, hist_;
+  DevBuf O_, P_, Wt_, W2_, q_, tw_, counts_, corner_, iota_, hist_, eord_;
+
+  static uint64_t bits(double v) {
+    uint64_t u;
+    std::memcpy(&u, &v, sizeof u);
+    return u;
+  }
   DevBuf psi_, omax_, dP_, list_, lcount_, tilemax_, pmax_, order_, llk_part_, D_;
 
   void upload_complex(DevBuf& d, const double* src, size_t count) {
@@ -941,7 +1021,7 @@ class Engine : public EngineBase {
       run(K_BIN, [&] {
         hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
                            corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
-                           lcount_.as<int>(), (const cplx<T>*)nullptr, M_, int(nyo_),
+                           lcount_.as<int>(), (const T*)nullptr, M_, int(nyo_),
                            int(nxo_), (T*)nullptr, tilemax_.as<T>());
       });
       run(K_OTHER, [&] {
@@ -967,7 +1047,7 @@ class Engine : public EngineBase {
     run(K_BIN, [&] {
       hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
                          corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
-                         lcount_.as<int>(), (const cplx<T>*)nullptr, M_, int(nyo_), int(nxo_),
+                         lcount_.as<int>(), (const T*)nullptr, M_, int(nyo_), int(nxo_),
                          (T*)nullptr, tilemax_.as<T>());
     });
     pass_a(ord, nb, false);
@@ -1059,7 +1139,7 @@ class Engine : public EngineBase {
     run(K_BIN, [&] {
       hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
                          corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
-                         lcount_.as<int>(), (const cplx<T>*)nullptr, M_, int(nyo_), int(nxo_),
+                         lcount_.as<int>(), (const T*)nullptr, M_, int(nyo_), int(nxo_),
                          (T*)nullptr, tilemax_.as<T>());
     });
     const Waves wv = stack_waves(ord, nb);
@@ -1233,10 +1313,11 @@ class Engine : public EngineBase {
   void object_blend(double inertia) {
     const int nt = ntiles();
     run(K_BIN, [&] {
+      norm2_planes();
       hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
                          corner_.as<int2>(), iota_.as<int>(), int(nframe_), N, ntx_,
                          int(nframe_), list_all_.as<int>(), count_all_.as<int>(),
-                         Wt_.as<cplx<T>>(), M_, int(nyo_), int(nxo_), D_.as<T>(),
+                         (const T*)W2_.as<T>(), M_, int(nyo_), int(nxo_), D_.as<T>(),
                          tilemax_.as<T>());
     });
     run(K_OVERLAP, [&] {
@@ -1331,12 +1412,21 @@ class Engine : public EngineBase {
     lcount_.ensure(sizeof(int) * size_t(ntiles()));
     tilemax_.ensure(sizeof(T) * size_t(ntiles()));
     D_.ensure(sizeof(T) * size_t(M_) * nyo_ * nxo_);
+    W2_.ensure(sizeof(T) * size_t(M_) * n);
     pmax_.ensure(sizeof(T) * 1);
     order_.ensure(sizeof(int) * std::max<size_t>(1, nepoch * nframe_));
+    eord_.ensure(sizeof(int) * std::max<size_t>(1, nframe_));
     llk_part_.ensure(sizeof(double) * 4 * nframe_ * SC::NBLK);
     cap_ = std::max(cap_, cap);
     if (nslice_ > 1) inc_.ensure(sizeof(cplx<T>) * (nslice_ - 1) * cap_ * M_ * n);
     ensure_positions(cap_);
+  }
+
+  // |W_m|² of the current probe terms into W2_, for the per-mode overlaps.
+  void norm2_planes() {
+    const int total = int(M_) * n;
+    hipLaunchKernelGGL(k_norm2_planes<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,
+                       (const cplx<T>*)Wt_.as<cplx<T>>(), W2_.as<T>(), total);
   }
 
   void probe_terms() {
@@ -1435,9 +1525,10 @@ class Engine : public EngineBase {
     const int nt = ntiles();
     if (upd_obj) {
       run(K_BIN, [&] {
+        norm2_planes();
         hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
                            corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
-                           lcount_.as<int>(), Wt_.as<cplx<T>>(), M_, int(nyo_), int(nxo_),
+                           lcount_.as<int>(), (const T*)W2_.as<T>(), M_, int(nyo_), int(nxo_),
                            D_.as<T>(), tilemax_.as<T>());
       });
       run(K_OVERLAP, [&] {
@@ -1454,12 +1545,12 @@ class Engine : public EngineBase {
                          stream_, psi_.as<cplx<T>>(), O_.as<cplx<T>>(), int(nxo_),
                          Wt_.as<cplx<T>>(), q_.as<cplx<T>>(), corner_.as<int2>(), ord, nb, M_,
                          omax_.as<T>(), int(S::NBLK), chunk, T(p.probe_alpha), int(upd_probe),
-                         dP_.as<cplx<T>>(), tw_.as<cplx<T>>(), 1);
+                         dP_.as<cplx<T>>(), tw_.as<cplx<T>>(), 1, int(upd_obj));
     });
     if (upd_obj) {
       run(K_OBJ_UPDATE, [&] {
         hipLaunchKernelGGL(k_obj_update_rpie<T>, dim3(nt), dim3(THREADS), 0, stream_,
-                           O_.as<cplx<T>>(), Wt_.as<cplx<T>>(), D_.as<T>(),
+                           O_.as<cplx<T>>(), D_.as<T>(),
                            psi_.as<cplx<T>>(), N, M_, corner_.as<int2>(), ord, list_.as<int>(),
                            lcount_.as<int>(), int(cap_), int(nyo_), int(nxo_), ntx_,
                            pmax_.as<T>(), T(p.obj_alpha), T(p.obj_step));

@@ -64,18 +64,53 @@ using RowShape = Lines<N, choose_e(N, sizeof(T) == 4 ? 16 : 8), THREADS, N>;
 // Pass C loops over frames with its probe terms held: 8 values per thread (float), 4 (double).
 template <typename T, int N>
 using RowShapeC = Lines<N, choose_e(N, sizeof(T) == 4 ? 8 : 4), THREADS, N>;
-// The column pass holds every mode of a 128-byte stripe of columns (16 float or 8 double
-// columns): 8 values per thread (float), 4 (double), so that four modes stay in registers at
-// several waves per SIMD.
+// The column pass holds every mode of a 64-byte stripe of columns (8 float or 4 double
+// columns): 8 values per thread (float), 4 (double), so that four modes stay in registers with
+// two workgroups per CU; neighbouring stripes share their 128-byte lines through L2
+// (block_xy).
 #ifndef PTYRS_COL_LINES_F32
-#define PTYRS_COL_LINES_F32 16
+#define PTYRS_COL_LINES_F32 8
 #endif
 #ifndef PTYRS_COL_LINES_F64
-#define PTYRS_COL_LINES_F64 8
+#define PTYRS_COL_LINES_F64 4
 #endif
 template <typename T, int N>
 using ColShape = Lines<N, choose_e(N, sizeof(T) == 4 ? 8 : 4), 512,
                        sizeof(T) == 4 ? PTYRS_COL_LINES_F32 : PTYRS_COL_LINES_F64>;
+
+// ---- workgroup placement --------------------------------------------------------------------------
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2): dispatch slot L runs on
+// XCD L % 8. The column passes renumber a 2D grid whose size divides by 8 so that consecutive
+// logical blocks (neighbouring column stripes of a plane, which share 128-byte lines) run on one
+// XCD at about the same time and share its L2.
+#ifndef PTYRS_XCD_REMAP
+#define PTYRS_XCD_REMAP 1
+#endif
+__device__ __forceinline__ int2 block_xy() {
+  if constexpr (PTYRS_XCD_REMAP) {
+    const int total = gridDim.x * gridDim.y;
+    const int L = blockIdx.y * gridDim.x + blockIdx.x;
+    if (total % 8 == 0) {
+      const int id = (L % 8) * (total / 8) + L / 8;
+      return int2{id % int(gridDim.x), id / int(gridDim.x)};
+    }
+  }
+  return int2{int(blockIdx.x), int(blockIdx.y)};
+}
+
+// The object tile of a tile kernel (1D grid of tiles, row-major): with PTYRS_TILE_REMAP each
+// XCD takes a contiguous band of tiles, whose views read nearby probe terms through one L2.
+#ifndef PTYRS_TILE_REMAP
+#define PTYRS_TILE_REMAP 0
+#endif
+__device__ __forceinline__ int tile_id() {
+  if constexpr (PTYRS_TILE_REMAP) {
+    const int total = gridDim.x;
+    const int L = blockIdx.x;
+    if (total % 8 == 0) return (L % 8) * (total / 8) + L / 8;
+  }
+  return int(blockIdx.x);
+}
 
 // ---- block reductions --------------------------------------------------------------------------
 template <int NT, typename V, typename Op>
@@ -118,6 +153,13 @@ __global__ void k_probe_terms(const cplx<T>* __restrict__ P, const cplx<T>* __re
                               cplx<T>* __restrict__ Wt, int n, int total) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < total) st(Wt + i, ld(P + i) * ld(q + i % n));
+}
+
+// W2[i] = |Wt[i]|² over the M planes of the probe terms (the per-mode overlap's summands).
+template <typename T>
+__global__ void k_norm2_planes(const cplx<T>* __restrict__ Wt, T* __restrict__ W2, int total) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < total) W2[i] = norm2(ld(Wt + i));
 }
 
 // ---- pass A: gather O·P·q and transform rows ---------------------------------------------------
@@ -220,22 +262,28 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
   // Columns: lanes run along the 128-byte stripe of lines; rows: along each line.
   const int tid = threadIdx.x;
   const int line = ROW ? tid / TL : tid % CG, jl = ROW ? tid % TL : tid / CG;
-  const int x = blockIdx.x * CG + line;
-  const int b = blockIdx.y;
+  const int2 bxy = block_xy();
+  const int x = bxy.x * CG + line;
+  const int b = bxy.y;
   const int f = order[b];
   T* re = lre + line * S::LD;
   T* im = lim + line * S::LD;
   auto at = [&](int e) { return ROW ? x * N + jl + TL * e : (jl + TL * e) * N + x; };
-  const float* cnt = counts + (size_t)f * n;
+  // Element e of the lane is at index i0 + e·step of every plane: one offset VGPR for all
+  // modes (buffer accesses), the step in SGPRs.
+  const uint32_t i0 = uint32_t(ROW ? x * N + jl : jl * N + x);
+  constexpr uint32_t step = ROW ? TL : TL * N;
+  constexpr uint32_t cb = sizeof(cplx<T>);
+  const Buf cnt = buf(counts + (size_t)f * n, n * 4u);
   float cv[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) cv[e] = cnt[at(e)];
+  for (int e = 0; e < E; ++e) cv[e] = bld1(cnt, i0 * 4u, e * step * 4u);
   cplx<T> v[M][E];
 #pragma unroll
   for (int m = 0; m < M; ++m) {
-    const cplx<T>* plane = psi + (size_t)(b * M + m) * n;
+    const Buf plane = buf(psi + (size_t)(b * M + m) * n, n * cb);
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[m][e] = ld(plane + at(e));
+    for (int e = 0; e < E; ++e) v[m][e] = bld<T>(plane, i0 * cb, e * step * cb);
   }
   stage_twiddles<T, N, S::NT>(stw, tw);
 #pragma unroll
@@ -312,22 +360,22 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       double s = block_reduce<S::NT>(acc[i], red, AddOp());
-      if (tid == 0) llk_part[((size_t)f * S::NBLK + blockIdx.x) * 4 + i] = s;
+      if (tid == 0) llk_part[((size_t)f * S::NBLK + bxy.x) * 4 + i] = s;
     }
   } else if constexpr (MODE == B_ICALC) {
     __shared__ double red[S::NT];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       double s = block_reduce<S::NT>(acc[i], red, AddOp());
-      if (tid == 0) ba.part[((size_t)b * S::NBLK + blockIdx.x) * 2 + i] = s;
+      if (tid == 0) ba.part[((size_t)b * S::NBLK + bxy.x) * 2 + i] = s;
     }
   } else {
 #pragma unroll
     for (int m = 0; m < M; ++m) {
       line_fft<T, N, E, true>(v[m], jl, re, im, stw);
-      cplx<T>* plane = psi + (size_t)(b * M + m) * n;
+      const Buf plane = buf(psi + (size_t)(b * M + m) * n, n * cb);
 #pragma unroll
-      for (int e = 0; e < E; ++e) st(plane + at(e), v[m][e]);
+      for (int e = 0; e < E; ++e) bst<T>(plane, i0 * cb, e * step * cb, v[m][e]);
     }
   }
 }
@@ -346,8 +394,9 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_t(cplx<T>* __rest
   __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
   __shared__ cplx<T> stw[N];
   const int tid = threadIdx.x, col = tid % CG, jl = tid / CG;
-  const int x = blockIdx.x * CG + col;
-  cplx<T>* plane = psi + blockIdx.z * field + (size_t)blockIdx.y * N * N;
+  const int2 bxy = block_xy();
+  const int x = bxy.x * CG + col;
+  cplx<T>* plane = psi + blockIdx.z * field + (size_t)bxy.y * N * N;
   cplx<T> v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = ld(plane + (jl + TL * e) * N + x);
@@ -367,9 +416,10 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_t(cplx<T>* __rest
 
 // ---- pass C: inverse rows, Δ = ψ' − W·O, probe direction ----------------------------------------
 // Grid (NBLK, M, nchunk). Each workgroup owns rows of mode m and runs over the batch slots of
-// its chunk in order: it transforms the rows back, stores Δ (rPIE) or χ (LSQML) over the
-// projected field, and adds conj(O·q)·Δ, divided by (1 − α)|O|² + α·max_window |O|² when
-// `weighted` (rPIE), into a per-chunk probe direction. The window max comes from pass A's
+// its chunk in order: it transforms the rows back, stores χ = Δ (LSQML) or, with store_wd, the
+// object step's numerator term conj(W_m)·Δ (rPIE) over the projected field, and adds
+// conj(O·q)·Δ, divided by (1 − α)|O|² + α·max_window |O|² when `weighted` (rPIE), into a
+// per-chunk probe direction. The window max comes from pass A's
 // `na` partial maxima per slot.
 template <typename T, int N>
 __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie(
@@ -377,7 +427,8 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie(
     const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ q,
     const int2* __restrict__ corner, const int* __restrict__ order, int nb, int M,
     const T* __restrict__ omax_part, int na, int chunk, T alpha, int probe_dir,
-    cplx<T>* __restrict__ dP_part, const cplx<T>* __restrict__ tw, int weighted) {
+    cplx<T>* __restrict__ dP_part, const cplx<T>* __restrict__ tw, int weighted,
+    int store_wd = 0) {
   using S = RowShapeC<T, N>;
   constexpr int E = S::E, TL = S::TL;
   constexpr int n = N * N;
@@ -441,7 +492,7 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie(
       const int x = jl + TL * e;
       const cplx<T> o = ow[e];
       const cplx<T> d = v[e] - w[e] * o;
-      st(prow + x, d);
+      st(prow + x, store_wd ? cmulc(w[e], d) : d);
       if (probe_dir) {
         const cplx<T> g = cmulc(o * qq[e], d);
         acc[e] = acc[e] + (weighted ? g * (T(1) / (inv * norm2(o) + alpha * omax)) : g);
@@ -485,12 +536,12 @@ constexpr int TILE_X = 64, TILE_Y = 4, TROWS = TILE_Y / (THREADS / TILE_X);
 template <typename T>
 __global__ __launch_bounds__(THREADS) void k_bin_tiles(
     const int2* __restrict__ corner, const int* __restrict__ order, int nb, int N, int ntx,
-    int cap, int* __restrict__ list, int* __restrict__ count, const cplx<T>* __restrict__ Wt,
+    int cap, int* __restrict__ list, int* __restrict__ count, const T* __restrict__ W2,
     int M, int nyo, int nxo, T* __restrict__ D, T* __restrict__ tilemax) {
   __shared__ int wave_n[THREADS / 64];
   __shared__ int base_s;
   __shared__ T red[THREADS];
-  const int t = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int t = tile_id(), tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ty0 = (t / ntx) * TILE_Y, tx0 = (t % ntx) * TILE_X;
   int* out = list + (size_t)t * cap;
   if (tid == 0) base_s = 0;
@@ -519,7 +570,7 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
   }
   const int nl = base_s;
   if (tid == 0) count[t] = nl;
-  if (Wt == nullptr) return;
+  if (W2 == nullptr) return;
   __shared__ int2 cs[THREADS];
   constexpr int MM = 4;
   const int X = tx0 + tid % TILE_X, Y0 = ty0 + tid / TILE_X;
@@ -540,7 +591,7 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
           if (y >= 0 && y < N) {
 #pragma unroll
             for (int m = 0; m < MM; ++m)
-              if (m < M) dm[r][m] += norm2(ld(Wt + (size_t)m * n + y * N + x));
+              if (m < M) dm[r][m] += W2[(size_t)m * n + y * N + x];
           }
         }
       }
@@ -581,18 +632,18 @@ __global__ __launch_bounds__(THREADS) void k_max_reduce(const T* __restrict__ v,
 
 // rPIE object step, pass 2: every object pixel owned by one thread adds, in slot order,
 // O += step · Σ_m Σ_views conj(W_m)·Δ_m / ((1 − α)·D_m + α·pmax) with D_m the batch overlap
-// of |W_m|² at the pixel (from k_bin_tiles), the sums Kahan-compensated. No atomics; the result
-// does not depend on the launch shape.
+// of |W_m|² at the pixel (from k_bin_tiles) and conj(W_m)·Δ_m as pass C stored it, the sums
+// Kahan-compensated. No atomics; the result does not depend on the launch shape.
 template <typename T>
 __global__ __launch_bounds__(THREADS) void k_obj_update_rpie(
-    cplx<T>* __restrict__ O, const cplx<T>* __restrict__ Wt, const T* __restrict__ D,
-    const cplx<T>* __restrict__ delta, int N, int M, const int2* __restrict__ corner,
+    cplx<T>* __restrict__ O, const T* __restrict__ D, const cplx<T>* __restrict__ wdelta, int N,
+    int M, const int2* __restrict__ corner,
     const int* __restrict__ order, const int* __restrict__ list, const int* __restrict__ count,
     int cap, int nyo, int nxo, int ntx, const T* __restrict__ pmax, T alpha, T step) {
   constexpr int MM = 4;
   constexpr int RS = THREADS / TILE_X;
   __shared__ int3 cs[THREADS];
-  const int t = blockIdx.x, tid = threadIdx.x;
+  const int t = tile_id(), tid = threadIdx.x;
   const int X = (t % ntx) * TILE_X + tid % TILE_X, Y0 = (t / ntx) * TILE_Y + tid / TILE_X;
   const int nl = count[t];
   if (nl == 0) return;
@@ -608,30 +659,36 @@ __global__ __launch_bounds__(THREADS) void k_obj_update_rpie(
       cs[tid] = int3{b, c.x, c.y};
     }
     __syncthreads();
-    for (int i = 0; i < k && X < nxo; ++i) {
-      const int x = X - cs[i].z;
-      if (x < 0 || x >= N) continue;
-      const cplx<T>* dv = delta + (size_t)cs[i].x * M * n;
+    // SU slots per step: their loads are issued together, then added in slot order.
+    constexpr int SU = 4;
+    for (int i0 = 0; i0 < k && X < nxo; i0 += SU) {
 #pragma unroll
       for (int r = 0; r < TROWS; ++r) {
-        const int y = Y0 + RS * r - cs[i].y;
-        if (y < 0 || y >= N) continue;
-        const int j = y * N + x;
-        cplx<T> dl[MM], w[MM];
+        cplx<T> g[SU][MM];
+        bool ok[SU];
 #pragma unroll
-        for (int m = 0; m < MM; ++m)
-          if (m < M) {
-            dl[m] = ld(dv + (size_t)m * n + j);
-            w[m] = ld(Wt + (size_t)m * n + j);
-          }
+        for (int u = 0; u < SU; ++u) {
+          const int i = i0 + u < k ? i0 + u : k - 1;  // cs holds k entries
+          const int x = X - cs[i].z, y = Y0 + RS * r - cs[i].y;
+          ok[u] = i0 + u < k && x >= 0 && x < N && y >= 0 && y < N;
+          if (ok[u]) {
+            const cplx<T>* dv = wdelta + (size_t)cs[i].x * M * n + y * N + x;
 #pragma unroll
-        for (int m = 0; m < MM; ++m)
-          if (m < M) {
-            const cplx<T> g = cmulc(w[m], dl[m]);
-            nr[r][m].add(g.x);
-            ni[r][m].add(g.y);
+            for (int m = 0; m < MM; ++m)
+              if (m < M) g[u][m] = ld(dv + (size_t)m * n);
           }
-        covered[r] = true;
+        }
+#pragma unroll
+        for (int u = 0; u < SU; ++u)
+          if (ok[u]) {
+#pragma unroll
+            for (int m = 0; m < MM; ++m)
+              if (m < M) {
+                nr[r][m].add(g[u][m].x);
+                ni[r][m].add(g[u][m].y);
+              }
+            covered[r] = true;
+          }
       }
     }
     __syncthreads();
@@ -733,7 +790,7 @@ __global__ __launch_bounds__(THREADS) void k_lsqml_obj_dir(
     const cplx<T>* __restrict__ q = nullptr, int q_frame = 0) {
   constexpr int RS = THREADS / TILE_X;
   __shared__ int3 cs[THREADS];
-  const int t = blockIdx.x, tid = threadIdx.x;
+  const int t = tile_id(), tid = threadIdx.x;
   const int X = (t % ntx) * TILE_X + tid % TILE_X, Y0 = (t / ntx) * TILE_Y + tid / TILE_X;
   const int nl = count[t];
   const int n = N * N;
@@ -933,7 +990,7 @@ __global__ __launch_bounds__(THREADS) void k_illumination(
     const int* __restrict__ count, int cap, int nyo, int nxo, int ntx) {
   constexpr int RS = THREADS / TILE_X;
   __shared__ int2 cs[THREADS];
-  const int t = blockIdx.x, tid = threadIdx.x;
+  const int t = tile_id(), tid = threadIdx.x;
   const int X = (t % ntx) * TILE_X + tid % TILE_X, Y0 = (t / ntx) * TILE_Y + tid / TILE_X;
   const int nl = count[t];
   const int n = N * N;
@@ -1039,8 +1096,9 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_gap_cols(cplx<T>* __re
   __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
   __shared__ cplx<T> stw[N];
   const int tid = threadIdx.x, col = tid % CG, jl = tid / CG;
-  const int x = blockIdx.x * CG + col;
-  cplx<T>* plane = psi + (size_t)blockIdx.y * N * N;
+  const int2 bxy = block_xy();
+  const int x = bxy.x * CG + col;
+  cplx<T>* plane = psi + (size_t)bxy.y * N * N;
   cplx<T> v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = ld(plane + (jl + TL * e) * N + x);
@@ -1184,8 +1242,9 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_cols(cplx<T>* __restri
   __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
   __shared__ cplx<T> stw[N];
   const int tid = threadIdx.x, col = tid % CG, jl = tid / CG;
-  const int x = blockIdx.x * CG + col;
-  cplx<T>* plane = psi + (size_t)blockIdx.y * N * N;
+  const int2 bxy = block_xy();
+  const int x = bxy.x * CG + col;
+  cplx<T>* plane = psi + (size_t)bxy.y * N * N;
   cplx<T> v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = ld(plane + (jl + TL * e) * N + x);
@@ -1539,7 +1598,7 @@ __global__ __launch_bounds__(THREADS) void k_obj_blend(
     const int* __restrict__ order, const int* __restrict__ list, const int* __restrict__ count,
     int cap, int nyo, int nxo, int ntx, const T* __restrict__ nmax, T inertia) {
   __shared__ int3 cs[THREADS];
-  const int t = blockIdx.x, tid = threadIdx.x;
+  const int t = tile_id(), tid = threadIdx.x;
   const int X = (t % ntx) * TILE_X + tid % TILE_X, Y = (t / ntx) * TILE_Y + tid / TILE_X;
   const int nl = count[t];
   const bool inside = X < nxo && Y < nyo;
@@ -1696,8 +1755,9 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b_gamma(
   __shared__ double red[S::NT];
   const int tid = threadIdx.x;
   const int line = ROW ? tid / TL : tid % CG, jl = ROW ? tid % TL : tid / CG;
-  const int x = blockIdx.x * CG + line;
-  const int b = blockIdx.y;
+  const int2 bxy = block_xy();
+  const int x = bxy.x * CG + line;
+  const int b = bxy.y;
   const int f = order[b];
   T* re = lre + line * S::LD;
   T* im = lim + line * S::LD;
@@ -1745,8 +1805,8 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b_gamma(
   const double rn = block_reduce<S::NT>(num, red, AddOp());
   const double rd = block_reduce<S::NT>(den, red, AddOp());
   if (tid == 0) {
-    part[((size_t)b * gridDim.x + blockIdx.x) * 2] = rn;
-    part[((size_t)b * gridDim.x + blockIdx.x) * 2 + 1] = rd;
+    part[((size_t)b * gridDim.x + bxy.x) * 2] = rn;
+    part[((size_t)b * gridDim.x + bxy.x) * 2 + 1] = rd;
   }
 }
 
