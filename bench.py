@@ -2,7 +2,8 @@
 
 Workload (the headline configuration): rPIE, 256×256 probe with 4 incoherent Hermite–Gauss
 modes, 1024×1024 object, 32×32 raster of 1024 positions (24 px step, ±2 px jitter), Poisson
-data at 1e7 photons per pattern, batches of 128 positions. A step is one rPIE iteration (epoch)
+data at 1e7 photons per pattern, rPIE's default parameters (RpieParams: steps 1, alphas 0.1,
+batches of 100 positions, ptyrs/src/engine/ptycho.rs:145-186). A step is one rPIE iteration (epoch)
 over all 1024 positions; `value` is positions·iterations per second of the complex-float
 (FP32) engine with the scan resident in HBM. The plain complex-double (FP64) engine runs the
 same epochs from the same start; `nrmse` compares the two objects after phase-ramp and
@@ -27,7 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
-BATCH = 128
+BATCH = 100               # RpieParams::default().batch_size
 N, M, NPOS = 256, 4, 1024
 # HBM bytes per dispatch of the dominant kernels at this workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes scaled by the copy probe's readings (the file gives the commands).
@@ -114,6 +115,7 @@ def main():
     ap.add_argument("--steps64", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--batch", type=int, default=BATCH, help="rPIE batch size (positions)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every GPU box of this build has one, and the "
@@ -132,7 +134,7 @@ def main():
           flush=True)
     scan = pa.Scan(d["counts"], d["y"], d["x"], d["wavelength"], d["distance"],
                    d["detector_pixel"])
-    params = dict(batch_size=BATCH, obj_step=1.0, probe_step=1.0, obj_alpha=0.1, probe_alpha=0.1,
+    params = dict(batch_size=a.batch, obj_step=1.0, probe_step=1.0, obj_alpha=0.1, probe_alpha=0.1,
                   seed=7)
 
     # Timed: captured epochs (hipGraph replays), no per-kernel events. Then a few more epochs
@@ -204,8 +206,8 @@ def main():
         "dtype": "c64",
         "data": "synthetic",
         "config": {"workload": "rPIE epoch, 256x256 probe x 4 modes, 1024 positions (32x32 raster), "
-                               "1024x1024 object, batch 128, 1e7 photons/pattern",
-                   "global_batch": BATCH, "positions": NPOS, "probe": [M, N, N],
+                               f"1024x1024 object, batch {a.batch}, 1e7 photons/pattern",
+                   "global_batch": a.batch, "positions": NPOS, "probe": [M, N, N],
                    "object": [1024, 1024]},
         "fp64": {"value": round(pos64, 1), "ms_per_step": round(1e3 * r64["seconds"] / r64["steps"], 3),
                  "steps": r64["steps"]},
@@ -219,7 +221,7 @@ def main():
                      "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved * 1e9 / HBM_PEAK, 4) if achieved else None,
-                     "traffic": measured_traffic(kern),
+                     "traffic": measured_traffic(kern) if a.batch == BATCH else None,
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_s * 1e3, 4)},
     }
     if not a.no_cpu:

@@ -582,17 +582,31 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
     if (tid < k) cs[tid] = corner[order[out[s + tid]]];
     __syncthreads();
     if (X < nxo) {
-      for (int i = 0; i < k; ++i) {
-        const int x = X - cs[i].y;
-        if (x < 0 || x >= N) continue;
+      // SU slots per step: their loads issued together, added in slot order.
+      constexpr int SU = 4;
+      for (int i0 = 0; i0 < k; i0 += SU) {
 #pragma unroll
         for (int r = 0; r < TROWS; ++r) {
-          const int y = Y0 + RS * r - cs[i].x;
-          if (y >= 0 && y < N) {
+          T w2[SU][MM];
+          bool ok[SU];
 #pragma unroll
-            for (int m = 0; m < MM; ++m)
-              if (m < M) dm[r][m] += W2[(size_t)m * n + y * N + x];
+          for (int u = 0; u < SU; ++u) {
+            const int i = i0 + u < k ? i0 + u : k - 1;  // cs holds k entries
+            const int x = X - cs[i].y, y = Y0 + RS * r - cs[i].x;
+            ok[u] = i0 + u < k && x >= 0 && x < N && y >= 0 && y < N;
+            if (ok[u]) {
+#pragma unroll
+              for (int m = 0; m < MM; ++m)
+                if (m < M) w2[u][m] = W2[(size_t)m * n + y * N + x];
+            }
           }
+#pragma unroll
+          for (int u = 0; u < SU; ++u)
+            if (ok[u]) {
+#pragma unroll
+              for (int m = 0; m < MM; ++m)
+                if (m < M) dm[r][m] += w2[u][m];
+            }
         }
       }
     }
