@@ -107,6 +107,23 @@ def cpu_baseline(d, sample=256):
                        f"(numpy FP64 oracle, {dt:.1f} s)")
 
 
+def lsqml_rates(pa, scan, d, a, steps=5):
+    """LSQML (Poisson, the same batches) on the same scan: positions·iter/s of `steps` epochs
+    after one warm-up, complex64 and complex128, and the LLK of both after them."""
+    out = {"cost": "poisson", "batch": a.batch, "steps": steps}
+    for dtype in ("c64", "c128"):
+        g = pa.GpuPtycho(scan, d["start_obj"], d["start_probe"], dtype=dtype)
+        g.lsqml(1, 0, a.batch, "poisson", 7)
+        g.synchronize()
+        t = time.perf_counter()
+        g.lsqml(steps, 0, a.batch, "poisson", 7)
+        g.synchronize()
+        dt = time.perf_counter() - t
+        out["value" if dtype == "c64" else "fp64"] = round(NPOS * steps / dt, 1)
+        out["llk_poisson_" + dtype] = g.llk()["poisson"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,6 +241,7 @@ def main():
                      "traffic": measured_traffic(kern) if a.batch == BATCH else None,
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_s * 1e3, 4)},
     }
+    out["lsqml"] = lsqml_rates(pa, scan, d, a)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(d)
     print(json.dumps(out), flush=True)

@@ -225,6 +225,18 @@ int ptyrs_engine_position(ptyrs_engine* e, double* out, size_t len, int* state);
 int ptyrs_engine_sampling(ptyrs_engine* e, int* sampling);
 int ptyrs_engine_set_sampling(ptyrs_engine* e, int sampling);
 
+/* The cost of the current state over every frame and its gradients, the adjoint of the
+ * exit-wave model (object window -> probe multiply -> slices -> exit transform -> cost), for a
+ * torch.autograd.Function: *value = the Poisson LLK sum Σ(icalc − iobs + iobs·ln(iobs/icalc))
+ * (cost PTYRS_POISSON; its gradients are those of the Poisson NLL) or Σ(√icalc − √iobs)²
+ * (PTYRS_AMPLITUDE) over unmasked pixels; obj_grad = 2·∂L/∂O* of every slice (len as the
+ * object's), probe_grad = 2·∂L/∂P* (len as the probe's), complex128 interleaved — for a real L
+ * of complex z, 2·∂L/∂z* is torch's gradient. Follows Ptycho::obj_grad / probe_grad and the
+ * multislice adjoint (ptyrs/src/engine/ptycho.rs:93-162, multislice.rs:266-442,
+ * features.rs:310-330). No position offsets. */
+int ptyrs_engine_grad(ptyrs_engine* e, int cost, double* value, double* obj_grad,
+                      size_t obj_len, double* probe_grad, size_t probe_len);
+
 /* GpuPtycho::llk (ptyrs/src/engine/gpu/mod.rs:853-868): normalised LLK of the current state. */
 int ptyrs_engine_llk(ptyrs_engine* e, ptyrs_llk_record* out);
 

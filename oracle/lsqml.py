@@ -290,3 +290,34 @@ def slice_llk(optics, views, iobs, nmode, nb_obs, obj, probe, run=None):
         waves = SliceWaves(optics, views, as_slices(obj), probe)
     psi = optics.exit_forward(waves.psi)
     return llk(iobs, nmode, psi).normalised(nb_obs)
+
+
+def cost_grads(optics, views, iobs, nmode, obj, probe, cost=POISSON):
+    """The cost over `views` and its gradients 2·∂L/∂O_s* (every slice) and 2·∂L/∂P*: L the
+    Poisson LLK sum (`POISSON`) or Σ(√icalc − √iobs)² (`AMPLITUDE`) over unmasked pixels; the
+    exit transform's and the slices' adjoints carry the detector-plane gradient back.
+    ptyrs/src/engine/ptycho.rs:93-162, ptyrs/src/engine/features.rs:310-330,
+    ptyrs/src/engine/multislice.rs:266-442"""
+    objects = as_slices(obj)
+    np_ = probe.shape[0]
+    waves = SliceWaves(optics, views, objects, probe)
+    far = optics.exit_forward(waves.psi)
+    nb = far.shape[0] // nmode
+    fa = far.reshape(nb, nmode, *far.shape[1:])
+    icalc = (fa.real ** 2 + fa.imag ** 2).sum(axis=1)
+    io = np.stack([iobs[v[0]] for v in views[::nmode]])
+    sums = llk(io, nmode, far)
+    masked = np.isnan(io)
+    ic = np.maximum(icalc, 1e-12)
+    if cost == POISSON:
+        value = sums.poisson
+        r = 2.0 * (1.0 - np.where(masked, 0.0, io) / ic)
+    else:
+        value = sums.euclidian / 4.0
+        r = 2.0 * (1.0 - np.sqrt(np.where(masked, 0.0, io) / ic))
+    r = np.where(masked, 0.0, r)
+    chis = waves.back_propagate(optics.exit_adjoint((fa * r[:, None]).reshape(far.shape)))
+    og = [waves.obj_adjoint(s, c) for s, c in enumerate(chis)]
+    d = waves.probe_adjoint(chis[0], np_)
+    pg = d.reshape(-1, np_, *d.shape[1:]).sum(axis=0)
+    return value, (og if isinstance(obj, (list, tuple)) else og[0]), pg

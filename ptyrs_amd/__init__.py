@@ -125,6 +125,7 @@ _SIGNATURES = [
     ("ptyrs_engine_position", [_P, _D, _SZ, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     ("ptyrs_engine_sampling", [_P, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     ("ptyrs_engine_set_sampling", [_P, ctypes.c_int], ctypes.c_int),
+    ("ptyrs_engine_grad", [_P, ctypes.c_int, _D, _D, _SZ, _D, _SZ], ctypes.c_int),
     ("ptyrs_engine_llk", [_P, ctypes.POINTER(_RecordC)], ctypes.c_int),
     ("ptyrs_engine_object", [_P, _D, _SZ], ctypes.c_int),
     ("ptyrs_engine_probe", [_P, _D, _SZ], ctypes.c_int),
@@ -478,6 +479,24 @@ class GpuPtycho:
         return [dict(frames=np.array(d["frames"]), a_far=np.array(d["a_far"]),
                      a_obj=np.array(d["a_obj"]).T, a_probe=np.array(d["a_probe"]))
                 for d in out]
+
+    def grad(self, cost="poisson"):
+        """The cost of the current state over every frame and its gradients, the adjoint of
+        the exit-wave model on the device: `(value, objects_grad, probe_grad)` with value the
+        Poisson LLK sum (`cost="poisson"`, the Poisson NLL up to a constant) or
+        Σ(√icalc − √iobs)² (`"amplitude"`), and the gradients 2·∂L/∂O* of every slice
+        (`[nslice, nobj, nyo, nxo]`) and 2·∂L/∂P* — torch's gradients of a real loss of complex
+        tensors (ptyrs/src/engine/ptycho.rs:93-162, multislice.rs:266-442)."""
+        if cost not in ("poisson", "amplitude"):
+            raise ValueError(f"cost must be \"poisson\" or \"amplitude\", got {cost!r}")
+        go = np.empty(self._objs_shape, dtype=np.complex128)
+        gp = np.empty(self._probe_shape, dtype=np.complex128)
+        v = ctypes.c_double()
+        _check(library().ptyrs_engine_grad(
+            self._h, 1 if cost == "poisson" else 0, ctypes.byref(v),
+            _ptr(go.view(np.float64), ctypes.c_double), go.size * 2,
+            _ptr(gp.view(np.float64), ctypes.c_double), gp.size * 2))
+        return v.value, go, gp
 
     def llk(self):
         """Normalised PyNX LLK of the current object and probe."""

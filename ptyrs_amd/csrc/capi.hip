@@ -315,6 +315,16 @@ int ptyrs_engine_set_sampling(ptyrs_engine* h, int sampling) {
   });
 }
 
+int ptyrs_engine_grad(ptyrs_engine* h, int cost, double* value, double* obj_grad,
+                     size_t obj_len, double* probe_grad, size_t probe_len) {
+  return guarded(h, true, [&](EngineBase& e) {
+    if (!value || !obj_grad || !probe_grad) throw Invalid("null output");
+    if (obj_len != e.object_len() || probe_len != e.probe_len())
+      throw Invalid("gradient buffer has the wrong length");
+    e.grad(cost, value, obj_grad, probe_grad);
+  });
+}
+
 int ptyrs_engine_llk(ptyrs_engine* h, ptyrs_llk_record* out) {
   return guarded(h, true, [&](EngineBase& e) {
     if (!out) throw Invalid("null output");

@@ -115,6 +115,7 @@ class EngineBase {
   virtual void project(const ProjParams& p, size_t ncycle) = 0;
   virtual void ml(const ptyrs_ml_params& p, size_t ncycle) = 0;
   virtual void llk(double out[4]) = 0;
+  virtual void grad(int cost, double* value, double* obj_grad, double* probe_grad) = 0;
   virtual void get_object(double* out) = 0;
   virtual void get_probe(double* out) = 0;
   virtual void set_object(const double* in) = 0;
@@ -1123,6 +1124,81 @@ class Engine : public EngineBase {
     });
   }
 
+  // The cost of the current state over every frame and its gradients 2·∂L/∂O_s* (every slice)
+  // and 2·∂L/∂P*: the adjoint of the exit-wave model (window, probe, slices, plane phase, exit
+  // transform), L the Poisson LLK sum Σ(icalc − iobs + iobs·ln(iobs/icalc)) (icalc where
+  // iobs = 0) or the amplitude cost Σ(√icalc − √iobs)² over unmasked pixels.
+  // ptyrs/src/engine/ptycho.rs:93-162, ptyrs/src/engine/features.rs:310-330,
+  // ptyrs/src/engine/multislice.rs:266-442
+  void grad(int cost, double* value, double* og, double* pg) override {
+    if (nobj_ != 1) throw Invalid("the GPU gradient runs one object mode");
+    if (M_ > 4) throw Invalid("the GPU engine runs at most 4 modes");
+    if (pos_on_) throw Invalid("the GPU gradient models no position offsets");
+    if (cost != PTYRS_POISSON && cost != PTYRS_AMPLITUDE)
+      throw Invalid("cost must be amplitude or poisson");
+    const size_t stack = std::min<size_t>(std::max<size_t>(nframe_, 1), kDefaultStack);
+    ensure_workspace(stack, 1);
+    ensure_lsqml(stack);
+    probe_terms();
+    const size_t S = nslice_, no = nyo_ * nxo_;
+    double v[4] = {0, 0, 0, 0};
+    if (nframe_) {
+      llk_pass(0);
+      PTYRS_HIP(hipMemcpyAsync(v, hist_.p, sizeof(double) * 4, hipMemcpyDeviceToHost, stream_));
+    } else {
+      PTYRS_HIP(hipMemsetAsync(dO_.p, 0, sizeof(cplx<T>) * S * no, stream_));
+      PTYRS_HIP(hipMemsetAsync(dPm_.p, 0, sizeof(cplx<T>) * nprobe_ * n, stream_));
+    }
+    const int nt = ntiles();
+    const Waves wv{P_.as<cplx<T>>(), 0, q_.as<cplx<T>>(), 0};
+    const cplx<T>* q0 = S == 1 ? q_.as<cplx<T>>() : nullptr;
+    cplx<T>* chi = psi_.as<cplx<T>>();
+    for (size_t f0 = 0; f0 < nframe_; f0 += stack) {
+      const int nb = int(std::min(stack, nframe_ - f0));
+      const int* ord = iota_.as<int>() + f0;
+      const int K = nb * M_;
+      const int acc = f0 > 0;
+      run(K_BIN, [&] {
+        hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                           corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
+                           lcount_.as<int>(), (const T*)nullptr, M_, int(nyo_), int(nxo_),
+                           (T*)nullptr, tilemax_.as<T>());
+      });
+      ms_forward(ord, nb, wv);
+      run(K_PASS_A, [&] { rows(false, K); });
+      if (cost == PTYRS_POISSON)
+        pass_b<B_GRAD>(ord, nb);
+      else
+        pass_b<B_GRAD_AMP>(ord, nb);
+      run(K_PASS_C, [&] { rows(true, K); });
+      for (size_t s = S; s-- > 0;) {
+        const auto in = wave(s, wv);
+        const cplx<T>* qs = s + 1 == S ? q_.as<cplx<T>>() : nullptr;
+        run(K_OBJ_UPDATE, [&] {
+          hipLaunchKernelGGL(k_lsqml_obj_dir<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                             slice_dir(s), in.first, (const cplx<T>*)chi, N, M_,
+                             corner_.as<int2>(), ord, list_.as<int>(), lcount_.as<int>(),
+                             int(cap_), int(nyo_), int(nxo_), ntx_, (const T*)nullptr,
+                             (const T*)nullptr, in.second, qs, 0, acc);
+        });
+        if (s == 0) break;
+        run(K_PASS_C, [&] {
+          window<MS_ADJ>(chi, s, nullptr, 0, qs, 0, ord, nb);
+          gap(chi, K, s - 1, true);
+        });
+      }
+      run(K_PROBE, [&] {
+        hipLaunchKernelGGL(k_ms_probe_dir<T>, dim3((n + 255) / 256, M_), dim3(256), 0, stream_,
+                           dPm_.as<cplx<T>>(), (const cplx<T>*)slice_obj(0), int(nxo_), q0,
+                           (const cplx<T>*)chi, corner_.as<int2>(), ord, nb, N, M_, T(1), acc);
+      });
+    }
+    download_complex(dO_, og, S * no);
+    download_complex(dPm_, pg, nprobe_ * n);
+    flush_timing();
+    *value = cost == PTYRS_POISSON ? v[0] : v[2] / 4.0;
+  }
+
   // One LSQML batch of the general model (ptyrs/src/engine/lsqml.rs:563-643 over
   // ptyrs/src/engine/multislice.rs): any number of slices, the probe planes moved by the
   // frames' offsets under PATCH. The exit waves through the slices, the detector operator as
@@ -1235,7 +1311,8 @@ class Engine : public EngineBase {
       run(K_PROBE, [&] {
         hipLaunchKernelGGL(k_ms_probe_dir<T>, dim3((n + 255) / 256, M_), dim3(256), 0, stream_,
                            dPm_.as<cplx<T>>(), (const cplx<T>*)slice_obj(0), int(nxo_), q0,
-                           (const cplx<T>*)chi, corner_.as<int2>(), ord, nb, N, M_);
+                           (const cplx<T>*)chi, corner_.as<int2>(), ord, nb, N, M_,
+                           T(1) / T(nb), 0);
       });
     }
     const int total = int(nprobe_ * n);

@@ -227,7 +227,7 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a(
 
 // ---- pass B: column transforms, the detector-plane operator, inverse column transforms --------
 enum PassB { B_RPIE = 0, B_LLK = 1, B_AMPLITUDE = 2, B_GAUSS = 3, B_ICALC = 4, B_POISSON = 5,
-             B_GRAD = 6 };
+             B_GRAD = 6, B_GRAD_AMP = 7 };
 
 // Per-slot values some pass-B modes read or write.
 template <typename T>
@@ -321,6 +321,16 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
       if (io >= 0.f) {
         const T ic = I * inv_n;
         r = T(2) * (T(1) - T(io) / (ic > T(1e-12) ? ic : T(1e-12))) * inv_n;
+      }
+#pragma unroll
+      for (int m = 0; m < M; ++m) v[m][e] = v[m][e] * r;
+    } else if constexpr (MODE == B_GRAD_AMP) {
+      // The amplitude cost (√icalc − √iobs)²: 2(1 − √(iobs/max(icalc, 1e-12))), 0 where masked
+      // (ptyrs/src/engine/features.rs:322-330).
+      T r = 0;
+      if (io >= 0.f) {
+        const T ic = I * inv_n;
+        r = T(2) * (T(1) - sqrt(T(io) / (ic > T(1e-12) ? ic : T(1e-12)))) * inv_n;
       }
 #pragma unroll
       for (int m = 0; m < M; ++m) v[m][e] = v[m][e] * r;
@@ -792,7 +802,7 @@ __global__ __launch_bounds__(THREADS) void k_lsqml_xi_sums(
 // Σ_m Σ_views conj(W·q)·χ_m (Kahan-compensated) and divides by √(pre² + floor); 0 where no view
 // covers it. W is W_m ([M][n]) or, with per_view, the slot's own plane ([slot·M + m][n], a
 // multislice slice's incident waves or a moved probe); q (nullable) a plane phase, per slot
-// ([slot][n]) with q_frame.
+// ([slot][n]) with q_frame. With `accumulate` the result is added to dO.
 // ptyrs/src/engine/lsqml.rs:183-188, :587-598; ptyrs/src/engine/ops.rs:279-310;
 // ptyrs/src/engine/multislice.rs:402-430
 template <typename T>
@@ -801,7 +811,7 @@ __global__ __launch_bounds__(THREADS) void k_lsqml_obj_dir(
     int N, int M, const int2* __restrict__ corner, const int* __restrict__ order,
     const int* __restrict__ list, const int* __restrict__ count, int cap, int nyo, int nxo,
     int ntx, const T* __restrict__ pre, const T* __restrict__ floor, int per_view = 0,
-    const cplx<T>* __restrict__ q = nullptr, int q_frame = 0) {
+    const cplx<T>* __restrict__ q = nullptr, int q_frame = 0, int accumulate = 0) {
   constexpr int RS = THREADS / TILE_X;
   __shared__ int3 cs[THREADS];
   const int t = tile_id(), tid = threadIdx.x;
@@ -852,7 +862,9 @@ __global__ __launch_bounds__(THREADS) void k_lsqml_obj_dir(
       const T pr = pre[p];
       inv = T(1) / sqrt(pr * pr + fl);
     }
-    st(dO + p, cplx<T>{sr[r].s * inv, si[r].s * inv});
+    cplx<T> v = cplx<T>{sr[r].s * inv, si[r].s * inv};
+    if (accumulate) v = v + ld(dO + p);
+    st(dO + p, v);
   }
 }
 
@@ -1158,13 +1170,14 @@ __global__ void k_ms_window(cplx<T>* __restrict__ out, const cplx<T>* __restrict
   }
 }
 
-// The probe direction of slice 0: dP_m = (1/nb)·Σ_b conj(O_0[win_b]·q)·χ_0[b·M + m], summed in
-// slot order (Kahan). Grid (ceil(n/256), M). ptyrs/src/engine/lsqml.rs:200-229
+// The probe direction of slice 0: dP_m = scale·Σ_b conj(O_0[win_b]·q)·χ_0[b·M + m], summed in
+// slot order (Kahan); scale 1/nb for LSQML's frame mean, 1 for a gradient's sum, which
+// `accumulate` adds to dP. Grid (ceil(n/256), M). ptyrs/src/engine/lsqml.rs:200-229
 template <typename T>
 __global__ void k_ms_probe_dir(cplx<T>* __restrict__ dP, const cplx<T>* __restrict__ O, int nxo,
                                const cplx<T>* __restrict__ q, const cplx<T>* __restrict__ chi,
                                const int2* __restrict__ corner, const int* __restrict__ order,
-                               int nb, int N, int M) {
+                               int nb, int N, int M, T scale, int accumulate) {
   const int n = N * N;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
@@ -1178,7 +1191,9 @@ __global__ void k_ms_probe_dir(cplx<T>* __restrict__ dP, const cplx<T>* __restri
     sr.add(g.x);
     si.add(g.y);
   }
-  st(dP + (size_t)m * n + j, cplx<T>{sr.s, si.s} * (T(1) / T(nb)));
+  cplx<T> v = cplx<T>{sr.s, si.s} * scale;
+  if (accumulate) v = v + ld(dP + (size_t)m * n + j);
+  st(dP + (size_t)m * n + j, v);
 }
 
 // Per slot and workgroup of a slice past the first: Re Σ conj(dop)·χ_s and Σ|dop|² over modes and
