@@ -159,7 +159,7 @@ class EngineBase {
 
   bool profiling_ = false;
   bool graph_ = true;
-  EpochGraph rpie_graph_, lsqml_graph_;
+  EpochGraph rpie_graph_, lsqml_graph_, proj_graph_, ml_graph_;
 
   // Replays `g` if its key matches, else captures `body` (the launches of one epoch on the
   // engine's stream, no allocation, memset or host wait inside) into it and launches it.
@@ -531,6 +531,11 @@ class Engine : public EngineBase {
       }
       // Every call's first cycle starts from ψ = PO (ptyrs/src/engine/ptycho.rs:264-266).
       const bool first = ic == 0;
+      const std::vector<uint64_t> key = {uint64_t(p.algorithm), uint64_t(first), bits(p.mix),
+                                         uint64_t(p.loops), uint64_t(p.upd_obj),
+                                         uint64_t(p.upd_probe), bits(p.obj_inertia),
+                                         bits(p.probe_inertia), stack, cap_};
+      epoch(proj_graph_, key, [&] {
       MixArgs<T> mix{MIX_NONE, carried_.as<cplx<T>>(), iter_.as<cplx<T>>(), T(0), T(0)};
       if (p.algorithm == PTYRS_DM && !first) {
         mix.kind = MIX_DM;
@@ -575,6 +580,7 @@ class Engine : public EngineBase {
           }
         }
       }
+      });
       if (rec && p.algorithm != PTYRS_AP) balance(1.0);
       ++cycle;
     }
@@ -647,6 +653,9 @@ class Engine : public EngineBase {
         llk_pass(slot);
         due.push_back({cycle, slot});
       }
+      const std::vector<uint64_t> key = {uint64_t(ic == 0), uint64_t(uo), uint64_t(up), stack,
+                                         cap_};
+      epoch(ml_graph_, key, [&] {
       // Gradients: 2·∂L/∂ψ* back-projected (ptyrs/src/engine/ptycho.rs:93-162).
       for (int s = 0; s < nstack; ++s) {
         const int f0 = int(s * stack);
@@ -686,7 +695,7 @@ class Engine : public EngineBase {
       // Polak–Ribière β over both arrays, then the direction; h keeps this cycle's g.
       run(K_OTHER, [&] {
         if (ic > 0) {
-          PTYRS_HIP(hipMemsetAsync(cg_sums_.p, 0, sizeof(double) * 2, stream_));
+          zero(cg_sums_.as<double>(), 2);
           if (uo) {
             hipLaunchKernelGGL(k_cg_dots<T>, dim3(grid_o), dim3(THREADS), 0, stream_,
                                gO_.as<cplx<T>>(), hO_.as<cplx<T>>(), no, cg_part_.as<double>());
@@ -753,6 +762,7 @@ class Engine : public EngineBase {
                              gam_.as<double>());
         hipLaunchKernelGGL(k_probe_terms<T>, dim3((np + 255) / 256), dim3(256), 0, stream_,
                            P_.as<cplx<T>>(), q_.as<cplx<T>>(), Wt_.as<cplx<T>>(), n, int(np));
+      });
       });
       ++cycle;
     }
@@ -1380,10 +1390,16 @@ class Engine : public EngineBase {
   // Chunks of a short last stack hold no partials: zero them so every stack's slab sums.
   void zero_partials(int s, int from, int nchunk) {
     const size_t cn = size_t(M_) * n;
-    PTYRS_HIP(hipMemsetAsync(pnum_part_.as<cplx<T>>() + (size_t(s) * nchunk + from) * cn, 0,
-                             sizeof(cplx<T>) * (nchunk - from) * cn, stream_));
-    PTYRS_HIP(hipMemsetAsync(pden_part_.as<T>() + (size_t(s) * nchunk + from) * n, 0,
-                             sizeof(T) * (nchunk - from) * n, stream_));
+    zero(pnum_part_.as<cplx<T>>() + (size_t(s) * nchunk + from) * cn, (nchunk - from) * cn);
+    zero(pden_part_.as<T>() + (size_t(s) * nchunk + from) * n, (nchunk - from) * size_t(n));
+  }
+
+  // count elements at p set to 0 by a fill kernel (capturable, unlike a memset).
+  template <typename U>
+  void zero(U* p, size_t count) {
+    const size_t blocks = std::min<size_t>(1024, (count + 255) / 256);
+    hipLaunchKernelGGL(k_zero<U>, dim3(unsigned(std::max<size_t>(blocks, 1))), dim3(256), 0,
+                       stream_, p, count);
   }
 
   // Object update of AP/DM/RAAR: tile lists over every frame, norm = Σ_m D_m, its max, blend.

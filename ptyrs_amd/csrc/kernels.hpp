@@ -155,6 +155,15 @@ __global__ void k_probe_terms(const cplx<T>* __restrict__ P, const cplx<T>* __re
   if (i < total) st(Wt + i, ld(P + i) * ld(q + i % n));
 }
 
+// p[0 .. count) = 0: a fill kernel rather than a memset node, so that it can sit in a captured
+// epoch.
+template <typename U>
+__global__ void k_zero(U* __restrict__ p, size_t count) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = U{};
+}
+
 // W2[i] = |Wt[i]|² over the M planes of the probe terms (the per-mode overlap's summands).
 template <typename T>
 __global__ void k_norm2_planes(const cplx<T>* __restrict__ Wt, T* __restrict__ W2, int total) {

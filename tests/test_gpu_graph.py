@@ -41,3 +41,25 @@ def test_lsqml_graph_replays_match_eager(cost):
         eng.lsqml(12, 2, 7, cost, 3)
         eng.lsqml(3, 1, 9, cost, 3)
     same(g, e)
+
+
+@pytest.mark.parametrize("name,kw", [("ap", dict(probe_inertia=0.001, calc_llk=1)),
+                                     ("dm", dict(calc_llk=2)),
+                                     ("raar", dict(calc_llk=1, loop_obj_probe=2))])
+def test_projection_graph_replays_match_eager(name, kw):
+    d = small_scan(n=64, nprobe=2, npos=30)
+    g, e = engines(d)
+    for eng in (g, e):
+        getattr(eng, name)(5, **kw)
+        getattr(eng, name)(3, **kw)
+    same(g, e)
+
+
+def test_ml_graph_replays_match_eager():
+    d = small_scan(n=64, nprobe=2, npos=30)
+    g, e = engines(d)
+    for eng in (g, e):
+        eng.ap(3, zero_phase_ramp=False)
+        eng.ml(5, update_probe=True, calc_llk=1)
+        eng.ml(3, calc_llk=2)
+    same(g, e)
