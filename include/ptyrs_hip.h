@@ -237,6 +237,11 @@ int ptyrs_engine_set_sampling(ptyrs_engine* e, int sampling);
 int ptyrs_engine_grad(ptyrs_engine* e, int cost, double* value, double* obj_grad,
                       size_t obj_len, double* probe_grad, size_t probe_len);
 
+/* The model's detector intensities Σ_m |U ψ_m|² of every frame at the current state, detector
+ * order, len = nframe·ny·nx: what the counts are compared with, pty-chi's forward-model output
+ * (ptyrs/tests/engine_lsqml_multislice.rs:146-182; the reference keeps them on the device). */
+int ptyrs_engine_intensities(ptyrs_engine* e, double* out, size_t len);
+
 /* GpuPtycho::llk (ptyrs/src/engine/gpu/mod.rs:853-868): normalised LLK of the current state. */
 int ptyrs_engine_llk(ptyrs_engine* e, ptyrs_llk_record* out);
 

@@ -126,6 +126,7 @@ _SIGNATURES = [
     ("ptyrs_engine_sampling", [_P, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     ("ptyrs_engine_set_sampling", [_P, ctypes.c_int], ctypes.c_int),
     ("ptyrs_engine_grad", [_P, ctypes.c_int, _D, _D, _SZ, _D, _SZ], ctypes.c_int),
+    ("ptyrs_engine_intensities", [_P, _D, _SZ], ctypes.c_int),
     ("ptyrs_engine_llk", [_P, ctypes.POINTER(_RecordC)], ctypes.c_int),
     ("ptyrs_engine_object", [_P, _D, _SZ], ctypes.c_int),
     ("ptyrs_engine_probe", [_P, _D, _SZ], ctypes.c_int),
@@ -497,6 +498,13 @@ class GpuPtycho:
             _ptr(go.view(np.float64), ctypes.c_double), go.size * 2,
             _ptr(gp.view(np.float64), ctypes.c_double), gp.size * 2))
         return v.value, go, gp
+
+    def intensities(self):
+        """The model's detector intensities Σ_m |U ψ_m|² of every frame, `[nframe, ny, nx]`,
+        detector order (the counts' layout)."""
+        out = np.empty((self.nframe,) + tuple(self._probe_shape[1:]))
+        _check(library().ptyrs_engine_intensities(self._h, _ptr(out, ctypes.c_double), out.size))
+        return out
 
     def llk(self):
         """Normalised PyNX LLK of the current object and probe."""

@@ -325,6 +325,14 @@ int ptyrs_engine_grad(ptyrs_engine* h, int cost, double* value, double* obj_grad
   });
 }
 
+int ptyrs_engine_intensities(ptyrs_engine* h, double* out, size_t len) {
+  return guarded(h, true, [&](EngineBase& e) {
+    if (!out || len != e.frame_len())
+      throw Invalid("intensity buffer has the wrong length");
+    e.intensities(out);
+  });
+}
+
 int ptyrs_engine_llk(ptyrs_engine* h, ptyrs_llk_record* out) {
   return guarded(h, true, [&](EngineBase& e) {
     if (!out) throw Invalid("null output");

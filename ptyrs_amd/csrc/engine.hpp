@@ -116,6 +116,7 @@ class EngineBase {
   virtual void ml(const ptyrs_ml_params& p, size_t ncycle) = 0;
   virtual void llk(double out[4]) = 0;
   virtual void grad(int cost, double* value, double* obj_grad, double* probe_grad) = 0;
+  virtual void intensities(double* out) = 0;
   virtual void get_object(double* out) = 0;
   virtual void get_probe(double* out) = 0;
   virtual void set_object(const double* in) = 0;
@@ -131,6 +132,8 @@ class EngineBase {
   size_t object_len() const { return 2 * nslice_ * nobj_ * nyo_ * nxo_; }
   size_t nslice() const { return nslice_; }
   size_t probe_len() const { return 2 * nprobe_ * ny_ * nx_; }
+  size_t nprobe() const { return nprobe_; }
+  size_t frame_len() const { return nframe_ * ny_ * nx_; }
 
   std::vector<Record> history;
   std::vector<StepRecord> lsqml_steps;  // the last LSQML epoch's
@@ -264,6 +267,8 @@ class Engine : public EngineBase {
     M_ = int(nprobe * nobj);
     nslice_ = pre.nslice;
     graph_ = pre.graph;
+    reverse_ = pre.exit.reverse;
+    dft_plane_ = pre.exit.dft_plane;
     plane_quad_[0] = pre.exit.plane_quad[0];
     plane_quad_[1] = pre.exit.plane_quad[1];
     // Sampling::auto (ptyrs/src/engine/sampling.rs:38-49): FullObject in the near field when
@@ -805,6 +810,7 @@ class Engine : public EngineBase {
   // Position offsets (PATCH): device [nframe] (dy, dx) in FP64; the moved probe planes of a
   // stack, the probe spectra, the moved last-slice phases, the window derivative planes.
   bool pos_on_ = false, pos_refine_ = false;
+  bool reverse_ = false, dft_plane_ = true;  // the counts' index map (capi.hip prepare)
   double plane_quad_[2] = {0, 0};
   DevBuf offs_, inc0_, spec_, qv_, pgrad_, gspec_, pos_part_, pos_sums_;
 
@@ -1207,6 +1213,47 @@ class Engine : public EngineBase {
     download_complex(dPm_, pg, nprobe_ * n);
     flush_timing();
     *value = cost == PTYRS_POISSON ? v[0] : v[2] / 4.0;
+  }
+
+  // The model's detector intensities Σ_m |U ψ_m|² of every frame at the current state (what
+  // the counts are compared with), detector order — pty-chi's forward-model output `y_pred`
+  // (ptyrs/tests/engine_lsqml_multislice.rs:146-182).
+  void intensities(double* out) override {
+    if (pos_on_ && sampling != PTYRS_PATCH)
+      throw Invalid("position offsets under FullObject sampling are not run on this GPU "
+                    "engine; set the PATCH sampling");
+    if (M_ > 4) throw Invalid("the GPU engine runs at most 4 modes");
+    if (nframe_ == 0) return;
+    const size_t stack = std::min<size_t>(nframe_, kDefaultStack);
+    ensure_workspace(stack, 1);
+    ensure_lsqml(stack);
+    probe_terms();
+    zero(prev_.as<T>(), 2);
+    std::vector<T> ic(stack * n);
+    for (size_t f0 = 0; f0 < nframe_; f0 += stack) {
+      const int nb = int(std::min(stack, nframe_ - f0));
+      const int* ord = iota_.as<int>() + f0;
+      if (nslice_ > 1 || pos_on_) {
+        ms_forward(ord, nb, stack_waves(ord, nb));
+        run(K_PASS_A, [&] { rows(false, nb * M_); });
+      } else {
+        pass_a(ord, nb, false);
+      }
+      BArgs<T> ba{afar_tmp_.as<T>(), prev_.as<T>(), xi_part_.as<double>(), icalc_.as<T>()};
+      pass_b<B_ICALC>(ord, nb, ba);
+      PTYRS_HIP(hipMemcpyAsync(ic.data(), icalc_.p, sizeof(T) * nb * n, hipMemcpyDeviceToHost,
+                               stream_));
+      PTYRS_HIP(hipStreamSynchronize(stream_));
+      const int h = dft_plane_ ? N / 2 : 0;
+      for (int b = 0; b < nb; ++b)
+        for (int ky = 0; ky < N; ++ky)
+          for (int kx = 0; kx < N; ++kx) {
+            const int dy = reverse_ ? (N - ky) % N : ky, dx = reverse_ ? (N - kx) % N : kx;
+            out[(f0 + b) * size_t(n) + size_t((dy + h) % N) * N + (dx + h) % N] =
+                double(ic[size_t(b) * n + size_t(ky) * N + kx]);
+          }
+    }
+    flush_timing();
   }
 
   // One LSQML batch of the general model (ptyrs/src/engine/lsqml.rs:563-643 over

@@ -1,0 +1,249 @@
+"""Parity at the benchmark configurations' own shapes, on the GPU through the C ABI.
+
+1. Plain rPIE, 64² probe, 256² object, 10×10 raster, 1e6 photons, 50 epochs: against the FP64
+   oracle on the same seeded inputs (complex128 to FP64 rounding, complex64 to 1e-4 on the LLK).
+2. The headline shape (256² × 4 modes, 1024² object, 1024 positions): complex64 against
+   complex128 after 8 rPIE epochs and 4 LSQML epochs, object NRMSE (aligned, illuminated
+   region) below 1e-4 and the LLK to 1e-6.
+3. ML's cost gradient, 128² × 2 modes, 512² object × 3 slices (exp(iφ − μ), φ ≤ 0.3 rad,
+   μ = 0.05φ), 10 keV, 10 nm pixels, 2 µm slices, 400 positions: against the oracle's adjoint
+   and against central differences of the GPU cost (1e-5); a few torch Adam steps through the
+   autograd Function lower the cost.
+4. Sixteen independent scans of uneven sizes (128² and 256² probes, 1–4 modes, Fermat scans at
+   half the probe's FWHM, 1e6–1e8 photons), their positions cut to 100–300 per scan to keep the
+   test short: every scan's rPIE and LSQML epochs lower its LLK.
+5. The large multislice shape (512² × 4 modes, 4096² object × 8 slices at 1 µm, 10 keV, 5 nm
+   pixels) with 300 of its 5000 Fermat positions perturbed by N(0, 3 px): LSQML with refined
+   positions lowers the LLK and moves the positions towards the truth.
+Data for 3–5 come from the GPU model itself (`intensities`) or the oracle's, with Poisson
+noise drawn on the host.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+import ptyrs_amd as pa
+from conftest import max_rel
+from ptyrs_amd import sim
+from test_gpu_rpie import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _scan(d):
+    return pa.Scan(d["counts"], d["y"], d["x"], d["wavelength"], d["distance"],
+                   d["detector_pixel"])
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_config1_rpie_50_epochs_matches_oracle(dtype):
+    from oracle.ptycho import far_field_ptycho
+    d = sim.scan("small")
+    g = pa.GpuPtycho(_scan(d), d["start_obj"], d["start_probe"], dtype=dtype)
+    g.rpie(50, obj_step=1.0, probe_step=1.0, obj_alpha=0.1, probe_alpha=0.1, calc_llk=1)
+    o = far_field_ptycho(d["counts"].astype(np.float64), d["y"], d["x"], d["start_obj"],
+                         d["start_probe"], d["wavelength"], d["distance"], d["detector_pixel"])
+    o.rpie(50, calc_llk=1)
+    want = [r[2].poisson for r in o.history]
+    tol = {"c128": 1e-9, "c64": 1e-4}[dtype]
+    print("config1", dtype, max_rel(g.history["poisson"], want), rel_l2(g.object, o.obj),
+          rel_l2(g.probe, o.probe))
+    assert max_rel(g.history["poisson"], want) < tol
+    assert rel_l2(g.object, o.obj) < tol * 30
+    assert rel_l2(g.probe, o.probe) < tol * 30
+
+
+def aligned_nrmse(a, b, weight):
+    w = weight / weight.sum()
+    ny, nx = a.shape[-2:]
+    yy, xx = np.mgrid[:ny, :nx]
+    ph = np.angle(a[0] * np.conj(b[0]))
+    A = np.stack([np.ones(ny * nx), yy.ravel(), xx.ravel()], axis=1) * np.sqrt(w.ravel())[:, None]
+    coef = np.linalg.lstsq(A, ph.ravel() * np.sqrt(w.ravel()), rcond=None)[0]
+    a2 = a * np.exp(-1j * (coef[1] * yy + coef[2] * xx))
+    a2 = a2 * np.exp(-1j * np.angle((w * a2 * np.conj(b)).sum()))
+    return float(np.sqrt((w * np.abs(a2 - b) ** 2).sum() / (w * np.abs(b) ** 2).sum()))
+
+
+def test_config2_headline_fp32_against_fp64():
+    d = sim.scan("headline")
+    scan = _scan(d)
+    out = {}
+    for dtype in ("c64", "c128"):
+        g = pa.GpuPtycho(scan, d["start_obj"], d["start_probe"], dtype=dtype)
+        g.rpie(8)
+        r = (g.object, g.probe, g.llk()["poisson"])
+        g.lsqml(4, 0, 100, "poisson", 7)
+        out[dtype] = r + (g.object, g.llk()["poisson"], g.corners, g.probe)
+    a, b = out["c64"], out["c128"]
+    n = d["n"]
+    ov = np.zeros(a[0].shape[-2:])
+    p2 = (np.abs(b[1]) ** 2).sum(axis=0)
+    for oy, ox in b[5]:
+        ov[oy:oy + n, ox:ox + n] += p2
+    print("config2 rpie", aligned_nrmse(a[0], b[0], ov), abs(a[2] - b[2]) / abs(b[2]),
+          "lsqml", aligned_nrmse(a[3], b[3], ov), abs(a[4] - b[4]) / abs(b[4]))
+    assert aligned_nrmse(a[0], b[0], ov) < 1e-4
+    assert abs(a[2] - b[2]) < 1e-6 * abs(b[2])
+    assert aligned_nrmse(a[3], b[3], ov) < 1e-4
+    assert abs(a[4] - b[4]) < 1e-6 * abs(b[4])
+
+
+# ---- configuration 3 -----------------------------------------------------------------------------
+
+LAMBDA_10KEV = 1.2398e-10
+
+
+def slice_object(side, nslice, seed, phase_max):
+    """Slices exp(iφ − 0.05φ), φ Gaussian-filtered noise in [0, phase_max]; each slice the same
+    field rolled by a different offset (one filtered field serves every slice)."""
+    phi = np.angle(sim.random_object(side, side, seed, amp=(1.0, 1.0), phase=(0.0, phase_max)))
+    out = [np.roll(phi, (37 * s, 53 * s), axis=(-2, -1)) for s in range(nslice)]
+    return np.stack([np.exp(1j * p - 0.05 * p) for p in out])  # [nslice, 1, side, side]
+
+
+def far_geometry(n, pixel):
+    dpix = 75e-6
+    return LAMBDA_10KEV, pixel * n * dpix / LAMBDA_10KEV, dpix
+
+
+def config3_state():
+    n, side, nslice, npos = 128, 512, 3, 400
+    pixel = 10e-9
+    lam, z, dpix = far_geometry(n, pixel)
+    probe = sim.hermite_gauss_probe(n, powers=(0.8, 0.2)) * 300.0
+    objects = slice_object(side, nslice, 31, 0.3)
+    fwhm = 2.355 * n / 12
+    y, x = sim.fermat_positions(npos, 0.3 * fwhm, pixel)
+    optics = pa.Optics.far_field(lam, z, dpix, [n, n]).with_slices(
+        [pa.Wavefield([n, n], lam, [pixel, pixel], 2e-6)] * (nslice - 1))
+    return n, objects, probe, y, x, optics, (lam, z, dpix)
+
+
+def counts_from_model(scan_args, objects, probe, optics, dtype, seed, photons=None):
+    """Poisson counts of the GPU model's intensities at the truth."""
+    y, x, geom, n = scan_args
+    zero = np.zeros((len(y), n, n), dtype=np.float32)
+    g = pa.GpuPtycho(pa.Scan(zero, y, x, *geom), objects, probe, optics=optics, dtype=dtype)
+    inten = g.intensities()
+    if photons is not None:
+        inten *= photons / inten.sum(axis=(1, 2)).mean()
+    return np.random.default_rng(seed).poisson(inten).astype(np.float32), g.corners
+
+
+def test_config3_gradient_full_size():
+    torch = pytest.importorskip("torch")
+    from ptyrs_amd.autograd import cost
+    from oracle.base import observed
+    from oracle.lsqml import cost_grads
+    from oracle.optics import Optics, Wavefield
+    n, objects, probe, y, x, optics, geom = config3_state()
+    counts, corners = counts_from_model((y, x, geom, n), objects, probe, optics, "c128", 41)
+    rng = np.random.default_rng(43)
+    start = objects * np.exp(1j * rng.uniform(-0.05, 0.05, objects.shape))
+    g = pa.GpuPtycho(pa.Scan(counts, y, x, *geom), start, probe * 0.9, optics=optics,
+                     dtype="c128")
+    v, go, gp = g.grad("poisson")
+    # The oracle's adjoint on the same state and corners (two probe modes, one object mode).
+    lam, z, dpix = geom
+    o_opt = Optics.far_field(lam, z, dpix, [n, n]).with_slices(
+        [Wavefield([n, n], lam, [10e-9, 10e-9], 2e-6)] * 2)
+    views = [(f, m, m, 0, int(corners[f, 0]), int(corners[f, 1]))
+             for f in range(len(y)) for m in range(2)]
+    iobs, _ = observed(counts.astype(np.float64))
+    wv, wo, wp = cost_grads(o_opt, views, iobs, 2, list(start), probe * 0.9)
+    assert abs(v - wv) <= 1e-9 * abs(wv)
+    for s in range(3):
+        assert rel_l2(go[s], wo[s]) < 1e-8, s
+    assert rel_l2(gp, wp) < 1e-8
+    # Central differences of the GPU cost.
+    def f(objs, pr):
+        g.set_object(objs)
+        g.set_probe(pr)
+        return g.grad("poisson")[0]
+    slope = lambda fn, h: (8 * (fn(h) - fn(-h)) - (fn(2 * h) - fn(-2 * h))) / (12 * h)
+    for s in range(3):
+        dd = np.zeros_like(start)
+        dd[s] = (rng.uniform(-0.5, 0.5, start[s].shape) + 1j * rng.uniform(-0.5, 0.5, start[s].shape))
+        want = slope(lambda t: f(start + t * dd, probe * 0.9), 1e-4)
+        got = float((np.conj(go) * dd).real.sum())
+        assert abs(got - want) <= 1e-5 * abs(want), s
+    dp = (rng.uniform(-0.5, 0.5, probe.shape) + 1j * rng.uniform(-0.5, 0.5, probe.shape)) * 30
+    want = slope(lambda t: f(start, probe * 0.9 + t * dp), 1e-4)
+    assert abs(float((np.conj(gp) * dp).real.sum()) - want) <= 1e-5 * abs(want)
+    # complex64 gradients against complex128 on the same state.
+    g32 = pa.GpuPtycho(pa.Scan(counts, y, x, *geom), start, probe * 0.9, optics=optics)
+    v32, go32, gp32 = g32.grad("poisson")
+    assert abs(v32 - v) <= 1e-5 * abs(v)
+    assert rel_l2(go32, go) < 1e-4 and rel_l2(gp32, gp) < 1e-4
+    # A few Adam steps on the slices through the autograd Function lower the cost.
+    ot = torch.tensor(start, dtype=torch.complex128, requires_grad=True)
+    pt = torch.tensor(probe * 0.9, dtype=torch.complex128)
+    opt = torch.optim.Adam([ot], lr=5e-3)
+    first = None
+    for _ in range(5):
+        opt.zero_grad()
+        loss = cost(g32, ot, pt)
+        first = loss.item() if first is None else first
+        loss.backward()
+        opt.step()
+    assert cost(g32, ot.detach(), pt).item() < first
+
+
+# ---- configuration 4 -----------------------------------------------------------------------------
+
+def test_config4_sixteen_uneven_scans():
+    rng = np.random.default_rng(404)
+    for k in range(16):
+        n = int(rng.choice([128, 256]))
+        nmode = int(rng.integers(1, 5))
+        npos = int(rng.integers(100, 301))
+        photons = float(10 ** rng.uniform(6, 8))
+        pixel = sim.pixel_size(n)
+        fwhm = 2.355 * n / 12
+        y, x = sim.fermat_positions(npos, 0.5 * fwhm, pixel)
+        ext = int(np.ceil(np.abs(np.r_[y, x]).max() / pixel)) * 2 + n + 64
+        obj = sim.random_object(ext, ext, 500 + k)
+        probe = sim.hermite_gauss_probe(n, powers=(0.7, 0.15, 0.1, 0.05)[:nmode])
+        geom = (sim.WAVELENGTH, sim.DISTANCE, sim.DETECTOR_PIXEL)
+        counts, _ = counts_from_model((y, x, geom, n), obj, probe, None, "c64", 600 + k,
+                                      photons)
+        start = np.exp(1j * np.random.default_rng(700 + k).uniform(-1, 1, obj.shape))
+        scale = math.sqrt(photons / (np.abs(probe) ** 2).sum())
+        g = pa.GpuPtycho(pa.Scan(counts, y, x, *geom), start, probe * scale)
+        l0 = g.llk()["poisson"]
+        g.rpie(2, calc_llk=1)
+        g.lsqml(2, 1, 50, "poisson", 7)
+        h = g.history["poisson"]
+        assert h[-1] < l0 and np.all(np.isfinite(h)), (k, n, nmode, npos, l0, list(h))
+
+
+# ---- configuration 5 -----------------------------------------------------------------------------
+
+def test_config5_multislice_positions_shape():
+    n, side, nslice, npos = 512, 4096, 8, 300
+    pixel = 5e-9
+    lam, z, dpix = far_geometry(n, pixel)
+    probe = sim.hermite_gauss_probe(n) * 1e3
+    objects = slice_object(side, nslice, 51, 0.1)
+    fwhm = 2.355 * n / 12
+    y, x = sim.fermat_positions(npos, 0.5 * fwhm, pixel)
+    optics = pa.Optics.far_field(lam, z, dpix, [n, n]).with_slices(
+        [pa.Wavefield([n, n], lam, [pixel, pixel], 1e-6)] * (nslice - 1))
+    counts, corners = counts_from_model((y, x, (lam, z, dpix), n), objects, probe, optics,
+                                        "c64", 53)
+    g = pa.GpuPtycho(pa.Scan(counts, y, x, lam, z, dpix), objects, probe, optics=optics)
+    assert g.sampling == "patch"
+    truth = np.zeros((npos, 2))
+    d0 = np.random.default_rng(55).normal(0.0, 3.0, (npos, 2))
+    g.set_features(position=d0)
+    l0 = g.llk()["poisson"]
+    g.lsqml(3, 1, 100, "poisson", 7)
+    h = g.history["poisson"]
+    d = g.features["position"]
+    print("config5", l0, list(h), np.sqrt((d0 ** 2).mean()), np.sqrt((d ** 2).mean()))
+    assert np.all(np.isfinite(h)) and h[-1] < l0
+    rms0 = np.sqrt(((d0 - truth) ** 2).mean())
+    assert np.sqrt(((d - truth) ** 2).mean()) < rms0

@@ -80,3 +80,18 @@ def test_multislice_runs_lsqml_and_llk_only():
     assert np.array_equal(g.objects, objects.astype(np.complex64).astype(np.complex128))
     with pytest.raises(ValueError):
         g.set_object(objects[0])
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_multislice_forward_matches_ptychi(dtype):
+    """The GPU model's detector intensities against pty-chi's forward-model output `y_pred` at
+    the start, frames 0..4 of every case (ptyrs/tests/engine_lsqml_multislice.rs:146-182)."""
+    tol = {"c128": 1e-12, "c64": 1e-5}[dtype]
+    for case in range(3):
+        g, _, _ = start(case, dtype)
+        got = g.intensities()
+        for r in table("lsqml_multislice_icalc_f64.txt"):
+            if r[0] != case:
+                continue
+            want = np.array(r[2:]).reshape(24, 24)
+            assert max_rel(got[int(r[1])].ravel(), want.ravel()) < tol, (case, r[1])
