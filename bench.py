@@ -92,19 +92,41 @@ def illumination(corners, probe, shape):
     return ov
 
 
-def cpu_baseline(d, sample=256):
-    """The FP64 oracle (numpy, one thread for its FFTs and loops) on one batch of `sample`
-    positions of the same workload: positions per second of one rPIE batch."""
-    from oracle.ptycho import far_field_ptycho
-    o = far_field_ptycho(d["counts"].astype(np.float64), d["y"], d["x"], d["start_obj"],
-                         d["start_probe"], d["wavelength"], d["distance"], d["detector_pixel"])
-    frames = list(range(sample))
-    t0 = time.perf_counter()
-    o.rpie_batch(frames, 1.0, 1.0, 0.1, 0.1)
-    dt = time.perf_counter() - t0
-    return dict(value=sample / dt, unit="positions·iter/s", cores=1, kind="port",
-                sample=f"one rPIE batch of {sample} positions at 256²×4 modes, 1024² object "
-                       f"(numpy FP64 oracle, {dt:.1f} s)")
+def cpu_threads():
+    """The host threads this process may use: OMP_NUM_THREADS if set, else its CPU affinity."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(d, batch):
+    """The C++ FP64 restatement of the reference's CPU rPIE (oracle/cpu_rpie.cpp, checked
+    against the numpy oracle) on the same workload: one full epoch (every position, batches of
+    `batch`, OpenMP over the host threads), and three batches on one thread."""
+    from oracle import cpu
+    from oracle.base import AddrTable, pynx_preprocess
+    from oracle.optics import Optics
+    optics = Optics.far_field(d["wavelength"], d["distance"], d["detector_pixel"], [N, N])
+    py_, px_ = pynx_preprocess(d["y"], d["x"], optics.pixel)
+    addr = AddrTable(py_, px_, optics.pixel, [N, N], d["start_obj"].shape[-2:], M, 1)
+    corners = np.c_[addr.oy, addr.ox]
+    threads = cpu_threads()
+    c = cpu.CpuRpie(d["counts"], corners, d["start_obj"], d["start_probe"], optics.quad,
+                    threads=threads)
+    t = time.perf_counter()
+    c.rpie(1, batch_size=batch)
+    dt = time.perf_counter() - t
+    c1 = cpu.CpuRpie(d["counts"], corners, d["start_obj"], d["start_probe"], optics.quad,
+                     threads=1)
+    t = time.perf_counter()
+    for s in range(3):
+        c1.batch(range(s * batch, (s + 1) * batch))
+    dt1 = time.perf_counter() - t
+    return dict(value=round(NPOS / dt, 1), unit="positions·iter/s", cores=threads, kind="port",
+                sample=f"one rPIE epoch ({NPOS} positions, batches of {batch}) at 256²×4 modes, "
+                       f"1024² object, FP64, C++ with OpenMP on {threads} threads: {dt:.2f} s",
+                single_thread=dict(value=round(3 * batch / dt1, 1), cores=1,
+                                   sample=f"three rPIE batches of {batch} positions on one "
+                                          f"thread: {dt1:.2f} s"))
 
 
 def lsqml_rates(pa, scan, d, a, steps=5):
@@ -243,7 +265,7 @@ def main():
     }
     out["lsqml"] = lsqml_rates(pa, scan, d, a)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(d)
+        out["cpu_baseline"] = cpu_baseline(d, a.batch)
     print(json.dumps(out), flush=True)
 
 
