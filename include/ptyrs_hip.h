@@ -225,6 +225,18 @@ int ptyrs_engine_position(ptyrs_engine* e, double* out, size_t len, int* state);
 int ptyrs_engine_sampling(ptyrs_engine* e, int* sampling);
 int ptyrs_engine_set_sampling(ptyrs_engine* e, int sampling);
 
+/* Multislice wave checkpointing for the adjoint (LSQML and ptyrs_engine_grad). The reference
+ * keeps the wave entering every slice of a stack (SliceWaves::new,
+ * ptyrs/src/engine/multislice.rs:86-171); every = 0 or 1 does the same. every = k >= 2 keeps
+ * the waves entering slices k, 2k, … and recomputes the others from those one segment at a
+ * time on the way back, so a stack holds (nslice − 1)/k + k − 1 wave sets instead of
+ * nslice − 1, for one more forward pass through the non-checkpoint slices; results are
+ * bit-identical. At most 64. */
+int ptyrs_engine_set_checkpoint(ptyrs_engine* e, size_t every);
+/* The interval set, and the device bytes the kept incident waves hold now (0 before the first
+ * multislice call after a change). */
+int ptyrs_engine_checkpoint(ptyrs_engine* e, size_t* every, size_t* wave_bytes);
+
 /* The cost of the current state over every frame and its gradients, the adjoint of the
  * exit-wave model (object window -> probe multiply -> slices -> exit transform -> cost), for a
  * torch.autograd.Function: *value = the Poisson LLK sum Σ(icalc − iobs + iobs·ln(iobs/icalc))

@@ -127,6 +127,8 @@ _SIGNATURES = [
     ("ptyrs_engine_set_sampling", [_P, ctypes.c_int], ctypes.c_int),
     ("ptyrs_engine_grad", [_P, ctypes.c_int, _D, _D, _SZ, _D, _SZ], ctypes.c_int),
     ("ptyrs_engine_intensities", [_P, _D, _SZ], ctypes.c_int),
+    ("ptyrs_engine_set_checkpoint", [_P, _SZ], ctypes.c_int),
+    ("ptyrs_engine_checkpoint", [_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)], ctypes.c_int),
     ("ptyrs_engine_llk", [_P, ctypes.POINTER(_RecordC)], ctypes.c_int),
     ("ptyrs_engine_object", [_P, _D, _SZ], ctypes.c_int),
     ("ptyrs_engine_probe", [_P, _D, _SZ], ctypes.c_int),
@@ -505,6 +507,22 @@ class GpuPtycho:
         out = np.empty((self.nframe,) + tuple(self._probe_shape[1:]))
         _check(library().ptyrs_engine_intensities(self._h, _ptr(out, ctypes.c_double), out.size))
         return out
+
+    def set_checkpoint(self, every):
+        """Keeps the wave entering every `every`-th slice for the multislice adjoint and
+        recomputes the others on the way back (0 or 1: keep every slice's, the reference's
+        behaviour); results are bit-identical."""
+        if int(every) < 0:
+            raise ValueError("checkpoint interval must be >= 0")
+        _check(library().ptyrs_engine_set_checkpoint(self._h, int(every)))
+
+    @property
+    def checkpoint(self):
+        """`(every, wave_bytes)`: the checkpoint interval and the device bytes the kept incident
+        waves hold."""
+        k, b = ctypes.c_size_t(), ctypes.c_size_t()
+        _check(library().ptyrs_engine_checkpoint(self._h, ctypes.byref(k), ctypes.byref(b)))
+        return k.value, b.value
 
     def llk(self):
         """Normalised PyNX LLK of the current object and probe."""

@@ -315,6 +315,20 @@ int ptyrs_engine_set_sampling(ptyrs_engine* h, int sampling) {
   });
 }
 
+int ptyrs_engine_set_checkpoint(ptyrs_engine* h, size_t every) {
+  return guarded(h, true, [&](EngineBase& e) {
+    if (every > 64) throw Invalid("checkpoint interval must be at most 64 slices");
+    e.set_checkpoint(every);
+  });
+}
+
+int ptyrs_engine_checkpoint(ptyrs_engine* h, size_t* every, size_t* wave_bytes) {
+  return guarded(h, false, [&](EngineBase& e) {
+    if (every) *every = e.checkpoint;
+    if (wave_bytes) *wave_bytes = e.wave_bytes();
+  });
+}
+
 int ptyrs_engine_grad(ptyrs_engine* h, int cost, double* value, double* obj_grad,
                      size_t obj_len, double* probe_grad, size_t probe_len) {
   return guarded(h, true, [&](EngineBase& e) {

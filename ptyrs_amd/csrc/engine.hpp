@@ -49,6 +49,13 @@ struct DevBuf {
     bytes = want;
     ++generation;
   }
+  // Frees the allocation (after the stream has drained); the next ensure allocates anew.
+  void release() {
+    if (p) PTYRS_HIP(hipFree(p));
+    p = nullptr;
+    bytes = 0;
+    ++generation;
+  }
   template <typename U>
   U* as() const { return static_cast<U*>(p); }
 };
@@ -126,6 +133,11 @@ class EngineBase {
   virtual void set_position(const double* offsets, bool refine) = 0;
   virtual int get_position(double* out) = 0;  // 0 none, 1 held, 2 refined
   int sampling = 0;                            // PTYRS_PATCH or PTYRS_FULL_OBJECT
+  // Multislice wave checkpointing: 0 or 1 keeps the wave entering every slice for the
+  // adjoint; k >= 2 keeps every k-th and recomputes the rest one segment at a time.
+  size_t checkpoint = 0;
+  virtual void set_checkpoint(size_t every) = 0;
+  virtual size_t wave_bytes() const = 0;  // device bytes holding the kept incident waves
 
   void synchronize() { PTYRS_HIP(hipStreamSynchronize(stream_)); }
 
@@ -901,11 +913,53 @@ class Engine : public EngineBase {
   }
 
   // Multislice state: the gaps' transfer functions, a plane of ones, the incident waves of
-  // slices 1.. of a stack ([nslice − 1][cap·M][n], natural layout).
+  // slices 1.. of a stack ([slot][cap·M][n], natural layout). Without checkpointing slice s
+  // has slot s − 1. With checkpoints every k slices, slices that are multiples of k keep slots
+  // 0 .. (S−1)/k − 1 and the others share k − 1 segment slots (slice s in slot
+  // (S−1)/k + s mod k − 1), which hold one segment [c+1, c+k) at a time: the forward pass
+  // leaves the top segment there, and the adjoint recomputes each lower one from its
+  // checkpoint c when it reaches it (`seg_` = the c held).
   DevBuf gaptf_, ones_, inc_;
+  size_t seg_ = 0;
   cplx<T>* slice_obj(size_t s) { return O_.as<cplx<T>>() + s * nobj_ * nyo_ * nxo_; }
   cplx<T>* slice_dir(size_t s) { return dO_.as<cplx<T>>() + s * nyo_ * nxo_; }
-  cplx<T>* incident(size_t s) { return inc_.as<cplx<T>>() + (s - 1) * cap_ * M_ * n; }
+  // The interval in use: at most nslice − 1 (a longer one would only add segment slots).
+  size_t every() const {
+    return checkpoint > 1 && nslice_ > 2 ? std::min(checkpoint, nslice_ - 1) : 1;
+  }
+  void set_checkpoint(size_t k) override {
+    synchronize();
+    checkpoint = k;
+    inc_.release();  // reallocated at the new slot count by the next call
+  }
+  size_t wave_bytes() const override { return inc_.bytes; }
+  size_t inc_slots() const {
+    if (nslice_ < 2) return 0;
+    const size_t k = every();
+    return k == 1 ? nslice_ - 1 : (nslice_ - 1) / k + k - 1;
+  }
+  size_t inc_slot(size_t s) const {
+    const size_t k = every();
+    if (k == 1) return s - 1;
+    return s % k == 0 ? s / k - 1 : (nslice_ - 1) / k + s % k - 1;
+  }
+  cplx<T>* incident(size_t s) { return inc_.as<cplx<T>>() + inc_slot(s) * cap_ * M_ * n; }
+
+  // The incident waves of the segment holding slice s, recomputed from its checkpoint if the
+  // segment slots hold another (the adjoint's descent through the slices).
+  void segment_for(size_t s, const Waves& w, const int* ord, int nb) {
+    const size_t k = every();
+    if (k == 1 || s % k == 0 || seg_ == s - s % k) return;
+    const size_t c = s - s % k, top = std::min(c + k - 1, nslice_ - 1);
+    run(K_PASS_A, [&] {
+      for (size_t t = c; t < top; ++t) {
+        auto in = wave(t, w);
+        window<MS_MUL>(incident(t + 1), t, in.first, in.second, nullptr, 0, ord, nb);
+        gap(incident(t + 1), nb * M_, t, false);
+      }
+    });
+    seg_ = c;
+  }
 
   // An angular-spectrum gap's propagation of `nplane` planes in place, or its adjoint.
   void gap(cplx<T>* planes, int nplane, size_t g, bool adjoint) {
@@ -951,6 +1005,7 @@ class Engine : public EngineBase {
       window<MS_MUL>(psi_.as<cplx<T>>(), S - 1, in.first, in.second, w.qlast, w.q_frame, ord,
                      nb);
     });
+    seg_ = (S - 1) / every() * every();
   }
 
   void rows(bool inverse, int nplane) {
@@ -1188,6 +1243,7 @@ class Engine : public EngineBase {
         pass_b<B_GRAD_AMP>(ord, nb);
       run(K_PASS_C, [&] { rows(true, K); });
       for (size_t s = S; s-- > 0;) {
+        segment_for(s, wv, ord, nb);
         const auto in = wave(s, wv);
         const cplx<T>* qs = s + 1 == S ? q_.as<cplx<T>>() : nullptr;
         run(K_OBJ_UPDATE, [&] {
@@ -1315,6 +1371,7 @@ class Engine : public EngineBase {
     });
     const size_t sp = S / 2;  // pty-chi's slice_for_correction
     for (size_t s = S; s-- > 0;) {
+      segment_for(s, wv, ord, nb);
       const auto in = wave(s, wv);
       const cplx<T>* qs = s + 1 == S ? wv.qlast : nullptr;
       const int qf = s + 1 == S ? wv.q_frame : 0;
@@ -1558,7 +1615,7 @@ class Engine : public EngineBase {
     eord_.ensure(sizeof(int) * std::max<size_t>(1, nframe_));
     llk_part_.ensure(sizeof(double) * 4 * nframe_ * SC::NBLK);
     cap_ = std::max(cap_, cap);
-    if (nslice_ > 1) inc_.ensure(sizeof(cplx<T>) * (nslice_ - 1) * cap_ * M_ * n);
+    if (nslice_ > 1) inc_.ensure(sizeof(cplx<T>) * inc_slots() * cap_ * M_ * n);
     ensure_positions(cap_);
   }
 

@@ -13,8 +13,9 @@
    half the probe's FWHM, 1e6–1e8 photons), their positions cut to 100–300 per scan to keep the
    test short: every scan's rPIE and LSQML epochs lower its LLK.
 5. The large multislice shape (512² × 4 modes, 4096² object × 8 slices at 1 µm, 10 keV, 5 nm
-   pixels) with 300 of its 5000 Fermat positions perturbed by N(0, 3 px): LSQML with refined
-   positions lowers the LLK and moves the positions towards the truth.
+   pixels) with 300 of its 5000 Fermat positions perturbed by N(0, 3 px), slice waves
+   checkpointed every 2 slices: LSQML with refined positions lowers the LLK and moves the
+   positions towards the truth.
 Data for 3–5 come from the GPU model itself (`intensities`) or the oracle's, with Poisson
 noise drawn on the host.
 """
@@ -235,6 +236,7 @@ def test_config5_multislice_positions_shape():
     counts, corners = counts_from_model((y, x, (lam, z, dpix), n), objects, probe, optics,
                                         "c64", 53)
     g = pa.GpuPtycho(pa.Scan(counts, y, x, lam, z, dpix), objects, probe, optics=optics)
+    g.set_checkpoint(2)  # the slice waves checkpointed every 2 slices in the adjoint
     assert g.sampling == "patch"
     truth = np.zeros((npos, 2))
     d0 = np.random.default_rng(55).normal(0.0, 3.0, (npos, 2))

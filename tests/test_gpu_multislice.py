@@ -95,3 +95,63 @@ def test_multislice_forward_matches_ptychi(dtype):
                 continue
             want = np.array(r[2:]).reshape(24, 24)
             assert max_rel(got[int(r[1])].ravel(), want.ravel()) < tol, (case, r[1])
+
+
+def deep_state(nslice=8, n=64, npos=40, seed=61):
+    """An eight-slice far-field scan with 2 probe modes and Poisson counts of its own model."""
+    from ptyrs_amd import sim
+    pixel = 10e-9
+    lam = 1.2398e-10
+    z = pixel * n * 75e-6 / lam
+    probe = sim.hermite_gauss_probe(n, powers=(0.8, 0.2)) * 100.0
+    y, x = sim.fermat_positions(npos, 6.0, pixel)
+    phi = np.angle(sim.random_object(192, 192, seed, amp=(1.0, 1.0), phase=(0.0, 0.2)))
+    objects = np.stack([np.exp(1j * np.roll(phi, (5 * s, 7 * s), axis=(-2, -1))
+                               - 0.05 * np.roll(phi, (5 * s, 7 * s), axis=(-2, -1)))
+                        for s in range(nslice)])
+    optics = pa.Optics.far_field(lam, z, 75e-6, [n, n]).with_slices(
+        [pa.Wavefield([n, n], lam, [pixel, pixel], 1e-6)] * (nslice - 1))
+    zero = np.zeros((npos, n, n), dtype=np.float32)
+    g = pa.GpuPtycho(pa.Scan(zero, y, x, lam, z, 75e-6), objects, probe, optics=optics,
+                     dtype="c128")
+    counts = np.random.default_rng(seed).poisson(g.intensities()).astype(np.float32)
+    start = objects * np.exp(1j * np.random.default_rng(seed + 1).uniform(-0.1, 0.1,
+                                                                         objects.shape))
+    return pa.Scan(counts, y, x, lam, z, 75e-6), start, probe * 0.9, optics
+
+
+@pytest.mark.parametrize("dtype", ["c64", "c128"])
+def test_checkpointed_waves_are_bit_identical(dtype):
+    """Keeping every 2nd or 3rd slice's incident wave and recomputing the rest in the adjoint
+    gives the same bits as keeping them all: the gradient, and LSQML with refined positions on
+    eight slices (and the three-slice fixture case), with fewer wave bytes held."""
+    scan, start, probe, optics = deep_state()
+    d0 = np.random.default_rng(5).normal(0.0, 1.0, (scan.counts.shape[0], 2))
+    out, held = {}, {}
+    for every in (0, 2, 3):
+        g = pa.GpuPtycho(scan, start, probe, optics=optics, dtype=dtype)
+        g.set_checkpoint(every)
+        v, go, gp = g.grad("poisson")
+        held[every] = g.checkpoint
+        g.set_features(position=d0)
+        g.lsqml(3, 1, 16, "poisson", 7)
+        out[every] = (v, go, gp, g.objects, g.probe, g.features["position"],
+                      g.history["poisson"])
+    for every in (2, 3):
+        for a, b in zip(out[every], out[0]):
+            assert np.array_equal(a, b), every
+        assert held[every][0] == every
+    # Eight slices: 7 incident wave sets kept without checkpoints, 4 with every 2 or 3.
+    assert held[2][1] * 7 == held[0][1] * 4 and held[3][1] == held[2][1]
+
+
+@pytest.mark.parametrize("every", [2, 3])
+def test_checkpointed_fixture_case_is_bit_identical(every):
+    runs = []
+    for k in (0, every):
+        g, _, _ = start(0, "c64")
+        g.set_checkpoint(k)
+        g.lsqml(5, 1, BATCH, "poisson", 7)
+        runs.append((g.objects, g.probe, g.history["poisson"]))
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
