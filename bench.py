@@ -10,7 +10,8 @@ same epochs from the same start; `nrmse` compares the two objects after phase-ra
 global-phase alignment on the illuminated region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-    python bench.py --copy-probe     # a plain 2 GiB device copy: the PMC calibration kernel
+    python bench.py --copy-probe     # a plain 2 GiB device copy: the PMC calibration kernel and
+                                     # the achievable bandwidth (run as a child by default)
 
 Prints one JSON line.
 """
@@ -50,7 +51,8 @@ def measured_traffic(kernel):
 
 def copy_probe(gib=2, reps=20):
     """A plain device copy of `gib` GiB (float32), the kernel whose FETCH_SIZE and WRITE_SIZE
-    calibrate the counters: its bytes are known. Prints its bandwidth."""
+    calibrate the counters (its bytes are known), and the achievable-bandwidth reference:
+    returns its figures (GB/s counts the bytes read plus written)."""
     import torch
     n = gib * (1 << 30) // 4
     src = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -62,9 +64,22 @@ def copy_probe(gib=2, reps=20):
         dst.copy_(src)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t) / reps
-    print(json.dumps({"copy_gib": gib, "read_bytes": n * 4, "write_bytes": n * 4,
-                      "ms": round(dt * 1e3, 4), "gb_per_s": round(2 * n * 4 / dt / 1e9, 1)}),
-          flush=True)
+    del src, dst
+    torch.cuda.empty_cache()
+    return {"copy_gib": gib, "read_bytes": n * 4, "write_bytes": n * 4,
+            "ms": round(dt * 1e3, 4), "gb_per_s": round(2 * n * 4 / dt / 1e9, 1)}
+
+
+def copy_bandwidth():
+    """`copy_probe` in a child process (torch's HIP runtime and the engine's are separate
+    libraries, one per process), before this process touches the GPU; None if it fails."""
+    import subprocess
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-probe"],
+                           capture_output=True, text=True, timeout=300)
+        return json.loads(r.stdout.strip().splitlines()[-1]) if r.returncode == 0 else None
+    except (subprocess.SubprocessError, ValueError, IndexError):
+        return None
 
 
 def aligned_nrmse(a, b, weight):
@@ -154,6 +169,8 @@ def main():
     ap.add_argument("--steps64", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--no-copy", action="store_true",
+                    help="skip the child copy-bandwidth process (counter passes)")
     ap.add_argument("--batch", type=int, default=BATCH, help="rPIE batch size (positions)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -161,8 +178,10 @@ def main():
               "reconstruction step is not sharded here (see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     if a.copy_probe:
-        copy_probe()
+        print(json.dumps(copy_probe()), flush=True)
         return
+
+    copy = None if a.no_copy else copy_bandwidth()
 
     import ptyrs_amd as pa
     from ptyrs_amd import sim
@@ -263,6 +282,10 @@ def main():
                      "traffic": measured_traffic(kern) if a.batch == BATCH else None,
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_s * 1e3, 4)},
     }
+    if copy:
+        out["roofline"]["copy_gb_per_s"] = copy["gb_per_s"]
+        if achieved:
+            out["roofline"]["frac_of_copy"] = round(achieved / copy["gb_per_s"], 4)
     out["lsqml"] = lsqml_rates(pa, scan, d, a)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(d, a.batch)

@@ -14,6 +14,7 @@
 #pragma once
 
 #include "cplx.hpp"
+#include "lds_layout.hpp"
 
 namespace ptyrs {
 
@@ -43,9 +44,14 @@ constexpr int n_stages(int n, int e) {
   }
   return s;
 }
-// LDS words per line region, padded so that power-of-two strides spread over the banks.
-constexpr int lds_pad(int i) { return i + (i >> 4); }
-constexpr int lds_line(int n) { return lds_pad(n) + 1; }
+// Placement of element i of a line among the line's LDS words, (i + (i >> SH)·MU)·ST: padded
+// so that the exchanges' strides spread over the banks, ST apart when the workgroup's lines are
+// interleaved (lds_layout.hpp picks SH, MU and ST per line shape).
+template <int SH, int MU, int ST>
+struct LdsAt {
+  static constexpr int at(int i) { return (i + (i >> SH) * MU) * ST; }
+};
+using LdsPlain = LdsAt<4, 1, 1>;
 
 // ---- small DFTs in registers, forward direction --------------------------------------------
 template <typename T>
@@ -150,9 +156,9 @@ __device__ __forceinline__ void dft(cplx<T>* a) {
 }
 
 // ---- one Stockham stage ----------------------------------------------------------------------
-// Stage S of the plan for (N, E): butterflies jb = jl + TL·u, u < E/R. `re`/`im` is this line's
-// LDS region (unused when TL == 1 or on the last stage).
-template <typename T, int N, int E, int S>
+// Stage S of the plan for (N, E): butterflies jb = jl + TL·u, u < E/R. `re`/`im` point at this
+// line's element 0 in LDS, its elements placed by A (unused when TL == 1 or on the last stage).
+template <typename T, int N, int E, int S, typename A>
 __device__ __forceinline__ void fft_stage(cplx<T> (&v)[E], int jl, T* re, T* im,
                                           const cplx<T>* __restrict__ tw) {
   constexpr int R = stage_radix(N, E, S);
@@ -200,7 +206,7 @@ __device__ __forceinline__ void fft_stage(cplx<T> (&v)[E], int jl, T* re, T* im,
         const int base = (jb / NS) * NS * R + jb % NS;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int idx = lds_pad(base + r * NS);
+          const int idx = A::at(base + r * NS);
           re[idx] = a[r].x;
           im[idx] = a[r].y;
         }
@@ -210,7 +216,7 @@ __device__ __forceinline__ void fft_stage(cplx<T> (&v)[E], int jl, T* re, T* im,
       __syncthreads();
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const int idx = lds_pad(jl + TL * e);
+        const int idx = A::at(jl + TL * e);
         v[e] = cplx<T>{re[idx], im[idx]};
       }
       __syncthreads();
@@ -218,18 +224,19 @@ __device__ __forceinline__ void fft_stage(cplx<T> (&v)[E], int jl, T* re, T* im,
   }
 }
 
-template <typename T, int N, int E, int S>
+template <typename T, int N, int E, int S, typename A>
 __device__ __forceinline__ void fft_stages(cplx<T> (&v)[E], int jl, T* re, T* im,
                                            const cplx<T>* __restrict__ tw) {
   if constexpr (S < n_stages(N, E)) {
-    fft_stage<T, N, E, S>(v, jl, re, im, tw);
-    fft_stages<T, N, E, S + 1>(v, jl, re, im, tw);
+    fft_stage<T, N, E, S, A>(v, jl, re, im, tw);
+    fft_stages<T, N, E, S + 1, A>(v, jl, re, im, tw);
   }
 }
 
-// Forward (INV = false) or inverse unnormalised DFT of one line. All threads of the workgroup
-// must call it together (it synchronises when the line spans several threads).
-template <typename T, int N, int E, bool INV>
+// Forward (INV = false) or inverse unnormalised DFT of one line, its LDS elements placed by A.
+// All threads of the workgroup must call it together (it synchronises when the line spans
+// several threads).
+template <typename T, int N, int E, bool INV, typename A = LdsPlain>
 __device__ __forceinline__ void line_fft(cplx<T> (&v)[E], int jl, T* re, T* im,
                                          const cplx<T>* __restrict__ tw) {
   static_assert(n_stages(N, E) > 0, "no radix plan for this line length");
@@ -237,7 +244,7 @@ __device__ __forceinline__ void line_fft(cplx<T> (&v)[E], int jl, T* re, T* im,
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e].y = -v[e].y;
   }
-  fft_stages<T, N, E, 0>(v, jl, re, im, tw);
+  fft_stages<T, N, E, 0, A>(v, jl, re, im, tw);
   if constexpr (INV) {
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e].y = -v[e].y;

@@ -18,6 +18,8 @@
 // projection, LLK and gradient scale leaves alone and the inverse transform takes back out.
 #pragma once
 
+#include <type_traits>
+
 #include "cplx.hpp"
 #include "fft.hpp"
 
@@ -45,6 +47,12 @@ constexpr int lines_per_block(int n, int tl, int maxt, int maxl) {
   return best;
 }
 
+// PTYRS_LDS_TABLE 0 places every line in its own region padded by i >> 4 (for comparison).
+#ifndef PTYRS_LDS_TABLE
+#define PTYRS_LDS_TABLE 1
+#endif
+constexpr LdsLay lds_plain(int n, int l) { return LdsLay{0, 4, 1, n + (n >> 4) + 1, l * (n + (n >> 4) + 1)}; }
+
 // A workgroup of L lines of N values, TL = N/E threads per line.
 template <int N, int E_, int MAXT, int MAXL>
 struct Lines {
@@ -53,9 +61,21 @@ struct Lines {
   static constexpr int TL = N / E;                            // threads per line
   static constexpr int L = lines_per_block(N, TL, MAXT, MAXL);
   static constexpr int NT = TL * L;                           // threads per workgroup
-  static constexpr int LD = lds_line(N);
   static constexpr int NBLK = N / L;                          // workgroups per plane
-  static constexpr int LDS_WORDS = TL > 1 ? L * LD : 1;
+  // LDS placement of the lines' exchanges for lanes along a line (row passes) or across the
+  // lines (column passes): element placement and the offset of line `line`'s element 0.
+  static constexpr LdsLay ROWL = PTYRS_LDS_TABLE ? lds_layout(N, E, L, true) : lds_plain(N, L);
+  static constexpr LdsLay COLL = PTYRS_LDS_TABLE ? lds_layout(N, E, L, false) : lds_plain(N, L);
+  using AtRow = LdsAt<ROWL.shift, ROWL.mul, ROWL.inter ? L : 1>;
+  using AtCol = LdsAt<COLL.shift, COLL.mul, COLL.inter ? L : 1>;
+  static constexpr int base_row(int line) { return ROWL.inter ? line : line * ROWL.ld; }
+  static constexpr int base_col(int line) { return COLL.inter ? line : line * COLL.ld; }
+  static constexpr int LDS_WORDS =
+      TL > 1 ? (ROWL.words > COLL.words ? ROWL.words : COLL.words) : 1;
+  // The last element of the last line lies inside the arrays under both placements.
+  static_assert(TL == 1 || (AtRow::at(N - 1) + base_row(L - 1) < LDS_WORDS &&
+                            AtCol::at(N - 1) + base_col(L - 1) < LDS_WORDS),
+                "LDS layout exceeds its arrays");
 };
 
 // Row passes hold one mode per workgroup: 16 values per thread (float), 8 (double).
@@ -224,7 +244,8 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a(
     T o2 = norm2(o);
     mx = o2 > mx ? o2 : mx;
   }
-  line_fft<T, N, E, false>(v, jl, lre + l * S::LD, lim + l * S::LD, stw);
+  line_fft<T, N, E, false, typename S::AtRow>(v, jl, lre + S::base_row(l), lim + S::base_row(l),
+                                              stw);
   cplx<T>* prow = psi + ((size_t)(b * M + m) * N + y) * N;
 #pragma unroll
   for (int e = 0; e < E; ++e) st(prow + jl + TL * e, v[e]);
@@ -271,12 +292,13 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
   // Columns: lanes run along the 128-byte stripe of lines; rows: along each line.
   const int tid = threadIdx.x;
   const int line = ROW ? tid / TL : tid % CG, jl = ROW ? tid % TL : tid / CG;
+  using At = std::conditional_t<ROW, typename S::AtRow, typename S::AtCol>;
   const int2 bxy = block_xy();
   const int x = bxy.x * CG + line;
   const int b = bxy.y;
   const int f = order[b];
-  T* re = lre + line * S::LD;
-  T* im = lim + line * S::LD;
+  T* re = lre + (ROW ? S::base_row(line) : S::base_col(line));
+  T* im = lim + (ROW ? S::base_row(line) : S::base_col(line));
   auto at = [&](int e) { return ROW ? x * N + jl + TL * e : (jl + TL * e) * N + x; };
   // Element e of the lane is at index i0 + e·step of every plane: one offset VGPR for all
   // modes (buffer accesses), the step in SGPRs.
@@ -296,7 +318,7 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
   }
   stage_twiddles<T, N, S::NT>(stw, tw);
 #pragma unroll
-  for (int m = 0; m < M; ++m) line_fft<T, N, E, false>(v[m], jl, re, im, stw);
+  for (int m = 0; m < M; ++m) line_fft<T, N, E, false, At>(v[m], jl, re, im, stw);
   const T inv_n = norm;
   double acc[4] = {0, 0, 0, 0};
   T afar = 0;
@@ -391,7 +413,7 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
   } else {
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-      line_fft<T, N, E, true>(v[m], jl, re, im, stw);
+      line_fft<T, N, E, true, At>(v[m], jl, re, im, stw);
       const Buf plane = buf(psi + (size_t)(b * M + m) * n, n * cb);
 #pragma unroll
       for (int e = 0; e < E; ++e) bst<T>(plane, i0 * cb, e * step * cb, v[m][e]);
@@ -420,15 +442,15 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_t(cplx<T>* __rest
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = ld(plane + (jl + TL * e) * N + x);
   stage_twiddles<T, N, S::NT>(stw, tw);
-  T* re = lre + col * S::LD;
-  T* im = lim + col * S::LD;
-  line_fft<T, N, E, INV>(v, jl, re, im, stw);
+  T* re = lre + S::base_col(col);
+  T* im = lim + S::base_col(col);
+  line_fft<T, N, E, INV, typename S::AtCol>(v, jl, re, im, stw);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const cplx<T> t = ld(tf + (jl + TL * e) * N + x);
     v[e] = INV ? v[e] * conj(t) : v[e] * t;
   }
-  line_fft<T, N, E, INV>(v, jl, re, im, stw);
+  line_fft<T, N, E, INV, typename S::AtCol>(v, jl, re, im, stw);
 #pragma unroll
   for (int e = 0; e < E; ++e) st(plane + (jl + TL * e) * N + x, v[e]);
 }
@@ -505,7 +527,8 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie(
         omax = t > omax ? t : omax;
       }
     }
-    line_fft<T, N, E, true>(v, jl, lre + l * S::LD, lim + l * S::LD, stw);
+    line_fft<T, N, E, true, typename S::AtRow>(v, jl, lre + S::base_row(l),
+                                               lim + S::base_row(l), stw);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int x = jl + TL * e;
@@ -1113,7 +1136,8 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_rows(cplx<T>* __restri
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = ld(row + jl + TL * e);
   stage_twiddles<T, N, S::NT>(stw, tw);
-  line_fft<T, N, E, INV>(v, jl, lre + line * S::LD, lim + line * S::LD, stw);
+  line_fft<T, N, E, INV, typename S::AtRow>(v, jl, lre + S::base_row(line),
+                                            lim + S::base_row(line), stw);
 #pragma unroll
   for (int e = 0; e < E; ++e) st(row + jl + TL * e, v[e]);
 }
@@ -1138,15 +1162,15 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_gap_cols(cplx<T>* __re
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = ld(plane + (jl + TL * e) * N + x);
   stage_twiddles<T, N, S::NT>(stw, tw);
-  T* re = lre + col * S::LD;
-  T* im = lim + col * S::LD;
-  line_fft<T, N, E, false>(v, jl, re, im, stw);
+  T* re = lre + S::base_col(col);
+  T* im = lim + S::base_col(col);
+  line_fft<T, N, E, false, typename S::AtCol>(v, jl, re, im, stw);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const cplx<T> t = ld(tf + (jl + TL * e) * N + x);
     v[e] = (CONJ ? v[e] * conj(t) : v[e] * t) * scale;
   }
-  line_fft<T, N, E, true>(v, jl, re, im, stw);
+  line_fft<T, N, E, true, typename S::AtCol>(v, jl, re, im, stw);
 #pragma unroll
   for (int e = 0; e < E; ++e) st(plane + (jl + TL * e) * N + x, v[e]);
 }
@@ -1287,7 +1311,8 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_cols(cplx<T>* __restri
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = ld(plane + (jl + TL * e) * N + x);
   stage_twiddles<T, N, S::NT>(stw, tw);
-  line_fft<T, N, E, INV>(v, jl, lre + col * S::LD, lim + col * S::LD, stw);
+  line_fft<T, N, E, INV, typename S::AtCol>(v, jl, lre + S::base_col(col),
+                                            lim + S::base_col(col), stw);
 #pragma unroll
   for (int e = 0; e < E; ++e) st(plane + (jl + TL * e) * N + x, v[e]);
 }
@@ -1525,7 +1550,8 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_proj(
     cplx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = ld(prow + jl + TL * e);
-    line_fft<T, N, E, true>(v, jl, lre + l * S::LD, lim + l * S::LD, stw);
+    line_fft<T, N, E, true, typename S::AtRow>(v, jl, lre + S::base_row(l),
+                                               lim + S::base_row(l), stw);
     const cplx<T>* orow = O + (size_t)(c.x + y) * nxo + c.y;
     cplx<T>* crow = carried + (((size_t)f * M + m) * N + y) * N;
 #pragma unroll
@@ -1762,11 +1788,11 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a3(
     a[e] = w * d + dw * o;
     bb[e] = dw * d;
   }
-  T* re = lre + l * S::LD;
-  T* im = lim + l * S::LD;
-  line_fft<T, N, E, false>(po, jl, re, im, stw);
-  line_fft<T, N, E, false>(a, jl, re, im, stw);
-  line_fft<T, N, E, false>(bb, jl, re, im, stw);
+  T* re = lre + S::base_row(l);
+  T* im = lim + S::base_row(l);
+  line_fft<T, N, E, false, typename S::AtRow>(po, jl, re, im, stw);
+  line_fft<T, N, E, false, typename S::AtRow>(a, jl, re, im, stw);
+  line_fft<T, N, E, false, typename S::AtRow>(bb, jl, re, im, stw);
   const size_t row = ((size_t)(b * M + m) * N + y) * N;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -1793,12 +1819,13 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b_gamma(
   __shared__ double red[S::NT];
   const int tid = threadIdx.x;
   const int line = ROW ? tid / TL : tid % CG, jl = ROW ? tid % TL : tid / CG;
+  using At = std::conditional_t<ROW, typename S::AtRow, typename S::AtCol>;
   const int2 bxy = block_xy();
   const int x = bxy.x * CG + line;
   const int b = bxy.y;
   const int f = order[b];
-  T* re = lre + line * S::LD;
-  T* im = lim + line * S::LD;
+  T* re = lre + (ROW ? S::base_row(line) : S::base_col(line));
+  T* im = lim + (ROW ? S::base_row(line) : S::base_col(line));
   auto at = [&](int e) { return ROW ? x * N + jl + TL * e : (jl + TL * e) * N + x; };
   stage_twiddles<T, N, S::NT>(stw, tw);
   T calc[E], r[E], s[E];
@@ -1814,9 +1841,9 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b_gamma(
       a[e] = ld(psi3 + field + i);
       bb[e] = ld(psi3 + 2 * field + i);
     }
-    line_fft<T, N, E, false>(po, jl, re, im, stw);
-    line_fft<T, N, E, false>(a, jl, re, im, stw);
-    line_fft<T, N, E, false>(bb, jl, re, im, stw);
+    line_fft<T, N, E, false, At>(po, jl, re, im, stw);
+    line_fft<T, N, E, false, At>(a, jl, re, im, stw);
+    line_fft<T, N, E, false, At>(bb, jl, re, im, stw);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       calc[e] += norm2(po[e]);
