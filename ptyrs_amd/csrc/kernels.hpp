@@ -583,6 +583,8 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
   __shared__ int wave_n[THREADS / 64];
   __shared__ int base_s;
   __shared__ T red[THREADS];
+  // The corners of the tile's first THREADS views, kept as they are found.
+  __shared__ int2 cs[THREADS];
   const int t = tile_id(), tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ty0 = (t / ntx) * TILE_Y, tx0 = (t % ntx) * TILE_X;
   int* out = list + (size_t)t * cap;
@@ -591,8 +593,9 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
   for (int s = 0; s < nb; s += THREADS) {
     const int b = s + tid;
     bool hit = false;
+    int2 c = {0, 0};
     if (b < nb) {
-      const int2 c = corner[order[b]];
+      c = corner[order[b]];
       hit = c.x < ty0 + TILE_Y && c.x + N > ty0 && c.y < tx0 + TILE_X && c.y + N > tx0;
     }
     const unsigned long long mask = __ballot(hit);
@@ -601,7 +604,10 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
     __syncthreads();
     int off = base_s;
     for (int w = 0; w < wave; ++w) off += wave_n[w];
-    if (hit) out[off + before] = b;
+    if (hit) {
+      out[off + before] = b;
+      if (off + before < THREADS) cs[off + before] = c;
+    }
     __syncthreads();
     if (tid == 0) {
       int tot = 0;
@@ -613,7 +619,6 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
   const int nl = base_s;
   if (tid == 0) count[t] = nl;
   if (W2 == nullptr) return;
-  __shared__ int2 cs[THREADS];
   constexpr int MM = 4;
   const int X = tx0 + tid % TILE_X, Y0 = ty0 + tid / TILE_X;
   constexpr int RS = THREADS / TILE_X;  // row stride between a thread's pixels
@@ -621,8 +626,10 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
   T dm[TROWS][MM] = {};
   for (int s = 0; s < nl; s += THREADS) {
     const int k = nl - s < THREADS ? nl - s : THREADS;
-    if (tid < k) cs[tid] = corner[order[out[s + tid]]];
-    __syncthreads();
+    if (s > 0) {
+      if (tid < k) cs[tid] = corner[order[out[s + tid]]];
+      __syncthreads();
+    }
     if (X < nxo) {
       // SU slots per step: their loads issued together, added in slot order.
       constexpr int SU = 4;
