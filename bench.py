@@ -10,6 +10,7 @@ same epochs from the same start; `nrmse` compares the two objects after phase-ra
 global-phase alignment on the illuminated region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --multi-scan [--steps K]   # 16 uneven scans: sequential vs grouped epochs
     python bench.py --copy-probe     # a plain 2 GiB device copy: the PMC calibration kernel and
                                      # the achievable bandwidth (run as a child by default)
 
@@ -161,6 +162,75 @@ def lsqml_rates(pa, scan, d, a, steps=5):
     return out
 
 
+def multi_scan(epochs=3, nscan=16, seed=404):
+    """Sixteen independent seeded scans of uneven sizes (probe 128² or 256², 1–4 Hermite–Gauss
+    modes, 200–3000 positions on a Fermat spiral at half the probe's FWHM, objects covering
+    them, Poisson data at 1e6–1e8 photons from the engine's own model), rPIE epochs of all of
+    them: one engine after another (each call waits for its device work), then all at once
+    (ptyrs_group_rpie, every engine's epochs on its own stream). Prints one JSON line."""
+    import ptyrs_amd as pa
+    from ptyrs_amd import sim
+    rng = np.random.default_rng(seed)
+    scans, t0 = [], time.perf_counter()
+    for k in range(nscan):
+        n = int(rng.choice([128, 256]))
+        nmode = int(rng.integers(1, 5))
+        npos = int(rng.integers(200, 3001))
+        photons = float(10 ** rng.uniform(6, 8))
+        pixel = sim.pixel_size(n)
+        y, x = sim.fermat_positions(npos, 0.5 * 2.355 * n / 12, pixel)
+        side = (int(np.ceil(np.abs(np.r_[y, x]).max() / pixel)) * 2 + n + 64 + 3) // 4 * 4
+        obj = sim.random_object(side, side, seed + 1 + k)
+        probe = sim.hermite_gauss_probe(n, powers=(0.7, 0.15, 0.1, 0.05)[:nmode])
+        geom = (sim.WAVELENGTH, sim.DISTANCE, sim.DETECTOR_PIXEL)
+        truth = pa.GpuPtycho(pa.Scan(np.zeros((npos, n, n), np.float32), y, x, *geom), obj,
+                             probe)
+        inten = truth.intensities()
+        del truth
+        inten *= photons / inten.sum(axis=(1, 2)).mean()
+        counts = np.random.default_rng(seed + 100 + k).poisson(inten).astype(np.float32)
+        del inten
+        start = np.exp(1j * np.random.default_rng(seed + 200 + k).uniform(-1, 1, obj.shape))
+        scale = math.sqrt(photons / (np.abs(probe) ** 2).sum())
+        g = pa.GpuPtycho(pa.Scan(counts, y, x, *geom), start, probe * scale)
+        scans.append(dict(engine=g, n=n, modes=nmode, positions=npos, photons=photons))
+        print(f"[bench] scan {k}: {n}² x {nmode} modes, {npos} positions", file=sys.stderr,
+              flush=True)
+    print(f"[bench] 16 scans ready in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    engines = [s["engine"] for s in scans]
+    total = sum(s["positions"] for s in scans)
+    for g in engines:  # warm-up: workspaces and epoch graphs
+        g.rpie(1)
+    t = time.perf_counter()
+    for g in engines:
+        g.rpie(epochs)
+    seq = time.perf_counter() - t
+    pa.rpie_group(engines, 1)
+    t = time.perf_counter()
+    pa.rpie_group(engines, epochs)
+    grp = time.perf_counter() - t
+    # The autotuner's choice of how many scans share the device (one epoch per candidate),
+    # then `epochs` epochs at that width.
+    from ptyrs_amd import tune
+    tuned = tune.tune_group(engines, cache=False)
+    t = time.perf_counter()
+    tune.run_waves(engines, epochs, width=tuned["width"])
+    tun = time.perf_counter() - t
+    best = min(grp, tun)
+    out = {"metric": "aggregate positions·iter/s, 16 independent scans of uneven sizes (rPIE)",
+           "value": round(total * epochs / best, 1), "unit": "positions·iter/s",
+           "sequential": round(total * epochs / seq, 1), "speedup": round(seq / best, 3),
+           "all_at_once": round(total * epochs / grp, 1),
+           "tuned": {"width": tuned["width"], "value": round(total * epochs / tun, 1),
+                     "epoch_seconds_by_width": {k: round(v, 4)
+                                                for k, v in tuned["seconds"].items()}},
+           "epochs": epochs, "positions": total, "dtype": "c64", "data": "synthetic",
+           "launch": "ptyrs_group_rpie: each engine's epoch a hipGraph replay on its own stream",
+           "scans": [{k: s[k] for k in ("n", "modes", "positions")} for s in scans]}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +239,8 @@ def main():
     ap.add_argument("--steps64", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--multi-scan", action="store_true",
+                    help="the sixteen-scan workload instead: sequential vs grouped epochs")
     ap.add_argument("--no-copy", action="store_true",
                     help="skip the child copy-bandwidth process (counter passes)")
     ap.add_argument("--batch", type=int, default=BATCH, help="rPIE batch size (positions)")
@@ -179,6 +251,9 @@ def main():
         sys.exit(2)
     if a.copy_probe:
         print(json.dumps(copy_probe()), flush=True)
+        return
+    if a.multi_scan:
+        multi_scan(a.steps)
         return
 
     copy = None if a.no_copy else copy_bandwidth()

@@ -183,6 +183,16 @@ int ptyrs_engine_create(const ptyrs_scan* scan, const ptyrs_optics* optics, cons
 /* GpuPtycho::rpie (ptyrs/src/engine/gpu/rpie.rs:31-66): nepoch epochs of rPIE. */
 int ptyrs_engine_rpie(ptyrs_engine* e, const ptyrs_rpie_params* params, size_t nepoch);
 
+/* rPIE on several independent scans at once: `nepoch` epochs of every engine (engines[i] with
+ * params[i]), each as ptyrs_engine_rpie would run it (same results, same LLK records), the
+ * engines' epochs enqueued round-robin, each a graph replay on its engine's own stream, so that
+ * scans of uneven sizes share the device instead of running one after another. Returns when
+ * all are done. Every engine's parameters are checked first: PTYRS_EINVAL changes none. A
+ * device failure aborts every engine of the group. No counterpart in the reference, which runs
+ * one scan per GpuPtycho call (ptyrs/src/engine/gpu/rpie.rs:31-62). */
+int ptyrs_group_rpie(ptyrs_engine* const* engines, size_t count,
+                     const ptyrs_rpie_params* params, size_t nepoch);
+
 /* GpuPtycho::ap, ::dm, ::raar (ptyrs/src/engine/gpu/ap.rs:28, dm.rs:32, raar.rs:17): ncycle
  * cycles of PyNX AP, DM or RAAR, the exit waves of DM and RAAR carried across its cycles. */
 int ptyrs_engine_ap(ptyrs_engine* e, const ptyrs_ap_params* params, size_t ncycle);

@@ -127,6 +127,7 @@ _SIGNATURES = [
     ("ptyrs_engine_set_sampling", [_P, ctypes.c_int], ctypes.c_int),
     ("ptyrs_engine_grad", [_P, ctypes.c_int, _D, _D, _SZ, _D, _SZ], ctypes.c_int),
     ("ptyrs_engine_intensities", [_P, _D, _SZ], ctypes.c_int),
+    ("ptyrs_group_rpie", [ctypes.POINTER(_P), _SZ, ctypes.POINTER(_RpieC), _SZ], ctypes.c_int),
     ("ptyrs_engine_set_checkpoint", [_P, _SZ], ctypes.c_int),
     ("ptyrs_engine_checkpoint", [_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)], ctypes.c_int),
     ("ptyrs_engine_llk", [_P, ctypes.POINTER(_RecordC)], ctypes.c_int),
@@ -434,6 +435,11 @@ class GpuPtycho:
     def rpie(self, nepoch, **params):
         """`nepoch` rPIE epochs; keywords are RpieParams fields (ptyrs/src/engine/ptycho.rs:145-186);
         a step of None leaves that array fixed."""
+        _check(library().ptyrs_engine_rpie(self._h, ctypes.byref(self._rpie_c(params)),
+                                           int(nepoch)))
+
+    @staticmethod
+    def _rpie_c(params):
         p = _overlay(_RPIE_PARAMS, params, "RpieParams")
         c = _RpieC()
         c.has_obj_step = p["obj_step"] is not None
@@ -444,7 +450,7 @@ class GpuPtycho:
         if int(p["batch_size"]) < 0 or int(p["calc_llk"]) < 0:
             raise ValueError("batch_size and calc_llk must be >= 0")
         c.batch_size, c.calc_llk, c.seed = int(p["batch_size"]), int(p["calc_llk"]), int(p["seed"])
-        _check(library().ptyrs_engine_rpie(self._h, ctypes.byref(c), int(nepoch)))
+        return c
 
     def lsqml(self, nepoch, calc_llk, batch_size, cost, seed, **kw):
         """`nepoch` LSQML epochs from a fresh LSQML (`cost` "amplitude" or "poisson"), an LLK
@@ -662,3 +668,17 @@ class GpuPtycho:
                                                  ctypes.byref(ms)))
             out[name] = (n.value, ms.value)
         return out
+
+
+def rpie_group(engines, nepoch, params=None):
+    """`nepoch` rPIE epochs of several independent scans at once (ptyrs_group_rpie): each
+    engine's epochs as `GpuPtycho.rpie` runs them, the engines side by side on the device.
+    `params` is one dict of RpieParams fields for all, or one dict per engine."""
+    engines = list(engines)
+    if params is None or isinstance(params, dict):
+        params = [params or {}] * len(engines)
+    if len(params) != len(engines):
+        raise ValueError(f"{len(params)} parameter sets for {len(engines)} engines")
+    hs = (ctypes.c_void_p * max(len(engines), 1))(*[g._h.value for g in engines])
+    cs = (_RpieC * max(len(engines), 1))(*[GpuPtycho._rpie_c(p) for p in params])
+    _check(library().ptyrs_group_rpie(hs, len(engines), cs, int(nepoch)))

@@ -203,6 +203,52 @@ int ptyrs_engine_rpie(ptyrs_engine* h, const ptyrs_rpie_params* p, size_t nepoch
   return guarded(h, true, [&](EngineBase& e) { e.rpie(*p, nepoch); });
 }
 
+int ptyrs_group_rpie(ptyrs_engine* const* engines, size_t count,
+                     const ptyrs_rpie_params* params, size_t nepoch) {
+  try {
+    if (!engines || !params) throw Invalid("null argument");
+    std::vector<EngineBase*> es;
+    for (size_t i = 0; i < count; ++i) {
+      ptyrs_engine* h = engines[i];
+      if (h == nullptr || !h->e) throw Invalid("null engine");
+      if (h->e->aborted)
+        throw Aborted("aborted: an earlier call failed part-way; rebuild the engine");
+      for (EngineBase* o : es)
+        if (o == h->e.get()) throw Invalid("an engine appears twice in the group");
+      es.push_back(h->e.get());
+    }
+    // Every engine's parameters are checked before any engine changes.
+    for (size_t i = 0; i < count; ++i) es[i]->rpie_check(params[i]);
+    size_t i = 0;
+    try {
+      for (; i < count; ++i) es[i]->rpie_begin(params[i], nepoch);
+      // Epoch by epoch, round-robin: each engine's epoch is one replay on its own stream, so
+      // the engines' epochs run side by side on the device.
+      bool more = true;
+      while (more) {
+        more = false;
+        for (EngineBase* e : es) more = e->rpie_epoch() || more;
+      }
+      for (EngineBase* e : es) e->rpie_end();
+    } catch (const Invalid&) {
+      throw;
+    } catch (const std::exception&) {
+      for (size_t k = 0; k < count; ++k) es[k]->aborted = true;
+      throw;
+    }
+    return PTYRS_OK;
+  } catch (const Invalid& x) {
+    last_error = x.what();
+    return PTYRS_EINVAL;
+  } catch (const Aborted& x) {
+    last_error = x.what();
+    return PTYRS_EABORTED;
+  } catch (const std::exception& x) {
+    last_error = x.what();
+    return PTYRS_EDEVICE;
+  }
+}
+
 namespace {
 ProjParams proj(int algorithm, int ho, double oi, int hp, double pi, double mix, size_t loops,
                 size_t calc_llk, int zpr) {

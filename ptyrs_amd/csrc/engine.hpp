@@ -118,6 +118,10 @@ class EngineBase {
     }
   }
   virtual void rpie(const ptyrs_rpie_params& p, size_t nepoch) = 0;
+  virtual void rpie_check(const ptyrs_rpie_params& p) = 0;
+  virtual void rpie_begin(const ptyrs_rpie_params& p, size_t nepoch) = 0;
+  virtual bool rpie_epoch() = 0;  // enqueues the next epoch; false once none is left
+  virtual void rpie_end() = 0;
   virtual void lsqml(const ptyrs_lsqml_params& p, size_t nepoch, size_t calc_llk) = 0;
   virtual void project(const ProjParams& p, size_t ncycle) = 0;
   virtual void ml(const ptyrs_ml_params& p, size_t ncycle) = 0;
@@ -365,59 +369,97 @@ class Engine : public EngineBase {
     for (int i = 0; i < 3; ++i) out[i] /= double(nb_obs_);
   }
 
+  // The rPIE call in progress (between rpie_begin and rpie_end).
+  struct RpieRun {
+    ptyrs_rpie_params p{};
+    size_t nepoch = 0, ic = 0;
+    std::vector<std::vector<std::pair<int, int>>> spans;  // per epoch: (offset, length)
+    std::vector<uint64_t> base;                           // the epoch graph's key prefix
+    std::vector<std::pair<size_t, int>> due;              // LLK records: (cycle, slot)
+    std::vector<int> order;                               // every epoch's batches, uploaded
+  } run_;
+
   void rpie(const ptyrs_rpie_params& p, size_t nepoch) override {
+    rpie_check(p);
+    rpie_begin(p, nepoch);
+    while (rpie_epoch()) {
+    }
+    rpie_end();
+  }
+
+  void rpie_check(const ptyrs_rpie_params& p) override {
     one_unitary("rPIE");
     if (nobj_ != 1) throw Invalid("rPIE needs exactly one object mode");
     if (p.batch_size == 0) throw Invalid("rPIE batch_size must be >= 1");
     if (M_ > 4) throw Invalid("the GPU engine runs at most 4 modes");
+  }
+
+  // An rPIE call in three steps, so that several engines can interleave their epochs on their
+  // own streams: begin uploads every epoch's batches (the host never waits inside the call),
+  // each rpie_epoch() enqueues the next epoch (a graph replay), end drains the LLK records
+  // and waits for the stream.
+  void rpie_begin(const ptyrs_rpie_params& p, size_t nepoch) override {
+    run_ = RpieRun{};
+    run_.p = p;
+    run_.nepoch = nepoch;
     if (nepoch == 0 || nframe_ == 0) {
       cycle += nepoch;
+      run_.nepoch = 0;
       return;
     }
     const size_t cap = std::min(p.batch_size, nframe_);
     ensure_workspace(cap, nepoch);
-    // Every epoch's batches, uploaded once: the host never waits inside the call.
     std::vector<int> order(nepoch * nframe_);
-    std::vector<std::vector<std::pair<int, int>>> spans(nepoch);
+    run_.spans.assign(nepoch, {});
     for (size_t ic = 0; ic < nepoch; ++ic) {
       auto batches = random_batches(nframe_, p.batch_size, p.seed, uint64_t(cycle + ic));
       size_t off = 0;
       for (auto& b : batches) {
         std::copy(b.begin(), b.end(), order.begin() + ic * nframe_ + off);
-        spans[ic].push_back({int(off), int(b.size())});
+        run_.spans[ic].push_back({int(off), int(b.size())});
         off += b.size();
       }
     }
     PTYRS_HIP(hipMemcpyAsync(order_.p, order.data(), sizeof(int) * order.size(),
                              hipMemcpyHostToDevice, stream_));
+    // The host copy must outlive the asynchronous upload.
+    run_.order = std::move(order);
     probe_terms();
     // Every epoch has the same batch lengths; its order is copied into the captured order
     // buffer ahead of the replay.
-    const std::vector<uint64_t> base = {uint64_t(p.has_obj_step), uint64_t(p.has_probe_step),
-                                        bits(p.obj_step), bits(p.probe_step), bits(p.obj_alpha),
-                                        bits(p.probe_alpha), cap_};
-    std::vector<std::pair<size_t, int>> due;  // (cycle, slot)
-    for (size_t ic = 0; ic < nepoch; ++ic) {
-      PTYRS_HIP(hipMemcpyAsync(eord_.p, order_.as<int>() + ic * nframe_, sizeof(int) * nframe_,
-                               hipMemcpyDeviceToDevice, stream_));
-      const int* ord = eord_.as<int>();
-      std::vector<uint64_t> key = base;
-      for (auto& s : spans[ic]) key.push_back(uint64_t(s.second));
-      epoch(rpie_graph_, key, [&] {
-        for (auto& s : spans[ic]) rpie_batch(p, ord + s.first, s.second);
-      });
-      const bool rec = p.calc_llk > 0 && (ic % p.calc_llk == 0 || ic + 1 == nepoch);
-      if (rec) {
-        if (due.size() == kHistSlots) drain(due, PTYRS_RPIE);
-        const int slot = int(due.size());
-        llk_pass(slot);
-        due.push_back({cycle, slot});
-      }
-      ++cycle;
+    run_.base = {uint64_t(p.has_obj_step), uint64_t(p.has_probe_step), bits(p.obj_step),
+                 bits(p.probe_step), bits(p.obj_alpha), bits(p.probe_alpha), cap_};
+  }
+
+  bool rpie_epoch() override {
+    const size_t ic = run_.ic;
+    if (ic >= run_.nepoch) return false;
+    const ptyrs_rpie_params& p = run_.p;
+    PTYRS_HIP(hipMemcpyAsync(eord_.p, order_.as<int>() + ic * nframe_, sizeof(int) * nframe_,
+                             hipMemcpyDeviceToDevice, stream_));
+    const int* ord = eord_.as<int>();
+    std::vector<uint64_t> key = run_.base;
+    for (auto& s : run_.spans[ic]) key.push_back(uint64_t(s.second));
+    epoch(rpie_graph_, key, [&] {
+      for (auto& s : run_.spans[ic]) rpie_batch(p, ord + s.first, s.second);
+    });
+    const bool rec = p.calc_llk > 0 && (ic % p.calc_llk == 0 || ic + 1 == run_.nepoch);
+    if (rec) {
+      if (run_.due.size() == kHistSlots) drain(run_.due, PTYRS_RPIE);
+      const int slot = int(run_.due.size());
+      llk_pass(slot);
+      run_.due.push_back({cycle, slot});
     }
-    drain(due, PTYRS_RPIE);
+    ++cycle;
+    ++run_.ic;
+    return run_.ic < run_.nepoch;
+  }
+
+  void rpie_end() override {
+    drain(run_.due, PTYRS_RPIE);
     synchronize();
     flush_timing();
+    run_ = RpieRun{};
   }
 
   // GpuPtycho::lsqml (ptyrs/src/engine/gpu/lsqml.rs:75-120) without OPR or positions: nepoch

@@ -1,0 +1,89 @@
+"""Launch-shape autotuning for several scans reconstructed at once.
+
+`rpie_group` (ptyrs_group_rpie) runs every engine's rPIE epochs side by side, each on its own
+stream. How many scans share the device at a time is a launch choice that leaves every result
+unchanged (each engine's epochs are bit-identical however the scans are grouped), so it can
+be tuned on the reconstruction's own epochs: `tune_group` times one epoch of every scan at
+each candidate width (scans taken largest first, in waves of that many scans), keeps the
+fastest, and caches it per size class — the multiset of (frame side, modes, dtype) and
+positions rounded to a power of two — in a JSON file.
+"""
+
+import json
+import os
+import pathlib
+import time
+
+from . import rpie_group
+
+CACHE = pathlib.Path(os.environ.get("PTYRS_TUNE_CACHE",
+                                    pathlib.Path.home() / ".cache" / "ptyrs_amd_tune.json"))
+WIDTHS = (1, 2, 4, 8, 16)
+
+
+def size_class(engines):
+    """The tuning key of a set of engines."""
+    parts = []
+    for g in engines:
+        n = g._probe_shape[-1]
+        pos = 1 << max(int(g.nframe) - 1, 0).bit_length()
+        parts.append(f"{n}x{g._probe_shape[0]}:{g.dtype}:{pos}")
+    return ",".join(sorted(parts))
+
+
+def _order(engines):
+    """Largest scans first (frames × plane × modes), so waves start with the longest work."""
+    return sorted(engines, key=lambda g: -g.nframe * g._probe_shape[-1] ** 2 * g._probe_shape[0])
+
+
+def run_waves(engines, nepoch, params=None, width=None):
+    """`nepoch` rPIE epochs of every engine, `width` scans at a time (all at once if None)."""
+    engines = list(engines)
+    if params is None or isinstance(params, dict):
+        params = [params or {}] * len(engines)
+    pairs = list(zip(engines, params))
+    order = _order(engines)
+    pairs.sort(key=lambda gp: order.index(gp[0]))
+    width = width or len(pairs)
+    for i in range(0, len(pairs), width):
+        wave = pairs[i:i + width]
+        rpie_group([g for g, _ in wave], nepoch, [p for _, p in wave])
+
+
+def _load():
+    try:
+        return json.loads(CACHE.read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def tune_group(engines, params=None, widths=WIDTHS, cache=True):
+    """The fastest width for these engines. Each candidate runs one ordinary rPIE epoch of
+    every engine (tuning advances the reconstruction by len(widths) epochs, with results
+    identical to any other grouping). Returns {"width", "seconds": {width: s}, "key"}."""
+    engines = list(engines)
+    key = size_class(engines)
+    if cache:
+        hit = _load().get(key)
+        if hit:
+            return dict(hit, key=key, cached=True)
+    times = {}
+    for w in widths:
+        if w > len(engines) and w != widths[0]:
+            continue
+        for g in engines:
+            g.synchronize()
+        t = time.perf_counter()
+        run_waves(engines, 1, params, w)
+        times[w] = time.perf_counter() - t
+    best = min(times, key=times.get)
+    out = {"width": best, "seconds": {str(k): v for k, v in times.items()}}
+    if cache:
+        data = _load()
+        data[key] = out
+        try:
+            CACHE.parent.mkdir(parents=True, exist_ok=True)
+            CACHE.write_text(json.dumps(data, indent=1))
+        except OSError:
+            pass
+    return dict(out, key=key, cached=False)

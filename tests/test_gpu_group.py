@@ -1,0 +1,75 @@
+"""Several independent scans in one call (ptyrs_group_rpie): each engine's rPIE epochs run on
+its own stream beside the others and give the same bits as the engine run alone; parameters
+are checked for every engine before any changes."""
+
+import numpy as np
+import pytest
+
+import ptyrs_amd as pa
+from ptyrs_amd import sim
+from test_gpu_rpie import small_scan
+
+pytestmark = pytest.mark.gpu
+
+SCANS = [dict(n=64, nprobe=1, npos=30, seed=3), dict(n=128, nprobe=2, npos=20, seed=4),
+         dict(n=32, nprobe=3, npos=41, seed=5), dict(n=64, nprobe=4, npos=12, seed=6)]
+PARAMS = [dict(batch_size=7, calc_llk=1), dict(batch_size=20, calc_llk=2, obj_alpha=0.2),
+          dict(batch_size=100, calc_llk=1, probe_step=None), dict(batch_size=3, seed=11)]
+
+
+def engines(dtypes, **options):
+    out = []
+    for s, dt in zip(SCANS, dtypes):
+        d = small_scan(**s)
+        scan = pa.Scan(d["counts"], d["y"], d["x"], sim.WAVELENGTH, sim.DISTANCE,
+                       sim.DETECTOR_PIXEL)
+        out.append(pa.GpuPtycho(scan, d["obj"], d["probe"], dtype=dt, **options))
+    return out
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_group_matches_engines_run_alone(graph):
+    dtypes = ["c64", "c128", "c64", "c128"]
+    alone = engines(dtypes)
+    for g, p in zip(alone, PARAMS):
+        g.rpie(3, **p)
+    group = engines(dtypes, graph=graph)
+    pa.rpie_group(group, 3, PARAMS)
+    for a, b in zip(alone, group):
+        assert a.cycle == b.cycle == 3
+        assert np.array_equal(a.object, b.object)
+        assert np.array_equal(a.probe, b.probe)
+        ha, hb = a.history, b.history
+        assert np.array_equal(ha["cycle"], hb["cycle"])
+        assert np.array_equal(ha["poisson"], hb["poisson"])
+
+
+def test_group_checks_every_engine_first():
+    group = engines(["c64"] * 4)
+    before = [(g.object, g.probe) for g in group]
+    bad = [dict(p) for p in PARAMS]
+    bad[2]["batch_size"] = 0
+    with pytest.raises(ValueError):
+        pa.rpie_group(group, 2, bad)
+    for g, (o, p) in zip(group, before):
+        assert g.cycle == 0
+        assert np.array_equal(g.object, o) and np.array_equal(g.probe, p)
+    with pytest.raises(ValueError):
+        pa.rpie_group([group[0], group[0]], 1)
+    pa.rpie_group(group, 1, {"batch_size": 10})
+    assert [g.cycle for g in group] == [1] * 4
+
+
+def test_tuned_waves_give_the_same_bits(tmp_path, monkeypatch):
+    from ptyrs_amd import tune
+    monkeypatch.setattr(tune, "CACHE", tmp_path / "tune.json")
+    dtypes = ["c64"] * 4
+    a, b = engines(dtypes), engines(dtypes)
+    r = tune.tune_group(a, PARAMS, widths=(1, 2, 4))
+    assert r["width"] in (1, 2, 4) and not r["cached"]
+    assert tune.tune_group(a, PARAMS)["cached"]
+    for g, p in zip(b, PARAMS):
+        g.rpie(3, **p)
+    for g, h in zip(a, b):
+        assert g.cycle == h.cycle == 3
+        assert np.array_equal(g.object, h.object) and np.array_equal(g.probe, h.probe)
