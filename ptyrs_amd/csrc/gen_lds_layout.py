@@ -6,13 +6,14 @@ double), chosen to minimise LDS bank conflicts.
 
 The exchanges of a mixed-radix Stockham line FFT (fft.hpp) write element
 `(jb / NS)·NS·R + jb mod NS + r·NS` and read element `jl + TL·e` of each line. A layout maps
-(line, element i) to an LDS word, either one region per line, `line·LD + i + (i >> SH)·MU`, or
-interleaved across the workgroup's L lines, `(i + (i >> SH)·MU)·L + line`. For each shape and
-for both lane maps (lanes along a line: row passes; lanes across lines: column passes) this
-script counts, per LDS instruction of a wave, the most lanes that fall on one bank, under three
-bank models (32 banks over 32-lane halves, 64 banks over the whole wave, 64 banks over halves),
-and keeps the layout with the smallest worst-model mean, then the smallest mean over models,
-then the fewest words (at most 5/4 of the plain padded layout's).
+(line, element i) to an LDS element (a float or a double: one or two 4-byte banks), either one
+region per line, `line·LD + i + (i >> SH)·MU`, or interleaved across the workgroup's L lines,
+`(i + (i >> SH)·MU)·L + line`. For each shape and for both lane maps (lanes along a line: row
+passes; lanes across lines: column passes) this script counts, per LDS instruction of a wave,
+the most lanes that fall on one bank, under three bank models (32 banks over 32-lane halves,
+64 banks over the whole wave, 64 banks over halves), and keeps the layout with the smallest
+worst-model mean, then the smallest mean over models, then the fewest elements (at most 5/4 of
+the plain padded layout's).
 """
 
 import pathlib
@@ -61,15 +62,16 @@ def lines_per_block(n, tl, maxt, maxl):
 
 
 def shapes():
-    """(N, E, L) of RowShape, RowShapeC and ColShape (kernels.hpp) for float and double."""
+    """(N, E, L, element bytes) of RowShape, RowShapeC and ColShape (kernels.hpp) for float
+    and double."""
     out = set()
     for size in (4, 8):
         for n in SIDES:
             e = choose_e(n, 16 if size == 4 else 8)
-            out.add((n, e, lines_per_block(n, n // e, THREADS, n)))
+            out.add((n, e, lines_per_block(n, n // e, THREADS, n), size))
             e = choose_e(n, 8 if size == 4 else 4)
-            out.add((n, e, lines_per_block(n, n // e, THREADS, n)))
-            out.add((n, e, lines_per_block(n, n // e, 512, 8 if size == 4 else 4)))
+            out.add((n, e, lines_per_block(n, n // e, THREADS, n), size))
+            out.add((n, e, lines_per_block(n, n // e, 512, 8 if size == 4 else 4), size))
     return sorted(out)
 
 
@@ -97,22 +99,27 @@ def instructions(n, e, l, along):
     return np.array(rows) if rows else None
 
 
-def score(addr, banks, half):
-    """Mean over instructions and lane groups of the most lanes on one bank."""
+def score(addr, banks, half, words=1):
+    """Mean over instructions and lane groups of the most lanes on one bank; an element of
+    `words` 4-byte words occupies that many consecutive banks."""
     nin, nl = addr.shape
     group = 32 if half else 64
     ng = (nl + group - 1) // group
     pad = ng * group - nl
-    a = np.pad(addr % banks, ((0, 0), (0, pad)), constant_values=-1).reshape(nin * ng, group)
+    a = np.pad(addr, ((0, 0), (0, pad)), constant_values=-1).reshape(nin * ng, group)
     valid = a >= 0
-    key = np.where(valid, np.arange(nin * ng)[:, None] * banks + a, nin * ng * banks)
-    counts = np.bincount(key.ravel(), minlength=nin * ng * banks + 1)[:-1]
-    counts = counts.reshape(nin * ng, banks).max(axis=1)
+    total = np.zeros((nin * ng, banks), dtype=np.int64)
+    for w in range(words):
+        b = (a * words + w) % banks
+        key = np.where(valid, np.arange(nin * ng)[:, None] * banks + b, nin * ng * banks)
+        total += np.bincount(key.ravel(), minlength=nin * ng * banks + 1)[:-1].reshape(
+            nin * ng, banks)
     has = valid.any(axis=1)
-    return float(counts[has].mean())
+    return float(total.max(axis=1)[has].mean())
 
 
-def best_layout(n, e, l, along):
+def best_layout(n, e, l, along, size):
+    words = size // 4
     ins = instructions(n, e, l, along)
     plain = l * (n + (n >> 4) + 1)
     if ins is None:
@@ -129,35 +136,36 @@ def best_layout(n, e, l, along):
                     cands.append((0, sh, mu, ld, ld * l))
     best = None
     for c in cands:
-        inter, sh, mu, ld, words = c
+        inter, sh, mu, ld, elems = c
         p = i + (i >> sh) * mu
         addr = p * l + line if inter else line * ld + p
-        s = [score(addr, b, h) for b, h in MODELS]
-        key = (round(max(s), 6), round(sum(s), 6), words, c)
+        s = [score(addr, b, h, words) for b, h in MODELS]
+        key = (round(max(s), 6), round(sum(s), 6), elems, c)
         if best is None or key < best[0]:
             best = (key, c, s)
     cur = line * (n + (n >> 4) + 1) + i + (i >> 4)
-    return best[1], ([score(cur, b, h) for b, h in MODELS], best[2])
+    return best[1], ([score(cur, b, h, words) for b, h in MODELS], best[2])
 
 
 def main():
     lines = []
     report = []
-    for (n, e, l) in shapes():
+    for (n, e, l, size) in shapes():
         for along in (True, False):
-            c, s = best_layout(n, e, l, along)
-            lines.append(f"  if (n == {n} && e == {e} && l == {l} && along == {int(along)}) "
+            c, s = best_layout(n, e, l, along, size)
+            lines.append(f"  if (n == {n} && e == {e} && l == {l} && along == {int(along)} && "
+                         f"bytes == {size}) "
                          f"return LdsLay{{{c[0]}, {c[1]}, {c[2]}, {c[3]}, {c[4]}}};")
             if s:
-                report.append(f"// N={n} E={e} L={l} {'along' if along else 'across'}: "
+                report.append(f"// N={n} E={e} L={l} {size}-byte {'along' if along else 'across'}: "
                               f"plain {[round(x, 2) for x in s[0]]} -> "
                               f"{[round(x, 2) for x in s[1]]}")
     text = f'''// Generated by gen_lds_layout.py (same folder); do not edit by hand.
 // LDS placement of the line FFTs' exchanges per line shape (N values per line, E per thread,
-// L lines per workgroup) and lane map (along: lanes run along a line; across: lanes run across
-// the lines): element i of line `line` sits at
+// L lines per workgroup, element bytes) and lane map (along: lanes run along a line; across:
+// lanes run across the lines): element i of line `line` sits at
 //   interleaved: (i + (i >> shift)·mul)·L + line,   else: line·ld + i + (i >> shift)·mul,
-// in an array of `words` words. Most lanes on one LDS bank per instruction (mean; models:
+// in an array of `words` elements. Most lanes on one LDS bank per instruction (mean; models:
 // 32 banks per half-wave, 64 per wave, 64 per half-wave), the plain padded layout -> this one:
 {chr(10).join(report)}
 #pragma once
@@ -168,7 +176,7 @@ struct LdsLay {{
   int inter, shift, mul, ld, words;
 }};
 
-constexpr LdsLay lds_layout(int n, int e, int l, bool along) {{
+constexpr LdsLay lds_layout(int n, int e, int l, bool along, int bytes) {{
 {chr(10).join(lines)}
   return LdsLay{{0, 4, 1, n + (n >> 4) + 1, l * (n + (n >> 4) + 1)}};
 }}

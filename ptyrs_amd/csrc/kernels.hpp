@@ -53,8 +53,8 @@ constexpr int lines_per_block(int n, int tl, int maxt, int maxl) {
 #endif
 constexpr LdsLay lds_plain(int n, int l) { return LdsLay{0, 4, 1, n + (n >> 4) + 1, l * (n + (n >> 4) + 1)}; }
 
-// A workgroup of L lines of N values, TL = N/E threads per line.
-template <int N, int E_, int MAXT, int MAXL>
+// A workgroup of L lines of N values, TL = N/E threads per line, elements of WB bytes.
+template <int N, int E_, int MAXT, int MAXL, int WB>
 struct Lines {
   static constexpr int E = E_;
   static_assert(E > 0, "unsupported frame side");
@@ -64,8 +64,10 @@ struct Lines {
   static constexpr int NBLK = N / L;                          // workgroups per plane
   // LDS placement of the lines' exchanges for lanes along a line (row passes) or across the
   // lines (column passes): element placement and the offset of line `line`'s element 0.
-  static constexpr LdsLay ROWL = PTYRS_LDS_TABLE ? lds_layout(N, E, L, true) : lds_plain(N, L);
-  static constexpr LdsLay COLL = PTYRS_LDS_TABLE ? lds_layout(N, E, L, false) : lds_plain(N, L);
+  static constexpr LdsLay ROWL =
+      PTYRS_LDS_TABLE ? lds_layout(N, E, L, true, WB) : lds_plain(N, L);
+  static constexpr LdsLay COLL =
+      PTYRS_LDS_TABLE ? lds_layout(N, E, L, false, WB) : lds_plain(N, L);
   using AtRow = LdsAt<ROWL.shift, ROWL.mul, ROWL.inter ? L : 1>;
   using AtCol = LdsAt<COLL.shift, COLL.mul, COLL.inter ? L : 1>;
   static constexpr int base_row(int line) { return ROWL.inter ? line : line * ROWL.ld; }
@@ -80,10 +82,10 @@ struct Lines {
 
 // Row passes hold one mode per workgroup: 16 values per thread (float), 8 (double).
 template <typename T, int N>
-using RowShape = Lines<N, choose_e(N, sizeof(T) == 4 ? 16 : 8), THREADS, N>;
+using RowShape = Lines<N, choose_e(N, sizeof(T) == 4 ? 16 : 8), THREADS, N, sizeof(T)>;
 // Pass C loops over frames with its probe terms held: 8 values per thread (float), 4 (double).
 template <typename T, int N>
-using RowShapeC = Lines<N, choose_e(N, sizeof(T) == 4 ? 8 : 4), THREADS, N>;
+using RowShapeC = Lines<N, choose_e(N, sizeof(T) == 4 ? 8 : 4), THREADS, N, sizeof(T)>;
 // The column pass holds every mode of a 64-byte stripe of columns (8 float or 4 double
 // columns): 8 values per thread (float), 4 (double), so that four modes stay in registers with
 // two workgroups per CU; neighbouring stripes share their 128-byte lines through L2
@@ -96,7 +98,7 @@ using RowShapeC = Lines<N, choose_e(N, sizeof(T) == 4 ? 8 : 4), THREADS, N>;
 #endif
 template <typename T, int N>
 using ColShape = Lines<N, choose_e(N, sizeof(T) == 4 ? 8 : 4), 512,
-                       sizeof(T) == 4 ? PTYRS_COL_LINES_F32 : PTYRS_COL_LINES_F64>;
+                       sizeof(T) == 4 ? PTYRS_COL_LINES_F32 : PTYRS_COL_LINES_F64, sizeof(T)>;
 
 // ---- workgroup placement --------------------------------------------------------------------------
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2): dispatch slot L runs on
