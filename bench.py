@@ -314,11 +314,13 @@ def main():
     launches, ms = stats[kern]
     c = 8  # bytes per complex64
     n = N * N
+    r = 4  # bytes per float
     per_frame = {
         "pass_a": n * c + M * n * c,                 # O window in, rows-transformed Ψ out
         "pass_b": 2 * M * n * c + 4 * n,             # Ψ in and out, float counts in
-        "pass_c": 2 * M * n * c + n * c,             # Ψ' in, Δ out, O window in
-        "obj_update": M * n * c,                     # Δ in (object read-modify-write amortised)
+        # Ψ' and the overlaps D_m in, O window in, the mode-summed object term out
+        "pass_c": M * n * c + M * n * r + n * c + n * c,
+        "obj_update": n * c,                         # the summed term in (object RMW amortised)
     }
     frames_per_launch = NPOS * prof_steps / max(launches, 1)
     alg_bytes = per_frame.get(kern, 0) * frames_per_launch

@@ -1196,11 +1196,10 @@ class Engine : public EngineBase {
     const int chunk = chunk_of(nb);
     const int nchunk = (nb + chunk - 1) / chunk;
     run(K_PASS_C, [&] {
-      hipLaunchKernelGGL((k_pass_c_rpie<T, N>), dim3(SR::NBLK, M_, nchunk), dim3(SR::NT), 0,
+      hipLaunchKernelGGL((k_pass_c_chi<T, N>), dim3(SR::NBLK, M_, nchunk), dim3(SR::NT), 0,
                          stream_, psi_.as<cplx<T>>(), O_.as<cplx<T>>(), int(nxo_),
                          Wt_.as<cplx<T>>(), q_.as<cplx<T>>(), corner_.as<int2>(), ord, nb, M_,
-                         omax_.as<T>(), int(S::NBLK), chunk, T(0), 1, dP_.as<cplx<T>>(),
-                         tw_.as<cplx<T>>(), 0);
+                         chunk, 1, dP_.as<cplx<T>>(), tw_.as<cplx<T>>());
     });
     run(K_OBJ_UPDATE, [&] {
       hipLaunchKernelGGL(k_lsqml_obj_dir<T>, dim3(nt), dim3(THREADS), 0, stream_,
@@ -1638,13 +1637,20 @@ class Engine : public EngineBase {
     int nchunk = std::max(1, std::min(nb, (1024 + per - 1) / per));
     return (nb + nchunk - 1) / nchunk;
   }
+  // The same for rPIE's mode-summing pass C, whose workgroups hold G rows of every mode.
+  int chunk_of_m(int nb) const {
+    const int per = N / rows_per_group(N, SR::L, int(M_));
+    int nchunk = std::max(1, std::min(nb, (1024 + per - 1) / per));
+    return (nb + nchunk - 1) / nchunk;
+  }
 
   void ensure_workspace(size_t cap, size_t nepoch) {
     cap = std::max<size_t>(cap, 1);
     psi_.ensure(sizeof(cplx<T>) * cap * M_ * n);
     omax_.ensure(sizeof(T) * cap * S::NBLK);
-    // chunk_of never cuts more chunks than this, whatever the batch length.
-    const size_t per = size_t(SR::NBLK) * M_;
+    // chunk_of and chunk_of_m never cut more chunks than this, whatever the batch length.
+    const size_t per =
+        std::min(size_t(SR::NBLK) * M_, size_t(N / rows_per_group(N, SR::L, int(M_))));
     const size_t nchunk = std::min(cap, (1024 + per - 1) / per);
     dP_.ensure(sizeof(cplx<T>) * std::max<size_t>(nchunk, 1) * M_ * n);
     list_.ensure(sizeof(int) * size_t(ntiles()) * cap);
@@ -1777,22 +1783,33 @@ class Engine : public EngineBase {
     }
     pass_a(ord, nb, upd_probe);
     pass_b<B_RPIE>(ord, nb);
-    const int chunk = chunk_of(nb);
+    const int chunk = chunk_of_m(nb);
     const int nchunk = (nb + chunk - 1) / chunk;
     run(K_PASS_C, [&] {
-      hipLaunchKernelGGL((k_pass_c_rpie<T, N>), dim3(SR::NBLK, M_, nchunk), dim3(SR::NT), 0,
-                         stream_, psi_.as<cplx<T>>(), O_.as<cplx<T>>(), int(nxo_),
-                         Wt_.as<cplx<T>>(), q_.as<cplx<T>>(), corner_.as<int2>(), ord, nb, M_,
-                         omax_.as<T>(), int(S::NBLK), chunk, T(p.probe_alpha), int(upd_probe),
-                         dP_.as<cplx<T>>(), tw_.as<cplx<T>>(), 1, int(upd_obj));
+      auto launch = [&](auto mm) {
+        constexpr int MM = decltype(mm)::value;
+        constexpr int G = rows_per_group(N, SR::L, MM);
+        hipLaunchKernelGGL((k_pass_c_rpie_sum<T, N, MM>), dim3(N / G, nchunk),
+                           dim3(MM * G * SR::TL), 0, stream_, psi_.as<cplx<T>>(), O_.as<cplx<T>>(), int(nxo_),
+                           Wt_.as<cplx<T>>(), q_.as<cplx<T>>(), corner_.as<int2>(), ord, nb,
+                           omax_.as<T>(), int(S::NBLK), chunk, T(p.probe_alpha),
+                           int(upd_probe), dP_.as<cplx<T>>(), tw_.as<cplx<T>>(), int(upd_obj),
+                           (const T*)D_.as<T>(), int(nyo_), (const T*)pmax_.as<T>(),
+                           T(p.obj_alpha));
+      };
+      switch (M_) {
+        case 1: launch(std::integral_constant<int, 1>{}); break;
+        case 2: launch(std::integral_constant<int, 2>{}); break;
+        case 3: launch(std::integral_constant<int, 3>{}); break;
+        default: launch(std::integral_constant<int, 4>{}); break;
+      }
     });
     if (upd_obj) {
       run(K_OBJ_UPDATE, [&] {
-        hipLaunchKernelGGL(k_obj_update_rpie<T>, dim3(nt), dim3(THREADS), 0, stream_,
-                           O_.as<cplx<T>>(), D_.as<T>(),
-                           psi_.as<cplx<T>>(), N, M_, corner_.as<int2>(), ord, list_.as<int>(),
-                           lcount_.as<int>(), int(cap_), int(nyo_), int(nxo_), ntx_,
-                           pmax_.as<T>(), T(p.obj_alpha), T(p.obj_step));
+        hipLaunchKernelGGL(k_obj_update_rpie_sum<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                           O_.as<cplx<T>>(), (const cplx<T>*)psi_.as<cplx<T>>(), N, M_,
+                           corner_.as<int2>(), ord, list_.as<int>(), lcount_.as<int>(),
+                           int(cap_), int(nyo_), int(nxo_), ntx_, T(p.obj_step));
       });
     }
     if (upd_probe) {

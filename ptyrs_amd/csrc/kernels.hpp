@@ -458,20 +458,16 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_t(cplx<T>* __rest
 }
 
 // ---- pass C: inverse rows, Δ = ψ' − W·O, probe direction ----------------------------------------
-// Grid (NBLK, M, nchunk). Each workgroup owns rows of mode m and runs over the batch slots of
-// its chunk in order: it transforms the rows back, stores χ = Δ (LSQML) or, with store_wd, the
-// object step's numerator term conj(W_m)·Δ (rPIE) over the projected field, and adds
-// conj(O·q)·Δ, divided by (1 − α)|O|² + α·max_window |O|² when `weighted` (rPIE), into a
-// per-chunk probe direction. The window max comes from pass A's
-// `na` partial maxima per slot.
+// LSQML's pass C. Grid (NBLK, M, nchunk). Each workgroup owns rows of mode m and runs over the
+// batch slots of its chunk in order: it transforms the rows back, stores χ = ψ' − W·O over the
+// projected field, and adds conj(O·q)·χ into a per-chunk probe direction.
+// ptyrs/src/engine/lsqml.rs:97-228
 template <typename T, int N>
-__global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie(
+__global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_chi(
     cplx<T>* __restrict__ psi, const cplx<T>* __restrict__ O, int nxo,
     const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ q,
-    const int2* __restrict__ corner, const int* __restrict__ order, int nb, int M,
-    const T* __restrict__ omax_part, int na, int chunk, T alpha, int probe_dir,
-    cplx<T>* __restrict__ dP_part, const cplx<T>* __restrict__ tw, int weighted,
-    int store_wd = 0) {
+    const int2* __restrict__ corner, const int* __restrict__ order, int nb, int M, int chunk,
+    int probe_dir, cplx<T>* __restrict__ dP_part, const cplx<T>* __restrict__ tw) {
   using S = RowShapeC<T, N>;
   constexpr int E = S::E, TL = S::TL;
   constexpr int n = N * N;
@@ -491,7 +487,6 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie(
     qq[e] = ld(q + (size_t)y * N + x);
     acc[e] = cplx<T>{0, 0};
   }
-  const T inv = T(1) - alpha;
   // The next slot's rows are loaded before this slot's transform.
   cplx<T> nxt[E], onx[E];
   if (b0 < b1) {
@@ -522,13 +517,6 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie(
         onx[e] = ld(on + jl + TL * e);
       }
     }
-    T omax = 0;
-    if (weighted) {
-      for (int i = 0; i < na; ++i) {
-        T t = omax_part[(size_t)b * na + i];
-        omax = t > omax ? t : omax;
-      }
-    }
     line_fft<T, N, E, true, typename S::AtRow>(v, jl, lre + S::base_row(l),
                                                lim + S::base_row(l), stw);
 #pragma unroll
@@ -536,12 +524,143 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie(
       const int x = jl + TL * e;
       const cplx<T> o = ow[e];
       const cplx<T> d = v[e] - w[e] * o;
-      st(prow + x, store_wd ? cmulc(w[e], d) : d);
-      if (probe_dir) {
-        const cplx<T> g = cmulc(o * qq[e], d);
-        acc[e] = acc[e] + (weighted ? g * (T(1) / (inv * norm2(o) + alpha * omax)) : g);
+      st(prow + x, d);
+      if (probe_dir) acc[e] = acc[e] + cmulc(o * qq[e], d);
+    }
+  }
+  if (probe_dir) {
+    cplx<T>* out = dP_part + ((size_t)ch * M + m) * n + (size_t)y * N;
+#pragma unroll
+    for (int e = 0; e < E; ++e) st(out + jl + TL * e, acc[e]);
+  }
+}
+
+// Rows per mode group of rPIE's mode-summing pass C: the largest divisor of N with the M groups
+// within pass C's lines per workgroup.
+constexpr int rows_per_group(int n, int l, int m) {
+  int best = 1;
+  for (int d = 1; d * m <= l; ++d)
+    if (n % d == 0) best = d;
+  return best;
+}
+
+// rPIE's pass C with the modes summed. Grid (N/G, nchunk) of M·G·TL threads: mode group m
+// (G rows of TL threads each, pass C's line shape) owns rows of mode m, and the groups run over
+// the batch slots of their chunk together. Per slot each group transforms its rows back, forms
+// Δ_m = ψ'_m − W_m·O, and puts conj(W_m)·Δ_m/((1 − α_o)·D_m + α_o·pmax) in LDS — D_m the batch
+// overlap of |W_m|² at the window pixel, so the object update only sums over views. One slot
+// later (after the next transform's barriers) the groups add those terms over the modes, in
+// mode order, each group for every M-th of its elements, into the slot's mode-0 plane; the
+// terms alternate between two LDS buffers. With `probe_dir` each group adds conj(O·q)·Δ_m
+// divided by (1 − α_p)|O|² + α_p·max_window |O|² into its mode's per-chunk probe direction.
+// ptyrs/src/engine/ptycho.rs:588-687
+template <typename T, int N, int M>
+__global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie_sum(
+    cplx<T>* __restrict__ psi, const cplx<T>* __restrict__ O, int nxo,
+    const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ q,
+    const int2* __restrict__ corner, const int* __restrict__ order, int nb,
+    const T* __restrict__ omax_part, int na, int chunk, T probe_alpha, int probe_dir,
+    cplx<T>* __restrict__ dP_part, const cplx<T>* __restrict__ tw, int upd_obj,
+    const T* __restrict__ D, int nyo, const T* __restrict__ pmax, T obj_alpha) {
+  using S = RowShapeC<T, N>;
+  constexpr int E = S::E, TL = S::TL;
+  constexpr int G = rows_per_group(N, S::L, M);
+  constexpr int n = N * N;
+  constexpr int TW = M * G * N;  // terms per buffer
+  __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
+  __shared__ cplx<T> stw[N];
+  __shared__ cplx<T> terms[2 * TW];
+  const int tid = threadIdx.x;
+  const int m = tid / (G * TL), l = (tid / TL) % G, jl = tid % TL;
+  const int line = m * G + l;
+  const int y = blockIdx.x * G + l;
+  const int ch = blockIdx.y;
+  const int b0 = ch * chunk;
+  const int b1 = b0 + chunk < nb ? b0 + chunk : nb;
+  stage_twiddles<T, N, G * M * TL>(stw, tw);
+  cplx<T> w[E], qq[E], acc[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int x = jl + TL * e;
+    w[e] = ld(Wt + (size_t)m * n + (size_t)y * N + x);
+    qq[e] = ld(q + (size_t)y * N + x);
+    acc[e] = cplx<T>{0, 0};
+  }
+  const T inv_p = T(1) - probe_alpha, inv_o = T(1) - obj_alpha;
+  const T reg = upd_obj ? obj_alpha * pmax[0] : T(0);
+  const size_t no = (size_t)nyo * nxo;
+  // Slot `sb`'s sum over the modes of this thread's share of the row, from terms buffer `buf`.
+  auto store_sum = [&](int sb, int buf) {
+    cplx<T>* out = psi + ((size_t)(sb * M) * N + y) * N + jl;
+    const cplx<T>* t = terms + buf * TW + l * N + jl;
+#pragma unroll
+    for (int e = m; e < E; e += M) {
+      cplx<T> sum = t[TL * e];
+#pragma unroll
+      for (int k = 1; k < M; ++k) sum = sum + t[k * G * N + TL * e];
+      st(out + TL * e, sum);
+    }
+  };
+  // The next slot's rows (field, object window, overlap) are loaded before this slot's
+  // transform.
+  cplx<T> nxt[E], onx[E];
+  T dnx[E];
+  if (b0 < b1) {
+    const cplx<T>* p0 = psi + ((size_t)(b0 * M + m) * N + y) * N;
+    const int2 c0 = corner[order[b0]];
+    const size_t r0 = (size_t)(c0.x + y) * nxo + c0.y;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      nxt[e] = ld(p0 + jl + TL * e);
+      onx[e] = ld(O + r0 + jl + TL * e);
+      dnx[e] = upd_obj ? D[m * no + r0 + jl + TL * e] : T(0);
+    }
+  }
+  for (int b = b0; b < b1; ++b) {
+    cplx<T> v[E], ow[E];
+    T dw[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      v[e] = nxt[e];
+      ow[e] = onx[e];
+      dw[e] = dnx[e];
+    }
+    if (b + 1 < b1) {
+      const cplx<T>* pn = psi + ((size_t)((b + 1) * M + m) * N + y) * N;
+      const int2 cn = corner[order[b + 1]];
+      const size_t rn = (size_t)(cn.x + y) * nxo + cn.y;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        nxt[e] = ld(pn + jl + TL * e);
+        onx[e] = ld(O + rn + jl + TL * e);
+        dnx[e] = upd_obj ? D[m * no + rn + jl + TL * e] : T(0);
       }
     }
+    T omax = 0;
+    if (probe_dir) {
+      for (int i = 0; i < na; ++i) {
+        const T t = omax_part[(size_t)b * na + i];
+        omax = t > omax ? t : omax;
+      }
+    }
+    // The transform's barriers also order the previous slot's term writes before the reads
+    // below, and this slot's reads of buffer (b − 1) before the writes of slot b + 1.
+    line_fft<T, N, E, true, typename S::AtRow>(v, jl, lre + S::base_row(line),
+                                               lim + S::base_row(line), stw);
+    if (upd_obj && b > b0) store_sum(b - 1, (b - 1) & 1);
+    cplx<T>* trow = terms + (b & 1) * TW + line * N + jl;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const cplx<T> o = ow[e];
+      const cplx<T> d = v[e] - w[e] * o;
+      if (upd_obj) trow[TL * e] = cmulc(w[e], d) * (T(1) / (inv_o * dw[e] + reg));
+      if (probe_dir)
+        acc[e] = acc[e] + cmulc(o * qq[e], d) * (T(1) / (inv_p * norm2(o) + probe_alpha * omax));
+    }
+  }
+  if (upd_obj && b1 > b0) {
+    __syncthreads();
+    store_sum(b1 - 1, (b1 - 1) & 1);
   }
   if (probe_dir) {
     cplx<T>* out = dP_part + ((size_t)ch * M + m) * n + (size_t)y * N;
@@ -695,17 +814,18 @@ __global__ __launch_bounds__(THREADS) void k_max_reduce(const T* __restrict__ v,
   if (threadIdx.x == 0) out[0] = r;
 }
 
-// rPIE object step, pass 2: every object pixel owned by one thread adds, in slot order,
-// O += step · Σ_m Σ_views conj(W_m)·Δ_m / ((1 − α)·D_m + α·pmax) with D_m the batch overlap
-// of |W_m|² at the pixel (from k_bin_tiles) and conj(W_m)·Δ_m as pass C stored it, the sums
-// Kahan-compensated. No atomics; the result does not depend on the launch shape.
+#ifndef PTYRS_OBJ_SU
+#define PTYRS_OBJ_SU 4
+#endif
+// rPIE's object update from k_pass_c_rpie_sum's per-slot sums (stored in each slot's mode-0
+// plane, every mode's term already divided by its denominator): every object pixel owned by one
+// thread adds, in slot order and Kahan-compensated, the terms of the views over it, then
+// O += step·Σ. No atomics; the result does not depend on the launch shape.
 template <typename T>
-__global__ __launch_bounds__(THREADS) void k_obj_update_rpie(
-    cplx<T>* __restrict__ O, const T* __restrict__ D, const cplx<T>* __restrict__ wdelta, int N,
-    int M, const int2* __restrict__ corner,
-    const int* __restrict__ order, const int* __restrict__ list, const int* __restrict__ count,
-    int cap, int nyo, int nxo, int ntx, const T* __restrict__ pmax, T alpha, T step) {
-  constexpr int MM = 4;
+__global__ __launch_bounds__(THREADS) void k_obj_update_rpie_sum(
+    cplx<T>* __restrict__ O, const cplx<T>* __restrict__ wsum, int N, int M,
+    const int2* __restrict__ corner, const int* __restrict__ order, const int* __restrict__ list,
+    const int* __restrict__ count, int cap, int nyo, int nxo, int ntx, T step) {
   constexpr int RS = THREADS / TILE_X;
   __shared__ int3 cs[THREADS];
   const int t = tile_id(), tid = threadIdx.x;
@@ -713,7 +833,7 @@ __global__ __launch_bounds__(THREADS) void k_obj_update_rpie(
   const int nl = count[t];
   if (nl == 0) return;
   const int n = N * N;
-  kahan<T> nr[TROWS][MM], ni[TROWS][MM];
+  kahan<T> nr[TROWS], ni[TROWS];
   bool covered[TROWS] = {};
   const int* lst = list + (size_t)t * cap;
   for (int s = 0; s < nl; s += THREADS) {
@@ -725,53 +845,36 @@ __global__ __launch_bounds__(THREADS) void k_obj_update_rpie(
     }
     __syncthreads();
     // SU slots per step: their loads are issued together, then added in slot order.
-    constexpr int SU = 4;
+    constexpr int SU = PTYRS_OBJ_SU;
     for (int i0 = 0; i0 < k && X < nxo; i0 += SU) {
 #pragma unroll
       for (int r = 0; r < TROWS; ++r) {
-        cplx<T> g[SU][MM];
+        cplx<T> g[SU];
         bool ok[SU];
 #pragma unroll
         for (int u = 0; u < SU; ++u) {
           const int i = i0 + u < k ? i0 + u : k - 1;  // cs holds k entries
           const int x = X - cs[i].z, y = Y0 + RS * r - cs[i].y;
           ok[u] = i0 + u < k && x >= 0 && x < N && y >= 0 && y < N;
-          if (ok[u]) {
-            const cplx<T>* dv = wdelta + (size_t)cs[i].x * M * n + y * N + x;
-#pragma unroll
-            for (int m = 0; m < MM; ++m)
-              if (m < M) g[u][m] = ld(dv + (size_t)m * n);
-          }
+          if (ok[u]) g[u] = ld(wsum + (size_t)cs[i].x * M * n + y * N + x);
         }
 #pragma unroll
         for (int u = 0; u < SU; ++u)
           if (ok[u]) {
-#pragma unroll
-            for (int m = 0; m < MM; ++m)
-              if (m < M) {
-                nr[r][m].add(g[u][m].x);
-                ni[r][m].add(g[u][m].y);
-              }
+            nr[r].add(g[u].x);
+            ni[r].add(g[u].y);
             covered[r] = true;
           }
       }
     }
     __syncthreads();
   }
-  const T reg = alpha * pmax[0];
-  const size_t no = (size_t)nyo * nxo;
 #pragma unroll
   for (int r = 0; r < TROWS; ++r) {
     const int Y = Y0 + RS * r;
     if (!covered[r] || Y >= nyo) continue;
-    const size_t p = (size_t)Y * nxo + X;
-    cplx<T> upd = {0, 0};
-#pragma unroll
-    for (int m = 0; m < MM; ++m)
-      if (m < M)
-        upd = upd + cplx<T>{nr[r][m].s, ni[r][m].s} * (T(1) / ((T(1) - alpha) * D[m * no + p] + reg));
-    cplx<T>* o = O + p;
-    st(o, ld(o) + upd * step);
+    cplx<T>* o = O + (size_t)Y * nxo + X;
+    st(o, ld(o) + cplx<T>{nr[r].s, ni[r].s} * step);
   }
 }
 
