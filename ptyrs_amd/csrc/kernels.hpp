@@ -594,7 +594,8 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie_sum(
     cplx<T>* out = psi + ((size_t)(sb * M) * N + y) * N + jl;
     const cplx<T>* t = terms + buf * TW + l * N + jl;
 #pragma unroll
-    for (int e = m; e < E; e += M) {
+    for (int e = 0; e < E; ++e) {
+      if (e % M != m) continue;
       cplx<T> sum = t[TL * e];
 #pragma unroll
       for (int k = 1; k < M; ++k) sum = sum + t[k * G * N + TL * e];
