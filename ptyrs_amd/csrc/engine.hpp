@@ -1097,8 +1097,13 @@ class Engine : public EngineBase {
     }
   }
 
+  static constexpr size_t kIllumChunk = 1024;
+  size_t illum_chunk() const { return std::max<size_t>(1, std::min(nframe_, kIllumChunk)); }
+
   void ensure_lsqml(size_t cap) {
     const size_t no = nyo_ * nxo_;
+    list_all_.ensure(sizeof(int) * size_t(ntiles()) * illum_chunk());
+    count_all_.ensure(sizeof(int) * size_t(ntiles()));
     dO_.ensure(sizeof(cplx<T>) * nslice_ * no);
     pre_.ensure(sizeof(T) * no);
     illum_.ensure(sizeof(T) * no);
@@ -1128,21 +1133,22 @@ class Engine : public EngineBase {
     const size_t no = nyo_ * nxo_;
     PTYRS_HIP(hipMemsetAsync(illum_.p, 0, sizeof(T) * no, stream_));
     const int nt = ntiles();
-    const int step = int(cap_);
+    // Up to kIllumChunk frames' views at once: one tile list and one illumination pass each.
+    const int step = int(illum_chunk());
     for (int f0 = 0; f0 < int(nframe_); f0 += step) {
       const int nb = std::min(step, int(nframe_) - f0);
       const int* ord = iota_.as<int>() + f0;
       run(K_BIN, [&] {
         hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
-                           corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
-                           lcount_.as<int>(), (const T*)nullptr, M_, int(nyo_),
-                           int(nxo_), (T*)nullptr, tilemax_.as<T>());
+                           corner_.as<int2>(), ord, nb, N, ntx_, step, list_all_.as<int>(),
+                           count_all_.as<int>(), (const T*)nullptr, M_, int(nyo_), int(nxo_),
+                           (T*)nullptr, tilemax_.as<T>());
       });
       run(K_OTHER, [&] {
         hipLaunchKernelGGL(k_illumination<T>, dim3(nt), dim3(THREADS), 0, stream_,
                            illum_.as<T>(), P_.as<cplx<T>>(), N, M_, corner_.as<int2>(), ord,
-                           list_.as<int>(), lcount_.as<int>(), int(cap_), int(nyo_), int(nxo_),
-                           ntx_);
+                           list_all_.as<int>(), count_all_.as<int>(), step, int(nyo_),
+                           int(nxo_), ntx_);
       });
     }
     const double init = max_frame_total_ / (4.0 * double(n) * scale_);
@@ -1790,7 +1796,7 @@ class Engine : public EngineBase {
         constexpr int MM = decltype(mm)::value;
         constexpr int G = rows_per_group(N, SR::L, MM);
         hipLaunchKernelGGL((k_pass_c_rpie_sum<T, N, MM>), dim3(N / G, nchunk),
-                           dim3(MM * G * SR::TL), 0, stream_, psi_.as<cplx<T>>(), O_.as<cplx<T>>(), int(nxo_),
+                           dim3(pass_c_sum_threads<T, N, MM>()), 0, stream_, psi_.as<cplx<T>>(), O_.as<cplx<T>>(), int(nxo_),
                            Wt_.as<cplx<T>>(), q_.as<cplx<T>>(), corner_.as<int2>(), ord, nb,
                            omax_.as<T>(), int(S::NBLK), chunk, T(p.probe_alpha),
                            int(upd_probe), dP_.as<cplx<T>>(), tw_.as<cplx<T>>(), int(upd_obj),

@@ -554,8 +554,14 @@ constexpr int rows_per_group(int n, int l, int m) {
 // terms alternate between two LDS buffers. With `probe_dir` each group adds conj(O·q)·Δ_m
 // divided by (1 − α_p)|O|² + α_p·max_window |O|² into its mode's per-chunk probe direction.
 // ptyrs/src/engine/ptycho.rs:588-687
+// Threads of k_pass_c_rpie_sum: M groups of G rows of pass C's line width.
 template <typename T, int N, int M>
-__global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie_sum(
+constexpr int pass_c_sum_threads() {
+  return M * rows_per_group(N, RowShapeC<T, N>::L, M) * RowShapeC<T, N>::TL;
+}
+
+template <typename T, int N, int M>
+__global__ __launch_bounds__((pass_c_sum_threads<T, N, M>())) void k_pass_c_rpie_sum(
     cplx<T>* __restrict__ psi, const cplx<T>* __restrict__ O, int nxo,
     const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ q,
     const int2* __restrict__ corner, const int* __restrict__ order, int nb,
@@ -567,7 +573,13 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie_sum(
   constexpr int G = rows_per_group(N, S::L, M);
   constexpr int n = N * N;
   constexpr int TW = M * G * N;  // terms per buffer
-  __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
+  // The M·G lines exchange through pass C's LDS placement when they fit its L lines, else
+  // (M > L, one row per group) through one plain padded region per line.
+  constexpr bool FITS = M * G <= S::L;
+  constexpr int PLD = N + (N >> 4) + 1;
+  using At = std::conditional_t<FITS, typename S::AtRow, LdsPlain>;
+  constexpr int LDW = FITS ? S::LDS_WORDS : M * G * PLD;
+  __shared__ T lre[LDW], lim[LDW];
   __shared__ cplx<T> stw[N];
   __shared__ cplx<T> terms[2 * TW];
   const int tid = threadIdx.x;
@@ -646,8 +658,8 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_rpie_sum(
     }
     // The transform's barriers also order the previous slot's term writes before the reads
     // below, and this slot's reads of buffer (b − 1) before the writes of slot b + 1.
-    line_fft<T, N, E, true, typename S::AtRow>(v, jl, lre + S::base_row(line),
-                                               lim + S::base_row(line), stw);
+    const int lb = FITS ? S::base_row(line) : line * PLD;
+    line_fft<T, N, E, true, At>(v, jl, lre + lb, lim + lb, stw);
     if (upd_obj && b > b0) store_sum(b - 1, (b - 1) & 1);
     cplx<T>* trow = terms + (b & 1) * TW + line * N + jl;
 #pragma unroll
@@ -1170,14 +1182,22 @@ __global__ __launch_bounds__(THREADS) void k_illumination(
     const int k = nl - s < THREADS ? nl - s : THREADS;
     if (tid < k) cs[tid] = corner[order[list[(size_t)t * cap + s + tid]]];
     __syncthreads();
-    for (int i = 0; i < k && X < nxo; ++i) {
-      const int x = X - cs[i].y;
-      if (x < 0 || x >= N) continue;
+    // SU views per step: their loads issued together, added in view order.
+    constexpr int SU = 8;
+    for (int i0 = 0; i0 < k && X < nxo; i0 += SU) {
 #pragma unroll
       for (int r = 0; r < TROWS; ++r) {
-        const int y = Y0 + RS * r - cs[i].x;
-        if (y >= 0 && y < N)
-          for (int m = 0; m < M; ++m) sum[r] += norm2(ld(P + (size_t)m * n + y * N + x));
+        T a[SU];
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+          const int i = i0 + u < k ? i0 + u : k - 1;  // cs holds k entries
+          const int x = X - cs[i].y, y = Y0 + RS * r - cs[i].x;
+          a[u] = 0;
+          if (i0 + u < k && x >= 0 && x < N && y >= 0 && y < N)
+            for (int m = 0; m < M; ++m) a[u] += norm2(ld(P + (size_t)m * n + y * N + x));
+        }
+#pragma unroll
+        for (int u = 0; u < SU; ++u) sum[r] += a[u];
       }
     }
     __syncthreads();

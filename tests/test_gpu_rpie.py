@@ -170,3 +170,19 @@ def test_banded_angular_spectrum_runs_lsqml_and_llk_only():
     assert np.isfinite(g.llk()["poisson"])
     g.lsqml(1, 1, 8, "poisson", 7)
     assert len(g.history["poisson"]) == 1
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("nprobe", [1, 3, 4])
+@pytest.mark.parametrize("n", [16, 24, 32, 48, 64, 96, 128, 256, 512])
+def test_rpie_every_side_and_mode_count_matches_oracle(n, nprobe, dtype):
+    """Two rPIE epochs in batches of 4 at every frame side and 1, 3 or 4 modes: the launch
+    shapes of every (side, modes, precision) the passes instantiate."""
+    d = small_scan(n=n, nprobe=nprobe, npos=6, masked=False)
+    g, o = gpu(d, dtype), oracle(d)
+    o0, p0 = o.obj.copy(), o.probe.copy()
+    g.rpie(2, batch_size=4, calc_llk=1)
+    o.rpie(2, batch_size=4, calc_llk=1)
+    assert max_rel(g.history["poisson"], [r[2].poisson for r in o.history]) < TOL[dtype]
+    assert rel_l2(g.object - o0, o.obj - o0) < TOL[dtype] * 10
+    assert rel_l2(g.probe - p0, o.probe - p0) < TOL[dtype] * 10
