@@ -11,6 +11,7 @@ global-phase alignment on the illuminated region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
     python bench.py --multi-scan [--steps K]   # 16 uneven scans: sequential vs grouped epochs
+    python bench.py --config5 [--steps K]      # multislice LSQML with position refinement
     python bench.py --copy-probe     # a plain 2 GiB device copy: the PMC calibration kernel and
                                      # the achievable bandwidth (run as a child by default)
 
@@ -101,11 +102,22 @@ def aligned_nrmse(a, b, weight):
 
 
 def illumination(corners, probe, shape):
+    """Σ over views of Σ_m |P_m|² placed at the views' corners: the NRMSE weight."""
+    n = probe.shape[-1]
     ov = np.zeros(shape[-2:])
     p2 = (np.abs(probe) ** 2).sum(axis=0)
     for oy, ox in corners:
-        ov[oy:oy + N, ox:ox + N] += p2
+        ov[oy:oy + n, ox:ox + n] += p2
     return ov
+
+
+def phase_nrmse(a, b, weight):
+    """NRMSE of `a` against `b` after a global phase (weighted by `weight`): for two runs from
+    the same start, where no phase ramp separates them."""
+    w = weight / weight.sum()
+    g = (w * a * np.conj(b)).sum()
+    a2 = a * np.exp(-1j * np.angle(g))
+    return float(np.sqrt((w * np.abs(a2 - b) ** 2).sum() / (w * np.abs(b) ** 2).sum()))
 
 
 def cpu_threads():
@@ -231,6 +243,74 @@ def multi_scan(epochs=3, nscan=16, seed=404):
     print(json.dumps(out), flush=True)
 
 
+def config5(epochs=2, npos=5000, seed=505):
+    """The large multislice configuration: 512² probe × 4 modes, 4096² object × 8 slices
+    (exp(iφ − 0.05φ), φ ≤ 0.1 rad) at 1 µm, 10 keV, 5 nm pixels, `npos` positions on a Fermat
+    spiral (25 px apart) whose model positions start perturbed by N(0, 3 px); Poisson data from
+    the engine's own model at the true positions. LSQML (Poisson, batches of 100) with every
+    position refined, slice waves checkpointed every 2 slices, `epochs` epochs in complex64 and
+    in complex128 from the same start. Prints one JSON line: positions·iter/s of both, the
+    position RMSE against the truth before and after, the LLK, and the complex64 object's NRMSE
+    against the complex128 one (aligned, illuminated region)."""
+    import ptyrs_amd as pa
+    from ptyrs_amd import sim
+    n, side, nslice, pixel = 512, 4096, 8, 5e-9
+    lam = 1.2398e-10
+    dpix = 75e-6
+    z = pixel * n * dpix / lam
+    probe = sim.hermite_gauss_probe(n) * 1e3
+    phi = np.angle(sim.random_object(side, side, seed, amp=(1.0, 1.0), phase=(0.0, 0.1)))
+    objects = np.stack([np.exp(1j * np.roll(phi, (37 * k, 53 * k), axis=(-2, -1))
+                               - 0.05 * np.roll(phi, (37 * k, 53 * k), axis=(-2, -1)))
+                        for k in range(nslice)])
+    del phi
+    y, x = sim.fermat_positions(npos, 25.0, pixel)
+    optics = pa.Optics.far_field(lam, z, dpix, [n, n]).with_slices(
+        [pa.Wavefield([n, n], lam, [pixel, pixel], 1e-6)] * (nslice - 1))
+    t0 = time.perf_counter()
+    truth = pa.GpuPtycho(pa.Scan(np.zeros((npos, n, n), np.float32), y, x, lam, z, dpix),
+                         objects, probe, optics=optics)
+    inten = truth.intensities()
+    del truth
+    counts = np.empty(inten.shape, np.float32)
+    rng = np.random.default_rng(seed + 1)
+    for f0 in range(0, npos, 250):
+        counts[f0:f0 + 250] = rng.poisson(inten[f0:f0 + 250])
+    del inten
+    print(f"[bench] config 5 data ready in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    d0 = np.random.default_rng(seed + 2).normal(0.0, 3.0, (npos, 2))
+    scan = pa.Scan(counts, y, x, lam, z, dpix)
+    out = {"metric": "positions/sec/iter, multislice LSQML with position refinement "
+                     "(512² x 4 modes, 4096² x 8 slices, 5000 positions)",
+           "unit": "positions·iter/s", "epochs": epochs, "data": "synthetic",
+           "position_rmse_start_px": float(np.sqrt((d0 ** 2).mean()))}
+    res = {}
+    for dtype in ("c64", "c128"):
+        g = pa.GpuPtycho(scan, objects, probe, optics=optics, dtype=dtype)
+        g.set_checkpoint(2)
+        g.set_features(position=d0)
+        l0 = g.llk()["poisson"]
+        g.synchronize()
+        t = time.perf_counter()
+        g.lsqml(epochs, 0, 100, "poisson", 7)
+        g.synchronize()
+        dt = time.perf_counter() - t
+        dpos = g.features["position"]
+        res[dtype] = g
+        out[dtype] = {"value": round(npos * epochs / dt, 1), "seconds": round(dt, 3),
+                      "llk_start": l0, "llk_end": g.llk()["poisson"],
+                      "position_rmse_px": float(np.sqrt((dpos ** 2).mean())),
+                      "wave_bytes": g.checkpoint[1]}
+        print(f"[bench] config 5 {dtype}: {epochs} epochs in {dt:.2f} s", file=sys.stderr,
+              flush=True)
+    o32, o64 = res["c64"].objects, res["c128"].objects
+    weight = illumination(res["c128"].corners, res["c128"].probe, o64.shape[-2:])
+    out["nrmse_objects"] = [phase_nrmse(o32[k, 0], o64[k, 0], weight) for k in range(nslice)]
+    out["value"] = out["c64"]["value"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,6 +321,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--multi-scan", action="store_true",
                     help="the sixteen-scan workload instead: sequential vs grouped epochs")
+    ap.add_argument("--config5", action="store_true",
+                    help="the large multislice workload with position refinement instead")
     ap.add_argument("--no-copy", action="store_true",
                     help="skip the child copy-bandwidth process (counter passes)")
     ap.add_argument("--batch", type=int, default=BATCH, help="rPIE batch size (positions)")
@@ -254,6 +336,9 @@ def main():
         return
     if a.multi_scan:
         multi_scan(a.steps)
+        return
+    if a.config5:
+        config5(a.steps)
         return
 
     copy = None if a.no_copy else copy_bandwidth()
