@@ -996,8 +996,8 @@ class Engine : public EngineBase {
     run(K_PASS_A, [&] {
       for (size_t t = c; t < top; ++t) {
         auto in = wave(t, w);
-        window<MS_MUL>(incident(t + 1), t, in.first, in.second, nullptr, 0, ord, nb);
-        gap(incident(t + 1), nb * M_, t, false);
+        window_rows<MS_MUL>(incident(t + 1), t, in.first, in.second, nullptr, 0, ord, nb);
+        gap_tail(incident(t + 1), nb * M_, t, false);
       }
     });
     seg_ = c;
@@ -1005,10 +1005,14 @@ class Engine : public EngineBase {
 
   // An angular-spectrum gap's propagation of `nplane` planes in place, or its adjoint.
   void gap(cplx<T>* planes, int nplane, size_t g, bool adjoint) {
-    const cplx<T>* tf = gaptf_.as<cplx<T>>() + g * n;
-    const T inv_n = T(1) / T(n);
     hipLaunchKernelGGL((k_rows<T, N, false>), dim3(S::NBLK, nplane), dim3(S::NT), 0, stream_,
                        planes, tw_.as<cplx<T>>());
+    gap_tail(planes, nplane, g, adjoint);
+  }
+  // The gap after its forward row DFTs (done by k_rows_ms with the window product before it).
+  void gap_tail(cplx<T>* planes, int nplane, size_t g, bool adjoint) {
+    const cplx<T>* tf = gaptf_.as<cplx<T>>() + g * n;
+    const T inv_n = T(1) / T(n);
     if (adjoint)
       hipLaunchKernelGGL((k_gap_cols<T, N, true>), dim3(SC::NBLK, nplane), dim3(SC::NT), 0,
                          stream_, planes, tf, tw_.as<cplx<T>>(), inv_n);
@@ -1017,6 +1021,15 @@ class Engine : public EngineBase {
                          stream_, planes, tf, tw_.as<cplx<T>>(), inv_n);
     hipLaunchKernelGGL((k_rows<T, N, true>), dim3(S::NBLK, nplane), dim3(S::NT), 0, stream_,
                        planes, tw_.as<cplx<T>>());
+  }
+
+  // window<OP> followed by the forward row DFTs of its output, in one pass.
+  template <int OP>
+  void window_rows(cplx<T>* out, size_t s, const cplx<T>* w, int w_view, const cplx<T>* q,
+                   int q_frame, const int* ord, int nb) {
+    hipLaunchKernelGGL((k_rows_ms<T, N, OP>), dim3(S::NBLK, nb * M_), dim3(S::NT), 0, stream_,
+                       out, (const cplx<T>*)slice_obj(s), int(nxo_), w, w_view, q,
+                       corner_.as<int2>(), ord, M_, q_frame, tw_.as<cplx<T>>());
   }
 
   template <int OP>
@@ -1040,8 +1053,8 @@ class Engine : public EngineBase {
     run(K_PASS_A, [&] {
       for (size_t s = 0; s + 1 < S; ++s) {
         auto in = wave(s, w);
-        window<MS_MUL>(incident(s + 1), s, in.first, in.second, nullptr, 0, ord, nb);
-        gap(incident(s + 1), nb * M_, s, false);
+        window_rows<MS_MUL>(incident(s + 1), s, in.first, in.second, nullptr, 0, ord, nb);
+        gap_tail(incident(s + 1), nb * M_, s, false);
       }
       auto in = wave(S - 1, w);
       window<MS_MUL>(psi_.as<cplx<T>>(), S - 1, in.first, in.second, w.qlast, w.q_frame, ord,
@@ -1302,8 +1315,8 @@ class Engine : public EngineBase {
         });
         if (s == 0) break;
         run(K_PASS_C, [&] {
-          window<MS_ADJ>(chi, s, nullptr, 0, qs, 0, ord, nb);
-          gap(chi, K, s - 1, true);
+          window_rows<MS_ADJ>(chi, s, nullptr, 0, qs, 0, ord, nb);
+          gap_tail(chi, K, s - 1, true);
         });
       }
       run(K_PROBE, [&] {
@@ -1440,8 +1453,8 @@ class Engine : public EngineBase {
                            step + 1 + s);
       });
       run(K_PASS_C, [&] {
-        window<MS_ADJ>(chi, s, nullptr, 0, qs, qf, ord, nb);
-        gap(chi, K, s - 1, true);
+        window_rows<MS_ADJ>(chi, s, nullptr, 0, qs, qf, ord, nb);
+        gap_tail(chi, K, s - 1, true);
       });
     }
     // Slice 0: the probe direction (moved back per frame with offsets), then the joint step.

@@ -1336,6 +1336,45 @@ __global__ void k_ms_window(cplx<T>* __restrict__ out, const cplx<T>* __restrict
   }
 }
 
+// The window product of k_ms_window (MS_MUL: out = O_s[win]·w·q; MS_ADJ: out = conj(O_s[win]·q)
+// ·out, in place) fused with the forward row DFT of its result, the first pass of the gap that
+// follows: one pass over the planes instead of two. Grid (NBLK, nb·M).
+template <typename T, int N, int OP>
+__global__ __launch_bounds__((RowShape<T, N>::NT)) void k_rows_ms(
+    cplx<T>* __restrict__ out, const cplx<T>* __restrict__ O, int nxo,
+    const cplx<T>* __restrict__ w, int w_view, const cplx<T>* __restrict__ q,
+    const int2* __restrict__ corner, const int* __restrict__ order, int M, int q_frame,
+    const cplx<T>* __restrict__ tw) {
+  using S = RowShape<T, N>;
+  constexpr int E = S::E, TL = S::TL;
+  constexpr int n = N * N;
+  __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
+  __shared__ cplx<T> stw[N];
+  const int tid = threadIdx.x, line = tid / TL, jl = tid % TL;
+  const int y = blockIdx.x * S::L + line;
+  const int k = blockIdx.y, b = k / M, m = k % M;
+  const int2 c = corner[order[b]];
+  const cplx<T>* orow = O + (size_t)(c.x + y) * nxo + c.y;
+  const cplx<T>* qrow = q ? q + (q_frame ? (size_t)b * n : 0) + (size_t)y * N : nullptr;
+  cplx<T>* row = out + (size_t)k * n + (size_t)y * N;
+  cplx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int x = jl + TL * e;
+    cplx<T> o = ld(orow + x);
+    if (qrow) o = o * ld(qrow + x);
+    if constexpr (OP == MS_ADJ)
+      v[e] = cmulc(o, ld(row + x));
+    else
+      v[e] = o * ld(w + (size_t)(w_view ? k : m) * n + (size_t)y * N + x);
+  }
+  stage_twiddles<T, N, S::NT>(stw, tw);
+  line_fft<T, N, E, false, typename S::AtRow>(v, jl, lre + S::base_row(line),
+                                              lim + S::base_row(line), stw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) st(row + jl + TL * e, v[e]);
+}
+
 // The probe direction of slice 0: dP_m = scale·Σ_b conj(O_0[win_b]·q)·χ_0[b·M + m], summed in
 // slot order (Kahan); scale 1/nb for LSQML's frame mean, 1 for a gradient's sum, which
 // `accumulate` adds to dP. Grid (ceil(n/256), M). ptyrs/src/engine/lsqml.rs:200-229
