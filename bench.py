@@ -11,7 +11,7 @@ global-phase alignment on the illuminated region.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
     python bench.py --multi-scan [--steps K]   # 16 uneven scans: sequential vs grouped epochs
-    python bench.py --config5 [--steps K]      # multislice LSQML with position refinement
+    python bench.py --large-multislice [--steps K]   # 512² x 4, 4096² x 8 slices, refined positions
     python bench.py --copy-probe     # a plain 2 GiB device copy: the PMC calibration kernel and
                                      # the achievable bandwidth (run as a child by default)
 
@@ -243,7 +243,7 @@ def multi_scan(epochs=3, nscan=16, seed=404):
     print(json.dumps(out), flush=True)
 
 
-def config5(epochs=2, npos=5000, seed=505):
+def large_multislice(epochs=2, npos=5000, seed=505):
     """The large multislice configuration: 512² probe × 4 modes, 4096² object × 8 slices
     (exp(iφ − 0.05φ), φ ≤ 0.1 rad) at 1 µm, 10 keV, 5 nm pixels, `npos` positions on a Fermat
     spiral (25 px apart) whose model positions start perturbed by N(0, 3 px); Poisson data from
@@ -277,7 +277,7 @@ def config5(epochs=2, npos=5000, seed=505):
     for f0 in range(0, npos, 250):
         counts[f0:f0 + 250] = rng.poisson(inten[f0:f0 + 250])
     del inten
-    print(f"[bench] config 5 data ready in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+    print(f"[bench] large multislice data ready in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
           flush=True)
     d0 = np.random.default_rng(seed + 2).normal(0.0, 3.0, (npos, 2))
     scan = pa.Scan(counts, y, x, lam, z, dpix)
@@ -302,7 +302,7 @@ def config5(epochs=2, npos=5000, seed=505):
                       "llk_start": l0, "llk_end": g.llk()["poisson"],
                       "position_rmse_px": float(np.sqrt((dpos ** 2).mean())),
                       "wave_bytes": g.checkpoint[1]}
-        print(f"[bench] config 5 {dtype}: {epochs} epochs in {dt:.2f} s", file=sys.stderr,
+        print(f"[bench] large multislice {dtype}: {epochs} epochs in {dt:.2f} s", file=sys.stderr,
               flush=True)
     o32, o64 = res["c64"].objects, res["c128"].objects
     weight = illumination(res["c128"].corners, res["c128"].probe, o64.shape[-2:])
@@ -321,7 +321,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--multi-scan", action="store_true",
                     help="the sixteen-scan workload instead: sequential vs grouped epochs")
-    ap.add_argument("--config5", action="store_true",
+    ap.add_argument("--large-multislice", action="store_true",
                     help="the large multislice workload with position refinement instead")
     ap.add_argument("--no-copy", action="store_true",
                     help="skip the child copy-bandwidth process (counter passes)")
@@ -329,7 +329,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every GPU box of this build has one, and the "
-              "reconstruction step is not sharded here (see DESIGN.md)", file=sys.stderr)
+              "reconstruction step is not sharded here", file=sys.stderr)
         sys.exit(2)
     if a.copy_probe:
         print(json.dumps(copy_probe()), flush=True)
@@ -337,8 +337,8 @@ def main():
     if a.multi_scan:
         multi_scan(a.steps)
         return
-    if a.config5:
-        config5(a.steps)
+    if a.large_multislice:
+        large_multislice(a.steps)
         return
 
     copy = None if a.no_copy else copy_bandwidth()

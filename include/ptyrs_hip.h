@@ -293,7 +293,7 @@ int ptyrs_engine_synchronize(ptyrs_engine* e);
 /* Kernel timing with HIP events on the engine's stream, for measurement: enable = 1 starts
  * recording every launch, 0 stops. ptyrs_engine_kernel_stats reports, for kernel name `name`
  * (as listed by ptyrs_kernel_names), the launches recorded and their summed duration in ms.
- * No reference counterpart (the reference times with nsys/ncu, DESIGN.md:47). */
+ * No reference counterpart (the reference times its kernels with external profilers). */
 int ptyrs_engine_profile(ptyrs_engine* e, int enable);
 int ptyrs_engine_kernel_stats(ptyrs_engine* e, const char* name, size_t* launches, double* total_ms);
 const char* ptyrs_kernel_names(void);

@@ -1,4 +1,4 @@
-"""Parity at the benchmark configurations' own shapes, on the GPU through the C ABI.
+"""Parity at the benchmark workloads' own shapes, on the GPU through the C ABI.
 
 1. Plain rPIE, 64² probe, 256² object, 10×10 raster, 1e6 photons, 50 epochs: against the FP64
    oracle on the same seeded inputs (complex128 to FP64 rounding, complex64 to 1e-4 on the LLK).
@@ -39,7 +39,7 @@ def _scan(d):
 
 
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
-def test_config1_rpie_50_epochs_matches_oracle(dtype):
+def test_rpie_64_probe_100_positions_50_epochs_matches_oracle(dtype):
     from oracle.ptycho import far_field_ptycho
     d = sim.scan("small")
     g = pa.GpuPtycho(_scan(d), d["start_obj"], d["start_probe"], dtype=dtype)
@@ -49,7 +49,7 @@ def test_config1_rpie_50_epochs_matches_oracle(dtype):
     o.rpie(50, calc_llk=1)
     want = [r[2].poisson for r in o.history]
     tol = {"c128": 1e-9, "c64": 1e-4}[dtype]
-    print("config1", dtype, max_rel(g.history["poisson"], want), rel_l2(g.object, o.obj),
+    print("rpie 64²", dtype, max_rel(g.history["poisson"], want), rel_l2(g.object, o.obj),
           rel_l2(g.probe, o.probe))
     assert max_rel(g.history["poisson"], want) < tol
     assert rel_l2(g.object, o.obj) < tol * 30
@@ -68,7 +68,7 @@ def aligned_nrmse(a, b, weight):
     return float(np.sqrt((w * np.abs(a2 - b) ** 2).sum() / (w * np.abs(b) ** 2).sum()))
 
 
-def test_config2_headline_fp32_against_fp64():
+def test_headline_256_probe_4_modes_fp32_against_fp64():
     d = sim.scan("headline")
     scan = _scan(d)
     out = {}
@@ -84,7 +84,7 @@ def test_config2_headline_fp32_against_fp64():
     p2 = (np.abs(b[1]) ** 2).sum(axis=0)
     for oy, ox in b[5]:
         ov[oy:oy + n, ox:ox + n] += p2
-    print("config2 rpie", aligned_nrmse(a[0], b[0], ov), abs(a[2] - b[2]) / abs(b[2]),
+    print("headline rpie", aligned_nrmse(a[0], b[0], ov), abs(a[2] - b[2]) / abs(b[2]),
           "lsqml", aligned_nrmse(a[3], b[3], ov), abs(a[4] - b[4]) / abs(b[4]))
     assert aligned_nrmse(a[0], b[0], ov) < 1e-4
     assert abs(a[2] - b[2]) < 1e-6 * abs(b[2])
@@ -92,7 +92,7 @@ def test_config2_headline_fp32_against_fp64():
     assert abs(a[4] - b[4]) < 1e-6 * abs(b[4])
 
 
-# ---- configuration 3 -----------------------------------------------------------------------------
+# ---- multislice gradient: 128² × 2 modes, 512² × 3 slices, 400 positions -------------------------
 
 LAMBDA_10KEV = 1.2398e-10
 
@@ -110,7 +110,7 @@ def far_geometry(n, pixel):
     return LAMBDA_10KEV, pixel * n * dpix / LAMBDA_10KEV, dpix
 
 
-def config3_state():
+def gradient_workload():
     n, side, nslice, npos = 128, 512, 3, 400
     pixel = 10e-9
     lam, z, dpix = far_geometry(n, pixel)
@@ -134,13 +134,13 @@ def counts_from_model(scan_args, objects, probe, optics, dtype, seed, photons=No
     return np.random.default_rng(seed).poisson(inten).astype(np.float32), g.corners
 
 
-def test_config3_gradient_full_size():
+def test_multislice_gradient_128_probe_3_slices_400_positions():
     torch = pytest.importorskip("torch")
     from ptyrs_amd.autograd import cost
     from oracle.base import observed
     from oracle.lsqml import cost_grads
     from oracle.optics import Optics, Wavefield
-    n, objects, probe, y, x, optics, geom = config3_state()
+    n, objects, probe, y, x, optics, geom = gradient_workload()
     counts, corners = counts_from_model((y, x, geom, n), objects, probe, optics, "c128", 41)
     rng = np.random.default_rng(43)
     start = objects * np.exp(1j * rng.uniform(-0.05, 0.05, objects.shape))
@@ -193,9 +193,9 @@ def test_config3_gradient_full_size():
     assert cost(g32, ot.detach(), pt).item() < first
 
 
-# ---- configuration 4 -----------------------------------------------------------------------------
+# ---- sixteen independent scans of uneven sizes -------------------------------------------------------
 
-def test_config4_sixteen_uneven_scans():
+def test_sixteen_uneven_scans():
     rng = np.random.default_rng(404)
     for k in range(16):
         n = int(rng.choice([128, 256]))
@@ -221,9 +221,9 @@ def test_config4_sixteen_uneven_scans():
         assert h[-1] < l0 and np.all(np.isfinite(h)), (k, n, nmode, npos, l0, list(h))
 
 
-# ---- configuration 5 -----------------------------------------------------------------------------
+# ---- 512² × 4 modes, 4096² × 8 slices, refined positions -----------------------------------------
 
-def test_config5_multislice_positions_shape():
+def test_large_multislice_with_refined_positions():
     n, side, nslice, npos = 512, 4096, 8, 300
     pixel = 5e-9
     lam, z, dpix = far_geometry(n, pixel)
@@ -245,7 +245,7 @@ def test_config5_multislice_positions_shape():
     g.lsqml(3, 1, 100, "poisson", 7)
     h = g.history["poisson"]
     d = g.features["position"]
-    print("config5", l0, list(h), np.sqrt((d0 ** 2).mean()), np.sqrt((d ** 2).mean()))
+    print("large multislice", l0, list(h), np.sqrt((d0 ** 2).mean()), np.sqrt((d ** 2).mean()))
     assert np.all(np.isfinite(h)) and h[-1] < l0
     rms0 = np.sqrt(((d0 - truth) ** 2).mean())
     assert np.sqrt(((d - truth) ** 2).mean()) < rms0
