@@ -192,6 +192,10 @@ int ptyrs_engine_rpie(ptyrs_engine* e, const ptyrs_rpie_params* params, size_t n
  * one scan per GpuPtycho call (ptyrs/src/engine/gpu/rpie.rs:31-62). */
 int ptyrs_group_rpie(ptyrs_engine* const* engines, size_t count,
                      const ptyrs_rpie_params* params, size_t nepoch);
+/* The same with every engine's epoch captured in one graph, a branch per engine, launched once
+ * per epoch (same results). No counterpart in the reference. */
+int ptyrs_group_rpie_graph(ptyrs_engine* const* engines, size_t count,
+                           const ptyrs_rpie_params* params, size_t nepoch);
 
 /* GpuPtycho::ap, ::dm, ::raar (ptyrs/src/engine/gpu/ap.rs:28, dm.rs:32, raar.rs:17): ncycle
  * cycles of PyNX AP, DM or RAAR, the exit waves of DM and RAAR carried across its cycles. */

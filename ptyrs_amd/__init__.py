@@ -128,6 +128,8 @@ _SIGNATURES = [
     ("ptyrs_engine_grad", [_P, ctypes.c_int, _D, _D, _SZ, _D, _SZ], ctypes.c_int),
     ("ptyrs_engine_intensities", [_P, _D, _SZ], ctypes.c_int),
     ("ptyrs_group_rpie", [ctypes.POINTER(_P), _SZ, ctypes.POINTER(_RpieC), _SZ], ctypes.c_int),
+    ("ptyrs_group_rpie_graph", [ctypes.POINTER(_P), _SZ, ctypes.POINTER(_RpieC), _SZ],
+     ctypes.c_int),
     ("ptyrs_engine_set_checkpoint", [_P, _SZ], ctypes.c_int),
     ("ptyrs_engine_checkpoint", [_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)], ctypes.c_int),
     ("ptyrs_engine_llk", [_P, ctypes.POINTER(_RecordC)], ctypes.c_int),
@@ -670,10 +672,12 @@ class GpuPtycho:
         return out
 
 
-def rpie_group(engines, nepoch, params=None):
+def rpie_group(engines, nepoch, params=None, one_graph=False):
     """`nepoch` rPIE epochs of several independent scans at once (ptyrs_group_rpie): each
-    engine's epochs as `GpuPtycho.rpie` runs them, the engines side by side on the device.
-    `params` is one dict of RpieParams fields for all, or one dict per engine."""
+    engine's epochs as `GpuPtycho.rpie` runs them, the engines side by side on the device, each
+    epoch a graph replay on the engine's own stream — or, with `one_graph`, every engine's epoch
+    in one graph launch (ptyrs_group_rpie_graph). `params` is one dict of RpieParams fields for
+    all, or one dict per engine."""
     engines = list(engines)
     if params is None or isinstance(params, dict):
         params = [params or {}] * len(engines)
@@ -681,4 +685,5 @@ def rpie_group(engines, nepoch, params=None):
         raise ValueError(f"{len(params)} parameter sets for {len(engines)} engines")
     hs = (ctypes.c_void_p * max(len(engines), 1))(*[g._h.value for g in engines])
     cs = (_RpieC * max(len(engines), 1))(*[GpuPtycho._rpie_c(p) for p in params])
-    _check(library().ptyrs_group_rpie(hs, len(engines), cs, int(nepoch)))
+    f = library().ptyrs_group_rpie_graph if one_graph else library().ptyrs_group_rpie
+    _check(f(hs, len(engines), cs, int(nepoch)))

@@ -203,8 +203,87 @@ int ptyrs_engine_rpie(ptyrs_engine* h, const ptyrs_rpie_params* p, size_t nepoch
   return guarded(h, true, [&](EngineBase& e) { e.rpie(*p, nepoch); });
 }
 
-int ptyrs_group_rpie(ptyrs_engine* const* engines, size_t count,
-                     const ptyrs_rpie_params* params, size_t nepoch) {
+namespace {
+
+// Every engine's next epoch captured in one graph, a branch per engine on its own stream
+// (forked from and joined back to a group stream), replayed while the engines' keys hold.
+void group_epochs_one_graph(std::vector<EngineBase*>& es) {
+  hipStream_t gs = nullptr;
+  PTYRS_HIP(hipStreamCreateWithFlags(&gs, hipStreamNonBlocking));
+  std::vector<hipEvent_t> ev(2 * es.size() + 2, nullptr);
+  for (auto& e : ev) PTYRS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  hipEvent_t fork = ev[0], done = ev[1];
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint64_t> held;
+  auto cleanup = [&] {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    (void)hipStreamSynchronize(gs);
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(gs);
+  };
+  try {
+    for (;;) {
+      std::vector<size_t> act;
+      for (size_t k = 0; k < es.size(); ++k)
+        if (es[k]->rpie_pending()) act.push_back(k);
+      if (act.empty()) break;
+      std::vector<uint64_t> key;
+      for (size_t k : act) {
+        const auto kk = es[k]->rpie_epoch_order();
+        key.push_back(uint64_t(k));
+        key.push_back(kk.size());
+        key.insert(key.end(), kk.begin(), kk.end());
+        PTYRS_HIP(hipEventRecord(ev[2 + k], es[k]->stream()));
+        PTYRS_HIP(hipStreamWaitEvent(gs, ev[2 + k], 0));
+      }
+      if (exec == nullptr || key != held) {
+        if (exec) PTYRS_HIP(hipGraphExecDestroy(exec));
+        exec = nullptr;
+        hipGraph_t graph = nullptr;
+        PTYRS_HIP(hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal));
+        try {
+          PTYRS_HIP(hipEventRecord(fork, gs));
+          for (size_t k : act) {
+            PTYRS_HIP(hipStreamWaitEvent(es[k]->stream(), fork, 0));
+            es[k]->rpie_epoch_body();
+            PTYRS_HIP(hipEventRecord(ev[2 + es.size() + k], es[k]->stream()));
+            PTYRS_HIP(hipStreamWaitEvent(gs, ev[2 + es.size() + k], 0));
+          }
+        } catch (...) {
+          hipGraph_t dropped = nullptr;
+          (void)hipStreamEndCapture(gs, &dropped);
+          if (dropped) (void)hipGraphDestroy(dropped);
+          throw;
+        }
+        PTYRS_HIP(hipStreamEndCapture(gs, &graph));
+        const hipError_t inst = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (inst != hipSuccess) {
+          exec = nullptr;
+          throw DeviceError(std::string("hipGraphInstantiate: ") + hipGetErrorString(inst));
+        }
+        held = key;
+      }
+      PTYRS_HIP(hipGraphLaunch(exec, gs));
+      PTYRS_HIP(hipEventRecord(done, gs));
+      for (size_t k : act) {
+        PTYRS_HIP(hipStreamWaitEvent(es[k]->stream(), done, 0));
+        es[k]->rpie_epoch_after();
+      }
+    }
+  } catch (...) {
+    cleanup();
+    throw;
+  }
+  cleanup();
+}
+
+}  // namespace
+
+namespace {
+
+int group_rpie(ptyrs_engine* const* engines, size_t count, const ptyrs_rpie_params* params,
+               size_t nepoch, bool one_graph) {
   try {
     if (!engines || !params) throw Invalid("null argument");
     std::vector<EngineBase*> es;
@@ -222,12 +301,16 @@ int ptyrs_group_rpie(ptyrs_engine* const* engines, size_t count,
     size_t i = 0;
     try {
       for (; i < count; ++i) es[i]->rpie_begin(params[i], nepoch);
-      // Epoch by epoch, round-robin: each engine's epoch is one replay on its own stream, so
-      // the engines' epochs run side by side on the device.
-      bool more = true;
-      while (more) {
-        more = false;
-        for (EngineBase* e : es) more = e->rpie_epoch() || more;
+      if (one_graph) {
+        group_epochs_one_graph(es);
+      } else {
+        // Epoch by epoch, round-robin: each engine's epoch is one replay on its own stream, so
+        // the engines' epochs run side by side on the device.
+        bool more = true;
+        while (more) {
+          more = false;
+          for (EngineBase* e : es) more = e->rpie_epoch() || more;
+        }
       }
       for (EngineBase* e : es) e->rpie_end();
     } catch (const Invalid&) {
@@ -247,6 +330,18 @@ int ptyrs_group_rpie(ptyrs_engine* const* engines, size_t count,
     last_error = x.what();
     return PTYRS_EDEVICE;
   }
+}
+
+}  // namespace
+
+int ptyrs_group_rpie(ptyrs_engine* const* engines, size_t count,
+                     const ptyrs_rpie_params* params, size_t nepoch) {
+  return group_rpie(engines, count, params, nepoch, false);
+}
+
+int ptyrs_group_rpie_graph(ptyrs_engine* const* engines, size_t count,
+                           const ptyrs_rpie_params* params, size_t nepoch) {
+  return group_rpie(engines, count, params, nepoch, true);
 }
 
 namespace {

@@ -122,6 +122,11 @@ class EngineBase {
   virtual void rpie_begin(const ptyrs_rpie_params& p, size_t nepoch) = 0;
   virtual bool rpie_epoch() = 0;  // enqueues the next epoch; false once none is left
   virtual void rpie_end() = 0;
+  virtual std::vector<uint64_t> rpie_epoch_order() = 0;
+  virtual void rpie_epoch_body() = 0;
+  virtual bool rpie_epoch_after() = 0;
+  virtual bool rpie_pending() const = 0;  // an epoch of the call in progress is left
+  hipStream_t stream() const { return stream_; }
   virtual void lsqml(const ptyrs_lsqml_params& p, size_t nepoch, size_t calc_llk) = 0;
   virtual void project(const ProjParams& p, size_t ncycle) = 0;
   virtual void ml(const ptyrs_ml_params& p, size_t ncycle) = 0;
@@ -432,17 +437,31 @@ class Engine : public EngineBase {
   }
 
   bool rpie_epoch() override {
+    if (run_.ic >= run_.nepoch) return false;
+    std::vector<uint64_t> key = rpie_epoch_order();
+    epoch(rpie_graph_, key, [&] { rpie_epoch_body(); });
+    return rpie_epoch_after();
+  }
+
+  // The three parts of an epoch, for a caller that captures several engines' epochs in one
+  // graph: the epoch's order copied into the captured order buffer (returns the graph key of
+  // the body), the body's launches, and the LLK record and counters after it.
+  std::vector<uint64_t> rpie_epoch_order() override {
     const size_t ic = run_.ic;
-    if (ic >= run_.nepoch) return false;
-    const ptyrs_rpie_params& p = run_.p;
     PTYRS_HIP(hipMemcpyAsync(eord_.p, order_.as<int>() + ic * nframe_, sizeof(int) * nframe_,
                              hipMemcpyDeviceToDevice, stream_));
-    const int* ord = eord_.as<int>();
     std::vector<uint64_t> key = run_.base;
     for (auto& s : run_.spans[ic]) key.push_back(uint64_t(s.second));
-    epoch(rpie_graph_, key, [&] {
-      for (auto& s : run_.spans[ic]) rpie_batch(p, ord + s.first, s.second);
-    });
+    key.push_back(DevBuf::generation.load());
+    return key;
+  }
+  void rpie_epoch_body() override {
+    const int* ord = eord_.as<int>();
+    for (auto& s : run_.spans[run_.ic]) rpie_batch(run_.p, ord + s.first, s.second);
+  }
+  bool rpie_epoch_after() override {
+    const size_t ic = run_.ic;
+    const ptyrs_rpie_params& p = run_.p;
     const bool rec = p.calc_llk > 0 && (ic % p.calc_llk == 0 || ic + 1 == run_.nepoch);
     if (rec) {
       if (run_.due.size() == kHistSlots) drain(run_.due, PTYRS_RPIE);
@@ -454,6 +473,8 @@ class Engine : public EngineBase {
     ++run_.ic;
     return run_.ic < run_.nepoch;
   }
+
+  bool rpie_pending() const override { return run_.ic < run_.nepoch; }
 
   void rpie_end() override {
     drain(run_.due, PTYRS_RPIE);

@@ -1,12 +1,13 @@
 """Launch-shape autotuning for several scans reconstructed at once.
 
 `rpie_group` (ptyrs_group_rpie) runs every engine's rPIE epochs side by side, each on its own
-stream. How many scans share the device at a time is a launch choice that leaves every result
-unchanged (each engine's epochs are bit-identical however the scans are grouped), so it can
-be tuned on the reconstruction's own epochs: `tune_group` times one epoch of every scan at
-each candidate width (scans taken largest first, in waves of that many scans), keeps the
-fastest, and caches it per size class — the multiset of (frame side, modes, dtype) and
-positions rounded to a power of two — in a JSON file.
+stream, or (ptyrs_group_rpie_graph) in one graph per epoch. How many scans share the device at
+a time, and which of the two launches, are choices that leave every result unchanged (each
+engine's epochs are bit-identical however the scans are grouped), so they can be tuned on the
+reconstruction's own epochs: `tune_group` times one epoch of every scan for each candidate
+(scans taken largest first, in waves of `width` scans), keeps the fastest, and caches it per
+size class — the multiset of (frame side, modes, dtype) and positions rounded to a power of
+two — in a JSON file.
 """
 
 import json
@@ -36,8 +37,9 @@ def _order(engines):
     return sorted(engines, key=lambda g: -g.nframe * g._probe_shape[-1] ** 2 * g._probe_shape[0])
 
 
-def run_waves(engines, nepoch, params=None, width=None):
-    """`nepoch` rPIE epochs of every engine, `width` scans at a time (all at once if None)."""
+def run_waves(engines, nepoch, params=None, width=None, one_graph=False):
+    """`nepoch` rPIE epochs of every engine, `width` scans at a time (all at once if None),
+    each wave's epochs on the engines' streams or in one graph per epoch."""
     engines = list(engines)
     if params is None or isinstance(params, dict):
         params = [params or {}] * len(engines)
@@ -47,7 +49,7 @@ def run_waves(engines, nepoch, params=None, width=None):
     width = width or len(pairs)
     for i in range(0, len(pairs), width):
         wave = pairs[i:i + width]
-        rpie_group([g for g, _ in wave], nepoch, [p for _, p in wave])
+        rpie_group([g for g, _ in wave], nepoch, [p for _, p in wave], one_graph)
 
 
 def _load():
@@ -57,10 +59,12 @@ def _load():
         return {}
 
 
-def tune_group(engines, params=None, widths=WIDTHS, cache=True):
-    """The fastest width for these engines. Each candidate runs one ordinary rPIE epoch of
-    every engine (tuning advances the reconstruction by len(widths) epochs, with results
-    identical to any other grouping). Returns {"width", "seconds": {width: s}, "key"}."""
+def tune_group(engines, params=None, widths=WIDTHS, cache=True, modes=(False, True)):
+    """The fastest launch for these engines: a width and whether each wave's epochs go on the
+    engines' streams (one_graph False) or in one graph per epoch. Each candidate runs one
+    ordinary rPIE epoch of every engine (tuning advances the reconstruction by one epoch per
+    candidate, with results identical to any other grouping). Returns {"width", "one_graph",
+    "seconds": {"width/mode": s}, "key"}."""
     engines = list(engines)
     key = size_class(engines)
     if cache:
@@ -71,13 +75,15 @@ def tune_group(engines, params=None, widths=WIDTHS, cache=True):
     for w in widths:
         if w > len(engines) and w != widths[0]:
             continue
-        for g in engines:
-            g.synchronize()
-        t = time.perf_counter()
-        run_waves(engines, 1, params, w)
-        times[w] = time.perf_counter() - t
+        for one in modes:
+            for g in engines:
+                g.synchronize()
+            t = time.perf_counter()
+            run_waves(engines, 1, params, w, one)
+            times[(w, one)] = time.perf_counter() - t
     best = min(times, key=times.get)
-    out = {"width": best, "seconds": {str(k): v for k, v in times.items()}}
+    out = {"width": best[0], "one_graph": best[1],
+           "seconds": {f"{w}/{'graph' if one else 'streams'}": v for (w, one), v in times.items()}}
     if cache:
         data = _load()
         data[key] = out

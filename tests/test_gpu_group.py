@@ -27,14 +27,14 @@ def engines(dtypes, **options):
     return out
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_group_matches_engines_run_alone(graph):
+@pytest.mark.parametrize("graph,one_graph", [(True, False), (False, False), (True, True)])
+def test_group_matches_engines_run_alone(graph, one_graph):
     dtypes = ["c64", "c128", "c64", "c128"]
     alone = engines(dtypes)
     for g, p in zip(alone, PARAMS):
         g.rpie(3, **p)
     group = engines(dtypes, graph=graph)
-    pa.rpie_group(group, 3, PARAMS)
+    pa.rpie_group(group, 3, PARAMS, one_graph=one_graph)
     for a, b in zip(alone, group):
         assert a.cycle == b.cycle == 3
         assert np.array_equal(a.object, b.object)
@@ -66,10 +66,12 @@ def test_tuned_waves_give_the_same_bits(tmp_path, monkeypatch):
     dtypes = ["c64"] * 4
     a, b = engines(dtypes), engines(dtypes)
     r = tune.tune_group(a, PARAMS, widths=(1, 2, 4))
-    assert r["width"] in (1, 2, 4) and not r["cached"]
+    assert r["width"] in (1, 2, 4) and r["one_graph"] in (False, True) and not r["cached"]
     assert tune.tune_group(a, PARAMS)["cached"]
+    ran = len(r["seconds"])
+    assert ran == 6
     for g, p in zip(b, PARAMS):
-        g.rpie(3, **p)
+        g.rpie(ran, **p)
     for g, h in zip(a, b):
-        assert g.cycle == h.cycle == 3
+        assert g.cycle == h.cycle == ran
         assert np.array_equal(g.object, h.object) and np.array_equal(g.probe, h.probe)

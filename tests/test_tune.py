@@ -22,7 +22,7 @@ def test_size_class_rounds_positions_and_ignores_order():
 
 def test_waves_are_largest_first_and_cover_every_engine(monkeypatch):
     calls = []
-    monkeypatch.setattr(tune, "rpie_group", lambda gs, nepoch, ps: calls.append((gs, ps)))
+    monkeypatch.setattr(tune, "rpie_group", lambda gs, nepoch, ps, one: calls.append((gs, ps)))
     es = [fake(128, 1, 200), fake(256, 4, 100), fake(128, 2, 3000), fake(256, 1, 50)]
     ps = [{"batch_size": k} for k in range(4)]
     tune.run_waves(es, 1, ps, width=3)
@@ -39,7 +39,7 @@ def test_waves_are_largest_first_and_cover_every_engine(monkeypatch):
 
 def test_tuner_caches_its_choice(tmp_path, monkeypatch):
     monkeypatch.setattr(tune, "CACHE", tmp_path / "t.json")
-    monkeypatch.setattr(tune, "rpie_group", lambda gs, nepoch, ps: None)
+    monkeypatch.setattr(tune, "rpie_group", lambda gs, nepoch, ps, one: None)
     es = [fake(128, 1, 200), fake(256, 4, 100)]
     r = tune.tune_group(es, widths=(1, 2))
     assert r["width"] in (1, 2) and not r["cached"]
