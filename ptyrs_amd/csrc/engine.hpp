@@ -265,7 +265,8 @@ template <typename T, int N>
 class Engine : public EngineBase {
   using S = RowShape<T, N>;    // pass A
   using SR = RowShapeC<T, N>;  // pass C
-  using SC = ColShape<T, N>;   // pass B
+  using SC = ColShape<T, N>;   // column passes
+  using SB = BShape<T, N>;     // pass B
   static constexpr int n = N * N;
 
  public:
@@ -1145,7 +1146,7 @@ class Engine : public EngineBase {
     afar_tmp_.ensure(sizeof(T) * cap);
     prev_.ensure(sizeof(T) * 2);
     icalc_.ensure(sizeof(T) * cap * n);
-    xi_part_.ensure(sizeof(double) * 2 * cap * SC::NBLK);
+    xi_part_.ensure(sizeof(double) * 2 * cap * SB::NBLK);
     xi_part2_.ensure(sizeof(double) * 2 * cap * aux_blocks());
     dots_part_.ensure(sizeof(double) * 6 * cap * aux_blocks());
     sums_.ensure(sizeof(double) * 6 * cap);
@@ -1217,7 +1218,7 @@ class Engine : public EngineBase {
       pass_b<B_ICALC>(ord, nb, ba);
       run(K_OTHER, [&] {
         hipLaunchKernelGGL(k_lsqml_xi<T>, dim3(1), dim3(THREADS), 0, stream_,
-                           xi_part_.as<double>(), int(SC::NBLK), nb, prev_.as<T>(),
+                           xi_part_.as<double>(), int(SB::NBLK), nb, prev_.as<T>(),
                            afar_tmp_.as<T>(), prev_.as<T>() + 1);
         hipLaunchKernelGGL(k_lsqml_xi_sums<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
                            icalc_.as<T>(), counts_.as<float>(), ord, n, prev_.as<T>() + 1,
@@ -1427,7 +1428,7 @@ class Engine : public EngineBase {
       pass_b<B_ICALC>(ord, nb, ba);
       run(K_OTHER, [&] {
         hipLaunchKernelGGL(k_lsqml_xi<T>, dim3(1), dim3(THREADS), 0, stream_,
-                           xi_part_.as<double>(), int(SC::NBLK), nb, prev_.as<T>(),
+                           xi_part_.as<double>(), int(SB::NBLK), nb, prev_.as<T>(),
                            afar_tmp_.as<T>(), prev_.as<T>() + 1);
         hipLaunchKernelGGL(k_lsqml_xi_sums<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
                            icalc_.as<T>(), counts_.as<float>(), ord, n, prev_.as<T>() + 1,
@@ -1701,7 +1702,7 @@ class Engine : public EngineBase {
     pmax_.ensure(sizeof(T) * 1);
     order_.ensure(sizeof(int) * std::max<size_t>(1, nepoch * nframe_));
     eord_.ensure(sizeof(int) * std::max<size_t>(1, nframe_));
-    llk_part_.ensure(sizeof(double) * 4 * nframe_ * SC::NBLK);
+    llk_part_.ensure(sizeof(double) * 4 * nframe_ * SB::NBLK);
     cap_ = std::max(cap_, cap);
     if (nslice_ > 1) inc_.ensure(sizeof(cplx<T>) * inc_slots() * cap_ * M_ * n);
     ensure_positions(cap_);
@@ -1751,18 +1752,21 @@ class Engine : public EngineBase {
     constexpr bool fwd = MODE != B_POISSON;
     run(MODE == B_LLK ? K_LLK : K_PASS_B, [&] {
       if (angular_ && fwd) pass_t<false>(psi_.as<cplx<T>>(), nb * M_);
-      dim3 g(SC::NBLK, nb), b(SC::NT);
       const T norm = T(1) / (angular_ ? T(n) * T(n) : T(n));
-      auto args = [&](auto col, auto row) {
-        hipLaunchKernelGGL(angular_ ? row : col, g, b, 0, stream_, psi_.as<cplx<T>>(),
+      auto args = [&](auto mm) {
+        constexpr int MM = decltype(mm)::value;
+        auto col = k_pass_b<T, N, MM, MODE, false>;
+        auto row = k_pass_b<T, N, MM, MODE, true>;
+        hipLaunchKernelGGL(angular_ ? row : col, dim3(SB::NBLK, nb),
+                           dim3(pass_b_threads<T, N, MM>()), 0, stream_, psi_.as<cplx<T>>(),
                            counts_.as<float>(), ord, llk_part_.as<double>(), tw_.as<cplx<T>>(),
                            ba, norm);
       };
       switch (M_) {
-        case 1: args(k_pass_b<T, N, 1, MODE, false>, k_pass_b<T, N, 1, MODE, true>); break;
-        case 2: args(k_pass_b<T, N, 2, MODE, false>, k_pass_b<T, N, 2, MODE, true>); break;
-        case 3: args(k_pass_b<T, N, 3, MODE, false>, k_pass_b<T, N, 3, MODE, true>); break;
-        case 4: args(k_pass_b<T, N, 4, MODE, false>, k_pass_b<T, N, 4, MODE, true>); break;
+        case 1: args(std::integral_constant<int, 1>{}); break;
+        case 2: args(std::integral_constant<int, 2>{}); break;
+        case 3: args(std::integral_constant<int, 3>{}); break;
+        case 4: args(std::integral_constant<int, 4>{}); break;
         default: throw Invalid("the GPU engine runs at most 4 modes");
       }
       if (angular_ && back) pass_t<true>(psi_.as<cplx<T>>(), nb * M_);
@@ -1785,7 +1789,7 @@ class Engine : public EngineBase {
     }
     run(K_LLK, [&] {
       hipLaunchKernelGGL(k_llk_reduce, dim3(1), dim3(THREADS), 0, stream_,
-                         llk_part_.as<double>(), int(nframe_ * SC::NBLK),
+                         llk_part_.as<double>(), int(nframe_ * SB::NBLK),
                          hist_.as<double>() + 4 * slot);
     });
   }

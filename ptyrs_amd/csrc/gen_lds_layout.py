@@ -1,5 +1,5 @@
 """Writes lds_layout.hpp: the LDS placement of the line FFTs' exchanges for every line shape
-the kernels instantiate (frame sides 16 … 512; the row, pass-C and column shapes of float and
+the kernels instantiate (frame sides 16 … 512; the row, pass-C, column and pass-B shapes of float and
 double), chosen to minimise LDS bank conflicts.
 
     python ptyrs_amd/csrc/gen_lds_layout.py
@@ -62,13 +62,15 @@ def lines_per_block(n, tl, maxt, maxl):
 
 
 def shapes():
-    """(N, E, L, element bytes) of RowShape, RowShapeC and ColShape (kernels.hpp) for float
-    and double."""
+    """(N, E, L, element bytes) of RowShape, RowShapeC, ColShape and BShape (kernels.hpp) for
+    float and double."""
     out = set()
     for size in (4, 8):
         for n in SIDES:
             e = choose_e(n, 16 if size == 4 else 8)
             out.add((n, e, lines_per_block(n, n // e, THREADS, n), size))
+            cg = (8 if size == 4 else 4) // (2 if n >= 512 else 1)
+            out.add((n, e, lines_per_block(n, n // e, 1024, cg), size))
             e = choose_e(n, 8 if size == 4 else 4)
             out.add((n, e, lines_per_block(n, n // e, THREADS, n), size))
             out.add((n, e, lines_per_block(n, n // e, 512, 8 if size == 4 else 4), size))

@@ -270,10 +270,26 @@ struct BArgs {
   T* icalc;         // B_ICALC: model intensity per slot and pixel, DFT index order
 };
 
-// Grid (NBLK, nb). Each workgroup holds L columns of every mode of one frame. B_RPIE scales
-// the field by the pty-chi modulus sqrt(iobs + 1e-7)/(sqrt(icalc) + 1e-7) (masked pixels keep
-// their value), with the 1/n of the unnormalised round trip folded in; B_LLK writes the four
-// LLK sums of the workgroup's pixels (FP64) to llk_part[frame][blockIdx.x][4] and stores
+// Pass B's lines: per mode, a stripe of CG columns (or rows) of N values, E per thread (16
+// float / 8 double: one LDS exchange fewer than the column passes' 8 / 4). Every mode of the
+// stripe has its own group of threads and its own exchange region.
+template <typename T>
+constexpr int pass_b_lines(int n) {
+  return (sizeof(T) == 4 ? 8 : 4) / (n >= 512 ? 2 : 1);
+}
+template <typename T, int N>
+using BShape = Lines<N, choose_e(N, sizeof(T) == 4 ? 16 : 8), 1024, pass_b_lines<T>(N), sizeof(T)>;
+template <typename T, int N, int M>
+constexpr int pass_b_threads() {
+  return M * BShape<T, N>::NT;
+}
+
+// Grid (NBLK, nb) of BShape. Mode group m of each workgroup transforms mode m of the stripe;
+// the modes' |ψ_m|² meet in LDS, where the group owning a pixel sums them in mode order and
+// leaves the pixel's detector-plane factor for every group. B_RPIE
+// scales the field by the pty-chi modulus sqrt(iobs + 1e-7)/(sqrt(icalc) + 1e-7) (masked pixels
+// keep their value), with the 1/n of the unnormalised round trip folded in; B_LLK writes the
+// four LLK sums of the workgroup's pixels (FP64) to llk_part[frame][blockIdx.x][4] and stores
 // nothing.
 // B_GAUSS and B_POISSON scale the field by (1 − a_far·c)/n with LSQML's residual c (Gaussian
 // 2(1 − √iobs/(√icalc + 1e-9)) with a_far = 1/2, Poisson 1 − iobs/(icalc + 1e-6)), 0 where
@@ -283,44 +299,57 @@ struct BArgs {
 // angular spectrum's second row pass). `norm` is 1/n (one unnormalised DFT round trip) or 1/n²
 // (two).
 template <typename T, int N, int M, int MODE, bool ROW>
-__global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
+__global__ __launch_bounds__((pass_b_threads<T, N, M>())) void k_pass_b(
     cplx<T>* __restrict__ psi, const float* __restrict__ counts, const int* __restrict__ order,
     double* __restrict__ llk_part, const cplx<T>* __restrict__ tw, BArgs<T> ba, T norm) {
-  using S = ColShape<T, N>;
+  using S = BShape<T, N>;
   constexpr int E = S::E, TL = S::TL, CG = S::L;
+  constexpr int NT = pass_b_threads<T, N, M>();
+  // Each group's region holds its exchanges and, after the forward transform, its intensities.
+  constexpr int W = S::LDS_WORDS > CG * N ? S::LDS_WORDS : CG * N;
   constexpr int n = N * N;
-  __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
+  __shared__ T lre[M * W], lim[M * W];
   __shared__ cplx<T> stw[N];
-  // Columns: lanes run along the 128-byte stripe of lines; rows: along each line.
+  // Columns: lanes run along the stripe of lines; rows: along each line.
   const int tid = threadIdx.x;
-  const int line = ROW ? tid / TL : tid % CG, jl = ROW ? tid % TL : tid / CG;
+  const int m = tid / S::NT, tg = tid % S::NT;
+  const int line = ROW ? tg / TL : tg % CG, jl = ROW ? tg % TL : tg / CG;
   using At = std::conditional_t<ROW, typename S::AtRow, typename S::AtCol>;
   const int2 bxy = block_xy();
   const int x = bxy.x * CG + line;
   const int b = bxy.y;
   const int f = order[b];
-  T* re = lre + (ROW ? S::base_row(line) : S::base_col(line));
-  T* im = lim + (ROW ? S::base_row(line) : S::base_col(line));
+  const int lb = ROW ? S::base_row(line) : S::base_col(line);
+  T* re = lre + m * W + lb;
+  T* im = lim + m * W + lb;
   auto at = [&](int e) { return ROW ? x * N + jl + TL * e : (jl + TL * e) * N + x; };
-  // Element e of the lane is at index i0 + e·step of every plane: one offset VGPR for all
-  // modes (buffer accesses), the step in SGPRs.
+  // Element e of the lane is at index i0 + e·step of the plane: one offset VGPR (buffer
+  // accesses), the step in SGPRs.
   const uint32_t i0 = uint32_t(ROW ? x * N + jl : jl * N + x);
   constexpr uint32_t step = ROW ? TL : TL * N;
   constexpr uint32_t cb = sizeof(cplx<T>);
+  // Pixel e of the lane is owned by group e mod M: that group reads its count, forms its
+  // detector-plane operator once and shares it through LDS.
   const Buf cnt = buf(counts + (size_t)f * n, n * 4u);
   float cv[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) cv[e] = bld1(cnt, i0 * 4u, e * step * 4u);
-  cplx<T> v[M][E];
-#pragma unroll
-  for (int m = 0; m < M; ++m) {
-    const Buf plane = buf(psi + (size_t)(b * M + m) * n, n * cb);
-#pragma unroll
-    for (int e = 0; e < E; ++e) v[m][e] = bld<T>(plane, i0 * cb, e * step * cb);
+  for (int e = 0; e < E; ++e) {
+    if (e % M != m) continue;
+    cv[e] = bld1(cnt, i0 * 4u, e * step * 4u);
   }
-  stage_twiddles<T, N, S::NT>(stw, tw);
+  const Buf plane = buf(psi + (size_t)(b * M + m) * n, n * cb);
+  cplx<T> v[E];
 #pragma unroll
-  for (int m = 0; m < M; ++m) line_fft<T, N, E, false, At>(v[m], jl, re, im, stw);
+  for (int e = 0; e < E; ++e) v[e] = bld<T>(plane, i0 * cb, e * step * cb);
+  stage_twiddles<T, N, NT>(stw, tw);
+  line_fft<T, N, E, false, At>(v, jl, re, im, stw);
+  // The transform's last exchange has been read by every thread: the regions take the
+  // intensities, pixel p = jl + TL·e of line `line` of group k at lre[k·W + p·CG + line], and
+  // the owners' operators at lim[p·CG + line].
+  auto key = [&](int e) { return (jl + TL * e) * CG + line; };
+#pragma unroll
+  for (int e = 0; e < E; ++e) lre[m * W + key(e)] = norm2(v[e]);
+  __syncthreads();
   const T inv_n = norm;
   double acc[4] = {0, 0, 0, 0};
   T afar = 0;
@@ -329,44 +358,36 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
   if constexpr (MODE == B_ICALC) prev = double(ba.prev[0]);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
+    if (e % M != m) continue;
     const float io = cv[e];
     T I = 0;
 #pragma unroll
-    for (int m = 0; m < M; ++m) I += norm2(v[m][e]);
+    for (int k = 0; k < M; ++k) I += lre[k * W + key(e)];
+    T r = 0;
     if constexpr (MODE == B_RPIE) {
-      T r = inv_n;
+      r = inv_n;
       if (io >= 0.f) r = sqrt(T(io) + T(1e-7)) / (sqrt(I * inv_n) + T(1e-7)) * inv_n;
-#pragma unroll
-      for (int m = 0; m < M; ++m) v[m][e] = v[m][e] * r;
     } else if constexpr (MODE == B_AMPLITUDE) {
       // PyNX ApplyAmplitude: sqrt(iobs)/max(1e-12, sqrt(icalc)); masked pixels unchanged.
-      T r = inv_n;
+      r = inv_n;
       if (io >= 0.f) {
         T calc = sqrt(I * inv_n);
         r = sqrt(T(io)) / (calc > T(1e-12) ? calc : T(1e-12)) * inv_n;
       }
-#pragma unroll
-      for (int m = 0; m < M; ++m) v[m][e] = v[m][e] * r;
     } else if constexpr (MODE == B_GRAD) {
       // PyNX GradPoissonFourier: 2(1 − iobs/max(icalc, 1e-12)), 0 where masked
       // (ptyrs/src/engine/ops.rs:156-195).
-      T r = 0;
       if (io >= 0.f) {
         const T ic = I * inv_n;
         r = T(2) * (T(1) - T(io) / (ic > T(1e-12) ? ic : T(1e-12))) * inv_n;
       }
-#pragma unroll
-      for (int m = 0; m < M; ++m) v[m][e] = v[m][e] * r;
     } else if constexpr (MODE == B_GRAD_AMP) {
       // The amplitude cost (√icalc − √iobs)²: 2(1 − √(iobs/max(icalc, 1e-12))), 0 where masked
       // (ptyrs/src/engine/features.rs:322-330).
-      T r = 0;
       if (io >= 0.f) {
         const T ic = I * inv_n;
         r = T(2) * (T(1) - sqrt(T(io) / (ic > T(1e-12) ? ic : T(1e-12)))) * inv_n;
       }
-#pragma unroll
-      for (int m = 0; m < M; ++m) v[m][e] = v[m][e] * r;
     } else if constexpr (MODE == B_GAUSS || MODE == B_POISSON) {
       const T ic = I * inv_n;
       T c = 0;
@@ -374,9 +395,7 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
         if constexpr (MODE == B_GAUSS) c = T(2) * (T(1) - sqrt(T(io)) / (sqrt(ic) + T(1e-9)));
         else c = T(1) - T(io) / (ic + T(1e-6));
       }
-      const T r = (T(1) - (MODE == B_GAUSS ? T(0.5) : afar) * c) * inv_n;
-#pragma unroll
-      for (int m = 0; m < M; ++m) v[m][e] = v[m][e] * r;
+      r = (T(1) - (MODE == B_GAUSS ? T(0.5) : afar) * c) * inv_n;
     } else if constexpr (MODE == B_ICALC) {
       const T ic = I * inv_n;
       ba.icalc[(size_t)b * n + at(e)] = ic;
@@ -397,29 +416,31 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_b(
         acc[3] += ic;
       }
     }
+    if constexpr (MODE != B_LLK && MODE != B_ICALC) lim[key(e)] = r;
   }
   if constexpr (MODE == B_LLK) {
-    __shared__ double red[S::NT];
+    __shared__ double red[NT];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      double s = block_reduce<S::NT>(acc[i], red, AddOp());
+      double s = block_reduce<NT>(acc[i], red, AddOp());
       if (tid == 0) llk_part[((size_t)f * S::NBLK + bxy.x) * 4 + i] = s;
     }
   } else if constexpr (MODE == B_ICALC) {
-    __shared__ double red[S::NT];
+    __shared__ double red[NT];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      double s = block_reduce<S::NT>(acc[i], red, AddOp());
+      double s = block_reduce<NT>(acc[i], red, AddOp());
       if (tid == 0) ba.part[((size_t)b * S::NBLK + bxy.x) * 2 + i] = s;
     }
   } else {
+    __syncthreads();
 #pragma unroll
-    for (int m = 0; m < M; ++m) {
-      line_fft<T, N, E, true, At>(v[m], jl, re, im, stw);
-      const Buf plane = buf(psi + (size_t)(b * M + m) * n, n * cb);
+    for (int e = 0; e < E; ++e) v[e] = v[e] * lim[key(e)];
+    // Every group has read the operators before the inverse transform's exchange.
+    __syncthreads();
+    line_fft<T, N, E, true, At>(v, jl, re, im, stw);
 #pragma unroll
-      for (int e = 0; e < E; ++e) bst<T>(plane, i0 * cb, e * step * cb, v[m][e]);
-    }
+    for (int e = 0; e < E; ++e) bst<T>(plane, i0 * cb, e * step * cb, v[e]);
   }
 }
 
