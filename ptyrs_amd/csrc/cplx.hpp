@@ -40,6 +40,14 @@ template <typename T>
 __host__ __device__ __forceinline__ cplx<T> mul_mi(cplx<T> a) { return {a.y, -a.x}; }
 
 // 8- or 16-byte vector load/store of one complex value.
+// 1/x: the hardware reciprocal for float (v_rcp_f32, within 1 ulp), an IEEE division for
+// double.
+template <typename T>
+__device__ __forceinline__ T rcp(T x) {
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rcpf(x);
+  else return T(1) / x;
+}
+
 template <typename T>
 __device__ __forceinline__ cplx<T> ld(const cplx<T>* p) {
   auto v = *reinterpret_cast<const typename vec2<T>::type*>(p);

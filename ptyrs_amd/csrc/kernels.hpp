@@ -134,6 +134,15 @@ __device__ __forceinline__ int tile_id() {
   return int(blockIdx.x);
 }
 
+// A thread-group index tid / GROUP: when groups are whole waves it is the same in every lane
+// of a wave, and is read from the first lane so the compiler keeps it (and the plane addresses
+// and branches that follow from it) in scalar registers.
+template <int GROUP>
+__device__ __forceinline__ int wave_uniform(int v) {
+  if constexpr (GROUP % 64 == 0) return __builtin_amdgcn_readfirstlane(v);
+  else return v;
+}
+
 // ---- block reductions --------------------------------------------------------------------------
 template <int NT, typename V, typename Op>
 __device__ __forceinline__ V block_reduce(V v, V* red, Op op) {
@@ -312,7 +321,7 @@ __global__ __launch_bounds__((pass_b_threads<T, N, M>())) void k_pass_b(
   __shared__ cplx<T> stw[N];
   // Columns: lanes run along the stripe of lines; rows: along each line.
   const int tid = threadIdx.x;
-  const int m = tid / S::NT, tg = tid % S::NT;
+  const int m = wave_uniform<S::NT>(tid / S::NT), tg = tid % S::NT;
   const int line = ROW ? tg / TL : tg % CG, jl = ROW ? tg % TL : tg / CG;
   using At = std::conditional_t<ROW, typename S::AtRow, typename S::AtCol>;
   const int2 bxy = block_xy();
@@ -604,7 +613,7 @@ __global__ __launch_bounds__((pass_c_sum_threads<T, N, M>())) void k_pass_c_rpie
   __shared__ cplx<T> stw[N];
   __shared__ cplx<T> terms[2 * TW];
   const int tid = threadIdx.x;
-  const int m = tid / (G * TL), l = (tid / TL) % G, jl = tid % TL;
+  const int m = wave_uniform<G * TL>(tid / (G * TL)), l = (tid / TL) % G, jl = tid % TL;
   const int line = m * G + l;
   const int y = blockIdx.x * G + l;
   const int ch = blockIdx.y;
@@ -687,9 +696,9 @@ __global__ __launch_bounds__((pass_c_sum_threads<T, N, M>())) void k_pass_c_rpie
     for (int e = 0; e < E; ++e) {
       const cplx<T> o = ow[e];
       const cplx<T> d = v[e] - w[e] * o;
-      if (upd_obj) trow[TL * e] = cmulc(w[e], d) * (T(1) / (inv_o * dw[e] + reg));
+      if (upd_obj) trow[TL * e] = cmulc(w[e], d) * rcp(inv_o * dw[e] + reg);
       if (probe_dir)
-        acc[e] = acc[e] + cmulc(o * qq[e], d) * (T(1) / (inv_p * norm2(o) + probe_alpha * omax));
+        acc[e] = acc[e] + cmulc(o * qq[e], d) * rcp(inv_p * norm2(o) + probe_alpha * omax);
     }
   }
   if (upd_obj && b1 > b0) {
