@@ -1133,6 +1133,14 @@ class Engine : public EngineBase {
   }
 
   static constexpr size_t kIllumChunk = 1024;
+#ifndef PTYRS_PASSC_BLOCKS
+#define PTYRS_PASSC_BLOCKS 512
+#endif
+  // rPIE's pass C cuts a batch into chunks of slots so that its grid has about this many
+  // workgroups (each chunk leaves one probe-direction partial per mode). At 256² × 4 modes, 512
+  // is one round at two workgroups per CU: 0.98 ms per epoch, against 1.03 at 1024 and 1.73
+  // at 256.
+  static constexpr int kPassCBlocks = PTYRS_PASSC_BLOCKS;
   size_t illum_chunk() const { return std::max<size_t>(1, std::min(nframe_, kIllumChunk)); }
 
   void ensure_lsqml(size_t cap) {
@@ -1681,7 +1689,7 @@ class Engine : public EngineBase {
   // The same for rPIE's mode-summing pass C, whose workgroups hold G rows of every mode.
   int chunk_of_m(int nb) const {
     const int per = N / rows_per_group(N, SR::L, int(M_));
-    int nchunk = std::max(1, std::min(nb, (1024 + per - 1) / per));
+    int nchunk = std::max(1, std::min(nb, (kPassCBlocks + per - 1) / per));
     return (nb + nchunk - 1) / nchunk;
   }
 
@@ -1692,7 +1700,7 @@ class Engine : public EngineBase {
     // chunk_of and chunk_of_m never cut more chunks than this, whatever the batch length.
     const size_t per =
         std::min(size_t(SR::NBLK) * M_, size_t(N / rows_per_group(N, SR::L, int(M_))));
-    const size_t nchunk = std::min(cap, (1024 + per - 1) / per);
+    const size_t nchunk = std::min(cap, (size_t(std::max(kPassCBlocks, 1024)) + per - 1) / per);
     dP_.ensure(sizeof(cplx<T>) * std::max<size_t>(nchunk, 1) * M_ * n);
     list_.ensure(sizeof(int) * size_t(ntiles()) * cap);
     lcount_.ensure(sizeof(int) * size_t(ntiles()));
