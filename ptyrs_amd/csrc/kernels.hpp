@@ -144,20 +144,34 @@ __device__ __forceinline__ int wave_uniform(int v) {
 }
 
 // ---- block reductions --------------------------------------------------------------------------
+// Deterministic for a fixed block size: whole waves reduce by a butterfly of lane exchanges
+// and then the waves' results are combined in wave order (one barrier); other block sizes
+// reduce by a fixed tree in LDS. `red` holds NT values.
 template <int NT, typename V, typename Op>
 __device__ __forceinline__ V block_reduce(V v, V* red, Op op) {
-  // Fixed tree: deterministic for a fixed block size.
   const int tid = threadIdx.x;
-  red[tid] = v;
-  __syncthreads();
+  if constexpr (NT % 64 == 0) {
 #pragma unroll
-  for (int s = 1; s < NT; s <<= 1) {
-    if ((tid % (2 * s)) == 0 && tid + s < NT) red[tid] = op(red[tid], red[tid + s]);
+    for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
+    if ((tid & 63) == 0) red[tid >> 6] = v;
     __syncthreads();
+    V r = red[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) r = op(r, red[w]);
+    __syncthreads();
+    return r;
+  } else {
+    red[tid] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = 1; s < NT; s <<= 1) {
+      if ((tid % (2 * s)) == 0 && tid + s < NT) red[tid] = op(red[tid], red[tid + s]);
+      __syncthreads();
+    }
+    V r = red[0];
+    __syncthreads();
+    return r;
   }
-  V r = red[0];
-  __syncthreads();
-  return r;
 }
 
 struct MaxOp {
