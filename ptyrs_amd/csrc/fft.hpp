@@ -155,12 +155,48 @@ __device__ __forceinline__ void dft(cplx<T>* a) {
   else if constexpr (R == 16) dft16(a);
 }
 
+// ---- per-thread twiddles ----------------------------------------------------------------------
+// A thread that transforms several lines at one lane position jl uses the same twiddles in
+// every one: they can be read from the LDS table once into registers. Stage S (NS > 1, lines
+// over several threads) needs U·(R − 1) of them, at offset tw_cache_off(N, E, S).
+constexpr int tw_cache_off(int n, int e, int s) {
+  int off = 0;
+  for (int i = 0; i < s; ++i) {
+    const int r = stage_radix(n, e, i);
+    if (n / e > 1 && stage_ns(n, e, i) > 1) off += (e / r) * (r - 1);
+  }
+  return off;
+}
+constexpr int tw_cache_size(int n, int e) {
+  const int c = tw_cache_off(n, e, n_stages(n, e));
+  return c > 0 ? c : 1;
+}
+
+template <typename T, int N, int E, int S = 0>
+__device__ __forceinline__ void load_tw_cache(cplx<T>* c, int jl, const cplx<T>* tw) {
+  if constexpr (S < n_stages(N, E)) {
+    constexpr int R = stage_radix(N, E, S), NS = stage_ns(N, E, S), TL = N / E, U = E / R;
+    constexpr int OFF = tw_cache_off(N, E, S);
+    if constexpr (TL > 1 && NS > 1) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = (jl + TL * u) % NS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) c[OFF + u * (R - 1) + r - 1] = tw[k * r * (N / (NS * R))];
+      }
+    }
+    load_tw_cache<T, N, E, S + 1>(c, jl, tw);
+  }
+}
+
 // ---- one Stockham stage ----------------------------------------------------------------------
 // Stage S of the plan for (N, E): butterflies jb = jl + TL·u, u < E/R. `re`/`im` point at this
 // line's element 0 in LDS, its elements placed by A (unused when TL == 1 or on the last stage).
-template <typename T, int N, int E, int S, typename A>
+// CACHED: the twiddles come from `twc` (load_tw_cache for this jl) in registers.
+template <typename T, int N, int E, int S, typename A, bool CACHED>
 __device__ __forceinline__ void fft_stage(cplx<T> (&v)[E], int jl, T* re, T* im,
-                                          const cplx<T>* __restrict__ tw) {
+                                          const cplx<T>* __restrict__ tw,
+                                          const cplx<T>* twc) {
   constexpr int R = stage_radix(N, E, S);
   constexpr int NS = stage_ns(N, E, S);
   constexpr int TL = N / E;
@@ -193,9 +229,13 @@ __device__ __forceinline__ void fft_stage(cplx<T> (&v)[E], int jl, T* re, T* im,
       for (int r = 0; r < R; ++r) a[r] = v[u + U * r];
       const int jb = jl + TL * u;
       if constexpr (NS > 1) {
+        constexpr int OFF = tw_cache_off(N, E, S);
         const int k = jb % NS;
 #pragma unroll
-        for (int r = 1; r < R; ++r) a[r] = a[r] * tw[k * r * (N / (NS * R))];
+        for (int r = 1; r < R; ++r) {
+          if constexpr (CACHED && TL > 1) a[r] = a[r] * twc[OFF + u * (R - 1) + r - 1];
+          else a[r] = a[r] * tw[k * r * (N / (NS * R))];
+        }
       }
       dft<T, R>(a);
       if constexpr (LAST) {
@@ -224,27 +264,29 @@ __device__ __forceinline__ void fft_stage(cplx<T> (&v)[E], int jl, T* re, T* im,
   }
 }
 
-template <typename T, int N, int E, int S, typename A>
+template <typename T, int N, int E, int S, typename A, bool CACHED>
 __device__ __forceinline__ void fft_stages(cplx<T> (&v)[E], int jl, T* re, T* im,
-                                           const cplx<T>* __restrict__ tw) {
+                                           const cplx<T>* __restrict__ tw, const cplx<T>* twc) {
   if constexpr (S < n_stages(N, E)) {
-    fft_stage<T, N, E, S, A>(v, jl, re, im, tw);
-    fft_stages<T, N, E, S + 1, A>(v, jl, re, im, tw);
+    fft_stage<T, N, E, S, A, CACHED>(v, jl, re, im, tw, twc);
+    fft_stages<T, N, E, S + 1, A, CACHED>(v, jl, re, im, tw, twc);
   }
 }
 
-// Forward (INV = false) or inverse unnormalised DFT of one line, its LDS elements placed by A.
+// Forward (INV = false) or inverse unnormalised DFT of one line, its LDS elements placed by A,
+// its twiddles from `tw` (the LDS table) or, CACHED, from `twc` (load_tw_cache for this lane).
 // All threads of the workgroup must call it together (it synchronises when the line spans
 // several threads).
-template <typename T, int N, int E, bool INV, typename A = LdsPlain>
+template <typename T, int N, int E, bool INV, typename A = LdsPlain, bool CACHED = false>
 __device__ __forceinline__ void line_fft(cplx<T> (&v)[E], int jl, T* re, T* im,
-                                         const cplx<T>* __restrict__ tw) {
+                                         const cplx<T>* __restrict__ tw,
+                                         const cplx<T>* twc = nullptr) {
   static_assert(n_stages(N, E) > 0, "no radix plan for this line length");
   if constexpr (INV) {
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e].y = -v[e].y;
   }
-  fft_stages<T, N, E, 0, A>(v, jl, re, im, tw);
+  fft_stages<T, N, E, 0, A, CACHED>(v, jl, re, im, tw, twc);
   if constexpr (INV) {
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e].y = -v[e].y;

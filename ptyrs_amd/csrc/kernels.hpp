@@ -365,7 +365,12 @@ __global__ __launch_bounds__((pass_b_threads<T, N, M>())) void k_pass_b(
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = bld<T>(plane, i0 * cb, e * step * cb);
   stage_twiddles<T, N, NT>(stw, tw);
-  line_fft<T, N, E, false, At>(v, jl, re, im, stw);
+  // The forward and inverse transforms share the lane's twiddles, held in registers for float
+  // (for double they would cost a wave per SIMD).
+  constexpr bool TC = sizeof(T) == 4;
+  cplx<T> twc[TC ? tw_cache_size(N, E) : 1];
+  if constexpr (TC) load_tw_cache<T, N, E>(twc, jl, stw);
+  line_fft<T, N, E, false, At, TC>(v, jl, re, im, stw, twc);
   // The transform's last exchange has been read by every thread: the regions take the
   // intensities, pixel p = jl + TL·e of line `line` of group k at lre[k·W + p·CG + line], and
   // the owners' operators at lim[p·CG + line].
@@ -461,7 +466,7 @@ __global__ __launch_bounds__((pass_b_threads<T, N, M>())) void k_pass_b(
     for (int e = 0; e < E; ++e) v[e] = v[e] * lim[key(e)];
     // Every group has read the operators before the inverse transform's exchange.
     __syncthreads();
-    line_fft<T, N, E, true, At>(v, jl, re, im, stw);
+    line_fft<T, N, E, true, At, TC>(v, jl, re, im, stw, twc);
 #pragma unroll
     for (int e = 0; e < E; ++e) bst<T>(plane, i0 * cb, e * step * cb, v[e]);
   }
@@ -634,6 +639,9 @@ __global__ __launch_bounds__((pass_c_sum_threads<T, N, M>())) void k_pass_c_rpie
   const int b0 = ch * chunk;
   const int b1 = b0 + chunk < nb ? b0 + chunk : nb;
   stage_twiddles<T, N, G * M * TL>(stw, tw);
+  // Every slot's transform uses the lane's twiddles: held in registers.
+  cplx<T> twc[tw_cache_size(N, E)];
+  load_tw_cache<T, N, E>(twc, jl, stw);
   cplx<T> w[E], qq[E], acc[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -703,7 +711,7 @@ __global__ __launch_bounds__((pass_c_sum_threads<T, N, M>())) void k_pass_c_rpie
     // The transform's barriers also order the previous slot's term writes before the reads
     // below, and this slot's reads of buffer (b − 1) before the writes of slot b + 1.
     const int lb = FITS ? S::base_row(line) : line * PLD;
-    line_fft<T, N, E, true, At>(v, jl, lre + lb, lim + lb, stw);
+    line_fft<T, N, E, true, At, true>(v, jl, lre + lb, lim + lb, stw, twc);
     if (upd_obj && b > b0) store_sum(b - 1, (b - 1) & 1);
     cplx<T>* trow = terms + (b & 1) * TW + line * N + jl;
 #pragma unroll
