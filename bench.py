@@ -334,8 +334,8 @@ def main():
     ap.add_argument("--batch", type=int, default=BATCH, help="rPIE batch size (positions)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: every GPU box of this build has one, and the "
-              "reconstruction step is not sharded here", file=sys.stderr)
+        print("bench.py measures one GPU: the reconstruction step is not sharded across "
+              "GPUs (--gpus 1)", file=sys.stderr)
         sys.exit(2)
     if a.copy_probe:
         print(json.dumps(copy_probe()), flush=True)
