@@ -1856,6 +1856,19 @@ class Engine : public EngineBase {
         default: launch(std::integral_constant<int, 4>{}); break;
       }
     });
+    if (upd_obj && upd_probe) {
+      const int total = int(nprobe_ * n);
+      run(K_OBJ_UPDATE, [&] {
+        hipLaunchKernelGGL(k_rpie_updates<T>, dim3(nt + (total + THREADS - 1) / THREADS),
+                           dim3(THREADS), 0, stream_, O_.as<cplx<T>>(),
+                           (const cplx<T>*)psi_.as<cplx<T>>(), N, M_, corner_.as<int2>(), ord,
+                           list_.as<int>(), lcount_.as<int>(), int(cap_), int(nyo_), int(nxo_),
+                           ntx_, T(p.obj_step), nt, P_.as<cplx<T>>(), Wt_.as<cplx<T>>(),
+                           q_.as<cplx<T>>(), dP_.as<cplx<T>>(), nchunk, total,
+                           T(p.probe_step / nb));
+      });
+      return;
+    }
     if (upd_obj) {
       run(K_OBJ_UPDATE, [&] {
         hipLaunchKernelGGL(k_obj_update_rpie_sum<T>, dim3(nt), dim3(THREADS), 0, stream_,

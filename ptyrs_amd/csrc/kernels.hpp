@@ -734,12 +734,13 @@ __global__ __launch_bounds__((pass_c_sum_threads<T, N, M>())) void k_pass_c_rpie
   }
 }
 
-// P += s·Σ_chunks dP_part, then Wt = P·q.
+// P += s·Σ_chunks dP_part, then Wt = P·q, at element i.
 template <typename T>
-__global__ void k_probe_apply(cplx<T>* __restrict__ P, cplx<T>* __restrict__ Wt,
-                              const cplx<T>* __restrict__ q, const cplx<T>* __restrict__ dP_part,
-                              int nchunk, int n, int total, T s) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void probe_apply_at(int i, cplx<T>* __restrict__ P,
+                                               cplx<T>* __restrict__ Wt,
+                                               const cplx<T>* __restrict__ q,
+                                               const cplx<T>* __restrict__ dP_part, int nchunk,
+                                               int n, int total, T s) {
   if (i >= total) return;
   kahan<T> sr, si;
   for (int c = 0; c < nchunk; ++c) {
@@ -751,6 +752,14 @@ __global__ void k_probe_apply(cplx<T>* __restrict__ P, cplx<T>* __restrict__ Wt,
   p = p + cplx<T>{sr.s, si.s} * s;
   st(P + i, p);
   st(Wt + i, p * ld(q + i % n));
+}
+
+template <typename T>
+__global__ void k_probe_apply(cplx<T>* __restrict__ P, cplx<T>* __restrict__ Wt,
+                              const cplx<T>* __restrict__ q, const cplx<T>* __restrict__ dP_part,
+                              int nchunk, int n, int total, T s) {
+  probe_apply_at(int(blockIdx.x * blockDim.x + threadIdx.x), P, Wt, q, dP_part, nchunk, n, total,
+                 s);
 }
 
 // ---- object tiles --------------------------------------------------------------------------------
@@ -889,13 +898,13 @@ __global__ __launch_bounds__(THREADS) void k_max_reduce(const T* __restrict__ v,
 // thread adds, in slot order and Kahan-compensated, the terms of the views over it, then
 // O += step·Σ. No atomics; the result does not depend on the launch shape.
 template <typename T>
-__global__ __launch_bounds__(THREADS) void k_obj_update_rpie_sum(
-    cplx<T>* __restrict__ O, const cplx<T>* __restrict__ wsum, int N, int M,
+__device__ __forceinline__ void obj_update_rpie_tile(
+    int t, cplx<T>* __restrict__ O, const cplx<T>* __restrict__ wsum, int N, int M,
     const int2* __restrict__ corner, const int* __restrict__ order, const int* __restrict__ list,
     const int* __restrict__ count, int cap, int nyo, int nxo, int ntx, T step) {
   constexpr int RS = THREADS / TILE_X;
   __shared__ int3 cs[THREADS];
-  const int t = tile_id(), tid = threadIdx.x;
+  const int tid = threadIdx.x;
   const int X = (t % ntx) * TILE_X + tid % TILE_X, Y0 = (t / ntx) * TILE_Y + tid / TILE_X;
   const int nl = count[t];
   if (nl == 0) return;
@@ -943,6 +952,32 @@ __global__ __launch_bounds__(THREADS) void k_obj_update_rpie_sum(
     cplx<T>* o = O + (size_t)Y * nxo + X;
     st(o, ld(o) + cplx<T>{nr[r].s, ni[r].s} * step);
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_obj_update_rpie_sum(
+    cplx<T>* __restrict__ O, const cplx<T>* __restrict__ wsum, int N, int M,
+    const int2* __restrict__ corner, const int* __restrict__ order, const int* __restrict__ list,
+    const int* __restrict__ count, int cap, int nyo, int nxo, int ntx, T step) {
+  obj_update_rpie_tile(tile_id(), O, wsum, N, M, corner, order, list, count, cap, nyo, nxo, ntx,
+                       step);
+}
+
+// The object update's ntile workgroups and then the probe update's (THREADS elements each) in
+// one launch: the two read and write disjoint buffers.
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_rpie_updates(
+    cplx<T>* __restrict__ O, const cplx<T>* __restrict__ wsum, int N, int M,
+    const int2* __restrict__ corner, const int* __restrict__ order, const int* __restrict__ list,
+    const int* __restrict__ count, int cap, int nyo, int nxo, int ntx, T step, int ntile,
+    cplx<T>* __restrict__ P, cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ q,
+    const cplx<T>* __restrict__ dP_part, int nchunk, int total, T pstep) {
+  if (int(blockIdx.x) < ntile)
+    obj_update_rpie_tile(int(blockIdx.x), O, wsum, N, M, corner, order, list, count, cap, nyo,
+                         nxo, ntx, step);
+  else
+    probe_apply_at(int((blockIdx.x - ntile) * THREADS + threadIdx.x), P, Wt, q, dP_part, nchunk,
+                   N * N, total, pstep);
 }
 
 // ---- LSQML ------------------------------------------------------------------------------------------
