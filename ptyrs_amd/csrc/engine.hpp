@@ -431,6 +431,9 @@ class Engine : public EngineBase {
     // The host copy must outlive the asynchronous upload.
     run_.order = std::move(order);
     probe_terms();
+    // |W_m|² of the starting probe terms: each batch's object and probe update keeps them
+    // current for the next batch's overlaps.
+    if (p.has_obj_step) run(K_BIN, [&] { norm2_planes(); });
     // Every epoch has the same batch lengths; its order is copied into the captured order
     // buffer ahead of the replay.
     run_.base = {uint64_t(p.has_obj_step), uint64_t(p.has_probe_step), bits(p.obj_step),
@@ -1821,8 +1824,9 @@ class Engine : public EngineBase {
     const bool upd_obj = p.has_obj_step != 0, upd_probe = p.has_probe_step != 0;
     const int nt = ntiles();
     if (upd_obj) {
+      // W2_ holds |W_m|² of the batch-start probe terms (rpie_begin, then each batch's
+      // k_rpie_updates; without probe updates the probe does not change).
       run(K_BIN, [&] {
-        norm2_planes();
         hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
                            corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
                            lcount_.as<int>(), (const T*)W2_.as<T>(), M_, int(nyo_), int(nxo_),
@@ -1865,7 +1869,7 @@ class Engine : public EngineBase {
                            list_.as<int>(), lcount_.as<int>(), int(cap_), int(nyo_), int(nxo_),
                            ntx_, T(p.obj_step), nt, P_.as<cplx<T>>(), Wt_.as<cplx<T>>(),
                            q_.as<cplx<T>>(), dP_.as<cplx<T>>(), nchunk, total,
-                           T(p.probe_step / nb));
+                           T(p.probe_step / nb), W2_.as<T>());
       });
       return;
     }

@@ -734,13 +734,13 @@ __global__ __launch_bounds__((pass_c_sum_threads<T, N, M>())) void k_pass_c_rpie
   }
 }
 
-// P += s·Σ_chunks dP_part, then Wt = P·q, at element i.
+// P += s·Σ_chunks dP_part, then Wt = P·q (and, with W2, |Wt|²), at element i.
 template <typename T>
 __device__ __forceinline__ void probe_apply_at(int i, cplx<T>* __restrict__ P,
                                                cplx<T>* __restrict__ Wt,
                                                const cplx<T>* __restrict__ q,
                                                const cplx<T>* __restrict__ dP_part, int nchunk,
-                                               int n, int total, T s) {
+                                               int n, int total, T s, T* __restrict__ W2) {
   if (i >= total) return;
   kahan<T> sr, si;
   for (int c = 0; c < nchunk; ++c) {
@@ -751,7 +751,9 @@ __device__ __forceinline__ void probe_apply_at(int i, cplx<T>* __restrict__ P,
   cplx<T> p = ld(P + i);
   p = p + cplx<T>{sr.s, si.s} * s;
   st(P + i, p);
-  st(Wt + i, p * ld(q + i % n));
+  const cplx<T> w = p * ld(q + i % n);
+  st(Wt + i, w);
+  if (W2) W2[i] = norm2(w);
 }
 
 template <typename T>
@@ -759,7 +761,7 @@ __global__ void k_probe_apply(cplx<T>* __restrict__ P, cplx<T>* __restrict__ Wt,
                               const cplx<T>* __restrict__ q, const cplx<T>* __restrict__ dP_part,
                               int nchunk, int n, int total, T s) {
   probe_apply_at(int(blockIdx.x * blockDim.x + threadIdx.x), P, Wt, q, dP_part, nchunk, n, total,
-                 s);
+                 s, (T*)nullptr);
 }
 
 // ---- object tiles --------------------------------------------------------------------------------
@@ -964,20 +966,21 @@ __global__ __launch_bounds__(THREADS) void k_obj_update_rpie_sum(
 }
 
 // The object update's ntile workgroups and then the probe update's (THREADS elements each) in
-// one launch: the two read and write disjoint buffers.
+// one launch: the two read and write disjoint buffers. The probe update also leaves |W_m|² of
+// the new probe terms in W2 for the next batch's overlaps.
 template <typename T>
 __global__ __launch_bounds__(THREADS) void k_rpie_updates(
     cplx<T>* __restrict__ O, const cplx<T>* __restrict__ wsum, int N, int M,
     const int2* __restrict__ corner, const int* __restrict__ order, const int* __restrict__ list,
     const int* __restrict__ count, int cap, int nyo, int nxo, int ntx, T step, int ntile,
     cplx<T>* __restrict__ P, cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ q,
-    const cplx<T>* __restrict__ dP_part, int nchunk, int total, T pstep) {
+    const cplx<T>* __restrict__ dP_part, int nchunk, int total, T pstep, T* __restrict__ W2) {
   if (int(blockIdx.x) < ntile)
     obj_update_rpie_tile(int(blockIdx.x), O, wsum, N, M, corner, order, list, count, cap, nyo,
                          nxo, ntx, step);
   else
     probe_apply_at(int((blockIdx.x - ntile) * THREADS + threadIdx.x), P, Wt, q, dP_part, nchunk,
-                   N * N, total, pstep);
+                   N * N, total, pstep, W2);
 }
 
 // ---- LSQML ------------------------------------------------------------------------------------------
