@@ -1611,7 +1611,7 @@ class Engine : public EngineBase {
                          tilemax_.as<T>());
     });
     run(K_OVERLAP, [&] {
-      hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(THREADS), 0, stream_,
+      hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(MAX_THREADS), 0, stream_,
                          tilemax_.as<T>(), nt, pmax_.as<T>());
     });
     run(K_OBJ_UPDATE, [&] {
@@ -1649,7 +1649,7 @@ class Engine : public EngineBase {
       hipLaunchKernelGGL(k_probe_reduce<T>, dim3(nblk), dim3(THREADS), 0, stream_,
                          pnum_part_.as<cplx<T>>(), pden_part_.as<T>(), tot_chunks, n, M_,
                          pnum_.as<cplx<T>>(), pden_.as<T>(), pmaxpart_.as<T>());
-      hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(THREADS), 0, stream_,
+      hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(MAX_THREADS), 0, stream_,
                          pmaxpart_.as<T>(), nblk, pmax2_.as<T>());
       hipLaunchKernelGGL(k_probe_blend<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,
                          P_.as<cplx<T>>(), Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
@@ -1833,7 +1833,7 @@ class Engine : public EngineBase {
                            D_.as<T>(), tilemax_.as<T>());
       });
       run(K_OVERLAP, [&] {
-        hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(THREADS), 0, stream_,
+        hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(MAX_THREADS), 0, stream_,
                            tilemax_.as<T>(), nt, pmax_.as<T>());
       });
     }

@@ -882,13 +882,15 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
   if (tid == 0) tilemax[t] = r;
 }
 
+// The max of `count` non-negative values by one workgroup of MAX_THREADS.
+constexpr int MAX_THREADS = 1024;
 template <typename T>
-__global__ __launch_bounds__(THREADS) void k_max_reduce(const T* __restrict__ v, int count,
-                                                        T* __restrict__ out) {
-  __shared__ T red[THREADS];
+__global__ __launch_bounds__(MAX_THREADS) void k_max_reduce(const T* __restrict__ v, int count,
+                                                            T* __restrict__ out) {
+  __shared__ T red[MAX_THREADS];
   T m = 0;
-  for (int i = threadIdx.x; i < count; i += THREADS) m = v[i] > m ? v[i] : m;
-  T r = block_reduce<THREADS>(m, red, MaxOp());
+  for (int i = threadIdx.x; i < count; i += MAX_THREADS) m = v[i] > m ? v[i] : m;
+  T r = block_reduce<MAX_THREADS>(m, red, MaxOp());
   if (threadIdx.x == 0) out[0] = r;
 }
 
