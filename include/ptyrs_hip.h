@@ -76,12 +76,21 @@ typedef struct {
   const double* gaps;
 } ptyrs_optics;
 
-/* GpuOptions (ptyrs/src/engine/gpu/mod.rs:61-108) plus the element type. stack_size = 0
- * lets the engine choose. */
+/* GpuOptions (ptyrs/src/engine/gpu/mod.rs:61-108) plus the element type. stack_size: frames
+ * per stack for AP, DM, RAAR, ML and the LLK pass (0 lets the engine choose, the reference's
+ * None); a stack, and an rPIE or LSQML batch, of more than 65535 views is refused. graph:
+ * replay captured epochs (same bits either way). llk_slots >= 1: LLK records the device holds;
+ * the host waits for them only when that many are held or at the end of a call. fused: this
+ * engine's passes are always the fused ones, so 0 and 1 run the same kernels (the reference's
+ * M1 and M2 launch paths agree to rounding). psi_f16 (f16 carried exit waves for DM and RAAR)
+ * needs fused and is refused with PTYRS_EINVAL: this engine carries them in the element
+ * type. */
 typedef struct {
   size_t stack_size;
   int graph;
   size_t llk_slots;
+  int fused;
+  int psi_f16;
   int dtype;
 } ptyrs_gpu_options;
 
@@ -258,7 +267,7 @@ int ptyrs_engine_checkpoint(ptyrs_engine* e, size_t* every, size_t* wave_bytes);
  * (PTYRS_AMPLITUDE) over unmasked pixels; obj_grad = 2·∂L/∂O* of every slice (len as the
  * object's), probe_grad = 2·∂L/∂P* (len as the probe's), complex128 interleaved — for a real L
  * of complex z, 2·∂L/∂z* is torch's gradient. Follows Ptycho::obj_grad / probe_grad and the
- * multislice adjoint (ptyrs/src/engine/ptycho.rs:93-162, multislice.rs:266-442,
+ * multislice adjoint (ptyrs/src/engine/ptycho.rs:492-561, multislice.rs:266-442,
  * features.rs:310-330). No position offsets. */
 int ptyrs_engine_grad(ptyrs_engine* e, int cost, double* value, double* obj_grad,
                       size_t obj_len, double* probe_grad, size_t probe_len);
@@ -303,6 +312,47 @@ int ptyrs_engine_kernel_stats(ptyrs_engine* e, const char* name, size_t* launche
 const char* ptyrs_kernel_names(void);
 
 void ptyrs_engine_destroy(ptyrs_engine* e);
+
+/* Host-side scan preparation, the steps ptyrs_engine_create runs before it touches a device;
+ * none of these calls a device.
+ *
+ * Positions::pynx_preprocess along one axis (ptyrs/src/base/positions.rs:51-100): v[n] in
+ * metres, pixel the object pixel along that axis, out[n]; PTYRS_EINVAL where a position or a
+ * preprocessed one is not finite with |pos| <= 1e10. */
+int ptyrs_pynx_preprocess(const double* v, size_t n, double pixel, double* out);
+/* view_corner (ptyrs/src/base/addr.rs:118-127): the window's corner along one axis,
+ * (obj − frame)/2 + shift rounded half to even, or -1 when the window leaves the object. */
+int ptyrs_view_corner(size_t obj, size_t frame, double shift, int64_t* corner);
+/* required_obj_shape (ptyrs/src/base/addr.rs:129-149) of positions y, x (metres, as given;
+ * the reference applies it to preprocessed positions): out[0], out[1] = ny, nx. */
+int ptyrs_required_obj_shape(const double* y, const double* x, size_t n, double pixel_y,
+                             double pixel_x, size_t frame_y, size_t frame_x, size_t margin,
+                             size_t* out);
+/* AddrTable::new over the preprocessed positions (ptyrs/src/base/addr.rs:35-81), as
+ * ptyrs_engine_create builds it: the view corners [nframe][2] (oy, ox) of `scan` under
+ * `optics` (NULL: far field) in an object of nyo × nxo; len = 2·nframe. */
+int ptyrs_scan_corners(const ptyrs_scan* scan, const ptyrs_optics* optics, size_t nyo,
+                       size_t nxo, int64_t* out, size_t len);
+/* Observed::from_counts (ptyrs/src/engine/obs.rs:13-35): the counts as the device reads them,
+ * [nframe][ny][nx] float in the exit transform's DFT index order (for the far field the
+ * reference's fftshifted layout), -1 where masked; nb_obs and the unmasked total. out may be
+ * NULL (then len is not read). */
+int ptyrs_scan_observed(const ptyrs_scan* scan, const ptyrs_optics* optics, float* out,
+                        size_t len, size_t* nb_obs, double* total);
+/* quad_phase (ptyrs/src/engine/ops.rs:11-33): the far-field quadratic phase of a geometry,
+ * [ny][nx] complex interleaved in the natural (centred) layout, len = 2·ny·nx. */
+int ptyrs_quad_phase(double wavelength, double distance, double detector_pixel, size_t ny,
+                     size_t nx, double* out, size_t len);
+
+/* Fault injection for tests of the abort gate, process-wide, as the reference's test-only
+ * fault::arm (ptyrs/src/fault.rs:62-114): the exit `exit` fails at its hit after `pass` others,
+ * once, as a device failure (PTYRS_EDEVICE) raised before anything is enqueued, so every
+ * kernel already launched runs to its end. PTYRS_FAULT_LAUNCH is each kernel launch of an
+ * engine call, PTYRS_FAULT_CAPTURE each graph capture before it begins. exit = -1 disarms.
+ * ptyrs_test_fault_fired tells whether the armed exit has failed. */
+enum { PTYRS_FAULT_LAUNCH = 0, PTYRS_FAULT_CAPTURE = 1 };
+int ptyrs_test_arm_fault(int exit, size_t pass);
+int ptyrs_test_fault_fired(int* fired);
 
 /* Message of the last failed call on this thread; returns its length. */
 size_t ptyrs_last_error(char* buf, size_t len);

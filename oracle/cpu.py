@@ -2,7 +2,7 @@
 
 TEST INFRASTRUCTURE ONLY (see oracle/__init__.py): `bench.py`'s `cpu_baseline` times it and
 tests/test_oracle_cpu.py checks it against the numpy oracle's `Ptycho.rpie`
-(ptyrs/src/engine/ptycho.rs:445-687). `build()` compiles it with g++ and OpenMP (`library()` builds
+(ptyrs/src/engine/ptycho.rs:844-1086). `build()` compiles it with g++ and OpenMP (`library()` builds
 it when it is missing); the hot loops carry an FMA clone chosen at load time, so one build runs
 on any x86-64 host.
 """

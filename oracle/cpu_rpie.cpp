@@ -1,5 +1,5 @@
 // FP64 far-field rPIE batch on the CPU: a C++ restatement of the oracle's `Ptycho.rpie_batch`
-// (oracle/ptycho.py), which follows ptyrs/src/engine/ptycho.rs:501-687 (batch) and :518-546
+// (oracle/ptycho.py), which follows ptyrs/src/engine/ptycho.rs:900-1086 (batch) and :518-546
 // (modulus and Δ), with the far-field exit of ptyrs/src/engine/propagate.rs:52-62.
 //
 // TEST INFRASTRUCTURE ONLY: the CPU baseline timed beside the GPU, and checked against the

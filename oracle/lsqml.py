@@ -26,7 +26,7 @@ AMPLITUDE, POISSON = "amplitude", "poisson"
 
 
 def illumination(views, obj_shape, probe):
-    """Σ_views |P[probe_k]|² on each view's window. ptyrs/src/engine/lsqml.rs:399-416"""
+    """Σ_views |P[probe_k]|² on each view's window. ptyrs/src/engine/lsqml.rs:372-388"""
     ny, nx = probe.shape[-2:]
     out = np.zeros(obj_shape)
     p2 = probe.real ** 2 + probe.imag ** 2
@@ -36,7 +36,7 @@ def illumination(views, obj_shape, probe):
 
 
 def trim_mean(x):
-    """MATLAB trimmean(x, 10). ptyrs/src/engine/lsqml.rs:418-426"""
+    """MATLAB trimmean(x, 10). ptyrs/src/engine/lsqml.rs:390-398"""
     s = sorted(x)
     k = int(math.ceil(len(s) * 0.1 / 2.0 - 0.5))
     kept = s[k:len(s) - k]
@@ -44,7 +44,7 @@ def trim_mean(x):
 
 
 def frame_mean(d, np_):
-    """Mean over frames of planes b·np + m per mode. ptyrs/src/engine/lsqml.rs:381-397"""
+    """Mean over frames of planes b·np + m per mode. ptyrs/src/engine/lsqml.rs:353-370"""
     count = d.shape[0] // np_
     out = np.zeros((np_,) + d.shape[1:], dtype=d.dtype)
     for b in range(count):
@@ -62,7 +62,7 @@ def unique_waves(run, views, optics, objects, probe):
     """The slice waves of `views` (whole frames) at each frame's own probe planes: plane
     b·np + m of the b-th frame, moved by its offset under PATCH (q = the first slice's plane
     phase), and past one slice the last slice's phase displaced with it.
-    ptyrs/src/engine/lsqml.rs:474-530, ptyrs/src/engine/opr.rs:88-124"""
+    ptyrs/src/engine/lsqml.rs:483-536, ptyrs/src/engine/opr.rs:88-124"""
     nmode = 0
     while nmode < len(views) and views[nmode][0] == views[0][0]:
         nmode += 1
@@ -213,7 +213,7 @@ class LsqmlBatch:
 
 
 class Lsqml:
-    """LSQML state across epochs. ptyrs/src/engine/lsqml.rs:428-715"""
+    """LSQML state across epochs. ptyrs/src/engine/lsqml.rs:411-714"""
 
     def __init__(self, batch_size, cost=POISSON, seed=7, offsets=None, refine=False):
         """`offsets` ([nframe, 2] object pixels, PATCH model) are held, or refined after every
@@ -233,7 +233,7 @@ class Lsqml:
         return self.a_far.get(f, 0.5)
 
     def step(self, views, iobs, optics, obj, probe):
-        """`run_minibatch`: directions, step sizes, updates. ptyrs/src/engine/lsqml.rs:563-643
+        """`run_minibatch`: directions, step sizes, updates. ptyrs/src/engine/lsqml.rs:571-643
         Returns the object in the form given (one array, or a list of slices)."""
         pre = self.preconditioner
         floor = (0.1 * pre.max()) ** 2
@@ -253,7 +253,7 @@ class Lsqml:
         return out, probe, dict(frames=b.frames, a_far=b.a_far, a_obj=a_obj, a_probe=a_probe)
 
     def epoch(self, addr, iobs, optics, obj, probe):
-        """One epoch with pty-chi's preconditioner schedule. ptyrs/src/engine/lsqml.rs:645-714"""
+        """One epoch with pty-chi's preconditioner schedule. ptyrs/src/engine/lsqml.rs:650-714"""
         nframe, nmode = addr.nframe, addr.nmode
         views_all = addr.views()
         if self.epochs < 10 or self.epochs % 10 == 0:
@@ -276,7 +276,7 @@ class Lsqml:
         return obj, probe, steps
 
     def llk(self, ptycho, obj, probe):
-        """Normalised LLK of obj, probe over every frame. ptyrs/src/engine/lsqml.rs:532-546"""
+        """Normalised LLK of obj, probe over every frame. ptyrs/src/engine/lsqml.rs:538-555"""
         return slice_llk(ptycho.optics, ptycho.views_all, ptycho.iobs, ptycho.addr.nmode,
                          ptycho.nb_obs, obj, probe, self)
 
@@ -296,7 +296,7 @@ def cost_grads(optics, views, iobs, nmode, obj, probe, cost=POISSON):
     """The cost over `views` and its gradients 2·∂L/∂O_s* (every slice) and 2·∂L/∂P*: L the
     Poisson LLK sum (`POISSON`) or Σ(√icalc − √iobs)² (`AMPLITUDE`) over unmasked pixels; the
     exit transform's and the slices' adjoints carry the detector-plane gradient back.
-    ptyrs/src/engine/ptycho.rs:93-162, ptyrs/src/engine/features.rs:310-330,
+    ptyrs/src/engine/ptycho.rs:492-561, ptyrs/src/engine/features.rs:310-330,
     ptyrs/src/engine/multislice.rs:266-442"""
     objects = as_slices(obj)
     np_ = probe.shape[0]

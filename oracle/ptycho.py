@@ -144,7 +144,7 @@ def llk_due(calc_llk, ic, ncycle):
 
 
 def zero_phase_ramp(obj, probe):
-    """PyNX ZeroPhaseRamp. ptyrs/src/engine/ptycho.rs:733-782"""
+    """PyNX ZeroPhaseRamp. ptyrs/src/engine/ptycho.rs:593-603, :1132-1181"""
     ny, nx = probe.shape[-2:]
     far = fft2(to_shifted(probe))
     v_nat = to_natural((far.real ** 2 + far.imag ** 2).sum(axis=0))
@@ -168,7 +168,7 @@ def zero_phase_ramp(obj, probe):
 
 class Ptycho:
     """Reconstruction state: object, probe, carried exit waves, LLK history.
-    ptyrs/src/engine/ptycho.rs:188-289 (no features)"""
+    ptyrs/src/engine/ptycho.rs:188-285 (no features)"""
 
     def __init__(self, counts, y, x, obj, probe, optics):
         probe = np.array(probe, dtype=np.complex128)
@@ -199,7 +199,7 @@ class Ptycho:
         return obj_probe_to_psi(self.addr, self.views_all, self.obj, self.probe, self.quad)
 
     def llk(self):
-        """ptyrs/src/engine/ptycho.rs:453-466"""
+        """ptyrs/src/engine/ptycho.rs:452-466"""
         psi = self.optics.exit_forward(self.calc_psi())
         return llk(self.iobs, self.addr.nmode, psi).normalised(self.nb_obs)
 
@@ -207,13 +207,13 @@ class Ptycho:
         self.history.append((self.cycle, algorithm, value))
 
     def grad_psi(self):
-        """ptyrs/src/engine/ptycho.rs:148-162"""
+        """ptyrs/src/engine/ptycho.rs:547-561"""
         psi = self.optics.exit_forward(self.calc_psi())
         psi = poisson_grad(self.iobs, self.addr.nmode, psi)
         return self.optics.exit_adjoint(psi)
 
     def grads(self, obj, probe):
-        """2·∂L/∂O* and 2·∂L/∂P*. ptyrs/src/engine/ptycho.rs:93-114"""
+        """2·∂L/∂O* and 2·∂L/∂P*. ptyrs/src/engine/ptycho.rs:492-513"""
         g = self.grad_psi()
         go = (accumulate_obj(self.views_all, self.obj.shape, self.probe, self.quad, g)[0]
               if obj else np.zeros_like(self.obj))
@@ -222,7 +222,7 @@ class Ptycho:
         return go, gp
 
     def cg_gamma(self, d_obj, d_probe):
-        """ptyrs/src/engine/ptycho.rs:116-146 (no offsets)"""
+        """ptyrs/src/engine/ptycho.rs:515-545 (no offsets)"""
         v, q = self.views_all, self.quad
         po = obj_probe_to_psi(self.addr, v, self.obj, self.probe, q)
         a = (obj_probe_to_psi(self.addr, v, d_obj, self.probe, q)
@@ -232,13 +232,13 @@ class Ptycho:
         return cg_poisson_gamma(self.iobs, self.addr.nmode, po, a, b)
 
     def scale_obj_probe(self):
-        """PyNX ScaleObjProbe(absolute=True). ptyrs/src/engine/ptycho.rs:172-182"""
+        """PyNX ScaleObjProbe(absolute=True). ptyrs/src/engine/ptycho.rs:571-581"""
         calc = self.llk().nb_photons_calc
         total = float(np.nansum(self.iobs))
         self.split_scale(math.sqrt(total / calc))
 
     def split_scale(self, s):
-        """ptyrs/src/engine/ptycho.rs:184-192"""
+        """ptyrs/src/engine/ptycho.rs:583-591"""
         os_ = float(np.abs(self.obj).sum())
         ps = float(np.abs(self.probe).sum())
         fp, fo = math.sqrt(os_ / ps * s), math.sqrt(ps / os_ * s)
@@ -250,7 +250,7 @@ class Ptycho:
 
     def ap(self, ncycle, obj_inertia=0.01, probe_inertia=None, calc_llk=0,
            zero_phase_ramp=True):
-        """PyNX AP. ptyrs/src/engine/ptycho.rs:206-242"""
+        """PyNX AP. ptyrs/src/engine/ptycho.rs:605-641"""
         nmode = self.addr.nmode
         for ic in range(ncycle):
             self.psi = self.optics.exit_forward(self.calc_psi())
@@ -269,7 +269,7 @@ class Ptycho:
             self.zero_phase_ramp()
 
     def _update_obj_probe(self, inertia, loops):
-        """ptyrs/src/engine/ptycho.rs:334-356 (no positions)"""
+        """ptyrs/src/engine/ptycho.rs:733-755 (no positions)"""
         for _ in range(loops):
             for o, p in ((inertia[0], None), (None, inertia[1])):
                 if o is not None or p is not None:
@@ -278,7 +278,7 @@ class Ptycho:
 
     def dm(self, ncycle, obj_inertia=0.01, probe_inertia=0.001, alpha=0.02, loop_obj_probe=1,
            calc_llk=0, zero_phase_ramp=True):
-        """PyNX DM. ptyrs/src/engine/ptycho.rs:244-292"""
+        """PyNX DM. ptyrs/src/engine/ptycho.rs:643-691"""
         nmode, a = self.addr.nmode, alpha
         refl = None
         for ic in range(ncycle):
@@ -304,7 +304,7 @@ class Ptycho:
 
     def raar(self, ncycle, obj_inertia=0.01, probe_inertia=0.001, beta=0.9, loop_obj_probe=1,
              calc_llk=0, zero_phase_ramp=True):
-        """PyNX RAAR. ptyrs/src/engine/ptycho.rs:294-332"""
+        """PyNX RAAR. ptyrs/src/engine/ptycho.rs:693-731"""
         nmode, h = self.addr.nmode, 0.5 * beta
         iterate = None
         for ic in range(ncycle):
@@ -328,7 +328,7 @@ class Ptycho:
 
     def ml(self, ncycle, update_object=True, update_probe=False, calc_llk=0,
            zero_phase_ramp=True):
-        """PyNX ML Poisson conjugate gradient. ptyrs/src/engine/ptycho.rs:358-443"""
+        """PyNX ML Poisson conjugate gradient. ptyrs/src/engine/ptycho.rs:757-842"""
         self.scale_obj_probe()
         last = None
         d = [np.zeros_like(self.obj), np.zeros_like(self.probe)]
@@ -359,7 +359,7 @@ class Ptycho:
     def rpie(self, nepoch, obj_step=1.0, probe_step=1.0, obj_alpha=0.1, probe_alpha=0.1,
              batch_size=100, calc_llk=0, seed=7):
         """rPIE after pty-chi with the overlap-normalised object step.
-        ptyrs/src/engine/ptycho.rs:445-499"""
+        ptyrs/src/engine/ptycho.rs:844-898"""
         if self.addr.nobj != 1:
             raise ValueError("rPIE needs exactly one object mode")
         if batch_size <= 0:
@@ -373,7 +373,7 @@ class Ptycho:
 
     def rpie_delta(self, frames):
         """The batch's `psi` and `Δ = IFT(ψ') − ψ` (shifted layout).
-        ptyrs/src/engine/ptycho.rs:518-546"""
+        ptyrs/src/engine/ptycho.rs:900-945"""
         nmode = self.addr.nmode
         views = [v for f in frames for v in self.views_all[f * nmode:(f + 1) * nmode]]
         psi = obj_probe_to_psi(self.addr, views, self.obj, self.probe, self.quad)
@@ -388,7 +388,7 @@ class Ptycho:
         return views, psi, delta
 
     def rpie_batch(self, frames, obj_step, probe_step, obj_alpha, probe_alpha):
-        """One rPIE batch from the batch-start arrays. ptyrs/src/engine/ptycho.rs:501-687"""
+        """One rPIE batch from the batch-start arrays. ptyrs/src/engine/ptycho.rs:900-1086, :1089-1130"""
         ny, nx = self.probe.shape[-2:]
         nmode = self.addr.nmode
         nprobe = self.probe.shape[0]
@@ -430,7 +430,7 @@ class Ptycho:
 
 
 def far_field_ptycho(counts, y, x, obj, probe, wavelength, distance, detector_pixel):
-    """`Ptycho::new`: the far field of the scan geometry. ptyrs/src/engine/ptycho.rs:230-233"""
+    """`Ptycho::new`: the far field of the scan geometry. ptyrs/src/engine/ptycho.rs:218-221"""
     frame = list(np.asarray(counts).shape[-2:])
     optics = Optics.far_field(wavelength, distance, detector_pixel, frame)
     return Ptycho(counts, y, x, obj, probe, optics)

@@ -51,7 +51,8 @@ class _OpticsC(ctypes.Structure):
 
 class _OptionsC(ctypes.Structure):
     _fields_ = [("stack_size", ctypes.c_size_t), ("graph", ctypes.c_int),
-                ("llk_slots", ctypes.c_size_t), ("dtype", ctypes.c_int)]
+                ("llk_slots", ctypes.c_size_t), ("fused", ctypes.c_int),
+                ("psi_f16", ctypes.c_int), ("dtype", ctypes.c_int)]
 
 
 class _RpieC(ctypes.Structure):
@@ -147,6 +148,20 @@ _SIGNATURES = [
     ("ptyrs_kernel_names", [], ctypes.c_char_p),
     ("ptyrs_engine_destroy", [_P], None),
     ("ptyrs_last_error", [ctypes.c_char_p, _SZ], _SZ),
+    ("ptyrs_test_arm_fault", [ctypes.c_int, _SZ], ctypes.c_int),
+    ("ptyrs_test_fault_fired", [ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    ("ptyrs_pynx_preprocess", [_D, _SZ, ctypes.c_double, _D], ctypes.c_int),
+    ("ptyrs_view_corner", [_SZ, _SZ, ctypes.c_double, ctypes.POINTER(ctypes.c_int64)],
+     ctypes.c_int),
+    ("ptyrs_required_obj_shape", [_D, _D, _SZ, ctypes.c_double, ctypes.c_double, _SZ, _SZ, _SZ,
+                                  ctypes.POINTER(_SZ)], ctypes.c_int),
+    ("ptyrs_scan_corners", [ctypes.POINTER(_ScanC), ctypes.POINTER(_OpticsC), _SZ, _SZ,
+                            ctypes.POINTER(ctypes.c_int64), _SZ], ctypes.c_int),
+    ("ptyrs_scan_observed", [ctypes.POINTER(_ScanC), ctypes.POINTER(_OpticsC),
+                             ctypes.POINTER(ctypes.c_float), _SZ, ctypes.POINTER(_SZ), _D],
+     ctypes.c_int),
+    ("ptyrs_quad_phase", [ctypes.c_double, ctypes.c_double, ctypes.c_double, _SZ, _SZ, _D, _SZ],
+     ctypes.c_int),
 ]
 
 
@@ -316,6 +331,20 @@ class Optics:
         return o, (lam, z, dpix)
 
 
+def _scan_optics_c(scan, optics):
+    """The C structs of a scan under `optics` (None: the far field of the scan geometry)."""
+    sc = scan._c()
+    oc = None
+    if optics is not None:
+        r = optics._c(scan)
+        if isinstance(r, tuple):
+            oc, (lam, z, dpix) = r
+            sc.wavelength, sc.distance, sc.detector_pixel = lam, z, dpix
+        else:
+            oc = r
+    return sc, oc
+
+
 _GPU_OPTIONS = dict(stack_size=None, graph=True, llk_slots=8, fused=True, psi_f16=False,
                     dtype="c64")
 _AP_PARAMS = dict(obj_inertia=0.01, probe_inertia=None, calc_llk=0, zero_phase_ramp=True)
@@ -341,8 +370,10 @@ class GpuPtycho:
         opts = _overlay(_GPU_OPTIONS, options, "GpuOptions")
         if opts["dtype"] not in ("c64", "c128"):
             raise ValueError(f"dtype must be \"c64\" or \"c128\", got {opts['dtype']!r}")
-        if opts["psi_f16"]:
-            raise ValueError("psi_f16 is not supported by this engine")
+        if opts["stack_size"] is not None and int(opts["stack_size"]) < 1:
+            raise ValueError("stack_size must be >= 1")
+        if int(opts["llk_slots"]) < 1:
+            raise ValueError("llk_slots must be >= 1")
         lib = library()
         obj = np.ascontiguousarray(object, dtype=np.complex128)
         nslice = optics.nslice() if optics is not None else 1
@@ -358,19 +389,12 @@ class GpuPtycho:
             raise TypeError("probe must be complex128 [nmode, ny, nx]")
         if list(pr.shape[1:]) != scan.frame:
             raise ValueError(f"probe plane {list(pr.shape[1:])} differs from frame {scan.frame}")
-        sc = scan._c()
-        oc = None
-        if optics is not None:
-            r = optics._c(scan)
-            if isinstance(r, tuple):
-                oc, (lam, z, dpix) = r
-                sc.wavelength, sc.distance, sc.detector_pixel = lam, z, dpix
-            else:
-                oc = r
+        sc, oc = _scan_optics_c(scan, optics)
         oo = _OptionsC()
-        oo.stack_size = opts["stack_size"] or 0
+        oo.stack_size = int(opts["stack_size"] or 0)
         oo.graph = int(bool(opts["graph"]))
-        oo.llk_slots = opts["llk_slots"]
+        oo.llk_slots = int(opts["llk_slots"])
+        oo.fused, oo.psi_f16 = int(bool(opts["fused"])), int(bool(opts["psi_f16"]))
         oo.dtype = _C64 if opts["dtype"] == "c64" else _C128
         h = ctypes.c_void_p()
         _check(lib.ptyrs_engine_create(
@@ -497,7 +521,7 @@ class GpuPtycho:
         Poisson LLK sum (`cost="poisson"`, the Poisson NLL up to a constant) or
         Σ(√icalc − √iobs)² (`"amplitude"`), and the gradients 2·∂L/∂O* of every slice
         (`[nslice, nobj, nyo, nxo]`) and 2·∂L/∂P* — torch's gradients of a real loss of complex
-        tensors (ptyrs/src/engine/ptycho.rs:93-162, multislice.rs:266-442)."""
+        tensors (ptyrs/src/engine/ptycho.rs:492-561, multislice.rs:266-442)."""
         if cost not in ("poisson", "amplitude"):
             raise ValueError(f"cost must be \"poisson\" or \"amplitude\", got {cost!r}")
         go = np.empty(self._objs_shape, dtype=np.complex128)

@@ -7,7 +7,7 @@ holds the scan and computes on the GPU (there is no CPU path).
     loss = ptyrs_amd.autograd.cost(engine, objects, probe)   # objects [nslice, nobj, nyo, nxo]
     loss.backward()
 
-ptyrs/src/engine/ptycho.rs:93-162 (gradients), ptyrs/src/engine/multislice.rs:266-442 (their
+ptyrs/src/engine/ptycho.rs:492-561 (gradients), ptyrs/src/engine/multislice.rs:266-442 (their
 multislice adjoint).
 """
 

@@ -30,33 +30,119 @@ EngineBase* make_engine(int dtype, const Prepared& pre, const double* obj, size_
 
 thread_local std::string last_error;
 
-// Counts to the device's DFT index order, positions to view corners, and the exit model.
-// Follows ptyrs/src/engine/gpu/mod.rs:635-760 (shape checks, pynx_preprocess, AddrTable) and
-// ptyrs/src/engine/obs.rs:13-35 (the shifted observed intensities).
-Prepared prepare(const ptyrs_scan& s, const ptyrs_optics* optics, size_t nprobe, size_t nobj,
-                 size_t nyo, size_t nxo) {
+// The exit model of a scan under `optics` (NULL: the far field of the scan geometry).
+// Follows ptyrs/src/engine/propagate.rs:24-98 and ptyrs/src/base/geometry.rs:90-126.
+ExitModel prepare_exit(const ptyrs_scan& s, const ptyrs_optics* optics) {
   if (s.ny != s.nx) throw Invalid("frames must be square");
   if (s.ny == 0 || s.ny % 2) throw Invalid("frame sides must be even and non-zero");
-  if ((s.counts_u16 == nullptr) == (s.counts_f32 == nullptr))
-    throw Invalid("give exactly one of counts_u16 and counts_f32");
-  if (s.nframe && s.corners == nullptr && (s.y == nullptr || s.x == nullptr))
-    throw Invalid("positions missing");
-  if (nprobe == 0 || nobj == 0) throw Invalid("nprobe and nobj must be >= 1");
-  if (nyo < s.ny || nxo < s.nx) throw Invalid("object smaller than frame");
-  Prepared p;
-  p.nframe = s.nframe;
-  p.ny = s.ny;
-  p.nx = s.nx;
   const int N = int(s.ny);
   if (optics == nullptr || !optics->free_space) {
     if (!(s.wavelength > 0 && s.distance > 0 && s.detector_pixel > 0) ||
         !std::isfinite(s.wavelength * s.distance / s.detector_pixel))
       throw Invalid("wavelength, distance and detector pixel must be finite and > 0");
-    p.exit = far_field(N, N, s.wavelength, s.distance, s.detector_pixel);
-  } else {
-    p.exit = free_space(N, N, optics->wavelength, optics->pixel_y, optics->pixel_x,
-                        optics->distance);
+    return far_field(N, N, s.wavelength, s.distance, s.detector_pixel);
   }
+  return free_space(N, N, optics->wavelength, optics->pixel_y, optics->pixel_x,
+                    optics->distance);
+}
+
+// View corners [nframe][2] (oy, ox) in an object of nyo × nxo: the scan's corners as given,
+// or its positions through PyNX preprocessing and view_corner. Follows
+// ptyrs/src/base/positions.rs:51-100 and ptyrs/src/base/addr.rs:35-81.
+std::vector<int> prepare_corners(const ptyrs_scan& s, const ExitModel& exit, size_t nyo,
+                                 size_t nxo) {
+  const int N = int(s.ny);
+  if (nyo < s.ny || nxo < s.nx) throw Invalid("object smaller than frame");
+  if (s.nframe && s.corners == nullptr && (s.y == nullptr || s.x == nullptr))
+    throw Invalid("positions missing");
+  std::vector<int> corner(2 * s.nframe);
+  if (s.corners) {
+    for (size_t f = 0; f < s.nframe; ++f) {
+      const int64_t oy = s.corners[2 * f], ox = s.corners[2 * f + 1];
+      if (oy < 0 || ox < 0 || oy + N > int64_t(nyo) || ox + N > int64_t(nxo))
+        throw Invalid("corner of frame " + std::to_string(f) + " leaves the object");
+      corner[2 * f] = int(oy);
+      corner[2 * f + 1] = int(ox);
+    }
+    return corner;
+  }
+  std::vector<double> y(s.y, s.y + s.nframe), x(s.x, s.x + s.nframe);
+  for (size_t f = 0; f < s.nframe; ++f)
+    if (!(std::isfinite(y[f]) && std::isfinite(x[f]) && std::fabs(y[f]) <= 1e10 &&
+          std::fabs(x[f]) <= 1e10))
+      throw Invalid("positions must be finite with |pos| <= 1e10");
+  y = pynx_axis(y, exit.pixel[0]);
+  x = pynx_axis(x, exit.pixel[1]);
+  for (size_t f = 0; f < s.nframe; ++f) {
+    const long oy = view_corner(long(nyo), N, y[f] / exit.pixel[0]);
+    const long ox = view_corner(long(nxo), N, x[f] / exit.pixel[1]);
+    if (oy < 0 || ox < 0)
+      throw Invalid("frame " + std::to_string(f) +
+                    " leaves the object; size it with required_obj_shape");
+    corner[2 * f] = int(oy);
+    corner[2 * f + 1] = int(ox);
+  }
+  return corner;
+}
+
+// Counts to the device's DFT index order (the shifted observed intensities of
+// ptyrs/src/engine/obs.rs:13-35 for the far field), with nb_obs, the unmasked total and the
+// largest frame total. u16: 65535 is masked (ptyrs/src/base/counts.rs:3-7); float: a negative
+// value is masked, every other must be finite.
+void prepare_counts(const ptyrs_scan& s, const ExitModel& exit, Prepared& p) {
+  if ((s.counts_u16 == nullptr) == (s.counts_f32 == nullptr))
+    throw Invalid("give exactly one of counts_u16 and counts_f32");
+  const int N = int(s.ny);
+  const size_t n = size_t(N) * N;
+  p.counts.resize(s.nframe * n);
+  p.nb_obs = 0;
+  p.total = 0;
+  p.max_frame_total = 0;
+  for (size_t f = 0; f < s.nframe; ++f) {
+    double frame_total = 0;
+    for (int ky = 0; ky < N; ++ky)
+      for (int kx = 0; kx < N; ++kx) {
+        // DFT index (ky, kx) sees detector pixel k; the inverse transform flips it (the angular
+        // spectrum's two forward transforms land the field at −k).
+        int dy = ky, dx = kx;
+        if (exit.reverse) {
+          dy = (N - ky) % N;
+          dx = (N - kx) % N;
+        }
+        // A DFT plane's index to detector order, (d + N/2) mod N; a spatial plane's is its own.
+        const int h = exit.dft_plane ? N / 2 : 0;
+        const size_t src = f * n + size_t((dy + h) % N) * N + (dx + h) % N;
+        float v;
+        if (s.counts_u16) {
+          const uint16_t c = s.counts_u16[src];
+          v = c == 65535 ? -1.0f : float(c);
+        } else {
+          const float c = s.counts_f32[src];
+          if (!std::isfinite(c)) throw Invalid("counts must be finite");
+          v = c < 0 ? -1.0f : c;
+        }
+        if (v >= 0) {
+          ++p.nb_obs;
+          p.total += v;
+          frame_total += v;
+        }
+        p.counts[f * n + size_t(ky) * N + kx] = v;
+      }
+    p.max_frame_total = std::max(p.max_frame_total, frame_total);
+  }
+}
+
+// A scan, its optics and the object's shape as the device takes them. Follows
+// ptyrs/src/engine/gpu/mod.rs:635-760 (shape checks, pynx_preprocess, AddrTable).
+Prepared prepare(const ptyrs_scan& s, const ptyrs_optics* optics, size_t nprobe, size_t nobj,
+                 size_t nyo, size_t nxo) {
+  if (nprobe == 0 || nobj == 0) throw Invalid("nprobe and nobj must be >= 1");
+  Prepared p;
+  p.exit = prepare_exit(s, optics);
+  p.nframe = s.nframe;
+  p.ny = s.ny;
+  p.nx = s.nx;
+  const int N = int(s.ny);
   if (optics && optics->ngap) {
     // Optics::with_slices: every gap an angular-spectrum propagator over the frame
     // (ptyrs/src/engine/propagate.rs:84-98).
@@ -73,68 +159,8 @@ Prepared prepare(const ptyrs_scan& s, const ptyrs_optics* optics, size_t nprobe,
     }
     if (nobj != 1) throw Invalid("a multislice object has one mode");
   }
-  p.corner.resize(2 * s.nframe);
-  if (s.corners) {
-    for (size_t f = 0; f < s.nframe; ++f) {
-      const int64_t oy = s.corners[2 * f], ox = s.corners[2 * f + 1];
-      if (oy < 0 || ox < 0 || oy + N > int64_t(nyo) || ox + N > int64_t(nxo))
-        throw Invalid("corner of frame " + std::to_string(f) + " leaves the object");
-      p.corner[2 * f] = int(oy);
-      p.corner[2 * f + 1] = int(ox);
-    }
-  } else {
-    std::vector<double> y(s.y, s.y + s.nframe), x(s.x, s.x + s.nframe);
-    for (size_t f = 0; f < s.nframe; ++f)
-      if (!(std::isfinite(y[f]) && std::isfinite(x[f]) && std::fabs(y[f]) <= 1e10 &&
-            std::fabs(x[f]) <= 1e10))
-        throw Invalid("positions must be finite with |pos| <= 1e10");
-    y = pynx_axis(y, p.exit.pixel[0]);
-    x = pynx_axis(x, p.exit.pixel[1]);
-    for (size_t f = 0; f < s.nframe; ++f) {
-      const long oy = view_corner(long(nyo), N, y[f] / p.exit.pixel[0]);
-      const long ox = view_corner(long(nxo), N, x[f] / p.exit.pixel[1]);
-      if (oy < 0 || ox < 0)
-        throw Invalid("frame " + std::to_string(f) +
-                      " leaves the object; size it with required_obj_shape");
-      p.corner[2 * f] = int(oy);
-      p.corner[2 * f + 1] = int(ox);
-    }
-  }
-  const size_t n = size_t(N) * N;
-  p.counts.resize(s.nframe * n);
-  double frame_total = 0;
-  for (size_t f = 0; f < s.nframe; ++f) {
-    for (int ky = 0; ky < N; ++ky)
-      for (int kx = 0; kx < N; ++kx) {
-        // DFT index (ky, kx) sees detector pixel k; the inverse transform flips it (the angular
-        // spectrum's two forward transforms land the field at −k).
-        int dy = ky, dx = kx;
-        if (p.exit.reverse) {
-          dy = (N - ky) % N;
-          dx = (N - kx) % N;
-        }
-        // A DFT plane's index to detector order, (d + N/2) mod N; a spatial plane's is its own.
-        const int h = p.exit.dft_plane ? N / 2 : 0;
-        const size_t src = f * n + size_t((dy + h) % N) * N + (dx + h) % N;
-        float v;
-        if (s.counts_u16) {
-          const uint16_t c = s.counts_u16[src];
-          v = c == 65535 ? -1.0f : float(c);
-        } else {
-          const float c = s.counts_f32[src];
-          if (std::isnan(c)) throw Invalid("counts must not be NaN");
-          v = c < 0 ? -1.0f : c;
-        }
-        if (v >= 0) {
-          ++p.nb_obs;
-          p.total += v;
-          frame_total += v;
-        }
-        p.counts[f * n + size_t(ky) * N + kx] = v;
-      }
-    p.max_frame_total = std::max(p.max_frame_total, frame_total);
-    frame_total = 0;
-  }
+  p.corner = prepare_corners(s, p.exit, nyo, nxo);
+  prepare_counts(s, p.exit, p);
   return p;
 }
 
@@ -171,6 +197,24 @@ int guarded(ptyrs_engine* h, bool mutating, F&& f) {
 
 }  // namespace
 
+namespace {
+
+template <typename F>
+int host_call(F&& f) {
+  try {
+    f();
+    return PTYRS_OK;
+  } catch (const Invalid& x) {
+    last_error = x.what();
+    return PTYRS_EINVAL;
+  } catch (const std::exception& x) {
+    last_error = x.what();
+    return PTYRS_EDEVICE;
+  }
+}
+
+}  // namespace
+
 extern "C" {
 
 int ptyrs_engine_create(const ptyrs_scan* scan, const ptyrs_optics* optics, const double* obj,
@@ -180,8 +224,19 @@ int ptyrs_engine_create(const ptyrs_scan* scan, const ptyrs_optics* optics, cons
     if (!scan || !obj || !probe || !out) throw Invalid("null argument");
     const int dtype = options ? options->dtype : PTYRS_C64;
     if (dtype != PTYRS_C64 && dtype != PTYRS_C128) throw Invalid("dtype must be c64 or c128");
+    if (options) {
+      if (options->llk_slots == 0) throw Invalid("llk_slots must be >= 1");
+      if (options->psi_f16 && !options->fused) throw Invalid("psi_f16 needs fused passes");
+      if (options->psi_f16)
+        throw Invalid("psi_f16 is not supported: this engine carries exit waves in its element type");
+    }
     Prepared pre = prepare(*scan, optics, nprobe, nobj, nyo, nxo);
     pre.graph = options ? options->graph != 0 : true;
+    pre.stack_size = options ? options->stack_size : 0;
+    pre.llk_slots = options ? options->llk_slots : 8;
+    if (pre.stack_size && std::min(pre.stack_size, pre.nframe) * nprobe * nobj > 65535)
+      throw Invalid("a stack of " + std::to_string(pre.stack_size) +
+                    " frames holds more than 65535 views");
     auto h = std::make_unique<ptyrs_engine>();
     h->e.reset(make_engine(dtype, pre, obj, nobj, nyo, nxo, probe, nprobe, device));
     *out = h.release();
@@ -581,6 +636,99 @@ int ptyrs_engine_kernel_stats(ptyrs_engine* h, const char* name, size_t* launche
 }
 
 const char* ptyrs_kernel_names(void) { return kind_names(); }
+
+int ptyrs_test_arm_fault(int exit, size_t pass) {
+  return host_call([&] {
+    if (exit != -1 && exit != PTYRS_FAULT_LAUNCH && exit != PTYRS_FAULT_CAPTURE)
+      throw Invalid("unknown fault exit");
+    FaultArm& a = fault_arm();
+    std::lock_guard<std::mutex> g(a.m);
+    a.exit = exit;
+    a.pass = pass;
+    a.fired = false;
+    a.armed.store(exit != -1);
+  });
+}
+
+int ptyrs_test_fault_fired(int* fired) {
+  return host_call([&] {
+    if (!fired) throw Invalid("null argument");
+    FaultArm& a = fault_arm();
+    std::lock_guard<std::mutex> g(a.m);
+    *fired = a.fired ? 1 : 0;
+  });
+}
+
+int ptyrs_pynx_preprocess(const double* v, size_t n, double pixel, double* out) {
+  return host_call([&] {
+    if (n && (!v || !out)) throw Invalid("null argument");
+    for (size_t i = 0; i < n; ++i)
+      if (!(std::isfinite(v[i]) && std::fabs(v[i]) <= 1e10))
+        throw Invalid("positions must be finite with |pos| <= 1e10");
+    const std::vector<double> r = pynx_axis(std::vector<double>(v, v + n), pixel);
+    for (size_t i = 0; i < n; ++i)
+      if (!(std::isfinite(r[i]) && std::fabs(r[i]) <= 1e10))
+        throw Invalid("preprocessed positions must be finite with |pos| <= 1e10");
+    std::copy(r.begin(), r.end(), out);
+  });
+}
+
+int ptyrs_view_corner(size_t obj, size_t frame, double shift, int64_t* corner) {
+  return host_call([&] {
+    if (!corner) throw Invalid("null argument");
+    *corner = view_corner(long(obj), long(frame), shift);
+  });
+}
+
+int ptyrs_required_obj_shape(const double* y, const double* x, size_t n, double pixel_y,
+                             double pixel_x, size_t frame_y, size_t frame_x, size_t margin,
+                             size_t* out) {
+  return host_call([&] {
+    if (!out || (n && (!y || !x))) throw Invalid("null argument");
+    out[0] = required_obj_axis(y, n, pixel_y, frame_y, margin);
+    out[1] = required_obj_axis(x, n, pixel_x, frame_x, margin);
+  });
+}
+
+int ptyrs_scan_corners(const ptyrs_scan* scan, const ptyrs_optics* optics, size_t nyo,
+                       size_t nxo, int64_t* out, size_t len) {
+  return host_call([&] {
+    if (!scan || !out) throw Invalid("null argument");
+    if (len != 2 * scan->nframe) throw Invalid("corner buffer has the wrong length");
+    const ExitModel exit = prepare_exit(*scan, optics);
+    const std::vector<int> c = prepare_corners(*scan, exit, nyo, nxo);
+    for (size_t i = 0; i < len; ++i) out[i] = c[i];
+  });
+}
+
+int ptyrs_scan_observed(const ptyrs_scan* scan, const ptyrs_optics* optics, float* out,
+                        size_t len, size_t* nb_obs, double* total) {
+  return host_call([&] {
+    if (!scan) throw Invalid("null argument");
+    if (out && len != scan->nframe * scan->ny * scan->nx)
+      throw Invalid("observed buffer has the wrong length");
+    Prepared p;
+    const ExitModel exit = prepare_exit(*scan, optics);
+    prepare_counts(*scan, exit, p);
+    if (out) std::copy(p.counts.begin(), p.counts.end(), out);
+    if (nb_obs) *nb_obs = p.nb_obs;
+    if (total) *total = p.total;
+  });
+}
+
+int ptyrs_quad_phase(double wavelength, double distance, double detector_pixel, size_t ny,
+                     size_t nx, double* out, size_t len) {
+  return host_call([&] {
+    if (!out) throw Invalid("null argument");
+    if (ny == 0 || nx == 0 || ny % 2 || nx % 2) throw Invalid("frame sides must be even");
+    if (len != 2 * ny * nx) throw Invalid("phase buffer has the wrong length");
+    if (!(wavelength > 0 && distance > 0 && detector_pixel > 0) ||
+        !std::isfinite(wavelength * distance / detector_pixel))
+      throw Invalid("wavelength, distance and detector pixel must be finite and > 0");
+    const ExitModel m = far_field(int(ny), int(nx), wavelength, distance, detector_pixel);
+    std::copy(m.quad.begin(), m.quad.end(), out);
+  });
+}
 
 void ptyrs_engine_destroy(ptyrs_engine* h) { delete h; }
 

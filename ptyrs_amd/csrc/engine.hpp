@@ -7,6 +7,7 @@
 #include <atomic>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -21,6 +22,34 @@ namespace ptyrs {
     if (e_ != hipSuccess)                                                                 \
       throw DeviceError(std::string(#expr) + ": " + hipGetErrorString(e_));               \
   } while (0)
+
+// Fault injection for tests of the abort gate (ptyrs_test_arm_fault): the armed exit fails
+// at its hit after `pass` others, once, by throwing before anything is enqueued, so every
+// kernel already launched runs to its end. Process-wide, as the reference's test hook
+// (ptyrs/src/fault.rs:62-114); disarmed, an exit costs one relaxed atomic load.
+struct FaultArm {
+  std::atomic<bool> armed{false};
+  std::mutex m;
+  int exit = -1;
+  size_t pass = 0;
+  bool fired = false;
+};
+inline FaultArm& fault_arm() {
+  static FaultArm a;
+  return a;
+}
+inline void fault_at(int exit) {
+  FaultArm& a = fault_arm();
+  if (!a.armed.load(std::memory_order_relaxed)) return;
+  std::lock_guard<std::mutex> g(a.m);
+  if (a.exit != exit || a.fired) return;
+  if (a.pass > 0) {
+    --a.pass;
+    return;
+  }
+  a.fired = true;
+  throw DeviceError("injected fault at exit " + std::to_string(exit));
+}
 
 // Kernel names for timing; the index is the `kind` of a timed launch.
 enum Kind { K_PASS_A = 0, K_PASS_B, K_PASS_C, K_BIN, K_OVERLAP, K_OBJ_UPDATE, K_PROBE, K_LLK,
@@ -75,13 +104,15 @@ struct Prepared {
   double total = 0;
   double max_frame_total = 0;  // max over frames of the unmasked counts' sum
   bool graph = true;           // GpuOptions::graph: replay captured epochs
+  size_t stack_size = 0;       // GpuOptions::stack_size: frames per stack, 0 = the default
+  size_t llk_slots = 8;        // GpuOptions::llk_slots: LLK records the device holds
   ExitModel exit;
   // Multislice: object slices and, per gap, its angular-spectrum transfer function.
   size_t nslice = 1;
   std::vector<std::vector<double>> gaps;
 };
 
-// AP, DM and RAAR in one form (ptyrs/src/engine/ptycho.rs:206-356).
+// AP, DM and RAAR in one form (ptyrs/src/engine/ptycho.rs:605-755).
 struct ProjParams {
   int algorithm;  // PTYRS_AP, PTYRS_DM, PTYRS_RAAR
   bool upd_obj, upd_probe;
@@ -201,6 +232,7 @@ class EngineBase {
         g.exec = nullptr;
       }
       hipGraph_t graph = nullptr;
+      fault_at(PTYRS_FAULT_CAPTURE);
       PTYRS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
       try {
         body();
@@ -228,6 +260,7 @@ class EngineBase {
   // Runs `launch` on the engine's stream, timed with events when profiling.
   template <typename F>
   void run(int kind, F&& launch) {
+    fault_at(PTYRS_FAULT_LAUNCH);
     if (!profiling_) {
       launch();
       PTYRS_HIP(hipGetLastError());
@@ -289,6 +322,8 @@ class Engine : public EngineBase {
     M_ = int(nprobe * nobj);
     nslice_ = pre.nslice;
     graph_ = pre.graph;
+    stack_size_ = pre.stack_size;
+    hist_slots_ = std::max<size_t>(pre.llk_slots, 1);
     reverse_ = pre.exit.reverse;
     dft_plane_ = pre.exit.dft_plane;
     plane_quad_[0] = pre.exit.plane_quad[0];
@@ -307,7 +342,7 @@ class Engine : public EngineBase {
     counts_.ensure(sizeof(float) * std::max<size_t>(1, nframe_ * n));
     corner_.ensure(sizeof(int2) * std::max<size_t>(1, nframe_));
     iota_.ensure(sizeof(int) * std::max<size_t>(1, nframe_));
-    hist_.ensure(sizeof(double) * 4 * kHistSlots);
+    hist_.ensure(sizeof(double) * 4 * hist_slots_);
 
     upload_complex(O_, obj, nslice_ * nobj * nyo * nxo);
     if (nslice_ > 1) {
@@ -468,7 +503,7 @@ class Engine : public EngineBase {
     const ptyrs_rpie_params& p = run_.p;
     const bool rec = p.calc_llk > 0 && (ic % p.calc_llk == 0 || ic + 1 == run_.nepoch);
     if (rec) {
-      if (run_.due.size() == kHistSlots) drain(run_.due, PTYRS_RPIE);
+      if (run_.due.size() == hist_slots_) drain(run_.due, PTYRS_RPIE);
       const int slot = int(run_.due.size());
       llk_pass(slot);
       run_.due.push_back({cycle, slot});
@@ -543,7 +578,7 @@ class Engine : public EngineBase {
       }
       const bool rec = calc_llk > 0 && (ic % calc_llk == 0 || ic + 1 == nepoch);
       if (rec) {
-        if (due.size() == kHistSlots) drain(due, PTYRS_LSQML);
+        if (due.size() == hist_slots_) drain(due, PTYRS_LSQML);
         const int slot = int(due.size());
         llk_pass(slot);
         due.push_back({cycle, slot});
@@ -572,7 +607,7 @@ class Engine : public EngineBase {
     }
   }
 
-  // PyNX AP, DM, RAAR (ptyrs/src/engine/ptycho.rs:206-356), stacks of the whole scan; DM and
+  // PyNX AP, DM, RAAR (ptyrs/src/engine/ptycho.rs:605-755), stacks of the whole scan; DM and
   // RAAR carry their exit waves across the cycles of a call.
   void project(const ProjParams& p, size_t ncycle) override {
     one_unitary(p.algorithm == PTYRS_AP ? "AP" : p.algorithm == PTYRS_DM ? "DM" : "RAAR");
@@ -608,12 +643,12 @@ class Engine : public EngineBase {
     for (size_t ic = 0; ic < ncycle; ++ic) {
       const bool rec = p.calc_llk > 0 && (ic % p.calc_llk == 0 || ic + 1 == ncycle);
       if (rec) {
-        if (due.size() == kHistSlots) drain(due, p.algorithm);
+        if (due.size() == hist_slots_) drain(due, p.algorithm);
         const int slot = int(due.size());
         llk_pass(slot);
         due.push_back({cycle, slot});
       }
-      // Every call's first cycle starts from ψ = PO (ptyrs/src/engine/ptycho.rs:264-266).
+      // Every call's first cycle starts from ψ = PO (ptyrs/src/engine/ptycho.rs:663-668).
       const bool first = ic == 0;
       const std::vector<uint64_t> key = {uint64_t(p.algorithm), uint64_t(first), bits(p.mix),
                                          uint64_t(p.loops), uint64_t(p.upd_obj),
@@ -674,7 +709,7 @@ class Engine : public EngineBase {
     if (p.zero_phase_ramp) host_zero_phase_ramp();
   }
 
-  // PyNX ML, Poisson conjugate gradient (ptyrs/src/engine/ptycho.rs:358-443): scale_obj_probe,
+  // PyNX ML, Poisson conjugate gradient (ptyrs/src/engine/ptycho.rs:757-842): scale_obj_probe,
   // then per cycle the gradients, Polak–Ribière β, d = β·d − g, the Newton step γ along d and
   // O += γ·d_O, P += γ·d_P (or d = −g where γ is not finite).
   void ml(const ptyrs_ml_params& p, size_t ncycle) override {
@@ -732,7 +767,7 @@ class Engine : public EngineBase {
     for (size_t ic = 0; ic < ncycle; ++ic) {
       const bool rec = p.calc_llk > 0 && (ic % p.calc_llk == 0 || ic + 1 == ncycle);
       if (rec) {
-        if (due.size() == kHistSlots) drain(due, PTYRS_ML);
+        if (due.size() == hist_slots_) drain(due, PTYRS_ML);
         const int slot = int(due.size());
         llk_pass(slot);
         due.push_back({cycle, slot});
@@ -740,7 +775,7 @@ class Engine : public EngineBase {
       const std::vector<uint64_t> key = {uint64_t(ic == 0), uint64_t(uo), uint64_t(up), stack,
                                          cap_};
       epoch(ml_graph_, key, [&] {
-      // Gradients: 2·∂L/∂ψ* back-projected (ptyrs/src/engine/ptycho.rs:93-162).
+      // Gradients: 2·∂L/∂ψ* back-projected (ptyrs/src/engine/ptycho.rs:492-561).
       for (int s = 0; s < nstack; ++s) {
         const int f0 = int(s * stack);
         const int nb = std::min<int>(int(stack), int(nframe_) - f0);
@@ -857,7 +892,6 @@ class Engine : public EngineBase {
   }
 
  private:
-  static constexpr size_t kHistSlots = 64;
   static constexpr size_t kDefaultStack = 128;
   int M_ = 1;
   double scale_ = 1.0, max_frame_total_ = 0.0, total_ = 0.0;
@@ -918,7 +952,7 @@ class Engine : public EngineBase {
 
   // The probe planes of a stack's views moved by their frames' offsets as pty-chi PATCH moves
   // them, conj(q0)·DFT⁻¹(DFT(q0·P_m)·exp(−2πi f·d_b)), q0 the first slice's plane phase
-  // (ptyrs/src/engine/features.rs:386-429, lsqml.rs:474-530), into inc0_.
+  // (ptyrs/src/engine/features.rs:386-429, lsqml.rs:483-536), into inc0_.
   void moved_probes(const int* ord, int nb) {
     const int K = nb * M_;
     const size_t pn = size_t(M_) * n;
@@ -1206,7 +1240,7 @@ class Engine : public EngineBase {
     });
   }
 
-  // One LSQML batch (ptyrs/src/engine/lsqml.rs:563-643), single slice, no features.
+  // One LSQML batch (ptyrs/src/engine/lsqml.rs:571-643), single slice, no features.
   void lsqml_batch(const ptyrs_lsqml_params& p, const int* ord, int nb, int off) {
     const int nt = ntiles();
     const bool gauss = p.cost == PTYRS_AMPLITUDE;
@@ -1292,7 +1326,7 @@ class Engine : public EngineBase {
   // and 2·∂L/∂P*: the adjoint of the exit-wave model (window, probe, slices, plane phase, exit
   // transform), L the Poisson LLK sum Σ(icalc − iobs + iobs·ln(iobs/icalc)) (icalc where
   // iobs = 0) or the amplitude cost Σ(√icalc − √iobs)² over unmasked pixels.
-  // ptyrs/src/engine/ptycho.rs:93-162, ptyrs/src/engine/features.rs:310-330,
+  // ptyrs/src/engine/ptycho.rs:492-561, ptyrs/src/engine/features.rs:310-330,
   // ptyrs/src/engine/multislice.rs:266-442
   void grad(int cost, double* value, double* og, double* pg) override {
     if (nobj_ != 1) throw Invalid("the GPU gradient runs one object mode");
@@ -1405,7 +1439,7 @@ class Engine : public EngineBase {
     flush_timing();
   }
 
-  // One LSQML batch of the general model (ptyrs/src/engine/lsqml.rs:563-643 over
+  // One LSQML batch of the general model (ptyrs/src/engine/lsqml.rs:571-643 over
   // ptyrs/src/engine/multislice.rs): any number of slices, the probe planes moved by the
   // frames' offsets under PATCH. The exit waves through the slices, the detector operator as
   // for one slice, χ back through the slices with each slice's object direction and step
@@ -1583,6 +1617,8 @@ class Engine : public EngineBase {
 
   static constexpr int kAbsBlocks = 256;
   size_t stack_size_ = 0;
+  // LLK records the device holds before the host drains them (GpuOptions::llk_slots).
+  size_t hist_slots_ = 8;
 
   // Chunks of a short last stack hold no partials: zero them so every stack's slab sums.
   void zero_partials(int s, int from, int nchunk) {
@@ -1658,7 +1694,7 @@ class Engine : public EngineBase {
     });
   }
 
-  // split_scale(s) (ptyrs/src/engine/ptycho.rs:184-192); with `calc` (device, the model photons
+  // split_scale(s) (ptyrs/src/engine/ptycho.rs:583-591); with `calc` (device, the model photons
   // over unmasked pixels) s = √(Σ iobs / calc), scale_obj_probe (:172-182).
   void balance(double s, const double* calc = nullptr) {
     run(K_OTHER, [&] {
@@ -1698,6 +1734,10 @@ class Engine : public EngineBase {
 
   void ensure_workspace(size_t cap, size_t nepoch) {
     cap = std::max<size_t>(cap, 1);
+    // A stack's views fill one launch's grid y (the reference's limit, GpuOptions::stack_size).
+    if (cap * size_t(M_) > 65535)
+      throw Invalid("a stack of " + std::to_string(cap) + " frames of " + std::to_string(M_) +
+                    " modes holds more than 65535 views");
     psi_.ensure(sizeof(cplx<T>) * cap * M_ * n);
     omax_.ensure(sizeof(T) * cap * S::NBLK);
     // chunk_of and chunk_of_m never cut more chunks than this, whatever the batch length.
@@ -1807,8 +1847,8 @@ class Engine : public EngineBase {
 
   void drain(std::vector<std::pair<size_t, int>>& due, int algorithm) {
     if (due.empty()) return;
-    std::vector<double> h(4 * kHistSlots);
-    PTYRS_HIP(hipMemcpyAsync(h.data(), hist_.p, sizeof(double) * 4 * kHistSlots,
+    std::vector<double> h(4 * hist_slots_);
+    PTYRS_HIP(hipMemcpyAsync(h.data(), hist_.p, sizeof(double) * 4 * hist_slots_,
                              hipMemcpyDeviceToHost, stream_));
     PTYRS_HIP(hipStreamSynchronize(stream_));
     const double nb = double(nb_obs_);
@@ -1819,7 +1859,7 @@ class Engine : public EngineBase {
     due.clear();
   }
 
-  // One rPIE batch (ptyrs/src/engine/ptycho.rs:501-687) from the batch-start object and probe.
+  // One rPIE batch (ptyrs/src/engine/ptycho.rs:900-1086) from the batch-start object and probe.
   void rpie_batch(const ptyrs_rpie_params& p, const int* ord, int nb) {
     const bool upd_obj = p.has_obj_step != 0, upd_probe = p.has_probe_step != 0;
     const int nt = ntiles();

@@ -68,6 +68,17 @@ inline long view_corner(long obj, long frame, double shift) {
   return long(c);
 }
 
+// PyNX calc_obj_shape(..., margin, multiple=4) along one axis of preprocessed positions v
+// (metres, pixel p): 2·max(|ceil(v/p)| + 1) + frame + margin, rounded up to a multiple of 4.
+// Follows ptyrs/src/base/addr.rs:129-149.
+inline size_t required_obj_axis(const double* v, size_t count, double p, size_t frame,
+                                size_t margin) {
+  double ext = 0.0;
+  for (size_t i = 0; i < count; ++i) ext = std::fmax(ext, std::fabs(std::ceil(v[i] / p)) + 1.0);
+  const size_t len = size_t(2.0 * ext) + frame + margin;
+  return (len + 3) / 4 * 4;
+}
+
 // SplitMix64 output function and the per-epoch batches of ptyrs/src/engine/batch.rs:7-38:
 // Fisher–Yates from the end with draws from seed ^ mix(epoch), chunks sorted ascending.
 inline uint64_t mix64(uint64_t z) {
@@ -259,7 +270,7 @@ inline void host_dft(std::complex<double>* a, int n, int step) {
   for (int i = 0; i < n; ++i) a[size_t(i) * step] = y[i];
 }
 
-// PyNX ZeroPhaseRamp on host arrays (ptyrs/src/engine/ptycho.rs:733-782): the centre of mass
+// PyNX ZeroPhaseRamp on host arrays (ptyrs/src/engine/ptycho.rs:593-603, :1132-1181): the centre of mass
 // (cy, cx) of Σ_modes |fftshift(DFT(fftshift(P)))|², dx = cx − nx/2, dy = cy − ny/2, then
 // P *= exp(−2πi(dx·(x − nx/2)/nx + dy·(y − ny/2)/ny)) and
 // O *= exp(+2πi(dx·(x − nxo/2)/nx + dy·(y − nyo/2)/ny)). Arrays interleaved (re, im).

@@ -220,7 +220,7 @@ __global__ void k_norm2_planes(const cplx<T>* __restrict__ Wt, T* __restrict__ W
 // Grid (NBLK, nb·M). View (b, m) = (blockIdx.y / M, blockIdx.y % M). With `omax_part`, mode-0
 // workgroups also write max |O|² over their rows of the window.
 // The exit wave pass A transforms: PO itself, or DM's and RAAR's mix of PO with the waves
-// they carry across cycles (ptyrs/src/engine/ptycho.rs:254-332).
+// they carry across cycles (ptyrs/src/engine/ptycho.rs:643-731).
 enum Mix { MIX_NONE = 0, MIX_DM = 1, MIX_RAAR = 2, MIX_RAAR0 = 3 };
 template <typename T>
 struct MixArgs {
@@ -602,7 +602,7 @@ constexpr int rows_per_group(int n, int l, int m) {
 // mode order, each group for every M-th of its elements, into the slot's mode-0 plane; the
 // terms alternate between two LDS buffers. With `probe_dir` each group adds conj(O·q)·Δ_m
 // divided by (1 − α_p)|O|² + α_p·max_window |O|² into its mode's per-chunk probe direction.
-// ptyrs/src/engine/ptycho.rs:588-687
+// ptyrs/src/engine/ptycho.rs:900-1086
 // Threads of k_pass_c_rpie_sum: M groups of G rows of pass C's line width.
 template <typename T, int N, int M>
 constexpr int pass_c_sum_threads() {
@@ -1969,7 +1969,7 @@ __global__ __launch_bounds__(THREADS) void k_obj_blend(
 }
 
 // Σ|O| and Σ|P| in FP64: per-workgroup partials into part[2·blocks]. The host-side factors of
-// split_scale (ptyrs/src/engine/ptycho.rs:184-192) are then applied by k_balance.
+// split_scale (ptyrs/src/engine/ptycho.rs:583-591) are then applied by k_balance.
 template <typename T>
 __global__ __launch_bounds__(THREADS) void k_abs_sums(const cplx<T>* __restrict__ O, size_t no,
                                                       const cplx<T>* __restrict__ P, size_t np,
@@ -2003,7 +2003,7 @@ __global__ __launch_bounds__(THREADS) void k_balance(cplx<T>* __restrict__ O, si
   __shared__ double fac[2];
   if (threadIdx.x == 0) {
     // With `calc` (the model photons over unmasked pixels), s = √(total/calc): PyNX
-    // ScaleObjProbe(absolute=True), ptyrs/src/engine/ptycho.rs:172-182.
+    // ScaleObjProbe(absolute=True), ptyrs/src/engine/ptycho.rs:571-581.
     if (calc) s = sqrt(total / calc[0]);
     double so = 0, sp = 0;
     for (int i = 0; i < nparts; ++i) {
@@ -2160,7 +2160,7 @@ static __global__ void k_sum_pairs(const double* __restrict__ part, int count, d
 }
 
 // Polak–Ribière sums over one array in FP64: part[2·block] = Σ Re(conj(g)·(g − h)),
-// part[2·block + 1] = Σ |h|². ptyrs/src/engine/ptycho.rs:390-403
+// part[2·block + 1] = Σ |h|². ptyrs/src/engine/ptycho.rs:785-806
 template <typename T>
 __global__ __launch_bounds__(THREADS) void k_cg_dots(const cplx<T>* __restrict__ g,
                                                      const cplx<T>* __restrict__ h, size_t total,
