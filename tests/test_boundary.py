@@ -60,3 +60,41 @@ def test_scan_rejects_non_finite_counts_and_bad_geometry():
         pa.Scan(np.zeros((1, 4, 4)), [0.0], [0.0], -1e-10, 1.0, 1e-4)
     with pytest.raises(TypeError):
         pa.Scan(np.zeros((4, 4)), [0.0], [0.0], 1e-10, 1.0, 1e-4)
+
+
+def test_gpu_options_refusals_happen_before_any_device_call():
+    obj = np.ones((1, 64, 64), complex)
+    probe = np.ones((1, 32, 32), complex)
+    # GpuOptions (ptyrs/src/engine/gpu/mod.rs:61-108): stack_size Some(0) and llk_slots 0 are
+    # refused; psi_f16 needs fused, and this engine carries exit waves in its element type.
+    with pytest.raises(ValueError, match="stack_size"):
+        pa.GpuPtycho(_scan(), obj, probe, stack_size=0)
+    with pytest.raises(ValueError, match="llk_slots"):
+        pa.GpuPtycho(_scan(), obj, probe, llk_slots=0)
+    with pytest.raises(ValueError, match="psi_f16 needs fused"):
+        pa.GpuPtycho(_scan(), obj, probe, psi_f16=True, fused=False)
+    with pytest.raises(ValueError, match="psi_f16"):
+        pa.GpuPtycho(_scan(), obj, probe, psi_f16=True)
+    with pytest.raises(ValueError, match="65535 views"):
+        pa.GpuPtycho(pa.Scan(np.zeros((20000, 32, 32), np.uint16), np.zeros(20000),
+                             np.zeros(20000), 1.5e-10, 1.0, 75e-6),
+                     obj, np.ones((4, 32, 32), complex), stack_size=20000)
+
+
+def test_non_finite_float_counts_are_refused_by_the_library():
+    from ptyrs_amd import base as pb
+    c = np.zeros((1, 16, 16), np.float32)
+    s = pa.Scan(c, [0.0], [0.0], 1.0, 16.0, 1.0)
+    s.counts[0, 2, 2] = np.inf  # past the Scan's own check
+    with pytest.raises(ValueError, match="finite"):
+        pb.observed(s)
+
+
+def test_the_fault_hook_arms_and_disarms_without_a_device():
+    from ptyrs_amd import base as pb
+    pb.arm_fault("launch", 3)
+    assert not pb.fault_fired()
+    pb.arm_fault(None)
+    with pytest.raises(KeyError):
+        pb.arm_fault("nowhere")
+    assert pa.library().ptyrs_test_arm_fault(7, 0) == 1

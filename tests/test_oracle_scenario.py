@@ -1,0 +1,182 @@
+"""The seeded far-field scenario (oracle/scenario.py) and the far-field fixture families it
+unlocks, on the oracle.
+
+The scenario restates the reference's simulation (`common::dm_scenario`, ChaCha8 Poisson
+counts); it is pinned by the fixtures' own totals: the scaled start's Poisson LLK sum of
+`ml_grad_values_f64.txt` at 1e-12 (ptyrs/tests/engine_reconstruct.rs:244-256) can only match
+with every count drawn as the reference drew it. On that start: DM and RAAR against PyNX,
+rPIE against pty-chi (ptyrs/tests/engine_reconstruct.rs:161-236, LLK records at 1e-9), the ML
+object gradient against the independent f64 model (1e-12), and LSQML's step and three epochs
+against pty-chi (ptyrs/tests/engine_lsqml.rs:197-302, steps at 1e-13/1e-12, changes at 1e-12).
+"""
+
+import functools
+import math
+
+import numpy as np
+import pytest
+
+from conftest import max_rel, table
+from oracle import scenario as sc
+from oracle.lsqml import AMPLITUDE, POISSON, Lsqml, illumination
+
+
+@functools.lru_cache(maxsize=None)
+def start(nprobe):
+    return sc.parity_start(nprobe=nprobe)
+
+
+def fresh(nprobe):
+    """A copy of the scaled start that a run may change."""
+    p, inputs = start(nprobe)
+    counts, y, x, obj, probe = inputs
+    q = sc.far_field_ptycho(counts, y, x, obj, probe, *sc.GEOMETRY)
+    q.obj, q.probe = p.obj.copy(), p.probe.copy()
+    return q
+
+
+def test_chacha_core_matches_the_published_keystream():
+    # RFC 8439 A.1 test vector 1: ChaCha20, all-zero key and nonce, block 0.
+    want = ("76b8e0ada0f13d90405d6ae55386bd28bdd219b8a08ded1aa836efcc8b770dc7"
+            "da41597c5157488d7724e03fb8d84a376a43b8f41518a11cc387b669b2ee6586")
+    got = sc.chacha_blocks(bytes(32), 0, 1, 20)[0].astype("<u4").tobytes().hex()
+    assert got == want
+
+
+def test_poisson_is_seeded_and_masks():
+    # ptyrs/src/base/sim.rs:187-197
+    lam = np.array([0.0, 5.0, 100.0, 1e3]).reshape(1, 2, 2)
+    mask = np.array([False, False, False, True]).reshape(2, 2)
+    a, b = sc.poisson(lam, 7, mask), sc.poisson(lam, 7, mask)
+    assert np.array_equal(a, b) and a.ravel()[0] == 0 and a.ravel()[3] == 65535
+    with pytest.raises(ValueError):
+        sc.poisson(np.array([1e6]).reshape(1, 1, 1), 1)
+
+
+def test_poisson_sampler_has_the_poisson_moments():
+    # Both branches (Knuth below λ = 12, rejection above): mean and variance λ.
+    for lam in (3.0, 40.0):
+        v = sc.poisson(np.full((1, 100, 100), lam), 11).astype(float)
+        assert abs(v.mean() - lam) < 4 * math.sqrt(lam / v.size)
+        assert abs(v.var() / lam - 1.0) < 0.05
+
+
+def test_probe_modes_are_normalised_and_orthogonal():
+    # ptyrs/src/base/sim.rs:167-185
+    p = sc.sim_probe([32, 32], 3)
+    power = [(np.abs(p[k]) ** 2).sum() for k in range(3)]
+    assert abs(sum(power) - 1.0) < 1e-12 and abs(power[0] / power[1] - 4.0) < 1e-9
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        assert abs((p[a] * np.conj(p[b])).sum()) < 1e-6
+
+
+def test_fermat_spacing_and_raster_layout():
+    # ptyrs/src/base/sim.rs:199-215
+    y, x = sc.fermat(200, 1.0)
+    d = np.hypot(y[:, None] - y[None], x[:, None] - x[None])
+    np.fill_diagonal(d, np.inf)
+    assert 0.7 < d.min(axis=1).mean() < 1.3
+    ry, rx = sc.raster(2, 3, 2.0)
+    assert ry.tolist() == [-1.0, -1.0, -1.0, 1.0, 1.0, 1.0]
+    assert rx.tolist() == [-2.0, 0.0, 2.0, -2.0, 0.0, 2.0]
+
+
+def test_object_amplitude_and_phase_ranges():
+    # ptyrs/src/base/sim.rs:217-223
+    a = np.abs(sc.sim_object([64, 64]))
+    assert a.min() >= 0.45 and a.max() <= 0.96
+
+
+def test_scenario_counts_are_the_references():
+    # The scaled start's Poisson LLK sum and nb_obs (ptyrs/tests/engine_reconstruct.rs:244-256).
+    nb_obs, want = table("ml_grad_values_f64.txt")[0]
+    p, _ = start(2)
+    assert p.nb_obs == int(nb_obs)
+    got = p.llk().poisson * p.nb_obs
+    assert abs(got - want) <= 1e-12 * abs(want), (got, want)
+
+
+def test_obj_grad_matches_ml_grad_reference():
+    # ptyrs/tests/engine_reconstruct.rs:244-275
+    p, _ = start(2)
+    g = p.grads(True, False)[0]
+    rows = np.array(table("ml_grad_f64.txt"))
+    assert g.shape[0] == 1 and len(rows) == g.shape[1] * g.shape[2]
+    got = g[0][rows[:, 0].astype(int), rows[:, 1].astype(int)]
+    want = rows[:, 2] + 1j * rows[:, 3]
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+
+
+def history_matches(history, lag, name):
+    rows = table(name)
+    assert len(history) == len(rows)
+    for (cycle, _, l), r in zip(history, rows):
+        assert cycle == r[0] + lag
+        for a, b in zip((l.poisson, l.gaussian, l.euclidian, l.nb_photons_calc), r[1:]):
+            assert abs(a - b) < 1e-9 * abs(b), (cycle, a, b)
+
+
+@pytest.mark.parametrize("algorithm", ["dm", "raar"])
+def test_projection_llk_matches_pynx(algorithm):
+    # ptyrs/tests/engine_reconstruct.rs:184-223: record 0 is the start, the rest lag one cycle.
+    p = fresh(1)
+    first = p.llk()
+    getattr(p, algorithm)(21, calc_llk=1)
+    assert p.history[0][0] == 0 and p.history[0][2].poisson == first.poisson
+    history_matches(p.history[1:], 1, f"{algorithm}_pynx_f64.txt")
+
+
+RPIE_PTYCHI = dict(obj_step=1.0, probe_step=1.0, obj_alpha=0.5, probe_alpha=0.5, batch_size=4,
+                   calc_llk=1, seed=7)  # ptyrs/tests/common/ptychi/mod.rs:8-16
+
+
+def test_rpie_llk_matches_ptychi():
+    # ptyrs/tests/engine_reconstruct.rs:225-236
+    p = fresh(1)
+    p.rpie(20, **RPIE_PTYCHI)
+    history_matches(p.history, 0, "rpie_ptychi_f64.txt")
+
+
+def _changes(obj, probe, obj0, probe0, obj_file, probe_file, tol=1e-12):
+    for got, file, lead in ((obj - obj0, obj_file, 2), (probe - probe0, probe_file, 3)):
+        rows = np.array(table(file))
+        g = np.stack([got.real.ravel(), got.imag.ravel()], axis=1).ravel()
+        err = max_rel(g, rows[:, lead:lead + 2].ravel())
+        assert err < tol, (file, err)
+
+
+@pytest.mark.parametrize("cost,name", [(AMPLITUDE, "gaussian"), (POISSON, "poisson")])
+def test_lsqml_step_matches_ptychi(cost, name):
+    # One step on frames 0..20, the illumination as preconditioner
+    # (ptyrs/tests/engine_lsqml.rs:197-283).
+    p = fresh(2)
+    run = Lsqml(20, cost, 0)
+    run.preconditioner = illumination(p.views_all, p.obj.shape, p.probe)
+    views = p.views_all[:20 * p.addr.nmode]
+    o2, p2, st = run.step(views, p.iobs, p.optics, p.obj, p.probe)
+    rows = table(f"lsqml_{name}_steps_f64.txt")
+    assert st["frames"] == list(range(20)) and [int(r[0]) for r in rows] == list(range(20))
+    for key, col, tol in (("a_far", 1, 1e-13), ("a_obj", 2, 1e-12), ("a_probe", 3, 1e-12)):
+        assert max_rel(st[key], [r[col] for r in rows]) < tol, key
+    _changes(o2, p2, p.obj, p.probe, f"lsqml_{name}_obj_f64.txt", f"lsqml_{name}_probe_f64.txt")
+
+
+@pytest.mark.parametrize("cost,name", [(AMPLITUDE, "gaussian"), (POISSON, "poisson")])
+def test_lsqml_epochs_match_ptychi(cost, name):
+    # Three epochs of batches of 16, seed 7 (ptyrs/tests/engine_lsqml.rs:285-341).
+    p = fresh(2)
+    run = Lsqml(16, cost, 7)
+    rows = table(f"lsqml_epochs_{name}_steps_f64.txt")
+    obj, probe, nxt = p.obj, p.probe, 0
+    for e in range(3):
+        obj, probe, steps = run.epoch(p.addr, p.iobs, p.optics, obj, probe)
+        for st in steps:
+            part = rows[nxt:nxt + len(st["frames"])]
+            nxt += len(st["frames"])
+            assert all(r[0] == e for r in part) and st["frames"] == [int(r[1]) for r in part]
+            for key, col, tol in (("a_far", 2, 1e-13), ("a_obj", 3, 1e-12),
+                                  ("a_probe", 4, 1e-12)):
+                assert max_rel(st[key], [r[col] for r in part]) < tol, (e, key)
+    assert nxt == len(rows)
+    _changes(obj, probe, p.obj, p.probe, f"lsqml_epochs_{name}_obj_f64.txt",
+             f"lsqml_epochs_{name}_probe_f64.txt")
