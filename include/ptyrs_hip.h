@@ -202,7 +202,9 @@ int ptyrs_engine_rpie(ptyrs_engine* e, const ptyrs_rpie_params* params, size_t n
 int ptyrs_group_rpie(ptyrs_engine* const* engines, size_t count,
                      const ptyrs_rpie_params* params, size_t nepoch);
 /* The same with every engine's epoch captured in one graph, a branch per engine, launched once
- * per epoch (same results). No counterpart in the reference. */
+ * per epoch (same results). When any engine of the group was built with graph = 0 or has
+ * profiling on, the group runs as ptyrs_group_rpie instead (each engine eagerly or from its
+ * own graphs). No counterpart in the reference. */
 int ptyrs_group_rpie_graph(ptyrs_engine* const* engines, size_t count,
                            const ptyrs_rpie_params* params, size_t nepoch);
 

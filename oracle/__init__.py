@@ -10,12 +10,33 @@ Every function cites the reference file:line it follows. Layout conventions are 
 reference's (`ptyrs/src/engine/mod.rs:6-19`): exit waves `[nview, ny, nx]` stored FFT-shifted,
 view `k = frame * nmode + mode`, unitary 2D FFT, every object/probe sum in view order.
 
-Pinning (what checks each function; see tests/test_oracle_*.py):
-- base.py: reference unit-test values (`base/addr.rs:155-214`, `base/positions.rs:114-143`,
-  `engine/obs.rs:79-107`), random_batches against `refs/f64/random_batches.py`'s published
-  algorithm plus permutation invariants.
-- optics.py: `propagator_{select,cases,fields}_f64.txt` at 1e-14; FFT against a brute-force
-  DFT; unitarity and adjoint dot tests.
-- ptycho.py: ops hand values (`engine/ops.rs:461-502`), dot tests, the consistent-pair
-  invariant; rPIE against `rpie_near_*_f64.txt` (LLK trace, object and probe change) at 1e-12.
+What pins each module (the test files named run on the CPU):
+- base.py — tests/test_base_pins.py restates the reference's unit-test values:
+  `view_corner` half-even ties and edge rejection, `AddrTable` order, mode mapping and
+  rejection, `required_obj_shape` (`base/addr.rs:151-214`); the three `pynx_preprocess`
+  cases, including the shared sub-pixel offset (`base/positions.rs:102-143`); `observed`'s
+  shift, masking and totals, and the u16 MASKED branch (`engine/obs.rs:73-107`,
+  `base/counts.rs:140-198`). `random_batches` is pinned by the `rpie_near_*` runs below
+  (their batch order decides every traced value).
+- optics.py — tests/test_oracle_optics.py: `propagator_{select,cases,fields}_f64.txt` at
+  1e-14, `fft2` against a direct DFT, unitarity and adjoint dot tests; `quad_phase` by the
+  `engine/ops.rs:409-419` values in tests/test_base_pins.py.
+- ptycho.py — tests/test_base_pins.py (`engine/ops.rs:383-502` hand values for
+  `apply_amplitude` and `llk`, adjoint dot tests of the back-projections, the consistent-pair
+  invariant); tests/test_oracle_rpie.py (`rpie_near_*`, 9 runs × 30 epochs at 1e-12);
+  tests/test_oracle_pynx.py (AP, DM, RAAR, ML against `pynx_near_*`);
+  tests/test_oracle_scenario.py (`dm_pynx`, `raar_pynx`, `rpie_ptychi`, `ml_grad`).
+- lsqml.py — tests/test_oracle_lsqml.py (`lsqml_near_*`), tests/test_oracle_scenario.py
+  (`lsqml_{gaussian,poisson}_*`, `lsqml_epochs_*`), tests/test_oracle_gradient.py
+  (`cost_grads` against PyNX's ML gradient and five-point central differences).
+- multislice.py — tests/test_oracle_multislice.py (`lsqml_multislice_*`: forward
+  intensities, one step, 30 epochs; directions and tangent against central differences).
+- positions.py — tests/test_oracle_positions.py (`lsqml_near_pos_*`,
+  `lsqml_multislice_pos_*`; closed forms: a whole-pixel move is a roll, the MAD clip by hand).
+- scenario.py — tests/test_oracle_scenario.py: the ChaCha block function against RFC 8439's
+  A.1 keystream (20 rounds) and the reference's seeded scenario, whose Poisson LLK sum
+  equals `ml_grad_values_f64.txt`'s at 9e-16 (every count is the reference's); Poisson
+  sampler moments.
+- cpu.py / cpu_rpie.cpp (the CPU baseline) — tests/test_oracle_cpu.py: one batch at 1e-12
+  and three epochs at 1e-11 against ptycho.py's rPIE, any thread count bit-identical.
 """

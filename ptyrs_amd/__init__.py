@@ -408,6 +408,14 @@ class GpuPtycho:
         self.nslice = nslice
         self.nframe = scan.counts.shape[0]
         self.dtype = opts["dtype"]
+        self._inputs = (scan, device, optics, dict(options))
+
+    def _clone(self):
+        """A new engine on the same scan, device, optics and options, starting from this
+        engine's current object and probe (its own history and cycle count start empty)."""
+        scan, device, optics, options = self._inputs
+        obj = self.objects if self.nslice > 1 else self.object
+        return GpuPtycho(scan, obj, self.probe, device, optics, **options)
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -356,7 +356,11 @@ int group_rpie(ptyrs_engine* const* engines, size_t count, const ptyrs_rpie_para
     size_t i = 0;
     try {
       for (; i < count; ++i) es[i]->rpie_begin(params[i], nepoch);
-      if (one_graph) {
+      // One graph only when every engine captures its epochs: an engine built with graph off,
+      // or profiling its launches, runs eagerly, as ptyrs_engine_rpie would run it.
+      bool all_capture = true;
+      for (EngineBase* e : es) all_capture = all_capture && e->captures();
+      if (one_graph && all_capture) {
         group_epochs_one_graph(es);
       } else {
         // Epoch by epoch, round-robin: each engine's epoch is one replay on its own stream, so

@@ -214,6 +214,12 @@ class EngineBase {
 
   bool profiling_ = false;
   bool graph_ = true;
+
+ public:
+  // Whether this engine's epochs run as graph replays (graph option on, profiling off).
+  bool captures() const { return graph_ && !profiling_; }
+
+ protected:
   EpochGraph rpie_graph_, lsqml_graph_, proj_graph_, ml_graph_;
 
   // Replays `g` if its key matches, else captures `body` (the launches of one epoch on the

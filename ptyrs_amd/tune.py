@@ -61,26 +61,28 @@ def _load():
 
 def tune_group(engines, params=None, widths=WIDTHS, cache=True, modes=(False, True)):
     """The fastest launch for these engines: a width and whether each wave's epochs go on the
-    engines' streams (one_graph False) or in one graph per epoch. Each candidate runs one
-    ordinary rPIE epoch of every engine (tuning advances the reconstruction by one epoch per
-    candidate, with results identical to any other grouping). Returns {"width", "one_graph",
-    "seconds": {"width/mode": s}, "key"}."""
+    engines' streams (one_graph False) or in one graph per epoch. The candidates are timed on
+    copies of the engines (same scan, options, object and probe), one rPIE epoch of every copy
+    each, so the caller's engines are left as they were, cache hit or miss. Returns {"width",
+    "one_graph", "seconds": {"width/mode": s}, "key", "cached"}."""
     engines = list(engines)
     key = size_class(engines)
     if cache:
         hit = _load().get(key)
         if hit:
             return dict(hit, key=key, cached=True)
+    copies = [g._clone() for g in engines]
     times = {}
     for w in widths:
-        if w > len(engines) and w != widths[0]:
+        if w > len(copies) and w != widths[0]:
             continue
         for one in modes:
-            for g in engines:
+            for g in copies:
                 g.synchronize()
             t = time.perf_counter()
-            run_waves(engines, 1, params, w, one)
+            run_waves(copies, 1, params, w, one)
             times[(w, one)] = time.perf_counter() - t
+    del copies
     best = min(times, key=times.get)
     out = {"width": best[0], "one_graph": best[1],
            "seconds": {f"{w}/{'graph' if one else 'streams'}": v for (w, one), v in times.items()}}

@@ -65,13 +65,35 @@ def test_tuned_waves_give_the_same_bits(tmp_path, monkeypatch):
     monkeypatch.setattr(tune, "CACHE", tmp_path / "tune.json")
     dtypes = ["c64"] * 4
     a, b = engines(dtypes), engines(dtypes)
+    start = [(g.object, g.probe) for g in a]
     r = tune.tune_group(a, PARAMS, widths=(1, 2, 4))
     assert r["width"] in (1, 2, 4) and r["one_graph"] in (False, True) and not r["cached"]
+    assert len(r["seconds"]) == 6
+    # Tuning times copies: the engines are where they were, on a miss as on a hit.
+    for g, (o, p) in zip(a, start):
+        assert g.cycle == 0 and len(g.history["cycle"]) == 0
+        assert np.array_equal(g.object, o) and np.array_equal(g.probe, p)
     assert tune.tune_group(a, PARAMS)["cached"]
-    ran = len(r["seconds"])
-    assert ran == 6
+    tune.run_waves(a, 3, PARAMS, r["width"], r["one_graph"])
     for g, p in zip(b, PARAMS):
-        g.rpie(ran, **p)
+        g.rpie(3, **p)
     for g, h in zip(a, b):
-        assert g.cycle == h.cycle == ran
+        assert g.cycle == h.cycle == 3
         assert np.array_equal(g.object, h.object) and np.array_equal(g.probe, h.probe)
+
+
+def test_one_graph_group_with_an_eager_engine_runs_it_eagerly():
+    # An engine built with graph=False (or profiling) is not captured: the group falls back to
+    # the per-stream launch, with the bits of each engine run alone.
+    a = engines(["c64", "c64"])
+    b = engines(["c64", "c64"])
+    eager = engines(["c64"], graph=False)[0]
+    alone = engines(["c64"], graph=False)[0]
+    pa.rpie_group(a + [eager], 2, PARAMS[:2] + [PARAMS[2]], one_graph=True)
+    for g, p in zip(b + [alone], PARAMS[:3]):
+        g.rpie(2, **p)
+    for g, h in zip(a + [eager], b + [alone]):
+        assert np.array_equal(g.object, h.object) and np.array_equal(g.probe, h.probe)
+    a[0].profile(True)
+    pa.rpie_group(a, 1, PARAMS[:2], one_graph=True)
+    assert a[0].kernel_stats()["pass_a"][0] > 0

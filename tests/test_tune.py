@@ -8,9 +8,16 @@ import pytest
 from ptyrs_amd import tune
 
 
-def fake(n, modes, frames, dtype="c64"):
-    return types.SimpleNamespace(_probe_shape=(modes, n, n), nframe=frames, dtype=dtype,
-                                 synchronize=lambda: None)
+def fake(n, modes, frames, dtype="c64", clones=None):
+    g = types.SimpleNamespace(_probe_shape=(modes, n, n), nframe=frames, dtype=dtype,
+                              synchronize=lambda: None)
+    def clone():
+        c = fake(n, modes, frames, dtype)
+        if clones is not None:
+            clones.append(c)
+        return c
+    g._clone = clone
+    return g
 
 
 def test_size_class_rounds_positions_and_ignores_order():
@@ -39,9 +46,14 @@ def test_waves_are_largest_first_and_cover_every_engine(monkeypatch):
 
 def test_tuner_caches_its_choice(tmp_path, monkeypatch):
     monkeypatch.setattr(tune, "CACHE", tmp_path / "t.json")
-    monkeypatch.setattr(tune, "rpie_group", lambda gs, nepoch, ps, one: None)
-    es = [fake(128, 1, 200), fake(256, 4, 100)]
+    ran = []
+    monkeypatch.setattr(tune, "rpie_group", lambda gs, nepoch, ps, one: ran.extend(gs))
+    clones = []
+    es = [fake(128, 1, 200, clones=clones), fake(256, 4, 100, clones=clones)]
     r = tune.tune_group(es, widths=(1, 2))
+    # Only copies of the engines run epochs.
+    assert ran and all(any(g is c for c in clones) for g in ran)
+    assert not any(g is e for g in ran for e in es)
     assert r["width"] in (1, 2) and not r["cached"]
     again = tune.tune_group(es, widths=(1, 2))
     assert again["cached"] and again["width"] == r["width"]
