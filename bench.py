@@ -7,9 +7,12 @@ batches of 100 positions, ptyrs/src/engine/ptycho.rs:145-186). A step is one rPI
 over all 1024 positions; `value` is positions·iterations per second of the complex-float
 (FP32) engine with the scan resident in HBM. The plain complex-double (FP64) engine runs the
 same epochs from the same start; `nrmse` compares the two objects after phase-ramp and
-global-phase alignment on the illuminated region.
+global-phase alignment on the illuminated region, and `nrmse.vs_oracle` one epoch of each
+precision against the FP64 CPU restatement (oracle/cpu_rpie.cpp) from the same start. The line
+also carries LSQML's rates and kernel roofline and AP, DM, RAAR and ML rates on the same scan.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --algorithm lsqml|ap|dm|raar|ml [--steps K]   # one algorithm alone
     python bench.py --multi-scan [--steps K]   # 16 uneven scans: sequential vs grouped epochs
     python bench.py --large-multislice [--steps K]   # 512² x 4, 4096² x 8 slices, refined positions
     python bench.py --copy-probe     # a plain 2 GiB device copy: the PMC calibration kernel and
@@ -143,6 +146,7 @@ def cpu_baseline(d, batch):
     t = time.perf_counter()
     c.rpie(1, batch_size=batch)
     dt = time.perf_counter() - t
+    oracle_state = dict(obj=c.obj[None].copy(), probe=c.probe.copy(), corners=corners)
     c1 = cpu.CpuRpie(d["counts"], corners, d["start_obj"], d["start_probe"], optics.quad,
                      threads=1)
     t = time.perf_counter()
@@ -154,7 +158,92 @@ def cpu_baseline(d, batch):
                        f"1024² object, FP64, C++ with OpenMP on {threads} threads: {dt:.2f} s",
                 single_thread=dict(value=round(3 * batch / dt1, 1), cores=1,
                                    sample=f"three rPIE batches of {batch} positions on one "
-                                          f"thread: {dt1:.2f} s"))
+                                          f"thread: {dt1:.2f} s")), oracle_state
+
+
+def against_oracle(pa, scan, d, params, state):
+    """Object and probe NRMSE of one rPIE epoch of each GPU precision against the FP64 CPU
+    restatement's epoch from the same start, corners and batches (the object after phase-ramp
+    and global-phase alignment on the illuminated region, the probe after a global phase)."""
+    weight = illumination(state["corners"], state["probe"], state["obj"].shape)
+    out = {"oracle": "oracle/cpu_rpie.cpp (FP64), one epoch", "epochs": 1}
+    for dtype in ("c64", "c128"):
+        g = pa.GpuPtycho(scan, d["start_obj"], d["start_probe"], dtype=dtype)
+        if not np.array_equal(g.corners, state["corners"]):
+            raise RuntimeError("engine and oracle corners differ")
+        g.rpie(1, **params)
+        pw = np.ones(state["probe"].shape[-2:])
+        out[dtype] = {"object": aligned_nrmse(g.object, state["obj"], weight),
+                      "probe": float(np.sqrt(sum(phase_nrmse(g.probe[m], state["probe"][m], pw) ** 2
+                                                 for m in range(M)) / M)),
+                      "probe_rel_l2": float(np.linalg.norm(g.probe - state["probe"])
+                                            / np.linalg.norm(state["probe"]))}
+        del g
+    return out
+
+
+# Algorithmic bytes per position of each timed launch kind (c bytes per complex, r per real):
+# the reads and writes each kernel's arithmetic needs, counted once.
+def rpie_bytes(c, r):
+    n = N * N
+    return {
+        "pass_a": n * c + M * n * c,                 # O window in, rows-transformed Ψ out
+        "pass_b": 2 * M * n * c + 4 * n,             # Ψ in and out, float counts in
+        # Ψ' and the overlaps D_m in, O window in, the mode-summed object term out
+        "pass_c": M * n * c + M * n * r + n * c + n * c,
+        "obj_update": n * c,                         # the summed term in (object RMW amortised)
+    }
+
+
+def lsqml_bytes(c, r):
+    n = N * N
+    return {
+        "pass_a": n * c + M * n * c,
+        # icalc pass: Ψ and counts in, icalc out; Poisson pass: Ψ and counts in, Ψ out
+        "pass_b": 3 * M * n * c + 8 * n + n * r,
+        "pass_c": 2 * M * n * c + n * c,             # Ψ' and the O window in, χ out
+        "obj_dir": M * n * c,                        # χ of every view over the tile in
+        "dots": M * n * c + 2 * n * c,               # χ, the O and dO windows in
+    }
+
+
+def kernel_roofline(stats, per_frame, positions):
+    """Roofline of the launch kind with the largest total time: its algorithmic bytes per
+    launch (`per_frame` bytes × the positions one launch covers, `positions` over all its
+    launches) over its mean launch time from HIP events."""
+    kern = max(stats, key=lambda k: stats[k][1])
+    launches, ms = stats[kern]
+    frames_per_launch = positions / max(launches, 1)
+    alg_bytes = per_frame.get(kern, 0) * frames_per_launch
+    avg_s = ms / 1e3 / max(launches, 1)
+    achieved = alg_bytes / avg_s / 1e9 if alg_bytes else None
+    return {"kernel": kern, "bound": "hbm",
+            "achieved": round(achieved, 1) if achieved else None,
+            "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": round(achieved * 1e9 / HBM_PEAK, 4) if achieved else None,
+            "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_s * 1e3, 4)}
+
+
+def algorithm_rates(pa, scan, d, steps=3, algorithms=("ap", "dm", "raar", "ml")):
+    """AP, DM, RAAR and ML (their default parameters, no LLK records) on the same scan:
+    positions·cycle/s of `steps` cycles after one warm-up cycle, complex64 and complex128."""
+    out = {"steps": steps}
+    for alg in algorithms:
+        r = {}
+        for dtype in ("c64", "c128"):
+            g = pa.GpuPtycho(scan, d["start_obj"], d["start_probe"], dtype=dtype)
+            getattr(g, alg)(1)
+            g.synchronize()
+            t = time.perf_counter()
+            getattr(g, alg)(steps)
+            g.synchronize()
+            dt = time.perf_counter() - t
+            r["value" if dtype == "c64" else "fp64"] = round(NPOS * steps / dt, 1)
+            del g
+        r["speedup_fp32_vs_fp64"] = round(r["value"] / r["fp64"], 3)
+        out[alg] = r
+        print(f"[bench] {alg}: {r}", file=sys.stderr, flush=True)
+    return out
 
 
 def lsqml_rates(pa, scan, d, a, steps=5):
@@ -171,6 +260,14 @@ def lsqml_rates(pa, scan, d, a, steps=5):
         dt = time.perf_counter() - t
         out["value" if dtype == "c64" else "fp64"] = round(NPOS * steps / dt, 1)
         out["llk_poisson_" + dtype] = g.llk()["poisson"]
+        if dtype == "c64":
+            g.profile(True)
+            g.lsqml(1, 0, a.batch, "poisson", 7)
+            g.synchronize()
+            stats = g.kernel_stats()
+            g.profile(False)
+            out["kernels_ms_per_step"] = {k: round(v[1], 4) for k, v in stats.items() if v[0]}
+            out["roofline"] = kernel_roofline(stats, lsqml_bytes(8, 4), NPOS)
     return out
 
 
@@ -332,6 +429,8 @@ def main():
     ap.add_argument("--no-copy", action="store_true",
                     help="skip the child copy-bandwidth process (counter passes)")
     ap.add_argument("--batch", type=int, default=BATCH, help="rPIE batch size (positions)")
+    ap.add_argument("--algorithm", choices=("lsqml", "ap", "dm", "raar", "ml"), default=None,
+                    help="time one other algorithm alone on the headline scan (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the reconstruction step is not sharded across "
@@ -347,7 +446,7 @@ def main():
         large_multislice(a.steps)
         return
 
-    copy = None if a.no_copy else copy_bandwidth()
+    copy = None if (a.no_copy or a.algorithm) else copy_bandwidth()
 
     import ptyrs_amd as pa
     from ptyrs_amd import sim
@@ -360,6 +459,14 @@ def main():
                    d["detector_pixel"])
     params = dict(batch_size=a.batch, obj_step=1.0, probe_step=1.0, obj_alpha=0.1, probe_alpha=0.1,
                   seed=7)
+    if a.algorithm == "lsqml":
+        print(json.dumps({"workload": "headline scan", "lsqml": lsqml_rates(pa, scan, d, a, a.steps)}),
+              flush=True)
+        return
+    if a.algorithm is not None:
+        out = algorithm_rates(pa, scan, d, a.steps, (a.algorithm,))
+        print(json.dumps({"workload": "headline scan", **out}), flush=True)
+        return
 
     # Timed: captured epochs (hipGraph replays), no per-kernel events. Then a few more epochs
     # launched eagerly under HIP events on the engine's stream for the per-kernel times.
@@ -401,22 +508,9 @@ def main():
 
     # Dominant kernel: the largest total time under HIP events over the timed region.
     stats = r32["stats"]
-    kern = max(stats, key=lambda k: stats[k][1])
-    launches, ms = stats[kern]
-    c = 8  # bytes per complex64
-    n = N * N
-    r = 4  # bytes per float
-    per_frame = {
-        "pass_a": n * c + M * n * c,                 # O window in, rows-transformed Ψ out
-        "pass_b": 2 * M * n * c + 4 * n,             # Ψ in and out, float counts in
-        # Ψ' and the overlaps D_m in, O window in, the mode-summed object term out
-        "pass_c": M * n * c + M * n * r + n * c + n * c,
-        "obj_update": n * c,                         # the summed term in (object RMW amortised)
-    }
-    frames_per_launch = NPOS * prof_steps / max(launches, 1)
-    alg_bytes = per_frame.get(kern, 0) * frames_per_launch
-    avg_s = ms / 1e3 / max(launches, 1)
-    achieved = alg_bytes / avg_s / 1e9 if alg_bytes else None
+    roof = kernel_roofline(stats, rpie_bytes(8, 4), NPOS * prof_steps)
+    roof["traffic"] = measured_traffic(roof["kernel"]) if a.batch == BATCH else None
+    achieved = roof["achieved"]
 
     out = {
         "metric": "positions/sec/iter at 256² probe ×4 modes: FP32 vs FP64 plain at fixed NRMSE",
@@ -443,20 +537,20 @@ def main():
                   "llk_poisson_c64": llk32, "llk_poisson_c128": llk64},
         "epoch_launch": "hipGraph replay (kernel times from eager epochs under HIP events)",
         "kernels_ms_per_step": {k: round(v[1] / prof_steps, 4) for k, v in stats.items()},
-        "roofline": {"kernel": kern, "bound": "hbm",
-                     "achieved": round(achieved, 1) if achieved else None,
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved * 1e9 / HBM_PEAK, 4) if achieved else None,
-                     "traffic": measured_traffic(kern) if a.batch == BATCH else None,
-                     "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_s * 1e3, 4)},
+        "roofline": roof,
     }
     if copy:
         out["roofline"]["copy_gb_per_s"] = copy["gb_per_s"]
         if achieved:
             out["roofline"]["frac_of_copy"] = round(achieved / copy["gb_per_s"], 4)
+    for k in ("c64", "c128"):
+        del results[k]["engine"]
+    del g, g32, g64
     out["lsqml"] = lsqml_rates(pa, scan, d, a)
+    out["algorithms"] = algorithm_rates(pa, scan, d)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(d, a.batch)
+        out["cpu_baseline"], state = cpu_baseline(d, a.batch)
+        out["nrmse"]["vs_oracle"] = against_oracle(pa, scan, d, params, state)
     print(json.dumps(out), flush=True)
 
 

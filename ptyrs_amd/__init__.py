@@ -169,7 +169,7 @@ def library():
     """The loaded libptyrs_hip.so; raises if it has not been built."""
     global _lib
     if _lib is None:
-        path = os.environ.get("PTYRS_HIP_LIB", str(_LIB_PATH))
+        path = str(_LIB_PATH)
         if not os.path.exists(path):
             raise RuntimeError(f"{path} is missing: build it with `python -m ptyrs_amd.build`")
         lib = ctypes.CDLL(path)

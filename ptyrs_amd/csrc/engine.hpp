@@ -53,9 +53,10 @@ inline void fault_at(int exit) {
 
 // Kernel names for timing; the index is the `kind` of a timed launch.
 enum Kind { K_PASS_A = 0, K_PASS_B, K_PASS_C, K_BIN, K_OVERLAP, K_OBJ_UPDATE, K_PROBE, K_LLK,
-            K_OTHER, K_COUNT };
+            K_OTHER, K_OBJ_DIR, K_DOTS, K_COUNT };
 inline const char* kind_names() {
-  return "pass_a,pass_b,pass_c,bin_tiles,obj_overlap,obj_update,probe_update,llk,other";
+  return "pass_a,pass_b,pass_c,bin_tiles,obj_overlap,obj_update,probe_update,llk,other,"
+         "obj_dir,dots";
 }
 
 struct DevBuf {
@@ -1176,14 +1177,11 @@ class Engine : public EngineBase {
   }
 
   static constexpr size_t kIllumChunk = 1024;
-#ifndef PTYRS_PASSC_BLOCKS
-#define PTYRS_PASSC_BLOCKS 512
-#endif
   // rPIE's pass C cuts a batch into chunks of slots so that its grid has about this many
   // workgroups (each chunk leaves one probe-direction partial per mode). At 256² × 4 modes, 512
   // is one round at two workgroups per CU: 0.98 ms per epoch, against 1.03 at 1024 and 1.73
   // at 256.
-  static constexpr int kPassCBlocks = PTYRS_PASSC_BLOCKS;
+  static constexpr int kPassCBlocks = 512;
   size_t illum_chunk() const { return std::max<size_t>(1, std::min(nframe_, kIllumChunk)); }
 
   void ensure_lsqml(size_t cap) {
@@ -1293,7 +1291,7 @@ class Engine : public EngineBase {
                          Wt_.as<cplx<T>>(), q_.as<cplx<T>>(), corner_.as<int2>(), ord, nb, M_,
                          chunk, 1, dP_.as<cplx<T>>(), tw_.as<cplx<T>>());
     });
-    run(K_OBJ_UPDATE, [&] {
+    run(K_OBJ_DIR, [&] {
       hipLaunchKernelGGL(k_lsqml_obj_dir<T>, dim3(nt), dim3(THREADS), 0, stream_,
                          dO_.as<cplx<T>>(), Wt_.as<cplx<T>>(), psi_.as<cplx<T>>(), N, M_,
                          corner_.as<int2>(), ord, list_.as<int>(), lcount_.as<int>(), int(cap_),
@@ -1307,7 +1305,7 @@ class Engine : public EngineBase {
       hipLaunchKernelGGL(k_probe_mean<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,
                          dPm_.as<cplx<T>>(), dP_.as<cplx<T>>(), nchunk, total, nb);
     });
-    run(K_OTHER, [&] {
+    run(K_DOTS, [&] {
       hipLaunchKernelGGL(k_lsqml_dots<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
                          dO_.as<cplx<T>>(), O_.as<cplx<T>>(), int(nxo_), Wt_.as<cplx<T>>(),
                          dPm_.as<cplx<T>>(), q_.as<cplx<T>>(), psi_.as<cplx<T>>(), N, M_,

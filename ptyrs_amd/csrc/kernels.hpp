@@ -47,12 +47,6 @@ constexpr int lines_per_block(int n, int tl, int maxt, int maxl) {
   return best;
 }
 
-// PTYRS_LDS_TABLE 0 places every line in its own region padded by i >> 4 (for comparison).
-#ifndef PTYRS_LDS_TABLE
-#define PTYRS_LDS_TABLE 1
-#endif
-constexpr LdsLay lds_plain(int n, int l) { return LdsLay{0, 4, 1, n + (n >> 4) + 1, l * (n + (n >> 4) + 1)}; }
-
 // A workgroup of L lines of N values, TL = N/E threads per line, elements of WB bytes.
 template <int N, int E_, int MAXT, int MAXL, int WB>
 struct Lines {
@@ -64,10 +58,8 @@ struct Lines {
   static constexpr int NBLK = N / L;                          // workgroups per plane
   // LDS placement of the lines' exchanges for lanes along a line (row passes) or across the
   // lines (column passes): element placement and the offset of line `line`'s element 0.
-  static constexpr LdsLay ROWL =
-      PTYRS_LDS_TABLE ? lds_layout(N, E, L, true, WB) : lds_plain(N, L);
-  static constexpr LdsLay COLL =
-      PTYRS_LDS_TABLE ? lds_layout(N, E, L, false, WB) : lds_plain(N, L);
+  static constexpr LdsLay ROWL = lds_layout(N, E, L, true, WB);
+  static constexpr LdsLay COLL = lds_layout(N, E, L, false, WB);
   using AtRow = LdsAt<ROWL.shift, ROWL.mul, ROWL.inter ? L : 1>;
   using AtCol = LdsAt<COLL.shift, COLL.mul, COLL.inter ? L : 1>;
   static constexpr int base_row(int line) { return ROWL.inter ? line : line * ROWL.ld; }
@@ -90,49 +82,28 @@ using RowShapeC = Lines<N, choose_e(N, sizeof(T) == 4 ? 8 : 4), THREADS, N, size
 // columns): 8 values per thread (float), 4 (double), so that four modes stay in registers with
 // two workgroups per CU; neighbouring stripes share their 128-byte lines through L2
 // (block_xy).
-#ifndef PTYRS_COL_LINES_F32
-#define PTYRS_COL_LINES_F32 8
-#endif
-#ifndef PTYRS_COL_LINES_F64
-#define PTYRS_COL_LINES_F64 4
-#endif
 template <typename T, int N>
-using ColShape = Lines<N, choose_e(N, sizeof(T) == 4 ? 8 : 4), 512,
-                       sizeof(T) == 4 ? PTYRS_COL_LINES_F32 : PTYRS_COL_LINES_F64, sizeof(T)>;
+using ColShape = Lines<N, choose_e(N, sizeof(T) == 4 ? 8 : 4), 512, sizeof(T) == 4 ? 8 : 4,
+                       sizeof(T)>;
 
 // ---- workgroup placement --------------------------------------------------------------------------
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2): dispatch slot L runs on
 // XCD L % 8. The column passes renumber a 2D grid whose size divides by 8 so that consecutive
 // logical blocks (neighbouring column stripes of a plane, which share 128-byte lines) run on one
 // XCD at about the same time and share its L2.
-#ifndef PTYRS_XCD_REMAP
-#define PTYRS_XCD_REMAP 1
-#endif
 __device__ __forceinline__ int2 block_xy() {
-  if constexpr (PTYRS_XCD_REMAP) {
-    const int total = gridDim.x * gridDim.y;
-    const int L = blockIdx.y * gridDim.x + blockIdx.x;
-    if (total % 8 == 0) {
-      const int id = (L % 8) * (total / 8) + L / 8;
-      return int2{id % int(gridDim.x), id / int(gridDim.x)};
-    }
+  const int total = gridDim.x * gridDim.y;
+  const int L = blockIdx.y * gridDim.x + blockIdx.x;
+  if (total % 8 == 0) {
+    const int id = (L % 8) * (total / 8) + L / 8;
+    return int2{id % int(gridDim.x), id / int(gridDim.x)};
   }
   return int2{int(blockIdx.x), int(blockIdx.y)};
 }
 
-// The object tile of a tile kernel (1D grid of tiles, row-major): with PTYRS_TILE_REMAP each
-// XCD takes a contiguous band of tiles, whose views read nearby probe terms through one L2.
-#ifndef PTYRS_TILE_REMAP
-#define PTYRS_TILE_REMAP 0
-#endif
-__device__ __forceinline__ int tile_id() {
-  if constexpr (PTYRS_TILE_REMAP) {
-    const int total = gridDim.x;
-    const int L = blockIdx.x;
-    if (total % 8 == 0) return (L % 8) * (total / 8) + L / 8;
-  }
-  return int(blockIdx.x);
-}
+// The object tile of a tile kernel (1D grid of tiles, row-major), in dispatch order (tiles
+// dealt to the XCDs in contiguous bands measured slower).
+__device__ __forceinline__ int tile_id() { return int(blockIdx.x); }
 
 // A thread-group index tid / GROUP: when groups are whole waves it is the same in every lane
 // of a wave, and is read from the first lane so the compiler keeps it (and the plane addresses
@@ -894,9 +865,6 @@ __global__ __launch_bounds__(MAX_THREADS) void k_max_reduce(const T* __restrict_
   if (threadIdx.x == 0) out[0] = r;
 }
 
-#ifndef PTYRS_OBJ_SU
-#define PTYRS_OBJ_SU 4
-#endif
 // rPIE's object update from k_pass_c_rpie_sum's per-slot sums (stored in each slot's mode-0
 // plane, every mode's term already divided by its denominator): every object pixel owned by one
 // thread adds, in slot order and Kahan-compensated, the terms of the views over it, then
@@ -925,7 +893,7 @@ __device__ __forceinline__ void obj_update_rpie_tile(
     }
     __syncthreads();
     // SU slots per step: their loads are issued together, then added in slot order.
-    constexpr int SU = PTYRS_OBJ_SU;
+    constexpr int SU = 4;
     for (int i0 = 0; i0 < k && X < nxo; i0 += SU) {
 #pragma unroll
       for (int r = 0; r < TROWS; ++r) {
