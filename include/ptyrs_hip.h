@@ -281,6 +281,16 @@ int ptyrs_engine_intensities(ptyrs_engine* e, double* out, size_t len);
 
 /* GpuPtycho::llk (ptyrs/src/engine/gpu/mod.rs:853-868): normalised LLK of the current state. */
 int ptyrs_engine_llk(ptyrs_engine* e, ptyrs_llk_record* out);
+/* The LLK of the current state, as ptyrs_engine_llk gives it (the same bits), for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). No counterpart in the reference. The call allocates nothing: its
+ * exit-wave stacks, per-workgroup sums and result live in one static device array of 64 MiB
+ * per GPU, held until the process ends. A scan
+ * whose per-frame sums (32 B per frame and pass-B workgroup) and one frame's exit waves do not
+ * fit returns PTYRS_EINVAL, as do several slices, position offsets and profiling. Every call
+ * on a GPU shares that GPU's array: calls that overlap on one GPU (engines on several host
+ * threads) race and must be serialised by the caller. */
+int ptyrs_engine_llk_noalloc(ptyrs_engine* e, ptyrs_llk_record* out);
 
 /* GpuPtycho::objects / probe (ptyrs/src/engine/gpu/mod.rs:762-815): complex128 downloads of
  * len doubles (2·nslice·nobj·nyo·nxo, every slice; 2·nprobe·ny·nx). Allowed on an aborted

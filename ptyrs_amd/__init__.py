@@ -134,6 +134,7 @@ _SIGNATURES = [
     ("ptyrs_engine_set_checkpoint", [_P, _SZ], ctypes.c_int),
     ("ptyrs_engine_checkpoint", [_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)], ctypes.c_int),
     ("ptyrs_engine_llk", [_P, ctypes.POINTER(_RecordC)], ctypes.c_int),
+    ("ptyrs_engine_llk_noalloc", [_P, ctypes.POINTER(_RecordC)], ctypes.c_int),
     ("ptyrs_engine_object", [_P, _D, _SZ], ctypes.c_int),
     ("ptyrs_engine_probe", [_P, _D, _SZ], ctypes.c_int),
     ("ptyrs_engine_set_object", [_P, _D, _SZ], ctypes.c_int),
@@ -568,6 +569,15 @@ class GpuPtycho:
         """Normalised PyNX LLK of the current object and probe."""
         r = _RecordC()
         _check(library().ptyrs_engine_llk(self._h, ctypes.byref(r)))
+        return dict(poisson=r.poisson, gaussian=r.gaussian, euclidian=r.euclidian,
+                    nb_photons_calc=r.nb_photons_calc)
+
+    def llk_noalloc(self):
+        """The same LLK computed without allocating device memory (ptyrs_engine_llk_noalloc:
+        the work buffers live in one static 64 MiB array per GPU; calls overlapping on one
+        GPU race)."""
+        r = _RecordC()
+        _check(library().ptyrs_engine_llk_noalloc(self._h, ctypes.byref(r)))
         return dict(poisson=r.poisson, gaussian=r.gaussian, euclidian=r.euclidian,
                     nb_photons_calc=r.nb_photons_calc)
 

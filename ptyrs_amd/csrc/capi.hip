@@ -215,6 +215,36 @@ int host_call(F&& f) {
 
 }  // namespace
 
+// Scratch of ptyrs_engine_llk_noalloc: one array per GPU, held from the library's load on that
+// GPU until the process ends. Its device address is taken once per device with that device
+// current (hipGetSymbolAddress) and only that address is passed to kernels: on the host the
+// array's name is a stand-in, not a device address.
+constexpr size_t kLlkScratchBytes = size_t(64) << 20;
+__device__ __align__(256) static unsigned char scratch[kLlkScratchBytes];
+
+namespace {
+
+void* scratch_on(int device) {
+  static std::mutex m;
+  static std::vector<void*> ptrs;
+  std::lock_guard<std::mutex> g(m);
+  if (device < 0) throw Invalid("bad device");
+  if (size_t(device) < ptrs.size() && ptrs[device]) return ptrs[device];
+  int prev = 0;
+  PTYRS_HIP(hipGetDevice(&prev));
+  PTYRS_HIP(hipSetDevice(device));
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  (void)hipSetDevice(prev);
+  if (e != hipSuccess || p == nullptr)
+    throw DeviceError(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  if (ptrs.size() <= size_t(device)) ptrs.resize(device + 1, nullptr);
+  ptrs[device] = p;
+  return p;
+}
+
+}  // namespace
+
 extern "C" {
 
 int ptyrs_engine_create(const ptyrs_scan* scan, const ptyrs_optics* optics, const double* obj,
@@ -552,6 +582,21 @@ int ptyrs_engine_llk(ptyrs_engine* h, ptyrs_llk_record* out) {
     if (!out) throw Invalid("null output");
     double v[4];
     e.llk(v);
+    out->cycle = e.cycle;
+    out->algorithm = -1;
+    out->poisson = v[0];
+    out->gaussian = v[1];
+    out->euclidian = v[2];
+    out->nb_photons_calc = v[3];
+  });
+}
+
+int ptyrs_engine_llk_noalloc(ptyrs_engine* h, ptyrs_llk_record* out) {
+  return guarded(h, true, [&](EngineBase& e) {
+    if (!out) throw Invalid("null output");
+    void* s = scratch_on(e.device());
+    double v[4];
+    e.llk_noalloc(s, kLlkScratchBytes, v);
     out->cycle = e.cycle;
     out->algorithm = -1;
     out->poisson = v[0];

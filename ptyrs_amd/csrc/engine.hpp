@@ -163,6 +163,11 @@ class EngineBase {
   virtual void project(const ProjParams& p, size_t ncycle) = 0;
   virtual void ml(const ptyrs_ml_params& p, size_t ncycle) = 0;
   virtual void llk(double out[4]) = 0;
+  // The LLK with its exit-wave stack, per-workgroup sums and result in `scratch` (`bytes`
+  // long, device memory the caller owns) instead of the engine's workspace: no allocation.
+  virtual void llk_noalloc(void* scratch, size_t bytes, double out[4]) = 0;
+  // Bytes of scratch llk_noalloc needs for a stack of `frames` frames.
+  virtual size_t llk_scratch_bytes(size_t frames) const = 0;
   virtual void grad(int cost, double* value, double* obj_grad, double* probe_grad) = 0;
   virtual void intensities(double* out) = 0;
   virtual void get_object(double* out) = 0;
@@ -200,6 +205,7 @@ class EngineBase {
       for (auto& s : stats_) s = {0, 0.0};
     }
   }
+  int device() const { return device_; }
   void kernel_stats(int kind, size_t* launches, double* ms) {
     flush_timing();
     *launches = stats_[kind].first;
@@ -1736,6 +1742,48 @@ class Engine : public EngineBase {
     return (nb + nchunk - 1) / nchunk;
   }
 
+  // Scratch layout of llk_noalloc: the per-workgroup sums (4 doubles per frame and pass-B
+  // workgroup), the 4 result sums, then the stack, each part 256-byte aligned.
+  static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+  size_t llk_part_bytes() const { return align256(sizeof(double) * 4 * nframe_ * SB::NBLK); }
+  size_t llk_scratch_bytes(size_t frames) const override {
+    return llk_part_bytes() + 256 + sizeof(cplx<T>) * frames * M_ * n;
+  }
+
+  void llk_noalloc(void* scratch, size_t bytes, double out[4]) override {
+    if (nslice_ > 1 || pos_on_)
+      throw Invalid("the allocation-free LLK runs one slice without position offsets");
+    if (profiling_)
+      throw Invalid("the allocation-free LLK does not run with profiling on (it creates events)");
+    const size_t head = llk_part_bytes() + 256;
+    const size_t per_frame = sizeof(cplx<T>) * M_ * n;
+    if (bytes < head + per_frame)
+      throw Invalid("the allocation-free LLK needs " + std::to_string(llk_scratch_bytes(1)) +
+                    " bytes of scratch for this scan (" + std::to_string(nframe_) +
+                    " frames); it has " + std::to_string(bytes));
+    // Stacks as long as the scratch holds, at most 65535 views per launch.
+    const size_t frames = std::min<size_t>({(bytes - head) / per_frame,
+                                            std::max<size_t>(nframe_, 1),
+                                            size_t(65535) / size_t(M_)});
+    char* base = static_cast<char*>(scratch);
+    double* result = reinterpret_cast<double*>(base + llk_part_bytes());
+    struct Restore {
+      Engine* e;
+      ~Restore() {
+        e->stack_override_ = nullptr;
+        e->part_override_ = nullptr;
+      }
+    } restore{this};
+    part_override_ = reinterpret_cast<double*>(base);
+    stack_override_ = reinterpret_cast<cplx<T>*>(base + head);
+    probe_terms();
+    llk_pass_to(result, frames);
+    PTYRS_HIP(hipMemcpyAsync(out, result, sizeof(double) * 4, hipMemcpyDeviceToHost, stream_));
+    PTYRS_HIP(hipStreamSynchronize(stream_));
+    // Llk::normalised (ptyrs/src/engine/ops.rs:268-276): nb_photons_calc stays a total.
+    for (int i = 0; i < 3; ++i) out[i] /= double(nb_obs_);
+  }
+
   void ensure_workspace(size_t cap, size_t nepoch) {
     cap = std::max<size_t>(cap, 1);
     // A stack's views fill one launch's grid y (the reference's limit, GpuOptions::stack_size).
@@ -1783,7 +1831,7 @@ class Engine : public EngineBase {
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3(S::NBLK, nb * M_), dim3(S::NT), 0, stream_,
                          O_.as<cplx<T>>(), int(nxo_), Wt_.as<cplx<T>>(), corner_.as<int2>(), ord,
-                         M_, psi_.as<cplx<T>>(), omax ? omax_.as<T>() : nullptr,
+                         M_, stack(), omax ? omax_.as<T>() : nullptr,
                          tw_.as<cplx<T>>(), mix);
     };
     run(K_PASS_A, [&] {
@@ -1806,15 +1854,15 @@ class Engine : public EngineBase {
     constexpr bool back = MODE != B_LLK && MODE != B_ICALC;
     constexpr bool fwd = MODE != B_POISSON;
     run(MODE == B_LLK ? K_LLK : K_PASS_B, [&] {
-      if (angular_ && fwd) pass_t<false>(psi_.as<cplx<T>>(), nb * M_);
+      if (angular_ && fwd) pass_t<false>(stack(), nb * M_);
       const T norm = T(1) / (angular_ ? T(n) * T(n) : T(n));
       auto args = [&](auto mm) {
         constexpr int MM = decltype(mm)::value;
         auto col = k_pass_b<T, N, MM, MODE, false>;
         auto row = k_pass_b<T, N, MM, MODE, true>;
         hipLaunchKernelGGL(angular_ ? row : col, dim3(SB::NBLK, nb),
-                           dim3(pass_b_threads<T, N, MM>()), 0, stream_, psi_.as<cplx<T>>(),
-                           counts_.as<float>(), ord, llk_part_.as<double>(), tw_.as<cplx<T>>(),
+                           dim3(pass_b_threads<T, N, MM>()), 0, stream_, stack(),
+                           counts_.as<float>(), ord, llk_parts(), tw_.as<cplx<T>>(),
                            ba, norm);
       };
       switch (M_) {
@@ -1824,13 +1872,18 @@ class Engine : public EngineBase {
         case 4: args(std::integral_constant<int, 4>{}); break;
         default: throw Invalid("the GPU engine runs at most 4 modes");
       }
-      if (angular_ && back) pass_t<true>(psi_.as<cplx<T>>(), nb * M_);
+      if (angular_ && back) pass_t<true>(stack(), nb * M_);
     });
   }
 
   // One forward pass of every frame and the four LLK sums into history slot `slot`.
-  void llk_pass(int slot) {
-    const int step = int(std::max<size_t>(cap_, 1));
+  void llk_pass(int slot) { llk_pass_to(hist_.as<double>() + 4 * slot, cap_); }
+
+  // The same, `step_frames` frames per stack, the sums into `result`. Every frame's
+  // per-workgroup sums are kept and reduced in one fixed order, so the stack length changes
+  // no bit.
+  void llk_pass_to(double* result, size_t step_frames) {
+    const int step = int(std::max<size_t>(step_frames, 1));
     for (int f0 = 0; f0 < int(nframe_); f0 += step) {
       const int nb = std::min(step, int(nframe_) - f0);
       const int* ord = iota_.as<int>() + f0;
@@ -1843,11 +1896,17 @@ class Engine : public EngineBase {
       pass_b<B_LLK>(ord, nb);
     }
     run(K_LLK, [&] {
-      hipLaunchKernelGGL(k_llk_reduce, dim3(1), dim3(THREADS), 0, stream_,
-                         llk_part_.as<double>(), int(nframe_ * SB::NBLK),
-                         hist_.as<double>() + 4 * slot);
+      hipLaunchKernelGGL(k_llk_reduce, dim3(1), dim3(THREADS), 0, stream_, llk_parts(),
+                         int(nframe_ * SB::NBLK), result);
     });
   }
+
+  // The exit-wave stack and the LLK's per-workgroup sums: the engine's workspace, or the
+  // caller's scratch while llk_noalloc runs.
+  cplx<T>* stack_override_ = nullptr;
+  double* part_override_ = nullptr;
+  cplx<T>* stack() const { return stack_override_ ? stack_override_ : psi_.as<cplx<T>>(); }
+  double* llk_parts() const { return part_override_ ? part_override_ : llk_part_.as<double>(); }
 
   void drain(std::vector<std::pair<size_t, int>>& due, int algorithm) {
     if (due.empty()) return;
