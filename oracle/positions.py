@@ -86,3 +86,16 @@ def batch_position_step(d, frames, delta):
         mad = np.abs(dx[:, axis] - mean).sum() / n
         limit = min(10.0 * mad, POSITION_LIMIT)
         d[:, axis] += np.clip(dx[:, axis], -limit, limit)
+
+
+def position_step(d, sums):
+    """AP's, DM's and ML's position step from every frame's [num, den] (`sums` [nframe, 2, 2]):
+    per axis 0.3·num/(den + max den), clipped to ±0.1, less its mean over the frames; `d`
+    [nframe, 2] in place. ptyrs/src/engine/features.rs:1026-1047"""
+    for axis in (0, 1):
+        mx = max(0.0, float(sums[:, 1, axis].max()))
+        den = sums[:, 1, axis] + mx
+        with np.errstate(divide="ignore", invalid="ignore"):
+            step = np.where(den > 0.0, np.clip(POSITION_STEP * sums[:, 0, axis] / den,
+                                               -POSITION_LIMIT, POSITION_LIMIT), 0.0)
+        d[:, axis] += step - step.sum() / len(step)

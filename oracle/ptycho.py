@@ -1,5 +1,6 @@
 """PyNX operators and the AP, DM, RAAR, ML and rPIE loops of the reference's CPU engine in
-FP64, without features (no position offsets, scale or background).
+FP64. Of the features only the position offsets are modelled (PATCH sampling: the probe moved
+by each frame's sub-pixel offset in Fourier space), in the LLK and rPIE; no scale or background.
 
 Test infrastructure (see oracle/__init__.py). Layout: exit waves `psi[k]` shifted, `k` in
 table order; objects `[nobj, nyo, nxo]`, probes `[nprobe, ny, nx]`, natural layout.
@@ -10,7 +11,9 @@ import math
 import numpy as np
 
 from .base import AddrTable, observed, pynx_preprocess, random_batches, to_natural, to_shifted
-from .optics import Optics, fft2
+from .optics import Optics, fft2, freq
+from .positions import (EPS_POSITION, batch_position_step, gradient_directions, gradient_sums,
+                        move_planes, position_step, ramp)
 
 AP, DM, RAAR, RPIE, ML, BH, LSQML = "ap", "dm", "raar", "rpie", "ml", "bh", "lsqml"
 
@@ -190,13 +193,39 @@ class Ptycho:
         self.psi = np.zeros((len(self.views_all),) + tuple(frame), dtype=np.complex128)
         self.cycle = 0
         self.history = []
+        self.offsets = None   # Features::position under PATCH: [nframe, 2] object pixels
+        self.refine = False
+
+    def set_position(self, offsets, refine=True):
+        """Features::position (ptyrs/src/engine/features.rs:304-330): per-frame [dy, dx]
+        offsets in object pixels, refined by rPIE's steps unless held; None for none."""
+        self.offsets = None if offsets is None else np.array(offsets, dtype=np.float64)
+        self.refine = bool(refine) and offsets is not None
+
+    def moved_waves(self, views):
+        """Under the offsets, every view's wave W = DFT⁻¹(DFT(P·q)·exp(−2πi f·d)): its probe
+        mode times the plane phase translated by its frame's offset on the plane's periodic
+        grid, natural layout (`Shifted` under PATCH, ptyrs/src/engine/features.rs:545-640)."""
+        shape = self.probe.shape[-2:]
+        spec = np.fft.fft2(self.probe * self.quad)
+        return np.stack([np.fft.ifft2(spec[v[2]] * ramp(shape, self.offsets[v[0]]))
+                         for v in views])
 
     @property
     def quad(self):
         return self.optics.quad
 
     def calc_psi(self):
+        if self.offsets is not None:
+            return self._moved_psi(self.views_all)[1]
         return obj_probe_to_psi(self.addr, self.views_all, self.obj, self.probe, self.quad)
+
+    def _moved_psi(self, views):
+        """The moved waves W and the exit waves fftshift(W·O_win) of `views`."""
+        ny, nx = self.probe.shape[-2:]
+        w = self.moved_waves(views)
+        win = np.stack([self.obj[o, oy:oy + ny, ox:ox + nx] for (_, _, _, o, oy, ox) in views])
+        return w, to_shifted(w * win)
 
     def llk(self):
         """ptyrs/src/engine/ptycho.rs:452-466"""
@@ -213,23 +242,90 @@ class Ptycho:
         return self.optics.exit_adjoint(psi)
 
     def grads(self, obj, probe):
-        """2·∂L/∂O* and 2·∂L/∂P*. ptyrs/src/engine/ptycho.rs:492-513"""
+        """2·∂L/∂O* and 2·∂L/∂P*; under the offsets the back-projections through the moved
+        waves (`Shifted::accumulate_obj` / `accumulate_probe`,
+        ptyrs/src/engine/features.rs:865-958). ptyrs/src/engine/ptycho.rs:492-513"""
         g = self.grad_psi()
+        if self.offsets is not None:
+            return self._moved_grads(g, obj, probe)
         go = (accumulate_obj(self.views_all, self.obj.shape, self.probe, self.quad, g)[0]
               if obj else np.zeros_like(self.obj))
         gp = (accumulate_probe(self.views_all, self.obj, self.probe.shape, self.quad, g)[0]
               if probe else np.zeros_like(self.probe))
         return go, gp
 
+    def _moved_grads(self, g, obj, probe):
+        """Object: Σ g·conj(W) on the windows. Probe: per frame Y = Σ conj(O_win)·g over its
+        views of each mode, moved back by the frame's offset in Fourier space and summed,
+        times conj(q)."""
+        ny, nx = self.probe.shape[-2:]
+        views = self.views_all
+        w = self.moved_waves(views)
+        gn = to_natural(g)
+        go = np.zeros_like(self.obj)
+        gp = np.zeros_like(self.probe)
+        if obj:
+            for k, (_, _, _, o, oy, ox) in enumerate(views):
+                go[o, oy:oy + ny, ox:ox + nx] += gn[k] * np.conj(w[k])
+        if probe:
+            spec = np.zeros_like(self.probe)
+            nmode = self.addr.nmode
+            for f in range(self.addr.nframe):
+                y = np.zeros_like(self.probe)
+                for k in range(f * nmode, (f + 1) * nmode):
+                    _, _, pm, o, oy, ox = views[k]
+                    y[pm] += np.conj(self.obj[o, oy:oy + ny, ox:ox + nx]) * gn[k]
+                spec += np.fft.fft2(y) * ramp((ny, nx), -self.offsets[f])
+            gp = np.conj(self.quad) * np.fft.ifft2(spec)
+        return go, gp
+
     def cg_gamma(self, d_obj, d_probe):
-        """ptyrs/src/engine/ptycho.rs:515-545 (no offsets)"""
+        """ptyrs/src/engine/ptycho.rs:515-545; under the offsets the probe and d_probe terms
+        are their moved waves."""
         v, q = self.views_all, self.quad
-        po = obj_probe_to_psi(self.addr, v, self.obj, self.probe, q)
-        a = (obj_probe_to_psi(self.addr, v, d_obj, self.probe, q)
-             + obj_probe_to_psi(self.addr, v, self.obj, d_probe, q))
-        b = obj_probe_to_psi(self.addr, v, d_obj, d_probe, q)
+        if self.offsets is not None:
+            ny, nx = self.probe.shape[-2:]
+            w = self.moved_waves(v)
+            keep = self.probe
+            self.probe = d_probe
+            dw = self.moved_waves(v)
+            self.probe = keep
+            win = lambda x: np.stack([x[o, oy:oy + ny, ox:ox + nx] for (_, _, _, o, oy, ox) in v])
+            po = to_shifted(w * win(self.obj))
+            a = to_shifted(w * win(d_obj) + dw * win(self.obj))
+            b = to_shifted(dw * win(d_obj))
+        else:
+            po = obj_probe_to_psi(self.addr, v, self.obj, self.probe, q)
+            a = (obj_probe_to_psi(self.addr, v, d_obj, self.probe, q)
+                 + obj_probe_to_psi(self.addr, v, self.obj, d_probe, q))
+            b = obj_probe_to_psi(self.addr, v, d_obj, d_probe, q)
         po, a, b = (self.optics.exit_forward(x) for x in (po, a, b))
         return cg_poisson_gamma(self.iobs, self.addr.nmode, po, a, b)
+
+    def projected_position_sums(self):
+        """Per frame [num, den] of the position step from the magnitude projection of the
+        current estimate's exit waves x: num_a = Re Σ conj(x − W·O)·O·∂_a W, den_a = Σ |O·∂_a W|²
+        over the frame's views, ∂_a W = DFT⁻¹(DFT(P·q)·exp(−2πi f·d)·(−2πi f_a)).
+        ptyrs/src/engine/ptycho.rs:468-479, ptyrs/src/engine/features.rs:960-1025"""
+        ny, nx = self.probe.shape[-2:]
+        views = self.views_all
+        w, psi = self._moved_psi(views)
+        x = self.optics.exit_forward(psi)
+        x = apply_amplitude(self.iobs, self.addr.nmode, x)
+        x = to_natural(self.optics.exit_adjoint(x))
+        spec = np.fft.fft2(self.probe * self.quad)
+        fy = freq(np.arange(ny), ny)[:, None] * np.ones((1, nx))
+        fx = np.ones((ny, 1)) * freq(np.arange(nx), nx)[None, :]
+        sums = np.zeros((self.addr.nframe, 2, 2))
+        for k, (f, _, pm, o, oy, ox) in enumerate(views):
+            win = self.obj[o, oy:oy + ny, ox:ox + nx]
+            r = spec[pm] * ramp((ny, nx), self.offsets[f])
+            c = x[k] - w[k] * win
+            for axis, fa in enumerate((fy, fx)):
+                dpsi = win * np.fft.ifft2(r * (-2j * math.pi * fa))
+                sums[f, 0, axis] += float((np.conj(c) * dpsi).real.sum())
+                sums[f, 1, axis] += float((dpsi.real ** 2 + dpsi.imag ** 2).sum())
+        return sums
 
     def scale_obj_probe(self):
         """PyNX ScaleObjProbe(absolute=True). ptyrs/src/engine/ptycho.rs:571-581"""
@@ -334,6 +430,7 @@ class Ptycho:
         d = [np.zeros_like(self.obj), np.zeros_like(self.probe)]
         for ic in range(ncycle):
             rec = self.llk() if llk_due(calc_llk, ic, ncycle) else None
+            sums = self.projected_position_sums() if self.refine else None
             g = self.grads(update_object, update_probe)
             beta = 0.0
             if last is not None:
@@ -350,6 +447,8 @@ class Ptycho:
             else:
                 d = [-gi for gi in g]
             last = g
+            if sums is not None:
+                position_step(self.offsets, sums)
             if rec is not None:
                 self._record(ML, rec)
             self.cycle += 1
@@ -376,7 +475,10 @@ class Ptycho:
         ptyrs/src/engine/ptycho.rs:900-945"""
         nmode = self.addr.nmode
         views = [v for f in frames for v in self.views_all[f * nmode:(f + 1) * nmode]]
-        psi = obj_probe_to_psi(self.addr, views, self.obj, self.probe, self.quad)
+        if self.offsets is not None:
+            psi = self._moved_psi(views)[1]
+        else:
+            psi = obj_probe_to_psi(self.addr, views, self.obj, self.probe, self.quad)
         delta = self.optics.exit_forward(psi)
         io = self.iobs[frames]
         calc = np.sqrt(_mode_sum(delta, nmode))
@@ -389,6 +491,8 @@ class Ptycho:
 
     def rpie_batch(self, frames, obj_step, probe_step, obj_alpha, probe_alpha):
         """One rPIE batch from the batch-start arrays. ptyrs/src/engine/ptycho.rs:900-1086, :1089-1130"""
+        if self.offsets is not None:
+            return self._rpie_batch_moved(frames, obj_step, probe_step, obj_alpha, probe_alpha)
         ny, nx = self.probe.shape[-2:]
         nmode = self.addr.nmode
         nprobe = self.probe.shape[0]
@@ -427,6 +531,67 @@ class Ptycho:
             self.obj = self.obj + d_obj * obj_step
         if probe_step is not None:
             self.probe = self.probe + d_probe * (probe_step / nbatch)
+
+
+    def _rpie_batch_moved(self, frames, obj_step, probe_step, obj_alpha, probe_alpha):
+        """One rPIE batch under the offsets (PATCH), from the batch-start object, probe and
+        offsets: the views' waves W are the moved probe terms; the object step divides
+        conj(W)·Δ by the batch overlap of |W|² per probe mode; each frame's probe direction
+        conj(q·O_win)·Δ/((1 − α)|O_win|² + α·max) is moved back by the frame's offset before it
+        joins the batch mean; with `refine` the frames' position directions (gradient sums of
+        the window's Fourier derivative against mode 0's W and Δ) take the clipped SGD step.
+        ptyrs/src/engine/ptycho.rs:900-1086"""
+        ny, nx = self.probe.shape[-2:]
+        nmode = self.addr.nmode
+        nprobe = self.probe.shape[0]
+        nbatch = len(frames)
+        views, psi, delta = self.rpie_delta(frames)
+        w, _ = self._moved_psi(views)
+        delta_nat = to_natural(delta)
+        d_obj = None
+        if obj_step is not None:
+            nyo, nxo = self.obj.shape[-2:]
+            w2 = w.real ** 2 + w.imag ** 2
+            ov = np.zeros((nprobe, nyo, nxo))
+            for k, (_, _, p, _, oy, ox) in enumerate(views):
+                ov[p, oy:oy + ny, ox:ox + nx] += w2[k]
+            dens = np.stack([ov[p, oy:oy + ny, ox:ox + nx] for (_, _, p, _, oy, ox) in views])
+            pmax = float(dens.reshape(nbatch, nmode, ny, nx).sum(axis=1).max())
+            a = obj_alpha
+            d_obj = np.zeros_like(self.obj)
+            for k, (_, _, p, _, oy, ox) in enumerate(views):
+                d_obj[0, oy:oy + ny, ox:ox + nx] += (np.conj(w[k]) / ((1.0 - a) * dens[k] + a * pmax)
+                                                     * delta_nat[k])
+        d_probe = None
+        if probe_step is not None:
+            a = probe_alpha
+            d_probe = np.zeros_like(self.probe)
+            for b in range(nbatch):
+                fv = views[b * nmode:(b + 1) * nmode]
+                oy, ox = fv[0][4], fv[0][5]
+                win = self.obj[0, oy:oy + ny, ox:ox + nx]
+                w2 = win.real ** 2 + win.imag ** 2
+                wgt = np.conj(win * self.quad) / ((1.0 - a) * w2 + a * float(w2.max()))
+                frame = np.zeros_like(self.probe)
+                for m, v in enumerate(fv):
+                    frame[v[2]] += wgt * delta_nat[b * nmode + m]
+                d_probe += move_planes(frame, self.quad, -self.offsets[fv[0][0]])
+        d_pos = None
+        if self.refine:
+            sums = []
+            for b in range(nbatch):
+                k = b * nmode
+                _, _, _, o, oy, ox = views[k]
+                sums.append(gradient_sums(self.obj[o, oy:oy + ny, ox:ox + nx], w[k],
+                                          delta_nat[k]))
+            d_pos = gradient_directions(sums, EPS_POSITION
+                                        * self.optics.propagator.intensity_scale())
+        if obj_step is not None:
+            self.obj = self.obj + d_obj * obj_step
+        if probe_step is not None:
+            self.probe = self.probe + d_probe * (probe_step / nbatch)
+        if d_pos is not None:
+            batch_position_step(self.offsets, frames, d_pos)
 
 
 def far_field_ptycho(counts, y, x, obj, probe, wavelength, distance, detector_pixel):

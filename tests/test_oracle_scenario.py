@@ -180,3 +180,61 @@ def test_lsqml_epochs_match_ptychi(cost, name):
     assert nxt == len(rows)
     _changes(obj, probe, p.obj, p.probe, f"lsqml_epochs_{name}_obj_f64.txt",
              f"lsqml_epochs_{name}_probe_f64.txt")
+
+
+# rPIE with refined position offsets under PATCH (ptyrs/tests/engine_rpie_pos.rs:58-160):
+# dm_scenario(32, 40, 2, 2e5, 4.0) after scale_obj_probe, from the reference's start offsets.
+RPIE_POS = {
+    "step": dict(obj_step=0.9, probe_step=0.8, obj_alpha=0.3, probe_alpha=0.2, batch_size=20,
+                 calc_llk=1, seed=7, nepoch=1),
+    "epochs": dict(obj_step=1.0, probe_step=1.0, obj_alpha=0.5, probe_alpha=0.5, batch_size=4,
+                   calc_llk=1, seed=7, nepoch=3),
+}
+
+
+@pytest.mark.parametrize("case", ["step", "epochs"])
+def test_rpie_positions_match_ptychi(case):
+    kw = dict(RPIE_POS[case])
+    nepoch = kw.pop("nepoch")
+    p = fresh(2)
+    rows = np.array(table(f"rpie_pos_{case}_offsets_f64.txt"))
+    start_off = rows[:, 1:3]
+    p.set_position(start_off, refine=True)
+    o0, p0 = p.obj.copy(), p.probe.copy()
+    p.rpie(nepoch, **kw)
+    _changes(p.obj, p.probe, o0, p0, f"rpie_pos_{case}_obj_f64.txt",
+             f"rpie_pos_{case}_probe_f64.txt")
+    err = max_rel((p.offsets - start_off).ravel(), (rows[:, 3:5] - start_off).ravel())
+    assert err < 3e-12, err
+
+
+def test_rpie_positions_held_and_zero_are_plain_rpie():
+    # Zero offsets held: the moved waves are P·q up to FFT rounding, so the run is plain rPIE.
+    a, b = fresh(2), fresh(2)
+    b.set_position(np.zeros((len(b.iobs), 2)), refine=False)
+    a.rpie(2, **RPIE_PTYCHI)
+    b.rpie(2, **RPIE_PTYCHI)
+    for x, y in ((b.obj, a.obj), (b.probe, a.probe)):
+        assert np.abs(x - y).max() < 1e-12 * np.abs(y).max()
+    assert np.all(b.offsets == 0.0)
+
+
+def test_ml_positions_match_the_model():
+    # ML with refined offsets, four cycles of object and probe (ptyrs/tests/engine_ml_pos.rs:
+    # 51-117): changes against max|start| at 1e-13, offset changes at 1e-12 of their max.
+    p = fresh(2)
+    rows = np.array(table("ml_pos_offsets_f64.txt"))
+    off0 = rows[:, 1:3]
+    p.set_position(off0, refine=True)
+    o0, p0 = p.obj.copy(), p.probe.copy()
+    p.ml(4, update_object=True, update_probe=True, calc_llk=1, zero_phase_ramp=False)
+    for x, x0, file, lead in ((p.obj, o0, "ml_pos_obj_f64.txt", 2),
+                              (p.probe, p0, "ml_pos_probe_f64.txt", 3)):
+        r = np.array(table(file))
+        got = np.stack([(x - x0).real.ravel(), (x - x0).imag.ravel()], axis=1).ravel()
+        scale = max(np.abs(x0.real).max(), np.abs(x0.imag).max())
+        err = np.abs(got - r[:, lead:lead + 2].ravel()).max() / scale
+        assert err < 1e-13, (file, err)
+    want = (rows[:, 3:5] - off0).ravel()
+    err = np.abs((p.offsets - off0).ravel() - want).max() / np.abs(want).max()
+    assert err < 1e-12, err
