@@ -25,14 +25,16 @@ What pins each module (the test files named run on the CPU):
   `apply_amplitude` and `llk`, adjoint dot tests of the back-projections, the consistent-pair
   invariant); tests/test_oracle_rpie.py (`rpie_near_*`, 9 runs × 30 epochs at 1e-12);
   tests/test_oracle_pynx.py (AP, DM, RAAR, ML against `pynx_near_*`);
-  tests/test_oracle_scenario.py (`dm_pynx`, `raar_pynx`, `rpie_ptychi`, `ml_grad`).
+  tests/test_oracle_scenario.py (`dm_pynx`, `raar_pynx`, `rpie_ptychi`, `ml_grad`; rPIE and
+  ML under refined PATCH offsets: `rpie_pos_*` at 1e-12, `ml_pos_*` at 1e-13).
 - lsqml.py — tests/test_oracle_lsqml.py (`lsqml_near_*`), tests/test_oracle_scenario.py
   (`lsqml_{gaussian,poisson}_*`, `lsqml_epochs_*`), tests/test_oracle_gradient.py
   (`cost_grads` against PyNX's ML gradient and five-point central differences).
 - multislice.py — tests/test_oracle_multislice.py (`lsqml_multislice_*`: forward
   intensities, one step, 30 epochs; directions and tangent against central differences).
 - positions.py — tests/test_oracle_positions.py (`lsqml_near_pos_*`,
-  `lsqml_multislice_pos_*`; closed forms: a whole-pixel move is a roll, the MAD clip by hand).
+  `lsqml_multislice_pos_*`; closed forms: a whole-pixel move is a roll, the MAD clip by hand);
+  `position_step` through the `ml_pos_*` run.
 - scenario.py — tests/test_oracle_scenario.py: the ChaCha block function against RFC 8439's
   A.1 keystream (20 rounds) and the reference's seeded scenario, whose Poisson LLK sum
   equals `ml_grad_values_f64.txt`'s at 9e-16 (every count is the reference's); Poisson
