@@ -442,7 +442,12 @@ class Engine : public EngineBase {
   }
 
   void rpie_check(const ptyrs_rpie_params& p) override {
-    one_unitary("rPIE");
+    if (nslice_ > 1)
+      throw Invalid("rPIE runs one object slice; only LSQML and the LLK pass run several");
+    if (pos_on_ && sampling != PTYRS_PATCH)
+      throw Invalid("rPIE with position offsets under FullObject sampling is not run on this "
+                    "GPU engine; set the PATCH sampling");
+    unitary("rPIE");
     if (nobj_ != 1) throw Invalid("rPIE needs exactly one object mode");
     if (p.batch_size == 0) throw Invalid("rPIE batch_size must be >= 1");
     if (M_ > 4) throw Invalid("the GPU engine runs at most 4 modes");
@@ -463,6 +468,11 @@ class Engine : public EngineBase {
     }
     const size_t cap = std::min(p.batch_size, nframe_);
     ensure_workspace(cap, nepoch);
+    if (pos_on_) {
+      // Under offsets pass C leaves one probe direction per frame, moved back and averaged.
+      dP_.ensure(sizeof(cplx<T>) * cap_ * M_ * n);
+      dPm_.ensure(sizeof(cplx<T>) * M_ * n);
+    }
     std::vector<int> order(nepoch * nframe_);
     run_.spans.assign(nepoch, {});
     for (size_t ic = 0; ic < nepoch; ++ic) {
@@ -485,7 +495,8 @@ class Engine : public EngineBase {
     // Every epoch has the same batch lengths; its order is copied into the captured order
     // buffer ahead of the replay.
     run_.base = {uint64_t(p.has_obj_step), uint64_t(p.has_probe_step), bits(p.obj_step),
-                 bits(p.probe_step), bits(p.obj_alpha), bits(p.probe_alpha), cap_};
+                 bits(p.probe_step), bits(p.obj_alpha), bits(p.probe_alpha), cap_,
+                 uint64_t(pos_on_), uint64_t(pos_refine_)};
   }
 
   bool rpie_epoch() override {
@@ -630,7 +641,10 @@ class Engine : public EngineBase {
       throw Invalid("loop_obj_probe must be >= 1");
     if (ncycle == 0 || nframe_ == 0) {
       cycle += ncycle;
-      if (p.zero_phase_ramp && ncycle > 0) host_zero_phase_ramp();
+      if (p.zero_phase_ramp && ncycle > 0) {
+        device_zero_phase_ramp();
+        synchronize();
+      }
       return;
     }
     const size_t stack = std::min<size_t>(nframe_, stack_size_ ? stack_size_ : kDefaultStack);
@@ -717,9 +731,9 @@ class Engine : public EngineBase {
       ++cycle;
     }
     drain(due, p.algorithm);
+    if (p.zero_phase_ramp) device_zero_phase_ramp();
     synchronize();
     flush_timing();
-    if (p.zero_phase_ramp) host_zero_phase_ramp();
   }
 
   // PyNX ML, Poisson conjugate gradient (ptyrs/src/engine/ptycho.rs:757-842): scale_obj_probe,
@@ -771,7 +785,8 @@ class Engine : public EngineBase {
                            corner_.as<int2>(), iota_.as<int>(), int(nframe_), N, ntx_,
                            int(nframe_), list_all_.as<int>(), count_all_.as<int>(),
                            (const T*)nullptr, M_, int(nyo_), int(nxo_), (T*)nullptr,
-                           tilemax_.as<T>());
+                           tilemax_.as<T>(),
+                           (const cplx<T>*)nullptr);
       });
     }
     const int grid_o = int(std::min<size_t>(kAbsBlocks, (no + THREADS - 1) / THREADS));
@@ -899,9 +914,9 @@ class Engine : public EngineBase {
       ++cycle;
     }
     drain(due, PTYRS_ML);
+    if (p.zero_phase_ramp) device_zero_phase_ramp();
     synchronize();
     flush_timing();
-    if (p.zero_phase_ramp) host_zero_phase_ramp();
   }
 
  private:
@@ -966,7 +981,9 @@ class Engine : public EngineBase {
   // The probe planes of a stack's views moved by their frames' offsets as pty-chi PATCH moves
   // them, conj(q0)·DFT⁻¹(DFT(q0·P_m)·exp(−2πi f·d_b)), q0 the first slice's plane phase
   // (ptyrs/src/engine/features.rs:386-429, lsqml.rs:483-536), into inc0_.
-  void moved_probes(const int* ord, int nb) {
+  // With conj_q false (one slice) the planes keep q: W = DFT⁻¹(DFT(P·q)·exp(−2πi f·d_b)),
+  // the views' waves.
+  void moved_probes(const int* ord, int nb, bool conj_q = true) {
     const int K = nb * M_;
     const size_t pn = size_t(M_) * n;
     run(K_PROBE, [&] {
@@ -983,7 +1000,7 @@ class Engine : public EngineBase {
                          inc0_.as<cplx<T>>(), tw_.as<cplx<T>>());
       hipLaunchKernelGGL((k_rows<T, N, true>), dim3(S::NBLK, K), dim3(S::NT), 0, stream_,
                          inc0_.as<cplx<T>>(), tw_.as<cplx<T>>());
-      if (nslice_ == 1)
+      if (nslice_ == 1 && conj_q)
         hipLaunchKernelGGL((k_times_q<T, true>), dim3((n + 255) / 256, K), dim3(256), 0, stream_,
                            inc0_.as<cplx<T>>(), (const cplx<T>*)q_.as<cplx<T>>(), n);
     });
@@ -1232,7 +1249,8 @@ class Engine : public EngineBase {
         hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
                            corner_.as<int2>(), ord, nb, N, ntx_, step, list_all_.as<int>(),
                            count_all_.as<int>(), (const T*)nullptr, M_, int(nyo_), int(nxo_),
-                           (T*)nullptr, tilemax_.as<T>());
+                           (T*)nullptr, tilemax_.as<T>(),
+                           (const cplx<T>*)nullptr);
       });
       run(K_OTHER, [&] {
         hipLaunchKernelGGL(k_illumination<T>, dim3(nt), dim3(THREADS), 0, stream_,
@@ -1258,7 +1276,8 @@ class Engine : public EngineBase {
       hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
                          corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
                          lcount_.as<int>(), (const T*)nullptr, M_, int(nyo_), int(nxo_),
-                         (T*)nullptr, tilemax_.as<T>());
+                         (T*)nullptr, tilemax_.as<T>(),
+                           (const cplx<T>*)nullptr);
     });
     pass_a(ord, nb, false);
     BArgs<T> ba{afar_tmp_.as<T>(), prev_.as<T>(), xi_part_.as<double>(), icalc_.as<T>()};
@@ -1370,7 +1389,8 @@ class Engine : public EngineBase {
         hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
                            corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
                            lcount_.as<int>(), (const T*)nullptr, M_, int(nyo_), int(nxo_),
-                           (T*)nullptr, tilemax_.as<T>());
+                           (T*)nullptr, tilemax_.as<T>(),
+                           (const cplx<T>*)nullptr);
       });
       ms_forward(ord, nb, wv);
       run(K_PASS_A, [&] { rows(false, K); });
@@ -1466,7 +1486,8 @@ class Engine : public EngineBase {
       hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
                          corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
                          lcount_.as<int>(), (const T*)nullptr, M_, int(nyo_), int(nxo_),
-                         (T*)nullptr, tilemax_.as<T>());
+                         (T*)nullptr, tilemax_.as<T>(),
+                           (const cplx<T>*)nullptr);
     });
     const Waves wv = stack_waves(ord, nb);
     ms_forward(ord, nb, wv);
@@ -1603,7 +1624,8 @@ class Engine : public EngineBase {
   // The position sums of the batch's frames on slice s (ptyrs/src/engine/lsqml.rs:230-270):
   // the windows' Fourier derivatives against the wave entering the slice times its phase and χ_s.
   void position_sums(size_t s, const cplx<T>* w, const cplx<T>* q, int q_frame, const int* ord,
-                     int nb) {
+                     int nb, const cplx<T>* chi = nullptr) {
+    if (chi == nullptr) chi = psi_.as<cplx<T>>();
     cplx<T>* spec = pgrad_.as<cplx<T>>();
     cplx<T>* grad = spec + cap_ * n;
     run(K_OTHER, [&] {
@@ -1620,8 +1642,8 @@ class Engine : public EngineBase {
       hipLaunchKernelGGL((k_rows<T, N, true>), dim3(S::NBLK, 2 * nb), dim3(S::NT), 0, stream_,
                          grad, tw_.as<cplx<T>>());
       hipLaunchKernelGGL(k_pos_sums<T>, dim3(aux_blocks(), nb), dim3(THREADS), 0, stream_,
-                         (const cplx<T>*)grad, w, q, q_frame, (const cplx<T>*)psi_.as<cplx<T>>(),
-                         N, M_, pos_part_.as<double>());
+                         (const cplx<T>*)grad, w, q, q_frame, chi, N, M_,
+                         pos_part_.as<double>());
     });
   }
 
@@ -1654,7 +1676,8 @@ class Engine : public EngineBase {
                          corner_.as<int2>(), iota_.as<int>(), int(nframe_), N, ntx_,
                          int(nframe_), list_all_.as<int>(), count_all_.as<int>(),
                          (const T*)W2_.as<T>(), M_, int(nyo_), int(nxo_), D_.as<T>(),
-                         tilemax_.as<T>());
+                         tilemax_.as<T>(),
+                           (const cplx<T>*)nullptr);
     });
     run(K_OVERLAP, [&] {
       hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(MAX_THREADS), 0, stream_,
@@ -1718,13 +1741,36 @@ class Engine : public EngineBase {
     });
   }
 
-  void host_zero_phase_ramp() {
-    std::vector<double> o(object_len()), pr(probe_len());
-    get_object(o.data());
-    get_probe(pr.data());
-    zero_phase_ramp(o.data(), nobj_, nyo_, nxo_, pr.data(), nprobe_, N, N);
-    set_object(o.data());
-    set_probe(pr.data());
+  // zero_phase_ramp (ptyrs/src/engine/ptycho.rs:593-603, 1134-1181) on the device: the
+  // probe's spectrum (row and column DFTs of a copy), its centre of mass in FP64, then opposite
+  // linear phase ramps on the probe and the object, enqueued on the engine's stream.
+  DevBuf zspec_, zpart_, zd_;
+  static constexpr int kRampBlocks = 64;
+  void device_zero_phase_ramp() {
+    const size_t np = nprobe_ * n;
+    zspec_.ensure(sizeof(cplx<T>) * np);
+    zpart_.ensure(sizeof(double) * 3 * kRampBlocks);
+    zd_.ensure(sizeof(double) * 2);
+    run(K_OTHER, [&] {
+      PTYRS_HIP(hipMemcpyAsync(zspec_.p, P_.p, sizeof(cplx<T>) * np, hipMemcpyDeviceToDevice,
+                               stream_));
+      hipLaunchKernelGGL((k_rows<T, N, false>), dim3(S::NBLK, nprobe_), dim3(S::NT), 0, stream_,
+                         zspec_.as<cplx<T>>(), tw_.as<cplx<T>>());
+      hipLaunchKernelGGL((k_cols<T, N, false>), dim3(SC::NBLK, nprobe_), dim3(SC::NT), 0,
+                         stream_, zspec_.as<cplx<T>>(), tw_.as<cplx<T>>());
+      hipLaunchKernelGGL(k_spectrum_moments<T>, dim3(kRampBlocks), dim3(THREADS), 0, stream_,
+                         (const cplx<T>*)zspec_.as<cplx<T>>(), N, int(nprobe_),
+                         zpart_.as<double>());
+      hipLaunchKernelGGL(k_ramp_offset, dim3(1), dim3(THREADS), 0, stream_,
+                         (const double*)zpart_.as<double>(), kRampBlocks, N, zd_.as<double>());
+      hipLaunchKernelGGL(k_apply_ramp<T>, dim3(int((np + 255) / 256)), dim3(256), 0, stream_,
+                         P_.as<cplx<T>>(), int(nprobe_), N, N, (const double*)zd_.as<double>(),
+                         -1.0, N);
+      const size_t no = nobj_ * nyo_ * nxo_;
+      hipLaunchKernelGGL(k_apply_ramp<T>, dim3(int(std::min<size_t>((no + 255) / 256, 8192))),
+                         dim3(256), 0, stream_, O_.as<cplx<T>>(), int(nobj_), int(nyo_),
+                         int(nxo_), (const double*)zd_.as<double>(), 1.0, N);
+    });
   }
 
   int ntiles() const { return ntx_ * nty_; }
@@ -1826,13 +1872,16 @@ class Engine : public EngineBase {
     });
   }
 
+  // Pass A; `waves` (nullable) gives every view its own wave (plane view of it) instead of
+  // the probe terms Wt.
   void pass_a(const int* ord, int nb, bool omax,
-              MixArgs<T> mix = MixArgs<T>{MIX_NONE, nullptr, nullptr, T(0), T(0)}) {
+              MixArgs<T> mix = MixArgs<T>{MIX_NONE, nullptr, nullptr, T(0), T(0)},
+              const cplx<T>* waves = nullptr) {
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3(S::NBLK, nb * M_), dim3(S::NT), 0, stream_,
-                         O_.as<cplx<T>>(), int(nxo_), Wt_.as<cplx<T>>(), corner_.as<int2>(), ord,
-                         M_, stack(), omax ? omax_.as<T>() : nullptr,
-                         tw_.as<cplx<T>>(), mix);
+                         O_.as<cplx<T>>(), int(nxo_), waves ? waves : Wt_.as<cplx<T>>(),
+                         corner_.as<int2>(), ord, M_, stack(), omax ? omax_.as<T>() : nullptr,
+                         tw_.as<cplx<T>>(), mix, int(waves != nullptr));
     };
     run(K_PASS_A, [&] {
       switch (mix.kind) {
@@ -1924,6 +1973,10 @@ class Engine : public EngineBase {
 
   // One rPIE batch (ptyrs/src/engine/ptycho.rs:900-1086) from the batch-start object and probe.
   void rpie_batch(const ptyrs_rpie_params& p, const int* ord, int nb) {
+    if (pos_on_) {
+      rpie_batch_moved(p, ord, nb);
+      return;
+    }
     const bool upd_obj = p.has_obj_step != 0, upd_probe = p.has_probe_step != 0;
     const int nt = ntiles();
     if (upd_obj) {
@@ -1933,7 +1986,8 @@ class Engine : public EngineBase {
         hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
                            corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
                            lcount_.as<int>(), (const T*)W2_.as<T>(), M_, int(nyo_), int(nxo_),
-                           D_.as<T>(), tilemax_.as<T>());
+                           D_.as<T>(), tilemax_.as<T>(),
+                           (const cplx<T>*)nullptr);
       });
       run(K_OVERLAP, [&] {
         hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(MAX_THREADS), 0, stream_,
@@ -1954,7 +2008,7 @@ class Engine : public EngineBase {
                            omax_.as<T>(), int(S::NBLK), chunk, T(p.probe_alpha),
                            int(upd_probe), dP_.as<cplx<T>>(), tw_.as<cplx<T>>(), int(upd_obj),
                            (const T*)D_.as<T>(), int(nyo_), (const T*)pmax_.as<T>(),
-                           T(p.obj_alpha));
+                           T(p.obj_alpha), 0, (cplx<T>*)nullptr);
       };
       switch (M_) {
         case 1: launch(std::integral_constant<int, 1>{}); break;
@@ -1990,6 +2044,99 @@ class Engine : public EngineBase {
         hipLaunchKernelGGL(k_probe_apply<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,
                            P_.as<cplx<T>>(), Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
                            dP_.as<cplx<T>>(), nchunk, n, total, T(p.probe_step / nb));
+      });
+    }
+  }
+
+  // One rPIE batch under position offsets (PATCH; ptyrs/src/engine/ptycho.rs:900-1086 with
+  // Features::position, the reference GPU's rpie under its position model,
+  // ptyrs/src/engine/gpu/rpie.rs:69-133): every view's wave W = DFT⁻¹(DFT(P·q)·exp(−2πi f·d))
+  // (its frame's offset d) takes the place of the probe terms in the overlaps, pass A and pass
+  // C; pass C runs one slot per chunk, so its probe-direction partials are the frames' own,
+  // which are moved back by their offsets in Fourier space (conj(q)·DFT⁻¹(DFT(q·g)·exp(+2πi
+  // f·d))) and averaged; with refinement, mode 0's W and Δ give the position sums on the
+  // batch-start object, then the MAD-clipped SGD step (after the probe's move back, which
+  // reads the batch-start offsets).
+  void rpie_batch_moved(const ptyrs_rpie_params& p, const int* ord, int nb) {
+    const bool upd_obj = p.has_obj_step != 0, upd_probe = p.has_probe_step != 0;
+    const int nt = ntiles();
+    const int K = nb * M_;
+    moved_probes(ord, nb, false);
+    const cplx<T>* W = inc0_.as<cplx<T>>();
+    if (upd_obj) {
+      run(K_BIN, [&] {
+        hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                           corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
+                           lcount_.as<int>(), (const T*)nullptr, M_, int(nyo_), int(nxo_),
+                           D_.as<T>(), tilemax_.as<T>(), W);
+      });
+      run(K_OVERLAP, [&] {
+        hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(MAX_THREADS), 0, stream_,
+                           tilemax_.as<T>(), nt, pmax_.as<T>());
+      });
+    }
+    pass_a(ord, nb, upd_probe, MixArgs<T>{MIX_NONE, nullptr, nullptr, T(0), T(0)}, W);
+    pass_b<B_RPIE>(ord, nb);
+    cplx<T>* chi0 = pos_refine_ ? gspec_.as<cplx<T>>() : nullptr;
+    run(K_PASS_C, [&] {
+      auto launch = [&](auto mm) {
+        constexpr int MM = decltype(mm)::value;
+        constexpr int G = rows_per_group(N, SR::L, MM);
+        hipLaunchKernelGGL((k_pass_c_rpie_sum<T, N, MM>), dim3(N / G, nb),
+                           dim3(pass_c_sum_threads<T, N, MM>()), 0, stream_, psi_.as<cplx<T>>(),
+                           O_.as<cplx<T>>(), int(nxo_), W, q_.as<cplx<T>>(), corner_.as<int2>(),
+                           ord, nb, omax_.as<T>(), int(S::NBLK), 1, T(p.probe_alpha),
+                           int(upd_probe), dP_.as<cplx<T>>(), tw_.as<cplx<T>>(), int(upd_obj),
+                           (const T*)D_.as<T>(), int(nyo_), (const T*)pmax_.as<T>(),
+                           T(p.obj_alpha), 1, chi0);
+      };
+      switch (M_) {
+        case 1: launch(std::integral_constant<int, 1>{}); break;
+        case 2: launch(std::integral_constant<int, 2>{}); break;
+        case 3: launch(std::integral_constant<int, 3>{}); break;
+        default: launch(std::integral_constant<int, 4>{}); break;
+      }
+    });
+    if (pos_refine_) position_sums(0, W, nullptr, 0, ord, nb, chi0);
+    if (upd_obj) {
+      run(K_OBJ_UPDATE, [&] {
+        hipLaunchKernelGGL(k_obj_update_rpie_sum<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                           O_.as<cplx<T>>(), (const cplx<T>*)psi_.as<cplx<T>>(), N, M_,
+                           corner_.as<int2>(), ord, list_.as<int>(), lcount_.as<int>(),
+                           int(cap_), int(nyo_), int(nxo_), ntx_, T(p.obj_step));
+      });
+    }
+    if (upd_probe) {
+      cplx<T>* g = dP_.as<cplx<T>>();
+      const int total = int(nprobe_ * n);
+      run(K_PROBE, [&] {
+        hipLaunchKernelGGL((k_times_q<T, false>), dim3((n + 255) / 256, K), dim3(256), 0,
+                           stream_, g, (const cplx<T>*)q_.as<cplx<T>>(), n);
+        hipLaunchKernelGGL((k_rows<T, N, false>), dim3(S::NBLK, K), dim3(S::NT), 0, stream_, g,
+                           tw_.as<cplx<T>>());
+        hipLaunchKernelGGL((k_cols<T, N, false>), dim3(SC::NBLK, K), dim3(SC::NT), 0, stream_, g,
+                           tw_.as<cplx<T>>());
+        hipLaunchKernelGGL(k_unramp_mean<T>, dim3((n + 255) / 256, M_), dim3(256), 0, stream_,
+                           dPm_.as<cplx<T>>(), (const cplx<T>*)g,
+                           (const double2*)offs_.as<double2>(), ord, nb, N, M_,
+                           1.0 / (double(n) * double(nb)));
+        hipLaunchKernelGGL((k_cols<T, N, true>), dim3(SC::NBLK, M_), dim3(SC::NT), 0, stream_,
+                           dPm_.as<cplx<T>>(), tw_.as<cplx<T>>());
+        hipLaunchKernelGGL((k_rows<T, N, true>), dim3(S::NBLK, M_), dim3(S::NT), 0, stream_,
+                           dPm_.as<cplx<T>>(), tw_.as<cplx<T>>());
+        hipLaunchKernelGGL((k_times_q<T, true>), dim3((n + 255) / 256, M_), dim3(256), 0,
+                           stream_, dPm_.as<cplx<T>>(), (const cplx<T>*)q_.as<cplx<T>>(), n);
+        hipLaunchKernelGGL(k_probe_apply<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,
+                           P_.as<cplx<T>>(), Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
+                           (const cplx<T>*)dPm_.as<cplx<T>>(), 1, n, total, T(p.probe_step));
+      });
+    }
+    if (pos_refine_) {
+      const double eps = 1e-6 * scale_;  // EPS_POSITION·k (ptyrs/src/engine/features.rs:455)
+      run(K_OTHER, [&] {
+        hipLaunchKernelGGL(k_pos_step, dim3(1), dim3(1024), 0, stream_,
+                           (const double*)pos_part_.as<double>(), aux_blocks(), nb, int(nframe_),
+                           eps, ord, offs_.as<double2>(), pos_sums_.as<double>());
       });
     }
   }

@@ -238,83 +238,4 @@ inline ExitModel free_space(int ny, int nx, double wavelength, double py, double
   return m;
 }
 
-// In-place DFT of one line of length n with stride `step` (forward: exp(−2πi·jk/n)):
-// radix-2 decimation in time where n is a power of two, the direct sum otherwise.
-inline void host_dft(std::complex<double>* a, int n, int step) {
-  std::vector<std::complex<double>> x(n), y(n);
-  for (int i = 0; i < n; ++i) x[i] = a[size_t(i) * step];
-  if ((n & (n - 1)) == 0) {
-    int bits = 0;
-    while ((1 << bits) < n) ++bits;
-    for (int i = 0; i < n; ++i) {
-      int r = 0;
-      for (int b = 0; b < bits; ++b) r |= ((i >> b) & 1) << (bits - 1 - b);
-      y[r] = x[i];
-    }
-    for (int len = 2; len <= n; len <<= 1) {
-      for (int i = 0; i < n; i += len)
-        for (int k = 0; k < len / 2; ++k) {
-          const std::complex<double> w = std::polar(1.0, -2.0 * M_PI * k / len);
-          const std::complex<double> u = y[i + k], v = y[i + k + len / 2] * w;
-          y[i + k] = u + v;
-          y[i + k + len / 2] = u - v;
-        }
-    }
-  } else {
-    for (int k = 0; k < n; ++k) {
-      std::complex<double> s = 0;
-      for (int j = 0; j < n; ++j) s += x[j] * std::polar(1.0, -2.0 * M_PI * double(size_t(j) * k % n) / n);
-      y[k] = s;
-    }
-  }
-  for (int i = 0; i < n; ++i) a[size_t(i) * step] = y[i];
-}
-
-// PyNX ZeroPhaseRamp on host arrays (ptyrs/src/engine/ptycho.rs:593-603, :1132-1181): the centre of mass
-// (cy, cx) of Σ_modes |fftshift(DFT(fftshift(P)))|², dx = cx − nx/2, dy = cy − ny/2, then
-// P *= exp(−2πi(dx·(x − nx/2)/nx + dy·(y − ny/2)/ny)) and
-// O *= exp(+2πi(dx·(x − nxo/2)/nx + dy·(y − nyo/2)/ny)). Arrays interleaved (re, im).
-inline void zero_phase_ramp(double* obj, size_t nobj, size_t nyo, size_t nxo, double* probe,
-                            size_t nprobe, int ny, int nx) {
-  const size_t n = size_t(ny) * nx;
-  std::vector<double> v(n, 0.0);
-  std::vector<std::complex<double>> a(n);
-  auto shift = [&](size_t j) {
-    const size_t y = j / nx, x = j % nx;
-    return ((y + ny / 2) % ny) * nx + (x + nx / 2) % nx;
-  };
-  for (size_t m = 0; m < nprobe; ++m) {
-    const double* p = probe + 2 * m * n;
-    for (size_t j = 0; j < n; ++j) {
-      const size_t i = shift(j);
-      a[j] = {p[2 * i], p[2 * i + 1]};
-    }
-    for (int y = 0; y < ny; ++y) host_dft(a.data() + size_t(y) * nx, nx, 1);
-    for (int x = 0; x < nx; ++x) host_dft(a.data() + x, ny, nx);
-    for (size_t i = 0; i < n; ++i) v[i] += std::norm(a[shift(i)]);
-  }
-  double w = 0, wy = 0, wx = 0;
-  for (size_t i = 0; i < n; ++i) {
-    w += v[i];
-    wy += v[i] * double(i / nx);
-    wx += v[i] * double(i % nx);
-  }
-  const double dy = wy / w - double(ny / 2), dx = wx / w - double(nx / 2);
-  auto apply = [](double* arr, size_t planes, size_t py, size_t px, double sy, double sx,
-                  size_t hy, size_t hx) {
-    for (size_t k = 0; k < planes; ++k)
-      for (size_t y = 0; y < py; ++y)
-        for (size_t x = 0; x < px; ++x) {
-          const double ph = sy * (double(y) - double(hy)) + sx * (double(x) - double(hx));
-          const std::complex<double> c(std::cos(ph), std::sin(ph));
-          double* e = arr + 2 * ((k * py + y) * px + x);
-          const std::complex<double> r = std::complex<double>(e[0], e[1]) * c;
-          e[0] = r.real();
-          e[1] = r.imag();
-        }
-  };
-  apply(probe, nprobe, ny, nx, -2.0 * M_PI * dy / ny, -2.0 * M_PI * dx / nx, ny / 2, nx / 2);
-  apply(obj, nobj, nyo, nxo, 2.0 * M_PI * dy / ny, 2.0 * M_PI * dx / nx, nyo / 2, nxo / 2);
-}
-
 }  // namespace ptyrs

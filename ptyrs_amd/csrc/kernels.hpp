@@ -193,6 +193,7 @@ __global__ void k_norm2_planes(const cplx<T>* __restrict__ Wt, T* __restrict__ W
 // The exit wave pass A transforms: PO itself, or DM's and RAAR's mix of PO with the waves
 // they carry across cycles (ptyrs/src/engine/ptycho.rs:643-731).
 enum Mix { MIX_NONE = 0, MIX_DM = 1, MIX_RAAR = 2, MIX_RAAR0 = 3 };
+
 template <typename T>
 struct MixArgs {
   int kind;
@@ -206,7 +207,7 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a(
     const cplx<T>* __restrict__ O, int nxo, const cplx<T>* __restrict__ Wt,
     const int2* __restrict__ corner, const int* __restrict__ order, int M,
     cplx<T>* __restrict__ psi, T* __restrict__ omax_part, const cplx<T>* __restrict__ tw,
-    MixArgs<T> mix) {
+    MixArgs<T> mix, int w_view) {
   using S = RowShape<T, N>;
   constexpr int E = S::E, TL = S::TL;
   __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
@@ -219,7 +220,7 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a(
   const int f = order[b];
   const int2 c = corner[f];
   const cplx<T>* orow = O + (size_t)(c.x + y) * nxo + c.y;
-  const cplx<T>* wrow = Wt + ((size_t)m * N + y) * N;
+  const cplx<T>* wrow = Wt + ((size_t)(w_view ? int(blockIdx.y) : m) * N + y) * N;
   const size_t crow = (((size_t)f * M + m) * N + y) * N;
   cplx<T> v[E];
   T mx = 0;
@@ -573,7 +574,9 @@ constexpr int rows_per_group(int n, int l, int m) {
 // mode order, each group for every M-th of its elements, into the slot's mode-0 plane; the
 // terms alternate between two LDS buffers. With `probe_dir` each group adds conj(O·q)·Δ_m
 // divided by (1 − α_p)|O|² + α_p·max_window |O|² into its mode's per-chunk probe direction.
-// ptyrs/src/engine/ptycho.rs:900-1086
+// Under position offsets (`w_view`, one slot per chunk) W is the slot's own moved wave, plane
+// slot·M + m of Wt, and with `chi0` mode 0's Δ is also stored ([slot·M][n], the position sums'
+// χ). ptyrs/src/engine/ptycho.rs:900-1086
 // Threads of k_pass_c_rpie_sum: M groups of G rows of pass C's line width.
 template <typename T, int N, int M>
 constexpr int pass_c_sum_threads() {
@@ -587,7 +590,8 @@ __global__ __launch_bounds__((pass_c_sum_threads<T, N, M>())) void k_pass_c_rpie
     const int2* __restrict__ corner, const int* __restrict__ order, int nb,
     const T* __restrict__ omax_part, int na, int chunk, T probe_alpha, int probe_dir,
     cplx<T>* __restrict__ dP_part, const cplx<T>* __restrict__ tw, int upd_obj,
-    const T* __restrict__ D, int nyo, const T* __restrict__ pmax, T obj_alpha) {
+    const T* __restrict__ D, int nyo, const T* __restrict__ pmax, T obj_alpha, int w_view,
+    cplx<T>* __restrict__ chi0) {
   using S = RowShapeC<T, N>;
   constexpr int E = S::E, TL = S::TL;
   constexpr int G = rows_per_group(N, S::L, M);
@@ -617,7 +621,7 @@ __global__ __launch_bounds__((pass_c_sum_threads<T, N, M>())) void k_pass_c_rpie
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int x = jl + TL * e;
-    w[e] = ld(Wt + (size_t)m * n + (size_t)y * N + x);
+    w[e] = ld(Wt + (size_t)(w_view ? b0 * M + m : m) * n + (size_t)y * N + x);
     qq[e] = ld(q + (size_t)y * N + x);
     acc[e] = cplx<T>{0, 0};
   }
@@ -689,6 +693,7 @@ __global__ __launch_bounds__((pass_c_sum_threads<T, N, M>())) void k_pass_c_rpie
     for (int e = 0; e < E; ++e) {
       const cplx<T> o = ow[e];
       const cplx<T> d = v[e] - w[e] * o;
+      if (chi0 != nullptr && m == 0) st(chi0 + ((size_t)b * M * N + y) * N + jl + TL * e, d);
       if (upd_obj) trow[TL * e] = cmulc(w[e], d) * rcp(inv_o * dw[e] + reg);
       if (probe_dir)
         acc[e] = acc[e] + cmulc(o * qq[e], d) * rcp(inv_p * norm2(o) + probe_alpha * omax);
@@ -740,19 +745,22 @@ __global__ void k_probe_apply(cplx<T>* __restrict__ P, cplx<T>* __restrict__ Wt,
 constexpr int TILE_X = 64, TILE_Y = 4, TROWS = TILE_Y / (THREADS / TILE_X);
 
 // One workgroup per object tile: (1) the batch slots whose N×N window meets the tile, in slot
-// order, compacted with wave ballots into list[tile][·] and count[tile]; (2) with `Wt`, each
-// pixel's batch overlap D_m = Σ_views |W_m|² per mode into D[m][pixel] (W = P·q, |q| = 1) and
-// the tile max of Σ_m D_m (rPIE's regulariser max).
+// order, compacted with wave ballots into list[tile][·] and count[tile]; (2) with `W2` (|W_m|²
+// per mode, W = P·q) or `Wv` (every view's own wave W, plane slot·M + m, under position
+// offsets), each pixel's batch overlap D_m = Σ_views |W_m|² per mode into D[m][pixel] and the
+// tile max of Σ_m D_m (rPIE's regulariser max).
 template <typename T>
 __global__ __launch_bounds__(THREADS) void k_bin_tiles(
     const int2* __restrict__ corner, const int* __restrict__ order, int nb, int N, int ntx,
     int cap, int* __restrict__ list, int* __restrict__ count, const T* __restrict__ W2,
-    int M, int nyo, int nxo, T* __restrict__ D, T* __restrict__ tilemax) {
+    int M, int nyo, int nxo, T* __restrict__ D, T* __restrict__ tilemax,
+    const cplx<T>* __restrict__ Wv) {
   __shared__ int wave_n[THREADS / 64];
   __shared__ int base_s;
   __shared__ T red[THREADS];
-  // The corners of the tile's first THREADS views, kept as they are found.
+  // The corners (and batch slots) of the tile's first THREADS views, kept as they are found.
   __shared__ int2 cs[THREADS];
+  __shared__ int bs[THREADS];
   const int t = tile_id(), tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int ty0 = (t / ntx) * TILE_Y, tx0 = (t % ntx) * TILE_X;
   int* out = list + (size_t)t * cap;
@@ -774,7 +782,10 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
     for (int w = 0; w < wave; ++w) off += wave_n[w];
     if (hit) {
       out[off + before] = b;
-      if (off + before < THREADS) cs[off + before] = c;
+      if (off + before < THREADS) {
+        cs[off + before] = c;
+        bs[off + before] = b;
+      }
     }
     __syncthreads();
     if (tid == 0) {
@@ -786,7 +797,7 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
   }
   const int nl = base_s;
   if (tid == 0) count[t] = nl;
-  if (W2 == nullptr) return;
+  if (W2 == nullptr && Wv == nullptr) return;
   constexpr int MM = 4;
   const int X = tx0 + tid % TILE_X, Y0 = ty0 + tid / TILE_X;
   constexpr int RS = THREADS / TILE_X;  // row stride between a thread's pixels
@@ -795,7 +806,10 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
   for (int s = 0; s < nl; s += THREADS) {
     const int k = nl - s < THREADS ? nl - s : THREADS;
     if (s > 0) {
-      if (tid < k) cs[tid] = corner[order[out[s + tid]]];
+      if (tid < k) {
+        bs[tid] = out[s + tid];
+        cs[tid] = corner[order[bs[tid]]];
+      }
       __syncthreads();
     }
     if (X < nxo) {
@@ -816,7 +830,9 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
             if (ok[u]) {
 #pragma unroll
               for (int m = 0; m < MM; ++m)
-                if (m < M) w2[u][m] = W2[(size_t)m * n + y * N + x];
+                if (m < M)
+                  w2[u][m] = Wv ? norm2(Wv[((size_t)bs[i] * M + m) * n + y * N + x])
+                                : W2[(size_t)m * n + y * N + x];
             }
           }
 #pragma unroll
@@ -1313,6 +1329,72 @@ __global__ void k_lsqml_log_afar(const T* __restrict__ a, const int* __restrict_
   const T v = gauss ? T(0.5) : a[b];
   a_far_all[order[b]] = v;
   log[(size_t)b * ls] = double(v);
+}
+
+// ---- zero phase ramp (ptyrs/src/engine/ptycho.rs:1134-1181) ----------------------------------
+// The probe spectrum's centre of mass: per workgroup the FP64 sums of v, v·y and v·x over the
+// DFT indices k of the `M` planes `spec` (each the unnormalised DFT of a probe mode), v the
+// modes' |spec|² summed at k and (y, x) the fftshifted index of k. The DFT of the fftshifted
+// probe is this one times (−1)^(ky+kx) on even sides, so the moduli are the same.
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_spectrum_moments(const cplx<T>* __restrict__ spec,
+                                                              int N, int M,
+                                                              double* __restrict__ part) {
+  const int n = N * N;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * THREADS + threadIdx.x; i < n; i += gridDim.x * THREADS) {
+    double v = 0.0;
+    for (int m = 0; m < M; ++m) v += double(norm2(spec[(size_t)m * n + i]));
+    const int y = (i / N + N / 2) % N, x = (i % N + N / 2) % N;
+    acc[0] += v;
+    acc[1] += v * double(y);
+    acc[2] += v * double(x);
+  }
+  __shared__ double red[THREADS];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double r = block_reduce<THREADS>(acc[k], red, AddOp());
+    if (threadIdx.x == 0) part[blockIdx.x * 3 + k] = r;
+  }
+}
+
+// The centre of mass's offset from the plane's centre, (dy, dx) in pixels, from `nb`
+// workgroups' sums (added in workgroup order per thread, then across the workgroup).
+__global__ __launch_bounds__(THREADS) static void k_ramp_offset(const double* __restrict__ part,
+                                                                int nb, int N,
+                                                                double* __restrict__ d) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += THREADS)
+    for (int k = 0; k < 3; ++k) acc[k] += part[b * 3 + k];
+  __shared__ double red[THREADS];
+  double s[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) s[k] = block_reduce<THREADS>(acc[k], red, AddOp());
+  if (threadIdx.x == 0) {
+    d[0] = s[1] / s[0] - double(N / 2);
+    d[1] = s[2] / s[0] - double(N / 2);
+  }
+}
+
+// `planes` planes of py × px multiplied by exp(i·(sy·(y − py/2) + sx·(x − px/2))), the phase
+// in FP64 with s = sgn·2π·d/N (the probe takes sgn = −1, the object +1).
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_ramp(cplx<T>* __restrict__ a, int planes, int py,
+                                                    int px, const double* __restrict__ d,
+                                                    double sgn, int N) {
+  const size_t total = size_t(planes) * py * px;
+  const double sy = sgn * 2.0 * M_PI * d[0] / double(N);
+  const double sx = sgn * 2.0 * M_PI * d[1] / double(N);
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < total;
+       i += size_t(gridDim.x) * 256) {
+    const size_t r = i % (size_t(py) * px);
+    const int y = int(r / px), x = int(r % px);
+    const double ph = sy * (double(y) - double(py / 2)) + sx * (double(x) - double(px / 2));
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    const double re = double(a[i].x), im = double(a[i].y);
+    a[i] = cplx<T>{T(re * cs - im * sn), T(re * sn + im * cs)};
+  }
 }
 
 // ---- multislice (ptyrs/src/engine/multislice.rs) ----------------------------------------------------

@@ -125,7 +125,9 @@ def test_position_feature_api():
     assert np.array_equal(f["position"], d0) and f["hold"] == ["position"]
     g.lsqml(1, 1, 6, "poisson", 1)
     assert np.array_equal(g.features["position"], d0)
-    for name in ("rpie", "ap", "dm", "raar", "ml"):
+    g.rpie(1, batch_size=5)  # held offsets: rPIE models them and leaves them
+    assert np.array_equal(g.features["position"], d0)
+    for name in ("ap", "dm", "raar", "ml"):
         with pytest.raises(ValueError, match="position"):
             getattr(g, name)(1)
     with pytest.raises(ValueError):
@@ -137,3 +139,31 @@ def test_position_feature_api():
     g.set_features()
     assert g.features["position"] is None
     g.rpie(1)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("nprobe,batch,refine", [(1, 7, True), (3, 30, True), (2, 5, False)])
+def test_rpie_positions_match_oracle(dtype, nprobe, batch, refine):
+    # rPIE under PATCH offsets (ptyrs/src/engine/ptycho.rs:900-1086 with Features::position):
+    # far field, masked pixels, 1-3 modes, refined or held, two epochs with LLK records.
+    d = small_scan(n=64, nprobe=nprobe, npos=30, seed=11)
+    d0 = np.random.default_rng(12).uniform(-0.4, 0.4, (30, 2))
+    o = oracle(d)
+    o.set_position(d0, refine=refine)
+    kw = dict(batch_size=batch, obj_alpha=0.2, probe_alpha=0.3, calc_llk=1, seed=5)
+    o.rpie(2, **kw)
+    g = gpu(d, dtype)
+    g.set_features(position=d0, hold=() if refine else ("position",))
+    g.rpie(2, **kw)
+    tol = TOL[dtype]
+    want = [r[2].poisson for r in o.history]
+    print("rpie positions", dtype, nprobe, max_rel(g.history["poisson"], want),
+          rel_l2(g.object - d["obj"], o.obj - d["obj"]), rel_l2(g.probe, o.probe))
+    assert max_rel(g.history["poisson"], want) < tol
+    assert rel_l2(g.object - d["obj"], o.obj - d["obj"]) < tol * 10
+    assert rel_l2(g.probe, o.probe) < tol * 10
+    got = g.features["position"]
+    if refine:
+        assert max_rel((got - d0).ravel(), (o.offsets - d0).ravel()) < tol * 100
+    else:
+        assert np.array_equal(got, d0)
