@@ -241,6 +241,16 @@ def algorithm_rates(pa, scan, d, steps=3, algorithms=("ap", "dm", "raar", "ml"))
             r["value" if dtype == "c64" else "fp64"] = round(NPOS * steps / dt, 1)
             del g
         r["speedup_fp32_vs_fp64"] = round(r["value"] / r["fp64"], 3)
+        if alg in ("dm", "raar"):
+            # GpuOptions::psi_f16: the carried exit waves as f16 pairs.
+            g = pa.GpuPtycho(scan, d["start_obj"], d["start_probe"], psi_f16=True)
+            getattr(g, alg)(1)
+            g.synchronize()
+            t = time.perf_counter()
+            getattr(g, alg)(steps)
+            g.synchronize()
+            r["psi_f16"] = round(NPOS * steps / (time.perf_counter() - t), 1)
+            del g
         out[alg] = r
         print(f"[bench] {alg}: {r}", file=sys.stderr, flush=True)
     return out

@@ -82,9 +82,10 @@ typedef struct {
  * replay captured epochs (same bits either way). llk_slots >= 1: LLK records the device holds;
  * the host waits for them only when that many are held or at the end of a call. fused: this
  * engine's passes are always the fused ones, so 0 and 1 run the same kernels (the reference's
- * M1 and M2 launch paths agree to rounding). psi_f16 (f16 carried exit waves for DM and RAAR)
- * needs fused and is refused with PTYRS_EINVAL: this engine carries them in the element
- * type. */
+ * M1 and M2 launch paths agree to rounding). psi_f16: DM's and RAAR's carried exit waves (and
+ * RAAR's iterate) held as one pair of IEEE halves per value, times a power of two fixed from
+ * the counts (ptyrs/src/gpu/psi.rs:40-50), half the bytes of complex64; needs fused and
+ * dtype c64, else PTYRS_EINVAL. */
 typedef struct {
   size_t stack_size;
   int graph;

@@ -257,13 +257,14 @@ int ptyrs_engine_create(const ptyrs_scan* scan, const ptyrs_optics* optics, cons
     if (options) {
       if (options->llk_slots == 0) throw Invalid("llk_slots must be >= 1");
       if (options->psi_f16 && !options->fused) throw Invalid("psi_f16 needs fused passes");
-      if (options->psi_f16)
-        throw Invalid("psi_f16 is not supported: this engine carries exit waves in its element type");
+      if (options->psi_f16 && dtype != PTYRS_C64)
+        throw Invalid("psi_f16 holds the f16 form of complex64 waves: dtype must be c64");
     }
     Prepared pre = prepare(*scan, optics, nprobe, nobj, nyo, nxo);
     pre.graph = options ? options->graph != 0 : true;
     pre.stack_size = options ? options->stack_size : 0;
     pre.llk_slots = options ? options->llk_slots : 8;
+    pre.psi_f16 = options ? options->psi_f16 != 0 : false;
     if (pre.stack_size && std::min(pre.stack_size, pre.nframe) * nprobe * nobj > 65535)
       throw Invalid("a stack of " + std::to_string(pre.stack_size) +
                     " frames holds more than 65535 views");

@@ -107,6 +107,7 @@ struct Prepared {
   bool graph = true;           // GpuOptions::graph: replay captured epochs
   size_t stack_size = 0;       // GpuOptions::stack_size: frames per stack, 0 = the default
   size_t llk_slots = 8;        // GpuOptions::llk_slots: LLK records the device holds
+  bool psi_f16 = false;        // GpuOptions::psi_f16: DM's and RAAR's carried waves as f16 pairs
   ExitModel exit;
   // Multislice: object slices and, per gap, its angular-spectrum transfer function.
   size_t nslice = 1;
@@ -333,10 +334,13 @@ class Engine : public EngineBase {
     max_frame_total_ = pre.max_frame_total;
     total_ = pre.total;
     M_ = int(nprobe * nobj);
+    if (M_ > kMaxModes) throw Invalid(modes_error());
     nslice_ = pre.nslice;
     graph_ = pre.graph;
     stack_size_ = pre.stack_size;
     hist_slots_ = std::max<size_t>(pre.llk_slots, 1);
+    psi_f16_ = pre.psi_f16;
+    half_scale_ = half_scale(pre.max_frame_total);
     reverse_ = pre.exit.reverse;
     dft_plane_ = pre.exit.dft_plane;
     plane_quad_[0] = pre.exit.plane_quad[0];
@@ -450,7 +454,7 @@ class Engine : public EngineBase {
     unitary("rPIE");
     if (nobj_ != 1) throw Invalid("rPIE needs exactly one object mode");
     if (p.batch_size == 0) throw Invalid("rPIE batch_size must be >= 1");
-    if (M_ > 4) throw Invalid("the GPU engine runs at most 4 modes");
+    if (M_ > kMaxModes) throw Invalid(modes_error());
   }
 
   // An rPIE call in three steps, so that several engines can interleave their epochs on their
@@ -554,7 +558,7 @@ class Engine : public EngineBase {
     if (p.batch_size == 0) throw Invalid("LSQML batch_size must be >= 1");
     if (p.cost != PTYRS_AMPLITUDE && p.cost != PTYRS_POISSON)
       throw Invalid("cost must be amplitude or poisson");
-    if (M_ > 4) throw Invalid("the GPU engine runs at most 4 modes");
+    if (M_ > kMaxModes) throw Invalid(modes_error());
     if (pos_on_ && sampling != PTYRS_PATCH)
       throw Invalid("position offsets under FullObject sampling are not run on this GPU "
                     "engine; set the PATCH sampling");
@@ -636,7 +640,7 @@ class Engine : public EngineBase {
   void project(const ProjParams& p, size_t ncycle) override {
     one_unitary(p.algorithm == PTYRS_AP ? "AP" : p.algorithm == PTYRS_DM ? "DM" : "RAAR");
     if (nobj_ != 1) throw Invalid("the GPU projection algorithms run one object mode");
-    if (M_ > 4) throw Invalid("the GPU engine runs at most 4 modes");
+    if (M_ > kMaxModes) throw Invalid(modes_error());
     if (p.algorithm != PTYRS_AP && p.loops == 0)
       throw Invalid("loop_obj_probe must be >= 1");
     if (ncycle == 0 || nframe_ == 0) {
@@ -650,8 +654,13 @@ class Engine : public EngineBase {
     const size_t stack = std::min<size_t>(nframe_, stack_size_ ? stack_size_ : kDefaultStack);
     ensure_workspace(stack, 1);
     const size_t nb_all = nframe_;
-    carried_.ensure(sizeof(cplx<T>) * nb_all * M_ * n);
-    if (p.algorithm == PTYRS_RAAR) iter_.ensure(sizeof(cplx<T>) * nb_all * M_ * n);
+    // psi_f16 holds DM's and RAAR's carried waves as f16 pairs (AP's stay complex).
+    const bool half = psi_f16_ && p.algorithm != PTYRS_AP;
+    DevBuf& cbuf = half ? carried16_ : carried_;
+    DevBuf& ibuf = half ? iter16_ : iter_;
+    const size_t wbytes = half ? sizeof(uint32_t) : sizeof(cplx<T>);
+    cbuf.ensure(wbytes * nb_all * M_ * n);
+    if (p.algorithm == PTYRS_RAAR) ibuf.ensure(wbytes * nb_all * M_ * n);
     const int nt = ntiles();
     list_all_.ensure(sizeof(int) * size_t(nt) * nb_all);
     count_all_.ensure(sizeof(int) * size_t(nt));
@@ -682,7 +691,9 @@ class Engine : public EngineBase {
                                          uint64_t(p.upd_probe), bits(p.obj_inertia),
                                          bits(p.probe_inertia), stack, cap_};
       epoch(proj_graph_, key, [&] {
-      MixArgs<T> mix{MIX_NONE, carried_.as<cplx<T>>(), iter_.as<cplx<T>>(), T(0), T(0)};
+      with_carried(half, [&](auto cw, auto iw) {
+      constexpr bool H = decltype(cw)::half;
+      MixArgs<T, H> mix{MIX_NONE, cw, iw, T(0), T(0)};
       if (p.algorithm == PTYRS_DM && !first) {
         mix.kind = MIX_DM;
         mix.c0 = T(2.0 - p.mix);
@@ -703,11 +714,10 @@ class Engine : public EngineBase {
         const int nck = (nb + ck - 1) / ck;
         const bool dm = p.algorithm == PTYRS_DM && !first;
         run(K_PASS_C, [&] {
-          hipLaunchKernelGGL((k_pass_c_proj<T, N>), dim3(SR::NBLK, M_, nck), dim3(SR::NT), 0,
-                             stream_, psi_.as<cplx<T>>(), carried_.as<cplx<T>>(),
-                             O_.as<cplx<T>>(), int(nxo_), Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
-                             corner_.as<int2>(), ord, nb, M_, ck, int(dm), T(2.0 - p.mix),
-                             T(1.0 - p.mix), int(ap_probe),
+          hipLaunchKernelGGL((k_pass_c_proj<T, N, H>), dim3(SR::NBLK, M_, nck), dim3(SR::NT), 0,
+                             stream_, psi_.as<cplx<T>>(), cw, O_.as<cplx<T>>(), int(nxo_),
+                             Wt_.as<cplx<T>>(), q_.as<cplx<T>>(), corner_.as<int2>(), ord, nb,
+                             M_, ck, int(dm), T(2.0 - p.mix), T(1.0 - p.mix), int(ap_probe),
                              pnum_part_.as<cplx<T>>() + size_t(s) * nchunk * M_ * n,
                              pden_part_.as<T>() + size_t(s) * nchunk * n, tw_.as<cplx<T>>());
         });
@@ -715,17 +725,18 @@ class Engine : public EngineBase {
       }
       const int tot_chunks = nstack * nchunk;
       if (p.algorithm == PTYRS_AP) {
-        if (p.upd_obj) object_blend(p.obj_inertia);
+        if (p.upd_obj) object_blend(p.obj_inertia, cw);
         if (p.upd_probe) probe_blend(tot_chunks, p.probe_inertia);
       } else {
         for (size_t l = 0; l < p.loops; ++l) {
-          if (p.upd_obj) object_blend(p.obj_inertia);
+          if (p.upd_obj) object_blend(p.obj_inertia, cw);
           if (p.upd_probe) {
-            probe_sums_all(stack, nstack, nchunk);
+            probe_sums_all(stack, nstack, nchunk, cw);
             probe_blend(tot_chunks, p.probe_inertia);
           }
         }
       }
+      });
       });
       if (rec && p.algorithm != PTYRS_AP) balance(1.0);
       ++cycle;
@@ -740,11 +751,20 @@ class Engine : public EngineBase {
   // then per cycle the gradients, Polak–Ribière β, d = β·d − g, the Newton step γ along d and
   // O += γ·d_O, P += γ·d_P (or d = −g where γ is not finite).
   void ml(const ptyrs_ml_params& p, size_t ncycle) override {
-    one_unitary("ML");
+    if (nslice_ > 1)
+      throw Invalid("ML runs one object slice; only LSQML and the LLK pass run several");
+    if (pos_on_ && sampling != PTYRS_PATCH)
+      throw Invalid("ML with position offsets under FullObject sampling is not run on this "
+                    "GPU engine; set the PATCH sampling");
+    unitary("ML");
     if (nobj_ != 1) throw Invalid("the GPU ML runs one object mode");
-    if (M_ > 4) throw Invalid("the GPU engine runs at most 4 modes");
+    if (M_ > kMaxModes) throw Invalid(modes_error());
     if (nframe_ == 0) {
       cycle += ncycle;
+      return;
+    }
+    if (pos_on_) {
+      ml_moved(p, ncycle);
       return;
     }
     const bool uo = p.update_object != 0, up = p.update_probe != 0;
@@ -813,9 +833,10 @@ class Engine : public EngineBase {
         const int ck = chunk_of(nb);
         const int nck = (nb + ck - 1) / ck;
         run(K_PASS_C, [&] {
-          hipLaunchKernelGGL((k_pass_c_proj<T, N>), dim3(SR::NBLK, M_, nck), dim3(SR::NT), 0,
-                             stream_, psi_.as<cplx<T>>(), carried_.as<cplx<T>>(),
-                             O_.as<cplx<T>>(), int(nxo_), Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
+          hipLaunchKernelGGL((k_pass_c_proj<T, N, false>), dim3(SR::NBLK, M_, nck),
+                             dim3(SR::NT), 0, stream_, psi_.as<cplx<T>>(),
+                             Carried<T, false>{carried_.as<cplx<T>>()}, O_.as<cplx<T>>(),
+                             int(nxo_), Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
                              corner_.as<int2>(), ord, nb, M_, ck, 0, T(0), T(0), int(up),
                              pnum_part_.as<cplx<T>>() + size_t(s) * nchunk * M_ * n,
                              pden_part_.as<T>() + size_t(s) * nchunk * n, tw_.as<cplx<T>>());
@@ -888,12 +909,10 @@ class Engine : public EngineBase {
                                gam_part_.as<double>() + size_t(f0) * SC::NBLK * 2,
                                tw_.as<cplx<T>>(), norm);
           };
-          switch (M_) {
-            case 1: go(k_pass_b_gamma<T, N, 1, false>, k_pass_b_gamma<T, N, 1, true>); break;
-            case 2: go(k_pass_b_gamma<T, N, 2, false>, k_pass_b_gamma<T, N, 2, true>); break;
-            case 3: go(k_pass_b_gamma<T, N, 3, false>, k_pass_b_gamma<T, N, 3, true>); break;
-            default: go(k_pass_b_gamma<T, N, 4, false>, k_pass_b_gamma<T, N, 4, true>); break;
-          }
+          with_modes([&](auto mm) {
+            constexpr int MM = decltype(mm)::value;
+            go(k_pass_b_gamma<T, N, MM, false>, k_pass_b_gamma<T, N, MM, true>);
+          });
         });
       }
       run(K_OTHER, [&] {
@@ -919,8 +938,272 @@ class Engine : public EngineBase {
     flush_timing();
   }
 
+  // PyNX ML under position offsets (PATCH; ptyrs/src/engine/ptycho.rs:757-842 with
+  // Features::position, the reference GPU's ml under its position model,
+  // ptyrs/src/engine/gpu/ml.rs:86-160): every view's wave is the probe terms moved by its
+  // frame's offset (kept for the whole scan in wall_); the object gradient sums conj(W)·g over
+  // the views, the probe gradient moves each frame's conj(O)·g back in Fourier space (its
+  // spectra summed stack by stack); γ's a and b take the moved waves of dP·q; with refinement
+  // the position sums come from the magnitude projection of the cycle's starting estimate and
+  // AP's, DM's and ML's step follows the update (ptyrs/src/engine/features.rs:960-1047).
+  DevBuf wall_, dwpos_, pos_all_, pstep_, pden1_;
+  void ml_moved(const ptyrs_ml_params& p, size_t ncycle) {
+    const bool uo = p.update_object != 0, up = p.update_probe != 0;
+    const size_t stack = std::min<size_t>(nframe_, stack_size_ ? stack_size_ : kDefaultStack);
+    ensure_workspace(stack, 1);
+    const size_t no = nobj_ * nyo_ * nxo_, np = nprobe_ * n;
+    const int nt = ntiles();
+    const int xb = aux_blocks();
+    carried_.ensure(sizeof(cplx<T>) * nframe_ * M_ * n);
+    wall_.ensure(sizeof(cplx<T>) * nframe_ * M_ * n);
+    pden1_.ensure(sizeof(T) * stack * n);
+    if (pos_refine_) {
+      dwpos_.ensure(sizeof(cplx<T>) * 2 * stack * M_ * n);
+      pos_all_.ensure(sizeof(double) * 4 * nframe_ * xb);
+      pstep_.ensure(sizeof(double) * 4 * nframe_);
+    }
+    psi3_.ensure(sizeof(cplx<T>) * 3 * stack * M_ * n);
+    for (DevBuf* b : {&gO_, &hO_, &dOc_}) b->ensure(sizeof(cplx<T>) * no);
+    for (DevBuf* b : {&gP_, &hP_, &dPc_, &dWt_}) b->ensure(sizeof(cplx<T>) * np);
+    list_all_.ensure(sizeof(int) * size_t(nt) * nframe_);
+    count_all_.ensure(sizeof(int) * size_t(nt));
+    abs_part_.ensure(sizeof(double) * 2 * kAbsBlocks);
+    cg_part_.ensure(sizeof(double) * 2 * kAbsBlocks);
+    cg_sums_.ensure(sizeof(double) * 2);
+    gam_part_.ensure(sizeof(double) * 2 * nframe_ * SC::NBLK);
+    gam_.ensure(sizeof(double) * 2);
+    const int nstack = int((nframe_ + stack - 1) / stack);
+    PTYRS_HIP(hipMemsetAsync(dOc_.p, 0, sizeof(cplx<T>) * no, stream_));
+    PTYRS_HIP(hipMemsetAsync(dPc_.p, 0, sizeof(cplx<T>) * np, stream_));
+    PTYRS_HIP(hipMemsetAsync(gO_.p, 0, sizeof(cplx<T>) * no, stream_));
+    PTYRS_HIP(hipMemsetAsync(gP_.p, 0, sizeof(cplx<T>) * np, stream_));
+    probe_terms();
+    // ScaleObjProbe(absolute=True) from the model photons of the start.
+    llk_pass(0);
+    balance(1.0, hist_.as<double>() + 3);
+    if (uo) {
+      run(K_BIN, [&] {
+        hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                           corner_.as<int2>(), iota_.as<int>(), int(nframe_), N, ntx_,
+                           int(nframe_), list_all_.as<int>(), count_all_.as<int>(),
+                           (const T*)nullptr, M_, int(nyo_), int(nxo_), (T*)nullptr,
+                           tilemax_.as<T>(), (const cplx<T>*)nullptr);
+      });
+    }
+    const int grid_o = int(std::min<size_t>(kAbsBlocks, (no + THREADS - 1) / THREADS));
+    const int grid_p = int(std::min<size_t>(kAbsBlocks, (np + THREADS - 1) / THREADS));
+    cplx<T>* wall = wall_.as<cplx<T>>();
+    cplx<T>* g = gspec_.as<cplx<T>>();
+    const size_t vplane = size_t(M_) * n;  // one frame's views
+    std::vector<std::pair<size_t, int>> due;
+    for (size_t ic = 0; ic < ncycle; ++ic) {
+      const bool rec = p.calc_llk > 0 && (ic % p.calc_llk == 0 || ic + 1 == ncycle);
+      if (rec) {
+        if (due.size() == hist_slots_) drain(due, PTYRS_ML);
+        const int slot = int(due.size());
+        llk_pass(slot);
+        due.push_back({cycle, slot});
+      }
+      const std::vector<uint64_t> key = {uint64_t(ic == 0), uint64_t(uo), uint64_t(up), stack,
+                                         cap_, 1, uint64_t(pos_refine_)};
+      epoch(ml_graph_, key, [&] {
+      // Every view's moved wave at this cycle's probe.
+      run(K_PROBE, [&] { probe_spectra(); });
+      for (int s = 0; s < nstack; ++s) {
+        const int f0 = int(s * stack);
+        const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+        moved_probes(iota_.as<int>() + f0, nb, false, wall + size_t(f0) * vplane, nullptr, true);
+      }
+      // Position sums of the magnitude projection of the starting estimate.
+      if (pos_refine_) {
+        for (int s = 0; s < nstack; ++s) {
+          const int f0 = int(s * stack);
+          const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+          const int* ord = iota_.as<int>() + f0;
+          const cplx<T>* ws = wall + size_t(f0) * vplane;
+          const int K = nb * M_;
+          pass_a(ord, nb, false, MixArgs<T>{MIX_NONE, {}, {}, T(0), T(0)}, ws);
+          pass_b<B_AMPLITUDE>(ord, nb);
+          run(K_OTHER, [&] {
+            rows(true, K);
+            hipLaunchKernelGGL(k_ramp_deriv_planes<T>, dim3((n + 255) / 256, K), dim3(256), 0,
+                               stream_, dwpos_.as<cplx<T>>(),
+                               (const cplx<T>*)spec_.as<cplx<T>>(),
+                               (const double2*)offs_.as<double2>(), ord, N, M_,
+                               1.0 / double(n));
+            hipLaunchKernelGGL((k_cols<T, N, true>), dim3(SC::NBLK, 2 * K), dim3(SC::NT), 0,
+                               stream_, dwpos_.as<cplx<T>>(), tw_.as<cplx<T>>());
+            hipLaunchKernelGGL((k_rows<T, N, true>), dim3(S::NBLK, 2 * K), dim3(S::NT), 0,
+                               stream_, dwpos_.as<cplx<T>>(), tw_.as<cplx<T>>());
+            hipLaunchKernelGGL(k_pos_sums_proj<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
+                               (const cplx<T>*)psi_.as<cplx<T>>(), ws,
+                               (const cplx<T>*)O_.as<cplx<T>>(), int(nxo_), corner_.as<int2>(),
+                               ord, (const cplx<T>*)dwpos_.as<cplx<T>>(), N, M_,
+                               pos_all_.as<double>() + size_t(f0) * xb * 4);
+          });
+        }
+      }
+      // Gradients: 2·∂L/∂ψ* through the moved waves.
+      for (int s = 0; s < nstack; ++s) {
+        const int f0 = int(s * stack);
+        const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+        const int* ord = iota_.as<int>() + f0;
+        const cplx<T>* ws = wall + size_t(f0) * vplane;
+        const int K = nb * M_;
+        pass_a(ord, nb, false, MixArgs<T>{MIX_NONE, {}, {}, T(0), T(0)}, ws);
+        pass_b<B_GRAD>(ord, nb);
+        // One slot per chunk: each frame's conj(q·O)·g of its own.
+        run(K_PASS_C, [&] {
+          hipLaunchKernelGGL((k_pass_c_proj<T, N, false>), dim3(SR::NBLK, M_, nb),
+                             dim3(SR::NT), 0, stream_, psi_.as<cplx<T>>(),
+                             Carried<T, false>{carried_.as<cplx<T>>()}, O_.as<cplx<T>>(),
+                             int(nxo_), Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
+                             corner_.as<int2>(), ord, nb, M_, 1, 0, T(0), T(0), int(up), g,
+                             pden1_.as<T>(), tw_.as<cplx<T>>());
+        });
+        if (up) {
+          run(K_PROBE, [&] {
+            hipLaunchKernelGGL((k_times_q<T, false>), dim3((n + 255) / 256, K), dim3(256), 0,
+                               stream_, g, (const cplx<T>*)q_.as<cplx<T>>(), n);
+            hipLaunchKernelGGL((k_rows<T, N, false>), dim3(S::NBLK, K), dim3(S::NT), 0, stream_,
+                               g, tw_.as<cplx<T>>());
+            hipLaunchKernelGGL((k_cols<T, N, false>), dim3(SC::NBLK, K), dim3(SC::NT), 0,
+                               stream_, g, tw_.as<cplx<T>>());
+            hipLaunchKernelGGL(k_unramp_mean<T>, dim3((n + 255) / 256, M_), dim3(256), 0,
+                               stream_, gP_.as<cplx<T>>(), (const cplx<T>*)g,
+                               (const double2*)offs_.as<double2>(), ord, nb, N, M_,
+                               1.0 / double(n), int(s > 0));
+          });
+        }
+      }
+      if (up) {
+        run(K_PROBE, [&] {
+          hipLaunchKernelGGL((k_cols<T, N, true>), dim3(SC::NBLK, M_), dim3(SC::NT), 0, stream_,
+                             gP_.as<cplx<T>>(), tw_.as<cplx<T>>());
+          hipLaunchKernelGGL((k_rows<T, N, true>), dim3(S::NBLK, M_), dim3(S::NT), 0, stream_,
+                             gP_.as<cplx<T>>(), tw_.as<cplx<T>>());
+          hipLaunchKernelGGL((k_times_q<T, true>), dim3((n + 255) / 256, M_), dim3(256), 0,
+                             stream_, gP_.as<cplx<T>>(), (const cplx<T>*)q_.as<cplx<T>>(), n);
+        });
+      }
+      if (uo) {
+        run(K_OBJ_UPDATE, [&] {
+          hipLaunchKernelGGL(k_lsqml_obj_dir<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                             gO_.as<cplx<T>>(), (const cplx<T>*)wall, carried_.as<cplx<T>>(),
+                             N, M_, corner_.as<int2>(), iota_.as<int>(), list_all_.as<int>(),
+                             count_all_.as<int>(), int(nframe_), int(nyo_), int(nxo_), ntx_,
+                             (const T*)nullptr, (const T*)nullptr, 1, (const cplx<T>*)nullptr,
+                             0, 0);
+        });
+      }
+      // Polak–Ribière β over both arrays, then the direction; h keeps this cycle's g.
+      run(K_OTHER, [&] {
+        if (ic > 0) {
+          zero(cg_sums_.as<double>(), 2);
+          if (uo) {
+            hipLaunchKernelGGL(k_cg_dots<T>, dim3(grid_o), dim3(THREADS), 0, stream_,
+                               gO_.as<cplx<T>>(), hO_.as<cplx<T>>(), no, cg_part_.as<double>());
+            hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(THREADS), 0, stream_,
+                               cg_part_.as<double>(), grid_o, cg_sums_.as<double>(), 1);
+          }
+          if (up) {
+            hipLaunchKernelGGL(k_cg_dots<T>, dim3(grid_p), dim3(THREADS), 0, stream_,
+                               gP_.as<cplx<T>>(), hP_.as<cplx<T>>(), np, cg_part_.as<double>());
+            hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(THREADS), 0, stream_,
+                               cg_part_.as<double>(), grid_p, cg_sums_.as<double>(), 1);
+          }
+        }
+        if (uo)
+          hipLaunchKernelGGL(k_cg_direction<T>, dim3((no + 255) / 256), dim3(256), 0, stream_,
+                             dOc_.as<cplx<T>>(), hO_.as<cplx<T>>(), gO_.as<cplx<T>>(), no,
+                             cg_sums_.as<double>(), int(ic == 0));
+        if (up)
+          hipLaunchKernelGGL(k_cg_direction<T>, dim3((np + 255) / 256), dim3(256), 0, stream_,
+                             dPc_.as<cplx<T>>(), hP_.as<cplx<T>>(), gP_.as<cplx<T>>(), np,
+                             cg_sums_.as<double>(), int(ic == 0));
+        hipLaunchKernelGGL(k_probe_terms<T>, dim3((np + 255) / 256), dim3(256), 0, stream_,
+                           dPc_.as<cplx<T>>(), q_.as<cplx<T>>(), dWt_.as<cplx<T>>(), n, int(np));
+      });
+      // Newton step γ along d with the moved waves of P·q and of dP·q.
+      run(K_PROBE, [&] { probe_spectra(dWt_.as<cplx<T>>()); });
+      const size_t field = stack * M_ * n;
+      for (int s = 0; s < nstack; ++s) {
+        const int f0 = int(s * stack);
+        const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+        const int* ord = iota_.as<int>() + f0;
+        moved_probes(ord, nb, false, inc0_.as<cplx<T>>(), nullptr, true);
+        run(K_PASS_A, [&] {
+          hipLaunchKernelGGL((k_pass_a3<T, N>), dim3(S::NBLK, nb * M_), dim3(S::NT), 0, stream_,
+                             O_.as<cplx<T>>(), dOc_.as<cplx<T>>(), int(nxo_),
+                             (const cplx<T>*)(wall + size_t(f0) * vplane),
+                             (const cplx<T>*)inc0_.as<cplx<T>>(), corner_.as<int2>(), ord, M_,
+                             field, psi3_.as<cplx<T>>(), tw_.as<cplx<T>>(), 1);
+        });
+        run(K_PASS_B, [&] {
+          if (angular_) pass_t<false>(psi3_.as<cplx<T>>(), nb * M_, 3, field);
+          const T norm = T(1) / (angular_ ? T(n) * T(n) : T(n));
+          auto go = [&](auto col, auto row) {
+            hipLaunchKernelGGL(angular_ ? row : col, dim3(SC::NBLK, nb), dim3(SC::NT), 0,
+                               stream_, psi3_.as<cplx<T>>(), field, counts_.as<float>(), ord,
+                               gam_part_.as<double>() + size_t(f0) * SC::NBLK * 2,
+                               tw_.as<cplx<T>>(), norm);
+          };
+          with_modes([&](auto mm) {
+            constexpr int MM = decltype(mm)::value;
+            go(k_pass_b_gamma<T, N, MM, false>, k_pass_b_gamma<T, N, MM, true>);
+          });
+        });
+      }
+      run(K_OTHER, [&] {
+        hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(THREADS), 0, stream_,
+                           gam_part_.as<double>(), int(nframe_ * SC::NBLK), gam_.as<double>(), 0);
+        if (uo)
+          hipLaunchKernelGGL(k_cg_step<T>, dim3((no + 255) / 256), dim3(256), 0, stream_,
+                             O_.as<cplx<T>>(), dOc_.as<cplx<T>>(), gO_.as<cplx<T>>(), no,
+                             gam_.as<double>());
+        if (up)
+          hipLaunchKernelGGL(k_cg_step<T>, dim3((np + 255) / 256), dim3(256), 0, stream_,
+                             P_.as<cplx<T>>(), dPc_.as<cplx<T>>(), gP_.as<cplx<T>>(), np,
+                             gam_.as<double>());
+        hipLaunchKernelGGL(k_probe_terms<T>, dim3((np + 255) / 256), dim3(256), 0, stream_,
+                           P_.as<cplx<T>>(), q_.as<cplx<T>>(), Wt_.as<cplx<T>>(), n, int(np));
+        if (pos_refine_)
+          hipLaunchKernelGGL(k_pos_step_all, dim3(1), dim3(1024), 0, stream_,
+                             (const double*)pos_all_.as<double>(), xb, int(nframe_),
+                             offs_.as<double2>(), pstep_.as<double>());
+      });
+      });
+      ++cycle;
+    }
+    drain(due, PTYRS_ML);
+    if (p.zero_phase_ramp) device_zero_phase_ramp();
+    synchronize();
+    flush_timing();
+  }
+
  private:
   static constexpr size_t kDefaultStack = 128;
+  // Views per frame (probe modes × object modes) the passes take: each is compiled per count.
+  static constexpr int kMaxModes = 8;
+  static std::string modes_error() {
+    return "the GPU engine runs at most " + std::to_string(kMaxModes) +
+           " modes (probe modes × object modes) per frame";
+  }
+  // Calls f with the views per frame as a compile-time constant.
+  template <typename F>
+  void with_modes(F&& f) {
+    switch (M_) {
+      case 1: f(std::integral_constant<int, 1>{}); break;
+      case 2: f(std::integral_constant<int, 2>{}); break;
+      case 3: f(std::integral_constant<int, 3>{}); break;
+      case 4: f(std::integral_constant<int, 4>{}); break;
+      case 5: f(std::integral_constant<int, 5>{}); break;
+      case 6: f(std::integral_constant<int, 6>{}); break;
+      case 7: f(std::integral_constant<int, 7>{}); break;
+      case 8: f(std::integral_constant<int, 8>{}); break;
+      default: throw Invalid(modes_error());
+    }
+  }
   int M_ = 1;
   double scale_ = 1.0, max_frame_total_ = 0.0, total_ = 0.0;
   // Angular-spectrum exit: the transfer function T on the unshifted grid, and whether it drops
@@ -983,27 +1266,36 @@ class Engine : public EngineBase {
   // (ptyrs/src/engine/features.rs:386-429, lsqml.rs:483-536), into inc0_.
   // With conj_q false (one slice) the planes keep q: W = DFT⁻¹(DFT(P·q)·exp(−2πi f·d_b)),
   // the views' waves.
-  void moved_probes(const int* ord, int nb, bool conj_q = true) {
+  // `src` (M planes, nullable) replaces the probe terms (the ML direction's dP·q); `out`
+  // (nullable) replaces inc0_; with `spectra` the spectra of `src` are already in spec_.
+  void moved_probes(const int* ord, int nb, bool conj_q = true, cplx<T>* out = nullptr,
+                    const cplx<T>* src = nullptr, bool spectra = false) {
     const int K = nb * M_;
-    const size_t pn = size_t(M_) * n;
+    if (out == nullptr) out = inc0_.as<cplx<T>>();
     run(K_PROBE, [&] {
-      PTYRS_HIP(hipMemcpyAsync(spec_.p, nslice_ == 1 ? Wt_.p : P_.p, sizeof(cplx<T>) * pn,
-                               hipMemcpyDeviceToDevice, stream_));
-      hipLaunchKernelGGL((k_rows<T, N, false>), dim3(S::NBLK, M_), dim3(S::NT), 0, stream_,
-                         spec_.as<cplx<T>>(), tw_.as<cplx<T>>());
-      hipLaunchKernelGGL((k_cols<T, N, false>), dim3(SC::NBLK, M_), dim3(SC::NT), 0, stream_,
-                         spec_.as<cplx<T>>(), tw_.as<cplx<T>>());
-      hipLaunchKernelGGL(k_ramp_planes<T>, dim3((n + 255) / 256, K), dim3(256), 0, stream_,
-                         inc0_.as<cplx<T>>(), (const cplx<T>*)spec_.as<cplx<T>>(),
+      if (!spectra) probe_spectra(src);
+      hipLaunchKernelGGL(k_ramp_planes<T>, dim3((n + 255) / 256, K), dim3(256), 0, stream_, out,
+                         (const cplx<T>*)spec_.as<cplx<T>>(),
                          (const double2*)offs_.as<double2>(), ord, N, M_, 1.0 / double(n));
-      hipLaunchKernelGGL((k_cols<T, N, true>), dim3(SC::NBLK, K), dim3(SC::NT), 0, stream_,
-                         inc0_.as<cplx<T>>(), tw_.as<cplx<T>>());
-      hipLaunchKernelGGL((k_rows<T, N, true>), dim3(S::NBLK, K), dim3(S::NT), 0, stream_,
-                         inc0_.as<cplx<T>>(), tw_.as<cplx<T>>());
+      hipLaunchKernelGGL((k_cols<T, N, true>), dim3(SC::NBLK, K), dim3(SC::NT), 0, stream_, out,
+                         tw_.as<cplx<T>>());
+      hipLaunchKernelGGL((k_rows<T, N, true>), dim3(S::NBLK, K), dim3(S::NT), 0, stream_, out,
+                         tw_.as<cplx<T>>());
       if (nslice_ == 1 && conj_q)
         hipLaunchKernelGGL((k_times_q<T, true>), dim3((n + 255) / 256, K), dim3(256), 0, stream_,
-                           inc0_.as<cplx<T>>(), (const cplx<T>*)q_.as<cplx<T>>(), n);
+                           out, (const cplx<T>*)q_.as<cplx<T>>(), n);
     });
+  }
+  // The unnormalised spectra of the probe terms (or of `src`, M planes) into spec_.
+  void probe_spectra(const cplx<T>* src = nullptr) {
+    const size_t pn = size_t(M_) * n;
+    const void* from = src ? (const void*)src : (nslice_ == 1 ? Wt_.p : P_.p);
+    PTYRS_HIP(hipMemcpyAsync(spec_.p, from, sizeof(cplx<T>) * pn, hipMemcpyDeviceToDevice,
+                             stream_));
+    hipLaunchKernelGGL((k_rows<T, N, false>), dim3(S::NBLK, M_), dim3(S::NT), 0, stream_,
+                       spec_.as<cplx<T>>(), tw_.as<cplx<T>>());
+    hipLaunchKernelGGL((k_cols<T, N, false>), dim3(SC::NBLK, M_), dim3(SC::NT), 0, stream_,
+                       spec_.as<cplx<T>>(), tw_.as<cplx<T>>());
   }
 
   // The wave entering slice 0 (P per mode, or the moved planes per view) and the last slice's
@@ -1168,6 +1460,36 @@ class Engine : public EngineBase {
                        stream_, psi, field, tf_.as<cplx<T>>(), tw_.as<cplx<T>>());
   }
   DevBuf psi3_, gO_, hO_, dOc_, gP_, hP_, dPc_, dWt_, cg_part_, cg_sums_, gam_part_, gam_;
+  // GpuOptions::psi_f16: DM's and RAAR's carried waves and RAAR's iterate as f16 pairs times
+  // half_scale_ (complex64 engines only).
+  bool psi_f16_ = false;
+  float half_scale_ = 1.f;
+  DevBuf carried16_, iter16_;
+  // The scale of f16 carried waves for frames whose counts sum to `max_frame_sum` at most: the
+  // largest power of two s with s²·2^ceil(log2 sum) ≤ 2^30, i.e. s·√sum ≤ 2^15, half the
+  // largest finite half (a projected frame's components are within √sum; the factor two is
+  // for DM's and RAAR's combinations). ptyrs/src/gpu/psi.rs:40-50
+  static float half_scale(double max_frame_sum) {
+    const uint64_t sum = uint64_t(std::ceil(std::max(max_frame_sum, 1.0)));
+    int log2 = 0;
+    while (log2 < 64 && (uint64_t(1) << log2) < sum) ++log2;  // ceil(log2(sum)), 0 for 1
+    const int k = 30 - log2;
+    const int e = k >= 0 ? k / 2 : -((-k + 1) / 2);  // floor(k / 2)
+    return std::ldexp(1.0f, e);
+  }
+  // Calls f with the carried waves (and RAAR's iterate) in the engine's storage form.
+  template <typename F>
+  void with_carried(bool half, F&& f) {
+    if constexpr (sizeof(T) == 4) {
+      if (half) {
+        const float us = 1.0f / half_scale_;
+        f(Carried<T, true>{carried16_.as<uint32_t>(), half_scale_, us},
+          Carried<T, true>{iter16_.as<uint32_t>(), half_scale_, us});
+        return;
+      }
+    }
+    f(Carried<T, false>{carried_.as<cplx<T>>()}, Carried<T, false>{iter_.as<cplx<T>>()});
+  }
   DevBuf carried_, iter_, list_all_, count_all_, pnum_part_, pden_part_, pnum_, pden_,
       pmaxpart_, pmax2_, abs_part_;
   DevBuf dO_, pre_, illum_, afar_all_, afar_tmp_, prev_, icalc_, xi_part_, xi_part2_, dots_part_,
@@ -1201,10 +1523,10 @@ class Engine : public EngineBase {
 
   static constexpr size_t kIllumChunk = 1024;
   // rPIE's pass C cuts a batch into chunks of slots so that its grid has about this many
-  // workgroups (each chunk leaves one probe-direction partial per mode). At 256² × 4 modes, 512
-  // is one round at two workgroups per CU: 0.98 ms per epoch, against 1.03 at 1024 and 1.73
-  // at 256.
-  static constexpr int kPassCBlocks = 512;
+  // workgroups (each chunk leaves one probe-direction partial per mode). At 256² × 4 modes pass
+  // C holds 150 VGPRs and 53 KB of LDS, three workgroups per CU: 768 is one round, 73 µs per
+  // launch against 87 at 512 and 85 at 1024.
+  static constexpr int kPassCBlocks = 768;
   size_t illum_chunk() const { return std::max<size_t>(1, std::min(nframe_, kIllumChunk)); }
 
   void ensure_lsqml(size_t cap) {
@@ -1359,7 +1681,7 @@ class Engine : public EngineBase {
   // ptyrs/src/engine/multislice.rs:266-442
   void grad(int cost, double* value, double* og, double* pg) override {
     if (nobj_ != 1) throw Invalid("the GPU gradient runs one object mode");
-    if (M_ > 4) throw Invalid("the GPU engine runs at most 4 modes");
+    if (M_ > kMaxModes) throw Invalid(modes_error());
     if (pos_on_) throw Invalid("the GPU gradient models no position offsets");
     if (cost != PTYRS_POISSON && cost != PTYRS_AMPLITUDE)
       throw Invalid("cost must be amplitude or poisson");
@@ -1435,7 +1757,7 @@ class Engine : public EngineBase {
     if (pos_on_ && sampling != PTYRS_PATCH)
       throw Invalid("position offsets under FullObject sampling are not run on this GPU "
                     "engine; set the PATCH sampling");
-    if (M_ > 4) throw Invalid("the GPU engine runs at most 4 modes");
+    if (M_ > kMaxModes) throw Invalid(modes_error());
     if (nframe_ == 0) return;
     const size_t stack = std::min<size_t>(nframe_, kDefaultStack);
     ensure_workspace(stack, 1);
@@ -1668,7 +1990,8 @@ class Engine : public EngineBase {
   }
 
   // Object update of AP/DM/RAAR: tile lists over every frame, norm = Σ_m D_m, its max, blend.
-  void object_blend(double inertia) {
+  template <bool H>
+  void object_blend(double inertia, Carried<T, H> cw) {
     const int nt = ntiles();
     run(K_BIN, [&] {
       norm2_planes();
@@ -1684,24 +2007,25 @@ class Engine : public EngineBase {
                          tilemax_.as<T>(), nt, pmax_.as<T>());
     });
     run(K_OBJ_UPDATE, [&] {
-      hipLaunchKernelGGL(k_obj_blend<T>, dim3(nt), dim3(THREADS), 0, stream_,
-                         O_.as<cplx<T>>(), Wt_.as<cplx<T>>(), D_.as<T>(),
-                         carried_.as<cplx<T>>(), N, M_, corner_.as<int2>(), iota_.as<int>(),
+      hipLaunchKernelGGL((k_obj_blend<T, H>), dim3(nt), dim3(THREADS), 0, stream_,
+                         O_.as<cplx<T>>(), Wt_.as<cplx<T>>(), D_.as<T>(), cw, N, M_,
+                         corner_.as<int2>(), iota_.as<int>(),
                          list_all_.as<int>(), count_all_.as<int>(), int(nframe_), int(nyo_),
                          int(nxo_), ntx_, pmax_.as<T>(), T(inertia));
     });
   }
 
   // Probe sums of the carried waves against the current object (DM, RAAR).
-  void probe_sums_all(size_t stack, int nstack, int nchunk) {
+  template <bool H>
+  void probe_sums_all(size_t stack, int nstack, int nchunk, Carried<T, H> cw) {
     for (int s = 0; s < nstack; ++s) {
       const int f0 = int(s * stack);
       const int nb = std::min<int>(int(stack), int(nframe_) - f0);
       const int ck = chunk_of(nb);
       const int nck = (nb + ck - 1) / ck;
       run(K_PROBE, [&] {
-        hipLaunchKernelGGL(k_probe_sums<T>, dim3((n + THREADS - 1) / THREADS, M_, nck),
-                           dim3(THREADS), 0, stream_, carried_.as<cplx<T>>(), O_.as<cplx<T>>(),
+        hipLaunchKernelGGL((k_probe_sums<T, H>), dim3((n + THREADS - 1) / THREADS, M_, nck),
+                           dim3(THREADS), 0, stream_, cw, O_.as<cplx<T>>(),
                            int(nxo_), q_.as<cplx<T>>(), corner_.as<int2>(),
                            iota_.as<int>() + f0, nb, N, M_, ck,
                            pnum_part_.as<cplx<T>>() + size_t(s) * nchunk * M_ * n,
@@ -1874,8 +2198,9 @@ class Engine : public EngineBase {
 
   // Pass A; `waves` (nullable) gives every view its own wave (plane view of it) instead of
   // the probe terms Wt.
+  template <bool H = false>
   void pass_a(const int* ord, int nb, bool omax,
-              MixArgs<T> mix = MixArgs<T>{MIX_NONE, nullptr, nullptr, T(0), T(0)},
+              MixArgs<T, H> mix = MixArgs<T, H>{MIX_NONE, {}, {}, T(0), T(0)},
               const cplx<T>* waves = nullptr) {
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3(S::NBLK, nb * M_), dim3(S::NT), 0, stream_,
@@ -1885,10 +2210,10 @@ class Engine : public EngineBase {
     };
     run(K_PASS_A, [&] {
       switch (mix.kind) {
-        case MIX_DM: go(k_pass_a<T, N, MIX_DM>); break;
-        case MIX_RAAR: go(k_pass_a<T, N, MIX_RAAR>); break;
-        case MIX_RAAR0: go(k_pass_a<T, N, MIX_RAAR0>); break;
-        default: go(k_pass_a<T, N, MIX_NONE>); break;
+        case MIX_DM: go(k_pass_a<T, N, MIX_DM, H>); break;
+        case MIX_RAAR: go(k_pass_a<T, N, MIX_RAAR, H>); break;
+        case MIX_RAAR0: go(k_pass_a<T, N, MIX_RAAR0, H>); break;
+        default: go(k_pass_a<T, N, MIX_NONE, H>); break;
       }
     });
   }
@@ -1914,13 +2239,7 @@ class Engine : public EngineBase {
                            counts_.as<float>(), ord, llk_parts(), tw_.as<cplx<T>>(),
                            ba, norm);
       };
-      switch (M_) {
-        case 1: args(std::integral_constant<int, 1>{}); break;
-        case 2: args(std::integral_constant<int, 2>{}); break;
-        case 3: args(std::integral_constant<int, 3>{}); break;
-        case 4: args(std::integral_constant<int, 4>{}); break;
-        default: throw Invalid("the GPU engine runs at most 4 modes");
-      }
+      with_modes(args);
       if (angular_ && back) pass_t<true>(stack(), nb * M_);
     });
   }
@@ -2010,12 +2329,7 @@ class Engine : public EngineBase {
                            (const T*)D_.as<T>(), int(nyo_), (const T*)pmax_.as<T>(),
                            T(p.obj_alpha), 0, (cplx<T>*)nullptr);
       };
-      switch (M_) {
-        case 1: launch(std::integral_constant<int, 1>{}); break;
-        case 2: launch(std::integral_constant<int, 2>{}); break;
-        case 3: launch(std::integral_constant<int, 3>{}); break;
-        default: launch(std::integral_constant<int, 4>{}); break;
-      }
+      with_modes(launch);
     });
     if (upd_obj && upd_probe) {
       const int total = int(nprobe_ * n);
@@ -2075,7 +2389,7 @@ class Engine : public EngineBase {
                            tilemax_.as<T>(), nt, pmax_.as<T>());
       });
     }
-    pass_a(ord, nb, upd_probe, MixArgs<T>{MIX_NONE, nullptr, nullptr, T(0), T(0)}, W);
+    pass_a(ord, nb, upd_probe, MixArgs<T>{MIX_NONE, {}, {}, T(0), T(0)}, W);
     pass_b<B_RPIE>(ord, nb);
     cplx<T>* chi0 = pos_refine_ ? gspec_.as<cplx<T>>() : nullptr;
     run(K_PASS_C, [&] {
@@ -2090,12 +2404,7 @@ class Engine : public EngineBase {
                            (const T*)D_.as<T>(), int(nyo_), (const T*)pmax_.as<T>(),
                            T(p.obj_alpha), 1, chi0);
       };
-      switch (M_) {
-        case 1: launch(std::integral_constant<int, 1>{}); break;
-        case 2: launch(std::integral_constant<int, 2>{}); break;
-        case 3: launch(std::integral_constant<int, 3>{}); break;
-        default: launch(std::integral_constant<int, 4>{}); break;
-      }
+      with_modes(launch);
     });
     if (pos_refine_) position_sums(0, W, nullptr, 0, ord, nb, chi0);
     if (upd_obj) {

@@ -194,20 +194,53 @@ __global__ void k_norm2_planes(const cplx<T>* __restrict__ Wt, T* __restrict__ W
 // they carry across cycles (ptyrs/src/engine/ptycho.rs:643-731).
 enum Mix { MIX_NONE = 0, MIX_DM = 1, MIX_RAAR = 2, MIX_RAAR0 = 3 };
 
+// Exit waves carried across cycles, value i of a [nframe][M][N][N] array: complex T, or with
+// H (GpuOptions::psi_f16) one pair of IEEE halves per value, the real part in the low half,
+// holding the value times `scale`, a power of two whose inverse `unscale` is exact, so the
+// format adds f16 rounding and nothing else. A component beyond the f16 range is stored as
+// the largest finite half of its sign; a NaN stays one. ptyrs-kernels/src/pixel.rs:360-442
+template <typename T, bool H>
+struct Carried {
+  static constexpr bool half = false;
+  cplx<T>* p;
+  __device__ cplx<T> get(size_t i) const { return ld(p + i); }
+  __device__ void put(size_t i, cplx<T> v) const { st(p + i, v); }
+};
 template <typename T>
-struct MixArgs {
-  int kind;
-  const cplx<T>* carried;  // [nframe][M][N][N]: DM's and RAAR's ψ
-  cplx<T>* iterate;        // [nframe][M][N][N]: RAAR's iterate, rewritten with the mix
-  T c0, c1;                // DM: (2 − a, 1 − a); RAAR: (1 − β/2, β/2)
+struct Carried<T, true> {
+  static constexpr bool half = true;
+  uint32_t* p;
+  float scale, unscale;
+  __device__ cplx<T> get(size_t i) const {
+    const uint32_t w = p[i];
+    const _Float16 re = __builtin_bit_cast(_Float16, uint16_t(w & 0xffffu));
+    const _Float16 im = __builtin_bit_cast(_Float16, uint16_t(w >> 16));
+    return cplx<T>{T(float(re) * unscale), T(float(im) * unscale)};
+  }
+  __device__ void put(size_t i, cplx<T> v) const {
+    // Comparisons keep a NaN (fminf/fmaxf would return the bound).
+    auto sat = [](float x) { return x > 65504.f ? 65504.f : (x < -65504.f ? -65504.f : x); };
+    const _Float16 re = _Float16(sat(float(v.x) * scale));
+    const _Float16 im = _Float16(sat(float(v.y) * scale));
+    p[i] = uint32_t(__builtin_bit_cast(uint16_t, re)) |
+           (uint32_t(__builtin_bit_cast(uint16_t, im)) << 16);
+  }
 };
 
-template <typename T, int N, int MIX>
+template <typename T, bool H = false>
+struct MixArgs {
+  int kind;
+  Carried<T, H> carried;  // DM's and RAAR's ψ
+  Carried<T, H> iterate;  // RAAR's iterate, rewritten with the mix
+  T c0, c1;               // DM: (2 − a, 1 − a); RAAR: (1 − β/2, β/2)
+};
+
+template <typename T, int N, int MIX, bool H = false>
 __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a(
     const cplx<T>* __restrict__ O, int nxo, const cplx<T>* __restrict__ Wt,
     const int2* __restrict__ corner, const int* __restrict__ order, int M,
     cplx<T>* __restrict__ psi, T* __restrict__ omax_part, const cplx<T>* __restrict__ tw,
-    MixArgs<T> mix, int w_view) {
+    MixArgs<T, H> mix, int w_view) {
   using S = RowShape<T, N>;
   constexpr int E = S::E, TL = S::TL;
   __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
@@ -230,13 +263,13 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a(
     cplx<T> o = ld(orow + x);
     v[e] = ld(wrow + x) * o;
     if constexpr (MIX == MIX_DM) {
-      v[e] = v[e] * mix.c0 - ld(mix.carried + crow + x) * mix.c1;
+      v[e] = v[e] * mix.c0 - mix.carried.get(crow + x) * mix.c1;
     } else if constexpr (MIX == MIX_RAAR) {
-      const cplx<T> it = ld(mix.iterate + crow + x);
-      v[e] = ld(mix.carried + crow + x) * mix.c0 + (v[e] * T(2) - it) * mix.c1;
-      st(mix.iterate + crow + x, v[e]);
+      const cplx<T> it = mix.iterate.get(crow + x);
+      v[e] = mix.carried.get(crow + x) * mix.c0 + (v[e] * T(2) - it) * mix.c1;
+      mix.iterate.put(crow + x, v[e]);
     } else if constexpr (MIX == MIX_RAAR0) {
-      st(mix.iterate + crow + x, v[e]);  // RAAR's first cycle: ψ = PO, iterate = ψ
+      mix.iterate.put(crow + x, v[e]);  // RAAR's first cycle: ψ = PO, iterate = ψ
     }
     T o2 = norm2(o);
     mx = o2 > mx ? o2 : mx;
@@ -605,7 +638,12 @@ __global__ __launch_bounds__((pass_c_sum_threads<T, N, M>())) void k_pass_c_rpie
   constexpr int LDW = FITS ? S::LDS_WORDS : M * G * PLD;
   __shared__ T lre[LDW], lim[LDW];
   __shared__ cplx<T> stw[N];
-  __shared__ cplx<T> terms[2 * TW];
+  // Two term buffers (slot b in buffer b mod 2) where they fit in LDS, else one, with a
+  // barrier between a slot's sums and the next slot's terms.
+  constexpr bool DB = 2 * LDW * sizeof(T) + N * sizeof(cplx<T>) + 2 * TW * sizeof(cplx<T>) <=
+                      160 * 1024;
+  constexpr int NBUF = DB ? 2 : 1;
+  __shared__ cplx<T> terms[NBUF * TW];
   const int tid = threadIdx.x;
   const int m = wave_uniform<G * TL>(tid / (G * TL)), l = (tid / TL) % G, jl = tid % TL;
   const int line = m * G + l;
@@ -617,12 +655,11 @@ __global__ __launch_bounds__((pass_c_sum_threads<T, N, M>())) void k_pass_c_rpie
   // Every slot's transform uses the lane's twiddles: held in registers.
   cplx<T> twc[tw_cache_size(N, E)];
   load_tw_cache<T, N, E>(twc, jl, stw);
-  cplx<T> w[E], qq[E], acc[E];
+  cplx<T> w[E], acc[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int x = jl + TL * e;
     w[e] = ld(Wt + (size_t)(w_view ? b0 * M + m : m) * n + (size_t)y * N + x);
-    qq[e] = ld(q + (size_t)y * N + x);
     acc[e] = cplx<T>{0, 0};
   }
   const T inv_p = T(1) - probe_alpha, inv_o = T(1) - obj_alpha;
@@ -641,39 +678,22 @@ __global__ __launch_bounds__((pass_c_sum_threads<T, N, M>())) void k_pass_c_rpie
       st(out + TL * e, sum);
     }
   };
-  // The next slot's rows (field, object window, overlap) are loaded before this slot's
-  // transform.
-  cplx<T> nxt[E], onx[E];
-  T dnx[E];
-  if (b0 < b1) {
-    const cplx<T>* p0 = psi + ((size_t)(b0 * M + m) * N + y) * N;
-    const int2 c0 = corner[order[b0]];
-    const size_t r0 = (size_t)(c0.x + y) * nxo + c0.y;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      nxt[e] = ld(p0 + jl + TL * e);
-      onx[e] = ld(O + r0 + jl + TL * e);
-      dnx[e] = upd_obj ? D[m * no + r0 + jl + TL * e] : T(0);
-    }
-  }
+  // Each slot's rows (field, object window, overlap) are loaded as the slot starts: held for
+  // the next slot, they would cost a workgroup per CU (two instead of three), which hides the
+  // loads' latency better than the prefetch did. The probe direction omits conj(q) until the
+  // store (q is the same at every slot).
   for (int b = b0; b < b1; ++b) {
     cplx<T> v[E], ow[E];
     T dw[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      v[e] = nxt[e];
-      ow[e] = onx[e];
-      dw[e] = dnx[e];
-    }
-    if (b + 1 < b1) {
-      const cplx<T>* pn = psi + ((size_t)((b + 1) * M + m) * N + y) * N;
-      const int2 cn = corner[order[b + 1]];
+    {
+      const cplx<T>* pn = psi + ((size_t)(b * M + m) * N + y) * N;
+      const int2 cn = corner[order[b]];
       const size_t rn = (size_t)(cn.x + y) * nxo + cn.y;
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        nxt[e] = ld(pn + jl + TL * e);
-        onx[e] = ld(O + rn + jl + TL * e);
-        dnx[e] = upd_obj ? D[m * no + rn + jl + TL * e] : T(0);
+        v[e] = ld(pn + jl + TL * e);
+        ow[e] = ld(O + rn + jl + TL * e);
+        dw[e] = upd_obj ? D[m * no + rn + jl + TL * e] : T(0);
       }
     }
     T omax = 0;
@@ -687,8 +707,11 @@ __global__ __launch_bounds__((pass_c_sum_threads<T, N, M>())) void k_pass_c_rpie
     // below, and this slot's reads of buffer (b − 1) before the writes of slot b + 1.
     const int lb = FITS ? S::base_row(line) : line * PLD;
     line_fft<T, N, E, true, At, true>(v, jl, lre + lb, lim + lb, stw, twc);
-    if (upd_obj && b > b0) store_sum(b - 1, (b - 1) & 1);
-    cplx<T>* trow = terms + (b & 1) * TW + line * N + jl;
+    if (upd_obj && b > b0) {
+      store_sum(b - 1, DB ? (b - 1) & 1 : 0);
+      if constexpr (!DB) __syncthreads();
+    }
+    cplx<T>* trow = terms + (DB ? b & 1 : 0) * TW + line * N + jl;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const cplx<T> o = ow[e];
@@ -696,17 +719,18 @@ __global__ __launch_bounds__((pass_c_sum_threads<T, N, M>())) void k_pass_c_rpie
       if (chi0 != nullptr && m == 0) st(chi0 + ((size_t)b * M * N + y) * N + jl + TL * e, d);
       if (upd_obj) trow[TL * e] = cmulc(w[e], d) * rcp(inv_o * dw[e] + reg);
       if (probe_dir)
-        acc[e] = acc[e] + cmulc(o * qq[e], d) * rcp(inv_p * norm2(o) + probe_alpha * omax);
+        acc[e] = acc[e] + cmulc(o, d) * rcp(inv_p * norm2(o) + probe_alpha * omax);
     }
   }
   if (upd_obj && b1 > b0) {
     __syncthreads();
-    store_sum(b1 - 1, (b1 - 1) & 1);
+    store_sum(b1 - 1, DB ? (b1 - 1) & 1 : 0);
   }
   if (probe_dir) {
     cplx<T>* out = dP_part + ((size_t)ch * M + m) * n + (size_t)y * N;
 #pragma unroll
-    for (int e = 0; e < E; ++e) st(out + jl + TL * e, acc[e]);
+    for (int e = 0; e < E; ++e)
+      st(out + jl + TL * e, cmulc(ld(q + (size_t)y * N + jl + TL * e), acc[e]));
   }
 }
 
@@ -798,14 +822,19 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
   const int nl = base_s;
   if (tid == 0) count[t] = nl;
   if (W2 == nullptr && Wv == nullptr) return;
+  // Modes in chunks of MM, each chunk's overlaps in registers; Σ_m D_m in mode order.
   constexpr int MM = 4;
   const int X = tx0 + tid % TILE_X, Y0 = ty0 + tid / TILE_X;
   constexpr int RS = THREADS / TILE_X;  // row stride between a thread's pixels
   const int n = N * N;
+  const size_t no = (size_t)nyo * nxo;
+  T tot[TROWS] = {};
+  for (int m0 = 0; m0 < M; m0 += MM) {
+  const int mc = M - m0 < MM ? M - m0 : MM;
   T dm[TROWS][MM] = {};
   for (int s = 0; s < nl; s += THREADS) {
     const int k = nl - s < THREADS ? nl - s : THREADS;
-    if (s > 0) {
+    if (s > 0 || (m0 > 0 && nl > THREADS)) {
       if (tid < k) {
         bs[tid] = out[s + tid];
         cs[tid] = corner[order[bs[tid]]];
@@ -830,9 +859,9 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
             if (ok[u]) {
 #pragma unroll
               for (int m = 0; m < MM; ++m)
-                if (m < M)
-                  w2[u][m] = Wv ? norm2(Wv[((size_t)bs[i] * M + m) * n + y * N + x])
-                                : W2[(size_t)m * n + y * N + x];
+                if (m < mc)
+                  w2[u][m] = Wv ? norm2(Wv[((size_t)bs[i] * M + m0 + m) * n + y * N + x])
+                                : W2[(size_t)(m0 + m) * n + y * N + x];
             }
           }
 #pragma unroll
@@ -840,16 +869,14 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
             if (ok[u]) {
 #pragma unroll
               for (int m = 0; m < MM; ++m)
-                if (m < M) dm[r][m] += w2[u][m];
+                if (m < mc) dm[r][m] += w2[u][m];
             }
         }
       }
     }
     __syncthreads();
   }
-  T sum = 0;
   if (X < nxo && nl > 0) {
-    const size_t no = (size_t)nyo * nxo;
 #pragma unroll
     for (int r = 0; r < TROWS; ++r) {
       const int Y = Y0 + RS * r;
@@ -858,12 +885,19 @@ __global__ __launch_bounds__(THREADS) void k_bin_tiles(
       T s = 0;
 #pragma unroll
       for (int m = 0; m < MM; ++m)
-        if (m < M) {
-          D[m * no + p] = dm[r][m];
+        if (m < mc) {
+          D[(m0 + m) * no + p] = dm[r][m];
           s += dm[r][m];
         }
-      sum = s > sum ? s : sum;
+      tot[r] += s;
     }
+  }
+  }
+  T sum = 0;
+  if (X < nxo && nl > 0) {
+#pragma unroll
+    for (int r = 0; r < TROWS; ++r)
+      if (Y0 + RS * r < nyo) sum = tot[r] > sum ? tot[r] : sum;
   }
   T r = block_reduce<THREADS>(sum, red, MaxOp());
   if (tid == 0) tilemax[t] = r;
@@ -1661,10 +1695,11 @@ __global__ void k_ramp_planes(cplx<T>* __restrict__ out, const cplx<T>* __restri
 
 // The frame mean of spectra moved back: out[m] = scale·Σ_b G[b·M + m]·exp(+2πi f·d_b), slots in
 // order (Kahan). Grid (ceil(n/256), M). ptyrs/src/engine/lsqml.rs:207-229
+// With `accumulate` the result is added to out (a sum over stacks, in stack order).
 template <typename T>
 __global__ void k_unramp_mean(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ G,
                               const double2* __restrict__ offs, const int* __restrict__ order,
-                              int nb, int N, int M, double scale) {
+                              int nb, int N, int M, double scale, int accumulate = 0) {
   const int n = N * N;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
@@ -1679,7 +1714,114 @@ __global__ void k_unramp_mean(cplx<T>* __restrict__ out, const cplx<T>* __restri
     sr.add(T(double(v.x) * c - double(v.y) * s));
     si.add(T(double(v.x) * s + double(v.y) * c));
   }
-  st(out + (size_t)m * n + j, cplx<T>{T(double(sr.s) * scale), T(double(si.s) * scale)});
+  cplx<T> r{T(double(sr.s) * scale), T(double(si.s) * scale)};
+  if (accumulate) r = r + ld(out + (size_t)m * n + j);
+  st(out + (size_t)m * n + j, r);
+}
+
+// The moved waves' derivatives over the offset: out[2k + a] = spec[m]·exp(−2πi f·d_b)·(−2πi
+// f_a)·scale for view k = b·M + m and axis a (0: dy, 1: dx), d_b = offs[order[b]]; their
+// inverse transforms are ∂W/∂d_a (ptyrs/src/engine/features.rs:370-384, 960-985). Grid
+// (ceil(n/256), nb·M).
+template <typename T>
+__global__ void k_ramp_deriv_planes(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ spec,
+                                    const double2* __restrict__ offs,
+                                    const int* __restrict__ order, int N, int M, double scale) {
+  const int n = N * N;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int k = blockIdx.y, b = k / M, m = k % M;
+  const double2 d = offs[order[b]];
+  const double fy = dft_freq(j / N, N), fx = dft_freq(j % N, N);
+  double s, c;
+  sincos(-2.0 * M_PI * (fy * d.x + fx * d.y), &s, &c);
+  const cplx<T> v = ld(spec + (size_t)m * n + j);
+  const double re = double(v.x) * c - double(v.y) * s, im = double(v.x) * s + double(v.y) * c;
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const double g = -2.0 * M_PI * (a == 0 ? fy : fx) * scale;  // times −2πi f_a: (re, im)·i·g
+    st(out + ((size_t)2 * k + a) * n + j, cplx<T>{T(-im * g), T(re * g)});
+  }
+}
+
+// Per slot and workgroup the position sums of AP's, DM's and ML's step over the slot's views
+// (every mode): num_a = Re Σ conj(x − W·O)·O·∂_a W and den_a = Σ |O·∂_a W|², x the projected
+// exit waves, W the moved waves ([slot·M + m][n]), ∂_a W from k_ramp_deriv_planes
+// ([2(slot·M + m) + a][n]), all natural layout. part[slot][blk] = {num_y, num_x, den_y, den_x}.
+// Grid (nblk, nb). ptyrs/src/engine/features.rs:987-1025
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_pos_sums_proj(
+    const cplx<T>* __restrict__ x, const cplx<T>* __restrict__ W, const cplx<T>* __restrict__ O,
+    int nxo, const int2* __restrict__ corner, const int* __restrict__ order,
+    const cplx<T>* __restrict__ dW, int N, int M, double* __restrict__ part) {
+  __shared__ double red[THREADS];
+  const int b = blockIdx.y;
+  const int n = N * N;
+  const int2 c = corner[order[b]];
+  double s[4] = {0, 0, 0, 0};
+  for (int j = blockIdx.x * THREADS + threadIdx.x; j < n; j += gridDim.x * THREADS) {
+    const cplx<T> o = ld(O + (size_t)(c.x + j / N) * nxo + c.y + j % N);
+    for (int m = 0; m < M; ++m) {
+      const size_t k = (size_t)b * M + m;
+      const cplx<T> r = ld(x + k * n + j) - ld(W + k * n + j) * o;
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const cplx<T> u = o * ld(dW + (2 * k + a) * n + j);
+        s[a] += double(cmulc(r, u).x);
+        s[2 + a] += double(norm2(u));
+      }
+    }
+  }
+  for (int i = 0; i < 4; ++i) {
+    const double r = block_reduce<THREADS>(s[i], red, AddOp());
+    if (threadIdx.x == 0) part[((size_t)b * gridDim.x + blockIdx.x) * 4 + i] = r;
+  }
+}
+
+// One workgroup: AP's, DM's and ML's position step over every frame from their sums (`part`,
+// nblk per frame, frames in order): per axis 0.3·num/(den + max_frames den) clipped to ±0.1
+// (0 where that denominator is 0), less its mean over the frames, added to the offsets (FP64).
+// `steps` holds 4·nframe values of scratch. ptyrs/src/engine/features.rs:1026-1047
+__global__ __launch_bounds__(1024) static void k_pos_step_all(const double* __restrict__ part,
+                                                              int nblk, int nframe,
+                                                              double2* __restrict__ offs,
+                                                              double* __restrict__ steps) {
+  __shared__ double red[1024];
+  const int tid = threadIdx.x;
+  double mden[2] = {0.0, 0.0};
+  for (int f = tid; f < nframe; f += 1024) {
+    double v[4] = {0, 0, 0, 0};
+    for (int i = 0; i < nblk; ++i)
+      for (int a = 0; a < 4; ++a) v[a] += part[((size_t)f * nblk + i) * 4 + a];
+    // num in steps[f][a], den in steps[nframe + f][a] until the max is known.
+    for (int a = 0; a < 2; ++a) {
+      steps[(size_t)f * 2 + a] = v[a];
+      steps[((size_t)nframe + f) * 2 + a] = v[2 + a];
+      mden[a] = v[2 + a] > mden[a] ? v[2 + a] : mden[a];
+    }
+  }
+  double mx[2];
+  for (int a = 0; a < 2; ++a) mx[a] = block_reduce<1024>(mden[a], red, MaxOp());
+  double sum[2] = {0, 0};
+  for (int f = tid; f < nframe; f += 1024)
+    for (int a = 0; a < 2; ++a) {
+      const double den = steps[((size_t)nframe + f) * 2 + a] + mx[a];
+      double st_ = 0.0;
+      if (den > 0.0) {
+        st_ = 0.3 * steps[(size_t)f * 2 + a] / den;
+        st_ = st_ < -0.1 ? -0.1 : (st_ > 0.1 ? 0.1 : st_);
+      }
+      steps[(size_t)f * 2 + a] = st_;
+      sum[a] += st_;
+    }
+  double mean[2];
+  for (int a = 0; a < 2; ++a) mean[a] = block_reduce<1024>(sum[a], red, AddOp()) / nframe;
+  for (int f = tid; f < nframe; f += 1024) {
+    double2 d = offs[f];
+    d.x += steps[(size_t)f * 2] - mean[0];
+    d.y += steps[(size_t)f * 2 + 1] - mean[1];
+    offs[f] = d;
+  }
 }
 
 // planes[k] *= q (CONJ: conj q) over `nplane` planes. Grid (ceil(n/256), nplane).
@@ -1831,9 +1973,9 @@ __global__ __launch_bounds__(1024) static void k_pos_step(const double* __restri
 // ψ' = Π (AP, RAAR) or Π + (1 − a)·(PO − R) with R = (2 − a)·PO − (1 − a)·ψ (DM), written to
 // carried[frame]. With `probe_sums` (AP: both updates read the old arrays) it adds
 // conj(q·O)·ψ' and |O|² of the slot's window into per-chunk partials.
-template <typename T, int N>
+template <typename T, int N, bool H = false>
 __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_proj(
-    const cplx<T>* __restrict__ psi, cplx<T>* __restrict__ carried, const cplx<T>* __restrict__ O,
+    const cplx<T>* __restrict__ psi, Carried<T, H> carried, const cplx<T>* __restrict__ O,
     int nxo, const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ q,
     const int2* __restrict__ corner, const int* __restrict__ order, int nb, int M, int chunk,
     int dm, T c0, T c1, int probe_sums, cplx<T>* __restrict__ num_part, T* __restrict__ den_part,
@@ -1869,7 +2011,7 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_proj(
     line_fft<T, N, E, true, typename S::AtRow>(v, jl, lre + S::base_row(l),
                                                lim + S::base_row(l), stw);
     const cplx<T>* orow = O + (size_t)(c.x + y) * nxo + c.y;
-    cplx<T>* crow = carried + (((size_t)f * M + m) * N + y) * N;
+    const size_t crow = (((size_t)f * M + m) * N + y) * N;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int x = jl + TL * e;
@@ -1877,10 +2019,10 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_proj(
       cplx<T> p = v[e];
       if (dm) {
         const cplx<T> po = w[e] * o;
-        const cplx<T> r = po * c0 - ld(crow + x) * c1;
+        const cplx<T> r = po * c0 - carried.get(crow + x) * c1;
         p = p + (po - r) * c1;
       }
-      st(crow + x, p);
+      carried.put(crow + x, p);
       if (probe_sums) {
         acc[e] = acc[e] + cmulc(o * qq[e], p);
         den[e] += norm2(o);
@@ -1899,9 +2041,9 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_proj(
 
 // Probe sums of carried waves against the current object (DM, RAAR: the probe update sees the
 // new object): grid (N·N/THREADS, M, nchunk), per-chunk partials of conj(q·O)·ψ and |O|².
-template <typename T>
+template <typename T, bool H = false>
 __global__ __launch_bounds__(THREADS) void k_probe_sums(
-    const cplx<T>* __restrict__ carried, const cplx<T>* __restrict__ O, int nxo,
+    Carried<T, H> carried, const cplx<T>* __restrict__ O, int nxo,
     const cplx<T>* __restrict__ q, const int2* __restrict__ corner, const int* __restrict__ order,
     int nb, int N, int M, int chunk, cplx<T>* __restrict__ num_part, T* __restrict__ den_part) {
   const int n = N * N;
@@ -1917,7 +2059,7 @@ __global__ __launch_bounds__(THREADS) void k_probe_sums(
     const int f = order[b];
     const int2 c = corner[f];
     const cplx<T> o = ld(O + (size_t)(c.x + y) * nxo + c.y + x);
-    acc = acc + cmulc(o * qq, ld(carried + ((size_t)f * M + m) * n + i));
+    acc = acc + cmulc(o * qq, carried.get(((size_t)f * M + m) * n + i));
     den += norm2(o);
   }
   st(num_part + ((size_t)ch * M + m) * n + i, acc);
@@ -1971,10 +2113,10 @@ __global__ void k_probe_blend(cplx<T>* __restrict__ P, cplx<T>* __restrict__ Wt,
 // PyNX blend of the object from tile-owned sums over every view: every pixel owned by one thread
 // sums Σ_m Σ_views conj(W_m)·ψ_m (Kahan) in view order, and O = (num + reg·O)/max(norm + reg,
 // 1e-12) with norm = Σ_m D_m from k_bin_tiles and reg = inertia·max(norm).
-template <typename T>
+template <typename T, bool H = false>
 __global__ __launch_bounds__(THREADS) void k_obj_blend(
     cplx<T>* __restrict__ O, const cplx<T>* __restrict__ Wt, const T* __restrict__ D,
-    const cplx<T>* __restrict__ carried, int N, int M, const int2* __restrict__ corner,
+    Carried<T, H> carried, int N, int M, const int2* __restrict__ corner,
     const int* __restrict__ order, const int* __restrict__ list, const int* __restrict__ count,
     int cap, int nyo, int nxo, int ntx, const T* __restrict__ nmax, T inertia) {
   __shared__ int3 cs[THREADS];
@@ -1997,9 +2139,9 @@ __global__ __launch_bounds__(THREADS) void k_obj_blend(
       const int y = Y - cs[i].y, x = X - cs[i].z;
       if (y >= 0 && y < N && x >= 0 && x < N) {
         const int j = y * N + x;
-        const cplx<T>* cv = carried + (size_t)cs[i].x * M * n + j;
+        const size_t cv = (size_t)cs[i].x * M * n + j;
         for (int m = 0; m < M; ++m) {
-          const cplx<T> g = cmulc(ld(Wt + (size_t)m * n + j), ld(cv + (size_t)m * n));
+          const cplx<T> g = cmulc(ld(Wt + (size_t)m * n + j), carried.get(cv + (size_t)m * n));
           sr.add(g.x);
           si.add(g.y);
         }
@@ -2076,13 +2218,14 @@ __global__ __launch_bounds__(THREADS) void k_balance(cplx<T>* __restrict__ O, si
 
 // ---- ML (PyNX Poisson conjugate gradient) ------------------------------------------------------------
 // Pass A of the Newton step: per view the rows of po = W·O, a = W·dO + dW·O and b = dW·dO
-// (dW = dP·q), each row-transformed, into psi3[field][slot][mode][N][N].
+// (dW = dP·q; with `w_view` W and dW are every view's own moved waves, plane slot·M + m),
+// each row-transformed, into psi3[field][slot][mode][N][N].
 template <typename T, int N>
 __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a3(
     const cplx<T>* __restrict__ O, const cplx<T>* __restrict__ dO, int nxo,
     const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ dWt,
     const int2* __restrict__ corner, const int* __restrict__ order, int M, size_t field,
-    cplx<T>* __restrict__ psi3, const cplx<T>* __restrict__ tw) {
+    cplx<T>* __restrict__ psi3, const cplx<T>* __restrict__ tw, int w_view = 0) {
   using S = RowShape<T, N>;
   constexpr int E = S::E, TL = S::TL;
   __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
@@ -2093,7 +2236,7 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a3(
   stage_twiddles<T, N, S::NT>(stw, tw);
   const int2 c = corner[order[b]];
   const size_t op = (size_t)(c.x + y) * nxo + c.y;
-  const size_t wp = ((size_t)m * N + y) * N;
+  const size_t wp = ((size_t)(w_view ? int(blockIdx.y) : m) * N + y) * N;
   cplx<T> po[E], a[E], bb[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {

@@ -34,7 +34,8 @@ def hermite_gauss_probe(n, powers=(0.7, 0.15, 0.1, 0.05), sigma=None, curvature=
     of width `sigma` (default n/12) with a defocus phase `curvature`·r²/σ² (the construction of
     ptyrs/src/base/sim.rs:93-125), mode k carrying `powers[k]` of a total of 1 (sum of
     |P|² over the plane)."""
-    orders = [(0, 0), (1, 0), (0, 1), (1, 1), (2, 0), (0, 2)][:len(powers)]
+    orders = [(0, 0), (1, 0), (0, 1), (1, 1), (2, 0), (0, 2), (2, 1), (1, 2), (3, 0),
+              (0, 3)][:len(powers)]
     sigma = sigma or n / 12
     v, u = (np.mgrid[:n, :n] - n / 2) / sigma
     r2 = u * u + v * v

@@ -66,15 +66,11 @@ def test_gpu_options_refusals_happen_before_any_device_call():
     obj = np.ones((1, 64, 64), complex)
     probe = np.ones((1, 32, 32), complex)
     # GpuOptions (ptyrs/src/engine/gpu/mod.rs:61-108): stack_size Some(0) and llk_slots 0 are
-    # refused; psi_f16 needs fused, and this engine carries exit waves in its element type.
+    # refused (psi_f16's refusals: test_psi_f16_needs_fused_passes_and_complex64).
     with pytest.raises(ValueError, match="stack_size"):
         pa.GpuPtycho(_scan(), obj, probe, stack_size=0)
     with pytest.raises(ValueError, match="llk_slots"):
         pa.GpuPtycho(_scan(), obj, probe, llk_slots=0)
-    with pytest.raises(ValueError, match="psi_f16 needs fused"):
-        pa.GpuPtycho(_scan(), obj, probe, psi_f16=True, fused=False)
-    with pytest.raises(ValueError, match="psi_f16"):
-        pa.GpuPtycho(_scan(), obj, probe, psi_f16=True)
     with pytest.raises(ValueError, match="65535 views"):
         pa.GpuPtycho(pa.Scan(np.zeros((20000, 32, 32), np.uint16), np.zeros(20000),
                              np.zeros(20000), 1.5e-10, 1.0, 75e-6),
@@ -98,3 +94,14 @@ def test_the_fault_hook_arms_and_disarms_without_a_device():
     with pytest.raises(KeyError):
         pb.arm_fault("nowhere")
     assert pa.library().ptyrs_test_arm_fault(7, 0) == 1
+
+
+def test_psi_f16_needs_fused_passes_and_complex64():
+    # GpuOptions::psi_f16 without fused passes is refused (ptyrs/tests/engine_gpu_f16.rs:139);
+    # its f16 pairs hold complex64 waves. Refused before any device call.
+    obj = np.ones((1, 64, 64), complex)
+    probe = np.ones((1, 32, 32), complex)
+    with pytest.raises(ValueError, match="psi_f16 needs"):
+        pa.GpuPtycho(_scan(), obj, probe, psi_f16=True, fused=False)
+    with pytest.raises(ValueError, match="c64"):
+        pa.GpuPtycho(_scan(), obj, probe, psi_f16=True, dtype="c128")

@@ -127,7 +127,9 @@ def test_position_feature_api():
     assert np.array_equal(g.features["position"], d0)
     g.rpie(1, batch_size=5)  # held offsets: rPIE models them and leaves them
     assert np.array_equal(g.features["position"], d0)
-    for name in ("ap", "dm", "raar", "ml"):
+    g.ml(1)  # ML models held offsets too
+    assert np.array_equal(g.features["position"], d0)
+    for name in ("ap", "dm", "raar"):
         with pytest.raises(ValueError, match="position"):
             getattr(g, name)(1)
     with pytest.raises(ValueError):
@@ -161,6 +163,34 @@ def test_rpie_positions_match_oracle(dtype, nprobe, batch, refine):
           rel_l2(g.object - d["obj"], o.obj - d["obj"]), rel_l2(g.probe, o.probe))
     assert max_rel(g.history["poisson"], want) < tol
     assert rel_l2(g.object - d["obj"], o.obj - d["obj"]) < tol * 10
+    assert rel_l2(g.probe, o.probe) < tol * 10
+    got = g.features["position"]
+    if refine:
+        assert max_rel((got - d0).ravel(), (o.offsets - d0).ravel()) < tol * 100
+    else:
+        assert np.array_equal(got, d0)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("nprobe,refine,up", [(1, True, True), (3, True, False), (2, False, True)])
+def test_ml_positions_match_oracle(dtype, nprobe, refine, up):
+    # ML under PATCH offsets (ptyrs/src/engine/ptycho.rs:757-842 with Features::position):
+    # far field, masked pixels, 1-3 modes, held or refined, three cycles with LLK records.
+    d = small_scan(n=64, nprobe=nprobe, npos=30, seed=13)
+    d0 = np.random.default_rng(14).uniform(-0.4, 0.4, (30, 2))
+    o = oracle(d)
+    o.set_position(d0, refine=refine)
+    kw = dict(update_object=True, update_probe=up, calc_llk=1)
+    o.ml(3, **kw)
+    g = gpu(d, dtype)
+    g.set_features(position=d0, hold=() if refine else ("position",))
+    g.ml(3, **kw)
+    tol = TOL[dtype]
+    want = [r[2].poisson for r in o.history]
+    print("ml positions", dtype, nprobe, max_rel(g.history["poisson"], want),
+          rel_l2(g.object, o.obj), rel_l2(g.probe, o.probe))
+    assert max_rel(g.history["poisson"], want) < tol
+    assert rel_l2(g.object, o.obj) < tol * 10
     assert rel_l2(g.probe, o.probe) < tol * 10
     got = g.features["position"]
     if refine:

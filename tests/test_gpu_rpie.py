@@ -30,7 +30,7 @@ def small_scan(n=64, nprobe=2, npos=30, photons=2e5, seed=3, masked=True):
     ext = int(np.ceil(np.abs(np.r_[y - y.mean(), x - x.mean()] / pixel).max())) + 2
     side = (2 * ext + n + 8 + 3) // 4 * 4
     obj = sim.random_object(side, side, seed)
-    probe = sim.hermite_gauss_probe(n, powers=[0.8, 0.2, 0.05, 0.02][:nprobe])
+    probe = sim.hermite_gauss_probe(n, powers=([0.8, 0.2, 0.05, 0.02] + [0.01] * 6)[:nprobe])
     cy, cx = sim.corners(y, x, pixel, obj.shape[-2:], n)
     I = sim.intensities(obj, probe, cy, cx, sim.quad_phase(n))
     s = photons / I.sum(axis=(1, 2)).mean()
