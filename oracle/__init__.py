@@ -26,7 +26,8 @@ What pins each module (the test files named run on the CPU):
   invariant); tests/test_oracle_rpie.py (`rpie_near_*`, 9 runs × 30 epochs at 1e-12);
   tests/test_oracle_pynx.py (AP, DM, RAAR, ML against `pynx_near_*`);
   tests/test_oracle_scenario.py (`dm_pynx`, `raar_pynx`, `rpie_ptychi`, `ml_grad`; rPIE and
-  ML under refined PATCH offsets: `rpie_pos_*` at 1e-12, `ml_pos_*` at 1e-13).
+  ML under refined PATCH offsets: `rpie_pos_*` at 1e-12, `ml_pos_*` at 1e-13; rPIE under
+  FULL_OBJECT: `rpie_full_object_{step,epochs,odd}_*` at 1e-12).
 - lsqml.py — tests/test_oracle_lsqml.py (`lsqml_near_*`), tests/test_oracle_scenario.py
   (`lsqml_{gaussian,poisson}_*`, `lsqml_epochs_*`), tests/test_oracle_gradient.py
   (`cost_grads` against PyNX's ML gradient and five-point central differences).
@@ -34,7 +35,14 @@ What pins each module (the test files named run on the CPU):
   intensities, one step, 30 epochs; directions and tangent against central differences).
 - positions.py — tests/test_oracle_positions.py (`lsqml_near_pos_*`,
   `lsqml_multislice_pos_*`; closed forms: a whole-pixel move is a roll, the MAD clip by hand);
-  `position_step` through the `ml_pos_*` run.
+  `position_step` through the `ml_pos_*` run; `derivative_sums` through the FULL_OBJECT runs.
+- sampling.py — tests/test_oracle_sampling.py restates `engine/sampling.rs:263-436`: windows of
+  a trigonometric polynomial are its interpolant and derivatives (past the plane's edge, odd
+  and even sides) at 1e-13, `place` is the adjoint of `windows`, `diag` the brute-force
+  diagonal of the normal operator, the zero-offset illumination the PATCH one, the AUTO rule;
+  then the FULL_OBJECT fixtures: `rpie_full_object_*` (tests/test_oracle_scenario.py) and
+  LSQML's `lsqml_near_pos_full_*` (cases 0 and 2) and `lsqml_multislice_pos_full_*` at 1e-12
+  (tests/test_oracle_positions.py).
 - scenario.py — tests/test_oracle_scenario.py: the ChaCha block function against RFC 8439's
   A.1 keystream (20 rounds) and the reference's seeded scenario, whose Poisson LLK sum
   equals `ml_grad_values_f64.txt`'s at 9e-16 (every count is the reference's); Poisson

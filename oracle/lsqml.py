@@ -1,5 +1,6 @@
 """pty-chi LSQML as the reference's CPU engine restates it, one object slice or several
-(`SliceWaves`), with position offsets under the PATCH model (refined or held) and no OPR:
+(`SliceWaves`), with position offsets (refined or held) under the PATCH or FullObject model
+and no OPR:
 one batch's directions (`LsqmlBatch`), the per-frame step sizes, the step and the epochs with
 the preconditioner schedule. An object is one `[nobj, nyo, nxo]` array, or a list of them, one
 per slice of the optics.
@@ -13,8 +14,9 @@ import numpy as np
 
 from .base import random_batches, to_natural, to_shifted
 from .multislice import SliceWaves
-from .positions import (EPS_POSITION, batch_position_step, gradient_directions, gradient_sums,
-                        move_planes, moved_quad)
+from .positions import (EPS_POSITION, batch_position_step, derivative_sums, gradient_directions,
+                        gradient_sums, move_planes, moved_quad)
+from .sampling import Sampler, auto_sampling, full_object_illumination, view_offsets
 from .ptycho import obj_probe_to_psi, llk
 
 EPS_AMPLITUDE = 1e-9   # ptyrs/src/engine/lsqml.rs:25-31
@@ -69,6 +71,10 @@ def unique_waves(run, views, optics, objects, probe):
     frames = [views[k][0] for k in range(0, len(views), nmode)]
     if run.offsets is None:
         return frames, nmode, None, SliceWaves(optics, views, objects, probe)
+    if run.sampling_of(optics, objects[0], probe) == "full_object":
+        sm = Sampler(objects[0].shape[-2:], probe.shape[-2:], views,
+                     view_offsets(views, run.offsets))
+        return frames, nmode, None, SliceWaves(optics, views, objects, probe, sampler=sm)
     np_ = probe.shape[0]
     q0 = optics.quad if len(objects) == 1 else None
     moves = [run.offsets[f].copy() for f in frames]
@@ -162,13 +168,20 @@ class LsqmlBatch:
         ptyrs/src/engine/lsqml.rs:230-270"""
         s = len(self.objects) // 2
         ny, nx = self.probe.shape[-2:]
+        sm = self.waves.sampler
+        if sm is not None:
+            gy, gx = sm.windows(self.objects[s], (0,)), sm.windows(self.objects[s], (1,))
         sums = []
         for b in range(len(self.frames)):
             k = b * self.nmode
             _, _, _, o, oy, ox = self.views[k]
             wq = self.waves.incident[s][k] * self.waves.quad(s, k)
+            chi = to_natural(self.chis[s][k])
+            if sm is not None:
+                sums.append(derivative_sums(gy[k], gx[k], wq, chi))
+                continue
             patch = self.objects[s][o, oy:oy + ny, ox:ox + nx]
-            sums.append(gradient_sums(patch, wq, to_natural(self.chis[s][k])))
+            sums.append(gradient_sums(patch, wq, chi))
         eps = EPS_POSITION * self.optics.propagator.intensity_scale()
         return gradient_directions(sums, eps)
 
@@ -178,7 +191,11 @@ class LsqmlBatch:
         d_obj = as_slices(d_obj)
         ny, nx = self.probe.shape[-2:]
         n = ny * nx
-        dpo = obj_probe_to_psi(None, self.views, self.objects[0], d_probe, self.waves.quad(0, 0))
+        # the shared views on what they read (under FullObject the sampled windows)
+        read = [(f, m, p, o, oy, ox) for (f, m, p, _, _, _), (_, _, _, o, oy, ox)
+                in zip(self.views, self.waves.views)]
+        dpo = obj_probe_to_psi(None, read, self.waves.objects[0], d_probe,
+                               self.waves.quad(0, 0))
         dop = self.waves.obj_tangent(0, d_obj[0])
         lam = LAMBDA0 * self.optics.propagator.intensity_scale() / n
         nb = len(self.frames)
@@ -215,9 +232,11 @@ class LsqmlBatch:
 class Lsqml:
     """LSQML state across epochs. ptyrs/src/engine/lsqml.rs:411-714"""
 
-    def __init__(self, batch_size, cost=POISSON, seed=7, offsets=None, refine=False):
-        """`offsets` ([nframe, 2] object pixels, PATCH model) are held, or refined after every
-        step with `refine` (Features::position, ptyrs/src/engine/features.rs:304-330)."""
+    def __init__(self, batch_size, cost=POISSON, seed=7, offsets=None, refine=False,
+                 sampling=None):
+        """`offsets` ([nframe, 2] object pixels) are held, or refined after every step with
+        `refine` (Features::position, ptyrs/src/engine/features.rs:304-330), modelled as
+        `sampling` ("patch" or "full_object"; None: Sampling::auto at the first use)."""
         if batch_size <= 0:
             raise ValueError("LSQML batch_size must be >= 1")
         if cost not in (AMPLITUDE, POISSON):
@@ -228,6 +247,15 @@ class Lsqml:
         self.a_far = {}
         self.offsets = None if offsets is None else np.array(offsets, dtype=np.float64)
         self.refine = refine and offsets is not None
+        self.sampling = sampling
+
+    def sampling_of(self, optics, obj, probe):
+        """The run's sampling, Sampling::auto of the optics, object and probe fixed at the first
+        use if none was set. ptyrs/src/engine/lsqml.rs:450-458"""
+        if self.sampling is None:
+            self.sampling = auto_sampling(optics.propagator.kind, obj.shape[-2:],
+                                          probe.shape[-2:])
+        return self.sampling
 
     def a_far_of(self, f):
         return self.a_far.get(f, 0.5)
@@ -257,7 +285,12 @@ class Lsqml:
         nframe, nmode = addr.nframe, addr.nmode
         views_all = addr.views()
         if self.epochs < 10 or self.epochs % 10 == 0:
-            illum = illumination(views_all, as_slices(obj)[0].shape, probe)
+            first = as_slices(obj)[0]
+            if (self.offsets is not None
+                    and self.sampling_of(optics, first, probe) == "full_object"):
+                illum = full_object_illumination(views_all, first.shape, probe, self.offsets)
+            else:
+                illum = illumination(views_all, first.shape, probe)
             pre = self.preconditioner
             if pre is None:
                 n = probe.shape[-1] * probe.shape[-2]

@@ -1,9 +1,11 @@
 """pty-chi's multislice exit wave over the oracle's arrays: the wave entering slice 0 is the
 view's probe plane; slice s multiplies the wave entering it by its object window, giving psi_s,
 which reaches slice s + 1 through gap s (an angular-spectrum propagator); the last slice's psi,
-times the plane phase q, is the exit wave. One slice is `obj_probe_to_psi` itself. No
-FullObject sampling: a frame's position offset enters through its moved probe planes (the
-caller's) and, past one slice, the last slice's plane phase displaced with them.
+times the plane phase q, is the exit wave. One slice is `obj_probe_to_psi` itself. Under
+PATCH a frame's position offset enters through its moved probe planes (the caller's) and, past
+one slice, the last slice's plane phase displaced with them; under FullObject (`sampler`) every
+slice is read through each view's window sampled at its corner plus its frame's offset, and
+the maps of an object change take that sampling, or its adjoint, around the same model.
 
 Test infrastructure (see oracle/__init__.py). Follows ptyrs/src/engine/multislice.rs:35-430.
 """
@@ -17,9 +19,15 @@ class SliceWaves:
     """The waves of `views` through the slices of `optics` at `objects` (one `[nobj, nyo, nxo]`
     array per slice) and `probe`. ptyrs/src/engine/multislice.rs:70-171"""
 
-    def __init__(self, optics, views, objects, probe, view_quads=None):
+    def __init__(self, optics, views, objects, probe, view_quads=None, sampler=None):
         if len(objects) != optics.nslice():
             raise ValueError("one object per slice of the optics")
+        self.sampler = sampler
+        self.obj_shape = objects[0].shape
+        if sampler is not None:
+            # what the views read: one sampled window per view, at corner (0, 0)
+            objects = [sampler.windows(o) for o in objects]
+            views = [(f, m, p, k, 0, 0) for k, (f, m, p, _, _, _) in enumerate(views)]
         self.optics, self.views, self.objects = optics, list(views), list(objects)
         # per view, the last slice's phase displaced by its frame's offset (more than one slice)
         self.view_quads = view_quads if len(objects) > 1 else None
@@ -69,20 +77,28 @@ class SliceWaves:
         out.reverse()
         return out
 
+    def windows(self, d):
+        """A change `d` of a slice as the views read it: `d` itself, under FullObject its
+        sampled windows. ptyrs/src/engine/multislice.rs:301-306"""
+        return d if self.sampler is None else self.sampler.windows(d)
+
     def obj_adjoint(self, s, chi_s):
-        """`Σ_views conj(q_s·in_s)·chi_s` scattered onto slice s.
+        """`Σ_views conj(q_s·in_s)·chi_s` scattered onto slice s (under FullObject onto the
+        views' windows, then placed by the sampling's adjoint).
         ptyrs/src/engine/multislice.rs:402-430"""
         ny, nx = self.shape
         out = np.zeros(self.objects[s].shape, dtype=np.complex128)
         for k, (_, _, _, o, oy, ox) in enumerate(self.views):
             out[o, oy:oy + ny, ox:ox + nx] += to_natural(chi_s[k]) * np.conj(
                 self.quad(s, k) * self.incident[s][k])
+        if self.sampler is not None:
+            return self.sampler.place(out, self.obj_shape[0])
         return out
 
     def obj_tangent(self, s, d):
         """The change of psi_s along `d` on slice s with the wave entering it held (shifted
         planes). ptyrs/src/engine/multislice.rs:385-397"""
-        nat = self._times_window(s, d, self.incident[s])
+        nat = self._times_window(s, self.windows(d), self.incident[s])
         return np.stack([to_shifted(p) for p in nat])
 
     def probe_adjoint(self, chi_0, np_):
@@ -108,7 +124,8 @@ class SliceWaves:
         linearisation of the slice model. ptyrs/src/engine/multislice.rs:359-383"""
         w = np.stack([d_probe[v[2]] for v in self.views])
         for s, (o, d) in enumerate(zip(self.objects, d_objects)):
-            out = self._times_window(s, o, w) + self._times_window(s, d, self.incident[s])
+            out = (self._times_window(s, o, w)
+                   + self._times_window(s, self.windows(d), self.incident[s]))
             if s < len(self.optics.gaps):
                 w = self.optics.gaps[s].forward(out)
         return np.stack([to_shifted(p) for p in out])

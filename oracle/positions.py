@@ -66,6 +66,18 @@ def gradient_sums(patch, w, chi):
     return num, den
 
 
+def derivative_sums(gy, gx, w, chi):
+    """`gradient_sums` of given window derivatives [∂O/∂y, ∂O/∂x] (under FullObject those of
+    the sampled window over the position); all natural layout.
+    ptyrs/src/engine/features.rs:474-492"""
+    num, den = [0.0, 0.0], [0.0, 0.0]
+    for axis, g in enumerate((gy, gx)):
+        u = g * w
+        num[axis] = float((np.conj(u) * chi).real.sum())
+        den[axis] = float((u.real ** 2 + u.imag ** 2).sum())
+    return num, den
+
+
 def gradient_directions(sums, eps):
     """`num / (den + max(max_batch den, eps))` per frame and axis.
     ptyrs/src/engine/features.rs:499-507"""

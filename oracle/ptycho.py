@@ -1,6 +1,8 @@
 """PyNX operators and the AP, DM, RAAR, ML and rPIE loops of the reference's CPU engine in
-FP64. Of the features only the position offsets are modelled (PATCH sampling: the probe moved
-by each frame's sub-pixel offset in Fourier space), in the LLK and rPIE; no scale or background.
+FP64. Of the features only the position offsets are modelled: under PATCH sampling (the probe
+moved by each frame's sub-pixel offset in Fourier space) in the LLK, rPIE and ML; under
+FullObject sampling (each view's window Fourier-interpolated at its corner plus the offset,
+`sampling.py`) in the LLK and rPIE. No scale or background.
 
 Test infrastructure (see oracle/__init__.py). Layout: exit waves `psi[k]` shifted, `k` in
 table order; objects `[nobj, nyo, nxo]`, probes `[nprobe, ny, nx]`, natural layout.
@@ -12,8 +14,9 @@ import numpy as np
 
 from .base import AddrTable, observed, pynx_preprocess, random_batches, to_natural, to_shifted
 from .optics import Optics, fft2, freq
-from .positions import (EPS_POSITION, batch_position_step, gradient_directions, gradient_sums,
-                        move_planes, position_step, ramp)
+from .positions import (EPS_POSITION, batch_position_step, derivative_sums, gradient_directions,
+                        gradient_sums, move_planes, position_step, ramp)
+from .sampling import Sampler, auto_sampling, view_offsets
 
 AP, DM, RAAR, RPIE, ML, BH, LSQML = "ap", "dm", "raar", "rpie", "ml", "bh", "lsqml"
 
@@ -193,8 +196,27 @@ class Ptycho:
         self.psi = np.zeros((len(self.views_all),) + tuple(frame), dtype=np.complex128)
         self.cycle = 0
         self.history = []
-        self.offsets = None   # Features::position under PATCH: [nframe, 2] object pixels
+        self.offsets = None   # Features::position: [nframe, 2] object pixels
         self.refine = False
+        # Sampling::auto (ptyrs/src/engine/ptycho.rs:271): "patch" or "full_object"
+        self.sampling = auto_sampling(optics.propagator.kind, obj.shape[-2:], frame)
+
+    def set_sampling(self, sampling):
+        """How the offsets enter the exit waves. ptyrs/src/engine/ptycho.rs:289-295"""
+        if sampling not in ("patch", "full_object"):
+            raise ValueError(sampling)
+        self.sampling = sampling
+
+    def _sampler(self, views):
+        """The FullObject sampling of `views` at their frames' offsets, None under PATCH."""
+        if self.offsets is None or self.sampling != "full_object":
+            return None
+        return Sampler(self.obj.shape[-2:], self.probe.shape[-2:], views,
+                       view_offsets(views, self.offsets))
+
+    def _patch_only(self, what):
+        if self.offsets is not None and self.sampling == "full_object":
+            raise ValueError(what + " under FullObject sampling is not modelled")
 
     def set_position(self, offsets, refine=True):
         """Features::position (ptyrs/src/engine/features.rs:304-330): per-frame [dy, dx]
@@ -205,8 +227,11 @@ class Ptycho:
     def moved_waves(self, views):
         """Under the offsets, every view's wave W = DFT⁻¹(DFT(P·q)·exp(−2πi f·d)): its probe
         mode times the plane phase translated by its frame's offset on the plane's periodic
-        grid, natural layout (`Shifted` under PATCH, ptyrs/src/engine/features.rs:545-640)."""
+        grid, natural layout (`Shifted` under PATCH, ptyrs/src/engine/features.rs:545-640);
+        under FullObject the unmoved P·q (features.rs:609-615)."""
         shape = self.probe.shape[-2:]
+        if self.sampling == "full_object":
+            return np.stack([self.probe[v[2]] * self.quad for v in views])
         spec = np.fft.fft2(self.probe * self.quad)
         return np.stack([np.fft.ifft2(spec[v[2]] * ramp(shape, self.offsets[v[0]]))
                          for v in views])
@@ -221,10 +246,16 @@ class Ptycho:
         return obj_probe_to_psi(self.addr, self.views_all, self.obj, self.probe, self.quad)
 
     def _moved_psi(self, views):
-        """The moved waves W and the exit waves fftshift(W·O_win) of `views`."""
+        """The moved waves W and the exit waves fftshift(W·O_win) of `views` (under FullObject
+        the sampled windows)."""
         ny, nx = self.probe.shape[-2:]
         w = self.moved_waves(views)
-        win = np.stack([self.obj[o, oy:oy + ny, ox:ox + nx] for (_, _, _, o, oy, ox) in views])
+        sm = self._sampler(views)
+        if sm is not None:
+            win = sm.windows(self.obj)
+        else:
+            win = np.stack([self.obj[o, oy:oy + ny, ox:ox + nx]
+                            for (_, _, _, o, oy, ox) in views])
         return w, to_shifted(w * win)
 
     def llk(self):
@@ -245,6 +276,7 @@ class Ptycho:
         """2·∂L/∂O* and 2·∂L/∂P*; under the offsets the back-projections through the moved
         waves (`Shifted::accumulate_obj` / `accumulate_probe`,
         ptyrs/src/engine/features.rs:865-958). ptyrs/src/engine/ptycho.rs:492-513"""
+        self._patch_only("ML")
         g = self.grad_psi()
         if self.offsets is not None:
             return self._moved_grads(g, obj, probe)
@@ -307,6 +339,7 @@ class Ptycho:
         current estimate's exit waves x: num_a = Re Σ conj(x − W·O)·O·∂_a W, den_a = Σ |O·∂_a W|²
         over the frame's views, ∂_a W = DFT⁻¹(DFT(P·q)·exp(−2πi f·d)·(−2πi f_a)).
         ptyrs/src/engine/ptycho.rs:468-479, ptyrs/src/engine/features.rs:960-1025"""
+        self._patch_only("ML")
         ny, nx = self.probe.shape[-2:]
         views = self.views_all
         w, psi = self._moved_psi(views)
@@ -491,6 +524,8 @@ class Ptycho:
 
     def rpie_batch(self, frames, obj_step, probe_step, obj_alpha, probe_alpha):
         """One rPIE batch from the batch-start arrays. ptyrs/src/engine/ptycho.rs:900-1086, :1089-1130"""
+        if self.offsets is not None and self.sampling == "full_object":
+            return self._rpie_batch_sampled(frames, obj_step, probe_step, obj_alpha, probe_alpha)
         if self.offsets is not None:
             return self._rpie_batch_moved(frames, obj_step, probe_step, obj_alpha, probe_alpha)
         ny, nx = self.probe.shape[-2:]
@@ -584,6 +619,64 @@ class Ptycho:
                 _, _, _, o, oy, ox = views[k]
                 sums.append(gradient_sums(self.obj[o, oy:oy + ny, ox:ox + nx], w[k],
                                           delta_nat[k]))
+            d_pos = gradient_directions(sums, EPS_POSITION
+                                        * self.optics.propagator.intensity_scale())
+        if obj_step is not None:
+            self.obj = self.obj + d_obj * obj_step
+        if probe_step is not None:
+            self.probe = self.probe + d_probe * (probe_step / nbatch)
+        if d_pos is not None:
+            batch_position_step(self.offsets, frames, d_pos)
+
+
+    def _rpie_batch_sampled(self, frames, obj_step, probe_step, obj_alpha, probe_alpha):
+        """One rPIE batch under FullObject sampling, from the batch-start arrays: the waves are
+        the unmoved P·q and each view reads its window sampled at its corner plus its frame's
+        offset; the object step's terms conj(W)·Δ/((1 − α)D_m + α·max) are placed by the
+        sampling's adjoint, D_m the batch's |W|² of probe mode m placed and read back through
+        the windows (real part floored at 0); the probe direction reads the sampled window
+        and is not moved; the position sums take the sampled window's derivatives over the
+        position. ptyrs/src/engine/ptycho.rs:900-1086, :1089-1130"""
+        ny, nx = self.probe.shape[-2:]
+        nmode = self.addr.nmode
+        nprobe = self.probe.shape[0]
+        nbatch = len(frames)
+        views, psi, delta = self.rpie_delta(frames)
+        sm = self._sampler(views)
+        w = self.moved_waves(views)
+        delta_nat = to_natural(delta)
+        d_obj = None
+        if obj_step is not None:
+            w2 = w.real ** 2 + w.imag ** 2
+            dens = np.zeros((len(views), ny, nx))
+            for m in range(nprobe):
+                ks = [k for k, v in enumerate(views) if v[2] == m]
+                if not ks:
+                    continue
+                patches = np.zeros((len(views), ny, nx), dtype=np.complex128)
+                patches[ks] = w2[ks]
+                read = sm.windows(sm.place(patches, 1))
+                dens[ks] = np.maximum(read[ks].real, 0.0)
+            pmax = float(dens.reshape(nbatch, nmode, ny, nx).sum(axis=1).max())
+            a = obj_alpha
+            d_obj = sm.place(np.conj(w) / ((1.0 - a) * dens + a * pmax) * delta_nat, 1)
+        d_probe = None
+        if probe_step is not None:
+            a = probe_alpha
+            windows = sm.windows(self.obj)
+            d_probe = np.zeros_like(self.probe)
+            for b in range(nbatch):
+                fv = views[b * nmode:(b + 1) * nmode]
+                win = windows[b * nmode]
+                w2 = win.real ** 2 + win.imag ** 2
+                wgt = np.conj(win * self.quad) / ((1.0 - a) * w2 + a * float(w2.max()))
+                for m, v in enumerate(fv):
+                    d_probe[v[2]] += wgt * delta_nat[b * nmode + m]
+        d_pos = None
+        if self.refine:
+            gy, gx = sm.windows(self.obj, (0,)), sm.windows(self.obj, (1,))
+            sums = [derivative_sums(gy[b * nmode], gx[b * nmode], w[b * nmode],
+                                    delta_nat[b * nmode]) for b in range(nbatch)]
             d_pos = gradient_directions(sums, EPS_POSITION
                                         * self.optics.propagator.intensity_scale())
         if obj_step is not None:

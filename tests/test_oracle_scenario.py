@@ -238,3 +238,30 @@ def test_ml_positions_match_the_model():
     want = (rows[:, 3:5] - off0).ravel()
     err = np.abs((p.offsets - off0).ravel() - want).max() / np.abs(want).max()
     assert err < 1e-12, err
+
+
+# The same runs under FULL_OBJECT (ptyrs/tests/engine_rpie_pos.rs:178-226): each view's window
+# Fourier-interpolated at its corner plus its offset; "odd" is the step on the scaled start's
+# 76 x 80 plane less its first row and column (both sides odd), each window one pixel nearer
+# the origin and so on the same pixels.
+@pytest.mark.parametrize("case", ["step", "epochs", "odd"])
+def test_rpie_full_object_positions_match_ptychi(case):
+    kw = dict(RPIE_POS["step" if case == "odd" else case])
+    nepoch = kw.pop("nepoch")
+    p = fresh(2)
+    if case == "odd":
+        _, inputs = start(2)
+        counts, y, x, _, _ = inputs
+        p = sc.far_field_ptycho(counts, y, x, p.obj[:, 1:, 1:].copy(), p.probe.copy(),
+                                *sc.GEOMETRY)
+        assert p.obj.shape[-2:] == (75, 79)
+    p.set_sampling("full_object")
+    rows = np.array(table(f"rpie_full_object_{case}_offsets_f64.txt"))
+    start_off = rows[:, 1:3]
+    p.set_position(start_off, refine=True)
+    o0, p0 = p.obj.copy(), p.probe.copy()
+    p.rpie(nepoch, **kw)
+    _changes(p.obj, p.probe, o0, p0, f"rpie_full_object_{case}_obj_f64.txt",
+             f"rpie_full_object_{case}_probe_f64.txt")
+    err = max_rel((p.offsets - start_off).ravel(), (rows[:, 3:5] - start_off).ravel())
+    assert err < 3e-12, err
