@@ -236,9 +236,11 @@ int ptyrs_engine_lsqml_slice_steps(ptyrs_engine* e, double* out, size_t len);
 
 /* GpuPtycho::set_features with Features::position only (ptyrs/src/engine/gpu/mod.rs:846-857,
  * ptyrs/src/engine/features.rs:304-330): offsets [nframe][2] (dy, dx) in object pixels,
- * len = 2·nframe, refined by every LSQML step when refine != 0, else held; offsets = NULL
- * stops modelling them. Offsets are kept in FP64 on the device. With offsets only LSQML and
- * the LLK pass run (the other algorithms fail with PTYRS_EINVAL), under the PATCH sampling. */
+ * len = 2·nframe, refined by every rPIE batch, LSQML step or ML cycle when refine != 0, else
+ * held; offsets = NULL stops modelling them. Offsets are kept in FP64 on the device. With
+ * offsets rPIE, ML, LSQML and the LLK pass run (AP, DM and RAAR fail with PTYRS_EINVAL); under
+ * FullObject sampling rPIE, LSQML, the LLK pass and the model intensities, with one object
+ * slice and mode. */
 int ptyrs_engine_set_position(ptyrs_engine* e, const double* offsets, size_t len, int refine);
 
 /* GpuPtycho::features' position (ptyrs/src/engine/gpu/mod.rs:823-825): *state = 0 (none),
@@ -246,8 +248,10 @@ int ptyrs_engine_set_position(ptyrs_engine* e, const double* offsets, size_t len
 int ptyrs_engine_position(ptyrs_engine* e, double* out, size_t len, int* state);
 
 /* GpuPtycho::sampling / set_sampling (ptyrs/src/engine/gpu/mod.rs:827-844): PTYRS_PATCH or
- * PTYRS_FULL_OBJECT, Sampling::auto of the optics, object and frame until set. Setting
- * PTYRS_FULL_OBJECT fails with PTYRS_EINVAL: this engine runs the PATCH model only. */
+ * PTYRS_FULL_OBJECT, Sampling::auto of the optics, object and frame until set. Under
+ * PTYRS_FULL_OBJECT each view's window is Fourier-interpolated from the whole object plane at
+ * its corner plus its offset (object sides up to 4096 pixels in complex64, 2048 in
+ * complex128); the algorithms that do not run it fail with PTYRS_EINVAL while offsets are set. */
 int ptyrs_engine_sampling(ptyrs_engine* e, int* sampling);
 int ptyrs_engine_set_sampling(ptyrs_engine* e, int sampling);
 

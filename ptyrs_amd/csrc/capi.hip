@@ -539,9 +539,8 @@ int ptyrs_engine_sampling(ptyrs_engine* h, int* sampling) {
 
 int ptyrs_engine_set_sampling(ptyrs_engine* h, int sampling) {
   return guarded(h, true, [&](EngineBase& e) {
-    if (sampling == PTYRS_FULL_OBJECT)
-      throw Invalid("FullObject sampling is not run on this GPU engine");
-    if (sampling != PTYRS_PATCH) throw Invalid("sampling must be PATCH or FULL_OBJECT");
+    if (sampling != PTYRS_PATCH && sampling != PTYRS_FULL_OBJECT)
+      throw Invalid("sampling must be PATCH or FULL_OBJECT");
     e.sampling = sampling;
   });
 }

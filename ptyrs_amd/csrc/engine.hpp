@@ -13,6 +13,7 @@
 
 #include "host.hpp"
 #include "kernels.hpp"
+#include "sampling.hpp"
 
 namespace ptyrs {
 
@@ -414,10 +415,8 @@ class Engine : public EngineBase {
   void set_probe(const double* in) override { upload_complex(P_, in, nprobe_ * n); }
 
   void llk(double out[4]) override {
-    if (pos_on_ && sampling != PTYRS_PATCH)
-      throw Invalid("position offsets under FullObject sampling are not run on this GPU "
-                    "engine; set the PATCH sampling");
     ensure_workspace(std::min<size_t>(nframe_, kDefaultStack), 1);
+    if (full_object()) fo_ensure(cap_);
     probe_terms();
     llk_pass(0);
     PTYRS_HIP(hipMemcpyAsync(out, hist_.p, sizeof(double) * 4, hipMemcpyDeviceToHost, stream_));
@@ -448,9 +447,6 @@ class Engine : public EngineBase {
   void rpie_check(const ptyrs_rpie_params& p) override {
     if (nslice_ > 1)
       throw Invalid("rPIE runs one object slice; only LSQML and the LLK pass run several");
-    if (pos_on_ && sampling != PTYRS_PATCH)
-      throw Invalid("rPIE with position offsets under FullObject sampling is not run on this "
-                    "GPU engine; set the PATCH sampling");
     unitary("rPIE");
     if (nobj_ != 1) throw Invalid("rPIE needs exactly one object mode");
     if (p.batch_size == 0) throw Invalid("rPIE batch_size must be >= 1");
@@ -472,7 +468,8 @@ class Engine : public EngineBase {
     }
     const size_t cap = std::min(p.batch_size, nframe_);
     ensure_workspace(cap, nepoch);
-    if (pos_on_) {
+    if (full_object()) fo_ensure(cap_);
+    if (pos_on_ && !full_object()) {
       // Under offsets pass C leaves one probe direction per frame, moved back and averaged.
       dP_.ensure(sizeof(cplx<T>) * cap_ * M_ * n);
       dPm_.ensure(sizeof(cplx<T>) * M_ * n);
@@ -500,7 +497,7 @@ class Engine : public EngineBase {
     // buffer ahead of the replay.
     run_.base = {uint64_t(p.has_obj_step), uint64_t(p.has_probe_step), bits(p.obj_step),
                  bits(p.probe_step), bits(p.obj_alpha), bits(p.probe_alpha), cap_,
-                 uint64_t(pos_on_), uint64_t(pos_refine_)};
+                 uint64_t(pos_on_), uint64_t(pos_refine_), uint64_t(sampling)};
   }
 
   bool rpie_epoch() override {
@@ -559,9 +556,9 @@ class Engine : public EngineBase {
     if (p.cost != PTYRS_AMPLITUDE && p.cost != PTYRS_POISSON)
       throw Invalid("cost must be amplitude or poisson");
     if (M_ > kMaxModes) throw Invalid(modes_error());
-    if (pos_on_ && sampling != PTYRS_PATCH)
-      throw Invalid("position offsets under FullObject sampling are not run on this GPU "
-                    "engine; set the PATCH sampling");
+    if (full_object() && nslice_ > 1)
+      throw Invalid("LSQML with position offsets under FullObject sampling runs one object "
+                    "slice on this GPU engine");
     if (nepoch == 0 || nframe_ == 0) {
       cycle += nepoch;
       return;
@@ -569,6 +566,7 @@ class Engine : public EngineBase {
     const size_t size = std::min(p.batch_size, nframe_);
     ensure_workspace(size, nepoch);
     ensure_lsqml(size);
+    if (full_object()) fo_ensure(cap_);
     std::vector<int> order(nepoch * nframe_);
     std::vector<std::vector<std::pair<int, int>>> spans(nepoch);
     for (size_t ic = 0; ic < nepoch; ++ic) {
@@ -595,7 +593,9 @@ class Engine : public EngineBase {
       PTYRS_HIP(hipMemcpyAsync(eord_.p, order_.as<int>() + ic * nframe_, sizeof(int) * nframe_,
                                hipMemcpyDeviceToDevice, stream_));
       const int* ord = eord_.as<int>();
-      if (nslice_ > 1 || pos_on_) {
+      if (full_object()) {
+        for (auto& s : spans[ic]) lsqml_batch_sampled(p, ord + s.first, s.second, s.first);
+      } else if (nslice_ > 1 || pos_on_) {
         for (auto& s : spans[ic]) lsqml_batch_ms(p, ord + s.first, s.second, s.first);
       } else {
         std::vector<uint64_t> key = {uint64_t(p.cost), cap_};
@@ -1231,6 +1231,212 @@ class Engine : public EngineBase {
     unitary(what);
   }
 
+  // ---- FullObject sampling (ptyrs/src/engine/sampling.rs:79-209) -----------------------------
+  // Under offsets and FullObject the probe stays put and each slot reads its window sampled
+  // at its frame's corner plus offset: the windows of a batch are laid out as one tall plane
+  // ([slot·N + y][x], fo_win_), read by the stack passes at the corners fo_tall_ ((slot·N, 0)
+  // per frame); the overlaps D_m are the same tall layout (fo_D_, [m][cap·N][N]). The object's
+  // transforms run at its own sides (AxisPlan, any length up to kAxisMax; fo_plan_[0] rows of
+  // nxo, [1] columns of nyo).
+  AxisPlan fo_plan_[2]{};
+  DevBuf fo_tw_[2], fo_spec_, fo_tmp_, fo_planes_, fo_win_, fo_dwin_, fo_F_, fo_acc_, fo_E_,
+      fo_pts_, fo_tall_, fo_D_, fo_part_;
+  size_t fo_cap_ = 0;
+  static constexpr int kFoBlocks = 256;
+  bool full_object() const { return pos_on_ && sampling == PTYRS_FULL_OBJECT; }
+
+  static AxisPlan axis_plan(int L) {
+    AxisPlan pl{};
+    pl.L = L;
+    int x = L;
+    auto push = [&](int r) {
+      if (pl.nst >= kAxisMaxStages) throw Invalid("object side has too many prime factors");
+      pl.radix[pl.nst++] = r;
+    };
+    while (x % 4 == 0) push(4), x /= 4;
+    while (x % 2 == 0) push(2), x /= 2;
+    for (int r = 3; x > 1; r += 2)
+      while (x % r == 0) push(r), x /= r;
+    return pl;
+  }
+  // The longest object side the FullObject transforms take: one line of it (two complex
+  // buffers) in 64 KiB of LDS.
+  static constexpr size_t fo_max_side() { return sizeof(T) == 4 ? kAxisMax : kAxisMax / 2; }
+
+  void fo_ensure(size_t cap) {
+    if (nslice_ != 1)
+      throw Invalid("FullObject sampling with several object slices is not run on this GPU "
+                    "engine");
+    if (nobj_ != 1)
+      throw Invalid("FullObject sampling with several object modes is not run on this GPU "
+                    "engine");
+    if (nyo_ < size_t(N) || nxo_ < size_t(N))
+      throw Invalid("FullObject sampling needs an object plane at least the frame's size");
+    if (nyo_ > fo_max_side() || nxo_ > fo_max_side())
+      throw Invalid("FullObject sampling takes object sides up to " +
+                    std::to_string(fo_max_side()) + " pixels on this GPU engine");
+    if (fo_plan_[0].L == 0) {
+      fo_plan_[0] = axis_plan(int(nxo_));
+      fo_plan_[1] = axis_plan(int(nyo_));
+      for (int a = 0; a < 2; ++a) {
+        const int L = fo_plan_[a].L;
+        std::vector<double> tw(2 * size_t(L));
+        for (int t = 0; t < L; ++t) {
+          const long double ang = -2.0L * 3.14159265358979323846264338327950288L * t / L;
+          tw[2 * t] = double(cosl(ang));
+          tw[2 * t + 1] = double(sinl(ang));
+        }
+        fo_tw_[a].ensure(sizeof(cplx<T>) * L);
+        upload_complex(fo_tw_[a], tw.data(), size_t(L));
+      }
+    }
+    cap = std::max<size_t>(cap, 1);
+    const size_t no = nyo_ * nxo_;
+    fo_spec_.ensure(sizeof(cplx<T>) * no);
+    fo_acc_.ensure(sizeof(cplx<T>) * no);
+    fo_F_.ensure(sizeof(cplx<T>) * M_ * no);
+    fo_tmp_.ensure(sizeof(cplx<T>) * std::max<size_t>(cap, M_) * N * nxo_);
+    fo_planes_.ensure(sizeof(cplx<T>) * cap * no);
+    fo_win_.ensure(sizeof(cplx<T>) * cap * n);
+    fo_dwin_.ensure(sizeof(cplx<T>) * cap * n);
+    fo_E_.ensure(sizeof(double2) * no);
+    fo_D_.ensure(sizeof(T) * M_ * cap * n);
+    fo_pts_.ensure(sizeof(double2) * cap);
+    fo_tall_.ensure(sizeof(int2) * std::max<size_t>(1, nframe_));
+    fo_part_.ensure(sizeof(T) * kFoBlocks);
+    fo_cap_ = std::max(fo_cap_, cap);
+  }
+
+  // One pass of object-side line transforms along `axis` (0: rows of nxo, 1: columns of nyo).
+  template <int LOAD, int STORE>
+  void axis_pass(int axis, bool inverse, const cplx<T>* in, void* out, const AxisGeom& g,
+                 int nplanes, PhaseArgs ph = PhaseArgs{nullptr, 0, 0, 0, 0}, T scale = T(1)) {
+    const AxisPlan& pl = fo_plan_[axis];
+    const int L = pl.L;
+    const int G = std::max(1, std::min(16, 1024 / L));
+    const int Lp = G > 1 ? L + 1 : L;
+    const size_t lds = 4 * size_t(G) * Lp * sizeof(T);
+    hipLaunchKernelGGL((k_axis_fft<T, LOAD, STORE>), dim3((g.nlines + G - 1) / G, nplanes),
+                       dim3(kAxisThreads), lds, stream_, in, out, fo_tw_[axis].as<cplx<T>>(), pl,
+                       int(inverse), g, G, ph, scale);
+  }
+  AxisGeom rows_geom(long long in_plane, int n_in, long long out_plane, int n_out, int nlines,
+                     int in_line, int out_line) const {
+    return AxisGeom{nlines, in_plane, in_line, 1, n_in, out_plane, out_line, 1, n_out};
+  }
+  AxisGeom cols_geom(long long in_plane, int n_in, long long out_plane, int n_out) const {
+    return AxisGeom{int(nxo_), in_plane, 1, int(nxo_), n_in, out_plane, 1, int(nxo_), n_out};
+  }
+
+  // The slots' points (corner + offset of each frame of `ord`) and the tall corners.
+  void fo_points(const int* ord, int nb) {
+    hipLaunchKernelGGL(k_fo_points, dim3((nb + 255) / 256), dim3(256), 0, stream_,
+                       corner_.as<int2>(), (const double2*)offs_.as<double2>(), ord, nb, N,
+                       fo_pts_.as<double2>(), fo_tall_.as<int2>());
+  }
+  // The unnormalised spectrum of object plane `obj` into fo_spec_.
+  void fo_object_spectrum(const cplx<T>* obj) {
+    const long long no = (long long)(nyo_ * nxo_);
+    axis_pass<AL_PLAIN, AS_PLAIN>(
+        0, false, obj, fo_spec_.p,
+        rows_geom(no, int(nxo_), no, int(nxo_), int(nyo_), int(nxo_), int(nxo_)), 1);
+    axis_pass<AL_PLAIN, AS_PLAIN>(1, false, fo_spec_.as<cplx<T>>(), fo_spec_.p,
+                                  cols_geom(no, int(nyo_), no, int(nyo_)), 1);
+  }
+  // Every slot's N × N window of the plane whose spectrum is `spec`, sampled at its point
+  // (with `deriv` 1 or 2 its derivative over the position along y or x): plane b of `out`
+  // at out + b·out_plane, real part floored at 0 with STORE = AS_REAL_FLOOR.
+  template <int STORE>
+  void fo_windows(const cplx<T>* spec, int nb, int deriv, void* out, long long out_plane) {
+    const long long no = (long long)(nyo_ * nxo_), tp = (long long)N * nxo_;
+    axis_pass<AL_PHASE, AS_PLAIN>(
+        1, true, spec, fo_tmp_.p, cols_geom(0, int(nyo_), tp, N), nb,
+        PhaseArgs{(const double2*)fo_pts_.as<double2>(), int(nyo_), int(nxo_), deriv, 1});
+    (void)no;
+    axis_pass<AL_PLAIN, STORE>(0, true, fo_tmp_.as<cplx<T>>(), out,
+                               rows_geom(tp, int(nxo_), out_plane, N, N, int(nxo_), N), nb,
+                               PhaseArgs{nullptr, 0, 0, 0, 0},
+                               T(1.0 / (double(nyo_) * double(nxo_))));
+  }
+  // The slots' N × N patches (plane b at patches + b·in_plane, natural layout) placed by the
+  // sampling's adjoint: the spectrum Σ_b conj(exp(+2πi f·t_b))·DFT(patch_b at the origin) into
+  // fo_acc_ (added to what it holds with `accumulate`).
+  void fo_place_spectrum(const cplx<T>* patches, long long in_plane, int nb, bool accumulate) {
+    const long long no = (long long)(nyo_ * nxo_), tp = (long long)N * nxo_;
+    axis_pass<AL_PLAIN, AS_PLAIN>(0, false, patches, fo_tmp_.p,
+                                  rows_geom(in_plane, N, tp, int(nxo_), N, N, int(nxo_)), nb);
+    axis_pass<AL_PLAIN, AS_PLAIN>(1, false, fo_tmp_.as<cplx<T>>(), fo_planes_.p,
+                                  cols_geom(tp, N, no, int(nyo_)), nb);
+    hipLaunchKernelGGL(k_fo_place_acc<T>, dim3(unsigned((no + 255) / 256)), dim3(256), 0,
+                       stream_, fo_acc_.as<cplx<T>>(), (const cplx<T>*)fo_planes_.as<cplx<T>>(),
+                       no, (const double2*)fo_pts_.as<double2>(), nb, int(nyo_), int(nxo_),
+                       int(accumulate));
+  }
+  // out (+)= scale·DFT⁻¹(fo_acc_)/(nyo·nxo) with STORE AS_ADD (AS_PLAIN: =; AS_REAL_FLOOR:
+  // the real part floored at 0 into a real plane); fo_acc_ is overwritten.
+  template <int STORE = AS_ADD>
+  void fo_add_placed(void* out, double scale) {
+    const long long no = (long long)(nyo_ * nxo_);
+    axis_pass<AL_PLAIN, AS_PLAIN>(
+        0, true, fo_acc_.as<cplx<T>>(), fo_acc_.p,
+        rows_geom(no, int(nxo_), no, int(nxo_), int(nyo_), int(nxo_), int(nxo_)), 1);
+    axis_pass<AL_PLAIN, STORE>(1, true, fo_acc_.as<cplx<T>>(), out,
+                               cols_geom(no, int(nyo_), no, int(nyo_)), 1,
+                               PhaseArgs{nullptr, 0, 0, 0, 0},
+                               T(scale / (double(nyo_) * double(nxo_))));
+  }
+  // pty-chi's illumination map under FullObject (ptyrs/src/engine/sampling.rs:51-75): every
+  // frame's Σ_m |P_m|² placed at its corner plus offset by the sampling's adjoint, real part
+  // floored at 0, into illum_.
+  void fo_illumination() {
+    const long long no = (long long)(nyo_ * nxo_), tp = (long long)N * nxo_;
+    axis_pass<AL_ABS2, AS_PLAIN>(0, false, P_.as<cplx<T>>(), fo_tmp_.p,
+                                 rows_geom(n, N, tp, int(nxo_), N, N, int(nxo_)), M_);
+    axis_pass<AL_PLAIN, AS_PLAIN>(1, false, fo_tmp_.as<cplx<T>>(), fo_F_.p,
+                                  cols_geom(tp, N, no, int(nyo_)), M_);
+    const int step = int(fo_cap_);
+    for (int f0 = 0; f0 < int(nframe_); f0 += step) {
+      const int nb = std::min(step, int(nframe_) - f0);
+      fo_points(iota_.as<int>() + f0, nb);
+      hipLaunchKernelGGL(k_fo_phase_acc, dim3(unsigned((no + 255) / 256)), dim3(256), 0, stream_,
+                         fo_E_.as<double2>(), (const double2*)fo_pts_.as<double2>(), nb,
+                         int(nyo_), int(nxo_), int(f0 > 0));
+    }
+    hipLaunchKernelGGL(k_fo_times_sum<T>, dim3(unsigned((no + 255) / 256)), dim3(256), 0,
+                       stream_, fo_acc_.as<cplx<T>>(), (const cplx<T>*)fo_F_.as<cplx<T>>(),
+                       (const double2*)fo_E_.as<double2>(), M_, no);
+    fo_add_placed<AS_REAL_FLOOR>(illum_.p, 1.0);
+  }
+  // The batch's sampled object windows (tall, fo_win_) from the object's spectrum in fo_spec_.
+  void fo_object_windows(int nb) {
+    fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 0, fo_win_.p, (long long)n);
+  }
+  // rPIE's overlaps under FullObject (ptyrs/src/engine/ptycho.rs:1089-1130): per mode m the
+  // batch's |W_m|² placed at every slot's point by the sampling's adjoint and read back through
+  // each slot's window, real part floored at 0, into fo_D_; their max over the slots and
+  // pixels of Σ_m into pmax_. The placed plane's transform back and forth is left out (the
+  // read takes the placed spectrum directly).
+  void fo_overlaps(int nb) {
+    const long long no = (long long)(nyo_ * nxo_), tp = (long long)N * nxo_;
+    axis_pass<AL_ABS2, AS_PLAIN>(0, false, Wt_.as<cplx<T>>(), fo_tmp_.p,
+                                 rows_geom(n, N, tp, int(nxo_), N, N, int(nxo_)), M_);
+    axis_pass<AL_PLAIN, AS_PLAIN>(1, false, fo_tmp_.as<cplx<T>>(), fo_F_.p,
+                                  cols_geom(tp, N, no, int(nyo_)), M_);
+    hipLaunchKernelGGL(k_fo_place_same<T>, dim3(unsigned((no + 255) / 256)), dim3(256), 0,
+                       stream_, fo_F_.as<cplx<T>>(), M_, (const double2*)fo_pts_.as<double2>(),
+                       nb, int(nyo_), int(nxo_));
+    for (int m = 0; m < M_; ++m)
+      fo_windows<AS_REAL_FLOOR>(fo_F_.as<cplx<T>>() + m * no, nb, 0,
+                                fo_D_.as<T>() + size_t(m) * fo_cap_ * n, (long long)n);
+    const long long count = (long long)nb * n;
+    const int blocks = int(std::min<long long>(kFoBlocks, (count + 255) / 256));
+    hipLaunchKernelGGL(k_fo_dens_max<T>, dim3(blocks), dim3(256), 0, stream_,
+                       (const T*)fo_D_.as<T>(), (long long)(fo_cap_ * n), M_, count,
+                       fo_part_.as<T>());
+    hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(MAX_THREADS), 0, stream_,
+                       (const T*)fo_part_.as<T>(), blocks, pmax_.as<T>());
+  }
+
   // Position offsets (PATCH): device [nframe] (dy, dx) in FP64; the moved probe planes of a
   // stack, the probe spectra, the moved last-slice phases, the window derivative planes.
   bool pos_on_ = false, pos_refine_ = false;
@@ -1560,11 +1766,14 @@ class Engine : public EngineBase {
   // from illum·max_f Σ iobs_f/(4·n·k); floor = (0.1·max pre)².
   void precondition(bool first) {
     const size_t no = nyo_ * nxo_;
-    PTYRS_HIP(hipMemsetAsync(illum_.p, 0, sizeof(T) * no, stream_));
     const int nt = ntiles();
     // Up to kIllumChunk frames' views at once: one tile list and one illumination pass each.
     const int step = int(illum_chunk());
-    for (int f0 = 0; f0 < int(nframe_); f0 += step) {
+    if (full_object())
+      run(K_OTHER, [&] { fo_illumination(); });
+    else
+      PTYRS_HIP(hipMemsetAsync(illum_.p, 0, sizeof(T) * no, stream_));
+    for (int f0 = 0; f0 < int(nframe_) && !full_object(); f0 += step) {
       const int nb = std::min(step, int(nframe_) - f0);
       const int* ord = iota_.as<int>() + f0;
       run(K_BIN, [&] {
@@ -1754,21 +1963,28 @@ class Engine : public EngineBase {
   // the counts are compared with), detector order — pty-chi's forward-model output `y_pred`
   // (ptyrs/tests/engine_lsqml_multislice.rs:146-182).
   void intensities(double* out) override {
-    if (pos_on_ && sampling != PTYRS_PATCH)
-      throw Invalid("position offsets under FullObject sampling are not run on this GPU "
-                    "engine; set the PATCH sampling");
     if (M_ > kMaxModes) throw Invalid(modes_error());
     if (nframe_ == 0) return;
     const size_t stack = std::min<size_t>(nframe_, kDefaultStack);
     ensure_workspace(stack, 1);
     ensure_lsqml(stack);
+    const bool fo = full_object();
+    if (fo) fo_ensure(cap_);
     probe_terms();
+    if (fo) run(K_OTHER, [&] { fo_object_spectrum(O_.as<cplx<T>>()); });
     zero(prev_.as<T>(), 2);
     std::vector<T> ic(stack * n);
     for (size_t f0 = 0; f0 < nframe_; f0 += stack) {
       const int nb = int(std::min(stack, nframe_ - f0));
       const int* ord = iota_.as<int>() + f0;
-      if (nslice_ > 1 || pos_on_) {
+      if (fo) {
+        run(K_OTHER, [&] {
+          fo_points(ord, nb);
+          fo_object_windows(nb);
+        });
+        pass_a(ord, nb, false, MixArgs<T>{MIX_NONE, {}, {}, T(0), T(0)}, nullptr,
+               fo_win_.as<cplx<T>>(), N, fo_tall_.as<int2>());
+      } else if (nslice_ > 1 || pos_on_) {
         ms_forward(ord, nb, stack_waves(ord, nb));
         run(K_PASS_A, [&] { rows(false, nb * M_); });
       } else {
@@ -1965,7 +2181,7 @@ class Engine : public EngineBase {
                          grad, tw_.as<cplx<T>>());
       hipLaunchKernelGGL(k_pos_sums<T>, dim3(aux_blocks(), nb), dim3(THREADS), 0, stream_,
                          (const cplx<T>*)grad, w, q, q_frame, chi, N, M_,
-                         pos_part_.as<double>());
+                         pos_part_.as<double>(), 1);
     });
   }
 
@@ -2199,14 +2415,19 @@ class Engine : public EngineBase {
   // Pass A; `waves` (nullable) gives every view its own wave (plane view of it) instead of
   // the probe terms Wt.
   template <bool H = false>
+  // `obj`, `obj_nxo` and `corners` (nullable) replace the object plane, its width and the
+  // frames' corners (FullObject's tall windows).
   void pass_a(const int* ord, int nb, bool omax,
               MixArgs<T, H> mix = MixArgs<T, H>{MIX_NONE, {}, {}, T(0), T(0)},
-              const cplx<T>* waves = nullptr) {
+              const cplx<T>* waves = nullptr, const cplx<T>* obj = nullptr, int obj_nxo = 0,
+              const int2* corners = nullptr) {
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3(S::NBLK, nb * M_), dim3(S::NT), 0, stream_,
-                         O_.as<cplx<T>>(), int(nxo_), waves ? waves : Wt_.as<cplx<T>>(),
-                         corner_.as<int2>(), ord, M_, stack(), omax ? omax_.as<T>() : nullptr,
-                         tw_.as<cplx<T>>(), mix, int(waves != nullptr));
+                         obj ? obj : O_.as<cplx<T>>(), obj ? obj_nxo : int(nxo_),
+                         waves ? waves : Wt_.as<cplx<T>>(),
+                         corners ? corners : corner_.as<int2>(), ord, M_, stack(),
+                         omax ? omax_.as<T>() : nullptr, tw_.as<cplx<T>>(), mix,
+                         int(waves != nullptr));
     };
     run(K_PASS_A, [&] {
       switch (mix.kind) {
@@ -2252,10 +2473,20 @@ class Engine : public EngineBase {
   // no bit.
   void llk_pass_to(double* result, size_t step_frames) {
     const int step = int(std::max<size_t>(step_frames, 1));
+    const bool fo = full_object();
+    if (fo) run(K_OTHER, [&] { fo_object_spectrum(O_.as<cplx<T>>()); });
     for (int f0 = 0; f0 < int(nframe_); f0 += step) {
       const int nb = std::min(step, int(nframe_) - f0);
       const int* ord = iota_.as<int>() + f0;
-      if (nslice_ > 1 || pos_on_) {
+      if (fo) {
+        // FullObject: the frames' sampled windows, read at their tall corners.
+        run(K_OTHER, [&] {
+          fo_points(ord, nb);
+          fo_object_windows(nb);
+        });
+        pass_a(ord, nb, false, MixArgs<T>{MIX_NONE, {}, {}, T(0), T(0)}, nullptr,
+               fo_win_.as<cplx<T>>(), N, fo_tall_.as<int2>());
+      } else if (nslice_ > 1 || pos_on_) {
         ms_forward(ord, nb, stack_waves(ord, nb));
         run(K_PASS_A, [&] { rows(false, nb * M_); });
       } else {
@@ -2292,6 +2523,10 @@ class Engine : public EngineBase {
 
   // One rPIE batch (ptyrs/src/engine/ptycho.rs:900-1086) from the batch-start object and probe.
   void rpie_batch(const ptyrs_rpie_params& p, const int* ord, int nb) {
+    if (full_object()) {
+      rpie_batch_sampled(p, ord, nb);
+      return;
+    }
     if (pos_on_) {
       rpie_batch_moved(p, ord, nb);
       return;
@@ -2438,6 +2673,194 @@ class Engine : public EngineBase {
         hipLaunchKernelGGL(k_probe_apply<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,
                            P_.as<cplx<T>>(), Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
                            (const cplx<T>*)dPm_.as<cplx<T>>(), 1, n, total, T(p.probe_step));
+      });
+    }
+    if (pos_refine_) {
+      const double eps = 1e-6 * scale_;  // EPS_POSITION·k (ptyrs/src/engine/features.rs:455)
+      run(K_OTHER, [&] {
+        hipLaunchKernelGGL(k_pos_step, dim3(1), dim3(1024), 0, stream_,
+                           (const double*)pos_part_.as<double>(), aux_blocks(), nb, int(nframe_),
+                           eps, ord, offs_.as<double2>(), pos_sums_.as<double>());
+      });
+    }
+  }
+
+  // One LSQML batch under offsets and FullObject sampling, one slice (ptyrs/src/engine/
+  // lsqml.rs:97-350 with SliceWaves' sampler, ptyrs/src/engine/multislice.rs:84-171, :385-442):
+  // the probe stays put and every slot reads its sampled window (the tall fo_win_); χ as without
+  // offsets; the object direction Σ_m conj(W_m)·χ_m on each window placed onto the object by the
+  // sampling's adjoint, then preconditioned; the probe direction read against the windows and
+  // not moved; the step sizes from the direction's sampled windows; the position directions
+  // from the windows' derivatives over the position against W = P·q and mode 0's χ.
+  void lsqml_batch_sampled(const ptyrs_lsqml_params& p, const int* ord, int nb, int off) {
+    const bool gauss = p.cost == PTYRS_AMPLITUDE;
+    const long long no = (long long)(nyo_ * nxo_);
+    run(K_BIN, [&] {
+      fo_points(ord, nb);
+      fo_object_spectrum(O_.as<cplx<T>>());
+      fo_object_windows(nb);
+    });
+    pass_a(ord, nb, false, MixArgs<T>{MIX_NONE, {}, {}, T(0), T(0)}, nullptr,
+           fo_win_.as<cplx<T>>(), N, fo_tall_.as<int2>());
+    BArgs<T> ba{afar_tmp_.as<T>(), prev_.as<T>(), xi_part_.as<double>(), icalc_.as<T>()};
+    const int xb = aux_blocks();
+    if (gauss) {
+      pass_b<B_GAUSS>(ord, nb, ba);
+    } else {
+      run(K_OTHER, [&] {
+        hipLaunchKernelGGL(k_lsqml_prev<T>, dim3(1), dim3(THREADS), 0, stream_,
+                           afar_all_.as<T>(), ord, nb, prev_.as<T>());
+      });
+      pass_b<B_ICALC>(ord, nb, ba);
+      run(K_OTHER, [&] {
+        hipLaunchKernelGGL(k_lsqml_xi<T>, dim3(1), dim3(THREADS), 0, stream_,
+                           xi_part_.as<double>(), int(SB::NBLK), nb, prev_.as<T>(),
+                           afar_tmp_.as<T>(), prev_.as<T>() + 1);
+        hipLaunchKernelGGL(k_lsqml_xi_sums<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
+                           icalc_.as<T>(), counts_.as<float>(), ord, n, prev_.as<T>() + 1,
+                           xi_part2_.as<double>());
+        hipLaunchKernelGGL(k_lsqml_xi<T>, dim3(1), dim3(THREADS), 0, stream_,
+                           xi_part2_.as<double>(), xb, nb, prev_.as<T>() + 1, afar_tmp_.as<T>(),
+                           (T*)nullptr);
+      });
+      pass_b<B_POISSON>(ord, nb, ba);
+    }
+    run(K_OTHER, [&] {
+      hipLaunchKernelGGL(k_lsqml_log_afar<T>, dim3((nb + 255) / 256), dim3(256), 0, stream_,
+                         afar_tmp_.as<T>(), ord, nb, afar_all_.as<T>(),
+                         log_.as<double>() + 3 * size_t(off), int(gauss), 3);
+    });
+    const int chunk = chunk_of(nb);
+    const int nchunk = (nb + chunk - 1) / chunk;
+    run(K_PASS_C, [&] {
+      hipLaunchKernelGGL((k_pass_c_chi<T, N>), dim3(SR::NBLK, M_, nchunk), dim3(SR::NT), 0,
+                         stream_, psi_.as<cplx<T>>(), fo_win_.as<cplx<T>>(), N,
+                         Wt_.as<cplx<T>>(), q_.as<cplx<T>>(), fo_tall_.as<int2>(), ord, nb, M_,
+                         chunk, 1, dP_.as<cplx<T>>(), tw_.as<cplx<T>>());
+    });
+    const cplx<T>* chi = psi_.as<cplx<T>>();
+    if (pos_refine_) {
+      // The windows' derivatives from the batch-start object's spectrum (still in fo_spec_).
+      cplx<T>* grad = pgrad_.as<cplx<T>>() + cap_ * n;
+      run(K_OTHER, [&] {
+        fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 1, grad, 2ll * n);
+        fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 2, grad + n, 2ll * n);
+        hipLaunchKernelGGL(k_pos_sums<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
+                           (const cplx<T>*)grad, (const cplx<T>*)Wt_.as<cplx<T>>(),
+                           (const cplx<T>*)nullptr, 0, chi, N, M_, pos_part_.as<double>(), 0);
+      });
+    }
+    run(K_OBJ_DIR, [&] {
+      hipLaunchKernelGGL(k_fo_chi_terms<T>, dim3((n + 255) / 256, nb), dim3(256), 0, stream_,
+                         fo_dwin_.as<cplx<T>>(), (const cplx<T>*)Wt_.as<cplx<T>>(), chi, M_, n);
+      fo_place_spectrum(fo_dwin_.as<cplx<T>>(), (long long)n, nb, false);
+      fo_add_placed<AS_PLAIN>(dO_.p, 1.0);
+      hipLaunchKernelGGL(k_fo_precondition<T>, dim3(unsigned((no + 255) / 256)), dim3(256), 0,
+                         stream_, dO_.as<cplx<T>>(), (const T*)pre_.as<T>(),
+                         (const T*)floor_.as<T>(), no);
+    });
+    const int total = int(nprobe_ * n);
+    // λ0 = 1.2e-7·k/n per pixel and mode (ptyrs/src/engine/lsqml.rs:31-33), summed over both.
+    const double lam = 1.2e-7 * scale_ * double(M_);
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_probe_mean<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,
+                         dPm_.as<cplx<T>>(), dP_.as<cplx<T>>(), nchunk, total, nb);
+    });
+    run(K_DOTS, [&] {
+      // The direction as the slots read it: its sampled windows (the object's spectrum is no
+      // longer needed).
+      fo_object_spectrum(dO_.as<cplx<T>>());
+      fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 0, fo_dwin_.p, (long long)n);
+      hipLaunchKernelGGL(k_lsqml_dots<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
+                         (const cplx<T>*)fo_dwin_.as<cplx<T>>(),
+                         (const cplx<T>*)fo_win_.as<cplx<T>>(), N, Wt_.as<cplx<T>>(),
+                         dPm_.as<cplx<T>>(), q_.as<cplx<T>>(), chi, N, M_, fo_tall_.as<int2>(),
+                         ord, dots_part_.as<double>());
+      hipLaunchKernelGGL(k_lsqml_step<T>, dim3(1), dim3(1024), 0, stream_,
+                         dots_part_.as<double>(), xb, nb, lam, M_, sums_.as<double>(),
+                         log_.as<double>() + 3 * size_t(off), step_.as<double>(), 3);
+    });
+    run(K_OBJ_UPDATE, [&] {
+      hipLaunchKernelGGL(k_lsqml_apply_obj<T>, dim3(unsigned((no + 255) / 256)), dim3(256), 0,
+                         stream_, O_.as<cplx<T>>(), dO_.as<cplx<T>>(), step_.as<double>(),
+                         size_t(no));
+    });
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_lsqml_apply_probe<T>, dim3((total + 255) / 256), dim3(256), 0,
+                         stream_, P_.as<cplx<T>>(), Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
+                         dPm_.as<cplx<T>>(), step_.as<double>(), n, total);
+    });
+    if (pos_refine_) {
+      const double eps = 1e-6 * scale_;  // EPS_POSITION·k (ptyrs/src/engine/features.rs:455)
+      run(K_OTHER, [&] {
+        hipLaunchKernelGGL(k_pos_step, dim3(1), dim3(1024), 0, stream_,
+                           (const double*)pos_part_.as<double>(), xb, nb, int(nframe_), eps, ord,
+                           offs_.as<double2>(), pos_sums_.as<double>());
+      });
+    }
+  }
+
+  // One rPIE batch under offsets and FullObject sampling (ptyrs/src/engine/ptycho.rs:900-1086
+  // with Shifted::sampler, the reference GPU's rpie under its position model,
+  // ptyrs/src/engine/gpu/rpie.rs:69-133): the waves are the unmoved probe terms; every slot
+  // reads its window sampled at its frame's corner plus offset (the tall windows fo_win_);
+  // the overlaps D_m are each mode's |W_m|² placed at every slot's point and read back through
+  // the windows (fo_overlaps); pass C sums the modes' terms per slot as without offsets, and
+  // the summed terms are placed onto the object by the sampling's adjoint; the probe
+  // direction is not moved; with refined offsets the position sums take the sampled window's
+  // derivatives over the position against mode 0's W and Δ.
+  void rpie_batch_sampled(const ptyrs_rpie_params& p, const int* ord, int nb) {
+    const bool upd_obj = p.has_obj_step != 0, upd_probe = p.has_probe_step != 0;
+    run(K_BIN, [&] {
+      fo_points(ord, nb);
+      fo_object_spectrum(O_.as<cplx<T>>());
+      fo_object_windows(nb);
+    });
+    if (upd_obj) run(K_OVERLAP, [&] { fo_overlaps(nb); });
+    pass_a(ord, nb, upd_probe, MixArgs<T>{MIX_NONE, {}, {}, T(0), T(0)}, nullptr,
+           fo_win_.as<cplx<T>>(), N, fo_tall_.as<int2>());
+    pass_b<B_RPIE>(ord, nb);
+    const int chunk = chunk_of_m(nb);
+    const int nchunk = (nb + chunk - 1) / chunk;
+    cplx<T>* chi0 = pos_refine_ ? gspec_.as<cplx<T>>() : nullptr;
+    run(K_PASS_C, [&] {
+      auto launch = [&](auto mm) {
+        constexpr int MM = decltype(mm)::value;
+        constexpr int G = rows_per_group(N, SR::L, MM);
+        hipLaunchKernelGGL((k_pass_c_rpie_sum<T, N, MM>), dim3(N / G, nchunk),
+                           dim3(pass_c_sum_threads<T, N, MM>()), 0, stream_, psi_.as<cplx<T>>(),
+                           fo_win_.as<cplx<T>>(), N, Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
+                           fo_tall_.as<int2>(), ord, nb, omax_.as<T>(), int(S::NBLK), chunk,
+                           T(p.probe_alpha), int(upd_probe), dP_.as<cplx<T>>(),
+                           tw_.as<cplx<T>>(), int(upd_obj), (const T*)fo_D_.as<T>(),
+                           int(fo_cap_ * N), (const T*)pmax_.as<T>(), T(p.obj_alpha), 0, chi0);
+      };
+      with_modes(launch);
+    });
+    if (pos_refine_) {
+      cplx<T>* grad = pgrad_.as<cplx<T>>() + cap_ * n;
+      run(K_OTHER, [&] {
+        fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 1, grad, 2ll * n);
+        fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 2, grad + n, 2ll * n);
+        hipLaunchKernelGGL(k_pos_sums<T>, dim3(aux_blocks(), nb), dim3(THREADS), 0, stream_,
+                           (const cplx<T>*)grad, (const cplx<T>*)Wt_.as<cplx<T>>(),
+                           (const cplx<T>*)nullptr, 0, (const cplx<T>*)chi0, N, M_,
+                           pos_part_.as<double>(), 0);
+      });
+    }
+    if (upd_obj) {
+      // The summed terms sit in each slot's mode-0 plane of the stack.
+      run(K_OBJ_UPDATE, [&] {
+        fo_place_spectrum(psi_.as<cplx<T>>(), (long long)M_ * n, nb, false);
+        fo_add_placed(O_.as<cplx<T>>(), p.obj_step);
+      });
+    }
+    if (upd_probe) {
+      const int total = int(nprobe_ * n);
+      run(K_PROBE, [&] {
+        hipLaunchKernelGGL(k_probe_apply<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,
+                           P_.as<cplx<T>>(), Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
+                           dP_.as<cplx<T>>(), nchunk, n, total, T(p.probe_step / nb));
       });
     }
     if (pos_refine_) {

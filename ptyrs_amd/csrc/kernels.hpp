@@ -1884,20 +1884,22 @@ __global__ void k_deriv(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ s
 
 // Per slot and workgroup the position sums of its first view (mode 0):
 // num_a = Re Σ conj(g_a·w·q)·χ, den_a = Σ |g_a·w·q|² over the workgroup's pixels, g_a the window's
-// derivative along axis a; w per slot and mode ([slot·M + m][n]), q nullable, per slot with
-// q_frame. part[b][blk] = {num_y, num_x, den_y, den_x}. Grid (nblk, nb).
-// ptyrs/src/engine/features.rs:457-497
+// derivative along axis a; w per slot and mode ([slot·M + m][n]) with w_slot, else mode 0's
+// plane for every slot; q nullable, per slot with q_frame. part[b][blk] = {num_y, num_x, den_y,
+// den_x}. Grid (nblk, nb). ptyrs/src/engine/features.rs:457-497
 template <typename T>
 __global__ __launch_bounds__(THREADS) void k_pos_sums(
     const cplx<T>* __restrict__ grad, const cplx<T>* __restrict__ w, const cplx<T>* __restrict__ q,
-    int q_frame, const cplx<T>* __restrict__ chi, int N, int M, double* __restrict__ part) {
+    int q_frame, const cplx<T>* __restrict__ chi, int N, int M, double* __restrict__ part,
+    int w_slot) {
   __shared__ double red[THREADS];
   const int b = blockIdx.y;
   const int n = N * N;
   double s[4] = {0, 0, 0, 0};
   const size_t k = (size_t)b * M * n;
+  const size_t kw = w_slot ? k : 0;
   for (int j = blockIdx.x * THREADS + threadIdx.x; j < n; j += gridDim.x * THREADS) {
-    cplx<T> wq = ld(w + k + j);
+    cplx<T> wq = ld(w + kw + j);
     if (q) wq = wq * ld(q + (q_frame ? (size_t)b * n : 0) + j);
     const cplx<T> c = ld(chi + k + j);
 #pragma unroll

@@ -1,0 +1,311 @@
+// FullObject sampling on the device (pty-chi's FULL_OBJECT shift operator; ptyrs/src/engine/
+// sampling.rs:79-209): a view reads its object plane at its corner plus its frame's offset t by
+// Fourier interpolation of the whole plane, patch = crop(DFT⁻¹(DFT(O)·exp(+2πi f·t))); the
+// adjoint puts a patch at the plane's origin, transforms it and sums conj(exp(+2πi f·t)) times
+// it over the views in the spectrum, then one inverse transform.
+//
+// The object plane has any side (its prime factors decide the cost, not whether it runs), so
+// its transforms are a runtime-length Stockham autosort over lines held in LDS: the plan's
+// stages are radices 4, 2, then the odd primes; stage s with radix R and Ns (the product of
+// the earlier radices) forms output o from R inputs as
+//   out[o] = Σ_b in[j + b·L/R]·w^(b·((j mod Ns) + a·Ns)·L/(Ns·R)),
+//   a = (o/Ns) mod R, j = (o/(Ns·R))·Ns + o mod Ns, w = exp(−2πi/L)
+// (the butterfly of Govindaraju et al., SC 2008, one output per thread, so any radix runs the
+// same code). Inverse transforms conjugate in and out; none is normalised.
+#pragma once
+
+#include "cplx.hpp"
+
+namespace ptyrs {
+
+constexpr int kAxisMaxStages = 24;
+constexpr int kAxisMax = 4096;   // longest object side the transforms take
+constexpr int kAxisThreads = 256;
+
+struct AxisPlan {
+  int L, nst;
+  int radix[kAxisMaxStages];
+};
+
+// Where a pass reads and writes: element i of line l of plane p at
+// base + p·plane + l·line + i·elem; n_in elements are read (the rest of the L are zero), the
+// first n_out written.
+struct AxisGeom {
+  int nlines;
+  long long in_plane;
+  int in_line, in_elem, n_in;
+  long long out_plane;
+  int out_line, out_elem, n_out;
+};
+
+enum AxisLoad { AL_PLAIN = 0, AL_PHASE = 1, AL_ABS2 = 2 };
+enum AxisStore { AS_PLAIN = 0, AS_REAL_FLOOR = 1, AS_ADD = 2 };
+
+// AL_PHASE: the loaded spectrum value times exp(+2πi(f_y·t_y + f_x·t_x)) of plane p's point
+// pts[p] (and by 2πi·f_a for a derivative along axis a: deriv 1 for y, 2 for x), its bins
+// (jy, jx) numpy's fftfreq of an ny × nx plane; `cols` says the lines are columns.
+struct PhaseArgs {
+  const double2* pts;
+  int ny, nx, deriv, cols;
+};
+
+__device__ __forceinline__ double fftfreq(int j, int n) {
+  return double(j < (n + 1) / 2 ? j : j - n) / double(n);
+}
+
+// exp(+2πi(fy·t.x + fx·t.y)) in FP64, the argument reduced to [−1/2, 1/2] turns first.
+__device__ __forceinline__ double2 sample_phase(double2 t, int jy, int jx, int ny, int nx) {
+  double a = fftfreq(jy, ny) * t.x + fftfreq(jx, nx) * t.y;
+  a -= rint(a);
+  double s, c;
+  sincospi(2.0 * a, &s, &c);
+  return make_double2(c, s);
+}
+
+// One pass of line transforms over planes. Grid (ceil(nlines/G), nplanes), kAxisThreads
+// threads, dynamic LDS 4·G·Lp·sizeof(T) (Lp = L + 1 when G > 1, so that a column load's lanes
+// fall on different banks).
+template <typename T, int LOAD, int STORE>
+__global__ __launch_bounds__(kAxisThreads) void k_axis_fft(const cplx<T>* __restrict__ in,
+                                                           void* __restrict__ out,
+                                                           const cplx<T>* __restrict__ tw,
+                                                           AxisPlan plan, int inverse,
+                                                           AxisGeom g, int G, PhaseArgs ph,
+                                                           T scale) {
+  extern __shared__ unsigned char axis_lds[];
+  const int L = plan.L, Lp = G > 1 ? L + 1 : L;
+  T* re0 = reinterpret_cast<T*>(axis_lds);
+  T* im0 = re0 + G * Lp;
+  T* re1 = im0 + G * Lp;
+  T* im1 = re1 + G * Lp;
+  const int p = blockIdx.y;
+  const int l0 = blockIdx.x * G;
+  const int gl = g.nlines - l0 < G ? g.nlines - l0 : G;
+  const int tot = G * L;
+  // Lanes run along a line where its elements are adjacent in memory, else across lines.
+  const bool along_in = g.in_elem == 1;
+  for (int idx = threadIdx.x; idx < tot; idx += kAxisThreads) {
+    const int gi = along_in ? idx / L : idx % G;
+    const int i = along_in ? idx % L : idx / G;
+    cplx<T> v{T(0), T(0)};
+    if (gi < gl && i < g.n_in) {
+      const int line = l0 + gi;
+      v = ld(in + p * g.in_plane + (long long)line * g.in_line + (long long)i * g.in_elem);
+      if constexpr (LOAD == AL_ABS2) {
+        v = cplx<T>{norm2(v), T(0)};
+      } else if constexpr (LOAD == AL_PHASE) {
+        const int jy = ph.cols ? i : line, jx = ph.cols ? line : i;
+        double2 e = sample_phase(ph.pts[p], jy, jx, ph.ny, ph.nx);
+        if (ph.deriv) {
+          const double w = 2.0 * M_PI * (ph.deriv == 1 ? fftfreq(jy, ph.ny) : fftfreq(jx, ph.nx));
+          e = make_double2(-e.y * w, e.x * w);  // times i·w
+        }
+        v = cplx<T>{T(double(v.x) * e.x - double(v.y) * e.y),
+                    T(double(v.x) * e.y + double(v.y) * e.x)};
+      }
+    }
+    if (inverse) v.y = -v.y;
+    re0[gi * Lp + i] = v.x;
+    im0[gi * Lp + i] = v.y;
+  }
+  __syncthreads();
+  T *sre = re0, *sim = im0, *dre = re1, *dim = im1;
+  int Ns = 1;
+  for (int s = 0; s < plan.nst; ++s) {
+    const int R = plan.radix[s];
+    const int LR = L / R, NR = Ns * R, stride = L / NR;
+    for (int idx = threadIdx.x; idx < tot; idx += kAxisThreads) {
+      const int gi = idx / L, o = idx % L;
+      const int a = (o / Ns) % R, j = (o / NR) * Ns + o % Ns;
+      const int base = ((j % Ns) + a * Ns) * stride;  // < L
+      const T* xr = sre + gi * Lp + j;
+      const T* xi = sim + gi * Lp + j;
+      T sr = T(0), si = T(0);
+      int t = 0;
+      for (int b = 0; b < R; ++b) {
+        const cplx<T> w = ld(tw + t);
+        const T ar = xr[b * LR], ai = xi[b * LR];
+        sr += ar * w.x - ai * w.y;
+        si += ar * w.y + ai * w.x;
+        t += base;
+        if (t >= L) t -= L;
+      }
+      dre[gi * Lp + o] = sr;
+      dim[gi * Lp + o] = si;
+    }
+    __syncthreads();
+    T* x = sre; sre = dre; dre = x;
+    x = sim; sim = dim; dim = x;
+    Ns *= R;
+  }
+  const bool along_out = g.out_elem == 1;
+  const int tout = G * g.n_out;
+  for (int idx = threadIdx.x; idx < tout; idx += kAxisThreads) {
+    const int gi = along_out ? idx / g.n_out : idx % G;
+    const int i = along_out ? idx % g.n_out : idx / G;
+    if (gi >= gl) continue;
+    cplx<T> v{sre[gi * Lp + i], sim[gi * Lp + i]};
+    if (inverse) v.y = -v.y;
+    v = v * scale;
+    const long long at =
+        p * g.out_plane + (long long)(l0 + gi) * g.out_line + (long long)i * g.out_elem;
+    if constexpr (STORE == AS_REAL_FLOOR) {
+      T* o = reinterpret_cast<T*>(out) + at;
+      *o = v.x > T(0) ? v.x : T(0);
+    } else if constexpr (STORE == AS_ADD) {
+      cplx<T>* o = reinterpret_cast<cplx<T>*>(out) + at;
+      st(o, ld(o) + v);
+    } else {
+      st(reinterpret_cast<cplx<T>*>(out) + at, v);
+    }
+  }
+}
+
+// The batch's sampling points and the tall layout its windows take: slot b's point is the
+// corner of its frame plus the frame's offset (t = (oy + dy, ox + dx)), and its window sits at
+// rows b·N.. of a [nb·N][N] plane, so the stack passes read it at corner (b·N, 0).
+__global__ static void k_fo_points(const int2* __restrict__ corner,
+                                   const double2* __restrict__ offs,
+                                   const int* __restrict__ order, int nb, int N,
+                                   double2* __restrict__ pts, int2* __restrict__ tall) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int f = order[b];
+  const int2 c = corner[f];
+  const double2 d = offs[f];
+  pts[b] = make_double2(double(c.x) + d.x, double(c.y) + d.y);
+  tall[f] = make_int2(b * N, 0);
+}
+
+// acc[j] (+)= Σ_b X[b][j]·conj(exp(+2πi f_j·t_b)) over the slots in order: the spectrum of the
+// patches placed by the sampling's adjoint (X the patches' transforms at the plane's origin).
+template <typename T>
+__global__ void k_fo_place_acc(cplx<T>* __restrict__ acc, const cplx<T>* __restrict__ X,
+                               long long xplane, const double2* __restrict__ pts, int nb, int ny,
+                               int nx, int accumulate) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ny * nx) return;
+  const int jy = j / nx, jx = j % nx;
+  double sr = 0, si = 0;
+  if (accumulate) {
+    const cplx<T> a = ld(acc + j);
+    sr = double(a.x);
+    si = double(a.y);
+  }
+  for (int b = 0; b < nb; ++b) {
+    const cplx<T> x = ld(X + b * xplane + j);
+    const double2 e = sample_phase(pts[b], jy, jx, ny, nx);
+    sr += double(x.x) * e.x + double(x.y) * e.y;
+    si += double(x.y) * e.x - double(x.x) * e.y;
+  }
+  st(acc + j, cplx<T>{T(sr), T(si)});
+}
+
+// S_m = F_m · Σ_b conj(exp(+2πi f·t_b)) for every mode plane m of F (in place): the spectrum of
+// one mode's |W|², the same at every slot, placed at every slot's point.
+template <typename T>
+__global__ void k_fo_place_same(cplx<T>* __restrict__ F, int nplane,
+                                const double2* __restrict__ pts, int nb, int ny, int nx) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ny * nx) return;
+  const int jy = j / nx, jx = j % nx;
+  double er = 0, ei = 0;
+  for (int b = 0; b < nb; ++b) {
+    const double2 e = sample_phase(pts[b], jy, jx, ny, nx);
+    er += e.x;
+    ei -= e.y;
+  }
+  const size_t n = size_t(ny) * nx;
+  for (int m = 0; m < nplane; ++m) {
+    const cplx<T> f = ld(F + m * n + j);
+    st(F + m * n + j, cplx<T>{T(double(f.x) * er - double(f.y) * ei),
+                              T(double(f.x) * ei + double(f.y) * er)});
+  }
+}
+
+// E[j] (+)= Σ_b conj(exp(+2πi f_j·t_b)) over the slots in order (FP64): the spectrum of a
+// unit patch at the origin placed at every slot's point, per transform of the patch.
+__global__ static void k_fo_phase_acc(double2* __restrict__ E, const double2* __restrict__ pts,
+                                      int nb, int ny, int nx, int accumulate) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ny * nx) return;
+  const int jy = j / nx, jx = j % nx;
+  double er = 0, ei = 0;
+  if (accumulate) {
+    er = E[j].x;
+    ei = E[j].y;
+  }
+  for (int b = 0; b < nb; ++b) {
+    const double2 e = sample_phase(pts[b], jy, jx, ny, nx);
+    er += e.x;
+    ei -= e.y;
+  }
+  E[j] = make_double2(er, ei);
+}
+
+// acc[j] = (Σ_m F_m[j])·E[j]: every frame's summed mode patches placed at its point.
+template <typename T>
+__global__ void k_fo_times_sum(cplx<T>* __restrict__ acc, const cplx<T>* __restrict__ F,
+                               const double2* __restrict__ E, int M, long long no) {
+  const long long j = blockIdx.x * 256ll + threadIdx.x;
+  if (j >= no) return;
+  double fr = 0, fi = 0;
+  for (int m = 0; m < M; ++m) {
+    const cplx<T> f = ld(F + m * no + j);
+    fr += double(f.x);
+    fi += double(f.y);
+  }
+  const double2 e = E[j];
+  st(acc + j, cplx<T>{T(fr * e.x - fi * e.y), T(fr * e.y + fi * e.x)});
+}
+
+// Per slot the modes' object-direction terms on its window: out[b][j] = Σ_m conj(W_m[j])·χ_bm[j]
+// (W = P·q), natural layout.
+template <typename T>
+__global__ void k_fo_chi_terms(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ W,
+                               const cplx<T>* __restrict__ chi, int M, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int b = blockIdx.y;
+  cplx<T> s{T(0), T(0)};
+  for (int m = 0; m < M; ++m)
+    s = s + cmulc(ld(W + (size_t)m * n + j), ld(chi + ((size_t)b * M + m) * n + j));
+  st(out + (size_t)b * n + j, s);
+}
+
+// dO /= √(pre² + floor) (LSQML's preconditioned object direction).
+template <typename T>
+__global__ void k_fo_precondition(cplx<T>* __restrict__ dO, const T* __restrict__ pre,
+                                  const T* __restrict__ floor, long long no) {
+  const long long p = blockIdx.x * 256ll + threadIdx.x;
+  if (p >= no) return;
+  const T pr = pre[p];
+  const T inv = T(1) / sqrt(pr * pr + floor[0]);
+  st(dO + p, ld(dO + p) * inv);
+}
+
+// Per workgroup the max over its pixels of Σ_m D_m on the slots' windows: D is [M][rows][N]
+// real with slot b's window at rows b·N (stride `mstride` between modes). Grid (blocks).
+template <typename T>
+__global__ __launch_bounds__(256) void k_fo_dens_max(const T* __restrict__ D, long long mstride,
+                                                     int M, long long count,
+                                                     T* __restrict__ part) {
+  __shared__ T red[256];
+  T mx = T(0);
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < count; i += gridDim.x * 256ll) {
+    T s = T(0);
+    for (int m = 0; m < M; ++m) s += D[m * mstride + i];
+    mx = s > mx ? s : mx;
+  }
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] = red[threadIdx.x + k] > red[threadIdx.x]
+                                                ? red[threadIdx.x + k]
+                                                : red[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+}  // namespace ptyrs

@@ -4,7 +4,7 @@ oracle, whose position model and step the reference's fixtures pin
 truth object with every frame moved by its offset, near field (angular spectrum; angular
 spectrum with an evanescent band), three slices, and the far field; the LLK after every epoch,
 the object, probe and offsets at the end. c128 to FP64 rounding, c64 to 1e-3. Plus the feature
-API: held offsets stay, other algorithms and FullObject sampling are refused."""
+API: held offsets stay, the algorithms that do not model them refuse them."""
 
 import numpy as np
 import pytest
@@ -132,8 +132,13 @@ def test_position_feature_api():
     for name in ("ap", "dm", "raar"):
         with pytest.raises(ValueError, match="position"):
             getattr(g, name)(1)
-    with pytest.raises(ValueError):
-        g.set_sampling("full_object")
+    g.set_sampling("full_object")   # held offsets under FullObject: rPIE and LSQML run
+    g.rpie(1, batch_size=5)
+    g.lsqml(1, 1, 6, "poisson", 1)
+    assert np.array_equal(g.features["position"], d0)
+    with pytest.raises(ValueError, match="FullObject"):
+        g.ml(1)
+    g.set_sampling("patch")
     with pytest.raises(ValueError):
         g.set_features(position=d0[:5])
     with pytest.raises(ValueError):
