@@ -31,6 +31,13 @@ What pins each module (the test files named run on the CPU):
 - lsqml.py — tests/test_oracle_lsqml.py (`lsqml_near_*`), tests/test_oracle_scenario.py
   (`lsqml_{gaussian,poisson}_*`, `lsqml_epochs_*`), tests/test_oracle_gradient.py
   (`cost_grads` against PyNX's ML gradient and five-point central differences).
+- opr.py — tests/test_oracle_opr.py restates `ptyrs/tests/engine_opr.rs` on the seeded
+  scenario: per-frame probe costs (`opr_values`, `lsqml_opr_values`, `lsqml_opr_int_values` at
+  1e-13), weight gradients against the fixtures (1e-13) and central differences (1e-8), the OPR
+  step (one eigenmode) and LSQML steps with three eigenmodes, with and without the main mode's
+  weights (steps, changes, eigenmodes, weights at 1e-12/1e-13), a one-frame batch, held
+  weights; LSQML with PATCH offsets and OPR (`lsqml_pos_opr_*`), and the far-field
+  `lsqml_pos_*` step and three epochs.
 - multislice.py — tests/test_oracle_multislice.py (`lsqml_multislice_*`: forward
   intensities, one step, 30 epochs; directions and tangent against central differences).
 - positions.py — tests/test_oracle_positions.py (`lsqml_near_pos_*`,

@@ -60,27 +60,39 @@ def as_slices(obj):
     return list(obj) if isinstance(obj, (list, tuple)) else [obj]
 
 
-def unique_waves(run, views, optics, objects, probe):
-    """The slice waves of `views` (whole frames) at each frame's own probe planes: plane
-    b·np + m of the b-th frame, moved by its offset under PATCH (q = the first slice's plane
-    phase), and past one slice the last slice's phase displaced with it.
-    ptyrs/src/engine/lsqml.rs:483-536, ptyrs/src/engine/opr.rs:88-124"""
+def unique_waves(run, views, optics, objects, probe, var=None):
+    """The slice waves of `views` (whole frames) at each frame's own probe planes (`var`'s
+    OPR planes, or the shared probe's): plane b·np + m of the b-th frame, moved by its offset
+    under PATCH (q = the first slice's plane phase), and past one slice the last slice's phase
+    displaced with it. ptyrs/src/engine/lsqml.rs:483-536, ptyrs/src/engine/opr.rs:88-124"""
     nmode = 0
     while nmode < len(views) and views[nmode][0] == views[0][0]:
         nmode += 1
     frames = [views[k][0] for k in range(0, len(views), nmode)]
+    np_ = probe.shape[0]
+    if var is not None:
+        uviews, planes = var.unique(views, probe)
+    else:
+        uviews = [(f, m, (k // nmode) * np_ + p, o, oy, ox)
+                  for k, (f, m, p, o, oy, ox) in enumerate(views)]
+        planes = None
     if run.offsets is None:
-        return frames, nmode, None, SliceWaves(optics, views, objects, probe)
+        if planes is None:
+            return frames, nmode, None, SliceWaves(optics, views, objects, probe)
+        return frames, nmode, None, SliceWaves(optics, uviews, objects, planes)
     if run.sampling_of(optics, objects[0], probe) == "full_object":
         sm = Sampler(objects[0].shape[-2:], probe.shape[-2:], views,
                      view_offsets(views, run.offsets))
-        return frames, nmode, None, SliceWaves(optics, views, objects, probe, sampler=sm)
-    np_ = probe.shape[0]
+        if planes is None:
+            return frames, nmode, None, SliceWaves(optics, views, objects, probe, sampler=sm)
+        return frames, nmode, None, SliceWaves(optics, uviews, objects, planes, sampler=sm)
     q0 = optics.quad if len(objects) == 1 else None
     moves = [run.offsets[f].copy() for f in frames]
-    planes = np.concatenate([move_planes(probe, q0, d) for d in moves])
-    uviews = [(f, m, (k // nmode) * np_ + p, o, oy, ox)
-              for k, (f, m, p, o, oy, ox) in enumerate(views)]
+    if planes is None:
+        planes = np.concatenate([move_planes(probe, q0, d) for d in moves])
+    else:
+        planes = np.concatenate([move_planes(planes[b * np_:(b + 1) * np_], q0, d)
+                                 for b, d in enumerate(moves)])
     quads = None
     if len(objects) > 1:
         quads = np.stack([moved_quad(optics, moves[k // nmode]) for k in range(len(views))])
@@ -90,12 +102,12 @@ def unique_waves(run, views, optics, objects, probe):
 class LsqmlBatch:
     """One batch's directions at one state. ptyrs/src/engine/lsqml.rs:97-181"""
 
-    def __init__(self, run, views, iobs, optics, obj, probe):
+    def __init__(self, run, views, iobs, optics, obj, probe, var=None):
         cost = run.cost
         ny, nx = probe.shape[-2:]
         self.objects, self.probe, self.optics = as_slices(obj), probe, optics
         self.frames, nmode, self.moves, self.waves = unique_waves(run, views, optics,
-                                                                  self.objects, probe)
+                                                                  self.objects, probe, var)
         self.views, self.nmode = views, nmode
         psi = self.waves.psi
         self.psi = psi
@@ -146,6 +158,11 @@ class LsqmlBatch:
     def frame_probe_directions(self):
         """Plane b·np + m: conj(q·O_view)·chi_m on slice 0. ptyrs/src/engine/lsqml.rs:200-205"""
         return self.waves.probe_adjoint(self.chis[0], self.probe.shape[0])
+
+    def frame_probe_norms(self):
+        """Plane b·np + m: |O_view|² on slice 0 (accumulate_probe's norm).
+        ptyrs/src/engine/ops.rs:312-343"""
+        return self.waves.probe_norms(self.probe.shape[0])
 
     def adjoint_move(self, d):
         """Frame b's planes of `d` moved back by −offset (the adjoint of the probe move).
@@ -260,18 +277,25 @@ class Lsqml:
     def a_far_of(self, f):
         return self.a_far.get(f, 0.5)
 
-    def step(self, views, iobs, optics, obj, probe):
-        """`run_minibatch`: directions, step sizes, updates. ptyrs/src/engine/lsqml.rs:571-643
-        Returns the object in the form given (one array, or a list of slices)."""
+    def step(self, views, iobs, optics, obj, probe, var=None):
+        """`run_minibatch`: directions, step sizes, updates, and `var`'s OPR step (its
+        eigenmodes and the batch's weights) from the state on entry.
+        ptyrs/src/engine/lsqml.rs:571-643. Returns the object in the form given (one array, or
+        a list of slices)."""
         pre = self.preconditioner
         floor = (0.1 * pre.max()) ** 2
-        b = LsqmlBatch(self, views, iobs, optics, obj, probe)
+        b = LsqmlBatch(self, views, iobs, optics, obj, probe, var)
         d_obj = [d / np.sqrt(pre * pre + floor) for d in b.obj_directions()]
-        d_probe = b.probe_direction()
+        np_ = probe.shape[0]
+        d = b.frame_probe_directions()
+        moved = b.adjoint_move(d)
+        d_probe = frame_mean(moved, np_)
         a_obj, a_probe, _ = b.step_sizes(d_obj, d_probe)
         d_pos = b.position_directions() if self.refine else None
         for f, a in zip(b.frames, b.a_far):
             self.a_far[f] = a
+        if var is not None:
+            var.step(b.frames, probe[0], moved, d, b.frame_probe_norms(), np_)
         per_slice = a_obj if len(b.objects) > 1 else [a_obj]
         new = [o + trim_mean(a) * d for o, a, d in zip(as_slices(obj), per_slice, d_obj)]
         probe = probe + (sum(a_probe) / len(a_probe)) * d_probe
@@ -280,7 +304,7 @@ class Lsqml:
         out = new if isinstance(obj, (list, tuple)) else new[0]
         return out, probe, dict(frames=b.frames, a_far=b.a_far, a_obj=a_obj, a_probe=a_probe)
 
-    def epoch(self, addr, iobs, optics, obj, probe):
+    def epoch(self, addr, iobs, optics, obj, probe, var=None):
         """One epoch with pty-chi's preconditioner schedule. ptyrs/src/engine/lsqml.rs:650-714"""
         nframe, nmode = addr.nframe, addr.nmode
         views_all = addr.views()
@@ -304,21 +328,23 @@ class Lsqml:
         steps = []
         for frames in random_batches(nframe, size, self.seed, epoch):
             views = [v for f in frames for v in views_all[f * nmode:(f + 1) * nmode]]
-            obj, probe, st = self.step(views, iobs, optics, obj, probe)
+            obj, probe, st = self.step(views, iobs, optics, obj, probe, var)
             steps.append(st)
         return obj, probe, steps
 
-    def llk(self, ptycho, obj, probe):
+    def llk(self, ptycho, obj, probe, var=None):
         """Normalised LLK of obj, probe over every frame. ptyrs/src/engine/lsqml.rs:538-555"""
         return slice_llk(ptycho.optics, ptycho.views_all, ptycho.iobs, ptycho.addr.nmode,
-                         ptycho.nb_obs, obj, probe, self)
+                         ptycho.nb_obs, obj, probe, self, var)
 
 
-def slice_llk(optics, views, iobs, nmode, nb_obs, obj, probe, run=None):
-    """Normalised LLK of the slice model's exit waves over `views`, at `run`'s offsets if given.
-    ptyrs/src/engine/lsqml.rs:538-555"""
-    if run is not None and run.offsets is not None:
-        waves = unique_waves(run, views, optics, as_slices(obj), probe)[3]
+def slice_llk(optics, views, iobs, nmode, nb_obs, obj, probe, run=None, var=None):
+    """Normalised LLK of the slice model's exit waves over `views`, at `run`'s offsets and
+    `var`'s per-frame probes if given. ptyrs/src/engine/lsqml.rs:538-555"""
+    if (run is not None and run.offsets is not None) or var is not None:
+        if run is None:
+            run = Lsqml(1)
+        waves = unique_waves(run, views, optics, as_slices(obj), probe, var)[3]
     else:
         waves = SliceWaves(optics, views, as_slices(obj), probe)
     psi = optics.exit_forward(waves.psi)

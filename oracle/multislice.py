@@ -119,6 +119,21 @@ class SliceWaves:
                 chi_0[k])
         return out
 
+    def probe_norms(self, np_):
+        """Plane b·np + m: |O_0[win]|² of the b-th frame's view of probe mode m.
+        ptyrs/src/engine/ops.rs:312-343"""
+        ny, nx = self.shape
+        nmode = 0
+        while nmode < len(self.views) and self.views[nmode][0] == self.views[0][0]:
+            nmode += 1
+        nb = len(self.views) // nmode
+        out = np.zeros((nb * np_, ny, nx))
+        obj = self.objects[0]
+        for k, (_, _, p, o, oy, ox) in enumerate(self.views):
+            w = obj[o, oy:oy + ny, ox:ox + nx]
+            out[(k // nmode) * np_ + p % np_] += w.real ** 2 + w.imag ** 2
+        return out
+
     def tangent(self, d_objects, d_probe):
         """The change of the exit waves along `d_objects` (one per slice) and `d_probe`: the
         linearisation of the slice model. ptyrs/src/engine/multislice.rs:359-383"""
