@@ -225,6 +225,27 @@ int ptyrs_engine_ml(ptyrs_engine* e, const ptyrs_ml_params* params, size_t ncycl
 int ptyrs_engine_lsqml(ptyrs_engine* e, const ptyrs_lsqml_params* params, size_t nepoch,
                        size_t calc_llk);
 
+/* ProbeVariation (ptyrs/src/engine/opr.rs:19-90): k >= 1 eigenmodes of probe mode 0
+ * ([k][ny][nx] complex128, interleaved re/im), their weights for every frame ([nframe][k]) and,
+ * unless intensity is NULL, the main mode's weight c_f of every frame ([nframe]); frame f sees
+ * c_f·P_0 + Σ_i w_fi·U_i as its probe mode 0. refine_weights / refine_intensity: refined by each
+ * step, else held (the eigenmodes move either way). */
+typedef struct {
+  size_t k;
+  double* eigenmodes;
+  double* weights;
+  double* intensity;
+  int refine_weights;
+  int refine_intensity;
+} ptyrs_opr;
+
+/* GpuPtycho::lsqml with a ProbeVariation (ptyrs/src/engine/gpu/lsqml.rs:75-120): LSQML epochs
+ * that model pty-chi's orthogonal probe relaxation of probe mode 0; with nepoch >= 1 the
+ * eigenmodes, weights and intensity of `var` are overwritten with the values the epochs leave.
+ * Not with position offsets (PTYRS_EINVAL). */
+int ptyrs_engine_lsqml_opr(ptyrs_engine* e, const ptyrs_lsqml_params* params, size_t nepoch,
+                           size_t calc_llk, ptyrs_opr* var);
+
 /* GpuPtycho::lsqml_steps (ptyrs/src/engine/gpu/lsqml.rs:122-127): the last LSQML epoch's steps,
  * one per frame in batch order (a_obj of slice 0); copies up to cap and sets *n to the number
  * held. */

@@ -92,6 +92,13 @@ class _LsqmlC(ctypes.Structure):
     _fields_ = [("batch_size", ctypes.c_size_t), ("cost", ctypes.c_int), ("seed", ctypes.c_uint64)]
 
 
+class _OprC(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_size_t), ("eigenmodes", ctypes.POINTER(ctypes.c_double)),
+                ("weights", ctypes.POINTER(ctypes.c_double)),
+                ("intensity", ctypes.POINTER(ctypes.c_double)),
+                ("refine_weights", ctypes.c_int), ("refine_intensity", ctypes.c_int)]
+
+
 class _StepC(ctypes.Structure):
     _fields_ = [("batch", ctypes.c_size_t), ("frame", ctypes.c_size_t), ("a_far", ctypes.c_double),
                 ("a_obj", ctypes.c_double), ("a_probe", ctypes.c_double)]
@@ -119,6 +126,8 @@ _SIGNATURES = [
     ("ptyrs_engine_raar", [_P, ctypes.POINTER(_RaarC), _SZ], ctypes.c_int),
     ("ptyrs_engine_ml", [_P, ctypes.POINTER(_MlC), _SZ], ctypes.c_int),
     ("ptyrs_engine_lsqml", [_P, ctypes.POINTER(_LsqmlC), _SZ, _SZ], ctypes.c_int),
+    ("ptyrs_engine_lsqml_opr", [_P, ctypes.POINTER(_LsqmlC), _SZ, _SZ, ctypes.POINTER(_OprC)],
+     ctypes.c_int),
     ("ptyrs_engine_lsqml_steps", [_P, ctypes.POINTER(_StepC), _SZ, ctypes.POINTER(_SZ)],
      ctypes.c_int),
     ("ptyrs_engine_lsqml_slice_steps", [_P, _D, _SZ], ctypes.c_int),
@@ -489,16 +498,52 @@ class GpuPtycho:
 
     def lsqml(self, nepoch, calc_llk, batch_size, cost, seed, **kw):
         """`nepoch` LSQML epochs from a fresh LSQML (`cost` "amplitude" or "poisson"), an LLK
-        record every `calc_llk` epochs and after the last (ptyrs-py/src/gpu.rs:150-192). The
-        probe variation (`opr`, `hold`) is not supported."""
+        record every `calc_llk` epochs and after the last (ptyrs-py/src/gpu.rs:150-192). With
+        `opr=(eigenmodes [k, ny, nx], weights [nframe, k], intensity [nframe] or None)` the
+        epochs model that orthogonal probe relaxation of probe mode 0 (`hold` may name
+        "weights" and "intensity") and return its `(eigenmodes, weights, intensity)` as they
+        leave them; None without `opr`."""
+        opr = kw.pop("opr", None)
+        hold = list(kw.pop("hold", ()))
         if kw:
-            raise TypeError(f"unsupported parameters {sorted(kw)}")
+            raise TypeError(f"unexpected keyword arguments {sorted(kw)}")
         if cost not in ("amplitude", "poisson"):
             raise ValueError(f'cost must be "amplitude" or "poisson", got {cost!r}')
+        for h in hold:
+            if h not in ("weights", "intensity"):
+                raise ValueError(f'hold takes "weights" and "intensity", got {h!r}')
         c = _LsqmlC()
         c.batch_size, c.cost, c.seed = int(batch_size), 0 if cost == "amplitude" else 1, int(seed)
-        _check(library().ptyrs_engine_lsqml(self._h, ctypes.byref(c), int(nepoch), int(calc_llk)))
-        return None
+        if opr is None:
+            if hold:
+                raise ValueError("hold names OPR values but no opr is given")
+            _check(library().ptyrs_engine_lsqml(self._h, ctypes.byref(c), int(nepoch),
+                                                int(calc_llk)))
+            return None
+        eig, w, inten = opr
+        eig = np.array(eig, dtype=np.complex128, order="C")
+        w = np.array(w, dtype=np.float64, order="C")
+        if eig.ndim != 3 or eig.shape[1:] != self._probe_shape[1:] or eig.shape[0] < 1:
+            raise ValueError(f"eigenmodes must be [k, ny, nx] for k >= 1, got {eig.shape}")
+        if w.shape != (self.nframe, eig.shape[0]):
+            raise ValueError(f"weights must be [nframe, k] = {(self.nframe, eig.shape[0])}, "
+                             f"got {w.shape}")
+        if inten is not None:
+            inten = np.array(inten, dtype=np.float64, order="C")
+            if inten.shape != (self.nframe,):
+                raise ValueError(f"{inten.size} main-mode weights for {self.nframe} frames")
+        elif "intensity" in hold:
+            raise ValueError("hold names the intensity but none is given")
+        v = _OprC()
+        v.k = eig.shape[0]
+        v.eigenmodes = _ptr(eig, ctypes.c_double)
+        v.weights = _ptr(w, ctypes.c_double)
+        v.intensity = _ptr(inten, ctypes.c_double) if inten is not None else None
+        v.refine_weights = int("weights" not in hold)
+        v.refine_intensity = int("intensity" not in hold)
+        _check(library().ptyrs_engine_lsqml_opr(self._h, ctypes.byref(c), int(nepoch),
+                                                int(calc_llk), ctypes.byref(v)))
+        return eig, w, inten
 
     def lsqml_steps(self):
         """The last LSQML epoch's steps as dicts of `frames`, `a_far`, `a_obj`

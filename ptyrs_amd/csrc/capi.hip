@@ -496,6 +496,15 @@ int ptyrs_engine_lsqml(ptyrs_engine* h, const ptyrs_lsqml_params* p, size_t nepo
   return guarded(h, true, [&](EngineBase& e) { e.lsqml(*p, nepoch, calc_llk); });
 }
 
+int ptyrs_engine_lsqml_opr(ptyrs_engine* h, const ptyrs_lsqml_params* p, size_t nepoch,
+                           size_t calc_llk, ptyrs_opr* var) {
+  if (!p || !var) {
+    last_error = "null params or variation";
+    return PTYRS_EINVAL;
+  }
+  return guarded(h, true, [&](EngineBase& e) { e.lsqml_opr(*p, nepoch, calc_llk, *var); });
+}
+
 int ptyrs_engine_lsqml_steps(ptyrs_engine* h, ptyrs_lsqml_step* out, size_t cap, size_t* n) {
   return guarded(h, false, [&](EngineBase& e) {
     if (n) *n = e.lsqml_steps.size();

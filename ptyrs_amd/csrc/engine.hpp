@@ -13,6 +13,7 @@
 
 #include "host.hpp"
 #include "kernels.hpp"
+#include "opr.hpp"
 #include "sampling.hpp"
 
 namespace ptyrs {
@@ -162,6 +163,8 @@ class EngineBase {
   virtual bool rpie_pending() const = 0;  // an epoch of the call in progress is left
   hipStream_t stream() const { return stream_; }
   virtual void lsqml(const ptyrs_lsqml_params& p, size_t nepoch, size_t calc_llk) = 0;
+  virtual void lsqml_opr(const ptyrs_lsqml_params& p, size_t nepoch, size_t calc_llk,
+                         ptyrs_opr& var) = 0;
   virtual void project(const ProjParams& p, size_t ncycle) = 0;
   virtual void ml(const ptyrs_ml_params& p, size_t ncycle) = 0;
   virtual void llk(double out[4]) = 0;
@@ -567,6 +570,7 @@ class Engine : public EngineBase {
     ensure_workspace(size, nepoch);
     ensure_lsqml(size);
     if (full_object()) fo_ensure(cap_);
+    if (opr_on_) opr_ensure(cap_);
     std::vector<int> order(nepoch * nframe_);
     std::vector<std::vector<std::pair<int, int>>> spans(nepoch);
     for (size_t ic = 0; ic < nepoch; ++ic) {
@@ -595,7 +599,7 @@ class Engine : public EngineBase {
       const int* ord = eord_.as<int>();
       if (full_object()) {
         for (auto& s : spans[ic]) lsqml_batch_sampled(p, ord + s.first, s.second, s.first);
-      } else if (nslice_ > 1 || pos_on_) {
+      } else if (nslice_ > 1 || pos_on_ || opr_on_) {
         for (auto& s : spans[ic]) lsqml_batch_ms(p, ord + s.first, s.second, s.first);
       } else {
         std::vector<uint64_t> key = {uint64_t(p.cost), cap_};
@@ -1231,6 +1235,144 @@ class Engine : public EngineBase {
     unitary(what);
   }
 
+  // ---- Orthogonal probe relaxation (ptyrs/src/engine/opr.rs) ---------------------------------
+  // During an LSQML call with a ProbeVariation: k eigenmodes of probe mode 0 (opr_U_, [k][n]),
+  // the weights of every frame (opr_w_, [nframe·k] FP64) and the main mode's weights (opr_c_,
+  // [nframe] FP64) when modelled; the stacks read each frame's own planes (stack_waves).
+  bool opr_on_ = false, opr_c_on_ = false, opr_refine_w_ = false, opr_refine_c_ = false;
+  int opr_k_ = 0;
+  DevBuf opr_U_, opr_w_, opr_c_, opr_d_, opr_vv_, opr_r_, opr_e_, opr_part_, opr_pe_, opr_pu_,
+      opr_puu_, opr_sc_;
+  static constexpr int kOprBlocks = 64;
+  int opr_blocks() const {
+    return std::max(1, std::min(kOprBlocks, (n + kOprThreads - 1) / kOprThreads));
+  }
+
+  void opr_ensure(size_t cap) {
+    inc0_.ensure(sizeof(cplx<T>) * cap * M_ * n);
+    opr_d_.ensure(sizeof(cplx<T>) * cap * n);
+    opr_vv_.ensure(sizeof(T) * cap * n);
+    opr_r_.ensure(sizeof(cplx<T>) * cap * n);
+    opr_e_.ensure(sizeof(cplx<T>) * n);
+    opr_part_.ensure(sizeof(double) * 2 * cap * aux_blocks());
+    opr_pe_.ensure(sizeof(double) * kOprBlocks);
+    opr_pu_.ensure(sizeof(double) * kOprBlocks);
+    opr_puu_.ensure(sizeof(double) * kOprBlocks);
+    opr_sc_.ensure(sizeof(double) * cap);
+  }
+
+  void lsqml_opr(const ptyrs_lsqml_params& p, size_t nepoch, size_t calc_llk,
+                 ptyrs_opr& var) override {
+    if (var.k == 0 || !var.eigenmodes || !var.weights)
+      throw Invalid("OPR needs k >= 1 eigenmodes and their weights");
+    if (pos_on_)
+      throw Invalid("LSQML with OPR and position offsets is not run on this GPU engine");
+    if (var.k > 64) throw Invalid("OPR takes at most 64 eigenmodes");
+    const size_t k = var.k;
+    for (size_t i = 0; i < 2 * k * n; ++i)
+      if (!std::isfinite(var.eigenmodes[i])) throw Invalid("eigenmodes must be finite");
+    for (size_t i = 0; i < k * nframe_; ++i)
+      if (!std::isfinite(var.weights[i])) throw Invalid("OPR weights must be finite");
+    for (size_t i = 0; var.intensity && i < nframe_; ++i)
+      if (!std::isfinite(var.intensity[i])) throw Invalid("main-mode weights must be finite");
+    opr_U_.ensure(sizeof(cplx<T>) * k * n);
+    upload_complex(opr_U_, var.eigenmodes, k * n);
+    opr_w_.ensure(sizeof(double) * std::max<size_t>(1, k * nframe_));
+    opr_c_.ensure(sizeof(double) * std::max<size_t>(1, nframe_));
+    if (nframe_) {
+      PTYRS_HIP(hipMemcpy(opr_w_.p, var.weights, sizeof(double) * k * nframe_,
+                          hipMemcpyHostToDevice));
+      if (var.intensity)
+        PTYRS_HIP(hipMemcpy(opr_c_.p, var.intensity, sizeof(double) * nframe_,
+                            hipMemcpyHostToDevice));
+    }
+    opr_k_ = int(k);
+    opr_c_on_ = var.intensity != nullptr;
+    opr_refine_w_ = var.refine_weights != 0;
+    opr_refine_c_ = var.refine_intensity != 0;
+    opr_on_ = true;
+    try {
+      lsqml(p, nepoch, calc_llk);
+    } catch (...) {
+      opr_on_ = false;
+      throw;
+    }
+    opr_on_ = false;
+    if (nepoch == 0 || nframe_ == 0) return;
+    download_complex(opr_U_, var.eigenmodes, k * n);
+    PTYRS_HIP(hipMemcpy(var.weights, opr_w_.p, sizeof(double) * k * nframe_,
+                        hipMemcpyDeviceToHost));
+    if (var.intensity)
+      PTYRS_HIP(hipMemcpy(var.intensity, opr_c_.p, sizeof(double) * nframe_,
+                          hipMemcpyDeviceToHost));
+  }
+
+  // OPRModeWeights.update_opr_probe_modes_and_weights on a batch (ptyrs/src/engine/opr.rs:
+  // 130-237): from slice 0's χ (psi_ after the back-propagation) and the batch-start object and
+  // probe, each slot's mode-0 direction d_b = conj(q·O[win])·χ_b0 and |O[win]|²; the main-mode
+  // weights' step; then, with two slots or more, per eigenmode in order its move along the
+  // residues' principal direction, its renormalisation, its weights' step and the residues'
+  // projection off it.
+  void opr_step(const int* ord, int nb) {
+    const int xb = aux_blocks(), eb = opr_blocks();
+    const cplx<T>* q0 = nslice_ == 1 ? q_.as<cplx<T>>() : nullptr;
+    const int k = opr_k_;
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_opr_frame_dir<T>, dim3((n + 255) / 256, nb), dim3(256), 0, stream_,
+                         opr_d_.as<cplx<T>>(), opr_vv_.as<T>(), (const cplx<T>*)slice_obj(0),
+                         int(nxo_), q0, (const cplx<T>*)psi_.as<cplx<T>>(), corner_.as<int2>(),
+                         ord, N, M_);
+      if (opr_c_on_ && opr_refine_c_) {
+        hipLaunchKernelGGL((k_opr_sums<T, 0>), dim3(xb, nb), dim3(kOprThreads), 0, stream_,
+                           (const cplx<T>*)P_.as<cplx<T>>(), (const cplx<T>*)opr_d_.as<cplx<T>>(),
+                           (const T*)opr_vv_.as<T>(), n, opr_part_.as<double>());
+        hipLaunchKernelGGL(k_opr_intensity, dim3(1), dim3(1), 0, stream_,
+                           (const double*)opr_part_.as<double>(), xb, nb, ord,
+                           opr_c_.as<double>());
+      }
+      if (nb < 2) return;
+      hipLaunchKernelGGL(k_opr_residues<T>, dim3((n + 255) / 256), dim3(256), 0, stream_,
+                         opr_r_.as<cplx<T>>(), (const cplx<T>*)opr_d_.as<cplx<T>>(), nb, n);
+      const double a_u = std::min(0.1, double(nb) / double(nframe_));
+      for (int i = 0; i < k; ++i) {
+        cplx<T>* U = opr_U_.as<cplx<T>>() + size_t(i) * n;
+        hipLaunchKernelGGL((k_opr_sums<T, 1>), dim3(xb, nb), dim3(kOprThreads), 0, stream_,
+                           (const cplx<T>*)U, (const cplx<T>*)opr_r_.as<cplx<T>>(),
+                           (const T*)nullptr, n, opr_part_.as<double>());
+        hipLaunchKernelGGL(k_opr_p, dim3(1), dim3(1), 0, stream_,
+                           (const double*)opr_part_.as<double>(), xb, nb, ord,
+                           (const double*)opr_w_.as<double>(), k, i, double(n),
+                           opr_sc_.as<double>());
+        hipLaunchKernelGGL(k_opr_e<T>, dim3(eb), dim3(kOprThreads), 0, stream_,
+                           opr_e_.as<cplx<T>>(), (const cplx<T>*)opr_r_.as<cplx<T>>(),
+                           (const double*)opr_sc_.as<double>(), nb, n, opr_pe_.as<double>());
+        hipLaunchKernelGGL(k_opr_axpy_norm<T>, dim3(eb), dim3(kOprThreads), 0, stream_, U,
+                           (const cplx<T>*)opr_e_.as<cplx<T>>(),
+                           (const double*)opr_pe_.as<double>(), eb, a_u, n,
+                           opr_pu_.as<double>());
+        hipLaunchKernelGGL(k_opr_normalise<T>, dim3(eb), dim3(kOprThreads), 0, stream_, U,
+                           (const double*)opr_pu_.as<double>(), eb, n, opr_puu_.as<double>());
+        if (opr_refine_w_) {
+          hipLaunchKernelGGL((k_opr_sums<T, 0>), dim3(xb, nb), dim3(kOprThreads), 0, stream_,
+                             (const cplx<T>*)U, (const cplx<T>*)opr_d_.as<cplx<T>>(),
+                             (const T*)opr_vv_.as<T>(), n, opr_part_.as<double>());
+          hipLaunchKernelGGL(k_opr_weights, dim3(1), dim3(1), 0, stream_,
+                             (const double*)opr_part_.as<double>(), xb, nb, ord,
+                             opr_w_.as<double>(), k, i);
+        }
+        if (i + 1 < k) {
+          hipLaunchKernelGGL((k_opr_sums<T, 2>), dim3(xb, nb), dim3(kOprThreads), 0, stream_,
+                             (const cplx<T>*)U, (const cplx<T>*)opr_r_.as<cplx<T>>(),
+                             (const T*)nullptr, n, opr_part_.as<double>());
+          hipLaunchKernelGGL(k_opr_project_out<T>, dim3((n + 255) / 256, nb), dim3(256), 0,
+                             stream_, opr_r_.as<cplx<T>>(), (const cplx<T>*)U,
+                             (const double*)opr_part_.as<double>(), xb,
+                             (const double*)opr_puu_.as<double>(), eb, n);
+        }
+      }
+    });
+  }
+
   // ---- FullObject sampling (ptyrs/src/engine/sampling.rs:79-209) -----------------------------
   // Under offsets and FullObject the probe stays put and each slot reads its window sampled
   // at its frame's corner plus offset: the windows of a batch are laid out as one tall plane
@@ -1514,6 +1656,20 @@ class Engine : public EngineBase {
   };
   Waves stack_waves(const int* ord, int nb) {
     Waves w{P_.as<cplx<T>>(), 0, q_.as<cplx<T>>(), 0};
+    if (opr_on_) {
+      // Each frame's own probe planes (OPR; never with offsets).
+      run(K_PROBE, [&] {
+        hipLaunchKernelGGL(k_opr_planes<T>, dim3((n + 255) / 256, nb * M_), dim3(256), 0,
+                           stream_, inc0_.as<cplx<T>>(), (const cplx<T>*)P_.as<cplx<T>>(),
+                           (const cplx<T>*)opr_U_.as<cplx<T>>(),
+                           (const double*)opr_w_.as<double>(),
+                           opr_c_on_ ? (const double*)opr_c_.as<double>() : nullptr, ord,
+                           opr_k_, M_, n);
+      });
+      w.w0 = inc0_.as<cplx<T>>();
+      w.w0_view = 1;
+      return w;
+    }
     if (!pos_on_) return w;
     moved_probes(ord, nb);
     w.w0 = inc0_.as<cplx<T>>();
@@ -2137,6 +2293,8 @@ class Engine : public EngineBase {
                          dots_part_.as<double>(), xb, nb, lam, int(S) * M_, sums_.as<double>(),
                          log, step, ls);
     });
+    // OPR's eigenmodes and weights from slice 0's χ and the state on entry.
+    if (opr_on_) opr_step(ord, nb);
     const size_t no = nyo_ * nxo_;
     run(K_OBJ_UPDATE, [&] {
       for (size_t s = 0; s < S; ++s)
@@ -2486,7 +2644,7 @@ class Engine : public EngineBase {
         });
         pass_a(ord, nb, false, MixArgs<T>{MIX_NONE, {}, {}, T(0), T(0)}, nullptr,
                fo_win_.as<cplx<T>>(), N, fo_tall_.as<int2>());
-      } else if (nslice_ > 1 || pos_on_) {
+      } else if (nslice_ > 1 || pos_on_ || opr_on_) {
         ms_forward(ord, nb, stack_waves(ord, nb));
         run(K_PASS_A, [&] { rows(false, nb * M_); });
       } else {

@@ -120,4 +120,5 @@ def test_lsqml_rejects_bad_params():
     with pytest.raises(ValueError):
         g.lsqml(1, 0, 4, "gaussian", 1)
     with pytest.raises(TypeError):
-        g.lsqml(1, 0, 4, "poisson", 1, opr=None)
+        g.lsqml(1, 0, 4, "poisson", 1, variation=None)
+    assert g.lsqml(1, 0, 4, "poisson", 1, opr=None) is None   # no probe variation
