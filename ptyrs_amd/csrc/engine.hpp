@@ -1241,8 +1241,8 @@ class Engine : public EngineBase {
   // [nframe] FP64) when modelled; the stacks read each frame's own planes (stack_waves).
   bool opr_on_ = false, opr_c_on_ = false, opr_refine_w_ = false, opr_refine_c_ = false;
   int opr_k_ = 0;
-  DevBuf opr_U_, opr_w_, opr_c_, opr_d_, opr_vv_, opr_r_, opr_e_, opr_part_, opr_pe_, opr_pu_,
-      opr_puu_, opr_sc_;
+  DevBuf opr_U_, opr_w_, opr_c_, opr_d_, opr_m_, opr_vv_, opr_r_, opr_e_, opr_part_, opr_pe_,
+      opr_pu_, opr_puu_, opr_sc_;
   static constexpr int kOprBlocks = 64;
   int opr_blocks() const {
     return std::max(1, std::min(kOprBlocks, (n + kOprThreads - 1) / kOprThreads));
@@ -1251,6 +1251,7 @@ class Engine : public EngineBase {
   void opr_ensure(size_t cap) {
     inc0_.ensure(sizeof(cplx<T>) * cap * M_ * n);
     opr_d_.ensure(sizeof(cplx<T>) * cap * n);
+    opr_m_.ensure(sizeof(cplx<T>) * cap * n);
     opr_vv_.ensure(sizeof(T) * cap * n);
     opr_r_.ensure(sizeof(cplx<T>) * cap * n);
     opr_e_.ensure(sizeof(cplx<T>) * n);
@@ -1265,8 +1266,8 @@ class Engine : public EngineBase {
                  ptyrs_opr& var) override {
     if (var.k == 0 || !var.eigenmodes || !var.weights)
       throw Invalid("OPR needs k >= 1 eigenmodes and their weights");
-    if (pos_on_)
-      throw Invalid("LSQML with OPR and position offsets is not run on this GPU engine");
+    if (full_object())
+      throw Invalid("LSQML with OPR under FullObject sampling is not run on this GPU engine");
     if (var.k > 64) throw Invalid("OPR takes at most 64 eigenmodes");
     const size_t k = var.k;
     for (size_t i = 0; i < 2 * k * n; ++i)
@@ -1331,8 +1332,11 @@ class Engine : public EngineBase {
                            opr_c_.as<double>());
       }
       if (nb < 2) return;
+      // The residues of the directions taken back through the frames' moves (the directions
+      // themselves without offsets).
+      const cplx<T>* moved = pos_on_ ? opr_m_.as<cplx<T>>() : opr_d_.as<cplx<T>>();
       hipLaunchKernelGGL(k_opr_residues<T>, dim3((n + 255) / 256), dim3(256), 0, stream_,
-                         opr_r_.as<cplx<T>>(), (const cplx<T>*)opr_d_.as<cplx<T>>(), nb, n);
+                         opr_r_.as<cplx<T>>(), moved, nb, n);
       const double a_u = std::min(0.1, double(nb) / double(nframe_));
       for (int i = 0; i < k; ++i) {
         cplx<T>* U = opr_U_.as<cplx<T>>() + size_t(i) * n;
@@ -1657,21 +1661,45 @@ class Engine : public EngineBase {
   Waves stack_waves(const int* ord, int nb) {
     Waves w{P_.as<cplx<T>>(), 0, q_.as<cplx<T>>(), 0};
     if (opr_on_) {
-      // Each frame's own probe planes (OPR; never with offsets).
+      // Each frame's own probe planes (OPR), moved by its offset with offsets:
+      // conj(q0)·DFT⁻¹(DFT(q0·P_b)·exp(−2πi f·d_b)) (ptyrs/src/engine/lsqml.rs:507-520).
+      const int K = nb * M_;
+      cplx<T>* pl = inc0_.as<cplx<T>>();
+      const cplx<T>* q0 = nslice_ == 1 ? q_.as<cplx<T>>() : nullptr;
       run(K_PROBE, [&] {
-        hipLaunchKernelGGL(k_opr_planes<T>, dim3((n + 255) / 256, nb * M_), dim3(256), 0,
-                           stream_, inc0_.as<cplx<T>>(), (const cplx<T>*)P_.as<cplx<T>>(),
+        hipLaunchKernelGGL(k_opr_planes<T>, dim3((n + 255) / 256, K), dim3(256), 0, stream_, pl,
+                           (const cplx<T>*)P_.as<cplx<T>>(),
                            (const cplx<T>*)opr_U_.as<cplx<T>>(),
                            (const double*)opr_w_.as<double>(),
                            opr_c_on_ ? (const double*)opr_c_.as<double>() : nullptr, ord,
                            opr_k_, M_, n);
+        if (pos_on_) {
+          if (q0)
+            hipLaunchKernelGGL((k_times_q<T, false>), dim3((n + 255) / 256, K), dim3(256), 0,
+                               stream_, pl, q0, n);
+          hipLaunchKernelGGL((k_rows<T, N, false>), dim3(S::NBLK, K), dim3(S::NT), 0, stream_,
+                             pl, tw_.as<cplx<T>>());
+          hipLaunchKernelGGL((k_cols<T, N, false>), dim3(SC::NBLK, K), dim3(SC::NT), 0,
+                             stream_, pl, tw_.as<cplx<T>>());
+          hipLaunchKernelGGL(k_ramp_planes<T>, dim3((n + 255) / 256, K), dim3(256), 0, stream_,
+                             pl, (const cplx<T>*)pl, (const double2*)offs_.as<double2>(), ord, N,
+                             M_, 1.0 / double(n), 1);
+          hipLaunchKernelGGL((k_cols<T, N, true>), dim3(SC::NBLK, K), dim3(SC::NT), 0, stream_,
+                             pl, tw_.as<cplx<T>>());
+          hipLaunchKernelGGL((k_rows<T, N, true>), dim3(S::NBLK, K), dim3(S::NT), 0, stream_,
+                             pl, tw_.as<cplx<T>>());
+          if (q0)
+            hipLaunchKernelGGL((k_times_q<T, true>), dim3((n + 255) / 256, K), dim3(256), 0,
+                               stream_, pl, q0, n);
+        }
       });
-      w.w0 = inc0_.as<cplx<T>>();
+      w.w0 = pl;
       w.w0_view = 1;
-      return w;
+      if (!pos_on_) return w;
+    } else {
+      if (!pos_on_) return w;
+      moved_probes(ord, nb);
     }
-    if (!pos_on_) return w;
-    moved_probes(ord, nb);
     w.w0 = inc0_.as<cplx<T>>();
     w.w0_view = 1;
     if (nslice_ > 1 && (plane_quad_[0] != 0.0 || plane_quad_[1] != 0.0)) {
@@ -2272,6 +2300,20 @@ class Engine : public EngineBase {
         if (q0)
           hipLaunchKernelGGL((k_times_q<T, true>), dim3((n + 255) / 256, M_), dim3(256), 0,
                              stream_, dPm_.as<cplx<T>>(), q0, n);
+        if (opr_on_) {
+          // Each frame's mode-0 direction moved back, for OPR's residues.
+          cplx<T>* mv = opr_m_.as<cplx<T>>();
+          hipLaunchKernelGGL(k_unramp_planes<T>, dim3((n + 255) / 256, nb), dim3(256), 0,
+                             stream_, mv, (const cplx<T>*)g, (const double2*)offs_.as<double2>(),
+                             ord, N, M_, 0, 1.0 / double(n));
+          hipLaunchKernelGGL((k_cols<T, N, true>), dim3(SC::NBLK, nb), dim3(SC::NT), 0, stream_,
+                             mv, tw_.as<cplx<T>>());
+          hipLaunchKernelGGL((k_rows<T, N, true>), dim3(S::NBLK, nb), dim3(S::NT), 0, stream_,
+                             mv, tw_.as<cplx<T>>());
+          if (q0)
+            hipLaunchKernelGGL((k_times_q<T, true>), dim3((n + 255) / 256, nb), dim3(256), 0,
+                               stream_, mv, q0, n);
+        }
       });
     } else {
       run(K_PROBE, [&] {

@@ -1674,15 +1674,16 @@ __device__ __forceinline__ double dft_freq(int k, int len) {
 
 // Spectra moved by the slots' offsets: out[b·M + m] = spec[m]·exp(−2πi f·d_b)·scale (the probe
 // move's translation phase, ptyrs/src/engine/features.rs:362-368), d_b = offs[order[b]] in
-// object pixels [dy, dx]. Grid (ceil(n/256), nb·M).
+// object pixels [dy, dx]; with per_view every view's own spectrum spec[b·M + m] (out may be
+// spec). Grid (ceil(n/256), nb·M).
 template <typename T>
-__global__ void k_ramp_planes(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ spec,
-                              const double2* __restrict__ offs, const int* __restrict__ order,
-                              int N, int M, double scale) {
+__global__ void k_ramp_planes(cplx<T>* out, const cplx<T>* spec, const double2* __restrict__ offs,
+                              const int* __restrict__ order, int N, int M, double scale,
+                              int per_view = 0) {
   const int n = N * N;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  const int k = blockIdx.y, b = k / M, m = k % M;
+  const int k = blockIdx.y, b = k / M, m = per_view ? k : k % M;
   const double2 d = offs[order[b]];
   const double ph = -2.0 * M_PI * (dft_freq(j / N, N) * d.x + dft_freq(j % N, N) * d.y);
   double s, c;
@@ -1717,6 +1718,25 @@ __global__ void k_unramp_mean(cplx<T>* __restrict__ out, const cplx<T>* __restri
   cplx<T> r{T(double(sr.s) * scale), T(double(si.s) * scale)};
   if (accumulate) r = r + ld(out + (size_t)m * n + j);
   st(out + (size_t)m * n + j, r);
+}
+
+// Each slot's spectrum of mode m moved back: out[b] = scale·G[b·M + m]·exp(+2πi f·d_b) (the
+// adjoint move of a frame's direction, ptyrs/src/engine/lsqml.rs:207-222). Grid (ceil(n/256),
+// nb).
+template <typename T>
+__global__ void k_unramp_planes(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ G,
+                                const double2* __restrict__ offs, const int* __restrict__ order,
+                                int N, int M, int m, double scale) {
+  const int n = N * N;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int b = blockIdx.y;
+  const double2 d = offs[order[b]];
+  double s, c;
+  sincos(2.0 * M_PI * (dft_freq(j / N, N) * d.x + dft_freq(j % N, N) * d.y), &s, &c);
+  const cplx<T> v = ld(G + ((size_t)b * M + m) * n + j);
+  st(out + (size_t)b * n + j, cplx<T>{T((double(v.x) * c - double(v.y) * s) * scale),
+                                      T((double(v.x) * s + double(v.y) * c) * scale)});
 }
 
 // The moved waves' derivatives over the offset: out[2k + a] = spec[m]·exp(−2πi f·d_b)·(−2πi

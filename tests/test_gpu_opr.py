@@ -3,7 +3,8 @@ oracle, whose OPR step the reference's fixtures pin (tests/test_oracle_opr.py): 
 seeded far-field scenario with its fixtures' eigenmodes and weights (one eigenmode; three; three
 with the main mode's weights), both costs, weights and intensity refined or held, two epochs:
 the LLK records, object, probe, eigenmodes, weights and main-mode weights. Also three slices
-(the angular-spectrum multislice fixture state). complex128 to FP64 rounding (1e-9), complex64
+(the angular-spectrum multislice fixture state) and refined PATCH offsets (the reference's
+lsqml_pos_opr state). complex128 to FP64 rounding (1e-9), complex64
 to 1e-3 of the changes."""
 
 import numpy as np
@@ -103,6 +104,39 @@ def test_opr_multislice_matches_oracle(dtype):
     assert max_rel((gw - w0).ravel(), (var.weights.reshape(-1, 2) - w0).ravel()) < tol * 10
 
 
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("cost", [AMPLITUDE, POISSON])
+def test_opr_with_positions_matches_oracle(dtype, cost):
+    # OPR with refined PATCH offsets: the reference's lsqml_pos_opr state (three eigenmodes,
+    # its weights and offsets), two epochs of batches of 10.
+    from test_oracle_opr import state
+    p = state()
+    eig = np.array(table("lsqml_pos_opr_eigen_f64.txt"))
+    wt = np.array(table("lsqml_pos_opr_weights_f64.txt"))
+    offs = np.array(table("lsqml_pos_opr_offsets_f64.txt"))[:, 1:3]
+    u0 = (eig[:, 3] + 1j * eig[:, 4]).reshape(3, 32, 32)
+    w0 = wt[:, 1:4].copy()
+    var = ProbeVariation(u0, w0.ravel())
+    run = Lsqml(10, cost, 3, offsets=offs, refine=True, sampling="patch")
+    obj, probe, llks = p.obj, p.probe, []
+    for _ in range(2):
+        obj, probe, _ = run.epoch(p.addr, p.iobs, p.optics, obj, probe, var)
+        llks.append(slice_llk(p.optics, p.views_all, p.iobs, p.addr.nmode, p.nb_obs, obj, probe,
+                              run, var).poisson)
+    g = engine(dtype)
+    obj0 = g.object
+    g.set_features(position=offs)
+    geig, gw, _ = g.lsqml(2, 1, 10, cost, 3, opr=(u0, w0, None))
+    tol = TOL[dtype]
+    print("opr pos", dtype, cost, max_rel(g.history["poisson"], llks), rel_l2(geig, var.eigenmodes))
+    assert max_rel(g.history["poisson"], llks) < tol
+    assert rel_l2(g.object - obj0, obj - obj0) < tol * 10
+    assert rel_l2(geig, var.eigenmodes) < tol * 10
+    assert max_rel((gw - w0).ravel(), (var.weights.reshape(-1, 3) - w0).ravel()) < tol * 10
+    got = g.features["position"]
+    assert max_rel((got - offs).ravel(), (run.offsets - offs).ravel()) < tol * 100
+
+
 def test_opr_refusals():
     g = engine("c64")
     k = 2
@@ -117,6 +151,7 @@ def test_opr_refusals():
     with pytest.raises(TypeError):
         g.lsqml(1, 1, 8, "poisson", 1, variation=None)
     g.set_features(position=np.zeros((40, 2)))
+    g.set_sampling("full_object")
     with pytest.raises(ValueError, match="OPR"):
         g.lsqml(1, 1, 8, "poisson", 1, opr=(eig + 1.0, w, None))
     assert g.cycle == 0
