@@ -255,6 +255,19 @@ int ptyrs_engine_lsqml_steps(ptyrs_engine* e, ptyrs_lsqml_step* out, size_t cap,
  * of slice s at step i of ptyrs_engine_lsqml_steps; len must be n·nslice. */
 int ptyrs_engine_lsqml_slice_steps(ptyrs_engine* e, double* out, size_t len);
 
+/* Launch shapes that change results only by rounding, no counterpart in the reference (whose
+ * GpuOptions::stack_size and StackRule cover the first): stack_size frames per stack of AP, DM,
+ * RAAR, ML and the LLK pass (0: the engine's default), and the workgroups per launch rPIE's
+ * (pass_c_blocks) and LSQML's (chi_blocks) pass C cut a batch into chunks for (0: the
+ * defaults, 768 and 1024; at most 8192). For ptyrs_amd.tune, which times them per size class. */
+typedef struct {
+  size_t stack_size;
+  int pass_c_blocks;
+  int chi_blocks;
+} ptyrs_launch;
+int ptyrs_engine_set_launch(ptyrs_engine* e, const ptyrs_launch* launch);
+int ptyrs_engine_launch(ptyrs_engine* e, ptyrs_launch* out);
+
 /* GpuPtycho::set_features with Features::position only (ptyrs/src/engine/gpu/mod.rs:846-857,
  * ptyrs/src/engine/features.rs:304-330): offsets [nframe][2] (dy, dx) in object pixels,
  * len = 2·nframe, refined by every rPIE batch, LSQML step or ML cycle when refine != 0, else

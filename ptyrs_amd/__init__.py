@@ -99,6 +99,11 @@ class _OprC(ctypes.Structure):
                 ("refine_weights", ctypes.c_int), ("refine_intensity", ctypes.c_int)]
 
 
+class _LaunchC(ctypes.Structure):
+    _fields_ = [("stack_size", ctypes.c_size_t), ("pass_c_blocks", ctypes.c_int),
+                ("chi_blocks", ctypes.c_int)]
+
+
 class _StepC(ctypes.Structure):
     _fields_ = [("batch", ctypes.c_size_t), ("frame", ctypes.c_size_t), ("a_far", ctypes.c_double),
                 ("a_obj", ctypes.c_double), ("a_probe", ctypes.c_double)]
@@ -128,6 +133,8 @@ _SIGNATURES = [
     ("ptyrs_engine_lsqml", [_P, ctypes.POINTER(_LsqmlC), _SZ, _SZ], ctypes.c_int),
     ("ptyrs_engine_lsqml_opr", [_P, ctypes.POINTER(_LsqmlC), _SZ, _SZ, ctypes.POINTER(_OprC)],
      ctypes.c_int),
+    ("ptyrs_engine_set_launch", [_P, ctypes.POINTER(_LaunchC)], ctypes.c_int),
+    ("ptyrs_engine_launch", [_P, ctypes.POINTER(_LaunchC)], ctypes.c_int),
     ("ptyrs_engine_lsqml_steps", [_P, ctypes.POINTER(_StepC), _SZ, ctypes.POINTER(_SZ)],
      ctypes.c_int),
     ("ptyrs_engine_lsqml_slice_steps", [_P, _D, _SZ], ctypes.c_int),
@@ -422,10 +429,17 @@ class GpuPtycho:
 
     def _clone(self):
         """A new engine on the same scan, device, optics and options, starting from this
-        engine's current object and probe (its own history and cycle count start empty)."""
+        engine's current object, probe, position offsets, sampling and launch shapes (its own
+        history and cycle count start empty)."""
         scan, device, optics, options = self._inputs
         obj = self.objects if self.nslice > 1 else self.object
-        return GpuPtycho(scan, obj, self.probe, device, optics, **options)
+        g = GpuPtycho(scan, obj, self.probe, device, optics, **options)
+        f = self.features
+        if f["position"] is not None:
+            g.set_features(position=f["position"], hold=f["hold"])
+        g.set_sampling(self.sampling)
+        g.set_launch(**self.launch)
+        return g
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -691,6 +705,28 @@ class GpuPtycho:
         position = out if state.value else None
         return dict(scale=None, background=None, position=position,
                     hold=["position"] if state.value == 1 else [])
+
+    @property
+    def launch(self):
+        """The launch shapes that change results only by rounding: `stack_size` (frames per
+        stack of AP, DM, RAAR, ML and the LLK pass, 0 the default), `pass_c_blocks` and
+        `chi_blocks` (workgroups per launch rPIE's and LSQML's pass C aim at)."""
+        c = _LaunchC()
+        _check(library().ptyrs_engine_launch(self._h, ctypes.byref(c)))
+        return dict(stack_size=c.stack_size, pass_c_blocks=c.pass_c_blocks,
+                    chi_blocks=c.chi_blocks)
+
+    def set_launch(self, **kw):
+        """Sets any of `launch`'s keys (0 restores a default)."""
+        unknown = set(kw) - {"stack_size", "pass_c_blocks", "chi_blocks"}
+        if unknown:
+            raise TypeError(f"unexpected keyword arguments {sorted(unknown)}")
+        cur = self.launch
+        cur.update(kw)
+        c = _LaunchC()
+        c.stack_size = int(cur["stack_size"])
+        c.pass_c_blocks, c.chi_blocks = int(cur["pass_c_blocks"]), int(cur["chi_blocks"])
+        _check(library().ptyrs_engine_set_launch(self._h, ctypes.byref(c)))
 
     @property
     def sampling(self):

@@ -524,6 +524,22 @@ int ptyrs_engine_lsqml_slice_steps(ptyrs_engine* h, double* out, size_t len) {
   });
 }
 
+int ptyrs_engine_set_launch(ptyrs_engine* h, const ptyrs_launch* l) {
+  if (!l) {
+    last_error = "null launch";
+    return PTYRS_EINVAL;
+  }
+  return guarded(h, true, [&](EngineBase& e) {
+    e.set_launch(l->stack_size, l->pass_c_blocks, l->chi_blocks);
+  });
+}
+
+int ptyrs_engine_launch(ptyrs_engine* h, ptyrs_launch* out) {
+  return guarded(h, false, [&](EngineBase& e) {
+    if (out) e.get_launch(&out->stack_size, &out->pass_c_blocks, &out->chi_blocks);
+  });
+}
+
 int ptyrs_engine_set_position(ptyrs_engine* h, const double* offsets, size_t len, int refine) {
   return guarded(h, true, [&](EngineBase& e) {
     if (offsets && len != e.corners.size())  // 2·nframe

@@ -165,6 +165,11 @@ class EngineBase {
   virtual void lsqml(const ptyrs_lsqml_params& p, size_t nepoch, size_t calc_llk) = 0;
   virtual void lsqml_opr(const ptyrs_lsqml_params& p, size_t nepoch, size_t calc_llk,
                          ptyrs_opr& var) = 0;
+  // Launch shapes that change results only by rounding (the autotuner's knobs): frames per
+  // stack of AP, DM, RAAR, ML and the LLK pass (0: the default), and the workgroups per launch
+  // that rPIE's and LSQML's pass C cut a batch into chunks for (0: the default).
+  virtual void set_launch(size_t stack_size, int pass_c_blocks, int chi_blocks) = 0;
+  virtual void get_launch(size_t* stack_size, int* pass_c_blocks, int* chi_blocks) const = 0;
   virtual void project(const ProjParams& p, size_t ncycle) = 0;
   virtual void ml(const ptyrs_ml_params& p, size_t ncycle) = 0;
   virtual void llk(double out[4]) = 0;
@@ -233,6 +238,7 @@ class EngineBase {
 
  protected:
   EpochGraph rpie_graph_, lsqml_graph_, proj_graph_, ml_graph_;
+  virtual uint64_t launch_key() const = 0;
 
   // Replays `g` if its key matches, else captures `body` (the launches of one epoch on the
   // engine's stream, no allocation, memset or host wait inside) into it and launches it.
@@ -244,6 +250,7 @@ class EngineBase {
       return;
     }
     key.push_back(DevBuf::generation.load());
+    key.push_back(uint64_t(launch_key()));
     if (g.exec == nullptr || g.key != key) {
       if (g.exec) {
         PTYRS_HIP(hipGraphExecDestroy(g.exec));
@@ -2518,14 +2525,36 @@ class Engine : public EngineBase {
   // Pass C chunking: enough workgroups to fill the chip, few partial probe sums.
   int chunk_of(int nb) const {
     const int per = SR::NBLK * M_;
-    int nchunk = std::max(1, std::min(nb, (1024 + per - 1) / per));
+    int nchunk = std::max(1, std::min(nb, (chi_blocks_ + per - 1) / per));
     return (nb + nchunk - 1) / nchunk;
   }
   // The same for rPIE's mode-summing pass C, whose workgroups hold G rows of every mode.
   int chunk_of_m(int nb) const {
     const int per = N / rows_per_group(N, SR::L, int(M_));
-    int nchunk = std::max(1, std::min(nb, (kPassCBlocks + per - 1) / per));
+    int nchunk = std::max(1, std::min(nb, (pass_c_blocks_ + per - 1) / per));
     return (nb + nchunk - 1) / nchunk;
+  }
+  int pass_c_blocks_ = kPassCBlocks, chi_blocks_ = kChiBlocks;
+  static constexpr int kChiBlocks = 1024;
+  static constexpr int kMaxLaunchBlocks = 8192;
+  uint64_t launch_key() const override {
+    return (uint64_t(pass_c_blocks_) << 32) | uint64_t(chi_blocks_);
+  }
+  void set_launch(size_t stack_size, int pass_c_blocks, int chi_blocks) override {
+    if (pass_c_blocks < 0 || pass_c_blocks > kMaxLaunchBlocks || chi_blocks < 0 ||
+        chi_blocks > kMaxLaunchBlocks)
+      throw Invalid("pass C workgroup targets must be in 0.." + std::to_string(kMaxLaunchBlocks));
+    if (stack_size * size_t(M_) > 65535)
+      throw Invalid("a stack of " + std::to_string(stack_size) + " frames of " +
+                    std::to_string(M_) + " modes holds more than 65535 views");
+    stack_size_ = stack_size;
+    pass_c_blocks_ = pass_c_blocks ? pass_c_blocks : kPassCBlocks;
+    chi_blocks_ = chi_blocks ? chi_blocks : kChiBlocks;
+  }
+  void get_launch(size_t* stack_size, int* pass_c_blocks, int* chi_blocks) const override {
+    if (stack_size) *stack_size = stack_size_;
+    if (pass_c_blocks) *pass_c_blocks = pass_c_blocks_;
+    if (chi_blocks) *chi_blocks = chi_blocks_;
   }
 
   // Scratch layout of llk_noalloc: the per-workgroup sums (4 doubles per frame and pass-B
@@ -2581,7 +2610,8 @@ class Engine : public EngineBase {
     // chunk_of and chunk_of_m never cut more chunks than this, whatever the batch length.
     const size_t per =
         std::min(size_t(SR::NBLK) * M_, size_t(N / rows_per_group(N, SR::L, int(M_))));
-    const size_t nchunk = std::min(cap, (size_t(std::max(kPassCBlocks, 1024)) + per - 1) / per);
+    const size_t nchunk =
+        std::min(cap, (size_t(std::max(pass_c_blocks_, chi_blocks_)) + per - 1) / per);
     dP_.ensure(sizeof(cplx<T>) * std::max<size_t>(nchunk, 1) * M_ * n);
     list_.ensure(sizeof(int) * size_t(ntiles()) * cap);
     lcount_.ensure(sizeof(int) * size_t(ntiles()));

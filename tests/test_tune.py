@@ -57,3 +57,45 @@ def test_tuner_caches_its_choice(tmp_path, monkeypatch):
     assert r["width"] in (1, 2) and not r["cached"]
     again = tune.tune_group(es, widths=(1, 2))
     assert again["cached"] and again["width"] == r["width"]
+
+
+def fake_engine(times, algorithm_calls, nframe=300):
+    """A stand-in engine whose copies run `algorithm` in `times[launch value]` seconds."""
+    state = {"launch": dict(stack_size=0, pass_c_blocks=768, chi_blocks=1024)}
+    g = types.SimpleNamespace(_probe_shape=(4, 256, 256), nframe=nframe, dtype="c64",
+                              nslice=1, object=types.SimpleNamespace(shape=(1, 1000, 1000)),
+                              synchronize=lambda: None)
+    g.launch = state["launch"]
+
+    def set_launch(**kw):
+        state["launch"].update(kw)
+    g.set_launch = set_launch
+
+    def clone():
+        c = fake_engine(times, algorithm_calls, nframe)
+        c.copy = True
+        return c
+    g._clone = clone
+
+    def run(*args, **kw):
+        algorithm_calls.append(dict(state["launch"]))
+        import time as _t
+        key = state["launch"]["pass_c_blocks"]
+        _t.sleep(times.get(key, 0.0))
+    g.rpie = run
+    return g
+
+
+def test_engine_tuner_times_copies_and_caches(tmp_path, monkeypatch):
+    monkeypatch.setattr(tune, "CACHE", tmp_path / "t.json")
+    calls = []
+    g = fake_engine({512: 0.03, 768: 0.001, 1024: 0.02}, calls)
+    r = tune.tune_engine(g, "rpie", {"batch_size": 100}, values=(512, 768, 1024))
+    assert r["name"] == "pass_c_blocks" and r["value"] == 768 and not r["cached"]
+    assert len(calls) == 6   # one untimed and one timed epoch per candidate, on the copy
+    assert g.launch["pass_c_blocks"] == 768
+    assert tune.engine_class(g, "rpie") == "rpie:256x4:s1:c64:1024x1024:512"
+    again = tune.tune_engine(g, "rpie", {"batch_size": 100})
+    assert again["cached"] and again["value"] == 768 and len(calls) == 6
+    with pytest.raises(ValueError):
+        tune.tune_engine(g, "bh")
