@@ -336,16 +336,22 @@ class Ptycho:
 
     def projected_position_sums(self):
         """Per frame [num, den] of the position step from the magnitude projection of the
-        current estimate's exit waves x: num_a = Re Σ conj(x − W·O)·O·∂_a W, den_a = Σ |O·∂_a W|²
-        over the frame's views, ∂_a W = DFT⁻¹(DFT(P·q)·exp(−2πi f·d)·(−2πi f_a)).
-        ptyrs/src/engine/ptycho.rs:468-479, ptyrs/src/engine/features.rs:960-1025"""
+        current estimate's exit waves (`position_sums` of them).
+        ptyrs/src/engine/ptycho.rs:468-479"""
         self._patch_only("ML")
+        x = self.optics.exit_forward(self._moved_psi(self.views_all)[1])
+        x = apply_amplitude(self.iobs, self.addr.nmode, x)
+        return self.position_sums(self.optics.exit_adjoint(x))
+
+    def position_sums(self, x):
+        """Per frame [num, den] of the position step of new exit waves x (shifted layout):
+        num_a = Re Σ conj(x − W·O)·O·∂_a W, den_a = Σ |O·∂_a W|² over the frame's views,
+        ∂_a W = DFT⁻¹(DFT(P·q)·exp(−2πi f·d)·(−2πi f_a)), W and O those of the current arrays.
+        ptyrs/src/engine/features.rs:960-1025"""
         ny, nx = self.probe.shape[-2:]
         views = self.views_all
-        w, psi = self._moved_psi(views)
-        x = self.optics.exit_forward(psi)
-        x = apply_amplitude(self.iobs, self.addr.nmode, x)
-        x = to_natural(self.optics.exit_adjoint(x))
+        w = self.moved_waves(views)
+        x = to_natural(x)
         spec = np.fft.fft2(self.probe * self.quad)
         fy = freq(np.arange(ny), ny)[:, None] * np.ones((1, nx))
         fx = np.ones((ny, 1)) * freq(np.arange(nx), nx)[None, :]
@@ -377,9 +383,71 @@ class Ptycho:
     def zero_phase_ramp(self):
         self.obj, self.probe = zero_phase_ramp(self.obj, self.probe)
 
+    def _update_from_psi(self, obj_inertia, probe_inertia):
+        """psi_to_obj_probe of self.psi, through the moved waves under the offsets.
+        ptyrs/src/engine/ptycho.rs:387-410"""
+        if self.offsets is not None:
+            self._patch_only("AP, DM and RAAR")
+            self._moved_psi_to_obj_probe(obj_inertia, probe_inertia)
+        else:
+            self.obj, self.probe = psi_to_obj_probe(self.views_all, self.quad, self.psi,
+                                                    self.obj, self.probe, obj_inertia,
+                                                    probe_inertia)
+
+    def _moved_psi_to_obj_probe(self, obj_inertia, probe_inertia):
+        """psi_to_obj_probe under PATCH offsets, both from the arrays on entry: the probe takes
+        the first conjugate-gradient step of min ‖psi − A·P'‖² + reg·‖P' − P‖² preconditioned by
+        Σ|O|² + reg (A the moved-wave model, reg = inertia·max Σ|O|²): s = g/D with g the
+        residual back-projected through the moves, P += (<g, s>/(‖A·s‖² + reg·‖s‖²))·s per mode
+        where <g, s> > 0; the object blends Σ psi·conj(W) and Σ|W|² with the waves of the probe
+        on entry. ptyrs/src/engine/features.rs:653-720, :851-958"""
+        ny, nx = self.probe.shape[-2:]
+        views = self.views_all
+        psi_n = to_natural(self.psi)
+        w = self.moved_waves(views)
+        win = np.stack([self.obj[o, oy:oy + ny, ox:ox + nx] for (_, _, _, o, oy, ox) in views])
+        new_probe = self.probe
+        if probe_inertia is not None:
+            resid = psi_n - w * win
+            nmode = self.addr.nmode
+            spec = np.zeros_like(self.probe)
+            norm = np.zeros(self.probe.shape)
+            for f in range(self.addr.nframe):
+                y = np.zeros_like(self.probe)
+                for k in range(f * nmode, (f + 1) * nmode):
+                    pm = views[k][2]
+                    y[pm] += np.conj(win[k]) * resid[k]
+                    norm[pm] += win[k].real ** 2 + win[k].imag ** 2
+                spec += np.fft.fft2(y) * ramp((ny, nx), -self.offsets[f])
+            num = np.conj(self.quad) * np.fft.ifft2(spec)
+            reg = float(norm.max()) * probe_inertia
+            d = np.maximum(norm + reg, 1e-12)
+            slope = [float((np.abs(num[m]) ** 2 / d[m]).sum()) for m in range(num.shape[0])]
+            step = num / d
+            curv = [reg * float((np.abs(step[m]) ** 2).sum()) for m in range(num.shape[0])]
+            keep = self.probe
+            self.probe = step
+            ws = self.moved_waves(views)
+            self.probe = keep
+            for k, v in enumerate(views):
+                curv[v[2]] += float((np.abs(ws[k] * win[k]) ** 2).sum())
+            new_probe = self.probe.copy()
+            for m in range(num.shape[0]):
+                if slope[m] > 0.0:
+                    new_probe[m] = new_probe[m] + step[m] * (slope[m] / curv[m])
+        if obj_inertia is not None:
+            num = np.zeros_like(self.obj)
+            norm = np.zeros(self.obj.shape)
+            for k, (_, _, _, o, oy, ox) in enumerate(views):
+                num[o, oy:oy + ny, ox:ox + nx] += psi_n[k] * np.conj(w[k])
+                norm[o, oy:oy + ny, ox:ox + nx] += w[k].real ** 2 + w[k].imag ** 2
+            self.obj = blend(self.obj, num, norm, obj_inertia)
+        self.probe = new_probe
+
     def ap(self, ncycle, obj_inertia=0.01, probe_inertia=None, calc_llk=0,
            zero_phase_ramp=True):
-        """PyNX AP. ptyrs/src/engine/ptycho.rs:605-641"""
+        """PyNX AP; under refined offsets the position sums of the projection, then the
+        update, then the position step. ptyrs/src/engine/ptycho.rs:605-641"""
         nmode = self.addr.nmode
         for ic in range(ncycle):
             self.psi = self.optics.exit_forward(self.calc_psi())
@@ -388,9 +456,10 @@ class Ptycho:
                 rec = llk(self.iobs, nmode, self.psi).normalised(self.nb_obs)
             self.psi = apply_amplitude(self.iobs, nmode, self.psi)
             self.psi = self.optics.exit_adjoint(self.psi)
-            self.obj, self.probe = psi_to_obj_probe(self.views_all, self.quad, self.psi,
-                                                    self.obj, self.probe, obj_inertia,
-                                                    probe_inertia)
+            sums = self.position_sums(self.psi) if self.refine else None
+            self._update_from_psi(obj_inertia, probe_inertia)
+            if sums is not None:
+                position_step(self.offsets, sums)
             if rec is not None:
                 self._record(AP, rec)
             self.cycle += 1
@@ -398,12 +467,16 @@ class Ptycho:
             self.zero_phase_ramp()
 
     def _update_obj_probe(self, inertia, loops):
-        """ptyrs/src/engine/ptycho.rs:733-755 (no positions)"""
+        """DM's and RAAR's update: the position sums while refined, `loops` times an object-only
+        then a probe-only psi_to_obj_probe, then the position step.
+        ptyrs/src/engine/ptycho.rs:733-755"""
+        sums = self.position_sums(self.psi) if self.refine else None
         for _ in range(loops):
             for o, p in ((inertia[0], None), (None, inertia[1])):
                 if o is not None or p is not None:
-                    self.obj, self.probe = psi_to_obj_probe(
-                        self.views_all, self.quad, self.psi, self.obj, self.probe, o, p)
+                    self._update_from_psi(o, p)
+        if sums is not None:
+            position_step(self.offsets, sums)
 
     def dm(self, ncycle, obj_inertia=0.01, probe_inertia=0.001, alpha=0.02, loop_obj_probe=1,
            calc_llk=0, zero_phase_ramp=True):

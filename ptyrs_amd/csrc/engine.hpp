@@ -649,11 +649,26 @@ class Engine : public EngineBase {
   // PyNX AP, DM, RAAR (ptyrs/src/engine/ptycho.rs:605-755), stacks of the whole scan; DM and
   // RAAR carry their exit waves across the cycles of a call.
   void project(const ProjParams& p, size_t ncycle) override {
-    one_unitary(p.algorithm == PTYRS_AP ? "AP" : p.algorithm == PTYRS_DM ? "DM" : "RAAR");
+    const char* name = p.algorithm == PTYRS_AP ? "AP" : p.algorithm == PTYRS_DM ? "DM" : "RAAR";
+    if (nslice_ > 1)
+      throw Invalid(std::string(name) +
+                    " runs one object slice; only LSQML and the LLK pass run several");
+    unitary(name);
+    if (pos_on_ && sampling != PTYRS_PATCH)
+      throw Invalid(std::string(name) +
+                    " with position offsets under FullObject sampling is not run on this GPU "
+                    "engine; set the PATCH sampling");
+    if (pos_on_ && psi_f16_ && p.algorithm != PTYRS_AP)
+      throw Invalid(std::string(name) +
+                    " with position offsets does not carry f16 waves on this GPU engine");
     if (nobj_ != 1) throw Invalid("the GPU projection algorithms run one object mode");
     if (M_ > kMaxModes) throw Invalid(modes_error());
     if (p.algorithm != PTYRS_AP && p.loops == 0)
       throw Invalid("loop_obj_probe must be >= 1");
+    if (pos_on_ && ncycle > 0 && nframe_ > 0) {
+      project_moved(p, ncycle);
+      return;
+    }
     if (ncycle == 0 || nframe_ == 0) {
       cycle += ncycle;
       if (p.zero_phase_ramp && ncycle > 0) {
@@ -756,6 +771,249 @@ class Engine : public EngineBase {
     if (p.zero_phase_ramp) device_zero_phase_ramp();
     synchronize();
     flush_timing();
+  }
+
+  // AP, DM and RAAR under PATCH offsets (ptyrs/src/engine/ptycho.rs:605-755 with
+  // Shifted::psi_to_obj_probe, ptyrs/src/engine/features.rs:653-720, 851-1047): every frame's
+  // moved wave W at the cycle's probe (wall_) takes the probe terms' place in pass A and in DM's
+  // PO of pass C; the position sums of the new carried waves against the arrays the cycle read;
+  // the object blends Σ conj(W)·ψ over Σ|W|²; the probe takes the first conjugate-gradient step
+  // of its least-squares problem (probe_cg_step); then the position step. AP updates both from
+  // the arrays on entry (the probe first, the object with the waves of the probe on entry); DM
+  // and RAAR loop object-only then probe-only updates, the probe seeing the new object.
+  void project_moved(const ProjParams& p, size_t ncycle) {
+    // The offsets' derivative planes (two per view) fill one launch's grid y.
+    const size_t stack = std::min<size_t>({nframe_, stack_size_ ? stack_size_ : kDefaultStack,
+                                           size_t(32767 / M_)});
+    ensure_workspace(stack, 1);
+    const int xb = aux_blocks();
+    const size_t vplane = size_t(M_) * n;
+    carried_.ensure(sizeof(cplx<T>) * nframe_ * vplane);
+    if (p.algorithm == PTYRS_RAAR) iter_.ensure(sizeof(cplx<T>) * nframe_ * vplane);
+    wall_.ensure(sizeof(cplx<T>) * nframe_ * vplane);
+    if (pos_refine_) {
+      dwpos_.ensure(sizeof(cplx<T>) * 2 * stack * vplane);
+      pos_all_.ensure(sizeof(double) * 4 * nframe_ * xb);
+      pstep_.ensure(sizeof(double) * 4 * nframe_);
+    }
+    const int nt = ntiles();
+    list_all_.ensure(sizeof(int) * size_t(nt) * nframe_);
+    count_all_.ensure(sizeof(int) * size_t(nt));
+    for (DevBuf* b : {&gP_, &dPc_, &dWt_}) b->ensure(sizeof(cplx<T>) * nprobe_ * n);
+    pden_.ensure(sizeof(T) * n);
+    pmax2_.ensure(sizeof(T) * 2);
+    cg_part_.ensure(sizeof(double) * 2 * size_t(M_) * xb);
+    vpow_.ensure(sizeof(double) * nframe_ * vplane / n * xb);
+    alpha_.ensure(sizeof(double) * M_);
+    abs_part_.ensure(sizeof(double) * 2 * kAbsBlocks);
+    const int nstack = int((nframe_ + stack - 1) / stack);
+    probe_terms();
+    cplx<T>* wall = wall_.as<cplx<T>>();
+    cplx<T>* carried = carried_.as<cplx<T>>();
+    std::vector<std::pair<size_t, int>> due;
+    for (size_t ic = 0; ic < ncycle; ++ic) {
+      const bool rec = p.calc_llk > 0 && (ic % p.calc_llk == 0 || ic + 1 == ncycle);
+      if (rec) {
+        if (due.size() == hist_slots_) drain(due, p.algorithm);
+        const int slot = int(due.size());
+        llk_pass(slot);
+        due.push_back({cycle, slot});
+      }
+      const bool first = ic == 0;
+      MixArgs<T> mix{MIX_NONE, Carried<T, false>{carried},
+                     Carried<T, false>{iter_.as<cplx<T>>()}, T(0), T(0)};
+      if (p.algorithm == PTYRS_DM && !first) {
+        mix.kind = MIX_DM;
+        mix.c0 = T(2.0 - p.mix);
+        mix.c1 = T(1.0 - p.mix);
+      } else if (p.algorithm == PTYRS_RAAR) {
+        mix.kind = first ? MIX_RAAR0 : MIX_RAAR;
+        mix.c0 = T(1.0 - 0.5 * p.mix);
+        mix.c1 = T(0.5 * p.mix);
+      }
+      const bool dm = p.algorithm == PTYRS_DM && !first;
+      moved_waves_all(stack, nstack);
+      for (int s = 0; s < nstack; ++s) {
+        const int f0 = int(s * stack);
+        const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+        const int* ord = iota_.as<int>() + f0;
+        const cplx<T>* ws = wall + size_t(f0) * vplane;
+        const int K = nb * M_;
+        pass_a(ord, nb, false, mix, ws);
+        pass_b<B_AMPLITUDE>(ord, nb);
+        const int ck = chunk_of(nb);
+        const int nck = (nb + ck - 1) / ck;
+        run(K_PASS_C, [&] {
+          hipLaunchKernelGGL((k_pass_c_proj<T, N, false>), dim3(SR::NBLK, M_, nck), dim3(SR::NT),
+                             0, stream_, psi_.as<cplx<T>>(), Carried<T, false>{carried},
+                             O_.as<cplx<T>>(), int(nxo_), ws, q_.as<cplx<T>>(),
+                             corner_.as<int2>(), ord, nb, M_, ck, int(dm), T(2.0 - p.mix),
+                             T(1.0 - p.mix), 0, (cplx<T>*)nullptr, (T*)nullptr,
+                             tw_.as<cplx<T>>(), 1);
+        });
+        if (pos_refine_) {
+          // The position sums of the new carried waves (spec_ holds the probe's spectra).
+          run(K_OTHER, [&] {
+            hipLaunchKernelGGL(k_ramp_deriv_planes<T>, dim3((n + 255) / 256, K), dim3(256), 0,
+                               stream_, dwpos_.as<cplx<T>>(),
+                               (const cplx<T>*)spec_.as<cplx<T>>(),
+                               (const double2*)offs_.as<double2>(), ord, N, M_, 1.0 / double(n));
+            hipLaunchKernelGGL((k_cols<T, N, true>), dim3(SC::NBLK, 2 * K), dim3(SC::NT), 0,
+                               stream_, dwpos_.as<cplx<T>>(), tw_.as<cplx<T>>());
+            hipLaunchKernelGGL((k_rows<T, N, true>), dim3(S::NBLK, 2 * K), dim3(S::NT), 0,
+                               stream_, dwpos_.as<cplx<T>>(), tw_.as<cplx<T>>());
+            hipLaunchKernelGGL(k_pos_sums_proj<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
+                               (const cplx<T>*)(carried + size_t(f0) * vplane), ws,
+                               (const cplx<T>*)O_.as<cplx<T>>(), int(nxo_), corner_.as<int2>(),
+                               ord, (const cplx<T>*)dwpos_.as<cplx<T>>(), N, M_,
+                               pos_all_.as<double>() + size_t(f0) * xb * 4);
+          });
+        }
+      }
+      if (p.algorithm == PTYRS_AP) {
+        if (p.upd_probe) probe_cg_step(stack, nstack, p.probe_inertia);
+        if (p.upd_obj) object_blend_moved(p.obj_inertia);
+      } else {
+        bool stale = false;  // wall_ is of an earlier probe
+        for (size_t l = 0; l < p.loops; ++l) {
+          if (p.upd_obj) {
+            if (stale) moved_waves_all(stack, nstack);
+            stale = false;
+            object_blend_moved(p.obj_inertia);
+          }
+          if (p.upd_probe) {
+            if (stale) moved_waves_all(stack, nstack);
+            probe_cg_step(stack, nstack, p.probe_inertia);
+            stale = true;
+          }
+        }
+      }
+      if (pos_refine_) {
+        run(K_OTHER, [&] {
+          hipLaunchKernelGGL(k_pos_step_all, dim3(1), dim3(1024), 0, stream_,
+                             (const double*)pos_all_.as<double>(), xb, int(nframe_),
+                             offs_.as<double2>(), pstep_.as<double>());
+        });
+      }
+      if (rec && p.algorithm != PTYRS_AP) balance(1.0);
+      ++cycle;
+    }
+    drain(due, p.algorithm);
+    if (p.zero_phase_ramp) device_zero_phase_ramp();
+    synchronize();
+    flush_timing();
+  }
+
+  // Every frame's moved wave W = DFT⁻¹(DFT(P·q)·exp(−2πi f·d)) at the current probe into
+  // wall_ ([frame·M + m][n]); spec_ keeps the probe terms' spectra.
+  void moved_waves_all(size_t stack, int nstack) {
+    run(K_PROBE, [&] { probe_spectra(); });
+    for (int s = 0; s < nstack; ++s) {
+      const int f0 = int(s * stack);
+      const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+      moved_probes(iota_.as<int>() + f0, nb, false,
+                   wall_.as<cplx<T>>() + size_t(f0) * M_ * n, nullptr, true);
+    }
+  }
+
+  // The object's blend through the moved waves: Σ conj(W)·ψ over Σ|W|² (+ reg·O).
+  void object_blend_moved(double inertia) {
+    const int nt = ntiles();
+    const cplx<T>* wall = wall_.as<cplx<T>>();
+    run(K_BIN, [&] {
+      hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                         corner_.as<int2>(), iota_.as<int>(), int(nframe_), N, ntx_,
+                         int(nframe_), list_all_.as<int>(), count_all_.as<int>(),
+                         (const T*)nullptr, M_, int(nyo_), int(nxo_), D_.as<T>(),
+                         tilemax_.as<T>(), wall);
+    });
+    run(K_OVERLAP, [&] {
+      hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(MAX_THREADS), 0, stream_,
+                         tilemax_.as<T>(), nt, pmax_.as<T>());
+    });
+    run(K_OBJ_UPDATE, [&] {
+      hipLaunchKernelGGL((k_obj_blend<T, false>), dim3(nt), dim3(THREADS), 0, stream_,
+                         O_.as<cplx<T>>(), wall, D_.as<T>(),
+                         Carried<T, false>{carried_.as<cplx<T>>()}, N, M_, corner_.as<int2>(),
+                         iota_.as<int>(), list_all_.as<int>(), count_all_.as<int>(),
+                         int(nframe_), int(nyo_), int(nxo_), ntx_, pmax_.as<T>(), T(inertia), 1);
+    });
+  }
+
+  // The probe's first conjugate-gradient step of min ‖ψ − A·P'‖² + reg·‖P' − P‖² under the
+  // moves (ptyrs/src/engine/features.rs:673-711): g = Σ_f the adjoint move of conj(O)·(ψ − W·O)
+  // times conj(q), D = Σ|O|² + reg (reg = inertia·max Σ|O|²), s = g/D, and per mode
+  // P += (<g, s> / (‖A·s‖² + reg·‖s‖²))·s where <g, s> > 0; A·s the moved waves of s·q times the
+  // windows. wall_ holds the current probe's waves.
+  void probe_cg_step(size_t stack, int nstack, double inertia) {
+    const int xb = aux_blocks();
+    const size_t vplane = size_t(M_) * n;
+    const int np = int(nprobe_ * n);
+    cplx<T>* g = gspec_.as<cplx<T>>();
+    for (int s = 0; s < nstack; ++s) {
+      const int f0 = int(s * stack);
+      const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+      const int* ord = iota_.as<int>() + f0;
+      const int K = nb * M_;
+      run(K_PROBE, [&] {
+        hipLaunchKernelGGL(k_resid_dir<T>, dim3((n + 255) / 256, K), dim3(256), 0, stream_, g,
+                           (const cplx<T>*)carried_.as<cplx<T>>(),
+                           (const cplx<T>*)(wall_.as<cplx<T>>() + size_t(f0) * vplane),
+                           (const cplx<T>*)O_.as<cplx<T>>(), int(nxo_), corner_.as<int2>(), ord,
+                           N, M_);
+        hipLaunchKernelGGL((k_rows<T, N, false>), dim3(S::NBLK, K), dim3(S::NT), 0, stream_, g,
+                           tw_.as<cplx<T>>());
+        hipLaunchKernelGGL((k_cols<T, N, false>), dim3(SC::NBLK, K), dim3(SC::NT), 0, stream_,
+                           g, tw_.as<cplx<T>>());
+        hipLaunchKernelGGL(k_unramp_mean<T>, dim3((n + 255) / 256, M_), dim3(256), 0, stream_,
+                           gP_.as<cplx<T>>(), (const cplx<T>*)g,
+                           (const double2*)offs_.as<double2>(), ord, nb, N, M_,
+                           1.0 / double(n), int(s > 0));
+      });
+    }
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL((k_cols<T, N, true>), dim3(SC::NBLK, M_), dim3(SC::NT), 0, stream_,
+                         gP_.as<cplx<T>>(), tw_.as<cplx<T>>());
+      hipLaunchKernelGGL((k_rows<T, N, true>), dim3(S::NBLK, M_), dim3(S::NT), 0, stream_,
+                         gP_.as<cplx<T>>(), tw_.as<cplx<T>>());
+      hipLaunchKernelGGL((k_times_q<T, true>), dim3((n + 255) / 256, M_), dim3(256), 0, stream_,
+                         gP_.as<cplx<T>>(), (const cplx<T>*)q_.as<cplx<T>>(), n);
+      hipLaunchKernelGGL(k_win_power<T>, dim3((n + 255) / 256), dim3(256), 0, stream_,
+                         pden_.as<T>(), (const cplx<T>*)O_.as<cplx<T>>(), int(nxo_),
+                         corner_.as<int2>(), iota_.as<int>(), int(nframe_), N);
+      hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(MAX_THREADS), 0, stream_,
+                         (const T*)pden_.as<T>(), n, pmax2_.as<T>());
+      hipLaunchKernelGGL(k_cg_probe_sums<T>, dim3(xb, M_), dim3(THREADS), 0, stream_,
+                         (const cplx<T>*)gP_.as<cplx<T>>(), (const T*)pden_.as<T>(),
+                         (const T*)pmax2_.as<T>(), T(inertia), n, dPc_.as<cplx<T>>(),
+                         cg_part_.as<double>());
+      // A·s: the moved waves of s·q against the windows.
+      hipLaunchKernelGGL(k_probe_terms<T>, dim3((np + 255) / 256), dim3(256), 0, stream_,
+                         dPc_.as<cplx<T>>(), q_.as<cplx<T>>(), dWt_.as<cplx<T>>(), n, np);
+      probe_spectra(dWt_.as<cplx<T>>());
+    });
+    for (int s = 0; s < nstack; ++s) {
+      const int f0 = int(s * stack);
+      const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+      const int* ord = iota_.as<int>() + f0;
+      moved_probes(ord, nb, false, inc0_.as<cplx<T>>(), nullptr, true);
+      run(K_PROBE, [&] {
+        hipLaunchKernelGGL(k_view_power<T>, dim3(xb, nb * M_), dim3(THREADS), 0, stream_,
+                           (const cplx<T>*)inc0_.as<cplx<T>>(), (const cplx<T>*)O_.as<cplx<T>>(),
+                           int(nxo_), corner_.as<int2>(), ord, N, M_, size_t(f0) * M_,
+                           vpow_.as<double>());
+      });
+    }
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_cg_probe_alpha, dim3(1), dim3(1), 0, stream_,
+                         (const double*)cg_part_.as<double>(), xb,
+                         (const double*)vpow_.as<double>(), xb, int(nframe_) * M_, M_,
+                         alpha_.as<double>());
+      hipLaunchKernelGGL(k_probe_axpy<T>, dim3((np + 255) / 256), dim3(256), 0, stream_,
+                         P_.as<cplx<T>>(), (const cplx<T>*)dPc_.as<cplx<T>>(),
+                         (const double*)alpha_.as<double>(), n, np);
+    });
+    probe_terms();
   }
 
   // PyNX ML, Poisson conjugate gradient (ptyrs/src/engine/ptycho.rs:757-842): scale_obj_probe,
@@ -957,10 +1215,12 @@ class Engine : public EngineBase {
   // spectra summed stack by stack); γ's a and b take the moved waves of dP·q; with refinement
   // the position sums come from the magnitude projection of the cycle's starting estimate and
   // AP's, DM's and ML's step follows the update (ptyrs/src/engine/features.rs:960-1047).
-  DevBuf wall_, dwpos_, pos_all_, pstep_, pden1_;
+  DevBuf wall_, dwpos_, pos_all_, pstep_, pden1_, vpow_, alpha_;
   void ml_moved(const ptyrs_ml_params& p, size_t ncycle) {
     const bool uo = p.update_object != 0, up = p.update_probe != 0;
-    const size_t stack = std::min<size_t>(nframe_, stack_size_ ? stack_size_ : kDefaultStack);
+    // The offsets' derivative planes (two per view) fill one launch's grid y.
+    const size_t stack = std::min<size_t>({nframe_, stack_size_ ? stack_size_ : kDefaultStack,
+                                           size_t(32767 / M_)});
     ensure_workspace(stack, 1);
     const size_t no = nobj_ * nyo_ * nxo_, np = nprobe_ * n;
     const int nt = ntiles();

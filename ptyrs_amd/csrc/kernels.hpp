@@ -1994,14 +1994,15 @@ __global__ __launch_bounds__(1024) static void k_pos_step(const double* __restri
 // slots of its chunk: inverse row transform of the projected field Π, then the new carried wave
 // ψ' = Π (AP, RAAR) or Π + (1 − a)·(PO − R) with R = (2 − a)·PO − (1 − a)·ψ (DM), written to
 // carried[frame]. With `probe_sums` (AP: both updates read the old arrays) it adds
-// conj(q·O)·ψ' and |O|² of the slot's window into per-chunk partials.
+// conj(q·O)·ψ' and |O|² of the slot's window into per-chunk partials. With w_view every slot's
+// mode has its own wave W (plane slot·M + m of Wt, a moved probe) in PO.
 template <typename T, int N, bool H = false>
 __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_proj(
     const cplx<T>* __restrict__ psi, Carried<T, H> carried, const cplx<T>* __restrict__ O,
     int nxo, const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ q,
     const int2* __restrict__ corner, const int* __restrict__ order, int nb, int M, int chunk,
     int dm, T c0, T c1, int probe_sums, cplx<T>* __restrict__ num_part, T* __restrict__ den_part,
-    const cplx<T>* __restrict__ tw) {
+    const cplx<T>* __restrict__ tw, int w_view = 0) {
   using S = RowShapeC<T, N>;
   constexpr int E = S::E, TL = S::TL;
   constexpr int n = N * N;
@@ -2040,7 +2041,9 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_proj(
       const cplx<T> o = ld(orow + x);
       cplx<T> p = v[e];
       if (dm) {
-        const cplx<T> po = w[e] * o;
+        const cplx<T> we =
+            w_view ? ld(Wt + ((size_t)b * M + m) * n + (size_t)y * N + x) : w[e];
+        const cplx<T> po = we * o;
         const cplx<T> r = po * c0 - carried.get(crow + x) * c1;
         p = p + (po - r) * c1;
       }
@@ -2134,13 +2137,14 @@ __global__ void k_probe_blend(cplx<T>* __restrict__ P, cplx<T>* __restrict__ Wt,
 
 // PyNX blend of the object from tile-owned sums over every view: every pixel owned by one thread
 // sums Σ_m Σ_views conj(W_m)·ψ_m (Kahan) in view order, and O = (num + reg·O)/max(norm + reg,
-// 1e-12) with norm = Σ_m D_m from k_bin_tiles and reg = inertia·max(norm).
+// 1e-12) with norm = Σ_m D_m from k_bin_tiles and reg = inertia·max(norm). With w_view W is the
+// frame's own wave (plane frame·M + m of Wt, a moved probe; the order is every frame's).
 template <typename T, bool H = false>
 __global__ __launch_bounds__(THREADS) void k_obj_blend(
     cplx<T>* __restrict__ O, const cplx<T>* __restrict__ Wt, const T* __restrict__ D,
     Carried<T, H> carried, int N, int M, const int2* __restrict__ corner,
     const int* __restrict__ order, const int* __restrict__ list, const int* __restrict__ count,
-    int cap, int nyo, int nxo, int ntx, const T* __restrict__ nmax, T inertia) {
+    int cap, int nyo, int nxo, int ntx, const T* __restrict__ nmax, T inertia, int w_view = 0) {
   __shared__ int3 cs[THREADS];
   const int t = tile_id(), tid = threadIdx.x;
   const int X = (t % ntx) * TILE_X + tid % TILE_X, Y = (t / ntx) * TILE_Y + tid / TILE_X;
@@ -2162,8 +2166,9 @@ __global__ __launch_bounds__(THREADS) void k_obj_blend(
       if (y >= 0 && y < N && x >= 0 && x < N) {
         const int j = y * N + x;
         const size_t cv = (size_t)cs[i].x * M * n + j;
+        const cplx<T>* wv = w_view ? Wt + (size_t)cs[i].x * M * n : Wt;
         for (int m = 0; m < M; ++m) {
-          const cplx<T> g = cmulc(ld(Wt + (size_t)m * n + j), carried.get(cv + (size_t)m * n));
+          const cplx<T> g = cmulc(ld(wv + (size_t)m * n + j), carried.get(cv + (size_t)m * n));
           sr.add(g.x);
           si.add(g.y);
         }
@@ -2180,6 +2185,122 @@ __global__ __launch_bounds__(THREADS) void k_obj_blend(
   const T d = norm + reg;
   const cplx<T> o = (cplx<T>{sr.s, si.s} + ld(O + p) * reg) * (T(1) / (d > T(1e-12) ? d : T(1e-12)));
   st(O + p, o);
+}
+
+// ---- The probe's conjugate-gradient step under moved waves (AP, DM, RAAR with PATCH offsets;
+// ptyrs/src/engine/features.rs:653-720) ---------------------------------------------------------
+// Per view of a stack the probe-space residual term g_v = conj(O[win])·(ψ_v − W_v·O[win]), ψ the
+// frame's carried wave, W its moved wave (stack-local plane). Grid (ceil(n/256), nb·M).
+template <typename T>
+__global__ void k_resid_dir(cplx<T>* __restrict__ g, const cplx<T>* __restrict__ carried,
+                            const cplx<T>* __restrict__ W, const cplx<T>* __restrict__ O, int nxo,
+                            const int2* __restrict__ corner, const int* __restrict__ order,
+                            int N, int M) {
+  const int n = N * N;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int v = blockIdx.y, b = v / M, m = v % M;
+  const int f = order[b];
+  const int2 c = corner[f];
+  const cplx<T> o = ld(O + (size_t)(c.x + j / N) * nxo + c.y + j % N);
+  const cplx<T> r = ld(carried + ((size_t)f * M + m) * n + j) - ld(W + (size_t)v * n + j) * o;
+  st(g + (size_t)v * n + j, cmulc(o, r));
+}
+
+// Σ_f |O[win_f]|² per window pixel over the frames `order[0..nb)` (FP64), the probe sums' norm
+// (the same for every mode). Grid (ceil(n/256)).
+template <typename T>
+__global__ void k_win_power(T* __restrict__ out, const cplx<T>* __restrict__ O, int nxo,
+                            const int2* __restrict__ corner, const int* __restrict__ order,
+                            int nb, int N) {
+  const int n = N * N;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int y = j / N, x = j % N;
+  double s = 0;
+  for (int b = 0; b < nb; ++b) {
+    const int2 c = corner[order[b]];
+    const cplx<T> o = ld(O + (size_t)(c.x + y) * nxo + c.y + x);
+    s += double(o.x) * double(o.x) + double(o.y) * double(o.y);
+  }
+  out[j] = T(s);
+}
+
+// Per mode and workgroup: the preconditioned step s = num/max(norm + reg, 1e-12) (into step),
+// and the partials of Σ |num|²/d (slope) and reg·Σ |s|² into part[(m·gridDim.x + blk)·2],
+// reg = inertia·max norm. Grid (nblk, M).
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_cg_probe_sums(
+    const cplx<T>* __restrict__ num, const T* __restrict__ norm, const T* __restrict__ nmax,
+    T inertia, int n, cplx<T>* __restrict__ step, double* __restrict__ part) {
+  __shared__ double red[THREADS];
+  const int m = blockIdx.y;
+  const T reg = nmax[0] * inertia;
+  double a = 0, c = 0;
+  for (int j = blockIdx.x * THREADS + threadIdx.x; j < n; j += gridDim.x * THREADS) {
+    const T dv = norm[j] + reg;
+    const T d = dv > T(1e-12) ? dv : T(1e-12);
+    const cplx<T> g = ld(num + (size_t)m * n + j);
+    const cplx<T> sv = g * (T(1) / d);
+    st(step + (size_t)m * n + j, sv);
+    a += double(norm2(g)) / double(d);
+    c += double(reg) * double(norm2(sv));
+  }
+  for (int i = 0; i < 2; ++i) {
+    double r = block_reduce<THREADS>(i == 0 ? a : c, red, AddOp());
+    if (threadIdx.x == 0) part[((size_t)m * gridDim.x + blockIdx.x) * 2 + i] = r;
+  }
+}
+
+// Per view of a stack and workgroup Σ |W_v·O[win]|² (FP64) into part[(v0 + v)·nblk + blk], v0
+// the stack's first view. Grid (nblk, nb·M).
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_view_power(const cplx<T>* __restrict__ W,
+                                                        const cplx<T>* __restrict__ O, int nxo,
+                                                        const int2* __restrict__ corner,
+                                                        const int* __restrict__ order, int N,
+                                                        int M, size_t v0,
+                                                        double* __restrict__ part) {
+  __shared__ double red[THREADS];
+  const int v = blockIdx.y, b = v / M;
+  const int n = N * N;
+  const int2 c = corner[order[b]];
+  double s = 0;
+  for (int j = blockIdx.x * THREADS + threadIdx.x; j < n; j += gridDim.x * THREADS) {
+    const cplx<T> o = ld(O + (size_t)(c.x + j / N) * nxo + c.y + j % N);
+    s += double(norm2(ld(W + (size_t)v * n + j) * o));
+  }
+  const double r = block_reduce<THREADS>(s, red, AddOp());
+  if (threadIdx.x == 0) part[(v0 + v) * gridDim.x + blockIdx.x] = r;
+}
+
+// One thread: per mode α_m = slope_m / (reg·Σ|s_m|² + Σ_views of m ‖W_s·O‖²) where the slope is
+// positive, else 0 (the views summed in view order). ptyrs/src/engine/features.rs:686-716
+__global__ static void k_cg_probe_alpha(const double* __restrict__ part, int nblk,
+                                        const double* __restrict__ vpart, int vblk, int nview,
+                                        int M, double* __restrict__ alpha) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int m = 0; m < M; ++m) {
+    double slope = 0, curv = 0;
+    for (int i = 0; i < nblk; ++i) {
+      slope += part[((size_t)m * nblk + i) * 2];
+      curv += part[((size_t)m * nblk + i) * 2 + 1];
+    }
+    for (int v = m; v < nview; v += M)
+      for (int i = 0; i < vblk; ++i) curv += vpart[(size_t)v * vblk + i];
+    alpha[m] = slope > 0.0 ? slope / curv : 0.0;
+  }
+}
+
+// P_m += α_m·s_m. Grid (ceil(total/256)).
+template <typename T>
+__global__ void k_probe_axpy(cplx<T>* __restrict__ P, const cplx<T>* __restrict__ step,
+                             const double* __restrict__ alpha, int n, int total) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const double a = alpha[i / n];
+  const cplx<T> s = ld(step + i), p = ld(P + i);
+  st(P + i, cplx<T>{T(double(p.x) + a * double(s.x)), T(double(p.y) + a * double(s.y))});
 }
 
 // Σ|O| and Σ|P| in FP64: per-workgroup partials into part[2·blocks]. The host-side factors of

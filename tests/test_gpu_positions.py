@@ -4,7 +4,8 @@ oracle, whose position model and step the reference's fixtures pin
 truth object with every frame moved by its offset, near field (angular spectrum; angular
 spectrum with an evanescent band), three slices, and the far field; the LLK after every epoch,
 the object, probe and offsets at the end. c128 to FP64 rounding, c64 to 1e-3. Plus the feature
-API: held offsets stay, the algorithms that do not model them refuse them."""
+API: held offsets stay, the algorithms that do not model them refuse them. AP, DM and RAAR
+under offsets, refined or held, against the oracle."""
 
 import numpy as np
 import pytest
@@ -129,9 +130,9 @@ def test_position_feature_api():
     assert np.array_equal(g.features["position"], d0)
     g.ml(1)  # ML models held offsets too
     assert np.array_equal(g.features["position"], d0)
-    for name in ("ap", "dm", "raar"):
-        with pytest.raises(ValueError, match="position"):
-            getattr(g, name)(1)
+    for name in ("ap", "dm", "raar"):   # held offsets: the projections model them
+        getattr(g, name)(1)
+        assert np.array_equal(g.features["position"], d0)
     g.set_sampling("full_object")   # held offsets under FullObject: rPIE and LSQML run
     g.rpie(1, batch_size=5)
     g.lsqml(1, 1, 6, "poisson", 1)
@@ -193,6 +194,40 @@ def test_ml_positions_match_oracle(dtype, nprobe, refine, up):
     tol = TOL[dtype]
     want = [r[2].poisson for r in o.history]
     print("ml positions", dtype, nprobe, max_rel(g.history["poisson"], want),
+          rel_l2(g.object, o.obj), rel_l2(g.probe, o.probe))
+    assert max_rel(g.history["poisson"], want) < tol
+    assert rel_l2(g.object, o.obj) < tol * 10
+    assert rel_l2(g.probe, o.probe) < tol * 10
+    got = g.features["position"]
+    if refine:
+        assert max_rel((got - d0).ravel(), (o.offsets - d0).ravel()) < tol * 100
+    else:
+        assert np.array_equal(got, d0)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("name,nprobe,refine,kw", [
+    ("ap", 1, True, dict(probe_inertia=0.01)), ("ap", 3, False, dict(probe_inertia=0.01)),
+    ("ap", 2, True, {}), ("dm", 2, True, {}), ("dm", 1, False, dict(loop_obj_probe=2)),
+    ("raar", 2, True, {}), ("raar", 3, True, dict(loop_obj_probe=2))])
+def test_projections_with_positions_match_oracle(dtype, name, nprobe, refine, kw):
+    # AP, DM and RAAR under PATCH offsets (ptyrs/src/engine/ptycho.rs:605-755 with
+    # Shifted::psi_to_obj_probe): far field, masked pixels, zero-mean offsets within 0.4 px,
+    # refined or held, three cycles with LLK records (the oracle pins the moved waves and the
+    # position sums by the reference's fixtures and the projections by the zero-offset runs).
+    d = small_scan(n=32, nprobe=nprobe, npos=25, seed=31)
+    d0 = np.random.default_rng(32).uniform(-0.4, 0.4, (25, 2))
+    d0 -= d0.mean(axis=0)
+    o = oracle(d)
+    o.set_position(d0, refine=refine)
+    params = dict(calc_llk=1, **kw)
+    getattr(o, name)(3, **params)
+    g = gpu(d, dtype)
+    g.set_features(position=d0, hold=() if refine else ("position",))
+    getattr(g, name)(3, **params)
+    tol = TOL[dtype]
+    want = [r[2].poisson for r in o.history]
+    print(name, nprobe, refine, kw, dtype, max_rel(g.history["poisson"], want),
           rel_l2(g.object, o.obj), rel_l2(g.probe, o.probe))
     assert max_rel(g.history["poisson"], want) < tol
     assert rel_l2(g.object, o.obj) < tol * 10

@@ -265,3 +265,78 @@ def test_rpie_full_object_positions_match_ptychi(case):
              f"rpie_full_object_{case}_probe_f64.txt")
     err = max_rel((p.offsets - start_off).ravel(), (rows[:, 3:5] - start_off).ravel())
     assert err < 3e-12, err
+
+
+# AP, DM and RAAR under PATCH offsets (ptyrs/src/engine/ptycho.rs:605-755 with
+# Shifted::psi_to_obj_probe, ptyrs/src/engine/features.rs:653-720): at zero offsets held the
+# moved waves are P·q and the probe's conjugate-gradient step is the plain blend (its step
+# length is 1 when nothing moves), so the runs are the plain ones to the transforms' rounding.
+@pytest.mark.parametrize("algorithm", ["ap", "dm", "raar"])
+def test_projections_under_zero_offsets_are_the_plain_ones(algorithm):
+    a, b = fresh(2), fresh(2)
+    b.set_position(np.zeros((len(b.iobs), 2)), refine=False)
+    kw = dict(calc_llk=1, zero_phase_ramp=False, probe_inertia=0.01)
+    getattr(a, algorithm)(4, **kw)
+    getattr(b, algorithm)(4, **kw)
+    for x, y in ((b.obj, a.obj), (b.probe, a.probe)):
+        assert np.abs(x - y).max() < 1e-11 * np.abs(y).max(), algorithm
+    assert max_rel([r[2].poisson for r in b.history], [r[2].poisson for r in a.history]) < 1e-12
+
+
+@pytest.mark.parametrize("algorithm", ["ap", "dm", "raar"])
+def test_projections_refine_offsets_toward_the_truth(algorithm):
+    # The scenario's own positions are the truth: offsets started off it by up to 0.3 px
+    # (zero mean: the step leaves the frames' mean, a shift of the whole object, alone) move
+    # back toward it.
+    p = fresh(2)
+    d0 = np.random.default_rng(5).uniform(-0.3, 0.3, (len(p.iobs), 2))
+    d0 -= d0.mean(axis=0)
+    p.set_position(d0.copy(), refine=True)
+    getattr(p, algorithm)(6, calc_llk=1, zero_phase_ramp=False, probe_inertia=0.01)
+    assert np.sqrt((p.offsets ** 2).mean()) < np.sqrt((d0 ** 2).mean())
+    llks = [r[2].poisson for r in p.history]
+    assert llks[-1] < llks[0]
+
+
+def test_moved_probe_step_minimises_along_its_direction():
+    # The probe's update under offsets (ptyrs/src/engine/features.rs:653-720) is one
+    # conjugate-gradient step of J(P') = Σ_views ‖ψ − W(P')·O_win‖² + reg·‖P' − P‖², reg =
+    # inertia·max Σ|O_win|², W(P') the moved waves: J is quadratic along the step, so the step's
+    # length α must be J's minimiser there (J(P + t·ΔP) ≥ J(P + ΔP) for every t) and J must fall.
+    from oracle.base import to_natural
+    from oracle.ptycho import apply_amplitude
+    p = fresh(2)
+    d0 = np.random.default_rng(8).uniform(-0.4, 0.4, (len(p.iobs), 2))
+    p.set_position(d0 - d0.mean(axis=0), refine=False)
+    nmode = p.addr.nmode
+    p.psi = p.optics.exit_adjoint(apply_amplitude(p.iobs, nmode,
+                                                  p.optics.exit_forward(p.calc_psi())))
+    views = p.views_all
+    ny, nx = p.probe.shape[-2:]
+    win = np.stack([p.obj[o, oy:oy + ny, ox:ox + nx] for (_, _, _, o, oy, ox) in views])
+    norm = np.zeros(p.probe.shape)
+    for k, v in enumerate(views):
+        norm[v[2]] += np.abs(win[k]) ** 2
+    inertia = 0.05
+    reg = inertia * norm.max()
+    psi_n = to_natural(p.psi)
+    p0 = p.probe.copy()
+
+    def cost(probe):
+        keep, p.probe = p.probe, probe
+        w = p.moved_waves(views)
+        p.probe = keep
+        return float((np.abs(psi_n - w * win) ** 2).sum() + reg * (np.abs(probe - p0) ** 2).sum())
+
+    p._moved_psi_to_obj_probe(None, inertia)
+    step = p.probe - p0
+    assert np.abs(step).max() > 1e-6 * np.abs(p0).max()
+    j1 = cost(p0 + step)
+    assert j1 < cost(p0)
+    for t in (0.8, 0.98, 1.02, 1.25):
+        assert cost(p0 + t * step) > j1
+    # Per mode too: scaling one mode's step alone also raises J.
+    for m in range(p0.shape[0]):
+        s = step.copy()
+        s[m] *= 1.1
+        assert cost(p0 + s) > j1
