@@ -38,6 +38,12 @@ What pins each module (the test files named run on the CPU):
   weights (steps, changes, eigenmodes, weights at 1e-12/1e-13), a one-frame batch, held
   weights; LSQML with PATCH offsets and OPR (`lsqml_pos_opr_*`), and the far-field
   `lsqml_pos_*` step and three epochs.
+- bh.py — tests/test_oracle_bh.py restates `ptyrs/tests/engine_bh.rs` and
+  `engine_bh_near.rs`: two epochs on the seeded far-field scenario with one batch of all frames
+  (conjugate gradient) and with batches of 16 (`bh_full_*`, `bh_mini_*`: β, α, ⟨g, D⟩, H(D, D)
+  at 1e-13, object changes at 1e-13, probe at 5e-12), the four near-field cases with both batch
+  sizes (`bh_near_*`); the gradient against central differences of the cost, the Hessian against
+  central differences of the gradient, its symmetry and bilinearity.
 - multislice.py — tests/test_oracle_multislice.py (`lsqml_multislice_*`: forward
   intensities, one step, 30 epochs; directions and tangent against central differences).
 - positions.py — tests/test_oracle_positions.py (`lsqml_near_pos_*`,

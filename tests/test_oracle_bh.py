@@ -1,0 +1,170 @@
+"""The oracle's BH (oracle/bh.py) against the reference's own BH fixtures and what the
+mathematics fixes: two epochs on the seeded far-field scenario (two probe modes, after
+scale_obj_probe) with one batch of all 40 frames (conjugate gradient across the epochs) and
+with batches of 16 (steepest descent), every step's β, α, ⟨g, D⟩ and H(D, D) to 1e-13 and the
+object and probe changes of each epoch (bh_{full,mini}_*, written by the reference's
+refs/f64/bh_step.py; ptyrs/tests/engine_bh.rs:164-243); the gradient against central
+differences of the cost and the Hessian against central differences of the gradient, and its
+symmetry (ptyrs/tests/engine_bh.rs:95-162)."""
+
+import numpy as np
+import pytest
+
+from conftest import max_rel, table
+from oracle import scenario as sc
+from oracle.bh import Bh, BhBatch, dot
+from oracle.ptycho import llk, obj_probe_to_psi
+
+
+def state():
+    p, _ = sc.parity_start(nprobe=2)
+    return p
+
+
+def run(batch_size, nepoch=2):
+    p = state()
+    bh = Bh(batch_size, probe_rho=0.01, seed=7)
+    obj, probe = p.obj, p.probe
+    steps, changes = [], []
+    for _ in range(nepoch):
+        o, pr, s = bh.epoch(p.addr, p.iobs, p.optics, obj, probe)
+        steps.append(s)
+        changes.append((o - obj, pr - probe))
+        obj, probe = o, pr
+    return steps, changes
+
+
+@pytest.mark.parametrize("batch_size,name", [(40, "bh_full"), (16, "bh_mini")])
+def test_epochs_match_fixtures(batch_size, name):
+    steps, changes = run(batch_size)
+    rows = table(f"{name}_steps_f64.txt")
+    got = [(e, s) for e, ss in enumerate(steps) for s in ss]
+    assert len(got) == len(rows)
+    for k, ((epoch, (frames, beta, alpha, top, bottom)), r) in enumerate(zip(got, rows)):
+        assert [k, epoch, len(frames)] == [int(r[0]), int(r[1]), int(r[2])]
+        assert frames == [int(f) for f in r[7:]]
+        if r[3] == 0.0:
+            assert beta == 0.0
+        else:
+            assert abs(beta - r[3]) < 1e-13 * abs(r[3]), (k, beta, r[3])
+        for got_v, want in ((alpha, r[4]), (top, r[5]), (bottom, r[6])):
+            assert abs(got_v - want) < 1e-13 * abs(want), (k, got_v, want)
+    obj_rows = np.array(table(f"{name}_obj_f64.txt"))
+    probe_rows = np.array(table(f"{name}_probe_f64.txt"))
+    for e, (do, dp) in enumerate(changes):
+        want = obj_rows[:, 2 + 2 * e] + 1j * obj_rows[:, 3 + 2 * e]
+        got_o = do[0][obj_rows[:, 0].astype(int), obj_rows[:, 1].astype(int)]
+        assert max_rel(np.r_[got_o.real, got_o.imag], np.r_[want.real, want.imag]) < 1e-13
+        want = probe_rows[:, 3 + 2 * e] + 1j * probe_rows[:, 4 + 2 * e]
+        idx = probe_rows[:, :3].astype(int)
+        got_p = dp[idx[:, 0], idx[:, 1], idx[:, 2]]
+        assert max_rel(np.r_[got_p.real, got_p.imag], np.r_[want.real, want.imag]) < 5e-12
+
+
+def _cost(p, views, obj, probe):
+    """F of the views without the amplitude offset: a quarter of the Euclidian LLK sum."""
+    frames = sorted({v[0] for v in views})
+    psi = p.optics.exit_forward(obj_probe_to_psi(None, views, obj, probe, p.optics.quad))
+    return llk(np.stack([p.iobs[f] for f in frames]), p.addr.nmode, psi).euclidian / 4.0
+
+
+def _direction(like, rng):
+    scale = np.abs(like).max()
+    return (rng.uniform(-0.5, 0.5, like.shape) + 1j * rng.uniform(-0.5, 0.5, like.shape)) * scale
+
+
+def test_gradient_and_hessian_match_central_differences():
+    p = state()
+    views = p.addr.views()[:16 * p.addr.nmode]
+    b = BhBatch(views, p.iobs, p.optics, p.obj, p.probe)
+    g = b.grads()
+    rng = np.random.default_rng(3)
+    d_o, d_p = _direction(p.obj, rng), _direction(p.probe, rng)
+    h = 1e-5
+    fo = (_cost(p, views, p.obj + h * d_o, p.probe) -
+          _cost(p, views, p.obj - h * d_o, p.probe)) / (2 * h)
+    fp = (_cost(p, views, p.obj, p.probe + h * d_p) -
+          _cost(p, views, p.obj, p.probe - h * d_p)) / (2 * h)
+    # the gradient carries the 1e-8 amplitude offset, the differences do not
+    assert abs(dot(g, (d_o, 0 * p.probe)) - fo) < 1e-7 * abs(fo)
+    assert abs(dot(g, (0 * p.obj, d_p)) - fp) < 1e-7 * abs(fp)
+    u = (_direction(p.obj, rng), _direction(p.probe, rng))
+    v = (_direction(p.obj, rng), _direction(p.probe, rng))
+    h_uv, h_vu = b.hessian(u, v), b.hessian(v, u)
+    assert abs(h_uv - h_vu) < 1e-13 * abs(h_uv)
+
+    def du(t):
+        bt = BhBatch(views, p.iobs, p.optics, p.obj + t * v[0], p.probe + t * v[1])
+        return dot(bt.grads(), u)
+
+    t = 1e-6
+    fd = (du(t) - du(-t)) / (2 * t)
+    assert abs(h_uv - fd) < 1e-7 * abs(fd)
+
+
+def test_hessian_is_bilinear_and_the_step_minimises_the_quadratic_model():
+    # H(u, v) is linear in each argument; along D the model F + t·⟨g, D⟩ + t²/2·H(D, D) is least
+    # at t = −⟨g, D⟩/H(D, D), the step BH takes with one batch of every frame.
+    p = state()
+    views = p.addr.views()
+    b = BhBatch(views, p.iobs, p.optics, p.obj, p.probe)
+    rng = np.random.default_rng(4)
+    u, v, w = [(_direction(p.obj, rng), _direction(p.probe, rng)) for _ in range(3)]
+    lin = b.hessian((u[0] + 2 * w[0], u[1] + 2 * w[1]), v)
+    assert abs(lin - b.hessian(u, v) - 2 * b.hessian(w, v)) < 1e-12 * abs(lin)
+    g = b.grads()
+    d = (-g[0], -g[1])
+    alpha = -dot(g, d) / b.hessian(d, d)
+    model = [t * dot(g, d) + 0.5 * t * t * b.hessian(d, d) for t in (0.9 * alpha, alpha,
+                                                                      1.1 * alpha)]
+    assert model[1] < model[0] and model[1] < model[2]
+    # and the cost falls along D over the step (the amplitude cost is near-quadratic there)
+    f0 = _cost(p, views, p.obj, p.probe)
+    assert _cost(p, views, p.obj + alpha * d[0], p.probe + alpha * d[1]) < f0
+
+
+def test_batch_size_is_checked_and_masked_pixels_are_not_read():
+    with pytest.raises(ValueError):
+        Bh(0)
+    p = state()
+    iobs = np.array(p.iobs)
+    iobs[3, 5, 7] = np.nan
+    b = BhBatch(p.addr.views(), iobs, p.optics, p.obj, p.probe)
+    masked = np.isnan(b.root)
+    assert masked.sum() == 1 + np.isnan(p.iobs).sum()
+    blk = b.td.reshape(b.root.shape[:1] + (p.addr.nmode,) + b.root.shape[1:])
+    assert np.all(blk.transpose(1, 0, 2, 3)[:, masked] == 0)
+
+
+@pytest.mark.parametrize("case", range(4))
+@pytest.mark.parametrize("method,batch_size", [(0, 16), (1, 6)])
+def test_near_field_epochs_match_fixtures(case, method, batch_size):
+    # pty-chi's planar near field on the near-field LSQML state (24² frames, two probe modes,
+    # 16 frames; angular spectrum, Fresnel, angular spectrum with an evanescent band, Fresnel
+    # back), probe ρ 1, two epochs: steps to 1e-13 (β 1e-11), changes to 5e-12
+    # (bh_near_*; ptyrs/tests/engine_bh_near.rs:103-176).
+    from test_oracle_lsqml import state as near_state
+    addr, obj0, probe0, cases = near_state()
+    optics, iobs, _, _ = cases[case]
+    bh = Bh(batch_size, probe_rho=1.0, seed=7)
+    obj, probe, got = obj0, probe0, []
+    for e in range(2):
+        obj, probe, s = bh.epoch(addr, iobs, optics, obj, probe)
+        got += [(e, x) for x in s]
+    rows = [r for r in table("bh_near_steps_f64.txt") if r[0] == case and r[1] == method]
+    assert len(got) == len(rows)
+    for k, ((epoch, (frames, beta, alpha, top, bottom)), r) in enumerate(zip(got, rows)):
+        assert [k, epoch, len(frames)] == [int(r[2]), int(r[3]), int(r[4])]
+        assert frames == [int(f) for f in r[9:]]
+        if r[5] == 0.0:
+            assert beta == 0.0
+        else:
+            assert abs(beta - r[5]) < 1e-11 * abs(r[5])
+        for got_v, want in ((alpha, r[6]), (top, r[7]), (bottom, r[8])):
+            assert abs(got_v - want) < 1e-13 * abs(want), (k, got_v, want)
+    for now, start, name, lead in ((obj, obj0, "bh_near_obj_f64.txt", 4),
+                                   (probe, probe0, "bh_near_probe_f64.txt", 5)):
+        rows = np.array([r for r in table(name) if r[0] == case and r[1] == method])
+        d = (now - start).ravel()
+        assert max_rel(np.stack([d.real, d.imag], axis=1).ravel(),
+                       rows[:, lead:lead + 2].ravel()) < 5e-12
