@@ -246,6 +246,39 @@ typedef struct {
 int ptyrs_engine_lsqml_opr(ptyrs_engine* e, const ptyrs_lsqml_params* params, size_t nepoch,
                            size_t calc_llk, ptyrs_opr* var);
 
+/* BhParams (ptyrs/src/engine/bh.rs:28-58): frames per batch (capped at the scan's; one batch of
+ * every frame runs conjugate gradient across the epochs of a call, smaller batches steepest
+ * descent), probe_rho (the probe's gradient is scaled by probe_rho²·k, k the exit's intensity
+ * scale: ny·nx in the far field), position_rho (pty-chi's offset ρ; offsets are not modelled by
+ * the GPU BH), seed of every epoch's random_batches. */
+typedef struct {
+  size_t batch_size;
+  double probe_rho;
+  double position_rho;
+  uint64_t seed;
+} ptyrs_bh_params;
+
+/* One BH batch's step of the last epoch (BhStep, ptyrs/src/engine/bh.rs:60-72): its `count`
+ * frames are entries first .. first + count of ptyrs_engine_bh_frames' order; β (0 in steepest
+ * descent and the first epoch), α (after the factor count/batch_size), top = −⟨g, D⟩ and
+ * bottom = H(D, D). */
+typedef struct {
+  size_t first, count;
+  double beta, alpha, top, bottom;
+} ptyrs_bh_step;
+
+/* GpuPtycho::bh (ptyrs/src/engine/gpu/bh.rs:33-68): nepoch epochs of pty-chi's BH on the
+ * amplitude cost from a fresh BH (epoch indices 0.. of the call), an LLK record every calc_llk
+ * epochs and after the last. One object slice and mode, no position offsets (PTYRS_EINVAL). */
+int ptyrs_engine_bh(ptyrs_engine* e, const ptyrs_bh_params* params, size_t nepoch,
+                    size_t calc_llk);
+
+/* GpuPtycho::bh_steps (ptyrs/src/engine/gpu/bh.rs:70-72): the last BH epoch's steps, *n of them
+ * (none before a BH call); at most cap are copied to out (which may be NULL). */
+int ptyrs_engine_bh_steps(ptyrs_engine* e, ptyrs_bh_step* out, size_t cap, size_t* n);
+/* The last BH epoch's frames, batch after batch (len = nframe after a BH call, else 0). */
+int ptyrs_engine_bh_frames(ptyrs_engine* e, size_t* out, size_t len);
+
 /* GpuPtycho::lsqml_steps (ptyrs/src/engine/gpu/lsqml.rs:122-127): the last LSQML epoch's steps,
  * one per frame in batch order (a_obj of slice 0); copies up to cap and sets *n to the number
  * held. */

@@ -92,6 +92,17 @@ class _LsqmlC(ctypes.Structure):
     _fields_ = [("batch_size", ctypes.c_size_t), ("cost", ctypes.c_int), ("seed", ctypes.c_uint64)]
 
 
+class _BhC(ctypes.Structure):
+    _fields_ = [("batch_size", ctypes.c_size_t), ("probe_rho", ctypes.c_double),
+                ("position_rho", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
+class _BhStepC(ctypes.Structure):
+    _fields_ = [("first", ctypes.c_size_t), ("count", ctypes.c_size_t),
+                ("beta", ctypes.c_double), ("alpha", ctypes.c_double),
+                ("top", ctypes.c_double), ("bottom", ctypes.c_double)]
+
+
 class _OprC(ctypes.Structure):
     _fields_ = [("k", ctypes.c_size_t), ("eigenmodes", ctypes.POINTER(ctypes.c_double)),
                 ("weights", ctypes.POINTER(ctypes.c_double)),
@@ -133,6 +144,10 @@ _SIGNATURES = [
     ("ptyrs_engine_lsqml", [_P, ctypes.POINTER(_LsqmlC), _SZ, _SZ], ctypes.c_int),
     ("ptyrs_engine_lsqml_opr", [_P, ctypes.POINTER(_LsqmlC), _SZ, _SZ, ctypes.POINTER(_OprC)],
      ctypes.c_int),
+    ("ptyrs_engine_bh", [_P, ctypes.POINTER(_BhC), _SZ, _SZ], ctypes.c_int),
+    ("ptyrs_engine_bh_steps", [_P, ctypes.POINTER(_BhStepC), _SZ, ctypes.POINTER(_SZ)],
+     ctypes.c_int),
+    ("ptyrs_engine_bh_frames", [_P, ctypes.POINTER(_SZ), _SZ], ctypes.c_int),
     ("ptyrs_engine_set_launch", [_P, ctypes.POINTER(_LaunchC)], ctypes.c_int),
     ("ptyrs_engine_launch", [_P, ctypes.POINTER(_LaunchC)], ctypes.c_int),
     ("ptyrs_engine_lsqml_steps", [_P, ctypes.POINTER(_StepC), _SZ, ctypes.POINTER(_SZ)],
@@ -558,6 +573,30 @@ class GpuPtycho:
         _check(library().ptyrs_engine_lsqml_opr(self._h, ctypes.byref(c), int(nepoch),
                                                 int(calc_llk), ctypes.byref(v)))
         return eig, w, inten
+
+    def bh(self, nepoch, calc_llk, batch_size, probe_rho, seed, *, position_rho=0.1):
+        """`nepoch` BH epochs from a fresh BH (pty-chi's bilinear-Hessian method on the amplitude
+        cost), an LLK record every `calc_llk` epochs and after the last; one batch of every
+        frame runs conjugate gradient across the call's epochs (ptyrs-py/src/gpu.rs:124-144)."""
+        c = _BhC()
+        c.batch_size, c.probe_rho = int(batch_size), float(probe_rho)
+        c.position_rho, c.seed = float(position_rho), int(seed)
+        _check(library().ptyrs_engine_bh(self._h, ctypes.byref(c), int(nepoch), int(calc_llk)))
+
+    def bh_steps(self):
+        """The last BH epoch's steps as `Bh.epoch` returns them, dicts of `frames`, `beta`,
+        `alpha`, `top`, `bottom`; none before a BH call (ptyrs-py/src/gpu.rs:146-149)."""
+        lib = library()
+        n = ctypes.c_size_t()
+        _check(lib.ptyrs_engine_bh_steps(self._h, None, 0, ctypes.byref(n)))
+        recs = (_BhStepC * max(n.value, 1))()
+        _check(lib.ptyrs_engine_bh_steps(self._h, recs, n.value, ctypes.byref(n)))
+        nf = self.nframe if n.value else 0
+        frames = np.zeros(max(nf, 1), dtype=np.uintp)
+        _check(lib.ptyrs_engine_bh_frames(
+            self._h, frames.ctypes.data_as(ctypes.POINTER(ctypes.c_size_t)), nf))
+        return [dict(frames=[int(f) for f in frames[r.first:r.first + r.count]], beta=r.beta,
+                     alpha=r.alpha, top=r.top, bottom=r.bottom) for r in recs[:n.value]]
 
     def lsqml_steps(self):
         """The last LSQML epoch's steps as dicts of `frames`, `a_far`, `a_obj`

@@ -1,0 +1,253 @@
+// pty-chi's BH (bilinear-Hessian method) on the amplitude cost: the kernels of a batch's
+// Hessian sums, step and update (ptyrs/src/engine/bh.rs:130-799, the reference GPU's
+// ptyrs/src/engine/gpu/bh.rs:399-600).
+//
+// A batch takes its gradient through the stack passes (pass B's B_BH mode keeps U·ψ and scales
+// the field by 2(1 − √iobs/(A + EPS))/n; pass C keeps r = Uᴴ(2·td) per view and sums the probe
+// gradient; the object gradient comes from the tile pass). With the displacement u = (g_O,
+// ρ²·g_P) and, for conjugate gradient, v = D′ (the previous direction), the Hessian H(a, b) of
+// the amplitude cost over the batch is
+//   Σ Re(conj(r)·(a_O·b_W + b_O·a_W))                          (second order, exit plane)
+//   + 2·Σ_unmasked [(1 − d/A)·Σ_m Re(conj(L_a)·L_b) + d·c_a·c_b/A³]   (detector plane)
+// with a_W = a_P·q, L_a = U(W·a_O + a_W·O), c_a = Σ_m Re(conj(U ψ)·L_a), d = √iobs, A =
+// √(Σ_m |U ψ|²) + EPS. The first term uses ⟨2·td, U x⟩ = ⟨Uᴴ(2·td), x⟩, so only L_u and L_v
+// are transformed. H(u, u), H(u, v) and H(v, v) come from one pass each way; H(D, D) for D =
+// β·v − u follows by bilinearity.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace ptyrs {
+
+// Pass A of the Hessian sums. Per view row: L_u = W·u_O + su·u_P·q·O and (NF = 2) L_v = W·v_O +
+// v_P·q·O, row-transformed into lf[field][slot·M + m]; the second-order sums over the row with
+// r (psi, natural layout): Re Σ conj(r)·2·u_O·u_W, Re Σ conj(r)·(u_O·v_W + v_O·u_W), Re Σ
+// conj(r)·2·v_O·v_W (FP64, per workgroup into part[(view·NBLK + blk)·3 + i]). Grid
+// (NBLK, nb·M) of RowShape.
+template <typename T, int N, int NF>
+__global__ __launch_bounds__((RowShape<T, N>::NT)) void k_bh_hess_a(
+    const cplx<T>* __restrict__ O, const cplx<T>* __restrict__ uO,
+    const cplx<T>* __restrict__ vO, int nxo, const cplx<T>* __restrict__ Wt,
+    const cplx<T>* __restrict__ uP, T su, const cplx<T>* __restrict__ vP,
+    const cplx<T>* __restrict__ q, const int2* __restrict__ corner,
+    const int* __restrict__ order, int M, const cplx<T>* __restrict__ r, size_t field,
+    cplx<T>* __restrict__ lf, double* __restrict__ part, const cplx<T>* __restrict__ tw) {
+  using S = RowShape<T, N>;
+  constexpr int E = S::E, TL = S::TL;
+  constexpr int n = N * N;
+  __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
+  __shared__ cplx<T> stw[N];
+  __shared__ double red[S::NT];
+  const int tid = threadIdx.x, l = tid / TL, jl = tid % TL;
+  const int y = blockIdx.x * S::L + l;
+  const int v = blockIdx.y, b = v / M, m = v % M;
+  stage_twiddles<T, N, S::NT>(stw, tw);
+  const int2 c = corner[order[b]];
+  const size_t op = (size_t)(c.x + y) * nxo + c.y;
+  const size_t pr = (size_t)m * n + (size_t)y * N;
+  const size_t row = ((size_t)v * N + y) * N;
+  cplx<T> lu[E], lv[E];
+  double s[3] = {0, 0, 0};
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int x = jl + TL * e;
+    const cplx<T> o = ld(O + op + x), w = ld(Wt + pr + x), qq = ld(q + (size_t)y * N + x);
+    const cplx<T> rr = ld(r + row + x);
+    const cplx<T> ao = ld(uO + op + x), aw = ld(uP + pr + x) * qq * su;
+    lu[e] = w * ao + aw * o;
+    s[0] += 2.0 * double(cmulc(rr, ao * aw).x);
+    if constexpr (NF == 2) {
+      const cplx<T> bo = ld(vO + op + x), bw = ld(vP + pr + x) * qq;
+      lv[e] = w * bo + bw * o;
+      s[1] += double(cmulc(rr, ao * bw + bo * aw).x);
+      s[2] += 2.0 * double(cmulc(rr, bo * bw).x);
+    }
+  }
+  T* re = lre + S::base_row(l);
+  T* im = lim + S::base_row(l);
+  line_fft<T, N, E, false, typename S::AtRow>(lu, jl, re, im, stw);
+  if constexpr (NF == 2) line_fft<T, N, E, false, typename S::AtRow>(lv, jl, re, im, stw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    st(lf + row + jl + TL * e, lu[e]);
+    if constexpr (NF == 2) st(lf + field + row + jl + TL * e, lv[e]);
+  }
+  // (u, v) and (v, v) are 0 with NF = 1.
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double t = block_reduce<S::NT>(s[i], red, AddOp());
+    if (tid == 0) part[((size_t)v * gridDim.x + blockIdx.x) * 3 + i] = t;
+  }
+}
+
+// Pass B of the Hessian sums: per frame and stripe, the column transforms of L_u (and L_v) per
+// mode (ROW: the angular spectrum's row transforms, after its column passes), then over the
+// unmasked pixels with U ψ from `keep` (already in the detector plane, unitary units), lu = c·L_u
+// (c² = `norm`, the unnormalised transforms' factor): A = √(Σ_m |U ψ|²) + EPS, d = √iobs, the
+// detector-plane terms 2[(1 − d/A)·Σ_m Re(conj(lu)·lv) + d·cu·cv/A³] for (u, u), (u, v), (v, v)
+// (FP64, per workgroup into part[(slot·gridDim.x + blk)·3 + i]). Grid (NBLK, nb) of ColShape.
+template <typename T, int N, int M, bool ROW, int NF>
+__global__ __launch_bounds__((ColShape<T, N>::NT)) void k_bh_hess_b(
+    const cplx<T>* __restrict__ lf, size_t field, const cplx<T>* __restrict__ keep,
+    const float* __restrict__ counts, const int* __restrict__ order, double* __restrict__ part,
+    const cplx<T>* __restrict__ tw, T norm) {
+  using S = ColShape<T, N>;
+  constexpr int E = S::E, TL = S::TL, CG = S::L;
+  constexpr int n = N * N;
+  constexpr double EPS = 1e-8;
+  __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
+  __shared__ cplx<T> stw[N];
+  __shared__ double red[S::NT];
+  const int tid = threadIdx.x;
+  const int line = ROW ? tid / TL : tid % CG, jl = ROW ? tid % TL : tid / CG;
+  using At = std::conditional_t<ROW, typename S::AtRow, typename S::AtCol>;
+  const int2 bxy = block_xy();
+  const int x = bxy.x * CG + line;
+  const int b = bxy.y;
+  const int f = order[b];
+  T* re = lre + (ROW ? S::base_row(line) : S::base_col(line));
+  T* im = lim + (ROW ? S::base_row(line) : S::base_col(line));
+  auto at = [&](int e) { return ROW ? x * N + jl + TL * e : (jl + TL * e) * N + x; };
+  stage_twiddles<T, N, S::NT>(stw, tw);
+  const T c = sqrt(norm);
+  double I[E], uu[E], uv[E], vv[E], cu[E], cv[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) I[e] = uu[e] = uv[e] = vv[e] = cu[e] = cv[e] = 0;
+  for (int m = 0; m < M; ++m) {
+    const size_t plane = (size_t)(b * M + m) * n;
+    cplx<T> a[E], bb[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      a[e] = ld(lf + plane + at(e));
+      if constexpr (NF == 2) bb[e] = ld(lf + field + plane + at(e));
+    }
+    line_fft<T, N, E, false, At>(a, jl, re, im, stw);
+    if constexpr (NF == 2) line_fft<T, N, E, false, At>(bb, jl, re, im, stw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const cplx<T> p = ld(keep + plane + at(e));
+      const cplx<T> la = a[e] * c;
+      I[e] += double(norm2(p));
+      uu[e] += double(norm2(la));
+      cu[e] += double(cmulc(p, la).x);
+      if constexpr (NF == 2) {
+        const cplx<T> lb = bb[e] * c;
+        uv[e] += double(cmulc(la, lb).x);
+        vv[e] += double(norm2(lb));
+        cv[e] += double(cmulc(p, lb).x);
+      }
+    }
+  }
+  double h[3] = {0, 0, 0};
+  const float* cnt = counts + (size_t)f * n;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const float io = cnt[at(e)];
+    if (io >= 0.f) {
+      const double A = sqrt(I[e]) + EPS, d = sqrt(double(io));
+      const double w = 1.0 - d / A, k3 = d / (A * A * A);
+      h[0] += 2.0 * (w * uu[e] + k3 * cu[e] * cu[e]);
+      if constexpr (NF == 2) {
+        h[1] += 2.0 * (w * uv[e] + k3 * cu[e] * cv[e]);
+        h[2] += 2.0 * (w * vv[e] + k3 * cv[e] * cv[e]);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double t = block_reduce<S::NT>(h[i], red, AddOp());
+    if (tid == 0) part[((size_t)b * gridDim.x + bxy.x) * 3 + i] = t;
+  }
+}
+
+// Per workgroup over the object (no values) and the probe (np): Σ|g_O|² + ρ²·Σ|g_P|² = ⟨g, Pg⟩
+// and (cg) Σ Re(conj(g)·D′) = ⟨g, D′⟩ into part[blk·2 + {0, 1}] (FP64).
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_bh_dots(
+    const cplx<T>* __restrict__ gO, const cplx<T>* __restrict__ dO, size_t no,
+    const cplx<T>* __restrict__ gP, const cplx<T>* __restrict__ dP, size_t np, double rho2,
+    int cg, double* __restrict__ part) {
+  __shared__ double red[THREADS];
+  double a = 0, b = 0;
+  const size_t total = no + np;
+  for (size_t i = (size_t)blockIdx.x * THREADS + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * THREADS) {
+    const bool obj = i < no;
+    const cplx<T> g = obj ? ld(gO + i) : ld(gP + (i - no));
+    const double gg = double(g.x) * g.x + double(g.y) * g.y;
+    a += obj ? gg : rho2 * gg;
+    if (cg) {
+      const cplx<T> d = obj ? ld(dO + i) : ld(dP + (i - no));
+      b += double(g.x) * d.x + double(g.y) * d.y;
+    }
+  }
+  a = block_reduce<THREADS>(a, red, AddOp());
+  b = block_reduce<THREADS>(b, red, AddOp());
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// One workgroup: the batch's step from the partial sums, in a fixed order. H(u, u), H(u, v),
+// H(v, v) = second-order (npa partials of 3) + detector-plane (npb partials of 3) terms; β =
+// H(u, v)/H(v, v) with cg, else 0; top = ⟨g, Pg⟩ − β·⟨g, D′⟩ (= −⟨g, D⟩ for D = β·D′ − Pg);
+// bottom = H(D, D) = β²·H(v, v) − 2β·H(u, v) + H(u, u); α = top/bottom·count/size. step =
+// {α, β}; log[0..4) = {β, α, top, bottom}. ptyrs/src/engine/bh.rs:729-760
+__global__ __launch_bounds__(THREADS) static void k_bh_step(
+    const double* __restrict__ dots, int ndots, const double* __restrict__ pa, int npa,
+    const double* __restrict__ pb, int npb, int cg, double frac, double* __restrict__ step,
+    double* __restrict__ log) {
+  __shared__ double red[THREADS];
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < ndots; i += THREADS) {
+    s[0] += dots[2 * (size_t)i];
+    s[1] += dots[2 * (size_t)i + 1];
+  }
+  for (int i = threadIdx.x; i < npa; i += THREADS)
+    for (int k = 0; k < 3; ++k) s[2 + k] += pa[3 * (size_t)i + k];
+  for (int i = threadIdx.x; i < npb; i += THREADS)
+    for (int k = 0; k < 3; ++k) s[5 + k] += pb[3 * (size_t)i + k];
+  double t[8];
+  for (int k = 0; k < 8; ++k) t[k] = block_reduce<THREADS>(s[k], red, AddOp());
+  if (threadIdx.x != 0) return;
+  const double huu = t[2] + t[5], huv = t[3] + t[6], hvv = t[4] + t[7];
+  const double beta = cg ? huv / hvv : 0.0;
+  const double top = t[0] - beta * t[1];
+  const double bottom = cg ? beta * beta * hvv - 2.0 * beta * huv + huu : huu;
+  const double alpha = top / bottom * frac;
+  step[0] = alpha;
+  step[1] = beta;
+  log[0] = beta;
+  log[1] = alpha;
+  log[2] = top;
+  log[3] = bottom;
+}
+
+// D = β·D′ − s·g over the object (s = 1) then the probe (s = ρ²); X += α·D; with `keep` D′ = D.
+template <typename T>
+__global__ void k_bh_update(cplx<T>* __restrict__ O, cplx<T>* __restrict__ dO,
+                            const cplx<T>* __restrict__ gO, size_t no, cplx<T>* __restrict__ P,
+                            cplx<T>* __restrict__ dP, const cplx<T>* __restrict__ gP, size_t np,
+                            double rho2, const double* __restrict__ step, int cg, int keep) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= no + np) return;
+  const double alpha = step[0], beta = step[1];
+  const bool obj = i < no;
+  const size_t j = obj ? i : i - no;
+  cplx<T>* X = obj ? O : P;
+  cplx<T>* D = obj ? dO : dP;
+  const cplx<T> g = ld((obj ? gO : gP) + j);
+  const double s = obj ? 1.0 : rho2;
+  double dr = -s * double(g.x), di = -s * double(g.y);
+  if (cg) {
+    const cplx<T> d0 = ld(D + j);
+    dr += beta * double(d0.x);
+    di += beta * double(d0.y);
+  }
+  const cplx<T> x = ld(X + j);
+  st(X + j, cplx<T>{T(double(x.x) + alpha * dr), T(double(x.y) + alpha * di)});
+  if (keep) st(D + j, cplx<T>{T(dr), T(di)});
+}
+
+}  // namespace ptyrs

@@ -505,6 +505,31 @@ int ptyrs_engine_lsqml_opr(ptyrs_engine* h, const ptyrs_lsqml_params* p, size_t 
   return guarded(h, true, [&](EngineBase& e) { e.lsqml_opr(*p, nepoch, calc_llk, *var); });
 }
 
+int ptyrs_engine_bh(ptyrs_engine* h, const ptyrs_bh_params* p, size_t nepoch, size_t calc_llk) {
+  if (!p) {
+    last_error = "null params";
+    return PTYRS_EINVAL;
+  }
+  return guarded(h, true, [&](EngineBase& e) { e.bh(*p, nepoch, calc_llk); });
+}
+
+int ptyrs_engine_bh_steps(ptyrs_engine* h, ptyrs_bh_step* out, size_t cap, size_t* n) {
+  return guarded(h, false, [&](EngineBase& e) {
+    if (n) *n = e.bh_steps.size();
+    for (size_t i = 0; i < e.bh_steps.size() && i < cap && out; ++i) {
+      const BhRecord& r = e.bh_steps[i];
+      out[i] = ptyrs_bh_step{r.first, r.count, r.beta, r.alpha, r.top, r.bottom};
+    }
+  });
+}
+
+int ptyrs_engine_bh_frames(ptyrs_engine* h, size_t* out, size_t len) {
+  return guarded(h, false, [&](EngineBase& e) {
+    if (len != e.bh_order.size() || (len && !out)) throw Invalid("buffer has the wrong length");
+    for (size_t i = 0; i < len; ++i) out[i] = size_t(e.bh_order[i]);
+  });
+}
+
 int ptyrs_engine_lsqml_steps(ptyrs_engine* h, ptyrs_lsqml_step* out, size_t cap, size_t* n) {
   return guarded(h, false, [&](EngineBase& e) {
     if (n) *n = e.lsqml_steps.size();

@@ -13,6 +13,7 @@
 
 #include "host.hpp"
 #include "kernels.hpp"
+#include "bh.hpp"
 #include "opr.hpp"
 #include "sampling.hpp"
 
@@ -134,6 +135,13 @@ struct StepRecord {
   std::vector<double> a_objs;  // every slice's a_obj, slice 0's first
 };
 
+// One BH batch's step (BhStep, ptyrs/src/engine/bh.rs:60-72): its frames (`count` of them from
+// `first` in the epoch's order), β, α, −⟨g, D⟩ and H(D, D).
+struct BhRecord {
+  size_t first, count;
+  double beta, alpha, top, bottom;
+};
+
 // A captured epoch (GpuOptions::graph, ptyrs/src/engine/gpu/mod.rs:110-179): one graph per
 // epoch kind, keyed by what its launches bake in (shapes, parameters, buffer generation).
 struct EpochGraph {
@@ -165,6 +173,7 @@ class EngineBase {
   virtual void lsqml(const ptyrs_lsqml_params& p, size_t nepoch, size_t calc_llk) = 0;
   virtual void lsqml_opr(const ptyrs_lsqml_params& p, size_t nepoch, size_t calc_llk,
                          ptyrs_opr& var) = 0;
+  virtual void bh(const ptyrs_bh_params& p, size_t nepoch, size_t calc_llk) = 0;
   // Launch shapes that change results only by rounding (the autotuner's knobs): frames per
   // stack of AP, DM, RAAR, ML and the LLK pass (0: the default), and the workgroups per launch
   // that rPIE's and LSQML's pass C cut a batch into chunks for (0: the default).
@@ -205,6 +214,8 @@ class EngineBase {
 
   std::vector<Record> history;
   std::vector<StepRecord> lsqml_steps;  // the last LSQML epoch's
+  std::vector<BhRecord> bh_steps;       // the last BH epoch's
+  std::vector<int> bh_order;            // the last BH epoch's frames, batch after batch
   size_t cycle = 0;
   bool aborted = false;
   std::vector<int> corners;
@@ -644,6 +655,78 @@ class Engine : public EngineBase {
       }
       ++bi;
     }
+  }
+
+  // pty-chi's BH on the amplitude cost (GpuPtycho::bh, ptyrs/src/engine/gpu/bh.rs:33-68;
+  // Bh::epoch, ptyrs/src/engine/bh.rs:684-799): `nepoch` epochs from a fresh BH over
+  // random_batches(nframe, batch_size, seed, epoch); with one batch of every frame, conjugate
+  // gradient from the second epoch of the call. An LLK record after the epochs `calc_llk` asks.
+  void bh(const ptyrs_bh_params& p, size_t nepoch, size_t calc_llk) override {
+    if (p.batch_size == 0) throw Invalid("BH batch_size must be >= 1");
+    if (!std::isfinite(p.probe_rho) || !std::isfinite(p.position_rho))
+      throw Invalid("BH probe_rho and position_rho must be finite");
+    if (nslice_ > 1) throw Invalid("BH runs one object slice on this GPU engine");
+    if (pos_on_) throw Invalid("BH with position offsets is not run on this GPU engine");
+    if (nobj_ != 1) throw Invalid("the GPU BH runs one object mode");
+    if (M_ > kMaxModes) throw Invalid(modes_error());
+    if (nepoch == 0 || nframe_ == 0) {
+      cycle += nepoch;
+      return;
+    }
+    const size_t size = std::min(p.batch_size, nframe_);
+    const bool full = size == nframe_;
+    ensure_workspace(size, nepoch);
+    ensure_bh(size);
+    std::vector<int> order(nepoch * nframe_);
+    std::vector<std::vector<std::pair<int, int>>> spans(nepoch);
+    for (size_t ic = 0; ic < nepoch; ++ic) {
+      auto batches = random_batches(nframe_, size, p.seed, uint64_t(ic));
+      size_t off = 0;
+      for (auto& b : batches) {
+        std::copy(b.begin(), b.end(), order.begin() + ic * nframe_ + off);
+        spans[ic].push_back({int(off), int(b.size())});
+        off += b.size();
+      }
+    }
+    PTYRS_HIP(hipMemcpyAsync(order_.p, order.data(), sizeof(int) * order.size(),
+                             hipMemcpyHostToDevice, stream_));
+    probe_terms();
+    const double rho2 = p.probe_rho * p.probe_rho * scale_;
+    std::vector<std::pair<size_t, int>> due;
+    for (size_t ic = 0; ic < nepoch; ++ic) {
+      PTYRS_HIP(hipMemcpyAsync(eord_.p, order_.as<int>() + ic * nframe_, sizeof(int) * nframe_,
+                               hipMemcpyDeviceToDevice, stream_));
+      const int* ord = eord_.as<int>();
+      const bool cg = full && ic > 0;
+      std::vector<uint64_t> key = {uint64_t(cg), uint64_t(full), cap_, bits_of(rho2)};
+      for (auto& s : spans[ic]) key.push_back(uint64_t(s.second));
+      epoch(bh_graph_, key, [&] {
+        int k = 0;
+        for (auto& s : spans[ic])
+          bh_batch(rho2, ord + s.first, s.second, double(s.second) / double(size), cg, full,
+                   k++);
+      });
+      const bool rec = calc_llk > 0 && (ic % calc_llk == 0 || ic + 1 == nepoch);
+      if (rec) {
+        if (due.size() == hist_slots_) drain(due, PTYRS_BH);
+        const int slot = int(due.size());
+        llk_pass(slot);
+        due.push_back({cycle, slot});
+      }
+      ++cycle;
+    }
+    drain(due, PTYRS_BH);
+    const auto& last = spans[nepoch - 1];
+    std::vector<double> lg(4 * last.size());
+    PTYRS_HIP(hipMemcpyAsync(lg.data(), bh_log_.p, sizeof(double) * lg.size(),
+                             hipMemcpyDeviceToHost, stream_));
+    synchronize();
+    flush_timing();
+    bh_steps.clear();
+    for (size_t k = 0; k < last.size(); ++k)
+      bh_steps.push_back(BhRecord{size_t(last[k].first), size_t(last[k].second), lg[4 * k],
+                                  lg[4 * k + 1], lg[4 * k + 2], lg[4 * k + 3]});
+    bh_order.assign(order.begin() + (nepoch - 1) * nframe_, order.end());
   }
 
   // PyNX AP, DM, RAAR (ptyrs/src/engine/ptycho.rs:605-755), stacks of the whole scan; DM and
@@ -2206,6 +2289,122 @@ class Engine : public EngineBase {
     pre_part_.ensure(sizeof(T) * kPreBlocks);
     floor_.ensure(sizeof(T) * 1);
     dPm_.ensure(sizeof(cplx<T>) * nprobe_ * n);
+  }
+
+  // BH: U·ψ of a batch's views, L_u and L_v (two fields of a batch), the gradient and the
+  // previous direction of the object and the probe, the partial sums, the step, the epoch's
+  // per-batch log {β, α, top, bottom}.
+  DevBuf bh_keep_, bh_l_, bh_gO_, bh_dO_, bh_gP_, bh_dP_, bh_dots_, bh_pa_, bh_pb_, bh_step_,
+      bh_log_;
+  EpochGraph bh_graph_;
+  static constexpr int kBhDotBlocks = 256;
+  void ensure_bh(size_t cap) {
+    const size_t no = nyo_ * nxo_, np = nprobe_ * n, views = cap_ * M_;
+    bh_keep_.ensure(sizeof(cplx<T>) * views * n);
+    bh_l_.ensure(sizeof(cplx<T>) * 2 * views * n);
+    for (DevBuf* b : {&bh_gO_, &bh_dO_}) b->ensure(sizeof(cplx<T>) * no);
+    for (DevBuf* b : {&bh_gP_, &bh_dP_}) b->ensure(sizeof(cplx<T>) * np);
+    bh_dots_.ensure(sizeof(double) * 2 * kBhDotBlocks);
+    bh_pa_.ensure(sizeof(double) * 3 * views * S::NBLK);
+    bh_pb_.ensure(sizeof(double) * 3 * cap_ * SC::NBLK);
+    bh_step_.ensure(sizeof(double) * 2);
+    bh_log_.ensure(sizeof(double) * 4 * ((nframe_ + cap - 1) / cap));
+  }
+  static uint64_t bits_of(double v) {
+    uint64_t u;
+    std::memcpy(&u, &v, sizeof u);
+    return u;
+  }
+
+  // One BH batch (Bh::epoch's loop body, ptyrs/src/engine/bh.rs:720-797): the gradient g at
+  // the batch-start arrays, the Hessian sums along u = (g_O, ρ²·g_P) and, with cg, v = D′,
+  // the step and the update; `keep` stores D as the next D′; log slot `slot`.
+  void bh_batch(double rho2, const int* ord, int nb, double frac, bool cg, bool keep,
+                int slot) {
+    const int nt = ntiles();
+    const size_t no = nyo_ * nxo_, np = nprobe_ * n;
+    const int total = int(np);
+    run(K_BIN, [&] {
+      hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                         corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
+                         lcount_.as<int>(), (const T*)nullptr, M_, int(nyo_), int(nxo_),
+                         (T*)nullptr, tilemax_.as<T>(), (const cplx<T>*)nullptr);
+    });
+    pass_a(ord, nb, false);
+    BArgs<T> ba{nullptr, nullptr, nullptr, nullptr};
+    ba.keep = bh_keep_.as<cplx<T>>();
+    pass_b<B_BH>(ord, nb, ba);
+    const int chunk = chunk_of(nb);
+    const int nchunk = (nb + chunk - 1) / chunk;
+    run(K_PASS_C, [&] {
+      hipLaunchKernelGGL((k_pass_c_chi<T, N>), dim3(SR::NBLK, M_, nchunk), dim3(SR::NT), 0,
+                         stream_, psi_.as<cplx<T>>(), O_.as<cplx<T>>(), int(nxo_),
+                         Wt_.as<cplx<T>>(), q_.as<cplx<T>>(), corner_.as<int2>(), ord, nb, M_,
+                         chunk, 1, dP_.as<cplx<T>>(), tw_.as<cplx<T>>(), 0);
+    });
+    run(K_OBJ_DIR, [&] {
+      hipLaunchKernelGGL(k_lsqml_obj_dir<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                         bh_gO_.as<cplx<T>>(), Wt_.as<cplx<T>>(), psi_.as<cplx<T>>(), N, M_,
+                         corner_.as<int2>(), ord, list_.as<int>(), lcount_.as<int>(), int(cap_),
+                         int(nyo_), int(nxo_), ntx_, (const T*)nullptr, (const T*)nullptr);
+    });
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_probe_mean<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,
+                         bh_gP_.as<cplx<T>>(), dP_.as<cplx<T>>(), nchunk, total, 1);
+    });
+    const int ndots =
+        int(std::min<size_t>(kBhDotBlocks, (no + np + THREADS - 1) / THREADS));
+    const size_t field = cap_ * M_ * n;
+    run(K_DOTS, [&] {
+      hipLaunchKernelGGL(k_bh_dots<T>, dim3(ndots), dim3(THREADS), 0, stream_,
+                         (const cplx<T>*)bh_gO_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_dO_.as<cplx<T>>(), no,
+                         (const cplx<T>*)bh_gP_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_dP_.as<cplx<T>>(), np, rho2, int(cg),
+                         bh_dots_.as<double>());
+    });
+    const T norm = T(1) / (angular_ ? T(n) * T(n) : T(n));
+    run(K_OTHER, [&] {
+      auto ha = cg ? k_bh_hess_a<T, N, 2> : k_bh_hess_a<T, N, 1>;
+      hipLaunchKernelGGL(ha, dim3(S::NBLK, nb * M_), dim3(S::NT), 0, stream_,
+                         (const cplx<T>*)O_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_gO_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_dO_.as<cplx<T>>(), int(nxo_),
+                         (const cplx<T>*)Wt_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_gP_.as<cplx<T>>(), T(rho2),
+                         (const cplx<T>*)bh_dP_.as<cplx<T>>(), (const cplx<T>*)q_.as<cplx<T>>(),
+                         (const int2*)corner_.as<int2>(), ord, M_,
+                         (const cplx<T>*)psi_.as<cplx<T>>(), field, bh_l_.as<cplx<T>>(),
+                         bh_pa_.as<double>(), (const cplx<T>*)tw_.as<cplx<T>>());
+      if (angular_) pass_t<false>(bh_l_.as<cplx<T>>(), nb * M_, cg ? 2 : 1, field);
+      with_modes([&](auto mm) {
+        constexpr int MM = decltype(mm)::value;
+        auto hb = angular_ ? (cg ? k_bh_hess_b<T, N, MM, true, 2> : k_bh_hess_b<T, N, MM, true, 1>)
+                           : (cg ? k_bh_hess_b<T, N, MM, false, 2>
+                                 : k_bh_hess_b<T, N, MM, false, 1>);
+        hipLaunchKernelGGL(hb, dim3(SC::NBLK, nb), dim3(SC::NT), 0, stream_,
+                           (const cplx<T>*)bh_l_.as<cplx<T>>(), field,
+                           (const cplx<T>*)bh_keep_.as<cplx<T>>(), counts_.as<float>(), ord,
+                           bh_pb_.as<double>(), (const cplx<T>*)tw_.as<cplx<T>>(), norm);
+      });
+      hipLaunchKernelGGL(k_bh_step, dim3(1), dim3(THREADS), 0, stream_,
+                         (const double*)bh_dots_.as<double>(), ndots,
+                         (const double*)bh_pa_.as<double>(), nb * M_ * S::NBLK,
+                         (const double*)bh_pb_.as<double>(), nb * SC::NBLK, int(cg), frac,
+                         bh_step_.as<double>(), bh_log_.as<double>() + 4 * size_t(slot));
+    });
+    run(K_OBJ_UPDATE, [&] {
+      const size_t all = no + np;
+      hipLaunchKernelGGL(k_bh_update<T>, dim3(unsigned((all + 255) / 256)), dim3(256), 0,
+                         stream_, O_.as<cplx<T>>(), bh_dO_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_gO_.as<cplx<T>>(), no, P_.as<cplx<T>>(),
+                         bh_dP_.as<cplx<T>>(), (const cplx<T>*)bh_gP_.as<cplx<T>>(), np, rho2,
+                         (const double*)bh_step_.as<double>(), int(cg), int(keep));
+    });
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_probe_terms<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,
+                         P_.as<cplx<T>>(), q_.as<cplx<T>>(), Wt_.as<cplx<T>>(), n, total);
+    });
   }
 
   static constexpr int kPreBlocks = 1024;

@@ -287,7 +287,7 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_pass_a(
 
 // ---- pass B: column transforms, the detector-plane operator, inverse column transforms --------
 enum PassB { B_RPIE = 0, B_LLK = 1, B_AMPLITUDE = 2, B_GAUSS = 3, B_ICALC = 4, B_POISSON = 5,
-             B_GRAD = 6, B_GRAD_AMP = 7 };
+             B_GRAD = 6, B_GRAD_AMP = 7, B_BH = 8 };
 
 // Per-slot values some pass-B modes read or write.
 template <typename T>
@@ -296,6 +296,7 @@ struct BArgs {
   const T* prev;    // B_ICALC: batch mean of the slots' last a_far
   double* part;     // B_ICALC: per-workgroup (num, den) of the Poisson xi solver
   T* icalc;         // B_ICALC: model intensity per slot and pixel, DFT index order
+  cplx<T>* keep = nullptr;  // B_BH: the detector-plane field U·ψ per view ([slot·M + m][n])
 };
 
 // Pass B's lines: per mode, a stripe of CG columns (or rows) of N values, E per thread (16
@@ -319,6 +320,9 @@ constexpr int pass_b_threads() {
 // keep their value), with the 1/n of the unnormalised round trip folded in; B_LLK writes the
 // four LLK sums of the workgroup's pixels (FP64) to llk_part[frame][blockIdx.x][4] and stores
 // nothing.
+// B_BH keeps U·ψ (the field times √norm) in ba.keep and scales the field by 2(1 − √iobs/(A +
+// 1e-8))·norm, A = √(icalc), 0 where masked: Uᴴ of it is BH's 2·∂F/∂(U ψ)* taken back
+// (ptyrs/src/engine/bh.rs:233-260).
 // B_GAUSS and B_POISSON scale the field by (1 − a_far·c)/n with LSQML's residual c (Gaussian
 // 2(1 − √iobs/(√icalc + 1e-9)) with a_far = 1/2, Poisson 1 − iobs/(icalc + 1e-6)), 0 where
 // masked (ptyrs/src/engine/lsqml.rs:97-170). B_ICALC stores icalc and the first pass of the
@@ -376,6 +380,12 @@ __global__ __launch_bounds__((pass_b_threads<T, N, M>())) void k_pass_b(
   cplx<T> twc[TC ? tw_cache_size(N, E) : 1];
   if constexpr (TC) load_tw_cache<T, N, E>(twc, jl, stw);
   line_fft<T, N, E, false, At, TC>(v, jl, re, im, stw, twc);
+  if constexpr (MODE == B_BH) {
+    const T c = sqrt(norm);
+    cplx<T>* kp = ba.keep + (size_t)(b * M + m) * n;
+#pragma unroll
+    for (int e = 0; e < E; ++e) st(kp + at(e), v[e] * c);
+  }
   // The transform's last exchange has been read by every thread: the regions take the
   // intensities, pixel p = jl + TL·e of line `line` of group k at lre[k·W + p·CG + line], and
   // the owners' operators at lim[p·CG + line].
@@ -420,6 +430,11 @@ __global__ __launch_bounds__((pass_b_threads<T, N, M>())) void k_pass_b(
       if (io >= 0.f) {
         const T ic = I * inv_n;
         r = T(2) * (T(1) - sqrt(T(io) / (ic > T(1e-12) ? ic : T(1e-12)))) * inv_n;
+      }
+    } else if constexpr (MODE == B_BH) {
+      if (io >= 0.f) {
+        const T A = sqrt(I * inv_n) + T(1e-8);
+        r = T(2) * (T(1) - sqrt(T(io)) / A) * inv_n;
       }
     } else if constexpr (MODE == B_GAUSS || MODE == B_POISSON) {
       const T ic = I * inv_n;
@@ -514,14 +529,16 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_t(cplx<T>* __rest
 // ---- pass C: inverse rows, Δ = ψ' − W·O, probe direction ----------------------------------------
 // LSQML's pass C. Grid (NBLK, M, nchunk). Each workgroup owns rows of mode m and runs over the
 // batch slots of its chunk in order: it transforms the rows back, stores χ = ψ' − W·O over the
-// projected field, and adds conj(O·q)·χ into a per-chunk probe direction.
+// projected field (sub_wo = 0: χ = ψ', BH's back-projected residual), and adds conj(O·q)·χ into
+// a per-chunk probe direction.
 // ptyrs/src/engine/lsqml.rs:97-228
 template <typename T, int N>
 __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_chi(
     cplx<T>* __restrict__ psi, const cplx<T>* __restrict__ O, int nxo,
     const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ q,
     const int2* __restrict__ corner, const int* __restrict__ order, int nb, int M, int chunk,
-    int probe_dir, cplx<T>* __restrict__ dP_part, const cplx<T>* __restrict__ tw) {
+    int probe_dir, cplx<T>* __restrict__ dP_part, const cplx<T>* __restrict__ tw,
+    int sub_wo = 1) {
   using S = RowShapeC<T, N>;
   constexpr int E = S::E, TL = S::TL;
   constexpr int n = N * N;
@@ -577,7 +594,7 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_chi(
     for (int e = 0; e < E; ++e) {
       const int x = jl + TL * e;
       const cplx<T> o = ow[e];
-      const cplx<T> d = v[e] - w[e] * o;
+      const cplx<T> d = sub_wo ? v[e] - w[e] * o : v[e];
       st(prow + x, d);
       if (probe_dir) acc[e] = acc[e] + cmulc(o * qq[e], d);
     }

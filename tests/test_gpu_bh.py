@@ -1,0 +1,154 @@
+"""BH (pty-chi's bilinear-Hessian method on the amplitude cost) on the GPU through the C ABI
+(ptyrs_engine_bh) against the reference's fixtures and the FP64 oracle, whose BH the fixtures
+pin (tests/test_oracle_bh.py): the seeded far-field scenario with one batch of all 40 frames
+(conjugate gradient from the second epoch) and with batches of 16 (bh_{full,mini}_*: every
+step's β, α, ⟨g, D⟩ and H(D, D), the object and probe changes), the near-field cases (angular
+spectrum, Fresnel forward and back, an evanescent band) against the oracle, LLK records, graph
+replays against eager launches, and the refusals. complex128 to FP64 rounding (1e-9), complex64
+to 1e-3 of the steps (1e-2 of the changes)."""
+
+import numpy as np
+import pytest
+
+import ptyrs_amd as pa
+from conftest import max_rel, table
+from oracle.bh import Bh
+from test_gpu_rpie import rel_l2
+from test_gpu_scenario import engine
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"c128": 1e-9, "c64": 1e-3}
+
+
+def check_steps(got, rows, tol, lead=0):
+    """`got` (bh_steps dicts) against fixture rows `step epoch count beta alpha top bottom
+    frames..` (after `lead` leading columns)."""
+    assert len(got) == len(rows)
+    for s, r in zip(got, rows):
+        r = r[lead:]
+        assert s["frames"] == [int(f) for f in r[7:]]
+        if r[3] == 0.0:
+            assert s["beta"] == 0.0
+        else:
+            assert abs(s["beta"] - r[3]) < tol * 10 * abs(r[3]), (s["beta"], r[3])
+        for name, want in (("alpha", r[4]), ("top", r[5]), ("bottom", r[6])):
+            assert abs(s[name] - want) < tol * abs(want), (name, s[name], want)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("batch_size,name", [(40, "bh_full"), (16, "bh_mini")])
+def test_far_field_matches_fixtures(dtype, batch_size, name):
+    # ptyrs/tests/engine_bh.rs:164-243 (probe ρ 0.01, seed 7, two epochs): each call is a
+    # fresh BH, so epoch 0's steps come from a one-epoch call and epoch 1's from a two-epoch one.
+    rows = table(f"{name}_steps_f64.txt")
+    tol = TOL[dtype]
+    g = engine(2, dtype)
+    g.bh(1, 0, batch_size, 0.01, 7)
+    check_steps(g.bh_steps(), [r for r in rows if r[1] == 0], tol)
+    g = engine(2, dtype)
+    obj0, probe0 = g.object, g.probe
+    g.bh(2, 1, batch_size, 0.01, 7)
+    check_steps(g.bh_steps(), [r for r in rows if r[1] == 1], tol)
+    assert list(g.history["algorithm"]) == ["bh", "bh"]
+    assert list(g.history["cycle"]) == [0, 1]
+    obj_rows = np.array(table(f"{name}_obj_f64.txt"))
+    probe_rows = np.array(table(f"{name}_probe_f64.txt"))
+    want_o = (obj_rows[:, 2] + obj_rows[:, 4]) + 1j * (obj_rows[:, 3] + obj_rows[:, 5])
+    do = (g.object - obj0)[0][obj_rows[:, 0].astype(int), obj_rows[:, 1].astype(int)]
+    idx = probe_rows[:, :3].astype(int)
+    want_p = (probe_rows[:, 3] + probe_rows[:, 5]) + 1j * (probe_rows[:, 4] + probe_rows[:, 6])
+    dp = (g.probe - probe0)[idx[:, 0], idx[:, 1], idx[:, 2]]
+    print(name, dtype, rel_l2(do, want_o), rel_l2(dp, want_p))
+    assert rel_l2(do, want_o) < tol * 10
+    assert rel_l2(dp, want_p) < tol * 10
+
+
+def near(case, dtype, graph=True):
+    from test_gpu_lsqml import _near_state
+    corners, obj, probe, counts = _near_state()
+    c = table("lsqml_near_cases_f64.txt")[case]
+    vals = np.array([r[2:] for r in counts if r[0] == case]).reshape(len(corners), 24, 24)
+    nf = len(corners)
+    scan = pa.Scan(vals.astype(np.float32), np.zeros(nf), np.zeros(nf), c[1], 1.0, 75e-6,
+                   corners=corners[:, 1:3].astype(np.int64))
+    w = pa.Wavefield([24, 24], c[1], [c[2], c[2]], c[3])
+    return pa.GpuPtycho(scan, obj, probe, optics=pa.Optics.free_space(w), dtype=dtype,
+                        graph=graph)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("case", [0, 1, 2, 3])
+@pytest.mark.parametrize("method,batch_size", [(0, 16), (1, 6)])
+def test_near_field_matches_fixtures(dtype, case, method, batch_size):
+    # pty-chi's planar near field (ptyrs/tests/engine_bh_near.rs:103-176): probe ρ 1, two
+    # epochs, the last epoch's steps and the changes over the run.
+    g = near(case, dtype)
+    obj0, probe0 = g.object, g.probe
+    g.bh(2, 0, batch_size, 1.0, 7)
+    tol = TOL[dtype]
+    rows = [r for r in table("bh_near_steps_f64.txt")
+            if r[0] == case and r[1] == method and r[3] == 1]
+    check_steps(g.bh_steps(), rows, tol, lead=2)
+    for now, start, name, lead in ((g.object, obj0, "bh_near_obj_f64.txt", 4),
+                                   (g.probe, probe0, "bh_near_probe_f64.txt", 5)):
+        r = np.array([t for t in table(name) if t[0] == case and t[1] == method])
+        want = r[:, lead] + 1j * r[:, lead + 1]
+        print(case, method, dtype, name, rel_l2((now - start).ravel(), want))
+        assert rel_l2((now - start).ravel(), want) < tol * 10
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_more_epochs_match_oracle(dtype):
+    # Four conjugate-gradient epochs and three of batches of 7 (a ragged last batch of 5) on the
+    # seeded scenario with one probe mode, against the oracle, with the LLK after each epoch.
+    from oracle import scenario as sc
+    from oracle.ptycho import llk, obj_probe_to_psi
+    for batch in (40, 7):
+        p, _ = sc.parity_start(nprobe=1)
+        bh = Bh(batch, probe_rho=0.02, seed=3)
+        obj, probe, want = p.obj, p.probe, []
+        nepoch = 4 if batch == 40 else 3
+        for _ in range(nepoch):
+            obj, probe, steps = bh.epoch(p.addr, p.iobs, p.optics, obj, probe)
+            psi = p.optics.exit_forward(obj_probe_to_psi(None, p.views_all, obj, probe, p.quad))
+            want.append(llk(p.iobs, p.addr.nmode, psi).normalised(p.nb_obs).poisson)
+        g = engine(1, dtype)
+        g.bh(nepoch, 1, batch, 0.02, 3)
+        tol = TOL[dtype]
+        assert max_rel(g.history["poisson"], want) < tol
+        got = g.bh_steps()
+        assert [s["frames"] for s in got] == [list(s[0]) for s in steps]
+        assert max_rel([s["alpha"] for s in got], [s[2] for s in steps]) < tol
+        assert rel_l2(g.object, obj) < tol * 10
+        assert rel_l2(g.probe, probe) < tol * 10
+
+
+@pytest.mark.parametrize("dtype", ["c64", "c128"])
+def test_graph_replays_match_eager(dtype):
+    out = []
+    for graph in (True, False):
+        g = near(0, dtype, graph=graph)
+        g.bh(3, 1, 16, 1.0, 7)
+        g.bh(2, 0, 6, 1.0, 9)
+        g.bh(2, 1, 16, 1.0, 7)
+        out.append((g.object, g.probe, g.history["poisson"], g.bh_steps()))
+    a, b = out
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert np.array_equal(a[2], b[2])
+    assert a[3] == b[3]
+
+
+def test_refusals():
+    g = engine(2, "c64")
+    with pytest.raises(ValueError, match="batch_size"):
+        g.bh(1, 0, 0, 0.01, 7)
+    with pytest.raises(ValueError):
+        g.bh(1, 0, 16, float("nan"), 7)
+    g.set_features(position=np.zeros((g.nframe, 2)))
+    with pytest.raises(ValueError, match="position"):
+        g.bh(1, 0, 16, 0.01, 7)
+    assert g.cycle == 0 and g.bh_steps() == []
+    g.set_features()
+    g.bh(0, 0, 16, 0.01, 7)
+    assert g.bh_steps() == []
