@@ -9,10 +9,11 @@ over all 1024 positions; `value` is positions·iterations per second of the comp
 same epochs from the same start; `nrmse` compares the two objects after phase-ramp and
 global-phase alignment on the illuminated region, and `nrmse.vs_oracle` one epoch of each
 precision against the FP64 CPU restatement (oracle/cpu_rpie.cpp) from the same start. The line
-also carries LSQML's rates and kernel roofline and AP, DM, RAAR and ML rates on the same scan.
+also carries LSQML's and BH's rates and kernel rooflines and AP, DM, RAAR and ML rates on the
+same scan.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-    python bench.py --algorithm lsqml|ap|dm|raar|ml [--steps K]   # one algorithm alone
+    python bench.py --algorithm lsqml|bh|ap|dm|raar|ml [--steps K]   # one algorithm alone
     python bench.py --multi-scan [--steps K]   # 16 uneven scans: sequential vs grouped epochs
     python bench.py --large-multislice [--steps K]   # 512² x 4, 4096² x 8 slices, refined positions
     python bench.py --copy-probe     # a plain 2 GiB device copy: the PMC calibration kernel and
@@ -207,6 +208,22 @@ def lsqml_bytes(c, r):
     }
 
 
+def bh_bytes(c, r, cg):
+    """BH's algorithmic bytes per frame and kernel kind; `cg`: the Hessian passes carry the
+    previous direction too (two fields)."""
+    n = N * N
+    nf = 2 if cg else 1
+    return {
+        "pass_a": n * c + M * n * c,
+        "pass_b": 3 * M * n * c + 4 * n,             # Ψ in and out, U·Ψ kept, counts in
+        "pass_c": 2 * M * n * c + n * c,             # Ψ' and the O window in, r out
+        "obj_dir": M * n * c,                        # r of every view over the tile in
+        # the O, u_O (and v_O) windows and r in, L_u (and L_v) row-transformed out
+        "hess_a": (1 + nf) * n * c + (1 + nf) * M * n * c,
+        "hess_b": (nf + 1) * M * n * c + 4 * n,      # L fields and U·Ψ in, counts in
+    }
+
+
 def kernel_roofline(stats, per_frame, positions):
     """Roofline of the launch kind with the largest total time: its algorithmic bytes per
     launch (`per_frame` bytes × the positions one launch covers, `positions` over all its
@@ -253,6 +270,41 @@ def algorithm_rates(pa, scan, d, steps=3, algorithms=("ap", "dm", "raar", "ml"))
             del g
         out[alg] = r
         print(f"[bench] {alg}: {r}", file=sys.stderr, flush=True)
+    return out
+
+
+def bh_rates(pa, scan, d, steps=3):
+    """BH (pty-chi's bilinear-Hessian method, probe ρ 0.01) on the same scan: batches of 100
+    (steepest descent) and one batch of all 1024 frames (conjugate gradient from the second
+    epoch), positions·iter/s of `steps` epochs after one warm-up, complex64 and complex128; the
+    conjugate-gradient run's kernel times and the roofline of its dominant kernel."""
+    out = {"probe_rho": 0.01, "steps": steps}
+    for name, batch in (("batches_100", 100), ("cg_all_frames", NPOS)):
+        r = {}
+        for dtype in ("c64", "c128"):
+            g = pa.GpuPtycho(scan, d["start_obj"], d["start_probe"], dtype=dtype)
+            g.bh(1, 0, batch, 0.01, 7)
+            g.synchronize()
+            t = time.perf_counter()
+            g.bh(steps, 0, batch, 0.01, 7)
+            g.synchronize()
+            dt = time.perf_counter() - t
+            r["value" if dtype == "c64" else "fp64"] = round(NPOS * steps / dt, 1)
+            r["llk_poisson_" + dtype] = g.llk()["poisson"]
+            if dtype == "c64" and batch == NPOS:
+                g.profile(True)
+                g.bh(2, 0, batch, 0.01, 7)
+                g.synchronize()
+                stats = g.kernel_stats()
+                g.profile(False)
+                r["kernels_ms_per_step"] = {k: round(v[1] / 2, 4) for k, v in stats.items()
+                                            if v[0]}
+                # the second epoch of the call is a conjugate-gradient one, the first is not:
+                # the roofline counts the two-field Hessian passes for both.
+                r["roofline"] = kernel_roofline(stats, bh_bytes(8, 4, True), NPOS * 2)
+            del g
+        out[name] = r
+        print(f"[bench] bh {name}: {r}", file=sys.stderr, flush=True)
     return out
 
 
@@ -439,7 +491,7 @@ def main():
     ap.add_argument("--no-copy", action="store_true",
                     help="skip the child copy-bandwidth process (counter passes)")
     ap.add_argument("--batch", type=int, default=BATCH, help="rPIE batch size (positions)")
-    ap.add_argument("--algorithm", choices=("lsqml", "ap", "dm", "raar", "ml"), default=None,
+    ap.add_argument("--algorithm", choices=("lsqml", "bh", "ap", "dm", "raar", "ml"), default=None,
                     help="time one other algorithm alone on the headline scan (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -471,6 +523,10 @@ def main():
                   seed=7)
     if a.algorithm == "lsqml":
         print(json.dumps({"workload": "headline scan", "lsqml": lsqml_rates(pa, scan, d, a, a.steps)}),
+              flush=True)
+        return
+    if a.algorithm == "bh":
+        print(json.dumps({"workload": "headline scan", "bh": bh_rates(pa, scan, d, a.steps)}),
               flush=True)
         return
     if a.algorithm is not None:
@@ -558,6 +614,7 @@ def main():
     del g, g32, g64
     out["lsqml"] = lsqml_rates(pa, scan, d, a)
     out["algorithms"] = algorithm_rates(pa, scan, d)
+    out["bh"] = bh_rates(pa, scan, d)
     if not a.no_cpu:
         out["cpu_baseline"], state = cpu_baseline(d, a.batch)
         out["nrmse"]["vs_oracle"] = against_oracle(pa, scan, d, params, state)

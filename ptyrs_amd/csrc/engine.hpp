@@ -56,10 +56,10 @@ inline void fault_at(int exit) {
 
 // Kernel names for timing; the index is the `kind` of a timed launch.
 enum Kind { K_PASS_A = 0, K_PASS_B, K_PASS_C, K_BIN, K_OVERLAP, K_OBJ_UPDATE, K_PROBE, K_LLK,
-            K_OTHER, K_OBJ_DIR, K_DOTS, K_COUNT };
+            K_OTHER, K_OBJ_DIR, K_DOTS, K_HESS_A, K_HESS_B, K_COUNT };
 inline const char* kind_names() {
   return "pass_a,pass_b,pass_c,bin_tiles,obj_overlap,obj_update,probe_update,llk,other,"
-         "obj_dir,dots";
+         "obj_dir,dots,hess_a,hess_b";
 }
 
 struct DevBuf {
@@ -2364,7 +2364,7 @@ class Engine : public EngineBase {
                          bh_dots_.as<double>());
     });
     const T norm = T(1) / (angular_ ? T(n) * T(n) : T(n));
-    run(K_OTHER, [&] {
+    run(K_HESS_A, [&] {
       auto ha = cg ? k_bh_hess_a<T, N, 2> : k_bh_hess_a<T, N, 1>;
       hipLaunchKernelGGL(ha, dim3(S::NBLK, nb * M_), dim3(S::NT), 0, stream_,
                          (const cplx<T>*)O_.as<cplx<T>>(),
@@ -2376,6 +2376,8 @@ class Engine : public EngineBase {
                          (const int2*)corner_.as<int2>(), ord, M_,
                          (const cplx<T>*)psi_.as<cplx<T>>(), field, bh_l_.as<cplx<T>>(),
                          bh_pa_.as<double>(), (const cplx<T>*)tw_.as<cplx<T>>());
+    });
+    run(K_HESS_B, [&] {
       if (angular_) pass_t<false>(bh_l_.as<cplx<T>>(), nb * M_, cg ? 2 : 1, field);
       with_modes([&](auto mm) {
         constexpr int MM = decltype(mm)::value;
@@ -2387,6 +2389,8 @@ class Engine : public EngineBase {
                            (const cplx<T>*)bh_keep_.as<cplx<T>>(), counts_.as<float>(), ord,
                            bh_pb_.as<double>(), (const cplx<T>*)tw_.as<cplx<T>>(), norm);
       });
+    });
+    run(K_OTHER, [&] {
       hipLaunchKernelGGL(k_bh_step, dim3(1), dim3(THREADS), 0, stream_,
                          (const double*)bh_dots_.as<double>(), ndots,
                          (const double*)bh_pa_.as<double>(), nb * M_ * S::NBLK,
