@@ -20,19 +20,23 @@
 namespace ptyrs {
 
 // Pass A of the Hessian sums. Per view row: L_u = W·u_O + su·u_P·q·O and (NF = 2) L_v = W·v_O +
-// v_P·q·O, row-transformed into lf[field][slot·M + m]; the second-order sums over the row with
+// v_P·q·O, row-transformed into lf[(slot·M + m)·NF + field] (a view's fields side by side);
+// the second-order sums over the row with
 // r (psi, natural layout): Re Σ conj(r)·2·u_O·u_W, Re Σ conj(r)·(u_O·v_W + v_O·u_W), Re Σ
 // conj(r)·2·v_O·v_W (FP64, per workgroup into part[(view·NBLK + blk)·3 + i]). Grid
-// (NBLK, nb·M) of RowShape.
+// (NBLK, nb·M) of HessShape: pass C's line shape (half pass A's values per thread), which keeps
+// a thread's loads in flight within its registers.
 template <typename T, int N, int NF>
-__global__ __launch_bounds__((RowShape<T, N>::NT)) void k_bh_hess_a(
+using HessShape = RowShapeC<T, N>;
+template <typename T, int N, int NF>
+__global__ __launch_bounds__((HessShape<T, N, NF>::NT)) void k_bh_hess_a(
     const cplx<T>* __restrict__ O, const cplx<T>* __restrict__ uO,
     const cplx<T>* __restrict__ vO, int nxo, const cplx<T>* __restrict__ Wt,
     const cplx<T>* __restrict__ uP, T su, const cplx<T>* __restrict__ vP,
     const cplx<T>* __restrict__ q, const int2* __restrict__ corner,
-    const int* __restrict__ order, int M, const cplx<T>* __restrict__ r, size_t field,
+    const int* __restrict__ order, int M, const cplx<T>* __restrict__ r,
     cplx<T>* __restrict__ lf, double* __restrict__ part, const cplx<T>* __restrict__ tw) {
-  using S = RowShape<T, N>;
+  using S = HessShape<T, N, NF>;
   constexpr int E = S::E, TL = S::TL;
   constexpr int n = N * N;
   __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
@@ -46,6 +50,7 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_bh_hess_a(
   const size_t op = (size_t)(c.x + y) * nxo + c.y;
   const size_t pr = (size_t)m * n + (size_t)y * N;
   const size_t row = ((size_t)v * N + y) * N;
+  const size_t lrow = ((size_t)v * NF * N + y) * N;
   cplx<T> lu[E], lv[E];
   double s[3] = {0, 0, 0};
 #pragma unroll
@@ -69,8 +74,8 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_bh_hess_a(
   if constexpr (NF == 2) line_fft<T, N, E, false, typename S::AtRow>(lv, jl, re, im, stw);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    st(lf + row + jl + TL * e, lu[e]);
-    if constexpr (NF == 2) st(lf + field + row + jl + TL * e, lv[e]);
+    st(lf + lrow + jl + TL * e, lu[e]);
+    if constexpr (NF == 2) st(lf + lrow + n + jl + TL * e, lv[e]);
   }
   // (u, v) and (v, v) are 0 with NF = 1.
 #pragma unroll
@@ -88,7 +93,7 @@ __global__ __launch_bounds__((RowShape<T, N>::NT)) void k_bh_hess_a(
 // (FP64, per workgroup into part[(slot·gridDim.x + blk)·3 + i]). Grid (NBLK, nb) of ColShape.
 template <typename T, int N, int M, bool ROW, int NF>
 __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_bh_hess_b(
-    const cplx<T>* __restrict__ lf, size_t field, const cplx<T>* __restrict__ keep,
+    const cplx<T>* __restrict__ lf, const cplx<T>* __restrict__ keep,
     const float* __restrict__ counts, const int* __restrict__ order, double* __restrict__ part,
     const cplx<T>* __restrict__ tw, T norm) {
   using S = ColShape<T, N>;
@@ -115,11 +120,12 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_bh_hess_b(
   for (int e = 0; e < E; ++e) I[e] = uu[e] = uv[e] = vv[e] = cu[e] = cv[e] = 0;
   for (int m = 0; m < M; ++m) {
     const size_t plane = (size_t)(b * M + m) * n;
+    const size_t lplane = plane * NF;
     cplx<T> a[E], bb[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      a[e] = ld(lf + plane + at(e));
-      if constexpr (NF == 2) bb[e] = ld(lf + field + plane + at(e));
+      a[e] = ld(lf + lplane + at(e));
+      if constexpr (NF == 2) bb[e] = ld(lf + lplane + n + at(e));
     }
     line_fft<T, N, E, false, At>(a, jl, re, im, stw);
     if constexpr (NF == 2) line_fft<T, N, E, false, At>(bb, jl, re, im, stw);
@@ -186,6 +192,22 @@ __global__ __launch_bounds__(THREADS) void k_bh_dots(
   if (threadIdx.x == 0) {
     part[2 * blockIdx.x] = a;
     part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// Per workgroup the sums of rows [blk·per, blk·per + per) of a [count][3] array (FP64, a fixed
+// order for a fixed count) into out[blk][3]: the first level of the Hessian sums' reduction.
+__global__ __launch_bounds__(THREADS) static void k_sum_rows3(const double* __restrict__ in,
+                                                              int count, int per,
+                                                              double* __restrict__ out) {
+  __shared__ double red[THREADS];
+  const int i0 = blockIdx.x * per, i1 = i0 + per < count ? i0 + per : count;
+  double s[3] = {0, 0, 0};
+  for (int i = i0 + threadIdx.x; i < i1; i += THREADS)
+    for (int k = 0; k < 3; ++k) s[k] += in[3 * (size_t)i + k];
+  for (int k = 0; k < 3; ++k) {
+    const double t = block_reduce<THREADS>(s[k], red, AddOp());
+    if (threadIdx.x == 0) out[3 * (size_t)blockIdx.x + k] = t;
   }
 }
 

@@ -2294,10 +2294,11 @@ class Engine : public EngineBase {
   // BH: U·ψ of a batch's views, L_u and L_v (two fields of a batch), the gradient and the
   // previous direction of the object and the probe, the partial sums, the step, the epoch's
   // per-batch log {β, α, top, bottom}.
-  DevBuf bh_keep_, bh_l_, bh_gO_, bh_dO_, bh_gP_, bh_dP_, bh_dots_, bh_pa_, bh_pb_, bh_step_,
-      bh_log_;
+  DevBuf bh_keep_, bh_l_, bh_gO_, bh_dO_, bh_gP_, bh_dP_, bh_dots_, bh_pa_, bh_pb_, bh_sum_,
+      bh_step_, bh_log_;
   EpochGraph bh_graph_;
   static constexpr int kBhDotBlocks = 256;
+  static constexpr int kBhSumBlocks = 128;  // first-level blocks of each Hessian partial array
   void ensure_bh(size_t cap) {
     const size_t no = nyo_ * nxo_, np = nprobe_ * n, views = cap_ * M_;
     bh_keep_.ensure(sizeof(cplx<T>) * views * n);
@@ -2305,8 +2306,9 @@ class Engine : public EngineBase {
     for (DevBuf* b : {&bh_gO_, &bh_dO_}) b->ensure(sizeof(cplx<T>) * no);
     for (DevBuf* b : {&bh_gP_, &bh_dP_}) b->ensure(sizeof(cplx<T>) * np);
     bh_dots_.ensure(sizeof(double) * 2 * kBhDotBlocks);
-    bh_pa_.ensure(sizeof(double) * 3 * views * S::NBLK);
+    bh_pa_.ensure(sizeof(double) * 3 * views * HessShape<T, N, 2>::NBLK);
     bh_pb_.ensure(sizeof(double) * 3 * cap_ * SC::NBLK);
+    bh_sum_.ensure(sizeof(double) * 3 * 2 * kBhSumBlocks);
     bh_step_.ensure(sizeof(double) * 2);
     bh_log_.ensure(sizeof(double) * 4 * ((nframe_ + cap - 1) / cap));
   }
@@ -2354,7 +2356,6 @@ class Engine : public EngineBase {
     });
     const int ndots =
         int(std::min<size_t>(kBhDotBlocks, (no + np + THREADS - 1) / THREADS));
-    const size_t field = cap_ * M_ * n;
     run(K_DOTS, [&] {
       hipLaunchKernelGGL(k_bh_dots<T>, dim3(ndots), dim3(THREADS), 0, stream_,
                          (const cplx<T>*)bh_gO_.as<cplx<T>>(),
@@ -2365,8 +2366,9 @@ class Engine : public EngineBase {
     });
     const T norm = T(1) / (angular_ ? T(n) * T(n) : T(n));
     run(K_HESS_A, [&] {
+      using HA = HessShape<T, N, 2>;
       auto ha = cg ? k_bh_hess_a<T, N, 2> : k_bh_hess_a<T, N, 1>;
-      hipLaunchKernelGGL(ha, dim3(S::NBLK, nb * M_), dim3(S::NT), 0, stream_,
+      hipLaunchKernelGGL(ha, dim3(HA::NBLK, nb * M_), dim3(HA::NT), 0, stream_,
                          (const cplx<T>*)O_.as<cplx<T>>(),
                          (const cplx<T>*)bh_gO_.as<cplx<T>>(),
                          (const cplx<T>*)bh_dO_.as<cplx<T>>(), int(nxo_),
@@ -2374,28 +2376,44 @@ class Engine : public EngineBase {
                          (const cplx<T>*)bh_gP_.as<cplx<T>>(), T(rho2),
                          (const cplx<T>*)bh_dP_.as<cplx<T>>(), (const cplx<T>*)q_.as<cplx<T>>(),
                          (const int2*)corner_.as<int2>(), ord, M_,
-                         (const cplx<T>*)psi_.as<cplx<T>>(), field, bh_l_.as<cplx<T>>(),
+                         (const cplx<T>*)psi_.as<cplx<T>>(), bh_l_.as<cplx<T>>(),
                          bh_pa_.as<double>(), (const cplx<T>*)tw_.as<cplx<T>>());
     });
     run(K_HESS_B, [&] {
-      if (angular_) pass_t<false>(bh_l_.as<cplx<T>>(), nb * M_, cg ? 2 : 1, field);
+      // A view's fields are adjacent planes: the column passes see nb·M·NF planes, at most
+      // 65535 per launch (grid y).
+      if (angular_) {
+        const int planes = nb * M_ * (cg ? 2 : 1);
+        for (int p0 = 0; p0 < planes; p0 += 65535)
+          pass_t<false>(bh_l_.as<cplx<T>>() + size_t(p0) * n, std::min(65535, planes - p0));
+      }
       with_modes([&](auto mm) {
         constexpr int MM = decltype(mm)::value;
         auto hb = angular_ ? (cg ? k_bh_hess_b<T, N, MM, true, 2> : k_bh_hess_b<T, N, MM, true, 1>)
                            : (cg ? k_bh_hess_b<T, N, MM, false, 2>
                                  : k_bh_hess_b<T, N, MM, false, 1>);
         hipLaunchKernelGGL(hb, dim3(SC::NBLK, nb), dim3(SC::NT), 0, stream_,
-                           (const cplx<T>*)bh_l_.as<cplx<T>>(), field,
+                           (const cplx<T>*)bh_l_.as<cplx<T>>(),
                            (const cplx<T>*)bh_keep_.as<cplx<T>>(), counts_.as<float>(), ord,
                            bh_pb_.as<double>(), (const cplx<T>*)tw_.as<cplx<T>>(), norm);
       });
     });
     run(K_OTHER, [&] {
+      // Two-level sums of the Hessian partials: at most kBhSumBlocks rows each, then the step.
+      const int cnt[2] = {nb * M_ * int(HessShape<T, N, 2>::NBLK), nb * int(SC::NBLK)};
+      const double* in[2] = {bh_pa_.as<double>(), bh_pb_.as<double>()};
+      int g[2];
+      for (int i = 0; i < 2; ++i) {
+        g[i] = std::min(kBhSumBlocks, (cnt[i] + THREADS - 1) / THREADS);
+        const int per = (cnt[i] + g[i] - 1) / g[i];
+        hipLaunchKernelGGL(k_sum_rows3, dim3(g[i]), dim3(THREADS), 0, stream_, in[i], cnt[i],
+                           per, bh_sum_.as<double>() + 3 * size_t(i) * kBhSumBlocks);
+      }
       hipLaunchKernelGGL(k_bh_step, dim3(1), dim3(THREADS), 0, stream_,
                          (const double*)bh_dots_.as<double>(), ndots,
-                         (const double*)bh_pa_.as<double>(), nb * M_ * S::NBLK,
-                         (const double*)bh_pb_.as<double>(), nb * SC::NBLK, int(cg), frac,
-                         bh_step_.as<double>(), bh_log_.as<double>() + 4 * size_t(slot));
+                         (const double*)bh_sum_.as<double>(), g[0],
+                         (const double*)bh_sum_.as<double>() + 3 * kBhSumBlocks, g[1], int(cg),
+                         frac, bh_step_.as<double>(), bh_log_.as<double>() + 4 * size_t(slot));
     });
     run(K_OBJ_UPDATE, [&] {
       const size_t all = no + np;
