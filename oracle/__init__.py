@@ -42,8 +42,10 @@ What pins each module (the test files named run on the CPU):
   `engine_bh_near.rs`: two epochs on the seeded far-field scenario with one batch of all frames
   (conjugate gradient) and with batches of 16 (`bh_full_*`, `bh_mini_*`: β, α, ⟨g, D⟩, H(D, D)
   at 1e-13, object changes at 1e-13, probe at 5e-12), the four near-field cases with both batch
-  sizes (`bh_near_*`); the gradient against central differences of the cost, the Hessian against
-  central differences of the gradient, its symmetry and bilinearity.
+  sizes (`bh_near_*`); refined PATCH offsets (`bh_pos_full_*`, `bh_pos_mini_*` at 3e-12); the
+  gradient against central differences of the cost, the Hessian against central differences of
+  the gradient (with and without offsets), its symmetry and bilinearity, zero offsets reducing
+  to the unmoved batch.
 - multislice.py — tests/test_oracle_multislice.py (`lsqml_multislice_*`: forward
   intensities, one step, 30 epochs; directions and tangent against central differences).
 - positions.py — tests/test_oracle_positions.py (`lsqml_near_pos_*`,

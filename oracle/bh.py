@@ -1,15 +1,20 @@
 """pty-chi's BH (the bilinear-Hessian method of Carlsson, Nikitin and Wendt) on the amplitude
-cost, one object slice, far- or near-field exit, no features: a batch's gradient and bilinear
-Hessian at one state (`BhBatch`) and the epochs (`Bh`): steepest descent on each batch, or with
-one batch of every frame conjugate gradient across epochs, each step's length from the Hessian.
+cost, one object slice, far- or near-field exit, with or without PATCH position offsets: a
+batch's gradient and bilinear Hessian at one state (`BhBatch`) and the epochs (`Bh`): steepest
+descent on each batch, or with one batch of every frame conjugate gradient across epochs, each
+step's length from the Hessian. Refined offsets join the variables (their gradient scaled by
+position_rho², their step 0.3·α·D_d).
 
 Test infrastructure (see oracle/__init__.py). Layout as oracle/ptycho.py: detector planes
 shifted, objects and probes natural. Follows ptyrs/src/engine/bh.rs.
 """
 
+import math
+
 import numpy as np
 
 from .base import random_batches, to_natural, to_shifted
+from .positions import POSITION_STEP, _freqs, ramp
 
 # pty-chi's offset on the modelled amplitude in the residual and the Hessian.
 EPS = 1e-8
@@ -27,16 +32,29 @@ def frames_of(views):
 
 class BhBatch:
     """The amplitude cost F = Σ_frames Σ_unmasked (A − √iobs)², A = √(Σ_m |U ψ_m|²), ψ_k =
-    P_k·O_k[win]·q, U the exit transform, of a batch of views (whole frames in order) at one
-    state, and its derivatives. Stores U ψ, A + EPS, √iobs (NaN where masked) and td = U ψ·(1 −
-    √iobs/(A + EPS)) (0 where masked), half the gradient of F over U ψ.
-    ptyrs/src/engine/bh.rs:130-272"""
+    W_k·O_k[win], U the exit transform, of a batch of views (whole frames in order) at one
+    state, and its derivatives. W_k is the probe mode times the plane phase, P·q, or under PATCH
+    offsets (one per frame of the batch, [dy, dx] object pixels) that product moved by its
+    frame's offset, W_b = DFT⁻¹(DFT(q·P)·exp(−2πi f·d_b)) (the copy conj(q)·W_b times q).
+    Stores U ψ, A + EPS, √iobs (NaN where masked) and td = U ψ·(1 − √iobs/(A + EPS)) (0 where
+    masked), half the gradient of F over U ψ. ptyrs/src/engine/bh.rs:130-272"""
 
-    def __init__(self, views, iobs, optics, obj, probe):
+    def __init__(self, views, iobs, optics, obj, probe, offsets=None, refine=False):
         self.views, self.optics = views, optics
         self.obj, self.probe = obj, probe
         self.frames, self.nmode = frames_of(views)
-        self.psi = optics.exit_forward(self._waves(obj, probe))
+        self.shape = probe.shape[-2:]
+        self.offsets = None if offsets is None else np.asarray(offsets, dtype=np.float64)
+        if self.offsets is not None and self.offsets.shape != (len(self.frames), 2):
+            raise ValueError("one offset per frame of the batch")
+        self.refine = refine and self.offsets is not None
+        if self.offsets is not None:
+            self.ramps = [ramp(self.shape, d) for d in self.offsets]
+            fy, fx = _freqs(self.shape)
+            self.f = (-2j * math.pi * fy, -2j * math.pi * fx)   # −2πi f_a
+            self.spec = np.fft.fft2(probe * optics.quad)
+        self.w = self._moved(probe)
+        self.psi = optics.exit_forward(self._exit(obj, self.w))
         nb = len(self.frames)
         blk = self.psi.reshape((nb, self.nmode) + self.psi.shape[1:])
         self.amp = np.sqrt((blk.real ** 2 + blk.imag ** 2).sum(axis=1)) + EPS
@@ -44,46 +62,115 @@ class BhBatch:
         w = np.where(np.isnan(self.root), 0.0, 1.0 - self.root / self.amp)
         self.td = (blk * w[:, None]).reshape(self.psi.shape)
 
-    def _waves(self, obj, probe, other=None):
-        """Shifted exit waves P_k·O_k[win]·q, or with `other` = (obj2, probe2) the bilinear
-        term P2_k·O_k[win] + P_k·O2_k[win] (times q)."""
-        ny, nx = self.probe.shape[-2:]
+    def _slot(self, k):
+        return k // self.nmode
+
+    def _moved(self, probe, terms=None):
+        """Per view: probe mode times q, moved by its frame's offset, or with `terms` = [(spec,
+        coefficient per frame)] the plane DFT⁻¹(Σ spec·c_b·exp(−2πi f·d_b)) (spec a per-mode
+        spectrum of q·X, c_b a per-frame multiplier of the frequency grid)."""
         q = self.optics.quad
-        out = np.empty((len(self.views), ny, nx), dtype=np.complex128)
-        for k, (_, _, p, o, oy, ox) in enumerate(self.views):
-            w = obj[o, oy:oy + ny, ox:ox + nx]
-            if other is None:
-                out[k] = to_shifted(probe[p] * w * q)
+        out = np.empty((len(self.views),) + tuple(self.shape), dtype=np.complex128)
+        if self.offsets is None:
+            for k, v in enumerate(self.views):
+                out[k] = probe[v[2]] * q
+            return out
+        spec = np.fft.fft2(probe * q) if terms is None else None
+        for k, v in enumerate(self.views):
+            b = self._slot(k)
+            if terms is None:
+                s = spec[v[2]]
             else:
-                w2 = other[0][o, oy:oy + ny, ox:ox + nx]
-                out[k] = to_shifted((other[1][p] * w + probe[p] * w2) * q)
+                s = sum(t[v[2]] * c(b) for t, c in terms)
+            out[k] = np.fft.ifft2(s * self.ramps[b])
+        return out
+
+    def _fd(self, d, b):
+        """Σ_a d[b][a]·(−2πi f_a): the first derivative's factor along a frame's offset change."""
+        return d[b][0] * self.f[0] + d[b][1] * self.f[1]
+
+    def _exit(self, obj, w, obj2=None, w2=None):
+        """Shifted exit-plane waves W_k·O_k[win], or with obj2, w2 the bilinear term W_k·O2_k +
+        W2_k·O_k."""
+        ny, nx = self.shape
+        out = np.empty((len(self.views), ny, nx), dtype=np.complex128)
+        for k, (_, _, _, o, oy, ox) in enumerate(self.views):
+            win = obj[o, oy:oy + ny, ox:ox + nx]
+            if obj2 is None:
+                out[k] = to_shifted(w[k] * win)
+            else:
+                out[k] = to_shifted(w[k] * obj2[o, oy:oy + ny, ox:ox + nx] + w2[k] * win)
         return out
 
     def grads(self):
-        """(2·∂F/∂O*, 2·∂F/∂P*): U^H(2·td) back-projected, the object's onto the windows
-        through conj(P·q), the probe's summed over the views through conj(O[win]·q).
-        ptyrs/src/engine/bh.rs:274-343"""
+        """(2·∂F/∂O*, 2·∂F/∂P*, ∂F/∂d): χ = Uᴴ(2·td) back-projected, the object's onto the
+        windows through conj(W), the probe's through each frame's copy, conj(q)·DFT⁻¹(DFT(conj(O)·
+        χ)·exp(+2πi f·d_b)) summed over the views, and with refined offsets frame b's Re Σ_views
+        conj(χ)·O·∂_a W_b (∂_a W_b = DFT⁻¹(DFT(q·P)·(−2πi f_a)·exp(−2πi f·d_b))); d is None
+        otherwise. ptyrs/src/engine/bh.rs:274-343"""
         chi = self.optics.exit_adjoint(2.0 * self.td)
-        ny, nx = self.probe.shape[-2:]
+        ny, nx = self.shape
         q = self.optics.quad
         g_obj = np.zeros_like(self.obj)
         g_probe = np.zeros_like(self.probe)
+        g_d = np.zeros((len(self.frames), 2)) if self.refine else None
         for k, (_, _, p, o, oy, ox) in enumerate(self.views):
             c = to_natural(chi[k])
-            g_obj[o, oy:oy + ny, ox:ox + nx] += c * np.conj(self.probe[p] * q)
-            g_probe[p] += c * np.conj(self.obj[o, oy:oy + ny, ox:ox + nx] * q)
-        return g_obj, g_probe
+            win = self.obj[o, oy:oy + ny, ox:ox + nx]
+            g_obj[o, oy:oy + ny, ox:ox + nx] += c * np.conj(self.w[k])
+            if self.offsets is None:
+                g_probe[p] += c * np.conj(win * q)
+                continue
+            b = self._slot(k)
+            s = np.fft.fft2(np.conj(win) * c)
+            g_probe[p] += np.conj(q) * np.fft.ifft2(s * np.conj(self.ramps[b]))
+            if self.refine:
+                for a in range(2):
+                    dw = np.fft.ifft2(self.spec[p] * self.f[a] * self.ramps[b])
+                    g_d[b, a] += float((np.conj(c) * win * dw).real.sum())
+        return g_obj, g_probe, g_d
+
+    def _lift(self, u):
+        """Per view the moved waves' first-order change along u: W(u_P) + Σ_a u_da·∂_a W."""
+        if self.offsets is None:
+            return self._moved(u[1])
+        su = np.fft.fft2(u[1] * self.optics.quad)
+        terms = [(su, lambda b: 1.0)]
+        if self.refine:
+            terms.append((self.spec, lambda b: self._fd(u[2], b)))
+        return self._moved(None, terms)
+
+    def _second(self, u, v):
+        """Per view the moved waves' second-order change along u and v: Σ_a (v_da·∂_a W(u_P) +
+        u_da·∂_a W(v_P)) + Σ_ac u_da·v_dc·∂_a ∂_c W; None without refined offsets."""
+        if not self.refine:
+            return None
+        q = self.optics.quad
+        su, sv = np.fft.fft2(u[1] * q), np.fft.fft2(v[1] * q)
+        return self._moved(None, [(su, lambda b: self._fd(v[2], b)),
+                                  (sv, lambda b: self._fd(u[2], b)),
+                                  (self.spec, lambda b: self._fd(u[2], b) * self._fd(v[2], b))])
 
     def hessian(self, u, v):
-        """The second derivative of F along the displacements u = (u_O, u_P) and v, pty-chi's
-        bilinear Hessian with A + EPS in place of A: Σ 2·Re(conj(td)·L_uv) + 2·Σ_unmasked
-        [(1 − d/A)·Σ_m Re(conj(L_u)·L_v) + d·c_u·c_v/A³], L_u = U(u_O·P + O·u_P)·q, L_uv =
-        U(u_O·v_P + v_O·u_P)·q, c_u = Σ_m Re(conj(U ψ)·L_u), d = √iobs.
-        ptyrs/src/engine/bh.rs:497-577"""
+        """The second derivative of F along the displacements u = (u_O, u_P[, u_d]) and v,
+        pty-chi's bilinear Hessian with A + EPS in place of A: Σ 2·Re(conj(td)·L_uv) +
+        2·Σ_unmasked [(1 − d/A)·Σ_m Re(conj(L_u)·L_v) + d·c_u·c_v/A³], L_u = U(u_O·W + O·W_u),
+        W_u the waves' first-order change, L_uv = U(u_O·W_v + v_O·W_u + O·W_uv), W_uv their
+        second-order change, c_u = Σ_m Re(conj(U ψ)·L_u), d = √iobs.
+        ptyrs/src/engine/bh.rs:437-577"""
         fwd = self.optics.exit_forward
-        lu = fwd(self._waves(self.obj, self.probe, u))
-        lv = fwd(self._waves(self.obj, self.probe, v))
-        luv = fwd(self._waves(u[0], u[1], v))
+        wu, wv = self._lift(u), self._lift(v)
+        lu = fwd(self._exit(self.obj, self.w, u[0], wu))
+        lv = fwd(self._exit(self.obj, self.w, v[0], wv))
+        ny, nx = self.shape
+        luv = np.empty((len(self.views), ny, nx), dtype=np.complex128)
+        w2 = self._second(u, v)
+        for k, (_, _, _, o, oy, ox) in enumerate(self.views):
+            t = u[0][o, oy:oy + ny, ox:ox + nx] * wv[k] + v[0][o, oy:oy + ny, ox:ox + nx] * wu[k]
+            if w2 is not None:
+                t = t + self.obj[o, oy:oy + ny, ox:ox + nx] * w2[k]
+            luv[k] = to_shifted(t)
+        luv = fwd(luv)
         h = float(2.0 * (np.conj(self.td) * luv).real.sum())
         nb = len(self.frames)
         sh = (nb, self.nmode) + self.psi.shape[1:]
@@ -99,31 +186,43 @@ class BhBatch:
 
 
 def dot(a, b):
-    """Σ Re(conj(a)·b) over (object, probe) pairs. ptyrs/src/engine/bh.rs:580-583"""
-    return float(sum((np.conj(x) * y).real.sum() for x, y in zip(a, b)))
+    """Σ Re(conj(a)·b) over the object and probe parts, plus Σ a_d·b_d over the offsets' part
+    where both have one. ptyrs/src/engine/bh.rs:79-90, 580-583"""
+    s = float(sum((np.conj(x) * y).real.sum() for x, y in zip(a[:2], b[:2])))
+    if len(a) > 2 and len(b) > 2 and a[2] is not None and b[2] is not None:
+        s += float((np.asarray(a[2]) * np.asarray(b[2])).sum())
+    return s
 
 
 class Bh:
-    """BH across epochs: batch size, probe ρ, seed and, with one batch of every frame, the
-    previous direction. ptyrs/src/engine/bh.rs:585-799"""
+    """BH across epochs: batch size, probe ρ, position ρ, seed, the PATCH offsets of every frame
+    (refined or held) and, with one batch of every frame, the previous direction (its offsets'
+    part by frame). ptyrs/src/engine/bh.rs:585-799"""
 
-    def __init__(self, batch_size, probe_rho=0.01, seed=7):
+    def __init__(self, batch_size, probe_rho=0.01, seed=7, position_rho=0.1, offsets=None,
+                 refine=True):
         if batch_size < 1:
             raise ValueError("BH batch_size must be >= 1")
         self.batch_size, self.probe_rho, self.seed = batch_size, probe_rho, seed
+        self.position_rho = position_rho
+        self.offsets = None if offsets is None else np.array(offsets, dtype=np.float64)
+        self.refine = refine and self.offsets is not None
         self.epochs = 0
         self.prev = None
 
     def epoch(self, addr, iobs, optics, obj, probe):
         """One epoch over random_batches(nframe, size, seed, epoch): per batch g = grads, Pg =
-        g with the probe part times ρ² = probe_rho²·(the exit's intensity scale), the
-        direction D = −Pg, or with one batch of every frame after the first epoch D = −Pg +
-        β·D′, β = H(Pg, D′)/H(D′, D′); α = −⟨g, D⟩/H(D, D)·count/size; O += α·D_O, P += α·D_P.
-        Returns (obj, probe, steps), steps (frames, β, α, top, bottom) per batch."""
+        g with the probe part times ρ² = probe_rho²·(the exit's intensity scale) and the
+        offsets' part times position_rho², the direction D = −Pg, or with one batch of every
+        frame after the first epoch D = −Pg + β·D′, β = H(Pg, D′)/H(D′, D′); α =
+        −⟨g, D⟩/H(D, D)·count/size; O += α·D_O, P += α·D_P and refined offsets += 0.3·α·D_d.
+        Returns (obj, probe, steps), steps (frames, β, α, top, bottom) per batch; the offsets
+        are in self.offsets."""
         nframe, nmode = addr.nframe, addr.nmode
         size = min(self.batch_size, nframe)
         full = size == nframe
         rho2 = self.probe_rho ** 2 * optics.propagator.intensity_scale()
+        pos2 = self.position_rho ** 2
         views_all = addr.views()
         epoch = self.epochs
         self.epochs += 1
@@ -131,21 +230,31 @@ class Bh:
         obj, probe = obj.copy(), probe.copy()
         for frames in random_batches(nframe, size, self.seed, epoch):
             views = [v for f in frames for v in views_all[f * nmode:(f + 1) * nmode]]
-            b = BhBatch(views, iobs, optics, obj, probe)
+            offs = None if self.offsets is None else self.offsets[frames]
+            b = BhBatch(views, iobs, optics, obj, probe, offs, self.refine)
             g = b.grads()
-            pg = (g[0], g[1] * rho2)
+            pg = (g[0], g[1] * rho2, None if g[2] is None else g[2] * pos2)
             prev, self.prev = self.prev, None
             if prev is not None and full:
-                beta = b.hessian(pg, prev) / b.hessian(prev, prev)
-                d = (beta * prev[0] - pg[0], beta * prev[1] - pg[1])
+                pd = None if prev[2] is None else prev[2][frames]
+                pv = (prev[0], prev[1], pd)
+                beta = b.hessian(pg, pv) / b.hessian(pv, pv)
+                d = (beta * pv[0] - pg[0], beta * pv[1] - pg[1],
+                     None if pd is None else beta * pd - pg[2])
             else:
-                beta, d = 0.0, (-pg[0], -pg[1])
+                beta = 0.0
+                d = (-pg[0], -pg[1], None if pg[2] is None else -pg[2])
             top = -dot(g, d)
             bottom = b.hessian(d, d)
             alpha = top / bottom * len(frames) / size
             obj = obj + alpha * d[0]
             probe = probe + alpha * d[1]
+            by_frame = None
+            if self.refine:
+                by_frame = np.zeros((nframe, 2))
+                by_frame[frames] = d[2]
+                self.offsets[frames] += POSITION_STEP * alpha * d[2]
             if full:
-                self.prev = d
+                self.prev = (d[0], d[1], by_frame)
             steps.append((frames, beta, alpha, top, bottom))
         return obj, probe, steps

@@ -168,3 +168,88 @@ def test_near_field_epochs_match_fixtures(case, method, batch_size):
         d = (now - start).ravel()
         assert max_rel(np.stack([d.real, d.imag], axis=1).ravel(),
                        rows[:, lead:lead + 2].ravel()) < 5e-12
+
+
+@pytest.mark.parametrize("batch_size,name", [(40, "bh_pos_full"), (16, "bh_pos_mini")])
+def test_patch_offsets_match_fixtures(batch_size, name):
+    # Refined PATCH offsets (the reference's model of its steps, refs/f64/bh_pos.py; probe ρ
+    # 0.01, position ρ 2, seed 7, two epochs from its offsets): steps, the object and probe
+    # changes over the run relative to the start's largest value, the offsets' changes relative
+    # to their largest, all to 3e-12 (ptyrs/tests/engine_bh_pos.rs:61-143).
+    p = state()
+    rows = np.array(table(f"{name}_offsets_f64.txt"))
+    d0 = rows[:, 1:3].copy()
+    bh = Bh(batch_size, probe_rho=0.01, seed=7, position_rho=2.0, offsets=d0, refine=True)
+    obj, probe, got = p.obj, p.probe, []
+    for e in range(2):
+        obj, probe, s = bh.epoch(p.addr, p.iobs, p.optics, obj, probe)
+        got += [(e, x) for x in s]
+    want = table(f"{name}_steps_f64.txt")
+    assert len(got) == len(want)
+    for k, ((epoch, (frames, beta, alpha, top, bottom)), r) in enumerate(zip(got, want)):
+        assert [k, epoch, len(frames)] == [int(r[0]), int(r[1]), int(r[2])]
+        assert frames == [int(f) for f in r[7:]]
+        if r[3] == 0.0:
+            assert beta == 0.0
+        for got_v, w in ((beta, r[3]), (alpha, r[4]), (top, r[5]), (bottom, r[6])):
+            if w != 0.0:
+                assert abs(got_v - w) < 3e-12 * abs(w), (k, got_v, w)
+    for now, start, fname, lead in ((obj, p.obj, f"{name}_obj_f64.txt", 2),
+                                    (probe, p.probe, f"{name}_probe_f64.txt", 3)):
+        r = np.array(table(fname))
+        d = (now - start).ravel()
+        got_c = np.stack([d.real, d.imag], axis=1).ravel()
+        want_c = r[:, lead:lead + 2].ravel()
+        assert np.abs(got_c - want_c).max() < 3e-12 * np.abs(start).max()
+    dd = (bh.offsets - d0).ravel()
+    want_d = (rows[:, 3:5] - d0).ravel()
+    assert np.abs(dd - want_d).max() < 3e-12 * np.abs(want_d).max()
+
+
+def test_patch_offsets_gradient_and_hessian_match_central_differences():
+    # The offsets' gradient and the Hessian over object, probe and offsets against central
+    # differences of the cost and of the gradient (ptyrs/src/engine/bh.rs:960-1010), held and
+    # refined offsets; at zero offsets the batch is the unmoved one.
+    p = state()
+    nb = 8
+    views = p.addr.views()[:nb * p.addr.nmode]
+    rng = np.random.default_rng(11)
+    d0 = rng.uniform(-0.4, 0.4, (nb, 2))
+
+    def cost(obj, probe, d):
+        b = BhBatch(views, p.iobs, p.optics, obj, probe, d, False)
+        keep = ~np.isnan(b.root)
+        a = b.amp - 1e-8
+        return float((((a - b.root) ** 2)[keep]).sum() +
+                     (2e-8 * b.root * np.log(b.amp))[keep].sum())
+
+    b = BhBatch(views, p.iobs, p.optics, p.obj, p.probe, d0, True)
+    g = b.grads()
+    w = (_direction(p.obj, rng), _direction(p.probe, rng), rng.uniform(-1, 1, (nb, 2)))
+    h = 1e-5
+    fd = (cost(p.obj + h * w[0], p.probe + h * w[1], d0 + h * w[2]) -
+          cost(p.obj - h * w[0], p.probe - h * w[1], d0 - h * w[2])) / (2 * h)
+    assert abs(dot(g, w) - fd) < 1e-7 * abs(fd)
+    u = (_direction(p.obj, rng), _direction(p.probe, rng), rng.uniform(-1, 1, (nb, 2)))
+    v = (_direction(p.obj, rng), _direction(p.probe, rng), rng.uniform(-1, 1, (nb, 2)))
+    h_uv, h_vu = b.hessian(u, v), b.hessian(v, u)
+    assert abs(h_uv - h_vu) < 1e-12 * abs(h_uv)
+
+    def du(t):
+        bt = BhBatch(views, p.iobs, p.optics, p.obj + t * v[0], p.probe + t * v[1],
+                     d0 + t * v[2], True)
+        return dot(bt.grads(), u)
+
+    t = 1e-6
+    fd = (du(t) - du(-t)) / (2 * t)
+    assert abs(h_uv - fd) < 1e-6 * abs(fd)
+    # held offsets: no offsets' part, the same waves
+    bh_held = BhBatch(views, p.iobs, p.optics, p.obj, p.probe, d0, False)
+    assert bh_held.grads()[2] is None
+    assert np.allclose(bh_held.psi, b.psi, rtol=0, atol=0)
+    zero = BhBatch(views, p.iobs, p.optics, p.obj, p.probe, np.zeros((nb, 2)), False)
+    plain = BhBatch(views, p.iobs, p.optics, p.obj, p.probe)
+    assert np.abs(zero.psi - plain.psi).max() < 1e-12 * np.abs(plain.psi).max()
+    gz, gp = zero.grads(), plain.grads()
+    for x, y in zip(gz[:2], gp[:2]):
+        assert np.abs(x - y).max() < 1e-12 * np.abs(y).max()
