@@ -249,8 +249,8 @@ int ptyrs_engine_lsqml_opr(ptyrs_engine* e, const ptyrs_lsqml_params* params, si
 /* BhParams (ptyrs/src/engine/bh.rs:28-58): frames per batch (capped at the scan's; one batch of
  * every frame runs conjugate gradient across the epochs of a call, smaller batches steepest
  * descent), probe_rho (the probe's gradient is scaled by probe_rho²·k, k the exit's intensity
- * scale: ny·nx in the far field), position_rho (pty-chi's offset ρ; offsets are not modelled by
- * the GPU BH), seed of every epoch's random_batches. */
+ * scale: ny·nx in the far field), position_rho (refined position offsets' gradient is scaled by
+ * position_rho², their step is 0.3·α·D_d), seed of every epoch's random_batches. */
 typedef struct {
   size_t batch_size;
   double probe_rho;
@@ -269,7 +269,8 @@ typedef struct {
 
 /* GpuPtycho::bh (ptyrs/src/engine/gpu/bh.rs:33-68): nepoch epochs of pty-chi's BH on the
  * amplitude cost from a fresh BH (epoch indices 0.. of the call), an LLK record every calc_llk
- * epochs and after the last. One object slice and mode, no position offsets (PTYRS_EINVAL). */
+ * epochs and after the last. One object slice and mode; position offsets under PATCH sampling,
+ * refined or held (under FullObject PTYRS_EINVAL). */
 int ptyrs_engine_bh(ptyrs_engine* e, const ptyrs_bh_params* params, size_t nepoch,
                     size_t calc_llk);
 
@@ -305,7 +306,8 @@ int ptyrs_engine_launch(ptyrs_engine* e, ptyrs_launch* out);
  * ptyrs/src/engine/features.rs:304-330): offsets [nframe][2] (dy, dx) in object pixels,
  * len = 2·nframe, refined by every rPIE batch, LSQML step or AP, DM, RAAR or ML cycle when
  * refine != 0, else held; offsets = NULL stops modelling them. Offsets are kept in FP64 on the
- * device. Under PATCH sampling every algorithm and the LLK pass run with offsets (AP, DM and
+ * device (BH refines them too). Under PATCH sampling every algorithm and the LLK pass run with
+ * offsets (AP, DM and
  * RAAR through Shifted::psi_to_obj_probe, ptyrs/src/engine/features.rs:653-720: the probe's
  * first conjugate-gradient step, DM and RAAR with complex waves only); under FullObject
  * sampling rPIE, LSQML, the LLK pass and the model intensities, with one object slice and

@@ -166,6 +166,192 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_bh_hess_b(
   }
 }
 
+// ---- BH under PATCH position offsets (ptyrs/src/engine/bh.rs:180-577) ---------------------------
+// Frame b's probe copy is conj(q)·W_b with W_b = DFT⁻¹(DFT(q·P)·exp(−2πi f·d_b)), so ψ = W_b·O. In
+// these "wave units" the displacement u = (u_O, u_P, u_d) changes W_b by A_u = DFT⁻¹((S_u +
+// S_P·F_u)·e_b) and to second order (u, v) by B_uv = DFT⁻¹((S_u·F_v + S_v·F_u + S_P·F_u·F_v)·e_b),
+// with S_X = DFT(q·X), F_u = −2πi Σ_a u_da·f_a and e_b = exp(−2πi f·d_b).
+
+// g[v] = conj(O[win])·r[v] per view of a batch (slot-local, natural layout). Grid
+// (ceil(n/256), nb·M).
+template <typename T>
+__global__ void k_bh_conj_o(cplx<T>* __restrict__ g, const cplx<T>* __restrict__ r,
+                            const cplx<T>* __restrict__ O, int nxo, const int2* __restrict__ corner,
+                            const int* __restrict__ order, int N, int M) {
+  const int n = N * N;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int v = blockIdx.y;
+  const int2 c = corner[order[v / M]];
+  const cplx<T> o = ld(O + (size_t)(c.x + j / N) * nxo + c.y + j % N);
+  st(g + (size_t)v * n + j, cmulc(o, ld(r + (size_t)v * n + j)));
+}
+
+// The offsets' gradient of each frame of a batch, Re Σ_views conj(r)·O·∂_a W_b, from G =
+// DFT(conj(O)·r) (unnormalised, [slot·M + m]) by Parseval: Re Σ_m Σ_k conj(G)·S_P[m]·(−2πi f_a)·
+// e_b / n into gd[b·2 + a] (FP64). One workgroup per frame. ptyrs/src/engine/bh.rs:290-317
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_bh_pos_grad(const cplx<T>* __restrict__ G,
+                                                         const cplx<T>* __restrict__ spec,
+                                                         const double2* __restrict__ offs,
+                                                         const int* __restrict__ order, int N,
+                                                         int M, double* __restrict__ gd) {
+  __shared__ double red[THREADS];
+  const int n = N * N, b = blockIdx.x;
+  const double2 d = offs[order[b]];
+  double acc[2] = {0, 0};
+  for (int m = 0; m < M; ++m)
+    for (int j = threadIdx.x; j < n; j += THREADS) {
+      const double fy = dft_freq(j / N, N), fx = dft_freq(j % N, N);
+      double sn, cs;
+      sincos(-2.0 * M_PI * (fy * d.x + fx * d.y), &sn, &cs);
+      const cplx<T> g = ld(G + ((size_t)b * M + m) * n + j), p = ld(spec + (size_t)m * n + j);
+      // t = conj(g)·p·e_b; Re(t·(−2πi f_a)) = 2π f_a·Im(t)
+      const double pr = double(p.x) * cs - double(p.y) * sn, pi = double(p.x) * sn + double(p.y) * cs;
+      const double ti = double(g.x) * pi - double(g.y) * pr;
+      acc[0] += 2.0 * M_PI * fy * ti;
+      acc[1] += 2.0 * M_PI * fx * ti;
+    }
+  for (int a = 0; a < 2; ++a) {
+    const double t = block_reduce<THREADS>(acc[a], red, AddOp());
+    if (threadIdx.x == 0) gd[2 * (size_t)b + a] = t / double(n);
+  }
+}
+
+// The spectra of a batch's wave changes along u and v, NP planes per view ([view·NP + p]):
+// A_u, B_uu (NP = 2) or A_u, A_v, B_uu, B_uv, B_vv (NP = 5), each times `scale`; S_u is the
+// spectrum of q·g_P and u_P = su·g_P, u_d = pos2·gd[slot] (gd null: 0), v_d = ddf[frame] (ddf
+// null: 0). Grid (ceil(n/256), nb·M).
+template <typename T, int NP>
+__global__ void k_bh_moved_planes(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ sp,
+                                  const cplx<T>* __restrict__ su_spec, double su,
+                                  const cplx<T>* __restrict__ sv_spec,
+                                  const double2* __restrict__ offs, const int* __restrict__ order,
+                                  const double* __restrict__ gd, double pos2,
+                                  const double* __restrict__ ddf, int N, int M, double scale) {
+  const int n = N * N;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int v = blockIdx.y, b = v / M, m = v % M, f = order[b];
+  const double2 d = offs[f];
+  const double fy = dft_freq(j / N, N), fx = dft_freq(j % N, N);
+  double sn, cs;
+  sincos(-2.0 * M_PI * (fy * d.x + fx * d.y), &sn, &cs);
+  // F = i·φ, φ = −2π Σ_a d_a·f_a
+  const double pu = gd ? -2.0 * M_PI * pos2 * (gd[2 * b] * fy + gd[2 * b + 1] * fx) : 0.0;
+  const double pv = ddf ? -2.0 * M_PI * (ddf[2 * f] * fy + ddf[2 * f + 1] * fx) : 0.0;
+  const size_t at = (size_t)m * n + j;
+  const cplx<T> P = ld(sp + at), U = ld(su_spec + at);
+  const double Pr = P.x, Pi = P.y, Ur = su * U.x, Ui = su * U.y;
+  // x·e_b·scale for x = (re, im)
+  auto put = [&](int p, double re, double im) {
+    st(out + ((size_t)v * NP + p) * n + j,
+       cplx<T>{T((re * cs - im * sn) * scale), T((re * sn + im * cs) * scale)});
+  };
+  // A_u = U + i·pu·P; B_uu = 2i·pu·U − pu²·P
+  put(0, Ur - pu * Pi, Ui + pu * Pr);
+  if constexpr (NP == 2) {
+    put(1, -2.0 * pu * Ui - pu * pu * Pr, 2.0 * pu * Ur - pu * pu * Pi);
+  } else {
+    const cplx<T> V = ld(sv_spec + at);
+    const double Vr = V.x, Vi = V.y;
+    put(1, Vr - pv * Pi, Vi + pv * Pr);                                         // A_v
+    put(2, -2.0 * pu * Ui - pu * pu * Pr, 2.0 * pu * Ur - pu * pu * Pi);        // B_uu
+    // B_uv = i·pv·U + i·pu·V − pu·pv·P
+    put(3, -pv * Ui - pu * Vi - pu * pv * Pr, pv * Ur + pu * Vr - pu * pv * Pi);
+    put(4, -2.0 * pv * Vi - pv * pv * Pr, 2.0 * pv * Vr - pv * pv * Pi);        // B_vv
+  }
+}
+
+// Pass A of the Hessian sums under offsets: as k_bh_hess_a with every view's own waves: W from
+// w ([view][n]), the changes A_u, A_v and B_uu, B_uv, B_vv from pl ([view·NP + p][n], natural):
+// L_u = W·u_O + A_u·O, L_v = W·v_O + A_v·O; second-order sums Re Σ conj(r)·(2·u_O·A_u + O·B_uu),
+// Re Σ conj(r)·(u_O·A_v + v_O·A_u + O·B_uv), Re Σ conj(r)·(2·v_O·A_v + O·B_vv).
+template <typename T, int N, int NF>
+__global__ __launch_bounds__((HessShape<T, N, NF>::NT)) void k_bh_hess_a_moved(
+    const cplx<T>* __restrict__ O, const cplx<T>* __restrict__ uO,
+    const cplx<T>* __restrict__ vO, int nxo, const cplx<T>* __restrict__ w,
+    const cplx<T>* __restrict__ pl, const int2* __restrict__ corner,
+    const int* __restrict__ order, int M, const cplx<T>* __restrict__ r,
+    cplx<T>* __restrict__ lf, double* __restrict__ part, const cplx<T>* __restrict__ tw) {
+  using S = HessShape<T, N, NF>;
+  constexpr int E = S::E, TL = S::TL;
+  constexpr int n = N * N;
+  constexpr int NP = NF == 2 ? 5 : 2;
+  __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
+  __shared__ cplx<T> stw[N];
+  __shared__ double red[S::NT];
+  const int tid = threadIdx.x, l = tid / TL, jl = tid % TL;
+  const int y = blockIdx.x * S::L + l;
+  const int v = blockIdx.y, b = v / M;
+  stage_twiddles<T, N, S::NT>(stw, tw);
+  const int2 c = corner[order[b]];
+  const size_t op = (size_t)(c.x + y) * nxo + c.y;
+  const size_t row = ((size_t)v * N + y) * N;
+  const size_t prow = ((size_t)v * NP * N + y) * N;
+  const size_t lrow = ((size_t)v * NF * N + y) * N;
+  cplx<T> lu[E], lv[E];
+  double s[3] = {0, 0, 0};
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int x = jl + TL * e;
+    const cplx<T> o = ld(O + op + x), ww = ld(w + row + x), rr = ld(r + row + x);
+    const cplx<T> ao = ld(uO + op + x), au = ld(pl + prow + x);
+    lu[e] = ww * ao + au * o;
+    if constexpr (NF == 1) {
+      const cplx<T> buu = ld(pl + prow + n + x);
+      s[0] += double(cmulc(rr, ao * au * T(2) + o * buu).x);
+    } else {
+      const cplx<T> av = ld(pl + prow + n + x), bo = ld(vO + op + x);
+      const cplx<T> buu = ld(pl + prow + 2 * n + x), buv = ld(pl + prow + 3 * n + x),
+                    bvv = ld(pl + prow + 4 * n + x);
+      lv[e] = ww * bo + av * o;
+      s[0] += double(cmulc(rr, ao * au * T(2) + o * buu).x);
+      s[1] += double(cmulc(rr, ao * av + bo * au + o * buv).x);
+      s[2] += double(cmulc(rr, bo * av * T(2) + o * bvv).x);
+    }
+  }
+  T* re = lre + S::base_row(l);
+  T* im = lim + S::base_row(l);
+  line_fft<T, N, E, false, typename S::AtRow>(lu, jl, re, im, stw);
+  if constexpr (NF == 2) line_fft<T, N, E, false, typename S::AtRow>(lv, jl, re, im, stw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    st(lf + lrow + jl + TL * e, lu[e]);
+    if constexpr (NF == 2) st(lf + lrow + n + jl + TL * e, lv[e]);
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double t = block_reduce<S::NT>(s[i], red, AddOp());
+    if (tid == 0) part[((size_t)v * gridDim.x + blockIdx.x) * 3 + i] = t;
+  }
+}
+
+// The refined offsets' step of a batch: D_d = β·D′_d[frame] (cg) − pos2·gd[slot]; offsets +=
+// 0.3·α·D_d; with `keep` D′_d[frame] = D_d. ptyrs/src/engine/bh.rs:778-787
+__global__ static void k_bh_update_pos(double2* __restrict__ offs, double* __restrict__ ddf,
+                                       const double* __restrict__ gd,
+                                       const int* __restrict__ order, int nb, double pos2,
+                                       const double* __restrict__ step, int cg, int keep) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int f = order[b];
+  const double alpha = step[0], beta = step[1];
+  double dy = -pos2 * gd[2 * b], dx = -pos2 * gd[2 * b + 1];
+  if (cg) {
+    dy += beta * ddf[2 * f];
+    dx += beta * ddf[2 * f + 1];
+  }
+  double2 o = offs[f];
+  o.x += 0.3 * alpha * dy;
+  o.y += 0.3 * alpha * dx;
+  offs[f] = o;
+  if (keep) {
+    ddf[2 * f] = dy;
+    ddf[2 * f + 1] = dx;
+  }
+}
+
 // Per workgroup over the object (no values) and the probe (np): Σ|g_O|² + ρ²·Σ|g_P|² = ⟨g, Pg⟩
 // and (cg) Σ Re(conj(g)·D′) = ⟨g, D′⟩ into part[blk·2 + {0, 1}] (FP64).
 template <typename T>
@@ -216,15 +402,27 @@ __global__ __launch_bounds__(THREADS) static void k_sum_rows3(const double* __re
 // H(u, v)/H(v, v) with cg, else 0; top = ⟨g, Pg⟩ − β·⟨g, D′⟩ (= −⟨g, D⟩ for D = β·D′ − Pg);
 // bottom = H(D, D) = β²·H(v, v) − 2β·H(u, v) + H(u, u); α = top/bottom·count/size. step =
 // {α, β}; log[0..4) = {β, α, top, bottom}. ptyrs/src/engine/bh.rs:729-760
+// Refined offsets (npos = the batch's frames, gd their gradient, ddf D′_d by frame) add
+// pos2·Σ|g_d|² and Σ g_d·D′_d to the two dots.
 __global__ __launch_bounds__(THREADS) static void k_bh_step(
     const double* __restrict__ dots, int ndots, const double* __restrict__ pa, int npa,
     const double* __restrict__ pb, int npb, int cg, double frac, double* __restrict__ step,
-    double* __restrict__ log) {
+    double* __restrict__ log, const double* __restrict__ gd = nullptr,
+    const double* __restrict__ ddf = nullptr, const int* __restrict__ order = nullptr,
+    int npos = 0, double pos2 = 0.0) {
   __shared__ double red[THREADS];
   double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = threadIdx.x; i < ndots; i += THREADS) {
     s[0] += dots[2 * (size_t)i];
     s[1] += dots[2 * (size_t)i + 1];
+  }
+  for (int b = threadIdx.x; b < npos; b += THREADS) {
+    const double gy = gd[2 * b], gx = gd[2 * b + 1];
+    s[0] += pos2 * (gy * gy + gx * gx);
+    if (cg) {
+      const int f = order[b];
+      s[1] += gy * ddf[2 * f] + gx * ddf[2 * f + 1];
+    }
   }
   for (int i = threadIdx.x; i < npa; i += THREADS)
     for (int k = 0; k < 3; ++k) s[2 + k] += pa[3 * (size_t)i + k];

@@ -666,7 +666,9 @@ class Engine : public EngineBase {
     if (!std::isfinite(p.probe_rho) || !std::isfinite(p.position_rho))
       throw Invalid("BH probe_rho and position_rho must be finite");
     if (nslice_ > 1) throw Invalid("BH runs one object slice on this GPU engine");
-    if (pos_on_) throw Invalid("BH with position offsets is not run on this GPU engine");
+    if (full_object())
+      throw Invalid("BH with position offsets under FullObject sampling is not run on this GPU "
+                    "engine; set the PATCH sampling");
     if (nobj_ != 1) throw Invalid("the GPU BH runs one object mode");
     if (M_ > kMaxModes) throw Invalid(modes_error());
     if (nepoch == 0 || nframe_ == 0) {
@@ -692,20 +694,30 @@ class Engine : public EngineBase {
                              hipMemcpyHostToDevice, stream_));
     probe_terms();
     const double rho2 = p.probe_rho * p.probe_rho * scale_;
+    const double pos2 = p.position_rho * p.position_rho;
+    if (pos_on_) ensure_bh_moved(size, full);
     std::vector<std::pair<size_t, int>> due;
     for (size_t ic = 0; ic < nepoch; ++ic) {
       PTYRS_HIP(hipMemcpyAsync(eord_.p, order_.as<int>() + ic * nframe_, sizeof(int) * nframe_,
                                hipMemcpyDeviceToDevice, stream_));
       const int* ord = eord_.as<int>();
       const bool cg = full && ic > 0;
-      std::vector<uint64_t> key = {uint64_t(cg), uint64_t(full), cap_, bits_of(rho2)};
-      for (auto& s : spans[ic]) key.push_back(uint64_t(s.second));
-      epoch(bh_graph_, key, [&] {
+      if (pos_on_) {
+        // Under offsets the batches launch eagerly, as the other position paths.
         int k = 0;
         for (auto& s : spans[ic])
-          bh_batch(rho2, ord + s.first, s.second, double(s.second) / double(size), cg, full,
-                   k++);
-      });
+          bh_batch_moved(rho2, pos2, ord + s.first, s.second, double(s.second) / double(size),
+                         cg, full, k++);
+      } else {
+        std::vector<uint64_t> key = {uint64_t(cg), uint64_t(full), cap_, bits_of(rho2)};
+        for (auto& s : spans[ic]) key.push_back(uint64_t(s.second));
+        epoch(bh_graph_, key, [&] {
+          int k = 0;
+          for (auto& s : spans[ic])
+            bh_batch(rho2, ord + s.first, s.second, double(s.second) / double(size), cg, full,
+                     k++);
+        });
+      }
       const bool rec = calc_llk > 0 && (ic % calc_llk == 0 || ic + 1 == nepoch);
       if (rec) {
         if (due.size() == hist_slots_) drain(due, PTYRS_BH);
@@ -2312,6 +2324,200 @@ class Engine : public EngineBase {
     bh_step_.ensure(sizeof(double) * 2);
     bh_log_.ensure(sizeof(double) * 4 * ((nframe_ + cap - 1) / cap));
   }
+  // BH under PATCH offsets: the batch's moved waves, the spectra of conj(O)·r per view, the
+  // spectra of q·g_P and q·D′_P, the wave changes (NP planes per view), the offsets' gradient
+  // per slot and the previous direction's offsets by frame.
+  DevBuf bh_w_, bh_gg_, bh_su_, bh_sv_, bh_pl_, bh_gd_, bh_ddf_;
+  void ensure_bh_moved(size_t cap, bool full) {
+    const size_t views = cap_ * M_;
+    for (DevBuf* b : {&bh_w_, &bh_gg_}) b->ensure(sizeof(cplx<T>) * views * n);
+    for (DevBuf* b : {&bh_su_, &bh_sv_}) b->ensure(sizeof(cplx<T>) * M_ * n);
+    bh_pl_.ensure(sizeof(cplx<T>) * views * (full ? 5 : 2) * n);
+    bh_gd_.ensure(sizeof(double) * 2 * cap);
+    bh_ddf_.ensure(sizeof(double) * 2 * std::max<size_t>(1, nframe_));
+  }
+  // The unnormalised spectra of q·src (M planes) into dst.
+  void q_spectra(const cplx<T>* src, cplx<T>* dst) {
+    const int np = int(M_) * n;
+    hipLaunchKernelGGL(k_probe_terms<T>, dim3((np + 255) / 256), dim3(256), 0, stream_, src,
+                       q_.as<cplx<T>>(), dst, n, np);
+    hipLaunchKernelGGL((k_rows<T, N, false>), dim3(S::NBLK, M_), dim3(S::NT), 0, stream_, dst,
+                       tw_.as<cplx<T>>());
+    hipLaunchKernelGGL((k_cols<T, N, false>), dim3(SC::NBLK, M_), dim3(SC::NT), 0, stream_, dst,
+                       tw_.as<cplx<T>>());
+  }
+  // Inverse transforms of `planes` planes from p, at most 65535 per launch (grid y).
+  void inverse_planes(cplx<T>* p, size_t planes) {
+    for (size_t p0 = 0; p0 < planes; p0 += 65535) {
+      const unsigned k = unsigned(std::min<size_t>(65535, planes - p0));
+      hipLaunchKernelGGL((k_cols<T, N, true>), dim3(SC::NBLK, k), dim3(SC::NT), 0, stream_,
+                         p + p0 * n, tw_.as<cplx<T>>());
+      hipLaunchKernelGGL((k_rows<T, N, true>), dim3(S::NBLK, k), dim3(S::NT), 0, stream_,
+                         p + p0 * n, tw_.as<cplx<T>>());
+    }
+  }
+
+  // One BH batch under PATCH offsets (ptyrs/src/engine/bh.rs:130-577; the reference GPU's
+  // bh_batch with its moved probes, ptyrs/src/engine/gpu/bh.rs:426-600): the views' waves are
+  // the frames' moved probe terms W_b; the probe gradient moves each frame's conj(O)·r back in
+  // Fourier space; refined offsets get their gradient by Parseval against DFT(q·P), their
+  // first and second derivatives enter the wave changes along u and v, and their step follows
+  // the update.
+  void bh_batch_moved(double rho2, double pos2, const int* ord, int nb, double frac, bool cg,
+                      bool keep, int slot) {
+    const int nt = ntiles();
+    const size_t no = nyo_ * nxo_, np = nprobe_ * n;
+    const int total = int(np), K = nb * M_;
+    const bool refine = pos_refine_;
+    const double2* offs = offs_.as<double2>();
+    run(K_BIN, [&] {
+      hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                         corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
+                         lcount_.as<int>(), (const T*)nullptr, M_, int(nyo_), int(nxo_),
+                         (T*)nullptr, tilemax_.as<T>(), (const cplx<T>*)nullptr);
+    });
+    cplx<T>* w = bh_w_.as<cplx<T>>();
+    moved_probes(ord, nb, false, w);  // W_b = DFT⁻¹(DFT(q·P)·e_b); spec_ = DFT(q·P)
+    pass_a(ord, nb, false, MixArgs<T>{MIX_NONE, {}, {}, T(0), T(0)}, w);
+    BArgs<T> ba{nullptr, nullptr, nullptr, nullptr};
+    ba.keep = bh_keep_.as<cplx<T>>();
+    pass_b<B_BH>(ord, nb, ba);
+    const int chunk = chunk_of(nb);
+    const int nchunk = (nb + chunk - 1) / chunk;
+    run(K_PASS_C, [&] {
+      hipLaunchKernelGGL((k_pass_c_chi<T, N>), dim3(SR::NBLK, M_, nchunk), dim3(SR::NT), 0,
+                         stream_, psi_.as<cplx<T>>(), O_.as<cplx<T>>(), int(nxo_),
+                         Wt_.as<cplx<T>>(), q_.as<cplx<T>>(), corner_.as<int2>(), ord, nb, M_,
+                         chunk, 0, dP_.as<cplx<T>>(), tw_.as<cplx<T>>(), 0);
+    });
+    run(K_OBJ_DIR, [&] {
+      hipLaunchKernelGGL(k_lsqml_obj_dir<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                         bh_gO_.as<cplx<T>>(), (const cplx<T>*)w, psi_.as<cplx<T>>(), N, M_,
+                         corner_.as<int2>(), ord, list_.as<int>(), lcount_.as<int>(), int(cap_),
+                         int(nyo_), int(nxo_), ntx_, (const T*)nullptr, (const T*)nullptr, 1);
+    });
+    cplx<T>* gg = bh_gg_.as<cplx<T>>();
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_bh_conj_o<T>, dim3((n + 255) / 256, K), dim3(256), 0, stream_, gg,
+                         (const cplx<T>*)psi_.as<cplx<T>>(), (const cplx<T>*)O_.as<cplx<T>>(),
+                         int(nxo_), (const int2*)corner_.as<int2>(), ord, N, M_);
+      hipLaunchKernelGGL((k_rows<T, N, false>), dim3(S::NBLK, K), dim3(S::NT), 0, stream_, gg,
+                         tw_.as<cplx<T>>());
+      hipLaunchKernelGGL((k_cols<T, N, false>), dim3(SC::NBLK, K), dim3(SC::NT), 0, stream_, gg,
+                         tw_.as<cplx<T>>());
+      // g_P = conj(q)·Σ_b DFT⁻¹(G_b·exp(+2πi f·d_b))
+      hipLaunchKernelGGL(k_unramp_mean<T>, dim3((n + 255) / 256, M_), dim3(256), 0, stream_,
+                         bh_gP_.as<cplx<T>>(), (const cplx<T>*)gg, offs, ord, nb, N, M_,
+                         1.0 / double(n), 0);
+      inverse_planes(bh_gP_.as<cplx<T>>(), M_);
+      hipLaunchKernelGGL((k_times_q<T, true>), dim3((n + 255) / 256, M_), dim3(256), 0, stream_,
+                         bh_gP_.as<cplx<T>>(), (const cplx<T>*)q_.as<cplx<T>>(), n);
+      if (refine)
+        hipLaunchKernelGGL(k_bh_pos_grad<T>, dim3(nb), dim3(THREADS), 0, stream_,
+                           (const cplx<T>*)gg, (const cplx<T>*)spec_.as<cplx<T>>(), offs, ord,
+                           N, M_, bh_gd_.as<double>());
+    });
+    const int ndots =
+        int(std::min<size_t>(kBhDotBlocks, (no + np + THREADS - 1) / THREADS));
+    run(K_DOTS, [&] {
+      hipLaunchKernelGGL(k_bh_dots<T>, dim3(ndots), dim3(THREADS), 0, stream_,
+                         (const cplx<T>*)bh_gO_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_dO_.as<cplx<T>>(), no,
+                         (const cplx<T>*)bh_gP_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_dP_.as<cplx<T>>(), np, rho2, int(cg),
+                         bh_dots_.as<double>());
+    });
+    const int NP = cg ? 5 : 2;
+    cplx<T>* pl = bh_pl_.as<cplx<T>>();
+    run(K_PROBE, [&] {
+      q_spectra(bh_gP_.as<cplx<T>>(), bh_su_.as<cplx<T>>());
+      if (cg) q_spectra(bh_dP_.as<cplx<T>>(), bh_sv_.as<cplx<T>>());
+      auto mp = cg ? k_bh_moved_planes<T, 5> : k_bh_moved_planes<T, 2>;
+      hipLaunchKernelGGL(mp, dim3((n + 255) / 256, K), dim3(256), 0, stream_, pl,
+                         (const cplx<T>*)spec_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_su_.as<cplx<T>>(), rho2,
+                         (const cplx<T>*)bh_sv_.as<cplx<T>>(), offs, ord,
+                         refine ? (const double*)bh_gd_.as<double>() : nullptr, pos2,
+                         refine && cg ? (const double*)bh_ddf_.as<double>() : nullptr, N, M_,
+                         1.0 / double(n));
+      inverse_planes(pl, size_t(K) * NP);
+    });
+    const T norm = T(1) / (angular_ ? T(n) * T(n) : T(n));
+    using HA = HessShape<T, N, 2>;
+    run(K_HESS_A, [&] {
+      auto ha = cg ? k_bh_hess_a_moved<T, N, 2> : k_bh_hess_a_moved<T, N, 1>;
+      hipLaunchKernelGGL(ha, dim3(HA::NBLK, K), dim3(HA::NT), 0, stream_,
+                         (const cplx<T>*)O_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_gO_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_dO_.as<cplx<T>>(), int(nxo_), (const cplx<T>*)w,
+                         (const cplx<T>*)pl, (const int2*)corner_.as<int2>(), ord, M_,
+                         (const cplx<T>*)psi_.as<cplx<T>>(), bh_l_.as<cplx<T>>(),
+                         bh_pa_.as<double>(), (const cplx<T>*)tw_.as<cplx<T>>());
+    });
+    bh_hess_b_and_step(ord, nb, cg, norm, ndots, frac, slot,
+                       refine ? (const double*)bh_gd_.as<double>() : nullptr, refine ? nb : 0,
+                       pos2);
+    run(K_OBJ_UPDATE, [&] {
+      const size_t all = no + np;
+      hipLaunchKernelGGL(k_bh_update<T>, dim3(unsigned((all + 255) / 256)), dim3(256), 0,
+                         stream_, O_.as<cplx<T>>(), bh_dO_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_gO_.as<cplx<T>>(), no, P_.as<cplx<T>>(),
+                         bh_dP_.as<cplx<T>>(), (const cplx<T>*)bh_gP_.as<cplx<T>>(), np, rho2,
+                         (const double*)bh_step_.as<double>(), int(cg), int(keep));
+      if (refine)
+        hipLaunchKernelGGL(k_bh_update_pos, dim3((nb + 255) / 256), dim3(256), 0, stream_,
+                           offs_.as<double2>(), bh_ddf_.as<double>(),
+                           (const double*)bh_gd_.as<double>(), ord, nb, pos2,
+                           (const double*)bh_step_.as<double>(), int(cg), int(keep));
+    });
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_probe_terms<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,
+                         P_.as<cplx<T>>(), q_.as<cplx<T>>(), Wt_.as<cplx<T>>(), n, total);
+    });
+  }
+
+  // The Hessian's detector-plane pass and the batch's step (both BH paths).
+  void bh_hess_b_and_step(const int* ord, int nb, bool cg, T norm, int ndots, double frac,
+                          int slot, const double* gd, int npos, double pos2) {
+    run(K_HESS_B, [&] {
+      // A view's fields are adjacent planes: the column passes see nb·M·NF planes, at most
+      // 65535 per launch (grid y).
+      if (angular_) {
+        const int planes = nb * M_ * (cg ? 2 : 1);
+        for (int p0 = 0; p0 < planes; p0 += 65535)
+          pass_t<false>(bh_l_.as<cplx<T>>() + size_t(p0) * n, std::min(65535, planes - p0));
+      }
+      with_modes([&](auto mm) {
+        constexpr int MM = decltype(mm)::value;
+        auto hb = angular_ ? (cg ? k_bh_hess_b<T, N, MM, true, 2> : k_bh_hess_b<T, N, MM, true, 1>)
+                           : (cg ? k_bh_hess_b<T, N, MM, false, 2>
+                                 : k_bh_hess_b<T, N, MM, false, 1>);
+        hipLaunchKernelGGL(hb, dim3(SC::NBLK, nb), dim3(SC::NT), 0, stream_,
+                           (const cplx<T>*)bh_l_.as<cplx<T>>(),
+                           (const cplx<T>*)bh_keep_.as<cplx<T>>(), counts_.as<float>(), ord,
+                           bh_pb_.as<double>(), (const cplx<T>*)tw_.as<cplx<T>>(), norm);
+      });
+    });
+    run(K_OTHER, [&] {
+      // Two-level sums of the Hessian partials: at most kBhSumBlocks rows each, then the step.
+      const int cnt[2] = {nb * M_ * int(HessShape<T, N, 2>::NBLK), nb * int(SC::NBLK)};
+      const double* in[2] = {bh_pa_.as<double>(), bh_pb_.as<double>()};
+      int g[2];
+      for (int i = 0; i < 2; ++i) {
+        g[i] = std::min(kBhSumBlocks, (cnt[i] + THREADS - 1) / THREADS);
+        const int per = (cnt[i] + g[i] - 1) / g[i];
+        hipLaunchKernelGGL(k_sum_rows3, dim3(g[i]), dim3(THREADS), 0, stream_, in[i], cnt[i],
+                           per, bh_sum_.as<double>() + 3 * size_t(i) * kBhSumBlocks);
+      }
+      hipLaunchKernelGGL(k_bh_step, dim3(1), dim3(THREADS), 0, stream_,
+                         (const double*)bh_dots_.as<double>(), ndots,
+                         (const double*)bh_sum_.as<double>(), g[0],
+                         (const double*)bh_sum_.as<double>() + 3 * kBhSumBlocks, g[1], int(cg),
+                         frac, bh_step_.as<double>(), bh_log_.as<double>() + 4 * size_t(slot),
+                         gd, (const double*)bh_ddf_.as<double>(), ord, npos, pos2);
+    });
+  }
+
   static uint64_t bits_of(double v) {
     uint64_t u;
     std::memcpy(&u, &v, sizeof u);
@@ -2379,42 +2585,7 @@ class Engine : public EngineBase {
                          (const cplx<T>*)psi_.as<cplx<T>>(), bh_l_.as<cplx<T>>(),
                          bh_pa_.as<double>(), (const cplx<T>*)tw_.as<cplx<T>>());
     });
-    run(K_HESS_B, [&] {
-      // A view's fields are adjacent planes: the column passes see nb·M·NF planes, at most
-      // 65535 per launch (grid y).
-      if (angular_) {
-        const int planes = nb * M_ * (cg ? 2 : 1);
-        for (int p0 = 0; p0 < planes; p0 += 65535)
-          pass_t<false>(bh_l_.as<cplx<T>>() + size_t(p0) * n, std::min(65535, planes - p0));
-      }
-      with_modes([&](auto mm) {
-        constexpr int MM = decltype(mm)::value;
-        auto hb = angular_ ? (cg ? k_bh_hess_b<T, N, MM, true, 2> : k_bh_hess_b<T, N, MM, true, 1>)
-                           : (cg ? k_bh_hess_b<T, N, MM, false, 2>
-                                 : k_bh_hess_b<T, N, MM, false, 1>);
-        hipLaunchKernelGGL(hb, dim3(SC::NBLK, nb), dim3(SC::NT), 0, stream_,
-                           (const cplx<T>*)bh_l_.as<cplx<T>>(),
-                           (const cplx<T>*)bh_keep_.as<cplx<T>>(), counts_.as<float>(), ord,
-                           bh_pb_.as<double>(), (const cplx<T>*)tw_.as<cplx<T>>(), norm);
-      });
-    });
-    run(K_OTHER, [&] {
-      // Two-level sums of the Hessian partials: at most kBhSumBlocks rows each, then the step.
-      const int cnt[2] = {nb * M_ * int(HessShape<T, N, 2>::NBLK), nb * int(SC::NBLK)};
-      const double* in[2] = {bh_pa_.as<double>(), bh_pb_.as<double>()};
-      int g[2];
-      for (int i = 0; i < 2; ++i) {
-        g[i] = std::min(kBhSumBlocks, (cnt[i] + THREADS - 1) / THREADS);
-        const int per = (cnt[i] + g[i] - 1) / g[i];
-        hipLaunchKernelGGL(k_sum_rows3, dim3(g[i]), dim3(THREADS), 0, stream_, in[i], cnt[i],
-                           per, bh_sum_.as<double>() + 3 * size_t(i) * kBhSumBlocks);
-      }
-      hipLaunchKernelGGL(k_bh_step, dim3(1), dim3(THREADS), 0, stream_,
-                         (const double*)bh_dots_.as<double>(), ndots,
-                         (const double*)bh_sum_.as<double>(), g[0],
-                         (const double*)bh_sum_.as<double>() + 3 * kBhSumBlocks, g[1], int(cg),
-                         frac, bh_step_.as<double>(), bh_log_.as<double>() + 4 * size_t(slot));
-    });
+    bh_hess_b_and_step(ord, nb, cg, norm, ndots, frac, slot, nullptr, 0, 0.0);
     run(K_OBJ_UPDATE, [&] {
       const size_t all = no + np;
       hipLaunchKernelGGL(k_bh_update<T>, dim3(unsigned((all + 255) / 256)), dim3(256), 0,

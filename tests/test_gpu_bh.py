@@ -4,8 +4,9 @@ pin (tests/test_oracle_bh.py): the seeded far-field scenario with one batch of a
 (conjugate gradient from the second epoch) and with batches of 16 (bh_{full,mini}_*: every
 step's β, α, ⟨g, D⟩ and H(D, D), the object and probe changes), the near-field cases (angular
 spectrum, Fresnel forward and back, an evanescent band) against the oracle, LLK records, graph
-replays against eager launches, and the refusals. complex128 to FP64 rounding (1e-9), complex64
-to 1e-3 of the steps (1e-2 of the changes)."""
+replays against eager launches, refined PATCH position offsets against the reference's
+bh_pos_{full,mini}_* and held ones against the oracle, and the refusals. complex128 to FP64
+rounding (1e-9), complex64 to 1e-3 of the steps (1e-2 of the changes)."""
 
 import numpy as np
 import pytest
@@ -146,9 +147,66 @@ def test_refusals():
     with pytest.raises(ValueError):
         g.bh(1, 0, 16, float("nan"), 7)
     g.set_features(position=np.zeros((g.nframe, 2)))
-    with pytest.raises(ValueError, match="position"):
+    g.set_sampling("full_object")
+    with pytest.raises(ValueError, match="FullObject"):
         g.bh(1, 0, 16, 0.01, 7)
     assert g.cycle == 0 and g.bh_steps() == []
     g.set_features()
     g.bh(0, 0, 16, 0.01, 7)
     assert g.bh_steps() == []
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("batch_size,name", [(40, "bh_pos_full"), (16, "bh_pos_mini")])
+def test_patch_offsets_match_fixtures(dtype, batch_size, name):
+    # Refined PATCH offsets (ptyrs/tests/engine_bh_pos.rs:61-162): probe ρ 0.01, position ρ 2,
+    # seed 7, two epochs from the fixture's offsets: each epoch's steps, the object, probe and
+    # offset changes over the run.
+    rows = np.array(table(f"{name}_offsets_f64.txt"))
+    d0 = rows[:, 1:3].copy()
+    steps = table(f"{name}_steps_f64.txt")
+    tol = TOL[dtype]
+    g = engine(2, dtype)
+    g.set_features(position=d0)
+    g.bh(1, 0, batch_size, 0.01, 7, position_rho=2.0)
+    check_steps(g.bh_steps(), [r for r in steps if r[1] == 0], tol)
+    g = engine(2, dtype)
+    g.set_features(position=d0)
+    obj0, probe0 = g.object, g.probe
+    g.bh(2, 1, batch_size, 0.01, 7, position_rho=2.0)
+    check_steps(g.bh_steps(), [r for r in steps if r[1] == 1], tol)
+    for now, start, fname, lead in ((g.object, obj0, f"{name}_obj_f64.txt", 2),
+                                    (g.probe, probe0, f"{name}_probe_f64.txt", 3)):
+        r = np.array(table(fname))
+        want = r[:, lead] + 1j * r[:, lead + 1]
+        print(name, dtype, fname, rel_l2((now - start).ravel(), want))
+        assert rel_l2((now - start).ravel(), want) < tol * 10
+    got = g.features["position"] - d0
+    want = rows[:, 3:5] - d0
+    print(name, dtype, "offsets", np.abs(got - want).max() / np.abs(want).max())
+    assert np.abs(got - want).max() < tol * 10 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("batch_size", [40, 7])
+def test_held_offsets_match_oracle(dtype, batch_size):
+    # Held PATCH offsets (the moved probe copies alone): three epochs against the oracle, one
+    # probe mode, offsets within 0.5 px; the offsets stay.
+    from oracle import scenario as sc
+    p, _ = sc.parity_start(nprobe=1)
+    d0 = np.random.default_rng(41).uniform(-0.5, 0.5, (p.addr.nframe, 2))
+    bh = Bh(batch_size, probe_rho=0.02, seed=5, offsets=d0, refine=False)
+    obj, probe = p.obj, p.probe
+    for _ in range(3):
+        obj, probe, steps = bh.epoch(p.addr, p.iobs, p.optics, obj, probe)
+    g = engine(1, dtype)
+    g.set_features(position=d0, hold=("position",))
+    g.bh(3, 0, batch_size, 0.02, 5)
+    tol = TOL[dtype]
+    got = g.bh_steps()
+    assert [s["frames"] for s in got] == [list(s[0]) for s in steps]
+    assert max_rel([s["alpha"] for s in got], [s[2] for s in steps]) < tol
+    assert max_rel([s["bottom"] for s in got], [s[4] for s in steps]) < tol
+    assert rel_l2(g.object, obj) < tol * 10
+    assert rel_l2(g.probe, probe) < tol * 10
+    assert np.array_equal(g.features["position"], d0)
