@@ -1,5 +1,6 @@
 """pty-chi's BH (the bilinear-Hessian method of Carlsson, Nikitin and Wendt) on the amplitude
-cost, one object slice, far- or near-field exit, with or without PATCH position offsets: a
+cost, far- or near-field exit, one object slice with or without PATCH position offsets or
+several slices (`BhSlices`, the object variable every slice, [nslice, nobj, nyo, nxo]): a
 batch's gradient and bilinear Hessian at one state (`BhBatch`) and the epochs (`Bh`): steepest
 descent on each batch, or with one batch of every frame conjugate gradient across epochs, each
 step's length from the Hessian. Refined offsets join the variables (their gradient scaled by
@@ -185,6 +186,75 @@ class BhBatch:
         return h
 
 
+class BhSlices:
+    """BhBatch through several object slices (no offsets): ψ the exit waves of SliceWaves, the
+    object variable every slice ([nslice, nobj, nyo, nxo]). The gradient back-propagates χ =
+    Uᴴ(2·td) through the gaps; the Hessian's L_u, L_v and L_uv are the tangent and the mixed
+    second derivative of the slice model along u and v. ptyrs/src/engine/bh.rs:130-577,
+    ptyrs/src/engine/multislice.rs:318-383"""
+
+    def __init__(self, views, iobs, optics, objects, probe):
+        from .multislice import SliceWaves
+        self.views, self.optics = views, optics
+        self.objects, self.probe = objects, probe
+        self.frames, self.nmode = frames_of(views)
+        self.sw = SliceWaves(optics, views, list(objects), probe)
+        self.psi = optics.exit_forward(self.sw.psi)
+        nb = len(self.frames)
+        blk = self.psi.reshape((nb, self.nmode) + self.psi.shape[1:])
+        self.amp = np.sqrt((blk.real ** 2 + blk.imag ** 2).sum(axis=1)) + EPS
+        self.root = np.sqrt(np.stack([iobs[f] for f in self.frames]))
+        w = np.where(np.isnan(self.root), 0.0, 1.0 - self.root / self.amp)
+        self.td = (blk * w[:, None]).reshape(self.psi.shape)
+
+    def grads(self):
+        """(2·∂F/∂O_s* for every slice, 2·∂F/∂P*, None): χ back through the slices, each slice's
+        conj(q_s·w_s)·χ_s placed, the probe's conj(q_0·O_0[win])·χ_0 summed over the views."""
+        chi = self.optics.exit_adjoint(2.0 * self.td)
+        back = self.sw.back_propagate(chi)
+        g_obj = np.stack([self.sw.obj_adjoint(s, back[s]) for s in range(len(self.objects))])
+        np_ = self.probe.shape[0]
+        per = self.sw.probe_adjoint(back[0], np_)
+        g_probe = per.reshape(-1, np_, *self.probe.shape[1:]).sum(axis=0)
+        return g_obj, g_probe, None
+
+    def _tangents(self, u, v):
+        """[L_u, L_v, L_uv] in the exit plane before U (natural layout): per slice s the changes
+        of the wave leaving it, q_s·(O_s·w_u,s + u_s·in_s), and the mixed one
+        q_s·(O_s·w_uv,s + u_s·w_v,s + v_s·w_u,s), carried through the gaps."""
+        sw = self.sw
+        wu = np.stack([u[1][v_[2]] for v_ in self.views])
+        wv = np.stack([v[1][v_[2]] for v_ in self.views])
+        wuv = np.zeros_like(wu)
+        last = len(self.objects) - 1
+        for s in range(last + 1):
+            o = self.objects[s]
+            ou = sw._times_window(s, o, wu) + sw._times_window(s, u[0][s], sw.incident[s])
+            ov = sw._times_window(s, o, wv) + sw._times_window(s, v[0][s], sw.incident[s])
+            ouv = (sw._times_window(s, o, wuv) + sw._times_window(s, u[0][s], wv)
+                   + sw._times_window(s, v[0][s], wu))
+            if s < last:
+                gap = self.optics.gaps[s]
+                wu, wv, wuv = gap.forward(ou), gap.forward(ov), gap.forward(ouv)
+        return ou, ov, ouv
+
+    def hessian(self, u, v):
+        """BhBatch.hessian through the slices."""
+        fwd = self.optics.exit_forward
+        lu, lv, luv = [fwd(np.stack([to_shifted(p) for p in x])) for x in self._tangents(u, v)]
+        h = float(2.0 * (np.conj(self.td) * luv).real.sum())
+        nb = len(self.frames)
+        sh = (nb, self.nmode) + self.psi.shape[1:]
+        lu, lv, psi = lu.reshape(sh), lv.reshape(sh), self.psi.reshape(sh)
+        uv = (np.conj(lu) * lv).real.sum(axis=1)
+        cu = (np.conj(psi) * lu).real.sum(axis=1)
+        cv = (np.conj(psi) * lv).real.sum(axis=1)
+        keep = ~np.isnan(self.root)
+        d, a = self.root[keep], self.amp[keep]
+        h += float((2.0 * ((1.0 - d / a) * uv[keep] + d * cu[keep] * cv[keep] / a ** 3)).sum())
+        return h
+
+
 def dot(a, b):
     """Σ Re(conj(a)·b) over the object and probe parts, plus Σ a_d·b_d over the offsets' part
     where both have one. ptyrs/src/engine/bh.rs:79-90, 580-583"""
@@ -231,7 +301,12 @@ class Bh:
         for frames in random_batches(nframe, size, self.seed, epoch):
             views = [v for f in frames for v in views_all[f * nmode:(f + 1) * nmode]]
             offs = None if self.offsets is None else self.offsets[frames]
-            b = BhBatch(views, iobs, optics, obj, probe, offs, self.refine)
+            if optics.nslice() > 1:
+                if offs is not None:
+                    raise ValueError("BH through several slices models no offsets here")
+                b = BhSlices(views, iobs, optics, obj, probe)
+            else:
+                b = BhBatch(views, iobs, optics, obj, probe, offs, self.refine)
             g = b.grads()
             pg = (g[0], g[1] * rho2, None if g[2] is None else g[2] * pos2)
             prev, self.prev = self.prev, None

@@ -253,3 +253,37 @@ def test_patch_offsets_gradient_and_hessian_match_central_differences():
     gz, gp = zero.grads(), plain.grads()
     for x, y in zip(gz[:2], gp[:2]):
         assert np.abs(x - y).max() < 1e-12 * np.abs(y).max()
+
+
+@pytest.mark.parametrize("case", range(3))
+@pytest.mark.parametrize("method,batch_size", [(0, 16), (1, 6)])
+def test_multislice_epochs_match_fixtures(case, method, batch_size):
+    # Three slices with unequal gaps on the multislice LSQML state (angular-spectrum exit,
+    # Fresnel, far field), probe ρ 1, three epochs: steps to 2e-13, every slice's and the
+    # probe's change to 2e-12 (bh_multislice_*, the step rule by autograd on the reference's
+    # forward model; ptyrs/tests/engine_bh_multislice.rs:114-185).
+    from test_oracle_multislice import state as ms_state
+    addr, objects, probe0, cases = ms_state()
+    optics, iobs = cases[case][0], cases[case][1]
+    obj0 = np.stack(objects)
+    bh = Bh(batch_size, probe_rho=1.0, seed=7)
+    obj, probe, got = obj0, probe0, []
+    for e in range(3):
+        obj, probe, s = bh.epoch(addr, iobs, optics, obj, probe)
+        got += [(e, x) for x in s]
+    rows = [r for r in table("bh_multislice_steps_f64.txt") if r[0] == case and r[1] == method]
+    assert len(got) == len(rows)
+    for k, ((epoch, (frames, beta, alpha, top, bottom)), r) in enumerate(zip(got, rows)):
+        assert [k, epoch] == [int(r[2]), int(r[3])]
+        assert frames == [int(f) for f in r[10:]]
+        if r[5] == 0.0:
+            assert beta == 0.0
+        else:
+            assert abs(beta - r[5]) < 2e-13 * abs(r[5]), (k, beta, r[5])
+        for got_v, want in ((alpha, r[6]), (top, r[7]), (bottom, r[8])):
+            assert abs(got_v - want) < 2e-13 * abs(want), (k, got_v, want)
+    for now, start, name in ((obj, obj0, "bh_multislice_obj_f64.txt"),
+                             (probe, probe0, "bh_multislice_probe_f64.txt")):
+        r = np.array([t for t in table(name) if t[0] == case and t[1] == method])
+        d = (now - start).ravel()
+        assert max_rel(np.stack([d.real, d.imag], axis=1).ravel(), r[:, 5:7].ravel()) < 2e-12
