@@ -327,6 +327,53 @@ __global__ __launch_bounds__((HessShape<T, N, NF>::NT)) void k_bh_hess_a_moved(
   }
 }
 
+// ---- BH through several object slices (ptyrs/src/engine/multislice.rs:318-383) ---------------
+// One slice of the Hessian's tangent chains: per view the changes of the wave leaving slice s
+// along u and (NF = 2) v, ou = q_s·(O_s·w_u + u_s·in) and ov = q_s·(O_s·w_v + v_s·in), written
+// at out[(view·NF + f)·n] (over w itself, in place), and the slice's second-order terms against
+// χ_s, the residual carried back to slice s's output: Re Σ conj(χ_s)·q_s·2·u_s·w_u,
+// Re Σ conj(χ_s)·q_s·(u_s·w_v + v_s·w_u), Re Σ conj(χ_s)·q_s·2·v_s·w_v (FP64, per workgroup
+// into part[(view·gridDim.x + blk)·3 + i]); ⟨r, J″(u, v)⟩ is their sum over the slices. At
+// slice 0 (w_mode) w_u = su·u_P[m] and w_v = v_P[m]; `in` is P[m] there (in_view 0), else the
+// views' incident waves. Grid (ceil(n/256), nb·M).
+template <typename T, int NF>
+__global__ __launch_bounds__(256) void k_bh_slice_tangent(
+    cplx<T>* out, const cplx<T>* w, const cplx<T>* __restrict__ uP, T su,
+    const cplx<T>* __restrict__ vP, int w_mode, const cplx<T>* __restrict__ in, int in_view,
+    const cplx<T>* __restrict__ O, const cplx<T>* __restrict__ uO,
+    const cplx<T>* __restrict__ vO, int nxo, const cplx<T>* __restrict__ q,
+    const cplx<T>* __restrict__ chi, const int2* __restrict__ corner,
+    const int* __restrict__ order, int N, int M, double* __restrict__ part) {
+  __shared__ double red[256];
+  const int n = N * N;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y, b = v / M, m = v % M;
+  double s[3] = {0, 0, 0};
+  if (j < n) {
+    const int2 c = corner[order[b]];
+    const size_t at = (size_t)(c.x + j / N) * nxo + c.y + j % N;
+    const cplx<T> o = ld(O + at), ao = ld(uO + at);
+    const cplx<T> qq = q ? ld(q + j) : cplx<T>{T(1), T(0)};
+    const cplx<T> iw = ld(in + (size_t)(in_view ? v : m) * n + j);
+    const cplx<T> cq = cmulc(qq, ld(chi + (size_t)v * n + j));  // conj(q)·χ
+    const cplx<T> wu = w_mode ? ld(uP + (size_t)m * n + j) * su : ld(w + ((size_t)v * NF) * n + j);
+    s[0] = 2.0 * double(cmulc(cq, ao * wu).x);
+    const cplx<T> ou = qq * (o * wu + ao * iw);
+    if constexpr (NF == 2) {
+      const cplx<T> bo = ld(vO + at);
+      const cplx<T> wv = w_mode ? ld(vP + (size_t)m * n + j) : ld(w + ((size_t)v * NF + 1) * n + j);
+      s[1] = double(cmulc(cq, ao * wv + bo * wu).x);
+      s[2] = 2.0 * double(cmulc(cq, bo * wv).x);
+      st(out + ((size_t)v * NF + 1) * n + j, qq * (o * wv + bo * iw));
+    }
+    st(out + ((size_t)v * NF) * n + j, ou);
+  }
+  for (int i = 0; i < 3; ++i) {
+    const double t = block_reduce<256>(s[i], red, AddOp());
+    if (threadIdx.x == 0) part[((size_t)v * gridDim.x + blockIdx.x) * 3 + i] = t;
+  }
+}
+
 // The refined offsets' step of a batch: D_d = β·D′_d[frame] (cg) − pos2·gd[slot]; offsets +=
 // 0.3·α·D_d; with `keep` D′_d[frame] = D_d. ptyrs/src/engine/bh.rs:778-787
 __global__ static void k_bh_update_pos(double2* __restrict__ offs, double* __restrict__ ddf,

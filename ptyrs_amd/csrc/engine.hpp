@@ -665,7 +665,9 @@ class Engine : public EngineBase {
     if (p.batch_size == 0) throw Invalid("BH batch_size must be >= 1");
     if (!std::isfinite(p.probe_rho) || !std::isfinite(p.position_rho))
       throw Invalid("BH probe_rho and position_rho must be finite");
-    if (nslice_ > 1) throw Invalid("BH runs one object slice on this GPU engine");
+    if (nslice_ > 1 && pos_on_)
+      throw Invalid("BH through several object slices with position offsets is not run on this "
+                    "GPU engine");
     if (full_object())
       throw Invalid("BH with position offsets under FullObject sampling is not run on this GPU "
                     "engine; set the PATCH sampling");
@@ -702,7 +704,13 @@ class Engine : public EngineBase {
                                hipMemcpyDeviceToDevice, stream_));
       const int* ord = eord_.as<int>();
       const bool cg = full && ic > 0;
-      if (pos_on_) {
+      if (nslice_ > 1) {
+        // Through several slices the batches launch eagerly, as LSQML's.
+        int k = 0;
+        for (auto& s : spans[ic])
+          bh_batch_ms(rho2, ord + s.first, s.second, double(s.second) / double(size), cg, full,
+                      k++);
+      } else if (pos_on_) {
         // Under offsets the batches launch eagerly, as the other position paths.
         int k = 0;
         for (auto& s : spans[ic])
@@ -2312,7 +2320,7 @@ class Engine : public EngineBase {
   static constexpr int kBhDotBlocks = 256;
   static constexpr int kBhSumBlocks = 128;  // first-level blocks of each Hessian partial array
   void ensure_bh(size_t cap) {
-    const size_t no = nyo_ * nxo_, np = nprobe_ * n, views = cap_ * M_;
+    const size_t no = nslice_ * nyo_ * nxo_, np = nprobe_ * n, views = cap_ * M_;
     bh_keep_.ensure(sizeof(cplx<T>) * views * n);
     bh_l_.ensure(sizeof(cplx<T>) * 2 * views * n);
     for (DevBuf* b : {&bh_gO_, &bh_dO_}) b->ensure(sizeof(cplx<T>) * no);
@@ -2323,7 +2331,16 @@ class Engine : public EngineBase {
     bh_sum_.ensure(sizeof(double) * 3 * 2 * kBhSumBlocks);
     bh_step_.ensure(sizeof(double) * 2);
     bh_log_.ensure(sizeof(double) * 4 * ((nframe_ + cap - 1) / cap));
+    if (nslice_ > 1) {
+      if (2 * views > 65535)
+        throw Invalid("BH through several slices runs batches of at most 32767 views");
+      bh_chi_.ensure(sizeof(cplx<T>) * nslice_ * views * n);
+      bh_t_.ensure(sizeof(cplx<T>) * 2 * views * n);
+      bh_ps_.ensure(sizeof(double) * 3 * nslice_ * views * ((n + 255) / 256));
+    }
   }
+  // BH through several slices: χ_s of every slice, the tangent chains' waves, their partials.
+  DevBuf bh_chi_, bh_t_, bh_ps_;
   // BH under PATCH offsets: the batch's moved waves, the spectra of conj(O)·r per view, the
   // spectra of q·g_P and q·D′_P, the wave changes (NP planes per view), the offsets' gradient
   // per slot and the previous direction's offsets by frame.
@@ -2476,9 +2493,113 @@ class Engine : public EngineBase {
     });
   }
 
-  // The Hessian's detector-plane pass and the batch's step (both BH paths).
+  // One BH batch through several object slices (ptyrs/src/engine/bh.rs:130-577 with
+  // SliceWaves, ptyrs/src/engine/multislice.rs:266-442): the exit waves through the slices,
+  // the gradient carried back through the gaps (each slice's χ_s kept), then the tangent
+  // chains along u = (g_O, ρ²·g_P) and v = D′ up through the slices, whose second-order terms
+  // meet χ_s slice by slice and whose exit changes L_u, L_v go through the detector-plane pass.
+  void bh_batch_ms(double rho2, const int* ord, int nb, double frac, bool cg, bool keep,
+                   int slot) {
+    const int nt = ntiles();
+    const size_t S = nslice_, no1 = nyo_ * nxo_, no = S * no1, np = nprobe_ * n;
+    const int total = int(np), K = nb * M_, NF = cg ? 2 : 1;
+    run(K_BIN, [&] {
+      hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                         corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
+                         lcount_.as<int>(), (const T*)nullptr, M_, int(nyo_), int(nxo_),
+                         (T*)nullptr, tilemax_.as<T>(), (const cplx<T>*)nullptr);
+    });
+    const Waves wv = stack_waves(ord, nb);
+    ms_forward(ord, nb, wv);
+    run(K_PASS_A, [&] { rows(false, K); });
+    BArgs<T> ba{nullptr, nullptr, nullptr, nullptr};
+    ba.keep = bh_keep_.as<cplx<T>>();
+    pass_b<B_BH>(ord, nb, ba);
+    cplx<T>* chi = psi_.as<cplx<T>>();
+    run(K_PASS_C, [&] { rows(true, K); });  // r = Uᴴ(2·td) at the last slice's output
+    const size_t vn = size_t(K) * n;
+    for (size_t s = S; s-- > 0;) {
+      segment_for(s, wv, ord, nb);
+      const auto in = wave(s, wv);
+      const cplx<T>* qs = s + 1 == S ? wv.qlast : nullptr;
+      PTYRS_HIP(hipMemcpyAsync(bh_chi_.as<cplx<T>>() + s * vn, chi, sizeof(cplx<T>) * vn,
+                               hipMemcpyDeviceToDevice, stream_));
+      run(K_OBJ_DIR, [&] {
+        hipLaunchKernelGGL(k_lsqml_obj_dir<T>, dim3(nt), dim3(THREADS), 0, stream_,
+                           bh_gO_.as<cplx<T>>() + s * no1, in.first, (const cplx<T>*)chi, N, M_,
+                           corner_.as<int2>(), ord, list_.as<int>(), lcount_.as<int>(), int(cap_),
+                           int(nyo_), int(nxo_), ntx_, (const T*)nullptr, (const T*)nullptr,
+                           in.second, qs, 0);
+      });
+      if (s == 0) break;
+      run(K_PASS_C, [&] {
+        window_rows<MS_ADJ>(chi, s, nullptr, 0, qs, 0, ord, nb);
+        gap_tail(chi, K, s - 1, true);
+      });
+    }
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_ms_probe_dir<T>, dim3((n + 255) / 256, M_), dim3(256), 0, stream_,
+                         bh_gP_.as<cplx<T>>(), (const cplx<T>*)slice_obj(0), int(nxo_),
+                         (const cplx<T>*)nullptr, (const cplx<T>*)chi, corner_.as<int2>(), ord, nb,
+                         N, M_, T(1), 0);
+    });
+    const int ndots =
+        int(std::min<size_t>(kBhDotBlocks, (no + np + THREADS - 1) / THREADS));
+    run(K_DOTS, [&] {
+      hipLaunchKernelGGL(k_bh_dots<T>, dim3(ndots), dim3(THREADS), 0, stream_,
+                         (const cplx<T>*)bh_gO_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_dO_.as<cplx<T>>(), no,
+                         (const cplx<T>*)bh_gP_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_dP_.as<cplx<T>>(), np, rho2, int(cg),
+                         bh_dots_.as<double>());
+    });
+    const int nblk = (n + 255) / 256;
+    for (size_t s = 0; s < S; ++s) {
+      if (s > 0) segment_for(s, wv, ord, nb);
+      const auto in = wave(s, wv);
+      const bool last = s + 1 == S;
+      cplx<T>* out = last ? bh_l_.as<cplx<T>>() : bh_t_.as<cplx<T>>();
+      run(K_HESS_A, [&] {
+        auto tk = cg ? k_bh_slice_tangent<T, 2> : k_bh_slice_tangent<T, 1>;
+        hipLaunchKernelGGL(tk, dim3(nblk, K), dim3(256), 0, stream_, out,
+                           (const cplx<T>*)bh_t_.as<cplx<T>>(),
+                           (const cplx<T>*)bh_gP_.as<cplx<T>>(), T(rho2),
+                           (const cplx<T>*)bh_dP_.as<cplx<T>>(), int(s == 0), in.first,
+                           in.second, (const cplx<T>*)slice_obj(s),
+                           (const cplx<T>*)(bh_gO_.as<cplx<T>>() + s * no1),
+                           (const cplx<T>*)(bh_dO_.as<cplx<T>>() + s * no1), int(nxo_),
+                           last ? wv.qlast : (const cplx<T>*)nullptr,
+                           (const cplx<T>*)(bh_chi_.as<cplx<T>>() + s * vn),
+                           (const int2*)corner_.as<int2>(), ord, N, M_,
+                           bh_ps_.as<double>() + s * size_t(K) * nblk * 3);
+        if (!last) gap(bh_t_.as<cplx<T>>(), K * NF, s, false);
+      });
+    }
+    run(K_HESS_A, [&] {
+      hipLaunchKernelGGL((k_rows<T, N, false>), dim3(S::NBLK, K * NF), dim3(S::NT), 0, stream_,
+                         bh_l_.as<cplx<T>>(), tw_.as<cplx<T>>());
+    });
+    const T norm = T(1) / (angular_ ? T(n) * T(n) : T(n));
+    bh_hess_b_and_step(ord, nb, cg, norm, ndots, frac, slot, nullptr, 0, 0.0,
+                       bh_ps_.as<double>(), int(S) * K * nblk);
+    run(K_OBJ_UPDATE, [&] {
+      const size_t all = no + np;
+      hipLaunchKernelGGL(k_bh_update<T>, dim3(unsigned((all + 255) / 256)), dim3(256), 0,
+                         stream_, O_.as<cplx<T>>(), bh_dO_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_gO_.as<cplx<T>>(), no, P_.as<cplx<T>>(),
+                         bh_dP_.as<cplx<T>>(), (const cplx<T>*)bh_gP_.as<cplx<T>>(), np, rho2,
+                         (const double*)bh_step_.as<double>(), int(cg), int(keep));
+    });
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_probe_terms<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,
+                         P_.as<cplx<T>>(), q_.as<cplx<T>>(), Wt_.as<cplx<T>>(), n, total);
+    });
+  }
+
+  // The Hessian's detector-plane pass and the batch's step (every BH path).
   void bh_hess_b_and_step(const int* ord, int nb, bool cg, T norm, int ndots, double frac,
-                          int slot, const double* gd, int npos, double pos2) {
+                          int slot, const double* gd, int npos, double pos2,
+                          const double* pa = nullptr, int npa = 0) {
     run(K_HESS_B, [&] {
       // A view's fields are adjacent planes: the column passes see nb·M·NF planes, at most
       // 65535 per launch (grid y).
@@ -2500,8 +2621,9 @@ class Engine : public EngineBase {
     });
     run(K_OTHER, [&] {
       // Two-level sums of the Hessian partials: at most kBhSumBlocks rows each, then the step.
-      const int cnt[2] = {nb * M_ * int(HessShape<T, N, 2>::NBLK), nb * int(SC::NBLK)};
-      const double* in[2] = {bh_pa_.as<double>(), bh_pb_.as<double>()};
+      const int cnt[2] = {pa ? npa : nb * M_ * int(HessShape<T, N, 2>::NBLK),
+                          nb * int(SC::NBLK)};
+      const double* in[2] = {pa ? pa : bh_pa_.as<double>(), bh_pb_.as<double>()};
       int g[2];
       for (int i = 0; i < 2; ++i) {
         g[i] = std::min(kBhSumBlocks, (cnt[i] + THREADS - 1) / THREADS);

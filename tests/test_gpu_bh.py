@@ -210,3 +210,33 @@ def test_held_offsets_match_oracle(dtype, batch_size):
     assert rel_l2(g.object, obj) < tol * 10
     assert rel_l2(g.probe, probe) < tol * 10
     assert np.array_equal(g.features["position"], d0)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("case", [0, 1, 2])
+@pytest.mark.parametrize("method,batch_size", [(0, 16), (1, 6)])
+def test_multislice_matches_fixtures(dtype, case, method, batch_size):
+    # Three slices with unequal gaps under the angular-spectrum, Fresnel and far-field exits
+    # (ptyrs/tests/engine_bh_multislice.rs:114-185): probe ρ 1, three epochs: the last epoch's
+    # steps, every slice's and the probe's change over the run.
+    from test_gpu_multislice import start as ms_start
+    g, objects, probe0 = ms_start(case, dtype)
+    g.bh(3, 1, batch_size, 1.0, 7)
+    tol = TOL[dtype]
+    rows = [r for r in table("bh_multislice_steps_f64.txt")
+            if r[0] == case and r[1] == method and r[3] == 2]
+    got = g.bh_steps()
+    assert len(got) == len(rows)
+    for s, r in zip(got, rows):
+        assert s["frames"] == [int(f) for f in r[10:]]
+        if r[5] != 0.0:
+            assert abs(s["beta"] - r[5]) < tol * 10 * abs(r[5]), (s["beta"], r[5])
+        for name, want in (("alpha", r[6]), ("top", r[7]), ("bottom", r[8])):
+            assert abs(s[name] - want) < tol * abs(want), (name, s[name], want)
+    for now, start, name in ((g.objects, objects, "bh_multislice_obj_f64.txt"),
+                             (g.probe, probe0, "bh_multislice_probe_f64.txt")):
+        r = np.array([t for t in table(name) if t[0] == case and t[1] == method])
+        want = r[:, 5] + 1j * r[:, 6]
+        print(case, method, dtype, name, rel_l2((now - start).ravel(), want))
+        assert rel_l2((now - start).ravel(), want) < tol * 10
+    assert list(g.history["algorithm"]) == ["bh"] * 3
