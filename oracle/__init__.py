@@ -43,7 +43,8 @@ What pins each module (the test files named run on the CPU):
   (conjugate gradient) and with batches of 16 (`bh_full_*`, `bh_mini_*`: β, α, ⟨g, D⟩, H(D, D)
   at 1e-13, object changes at 1e-13, probe at 5e-12), the four near-field cases with both batch
   sizes (`bh_near_*`); three slices with unequal gaps under the three exits (`bh_multislice_*`,
-  steps at 2e-13, every slice's and the probe's change at 2e-12); refined PATCH offsets (`bh_pos_full_*`, `bh_pos_mini_*` at 3e-12); the
+  steps at 2e-13, every slice's and the probe's change at 2e-12); refined offsets under PATCH (`bh_pos_full_*`, `bh_pos_mini_*`) and FullObject sampling
+  (`bh_full_object_full_*`, `bh_full_object_mini_*`, pty-chi's) at 3e-12; the
   gradient against central differences of the cost, the Hessian against central differences of
   the gradient (with and without offsets), its symmetry and bilinearity, zero offsets reducing
   to the unmoved batch.

@@ -1,5 +1,6 @@
 """pty-chi's BH (the bilinear-Hessian method of Carlsson, Nikitin and Wendt) on the amplitude
-cost, far- or near-field exit, one object slice with or without PATCH position offsets or
+cost, far- or near-field exit, one object slice with or without position offsets (PATCH, or
+FullObject: the object sampled at corner plus offset) or
 several slices (`BhSlices`, the object variable every slice, [nslice, nobj, nyo, nxo]): a
 batch's gradient and bilinear Hessian at one state (`BhBatch`) and the epochs (`Bh`): steepest
 descent on each batch, or with one batch of every frame conjugate gradient across epochs, each
@@ -16,6 +17,7 @@ import numpy as np
 
 from .base import random_batches, to_natural, to_shifted
 from .positions import POSITION_STEP, _freqs, ramp
+from .sampling import Sampler
 
 # pty-chi's offset on the modelled amplitude in the residual and the Hessian.
 EPS = 1e-8
@@ -40,21 +42,31 @@ class BhBatch:
     Stores U ψ, A + EPS, √iobs (NaN where masked) and td = U ψ·(1 − √iobs/(A + EPS)) (0 where
     masked), half the gradient of F over U ψ. ptyrs/src/engine/bh.rs:130-272"""
 
-    def __init__(self, views, iobs, optics, obj, probe, offsets=None, refine=False):
+    def __init__(self, views, iobs, optics, obj, probe, offsets=None, refine=False,
+                 sampling="patch"):
         self.views, self.optics = views, optics
         self.obj, self.probe = obj, probe
         self.frames, self.nmode = frames_of(views)
         self.shape = probe.shape[-2:]
-        self.offsets = None if offsets is None else np.asarray(offsets, dtype=np.float64)
-        if self.offsets is not None and self.offsets.shape != (len(self.frames), 2):
+        offsets = None if offsets is None else np.asarray(offsets, dtype=np.float64)
+        if offsets is not None and offsets.shape != (len(self.frames), 2):
             raise ValueError("one offset per frame of the batch")
-        self.refine = refine and self.offsets is not None
+        self.refine = refine and offsets is not None
+        # FullObject: every view reads the object sampled at its corner plus its frame's offset
+        # (ptyrs/src/engine/bh.rs:96-107); the probe terms stay.
+        self.sm = None
+        if offsets is not None and sampling == "full_object":
+            self.sm = Sampler(obj.shape[-2:], self.shape, views,
+                              [offsets[k // self.nmode] for k in range(len(views))])
+            offsets = None
+        self.offsets = offsets
         if self.offsets is not None:
             self.ramps = [ramp(self.shape, d) for d in self.offsets]
             fy, fx = _freqs(self.shape)
             self.f = (-2j * math.pi * fy, -2j * math.pi * fx)   # −2πi f_a
             self.spec = np.fft.fft2(probe * optics.quad)
         self.w = self._moved(probe)
+        self.win = self._reads(obj)
         self.psi = optics.exit_forward(self._exit(obj, self.w))
         nb = len(self.frames)
         blk = self.psi.reshape((nb, self.nmode) + self.psi.shape[1:])
@@ -90,18 +102,21 @@ class BhBatch:
         """Σ_a d[b][a]·(−2πi f_a): the first derivative's factor along a frame's offset change."""
         return d[b][0] * self.f[0] + d[b][1] * self.f[1]
 
+    def _reads(self, x, axes=()):
+        """What the views read of object-shaped x: the windows at their corners, under
+        FullObject sampled at corner plus offset (with `axes` its derivative over the offset)."""
+        if self.sm is not None:
+            return self.sm.windows(x, axes)
+        ny, nx = self.shape
+        return np.stack([x[o, oy:oy + ny, ox:ox + nx] for (_, _, _, o, oy, ox) in self.views])
+
     def _exit(self, obj, w, obj2=None, w2=None):
         """Shifted exit-plane waves W_k·O_k[win], or with obj2, w2 the bilinear term W_k·O2_k +
         W2_k·O_k."""
-        ny, nx = self.shape
-        out = np.empty((len(self.views), ny, nx), dtype=np.complex128)
-        for k, (_, _, _, o, oy, ox) in enumerate(self.views):
-            win = obj[o, oy:oy + ny, ox:ox + nx]
-            if obj2 is None:
-                out[k] = to_shifted(w[k] * win)
-            else:
-                out[k] = to_shifted(w[k] * obj2[o, oy:oy + ny, ox:ox + nx] + w2[k] * win)
-        return out
+        win = self._reads(obj)
+        if obj2 is None:
+            return np.stack([to_shifted(p) for p in w * win])
+        return np.stack([to_shifted(p) for p in w * self._reads(obj2) + w2 * win])
 
     def grads(self):
         """(2·∂F/∂O*, 2·∂F/∂P*, ∂F/∂d): χ = Uᴴ(2·td) back-projected, the object's onto the
@@ -115,6 +130,20 @@ class BhBatch:
         g_obj = np.zeros_like(self.obj)
         g_probe = np.zeros_like(self.probe)
         g_d = np.zeros((len(self.frames), 2)) if self.refine else None
+        if self.sm is not None:
+            # the reads' gradient placed by the sampling's adjoint; the offsets' through the
+            # reads' derivatives (ptyrs/src/engine/bh.rs:296-308)
+            c = np.stack([to_natural(x) for x in chi])
+            g_win = c * np.conj(self.w)
+            g_obj = self.sm.place(g_win, self.obj.shape[0])
+            for k, v in enumerate(self.views):
+                g_probe[v[2]] += c[k] * np.conj(self.win[k] * q)
+            if self.refine:
+                for a in range(2):
+                    dw = self.sm.windows(self.obj, (a,))
+                    for k in range(len(self.views)):
+                        g_d[k // self.nmode, a] += float((np.conj(g_win[k]) * dw[k]).real.sum())
+            return g_obj, g_probe, g_d
         for k, (_, _, p, o, oy, ox) in enumerate(self.views):
             c = to_natural(chi[k])
             win = self.obj[o, oy:oy + ny, ox:ox + nx]
@@ -152,6 +181,50 @@ class BhBatch:
                                   (sv, lambda b: self._fd(u[2], b)),
                                   (self.spec, lambda b: self._fd(u[2], b) * self._fd(v[2], b))])
 
+    def _fo_lift(self, u):
+        """FullObject: the reads' first-order change S(u_O) + Σ_a u_da·∂_a S(O)."""
+        r = self._reads(u[0])
+        if self.refine:
+            for a in range(2):
+                d = self.sm.windows(self.obj, (a,))
+                r = r + np.stack([u[2][k // self.nmode][a] * d[k]
+                                  for k in range(len(self.views))])
+        return r
+
+    def _fo_second(self, u, v):
+        """FullObject: the reads' second-order change Σ_a (v_da·∂_a S(u_O) + u_da·∂_a S(v_O)) +
+        Σ_ac u_da·v_dc·∂_a ∂_c S(O); None with held offsets."""
+        if not self.refine:
+            return None
+        nv = len(self.views)
+        cu = [u[2][k // self.nmode] for k in range(nv)]
+        cv = [v[2][k // self.nmode] for k in range(nv)]
+        out = np.zeros((nv,) + tuple(self.shape), dtype=np.complex128)
+        for a in range(2):
+            du, dv = self.sm.windows(u[0], (a,)), self.sm.windows(v[0], (a,))
+            for k in range(nv):
+                out[k] += cv[k][a] * du[k] + cu[k][a] * dv[k]
+            for c in range(2):
+                dd = self.sm.windows(self.obj, (a, c))
+                for k in range(nv):
+                    out[k] += cu[k][a] * cv[k][c] * dd[k]
+        return out
+
+    def _hessian_fo(self, u, v):
+        """The Hessian's exit-plane fields under FullObject: L_u = W·u′ + u_P·q·S(O), L_uv =
+        u′·v_P·q + v′·u_P·q + W·o″ (shifted)."""
+        q = self.optics.quad
+        wu = np.stack([u[1][x[2]] * q for x in self.views])
+        wv = np.stack([v[1][x[2]] * q for x in self.views])
+        ru, rv = self._fo_lift(u), self._fo_lift(v)
+        lu = self.w * ru + wu * self.win
+        lv = self.w * rv + wv * self.win
+        luv = ru * wv + rv * wu
+        o2 = self._fo_second(u, v)
+        if o2 is not None:
+            luv = luv + self.w * o2
+        return [np.stack([to_shifted(p) for p in x]) for x in (lu, lv, luv)]
+
     def hessian(self, u, v):
         """The second derivative of F along the displacements u = (u_O, u_P[, u_d]) and v,
         pty-chi's bilinear Hessian with A + EPS in place of A: Σ 2·Re(conj(td)·L_uv) +
@@ -160,18 +233,22 @@ class BhBatch:
         second-order change, c_u = Σ_m Re(conj(U ψ)·L_u), d = √iobs.
         ptyrs/src/engine/bh.rs:437-577"""
         fwd = self.optics.exit_forward
-        wu, wv = self._lift(u), self._lift(v)
-        lu = fwd(self._exit(self.obj, self.w, u[0], wu))
-        lv = fwd(self._exit(self.obj, self.w, v[0], wv))
-        ny, nx = self.shape
-        luv = np.empty((len(self.views), ny, nx), dtype=np.complex128)
-        w2 = self._second(u, v)
-        for k, (_, _, _, o, oy, ox) in enumerate(self.views):
-            t = u[0][o, oy:oy + ny, ox:ox + nx] * wv[k] + v[0][o, oy:oy + ny, ox:ox + nx] * wu[k]
-            if w2 is not None:
-                t = t + self.obj[o, oy:oy + ny, ox:ox + nx] * w2[k]
-            luv[k] = to_shifted(t)
-        luv = fwd(luv)
+        if self.sm is not None:
+            lu, lv, luv = [fwd(x) for x in self._hessian_fo(u, v)]
+        else:
+            wu, wv = self._lift(u), self._lift(v)
+            lu = fwd(self._exit(self.obj, self.w, u[0], wu))
+            lv = fwd(self._exit(self.obj, self.w, v[0], wv))
+            ny, nx = self.shape
+            luv = np.empty((len(self.views), ny, nx), dtype=np.complex128)
+            w2 = self._second(u, v)
+            for k, (_, _, _, o, oy, ox) in enumerate(self.views):
+                t = (u[0][o, oy:oy + ny, ox:ox + nx] * wv[k] +
+                     v[0][o, oy:oy + ny, ox:ox + nx] * wu[k])
+                if w2 is not None:
+                    t = t + self.obj[o, oy:oy + ny, ox:ox + nx] * w2[k]
+                luv[k] = to_shifted(t)
+            luv = fwd(luv)
         h = float(2.0 * (np.conj(self.td) * luv).real.sum())
         nb = len(self.frames)
         sh = (nb, self.nmode) + self.psi.shape[1:]
@@ -270,13 +347,14 @@ class Bh:
     part by frame). ptyrs/src/engine/bh.rs:585-799"""
 
     def __init__(self, batch_size, probe_rho=0.01, seed=7, position_rho=0.1, offsets=None,
-                 refine=True):
+                 refine=True, sampling="patch"):
         if batch_size < 1:
             raise ValueError("BH batch_size must be >= 1")
         self.batch_size, self.probe_rho, self.seed = batch_size, probe_rho, seed
         self.position_rho = position_rho
         self.offsets = None if offsets is None else np.array(offsets, dtype=np.float64)
         self.refine = refine and self.offsets is not None
+        self.sampling = sampling
         self.epochs = 0
         self.prev = None
 
@@ -306,7 +384,7 @@ class Bh:
                     raise ValueError("BH through several slices models no offsets here")
                 b = BhSlices(views, iobs, optics, obj, probe)
             else:
-                b = BhBatch(views, iobs, optics, obj, probe, offs, self.refine)
+                b = BhBatch(views, iobs, optics, obj, probe, offs, self.refine, self.sampling)
             g = b.grads()
             pg = (g[0], g[1] * rho2, None if g[2] is None else g[2] * pos2)
             prev, self.prev = self.prev, None

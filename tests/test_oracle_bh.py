@@ -170,16 +170,20 @@ def test_near_field_epochs_match_fixtures(case, method, batch_size):
                        rows[:, lead:lead + 2].ravel()) < 5e-12
 
 
-@pytest.mark.parametrize("batch_size,name", [(40, "bh_pos_full"), (16, "bh_pos_mini")])
-def test_patch_offsets_match_fixtures(batch_size, name):
-    # Refined PATCH offsets (the reference's model of its steps, refs/f64/bh_pos.py; probe ρ
-    # 0.01, position ρ 2, seed 7, two epochs from its offsets): steps, the object and probe
-    # changes over the run relative to the start's largest value, the offsets' changes relative
-    # to their largest, all to 3e-12 (ptyrs/tests/engine_bh_pos.rs:61-143).
+@pytest.mark.parametrize("batch_size,name,sampling", [
+    (40, "bh_pos_full", "patch"), (16, "bh_pos_mini", "patch"),
+    (40, "bh_full_object_full", "full_object"), (16, "bh_full_object_mini", "full_object")])
+def test_offsets_match_fixtures(batch_size, name, sampling):
+    # Refined offsets (probe ρ 0.01, position ρ 2, seed 7, two epochs from the fixture's
+    # offsets) under PATCH (the reference's model of its steps, refs/f64/bh_pos.py) and under
+    # FullObject (pty-chi's FULL_OBJECT BH, refs/py/ptychi_full_object.py): steps, the object
+    # and probe changes over the run relative to the start's largest value, the offsets' changes
+    # relative to their largest, all to 3e-12 (ptyrs/tests/engine_bh_pos.rs:61-184).
     p = state()
     rows = np.array(table(f"{name}_offsets_f64.txt"))
     d0 = rows[:, 1:3].copy()
-    bh = Bh(batch_size, probe_rho=0.01, seed=7, position_rho=2.0, offsets=d0, refine=True)
+    bh = Bh(batch_size, probe_rho=0.01, seed=7, position_rho=2.0, offsets=d0, refine=True,
+            sampling=sampling)
     obj, probe, got = p.obj, p.probe, []
     for e in range(2):
         obj, probe, s = bh.epoch(p.addr, p.iobs, p.optics, obj, probe)
