@@ -327,6 +327,147 @@ __global__ __launch_bounds__((HessShape<T, N, NF>::NT)) void k_bh_hess_a_moved(
   }
 }
 
+// ---- BH under FullObject sampling (ptyrs/src/engine/bh.rs:96-107, :380-577) -------------------
+// Every slot reads S(O), the object sampled at its corner plus offset (one N × N plane per slot,
+// natural layout); a displacement changes the reads by u′ = S(u_O) + Σ_a u_da·∂_a S(O) and, to
+// second order along u and v, by o″ = Σ_a (v_da·∂_a S(u_O) + u_da·∂_a S(v_O)) +
+// Σ_ac u_da·v_dc·∂_a ∂_c S(O).
+
+// Per slot Re Σ conj(g)·d_a over the slot's plane for a = y, x: g the reads' gradient
+// ([slot][n]), d the reads' derivatives ([slot·2 + a][n]); into gd[slot·2 + a] (FP64). One
+// workgroup per slot.
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_bh_win_dot(const cplx<T>* __restrict__ g,
+                                                        const cplx<T>* __restrict__ d, int n,
+                                                        double* __restrict__ gd) {
+  __shared__ double red[THREADS];
+  const int b = blockIdx.x;
+  double acc[2] = {0, 0};
+  for (int j = threadIdx.x; j < n; j += THREADS) {
+    const cplx<T> gv = ld(g + (size_t)b * n + j);
+    for (int a = 0; a < 2; ++a) acc[a] += double(cmulc(gv, ld(d + ((size_t)b * 2 + a) * n + j)).x);
+  }
+  for (int a = 0; a < 2; ++a) {
+    const double t = block_reduce<THREADS>(acc[a], red, AddOp());
+    if (threadIdx.x == 0) gd[2 * (size_t)b + a] = t;
+  }
+}
+
+// The reads' changes per slot: out[slot·NR + p] = u′ (p 0), v′ (1, NF = 2), o″_uu, o″_uv, o″_vv
+// (NR = 5; NF = 1: u′, o″_uu, NR = 2), from w[slot·9 + p] = S(u_O), S(v_O), ∂_y S(u_O),
+// ∂_x S(u_O), ∂_y S(v_O), ∂_x S(v_O), ∂_yy S(O), ∂_yx S(O), ∂_xx S(O) and d[slot·2 + a] =
+// ∂_a S(O); u_d = pos2·gd[slot], v_d = ddf[frame] (gd null: held offsets, no offset terms).
+// Grid (ceil(n/256), nb).
+template <typename T, int NF>
+__global__ void k_bh_fo_reads(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ w,
+                              const cplx<T>* __restrict__ d, const double* __restrict__ gd,
+                              double pos2, const double* __restrict__ ddf,
+                              const int* __restrict__ order, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int b = blockIdx.y;
+  constexpr int NR = NF == 2 ? 5 : 2;
+  auto at = [&](int p) { return ld(w + ((size_t)b * 9 + p) * n + j); };
+  const cplx<T> su = at(0);
+  cplx<T>* o = out + (size_t)b * NR * n + j;
+  if (!gd) {
+    st(o, su);
+    if constexpr (NF == 2) {
+      st(o + n, at(1));
+      st(o + 2 * n, cplx<T>{T(0), T(0)});
+      st(o + 3 * n, cplx<T>{T(0), T(0)});
+      st(o + 4 * n, cplx<T>{T(0), T(0)});
+    } else {
+      st(o + n, cplx<T>{T(0), T(0)});
+    }
+    return;
+  }
+  const T u0 = T(pos2 * gd[2 * b]), u1 = T(pos2 * gd[2 * b + 1]);
+  const cplx<T> dy = ld(d + ((size_t)b * 2) * n + j), dx = ld(d + ((size_t)b * 2 + 1) * n + j);
+  const cplx<T> dyy = at(6), dyx = at(7), dxx = at(8);
+  // Σ_ac a_a·c_c·∂_a∂_c S(O)
+  auto dd = [&](T a0, T a1, T c0, T c1) { return dyy * (a0 * c0) + dyx * (a0 * c1 + a1 * c0) + dxx * (a1 * c1); };
+  st(o, su + dy * u0 + dx * u1);
+  const cplx<T> uy = at(2), ux = at(3);
+  if constexpr (NF == 1) {
+    st(o + n, (uy * u0 + ux * u1) * T(2) + dd(u0, u1, u0, u1));
+  } else {
+    const int f = order[b];
+    const T v0 = T(ddf[2 * f]), v1 = T(ddf[2 * f + 1]);
+    const cplx<T> vy = at(4), vx = at(5);
+    st(o + n, at(1) + dy * v0 + dx * v1);
+    st(o + 2 * n, (uy * u0 + ux * u1) * T(2) + dd(u0, u1, u0, u1));
+    st(o + 3 * n, uy * v0 + ux * v1 + vy * u0 + vx * u1 + dd(u0, u1, v0, v1));
+    st(o + 4 * n, (vy * v0 + vx * v1) * T(2) + dd(v0, v1, v0, v1));
+  }
+}
+
+// Pass A of the Hessian sums under FullObject: as k_bh_hess_a with the reads per slot: S(O) from
+// win ([slot][n]) and u′, v′, o″_uu, o″_uv, o″_vv from rd ([slot·NR + p][n]): L_u = W·u′ +
+// u_W·S(O), L_v = W·v′ + v_W·S(O) (u_W = su·u_P·q, v_W = v_P·q); second-order sums
+// Re Σ conj(r)·(2·u′·u_W + W·o″_uu), Re Σ conj(r)·(u′·v_W + v′·u_W + W·o″_uv),
+// Re Σ conj(r)·(2·v′·v_W + W·o″_vv).
+template <typename T, int N, int NF>
+__global__ __launch_bounds__((HessShape<T, N, NF>::NT)) void k_bh_hess_a_fo(
+    const cplx<T>* __restrict__ win, const cplx<T>* __restrict__ rd,
+    const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ uP, T su,
+    const cplx<T>* __restrict__ vP, const cplx<T>* __restrict__ q, int M,
+    const cplx<T>* __restrict__ r, cplx<T>* __restrict__ lf, double* __restrict__ part,
+    const cplx<T>* __restrict__ tw) {
+  using S = HessShape<T, N, NF>;
+  constexpr int E = S::E, TL = S::TL;
+  constexpr int n = N * N;
+  constexpr int NR = NF == 2 ? 5 : 2;
+  __shared__ T lre[S::LDS_WORDS], lim[S::LDS_WORDS];
+  __shared__ cplx<T> stw[N];
+  __shared__ double red[S::NT];
+  const int tid = threadIdx.x, l = tid / TL, jl = tid % TL;
+  const int y = blockIdx.x * S::L + l;
+  const int v = blockIdx.y, b = v / M, m = v % M;
+  stage_twiddles<T, N, S::NT>(stw, tw);
+  const size_t wrow = ((size_t)b * N + y) * N;
+  const size_t rrow = ((size_t)b * NR * N + y) * N;
+  const size_t pr = (size_t)m * n + (size_t)y * N;
+  const size_t row = ((size_t)v * N + y) * N;
+  const size_t lrow = ((size_t)v * NF * N + y) * N;
+  cplx<T> lu[E], lv[E];
+  double s[3] = {0, 0, 0};
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int x = jl + TL * e;
+    const cplx<T> o = ld(win + wrow + x), w = ld(Wt + pr + x), qq = ld(q + (size_t)y * N + x);
+    const cplx<T> rr = ld(r + row + x);
+    const cplx<T> ao = ld(rd + rrow + x), aw = ld(uP + pr + x) * qq * su;
+    lu[e] = w * ao + aw * o;
+    if constexpr (NF == 1) {
+      const cplx<T> ouu = ld(rd + rrow + n + x);
+      s[0] += double(cmulc(rr, ao * aw * T(2) + w * ouu).x);
+    } else {
+      const cplx<T> bo = ld(rd + rrow + n + x), bw = ld(vP + pr + x) * qq;
+      const cplx<T> ouu = ld(rd + rrow + 2 * n + x), ouv = ld(rd + rrow + 3 * n + x),
+                    ovv = ld(rd + rrow + 4 * n + x);
+      lv[e] = w * bo + bw * o;
+      s[0] += double(cmulc(rr, ao * aw * T(2) + w * ouu).x);
+      s[1] += double(cmulc(rr, ao * bw + bo * aw + w * ouv).x);
+      s[2] += double(cmulc(rr, bo * bw * T(2) + w * ovv).x);
+    }
+  }
+  T* re = lre + S::base_row(l);
+  T* im = lim + S::base_row(l);
+  line_fft<T, N, E, false, typename S::AtRow>(lu, jl, re, im, stw);
+  if constexpr (NF == 2) line_fft<T, N, E, false, typename S::AtRow>(lv, jl, re, im, stw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    st(lf + lrow + jl + TL * e, lu[e]);
+    if constexpr (NF == 2) st(lf + lrow + n + jl + TL * e, lv[e]);
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double t = block_reduce<S::NT>(s[i], red, AddOp());
+    if (tid == 0) part[((size_t)v * gridDim.x + blockIdx.x) * 3 + i] = t;
+  }
+}
+
 // ---- BH through several object slices (ptyrs/src/engine/multislice.rs:318-383) ---------------
 // One slice of the Hessian's tangent chains: per view the changes of the wave leaving slice s
 // along u and (NF = 2) v, ou = q_s·(O_s·w_u + u_s·in) and ov = q_s·(O_s·w_v + v_s·in), written

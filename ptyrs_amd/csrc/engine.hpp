@@ -668,9 +668,9 @@ class Engine : public EngineBase {
     if (nslice_ > 1 && pos_on_)
       throw Invalid("BH through several object slices with position offsets is not run on this "
                     "GPU engine");
-    if (full_object())
-      throw Invalid("BH with position offsets under FullObject sampling is not run on this GPU "
-                    "engine; set the PATCH sampling");
+    if (full_object() && nobj_ != 1)
+      throw Invalid("FullObject sampling with several object modes is not run on this GPU "
+                    "engine");
     if (nobj_ != 1) throw Invalid("the GPU BH runs one object mode");
     if (M_ > kMaxModes) throw Invalid(modes_error());
     if (nepoch == 0 || nframe_ == 0) {
@@ -697,7 +697,12 @@ class Engine : public EngineBase {
     probe_terms();
     const double rho2 = p.probe_rho * p.probe_rho * scale_;
     const double pos2 = p.position_rho * p.position_rho;
-    if (pos_on_) ensure_bh_moved(size, full);
+    if (full_object()) {
+      fo_ensure(cap_);
+      ensure_bh_sampled(size);
+    } else if (pos_on_) {
+      ensure_bh_moved(size, full);
+    }
     std::vector<std::pair<size_t, int>> due;
     for (size_t ic = 0; ic < nepoch; ++ic) {
       PTYRS_HIP(hipMemcpyAsync(eord_.p, order_.as<int>() + ic * nframe_, sizeof(int) * nframe_,
@@ -710,6 +715,11 @@ class Engine : public EngineBase {
         for (auto& s : spans[ic])
           bh_batch_ms(rho2, ord + s.first, s.second, double(s.second) / double(size), cg, full,
                       k++);
+      } else if (full_object()) {
+        int k = 0;
+        for (auto& s : spans[ic])
+          bh_batch_sampled(rho2, pos2, ord + s.first, s.second,
+                           double(s.second) / double(size), cg, full, k++);
       } else if (pos_on_) {
         // Under offsets the batches launch eagerly, as the other position paths.
         int k = 0;
@@ -2589,6 +2599,144 @@ class Engine : public EngineBase {
                          (const cplx<T>*)bh_gO_.as<cplx<T>>(), no, P_.as<cplx<T>>(),
                          bh_dP_.as<cplx<T>>(), (const cplx<T>*)bh_gP_.as<cplx<T>>(), np, rho2,
                          (const double*)bh_step_.as<double>(), int(cg), int(keep));
+    });
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_probe_terms<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,
+                         P_.as<cplx<T>>(), q_.as<cplx<T>>(), Wt_.as<cplx<T>>(), n, total);
+    });
+  }
+
+  // BH under FullObject: the spectra of u_O and v_O, the reads' derivatives ∂_a S(O) per slot,
+  // the nine further reads per slot (k_bh_fo_reads' inputs) and the reads' changes.
+  DevBuf bh_fs_, bh_fd_, bh_fw_, bh_fr_;
+  void ensure_bh_sampled(size_t cap) {
+    bh_fs_.ensure(sizeof(cplx<T>) * 2 * nyo_ * nxo_);
+    bh_fd_.ensure(sizeof(cplx<T>) * 2 * cap_ * n);
+    bh_fw_.ensure(sizeof(cplx<T>) * 9 * cap_ * n);
+    bh_fr_.ensure(sizeof(cplx<T>) * 5 * cap_ * n);
+    bh_gd_.ensure(sizeof(double) * 2 * cap);
+    bh_ddf_.ensure(sizeof(double) * 2 * std::max<size_t>(1, nframe_));
+  }
+  // The unnormalised spectrum of object plane `obj` into dst.
+  void fo_spectrum_to(const cplx<T>* obj, cplx<T>* dst) {
+    const long long no = (long long)(nyo_ * nxo_);
+    axis_pass<AL_PLAIN, AS_PLAIN>(
+        0, false, obj, dst,
+        rows_geom(no, int(nxo_), no, int(nxo_), int(nyo_), int(nxo_), int(nxo_)), 1);
+    axis_pass<AL_PLAIN, AS_PLAIN>(1, false, dst, dst, cols_geom(no, int(nyo_), no, int(nyo_)), 1);
+  }
+
+  // One BH batch under FullObject sampling (ptyrs/src/engine/bh.rs:96-107, :274-577; the
+  // reference GPU's bh_sampled_batch, ptyrs/src/engine/gpu/bh.rs:613-760): every slot reads
+  // S(O) (the tall windows fo_win_), the probe terms are not moved; the reads' gradient per
+  // slot is placed onto the object by the sampling's adjoint and, with refined offsets, dotted
+  // with the reads' derivatives; the Hessian's reads change by u′, v′ and o″ (k_bh_fo_reads)
+  // from the windows of the spectra of O, u_O and v_O and their derivatives.
+  void bh_batch_sampled(double rho2, double pos2, const int* ord, int nb, double frac, bool cg,
+                        bool keep, int slot) {
+    const size_t no = nyo_ * nxo_, np = nprobe_ * n;
+    const int total = int(np), K = nb * M_;
+    const bool refine = pos_refine_;
+    run(K_BIN, [&] {
+      fo_points(ord, nb);
+      fo_object_spectrum(O_.as<cplx<T>>());
+      fo_object_windows(nb);
+    });
+    pass_a(ord, nb, false, MixArgs<T>{MIX_NONE, {}, {}, T(0), T(0)}, nullptr,
+           fo_win_.as<cplx<T>>(), N, fo_tall_.as<int2>());
+    BArgs<T> ba{nullptr, nullptr, nullptr, nullptr};
+    ba.keep = bh_keep_.as<cplx<T>>();
+    pass_b<B_BH>(ord, nb, ba);
+    const int chunk = chunk_of(nb);
+    const int nchunk = (nb + chunk - 1) / chunk;
+    run(K_PASS_C, [&] {
+      hipLaunchKernelGGL((k_pass_c_chi<T, N>), dim3(SR::NBLK, M_, nchunk), dim3(SR::NT), 0,
+                         stream_, psi_.as<cplx<T>>(), fo_win_.as<cplx<T>>(), N,
+                         Wt_.as<cplx<T>>(), q_.as<cplx<T>>(), fo_tall_.as<int2>(), ord, nb, M_,
+                         chunk, 1, dP_.as<cplx<T>>(), tw_.as<cplx<T>>(), 0);
+    });
+    const cplx<T>* r = psi_.as<cplx<T>>();
+    cplx<T>* gwin = fo_dwin_.as<cplx<T>>();
+    cplx<T>* fd = bh_fd_.as<cplx<T>>();
+    run(K_OBJ_DIR, [&] {
+      hipLaunchKernelGGL(k_fo_chi_terms<T>, dim3((n + 255) / 256, nb), dim3(256), 0, stream_,
+                         gwin, (const cplx<T>*)Wt_.as<cplx<T>>(), r, M_, n);
+      if (refine) {
+        fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 1, fd, 2ll * n);
+        fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 2, fd + n, 2ll * n);
+        hipLaunchKernelGGL(k_bh_win_dot<T>, dim3(nb), dim3(THREADS), 0, stream_,
+                           (const cplx<T>*)gwin, (const cplx<T>*)fd, n, bh_gd_.as<double>());
+      }
+      fo_place_spectrum(gwin, (long long)n, nb, false);
+      fo_add_placed<AS_PLAIN>(bh_gO_.p, 1.0);
+    });
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_probe_mean<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,
+                         bh_gP_.as<cplx<T>>(), dP_.as<cplx<T>>(), nchunk, total, 1);
+    });
+    const int ndots =
+        int(std::min<size_t>(kBhDotBlocks, (no + np + THREADS - 1) / THREADS));
+    run(K_DOTS, [&] {
+      hipLaunchKernelGGL(k_bh_dots<T>, dim3(ndots), dim3(THREADS), 0, stream_,
+                         (const cplx<T>*)bh_gO_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_dO_.as<cplx<T>>(), no,
+                         (const cplx<T>*)bh_gP_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_dP_.as<cplx<T>>(), np, rho2, int(cg),
+                         bh_dots_.as<double>());
+    });
+    // The reads along u and v: windows of the spectra of u_O, v_O and O.
+    cplx<T>* fw = bh_fw_.as<cplx<T>>();
+    cplx<T>* su = bh_fs_.as<cplx<T>>();
+    cplx<T>* sv = su + no;
+    const long long st9 = 9ll * n;
+    run(K_HESS_A, [&] {
+      fo_spectrum_to(bh_gO_.as<cplx<T>>(), su);
+      fo_windows<AS_PLAIN>(su, nb, 0, fw, st9);
+      if (cg) {
+        fo_spectrum_to(bh_dO_.as<cplx<T>>(), sv);
+        fo_windows<AS_PLAIN>(sv, nb, 0, fw + n, st9);
+      }
+      if (refine) {
+        fo_windows<AS_PLAIN>(su, nb, 1, fw + 2 * n, st9);
+        fo_windows<AS_PLAIN>(su, nb, 2, fw + 3 * n, st9);
+        if (cg) {
+          fo_windows<AS_PLAIN>(sv, nb, 1, fw + 4 * n, st9);
+          fo_windows<AS_PLAIN>(sv, nb, 2, fw + 5 * n, st9);
+        }
+        for (int d = 3; d <= 5; ++d)
+          fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, d, fw + (3 + d) * n, st9);
+      }
+      auto kr = cg ? k_bh_fo_reads<T, 2> : k_bh_fo_reads<T, 1>;
+      hipLaunchKernelGGL(kr, dim3((n + 255) / 256, nb), dim3(256), 0, stream_,
+                         bh_fr_.as<cplx<T>>(), (const cplx<T>*)fw, (const cplx<T>*)fd,
+                         refine ? (const double*)bh_gd_.as<double>() : nullptr, pos2,
+                         (const double*)bh_ddf_.as<double>(), ord, n);
+      using HA = HessShape<T, N, 2>;
+      auto ha = cg ? k_bh_hess_a_fo<T, N, 2> : k_bh_hess_a_fo<T, N, 1>;
+      hipLaunchKernelGGL(ha, dim3(HA::NBLK, K), dim3(HA::NT), 0, stream_,
+                         (const cplx<T>*)fo_win_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_fr_.as<cplx<T>>(), (const cplx<T>*)Wt_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_gP_.as<cplx<T>>(), T(rho2),
+                         (const cplx<T>*)bh_dP_.as<cplx<T>>(), (const cplx<T>*)q_.as<cplx<T>>(),
+                         M_, r, bh_l_.as<cplx<T>>(), bh_pa_.as<double>(),
+                         (const cplx<T>*)tw_.as<cplx<T>>());
+    });
+    const T norm = T(1) / (angular_ ? T(n) * T(n) : T(n));
+    bh_hess_b_and_step(ord, nb, cg, norm, ndots, frac, slot,
+                       refine ? (const double*)bh_gd_.as<double>() : nullptr, refine ? nb : 0,
+                       pos2);
+    run(K_OBJ_UPDATE, [&] {
+      const size_t all = no + np;
+      hipLaunchKernelGGL(k_bh_update<T>, dim3(unsigned((all + 255) / 256)), dim3(256), 0,
+                         stream_, O_.as<cplx<T>>(), bh_dO_.as<cplx<T>>(),
+                         (const cplx<T>*)bh_gO_.as<cplx<T>>(), no, P_.as<cplx<T>>(),
+                         bh_dP_.as<cplx<T>>(), (const cplx<T>*)bh_gP_.as<cplx<T>>(), np, rho2,
+                         (const double*)bh_step_.as<double>(), int(cg), int(keep));
+      if (refine)
+        hipLaunchKernelGGL(k_bh_update_pos, dim3((nb + 255) / 256), dim3(256), 0, stream_,
+                           offs_.as<double2>(), bh_ddf_.as<double>(),
+                           (const double*)bh_gd_.as<double>(), ord, nb, pos2,
+                           (const double*)bh_step_.as<double>(), int(cg), int(keep));
     });
     run(K_PROBE, [&] {
       hipLaunchKernelGGL(k_probe_terms<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,

@@ -42,8 +42,9 @@ enum AxisLoad { AL_PLAIN = 0, AL_PHASE = 1, AL_ABS2 = 2 };
 enum AxisStore { AS_PLAIN = 0, AS_REAL_FLOOR = 1, AS_ADD = 2 };
 
 // AL_PHASE: the loaded spectrum value times exp(+2πi(f_y·t_y + f_x·t_x)) of plane p's point
-// pts[p] (and by 2πi·f_a for a derivative along axis a: deriv 1 for y, 2 for x), its bins
-// (jy, jx) numpy's fftfreq of an ny × nx plane; `cols` says the lines are columns.
+// pts[p] (and by 2πi·f_a for a derivative along axis a: deriv 1 for y, 2 for x; by
+// (2πi)²·f_a·f_c for a second derivative: 3 yy, 4 yx, 5 xx), its bins (jy, jx) numpy's fftfreq
+// of an ny × nx plane; `cols` says the lines are columns.
 struct PhaseArgs {
   const double2* pts;
   int ny, nx, deriv, cols;
@@ -97,8 +98,16 @@ __global__ __launch_bounds__(kAxisThreads) void k_axis_fft(const cplx<T>* __rest
         const int jy = ph.cols ? i : line, jx = ph.cols ? line : i;
         double2 e = sample_phase(ph.pts[p], jy, jx, ph.ny, ph.nx);
         if (ph.deriv) {
-          const double w = 2.0 * M_PI * (ph.deriv == 1 ? fftfreq(jy, ph.ny) : fftfreq(jx, ph.nx));
-          e = make_double2(-e.y * w, e.x * w);  // times i·w
+          const double fy = fftfreq(jy, ph.ny), fx = fftfreq(jx, ph.nx);
+          if (ph.deriv <= 2) {
+            const double w = 2.0 * M_PI * (ph.deriv == 1 ? fy : fx);
+            e = make_double2(-e.y * w, e.x * w);  // times i·w
+          } else {
+            // second derivatives (3: yy, 4: yx, 5: xx): times (2πi)²·f_a·f_c
+            const double w = -4.0 * M_PI * M_PI *
+                             (ph.deriv == 3 ? fy * fy : ph.deriv == 4 ? fy * fx : fx * fx);
+            e = make_double2(e.x * w, e.y * w);
+          }
         }
         v = cplx<T>{T(double(v.x) * e.x - double(v.y) * e.y),
                     T(double(v.x) * e.y + double(v.y) * e.x)};

@@ -5,7 +5,8 @@ pin (tests/test_oracle_bh.py): the seeded far-field scenario with one batch of a
 step's β, α, ⟨g, D⟩ and H(D, D), the object and probe changes), the near-field cases (angular
 spectrum, Fresnel forward and back, an evanescent band) against the oracle, LLK records, graph
 replays against eager launches, refined PATCH position offsets against the reference's
-bh_pos_{full,mini}_* and held ones against the oracle, and the refusals. complex128 to FP64
+bh_pos_{full,mini}_* and under FullObject sampling against pty-chi's bh_full_object_*, held
+ones against the oracle, and the refusals. complex128 to FP64
 rounding (1e-9), complex64 to 1e-3 of the steps (1e-2 of the changes)."""
 
 import numpy as np
@@ -146,32 +147,32 @@ def test_refusals():
         g.bh(1, 0, 0, 0.01, 7)
     with pytest.raises(ValueError):
         g.bh(1, 0, 16, float("nan"), 7)
-    g.set_features(position=np.zeros((g.nframe, 2)))
-    g.set_sampling("full_object")
-    with pytest.raises(ValueError, match="FullObject"):
-        g.bh(1, 0, 16, 0.01, 7)
     assert g.cycle == 0 and g.bh_steps() == []
-    g.set_features()
     g.bh(0, 0, 16, 0.01, 7)
     assert g.bh_steps() == []
 
 
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
-@pytest.mark.parametrize("batch_size,name", [(40, "bh_pos_full"), (16, "bh_pos_mini")])
-def test_patch_offsets_match_fixtures(dtype, batch_size, name):
-    # Refined PATCH offsets (ptyrs/tests/engine_bh_pos.rs:61-162): probe ρ 0.01, position ρ 2,
-    # seed 7, two epochs from the fixture's offsets: each epoch's steps, the object, probe and
-    # offset changes over the run.
+@pytest.mark.parametrize("batch_size,name,sampling", [
+    (40, "bh_pos_full", "patch"), (16, "bh_pos_mini", "patch"),
+    (40, "bh_full_object_full", "full_object"), (16, "bh_full_object_mini", "full_object")])
+def test_offsets_match_fixtures(dtype, batch_size, name, sampling):
+    # Refined offsets under PATCH (the reference's model) and FullObject sampling (pty-chi's
+    # FULL_OBJECT BH) (ptyrs/tests/engine_bh_pos.rs:61-184): probe ρ 0.01, position ρ 2, seed 7,
+    # two epochs from the fixture's offsets: each epoch's steps, the object, probe and offset
+    # changes over the run.
     rows = np.array(table(f"{name}_offsets_f64.txt"))
     d0 = rows[:, 1:3].copy()
     steps = table(f"{name}_steps_f64.txt")
     tol = TOL[dtype]
     g = engine(2, dtype)
     g.set_features(position=d0)
+    g.set_sampling(sampling)
     g.bh(1, 0, batch_size, 0.01, 7, position_rho=2.0)
     check_steps(g.bh_steps(), [r for r in steps if r[1] == 0], tol)
     g = engine(2, dtype)
     g.set_features(position=d0)
+    g.set_sampling(sampling)
     obj0, probe0 = g.object, g.probe
     g.bh(2, 1, batch_size, 0.01, 7, position_rho=2.0)
     check_steps(g.bh_steps(), [r for r in steps if r[1] == 1], tol)
