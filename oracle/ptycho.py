@@ -276,14 +276,32 @@ class Ptycho:
         """2·∂L/∂O* and 2·∂L/∂P*; under the offsets the back-projections through the moved
         waves (`Shifted::accumulate_obj` / `accumulate_probe`,
         ptyrs/src/engine/features.rs:865-958). ptyrs/src/engine/ptycho.rs:492-513"""
-        self._patch_only("ML")
         g = self.grad_psi()
+        sm = self._sampler(self.views_all)
+        if sm is not None:
+            return self._sampled_grads(sm, g, obj, probe)
         if self.offsets is not None:
             return self._moved_grads(g, obj, probe)
         go = (accumulate_obj(self.views_all, self.obj.shape, self.probe, self.quad, g)[0]
               if obj else np.zeros_like(self.obj))
         gp = (accumulate_probe(self.views_all, self.obj, self.probe.shape, self.quad, g)[0]
               if probe else np.zeros_like(self.probe))
+        return go, gp
+
+    def _sampled_grads(self, sm, g, obj, probe):
+        """Under FullObject: the object's Σ g·conj(W) on each view's read placed by the
+        sampling's adjoint, the probe's Σ conj(S(O)·q)·g (W = P·q unmoved).
+        ptyrs/src/engine/features.rs:865-958"""
+        views = self.views_all
+        gn = to_natural(g)
+        q = self.quad
+        w = np.stack([self.probe[v[2]] * q for v in views])
+        go = sm.place(gn * np.conj(w), self.obj.shape[0]) if obj else np.zeros_like(self.obj)
+        gp = np.zeros_like(self.probe)
+        if probe:
+            win = sm.windows(self.obj)
+            for k, v in enumerate(views):
+                gp[v[2]] += gn[k] * np.conj(win[k] * q)
         return go, gp
 
     def _moved_grads(self, g, obj, probe):
@@ -315,7 +333,14 @@ class Ptycho:
         """ptyrs/src/engine/ptycho.rs:515-545; under the offsets the probe and d_probe terms
         are their moved waves."""
         v, q = self.views_all, self.quad
-        if self.offsets is not None:
+        sm = self._sampler(v)
+        if sm is not None:
+            w = np.stack([self.probe[x[2]] * q for x in v])
+            dw = np.stack([d_probe[x[2]] * q for x in v])
+            win, dwin = sm.windows(self.obj), sm.windows(d_obj)
+            po, a, b = (to_shifted(w * win), to_shifted(w * dwin + dw * win),
+                        to_shifted(dw * dwin))
+        elif self.offsets is not None:
             ny, nx = self.probe.shape[-2:]
             w = self.moved_waves(v)
             keep = self.probe
@@ -338,7 +363,6 @@ class Ptycho:
         """Per frame [num, den] of the position step from the magnitude projection of the
         current estimate's exit waves (`position_sums` of them).
         ptyrs/src/engine/ptycho.rs:468-479"""
-        self._patch_only("ML")
         x = self.optics.exit_forward(self._moved_psi(self.views_all)[1])
         x = apply_amplitude(self.iobs, self.addr.nmode, x)
         return self.position_sums(self.optics.exit_adjoint(x))
@@ -352,6 +376,20 @@ class Ptycho:
         views = self.views_all
         w = self.moved_waves(views)
         x = to_natural(x)
+        sm = self._sampler(views)
+        if sm is not None:
+            # FullObject: ∂psi = W·∂_a S(O), the sampled windows' derivatives
+            # (ptyrs/src/engine/features.rs:978-985)
+            win = sm.windows(self.obj)
+            dwin = [sm.windows(self.obj, (a,)) for a in range(2)]
+            sums = np.zeros((self.addr.nframe, 2, 2))
+            for k, v in enumerate(views):
+                c = x[k] - w[k] * win[k]
+                for axis in range(2):
+                    dpsi = w[k] * dwin[axis][k]
+                    sums[v[0], 0, axis] += float((np.conj(c) * dpsi).real.sum())
+                    sums[v[0], 1, axis] += float((dpsi.real ** 2 + dpsi.imag ** 2).sum())
+            return sums
         spec = np.fft.fft2(self.probe * self.quad)
         fy = freq(np.arange(ny), ny)[:, None] * np.ones((1, nx))
         fx = np.ones((ny, 1)) * freq(np.arange(nx), nx)[None, :]

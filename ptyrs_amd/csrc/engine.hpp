@@ -1135,14 +1135,15 @@ class Engine : public EngineBase {
   void ml(const ptyrs_ml_params& p, size_t ncycle) override {
     if (nslice_ > 1)
       throw Invalid("ML runs one object slice; only LSQML and the LLK pass run several");
-    if (pos_on_ && sampling != PTYRS_PATCH)
-      throw Invalid("ML with position offsets under FullObject sampling is not run on this "
-                    "GPU engine; set the PATCH sampling");
     unitary("ML");
     if (nobj_ != 1) throw Invalid("the GPU ML runs one object mode");
     if (M_ > kMaxModes) throw Invalid(modes_error());
     if (nframe_ == 0) {
       cycle += ncycle;
+      return;
+    }
+    if (full_object()) {
+      ml_sampled(p, ncycle);
       return;
     }
     if (pos_on_) {
@@ -1556,6 +1557,202 @@ class Engine : public EngineBase {
                              (const double*)pos_all_.as<double>(), xb, int(nframe_),
                              offs_.as<double2>(), pstep_.as<double>());
       });
+      });
+      ++cycle;
+    }
+    drain(due, PTYRS_ML);
+    if (p.zero_phase_ramp) device_zero_phase_ramp();
+    synchronize();
+    flush_timing();
+  }
+
+  // PyNX ML under FullObject sampling (ptyrs/src/engine/ptycho.rs:757-842 with Shifted's
+  // sampler; the reference GPU's ml under its position model, ptyrs/src/engine/gpu/ml.rs:1-160):
+  // the probe terms are not moved; every frame reads its object window sampled at its corner
+  // plus offset (the tall windows of a stack, from the object's spectrum once per cycle); the
+  // object gradient places each slot's Σ conj(W)·g by the sampling's adjoint, the probe
+  // gradient sums conj(S·q)·g; γ's a and b read the direction's sampled windows; with
+  // refinement the position sums take the windows' derivatives against W.
+  DevBuf fo_dd_, fo_dspec_;
+  void ml_sampled(const ptyrs_ml_params& p, size_t ncycle) {
+    const bool uo = p.update_object != 0, up = p.update_probe != 0;
+    const size_t stack = std::min<size_t>(nframe_, stack_size_ ? stack_size_ : kDefaultStack);
+    ensure_workspace(stack, 1);
+    fo_ensure(stack);
+    const size_t no = nyo_ * nxo_, np = nprobe_ * n;
+    const int xb = aux_blocks();
+    if (pos_refine_) {
+      fo_dd_.ensure(sizeof(cplx<T>) * 2 * stack * n);
+      pos_all_.ensure(sizeof(double) * 4 * nframe_ * xb);
+      pstep_.ensure(sizeof(double) * 4 * nframe_);
+    }
+    fo_dspec_.ensure(sizeof(cplx<T>) * no);
+    psi3_.ensure(sizeof(cplx<T>) * 3 * stack * M_ * n);
+    for (DevBuf* b : {&gO_, &hO_, &dOc_}) b->ensure(sizeof(cplx<T>) * no);
+    for (DevBuf* b : {&gP_, &hP_, &dPc_, &dWt_}) b->ensure(sizeof(cplx<T>) * np);
+    cg_part_.ensure(sizeof(double) * 2 * kAbsBlocks);
+    cg_sums_.ensure(sizeof(double) * 2);
+    gam_part_.ensure(sizeof(double) * 2 * nframe_ * SC::NBLK);
+    gam_.ensure(sizeof(double) * 2);
+    abs_part_.ensure(sizeof(double) * 2 * kAbsBlocks);
+    const int nstack = int((nframe_ + stack - 1) / stack);
+    PTYRS_HIP(hipMemsetAsync(dOc_.p, 0, sizeof(cplx<T>) * no, stream_));
+    PTYRS_HIP(hipMemsetAsync(dPc_.p, 0, sizeof(cplx<T>) * np, stream_));
+    PTYRS_HIP(hipMemsetAsync(gO_.p, 0, sizeof(cplx<T>) * no, stream_));
+    PTYRS_HIP(hipMemsetAsync(gP_.p, 0, sizeof(cplx<T>) * np, stream_));
+    probe_terms();
+    // ScaleObjProbe(absolute=True) from the model photons of the start.
+    llk_pass(0);
+    balance(1.0, hist_.as<double>() + 3);
+    const int grid_o = int(std::min<size_t>(kAbsBlocks, (no + THREADS - 1) / THREADS));
+    const int grid_p = int(std::min<size_t>(kAbsBlocks, (np + THREADS - 1) / THREADS));
+    const MixArgs<T> none{MIX_NONE, {}, {}, T(0), T(0)};
+    cplx<T>* win = fo_win_.as<cplx<T>>();
+    std::vector<std::pair<size_t, int>> due;
+    for (size_t ic = 0; ic < ncycle; ++ic) {
+      const bool rec = p.calc_llk > 0 && (ic % p.calc_llk == 0 || ic + 1 == ncycle);
+      if (rec) {
+        if (due.size() == hist_slots_) drain(due, PTYRS_ML);
+        const int slot = int(due.size());
+        llk_pass(slot);
+        due.push_back({cycle, slot});
+      }
+      run(K_BIN, [&] { fo_object_spectrum(O_.as<cplx<T>>()); });
+      // Position sums of the magnitude projection of the starting estimate.
+      if (pos_refine_) {
+        for (int s = 0; s < nstack; ++s) {
+          const int f0 = int(s * stack);
+          const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+          const int* ord = iota_.as<int>() + f0;
+          const int K = nb * M_;
+          run(K_BIN, [&] {
+            fo_points(ord, nb);
+            fo_object_windows(nb);
+          });
+          pass_a(ord, nb, false, none, nullptr, win, N, fo_tall_.as<int2>());
+          pass_b<B_AMPLITUDE>(ord, nb);
+          run(K_OTHER, [&] {
+            rows(true, K);
+            fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 1, fo_dd_.p, 2ll * n);
+            fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 2, fo_dd_.as<cplx<T>>() + n,
+                                 2ll * n);
+            hipLaunchKernelGGL(k_pos_sums_fo<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
+                               (const cplx<T>*)psi_.as<cplx<T>>(),
+                               (const cplx<T>*)Wt_.as<cplx<T>>(), (const cplx<T>*)win,
+                               (const cplx<T>*)fo_dd_.as<cplx<T>>(), n, M_,
+                               pos_all_.as<double>() + size_t(f0) * xb * 4);
+          });
+        }
+      }
+      // Gradients: 2·∂L/∂ψ* through the sampled windows.
+      for (int s = 0; s < nstack; ++s) {
+        const int f0 = int(s * stack);
+        const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+        const int* ord = iota_.as<int>() + f0;
+        const int K = nb * M_;
+        run(K_BIN, [&] {
+          fo_points(ord, nb);
+          fo_object_windows(nb);
+        });
+        pass_a(ord, nb, false, none, nullptr, win, N, fo_tall_.as<int2>());
+        pass_b<B_GRAD>(ord, nb);
+        const cplx<T>* g = psi_.as<cplx<T>>();
+        run(K_PASS_C, [&] { rows(true, K); });
+        if (up)
+          run(K_PROBE, [&] {
+            hipLaunchKernelGGL(k_ms_probe_dir<T>, dim3((n + 255) / 256, M_), dim3(256), 0,
+                               stream_, gP_.as<cplx<T>>(), (const cplx<T>*)win, N,
+                               (const cplx<T>*)q_.as<cplx<T>>(), g, fo_tall_.as<int2>(), ord, nb,
+                               N, M_, T(1), int(s > 0));
+          });
+        if (uo)
+          run(K_OBJ_UPDATE, [&] {
+            hipLaunchKernelGGL(k_fo_chi_terms<T>, dim3((n + 255) / 256, nb), dim3(256), 0,
+                               stream_, fo_dwin_.as<cplx<T>>(), (const cplx<T>*)Wt_.as<cplx<T>>(),
+                               g, M_, n);
+            fo_place_spectrum(fo_dwin_.as<cplx<T>>(), (long long)n, nb, s > 0);
+          });
+      }
+      if (uo) run(K_OBJ_UPDATE, [&] { fo_add_placed<AS_PLAIN>(gO_.p, 1.0); });
+      // Polak–Ribière β over both arrays, then the direction; h keeps this cycle's g.
+      run(K_OTHER, [&] {
+        if (ic > 0) {
+          zero(cg_sums_.as<double>(), 2);
+          if (uo) {
+            hipLaunchKernelGGL(k_cg_dots<T>, dim3(grid_o), dim3(THREADS), 0, stream_,
+                               gO_.as<cplx<T>>(), hO_.as<cplx<T>>(), no, cg_part_.as<double>());
+            hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(THREADS), 0, stream_,
+                               cg_part_.as<double>(), grid_o, cg_sums_.as<double>(), 1);
+          }
+          if (up) {
+            hipLaunchKernelGGL(k_cg_dots<T>, dim3(grid_p), dim3(THREADS), 0, stream_,
+                               gP_.as<cplx<T>>(), hP_.as<cplx<T>>(), np, cg_part_.as<double>());
+            hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(THREADS), 0, stream_,
+                               cg_part_.as<double>(), grid_p, cg_sums_.as<double>(), 1);
+          }
+        }
+        if (uo)
+          hipLaunchKernelGGL(k_cg_direction<T>, dim3((no + 255) / 256), dim3(256), 0, stream_,
+                             dOc_.as<cplx<T>>(), hO_.as<cplx<T>>(), gO_.as<cplx<T>>(), no,
+                             cg_sums_.as<double>(), int(ic == 0));
+        if (up)
+          hipLaunchKernelGGL(k_cg_direction<T>, dim3((np + 255) / 256), dim3(256), 0, stream_,
+                             dPc_.as<cplx<T>>(), hP_.as<cplx<T>>(), gP_.as<cplx<T>>(), np,
+                             cg_sums_.as<double>(), int(ic == 0));
+        hipLaunchKernelGGL(k_probe_terms<T>, dim3((np + 255) / 256), dim3(256), 0, stream_,
+                           dPc_.as<cplx<T>>(), q_.as<cplx<T>>(), dWt_.as<cplx<T>>(), n, int(np));
+      });
+      // Newton step γ along d: the three fields read S(O) and S(dO).
+      run(K_OTHER, [&] { fo_spectrum_to(dOc_.as<cplx<T>>(), fo_dspec_.as<cplx<T>>()); });
+      const size_t field = stack * M_ * n;
+      for (int s = 0; s < nstack; ++s) {
+        const int f0 = int(s * stack);
+        const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+        const int* ord = iota_.as<int>() + f0;
+        run(K_BIN, [&] {
+          fo_points(ord, nb);
+          fo_object_windows(nb);
+          fo_windows<AS_PLAIN>(fo_dspec_.as<cplx<T>>(), nb, 0, fo_dwin_.p, (long long)n);
+        });
+        run(K_PASS_A, [&] {
+          hipLaunchKernelGGL((k_pass_a3<T, N>), dim3(S::NBLK, nb * M_), dim3(S::NT), 0, stream_,
+                             (const cplx<T>*)win, (const cplx<T>*)fo_dwin_.as<cplx<T>>(), N,
+                             (const cplx<T>*)Wt_.as<cplx<T>>(),
+                             (const cplx<T>*)dWt_.as<cplx<T>>(), fo_tall_.as<int2>(), ord, M_,
+                             field, psi3_.as<cplx<T>>(), tw_.as<cplx<T>>(), 0);
+        });
+        run(K_PASS_B, [&] {
+          if (angular_) pass_t<false>(psi3_.as<cplx<T>>(), nb * M_, 3, field);
+          const T norm = T(1) / (angular_ ? T(n) * T(n) : T(n));
+          auto go = [&](auto col, auto row) {
+            hipLaunchKernelGGL(angular_ ? row : col, dim3(SC::NBLK, nb), dim3(SC::NT), 0,
+                               stream_, psi3_.as<cplx<T>>(), field, counts_.as<float>(), ord,
+                               gam_part_.as<double>() + size_t(f0) * SC::NBLK * 2,
+                               tw_.as<cplx<T>>(), norm);
+          };
+          with_modes([&](auto mm) {
+            constexpr int MM = decltype(mm)::value;
+            go(k_pass_b_gamma<T, N, MM, false>, k_pass_b_gamma<T, N, MM, true>);
+          });
+        });
+      }
+      run(K_OTHER, [&] {
+        hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(THREADS), 0, stream_,
+                           gam_part_.as<double>(), int(nframe_ * SC::NBLK), gam_.as<double>(), 0);
+        if (uo)
+          hipLaunchKernelGGL(k_cg_step<T>, dim3((no + 255) / 256), dim3(256), 0, stream_,
+                             O_.as<cplx<T>>(), dOc_.as<cplx<T>>(), gO_.as<cplx<T>>(), no,
+                             gam_.as<double>());
+        if (up)
+          hipLaunchKernelGGL(k_cg_step<T>, dim3((np + 255) / 256), dim3(256), 0, stream_,
+                             P_.as<cplx<T>>(), dPc_.as<cplx<T>>(), gP_.as<cplx<T>>(), np,
+                             gam_.as<double>());
+        hipLaunchKernelGGL(k_probe_terms<T>, dim3((np + 255) / 256), dim3(256), 0, stream_,
+                           P_.as<cplx<T>>(), q_.as<cplx<T>>(), Wt_.as<cplx<T>>(), n, int(np));
+        if (pos_refine_)
+          hipLaunchKernelGGL(k_pos_step_all, dim3(1), dim3(1024), 0, stream_,
+                             (const double*)pos_all_.as<double>(), xb, int(nframe_),
+                             offs_.as<double2>(), pstep_.as<double>());
       });
       ++cycle;
     }

@@ -282,6 +282,38 @@ __global__ void k_fo_chi_terms(cplx<T>* __restrict__ out, const cplx<T>* __restr
   st(out + (size_t)b * n + j, s);
 }
 
+// Per slot and workgroup the position sums of AP's, DM's and ML's step under FullObject
+// (ptyrs/src/engine/features.rs:960-1025 with `derivatives` of a sampled window): over the
+// slot's views (every mode) num_a = Re Σ conj(x − W·S)·W·∂_a S and den_a = Σ |W·∂_a S|², x the
+// projected exit waves ([slot·M + m][n], natural), W = P·q ([m][n]), S the slot's read
+// ([slot][n]) and ∂_a S its derivatives ([slot·2 + a][n]). part[slot][blk] = {num_y, num_x,
+// den_y, den_x}. Grid (nblk, nb).
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_pos_sums_fo(
+    const cplx<T>* __restrict__ x, const cplx<T>* __restrict__ W, const cplx<T>* __restrict__ win,
+    const cplx<T>* __restrict__ dwin, int n, int M, double* __restrict__ part) {
+  __shared__ double red[THREADS];
+  const int b = blockIdx.y;
+  double s[4] = {0, 0, 0, 0};
+  for (int j = blockIdx.x * THREADS + threadIdx.x; j < n; j += gridDim.x * THREADS) {
+    const cplx<T> o = ld(win + (size_t)b * n + j);
+    const cplx<T> dy = ld(dwin + ((size_t)b * 2) * n + j), dx = ld(dwin + ((size_t)b * 2 + 1) * n + j);
+    for (int m = 0; m < M; ++m) {
+      const cplx<T> w = ld(W + (size_t)m * n + j);
+      const cplx<T> r = ld(x + ((size_t)b * M + m) * n + j) - w * o;
+      const cplx<T> uy = w * dy, ux = w * dx;
+      s[0] += double(cmulc(r, uy).x);
+      s[1] += double(cmulc(r, ux).x);
+      s[2] += double(norm2(uy));
+      s[3] += double(norm2(ux));
+    }
+  }
+  for (int i = 0; i < 4; ++i) {
+    const double t = block_reduce<THREADS>(s[i], red, AddOp());
+    if (threadIdx.x == 0) part[((size_t)b * gridDim.x + blockIdx.x) * 4 + i] = t;
+  }
+}
+
 // dO /= √(pre² + floor) (LSQML's preconditioned object direction).
 template <typename T>
 __global__ void k_fo_precondition(cplx<T>* __restrict__ dO, const T* __restrict__ pre,

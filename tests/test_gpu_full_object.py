@@ -1,4 +1,4 @@
-"""rPIE, LSQML, the LLK and the model intensities under FullObject sampling on the GPU
+"""rPIE, ML, LSQML, the LLK and the model intensities under FullObject sampling on the GPU
 (pty-chi's FULL_OBJECT shift operator: each view's window Fourier-interpolated from the whole
 object plane at its corner plus its frame's offset, ptyrs/src/engine/sampling.rs): rPIE against
 the reference's own fixtures on its seeded far-field scenario (`rpie_full_object_*`,
@@ -164,7 +164,7 @@ def test_full_object_refusals():
     assert g.sampling == "patch"   # far field: Sampling::auto takes PATCH
     g.set_sampling("full_object")
     g.set_features(position=np.zeros((10, 2)))
-    for call in (lambda: g.ml(1), lambda: g.ap(1), lambda: g.dm(1)):
+    for call in (lambda: g.ap(1), lambda: g.dm(1)):
         with pytest.raises(ValueError):
             call()
     with pytest.raises(ValueError):
@@ -247,3 +247,34 @@ def test_full_object_intensities_match_oracle(dtype):
     g.set_features(position=d0, hold=("position",))
     err = float(np.abs(g.intensities() - want).max() / np.abs(want).max())
     assert err <= {"c64": 2e-5, "c128": 1e-12}[dtype], err
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("nprobe,refine,up", [(1, True, True), (3, True, False), (2, False, True)])
+def test_ml_full_object_matches_oracle(dtype, nprobe, refine, up):
+    # PyNX ML under FULL_OBJECT (the oracle's: the reads' gradient placed by the sampling's
+    # adjoint, γ through the direction's sampled windows, the position sums through the
+    # windows' derivatives): far field, masked pixels, offsets within 0.45 px, three cycles.
+    d = small_scan(n=32, nprobe=nprobe, npos=25, seed=19)
+    d0 = np.random.default_rng(20).uniform(-0.45, 0.45, (25, 2))
+    d0 -= d0.mean(axis=0)
+    o = oracle(d)
+    o.set_sampling("full_object")
+    o.set_position(d0, refine=refine)
+    o.ml(3, update_object=True, update_probe=up, calc_llk=1)
+    g = gpu(d, dtype)
+    g.set_sampling("full_object")
+    g.set_features(position=d0, hold=() if refine else ("position",))
+    g.ml(3, update_object=True, update_probe=up, calc_llk=1)
+    tol = PTOL[dtype]
+    want = [r[2].poisson for r in o.history]
+    print("ml full object", dtype, nprobe, refine, up, max_rel(g.history["poisson"], want),
+          rel_l2(g.object, o.obj), rel_l2(g.probe, o.probe))
+    assert max_rel(g.history["poisson"], want) < tol
+    assert rel_l2(g.object, o.obj) < tol * 10
+    assert rel_l2(g.probe, o.probe) < tol * 10
+    got = g.features["position"]
+    if refine:
+        assert max_rel((got - d0).ravel(), (o.offsets - d0).ravel()) < tol * 100
+    else:
+        assert np.array_equal(got, d0)

@@ -340,3 +340,56 @@ def test_moved_probe_step_minimises_along_its_direction():
         s = step.copy()
         s[m] *= 1.1
         assert cost(p0 + s) > j1
+
+
+def test_ml_full_object_zero_offsets_is_plain_ml():
+    # Under FullObject the views read the object sampled at corner plus offset; at zero offsets
+    # the samples are the windows themselves, so ML is the plain run to rounding
+    # (ptyrs/src/engine/features.rs:865-958, :978-985).
+    a, b = fresh(2), fresh(2)
+    b.set_position(np.zeros((len(b.iobs), 2)), refine=False)
+    b.set_sampling("full_object")
+    for p in (a, b):
+        p.ml(3, update_object=True, update_probe=True, calc_llk=1, zero_phase_ramp=False)
+    for x, y in ((b.obj, a.obj), (b.probe, a.probe)):
+        assert np.abs(x - y).max() < 1e-10 * np.abs(y).max()
+    assert max_rel([r[2].poisson for r in b.history], [r[2].poisson for r in a.history]) < 1e-12
+
+
+def test_ml_full_object_gradient_matches_central_differences():
+    # The gradients under FullObject (the reads' gradient placed by the sampling's adjoint, the
+    # probe's through the sampled windows) against central differences of the Poisson LLK sum.
+    from oracle.ptycho import llk
+    p = fresh(2)
+    d0 = np.random.default_rng(12).uniform(-0.4, 0.4, (len(p.iobs), 2))
+    p.set_position(d0, refine=False)
+    p.set_sampling("full_object")
+    go, gp = p.grads(True, True)
+    rng = np.random.default_rng(13)
+    do = (rng.normal(size=p.obj.shape) + 1j * rng.normal(size=p.obj.shape)) * 0.01
+    dp = (rng.normal(size=p.probe.shape) + 1j * rng.normal(size=p.probe.shape)) * 0.01 * \
+        np.abs(p.probe).max()
+
+    def cost(t):
+        o0, p0 = p.obj, p.probe
+        p.obj, p.probe = o0 + t * do, p0 + t * dp
+        psi = p.optics.exit_forward(p.calc_psi())
+        p.obj, p.probe = o0, p0
+        return llk(p.iobs, p.addr.nmode, psi).poisson
+
+    h = 1e-5
+    fd = (cost(h) - cost(-h)) / (2 * h)
+    an = float((np.conj(go) * do).real.sum() + (np.conj(gp) * dp).real.sum())
+    assert abs(an - fd) < 1e-6 * abs(fd)
+
+
+def test_ml_full_object_refines_offsets_toward_the_truth():
+    p = fresh(2)
+    d0 = np.random.default_rng(14).uniform(-0.3, 0.3, (len(p.iobs), 2))
+    d0 -= d0.mean(axis=0)
+    p.set_position(d0.copy(), refine=True)
+    p.set_sampling("full_object")
+    p.ml(6, update_object=True, update_probe=True, calc_llk=1, zero_phase_ramp=False)
+    assert np.sqrt((p.offsets ** 2).mean()) < np.sqrt((d0 ** 2).mean())
+    llks = [r[2].poisson for r in p.history]
+    assert llks[-1] < llks[0]
