@@ -1,8 +1,9 @@
 """PyNX operators and the AP, DM, RAAR, ML and rPIE loops of the reference's CPU engine in
 FP64. Of the features only the position offsets are modelled: under PATCH sampling (the probe
-moved by each frame's sub-pixel offset in Fourier space) in the LLK, rPIE and ML; under
-FullObject sampling (each view's window Fourier-interpolated at its corner plus the offset,
-`sampling.py`) in the LLK and rPIE. No scale or background.
+moved by each frame's sub-pixel offset in Fourier space) and under FullObject sampling (each
+view's window Fourier-interpolated at its corner plus the offset, `sampling.py`; AP's, DM's
+and RAAR's object by the PCG solve `solve_obj`) in the LLK and every algorithm. No scale or
+background.
 
 Test infrastructure (see oracle/__init__.py). Layout: exit waves `psi[k]` shifted, `k` in
 table order; objects `[nobj, nyo, nxo]`, probes `[nprobe, ny, nx]`, natural layout.
@@ -142,6 +143,44 @@ def psi_to_obj_probe(views, quad, psi, obj, probe, obj_inertia, probe_inertia):
     if obj_inertia is not None:
         obj = blend(obj, obj_acc[0], obj_acc[1], obj_inertia)
     return obj, probe
+
+
+OBJ_PCG = 10      # ptyrs/src/engine/sampling.rs:214
+OBJ_HELD = 1e-5   # ptyrs/src/engine/sampling.rs:221
+
+
+def solve_obj(sm, w, psi_n, obj, inertia):
+    """Under FullObject the object of min ‖psi − A·O'‖² + reg·‖O' − O‖², A the views' waves
+    `w` (natural) times their reads through the sampler `sm`: OBJ_PCG preconditioned
+    conjugate-gradient steps from `obj`, preconditioner D + reg with D = sm.diag(|w|²) and
+    reg = inertia·max D, pixels with D + reg ≤ OBJ_HELD·max D held (a zero share of every
+    step); a step along no positive curvature, or after a residual of no positive norm, is 0.
+    ptyrs/src/engine/features.rs:726-793"""
+    nobj = obj.shape[0]
+    d = sm.diag(w.real ** 2 + w.imag ** 2, nobj)
+    mx = float(d.max())
+    reg = mx * inertia
+    pre = np.where(d + reg > OBJ_HELD * mx, d + reg, np.inf)
+
+    def normal(x):
+        return sm.place(np.conj(w) * w * sm.windows(x), nobj)
+
+    r = sm.place(np.conj(w) * psi_n, nobj) - normal(obj)
+    rz = float((np.abs(r) ** 2 / pre).sum())
+    p = r / pre
+    x = obj.copy()
+    for step in range(OBJ_PCG):
+        ap = normal(p) + p * reg
+        pap = float((np.conj(p) * ap).real.sum())
+        alpha = rz / pap if pap > 0.0 else 0.0
+        x = x + p * alpha
+        r = r - ap * alpha
+        if step + 1 < OBJ_PCG:
+            nxt = float((np.abs(r) ** 2 / pre).sum())
+            beta = nxt / rz if rz > 0.0 else 0.0
+            p = r / pre + p * beta
+            rz = nxt
+    return x
 
 
 def llk_due(calc_llk, ic, ncycle):
@@ -424,13 +463,51 @@ class Ptycho:
     def _update_from_psi(self, obj_inertia, probe_inertia):
         """psi_to_obj_probe of self.psi, through the moved waves under the offsets.
         ptyrs/src/engine/ptycho.rs:387-410"""
-        if self.offsets is not None:
-            self._patch_only("AP, DM and RAAR")
+        if self.offsets is not None and self.sampling == "full_object":
+            self._sampled_psi_to_obj_probe(obj_inertia, probe_inertia)
+        elif self.offsets is not None:
             self._moved_psi_to_obj_probe(obj_inertia, probe_inertia)
         else:
             self.obj, self.probe = psi_to_obj_probe(self.views_all, self.quad, self.psi,
                                                     self.obj, self.probe, obj_inertia,
                                                     probe_inertia)
+
+    def _sampled_psi_to_obj_probe(self, obj_inertia, probe_inertia):
+        """psi_to_obj_probe under FullObject, both from the arrays on entry: the probe's first
+        conjugate-gradient step through the sampled windows S(O) (the waves are not moved, so
+        it is `blend` to rounding), then the object solve (`solve_obj`): OBJ_PCG = 10
+        conjugate-gradient steps on (AᴴA + reg)·O' = Aᴴ·psi + reg·O from O, A the views' waves
+        times their reads, preconditioned by D + reg (D the exact diagonal of AᴴA,
+        Sampler.diag; reg = inertia·max D), pixels with D + reg ≤ 1e-5·max D held.
+        ptyrs/src/engine/features.rs:653-793"""
+        views = self.views_all
+        sm = self._sampler(views)
+        q = self.quad
+        psi_n = to_natural(self.psi)
+        w = np.stack([self.probe[v[2]] * q for v in views])
+        win = sm.windows(self.obj)
+        new_probe = self.probe
+        if probe_inertia is not None:
+            resid = psi_n - w * win
+            num = np.zeros_like(self.probe)
+            norm = np.zeros(self.probe.shape)
+            for k, v in enumerate(views):
+                num[v[2]] += np.conj(win[k] * q) * resid[k]
+                norm[v[2]] += win[k].real ** 2 + win[k].imag ** 2
+            reg = float(norm.max()) * probe_inertia
+            d = np.maximum(norm + reg, 1e-12)
+            slope = [float((np.abs(num[m]) ** 2 / d[m]).sum()) for m in range(num.shape[0])]
+            step = num / d
+            curv = [reg * float((np.abs(step[m]) ** 2).sum()) for m in range(num.shape[0])]
+            for k, v in enumerate(views):
+                curv[v[2]] += float((np.abs(step[v[2]] * q * win[k]) ** 2).sum())
+            new_probe = self.probe.copy()
+            for m in range(num.shape[0]):
+                if slope[m] > 0.0:
+                    new_probe[m] = new_probe[m] + step[m] * (slope[m] / curv[m])
+        if obj_inertia is not None:
+            self.obj = solve_obj(sm, w, psi_n, self.obj, obj_inertia)
+        self.probe = new_probe
 
     def _moved_psi_to_obj_probe(self, obj_inertia, probe_inertia):
         """psi_to_obj_probe under PATCH offsets, both from the arrays on entry: the probe takes

@@ -393,3 +393,65 @@ def test_ml_full_object_refines_offsets_toward_the_truth():
     assert np.sqrt((p.offsets ** 2).mean()) < np.sqrt((d0 ** 2).mean())
     llks = [r[2].poisson for r in p.history]
     assert llks[-1] < llks[0]
+
+
+@pytest.mark.parametrize("algorithm", ["ap", "dm", "raar"])
+def test_projections_full_object_zero_offsets_are_the_plain_ones(algorithm):
+    # Under FullObject at zero offsets the reads are the windows, AᴴA is the diagonal D, the
+    # first PCG step is the blend on unheld pixels and the rest move nothing beyond rounding
+    # (ptyrs/src/engine/features.rs:726-734).
+    a, b = fresh(2), fresh(2)
+    b.set_position(np.zeros((len(b.iobs), 2)), refine=False)
+    b.set_sampling("full_object")
+    kw = dict(calc_llk=1, zero_phase_ramp=False, probe_inertia=0.01)
+    getattr(a, algorithm)(3, **kw)
+    getattr(b, algorithm)(3, **kw)
+    for x, y in ((b.obj, a.obj), (b.probe, a.probe)):
+        assert np.abs(x - y).max() < 1e-9 * np.abs(y).max(), algorithm
+    assert max_rel([r[2].poisson for r in b.history], [r[2].poisson for r in a.history]) < 1e-11
+
+
+def test_full_object_object_solve_descends_its_quadratic():
+    # solve_obj's PCG steps lower J(O') = ‖psi − A·O'‖² + reg·‖O' − O‖² at sub-pixel offsets,
+    # where AᴴA couples the pixels; more steps lower it further.
+    from oracle.base import to_natural
+    from oracle.ptycho import apply_amplitude, solve_obj
+    from oracle.sampling import Sampler, view_offsets
+    import oracle.ptycho as op
+    p = fresh(2)
+    d0 = np.random.default_rng(15).uniform(-0.45, 0.45, (len(p.iobs), 2))
+    p.set_position(d0, refine=False)
+    p.set_sampling("full_object")
+    nm = p.addr.nmode
+    psi = p.optics.exit_adjoint(apply_amplitude(p.iobs, nm, p.optics.exit_forward(p.calc_psi())))
+    views = p.views_all
+    sm = Sampler(p.obj.shape[-2:], p.probe.shape[-2:], views, view_offsets(views, p.offsets))
+    w = np.stack([p.probe[v[2]] * p.quad for v in views])
+    psi_n = to_natural(psi)
+    d = sm.diag(w.real ** 2 + w.imag ** 2, p.obj.shape[0])
+    reg = 0.01 * d.max()
+
+    def J(o):
+        return float((np.abs(psi_n - w * sm.windows(o)) ** 2).sum() +
+                     reg * (np.abs(o - p.obj) ** 2).sum())
+
+    keep = op.OBJ_PCG
+    try:
+        op.OBJ_PCG = 1
+        j1 = J(solve_obj(sm, w, psi_n, p.obj, 0.01))
+        op.OBJ_PCG = 10
+        j10 = J(solve_obj(sm, w, psi_n, p.obj, 0.01))
+    finally:
+        op.OBJ_PCG = keep
+    assert j10 < j1 < J(p.obj)
+
+
+@pytest.mark.parametrize("algorithm", ["ap", "dm"])
+def test_projections_full_object_refine_offsets_toward_the_truth(algorithm):
+    p = fresh(2)
+    d0 = np.random.default_rng(16).uniform(-0.3, 0.3, (len(p.iobs), 2))
+    d0 -= d0.mean(axis=0)
+    p.set_position(d0.copy(), refine=True)
+    p.set_sampling("full_object")
+    getattr(p, algorithm)(6, calc_llk=1, zero_phase_ramp=False, probe_inertia=0.01)
+    assert np.sqrt((p.offsets ** 2).mean()) < np.sqrt((d0 ** 2).mean())
