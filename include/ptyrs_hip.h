@@ -310,8 +310,10 @@ int ptyrs_engine_launch(ptyrs_engine* e, ptyrs_launch* out);
  * device (BH refines them too). Under PATCH sampling every algorithm and the LLK pass run with
  * offsets (AP, DM and RAAR through Shifted::psi_to_obj_probe,
  * ptyrs/src/engine/features.rs:653-720: the probe's first conjugate-gradient step, DM and RAAR
- * with complex waves only); under FullObject sampling rPIE, ML, LSQML, BH, the LLK pass and
- * the model intensities, with one object slice and mode. */
+ * with complex waves only); under FullObject sampling every algorithm (AP, DM and RAAR through
+ * the FullObject psi_to_obj_probe, ptyrs/src/engine/features.rs:653-793: the probe's first
+ * conjugate-gradient step through the sampled windows, the object's preconditioned CG solve),
+ * the LLK pass and the model intensities, with one object slice and mode. */
 int ptyrs_engine_set_position(ptyrs_engine* e, const double* offsets, size_t len, int refine);
 
 /* GpuPtycho::features' position (ptyrs/src/engine/gpu/mod.rs:823-825): *state = 0 (none),

@@ -767,10 +767,6 @@ class Engine : public EngineBase {
       throw Invalid(std::string(name) +
                     " runs one object slice; only LSQML and the LLK pass run several");
     unitary(name);
-    if (pos_on_ && sampling != PTYRS_PATCH)
-      throw Invalid(std::string(name) +
-                    " with position offsets under FullObject sampling is not run on this GPU "
-                    "engine; set the PATCH sampling");
     if (pos_on_ && psi_f16_ && p.algorithm != PTYRS_AP)
       throw Invalid(std::string(name) +
                     " with position offsets does not carry f16 waves on this GPU engine");
@@ -779,7 +775,10 @@ class Engine : public EngineBase {
     if (p.algorithm != PTYRS_AP && p.loops == 0)
       throw Invalid("loop_obj_probe must be >= 1");
     if (pos_on_ && ncycle > 0 && nframe_ > 0) {
-      project_moved(p, ncycle);
+      if (full_object())
+        project_sampled(p, ncycle);
+      else
+        project_moved(p, ncycle);
       return;
     }
     if (ncycle == 0 || nframe_ == 0) {
@@ -1127,6 +1126,279 @@ class Engine : public EngineBase {
                          (const double*)alpha_.as<double>(), n, np);
     });
     probe_terms();
+  }
+
+  // AP, DM and RAAR under FullObject offsets (ptyrs/src/engine/ptycho.rs:605-755 with the
+  // FullObject psi_to_obj_probe, ptyrs/src/engine/features.rs:653-793): the probe stays put and
+  // every frame reads its window sampled at its corner plus offset (fo_win_, read by passes A
+  // and C at the tall corners); the position sums of the new carried waves through the
+  // windows' derivatives; the probe takes the first conjugate-gradient step of its
+  // least-squares problem through the reads (probe_step_sampled); the object solves its
+  // least-squares problem by preconditioned conjugate gradients (object_solve_sampled). AP
+  // updates both from the arrays on entry; DM and RAAR loop object-only then probe-only
+  // updates, the probe seeing the new object.
+  void project_sampled(const ProjParams& p, size_t ncycle) {
+    const size_t stack = std::min<size_t>(nframe_, stack_size_ ? stack_size_ : kDefaultStack);
+    ensure_workspace(stack, 1);
+    fo_ensure(stack);
+    const int xb = aux_blocks();
+    const size_t vplane = size_t(M_) * n, no = nyo_ * nxo_;
+    carried_.ensure(sizeof(cplx<T>) * nframe_ * vplane);
+    if (p.algorithm == PTYRS_RAAR) iter_.ensure(sizeof(cplx<T>) * nframe_ * vplane);
+    if (pos_refine_) {
+      fo_dd_.ensure(sizeof(cplx<T>) * 2 * stack * n);
+      pos_all_.ensure(sizeof(double) * 4 * nframe_ * xb);
+      pstep_.ensure(sizeof(double) * 4 * nframe_);
+    }
+    for (DevBuf* b : {&gP_, &dPc_}) b->ensure(sizeof(cplx<T>) * nprobe_ * n);
+    for (DevBuf* b : {&gO_, &hO_, &dOc_}) b->ensure(sizeof(cplx<T>) * no);
+    fo_p2_.ensure(sizeof(cplx<T>) * n);
+    fo_pcg_d_.ensure(sizeof(T) * no);
+    fo_pcg_pinv_.ensure(sizeof(T) * no);
+    fo_pcg_sc_.ensure(sizeof(double) * 4);
+    pden_.ensure(sizeof(T) * n);
+    pmax2_.ensure(sizeof(T) * 2);
+    cg_part_.ensure(sizeof(double) * std::max<size_t>(2 * size_t(M_) * xb, kAbsBlocks));
+    vpow_.ensure(sizeof(double) * M_ * xb);
+    alpha_.ensure(sizeof(double) * M_);
+    abs_part_.ensure(sizeof(double) * 2 * kAbsBlocks);
+    const int nstack = int((nframe_ + stack - 1) / stack);
+    probe_terms();
+    cplx<T>* carried = carried_.as<cplx<T>>();
+    cplx<T>* win = fo_win_.as<cplx<T>>();
+    std::vector<std::pair<size_t, int>> due;
+    for (size_t ic = 0; ic < ncycle; ++ic) {
+      const bool rec = p.calc_llk > 0 && (ic % p.calc_llk == 0 || ic + 1 == ncycle);
+      if (rec) {
+        if (due.size() == hist_slots_) drain(due, p.algorithm);
+        const int slot = int(due.size());
+        llk_pass(slot);
+        due.push_back({cycle, slot});
+      }
+      const bool first = ic == 0;
+      MixArgs<T> mix{MIX_NONE, Carried<T, false>{carried},
+                     Carried<T, false>{iter_.as<cplx<T>>()}, T(0), T(0)};
+      if (p.algorithm == PTYRS_DM && !first) {
+        mix.kind = MIX_DM;
+        mix.c0 = T(2.0 - p.mix);
+        mix.c1 = T(1.0 - p.mix);
+      } else if (p.algorithm == PTYRS_RAAR) {
+        mix.kind = first ? MIX_RAAR0 : MIX_RAAR;
+        mix.c0 = T(1.0 - 0.5 * p.mix);
+        mix.c1 = T(0.5 * p.mix);
+      }
+      const bool dm = p.algorithm == PTYRS_DM && !first;
+      run(K_BIN, [&] { fo_object_spectrum(O_.as<cplx<T>>()); });
+      for (int s = 0; s < nstack; ++s) {
+        const int f0 = int(s * stack);
+        const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+        const int* ord = iota_.as<int>() + f0;
+        run(K_BIN, [&] {
+          fo_points(ord, nb);
+          fo_object_windows(nb);
+        });
+        pass_a(ord, nb, false, mix, nullptr, win, N, fo_tall_.as<int2>());
+        pass_b<B_AMPLITUDE>(ord, nb);
+        const int ck = chunk_of(nb);
+        const int nck = (nb + ck - 1) / ck;
+        run(K_PASS_C, [&] {
+          hipLaunchKernelGGL((k_pass_c_proj<T, N, false>), dim3(SR::NBLK, M_, nck), dim3(SR::NT),
+                             0, stream_, psi_.as<cplx<T>>(), Carried<T, false>{carried},
+                             (const cplx<T>*)win, N, Wt_.as<cplx<T>>(), q_.as<cplx<T>>(),
+                             fo_tall_.as<int2>(), ord, nb, M_, ck, int(dm), T(2.0 - p.mix),
+                             T(1.0 - p.mix), 0, (cplx<T>*)nullptr, (T*)nullptr,
+                             tw_.as<cplx<T>>(), 0);
+        });
+        if (pos_refine_) {
+          // The position sums of the new carried waves (fo_spec_ holds the object's spectrum).
+          run(K_OTHER, [&] {
+            fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 1, fo_dd_.p, 2ll * n);
+            fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 2, fo_dd_.as<cplx<T>>() + n,
+                                 2ll * n);
+            hipLaunchKernelGGL(k_pos_sums_fo<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
+                               (const cplx<T>*)(carried + size_t(f0) * vplane),
+                               (const cplx<T>*)Wt_.as<cplx<T>>(), (const cplx<T>*)win,
+                               (const cplx<T>*)fo_dd_.as<cplx<T>>(), n, M_,
+                               pos_all_.as<double>() + size_t(f0) * xb * 4);
+          });
+        }
+      }
+      if (p.algorithm == PTYRS_AP) {
+        // Both from the arrays on entry: the probe's step is applied after the object's solve,
+        // which reads the probe on entry.
+        if (p.upd_probe) probe_step_sampled(stack, nstack, p.probe_inertia);
+        if (p.upd_obj) object_solve_sampled(stack, nstack, p.obj_inertia);
+        if (p.upd_probe) probe_apply();
+      } else {
+        bool stale = false;  // fo_spec_ is not the object's spectrum
+        for (size_t l = 0; l < p.loops; ++l) {
+          if (p.upd_obj) {
+            object_solve_sampled(stack, nstack, p.obj_inertia);
+            stale = true;
+          }
+          if (p.upd_probe) {
+            if (stale) run(K_BIN, [&] { fo_object_spectrum(O_.as<cplx<T>>()); });
+            stale = false;
+            probe_step_sampled(stack, nstack, p.probe_inertia);
+            probe_apply();
+          }
+        }
+      }
+      if (pos_refine_) {
+        run(K_OTHER, [&] {
+          hipLaunchKernelGGL(k_pos_step_all, dim3(1), dim3(1024), 0, stream_,
+                             (const double*)pos_all_.as<double>(), xb, int(nframe_),
+                             offs_.as<double2>(), pstep_.as<double>());
+        });
+      }
+      if (rec && p.algorithm != PTYRS_AP) balance(1.0);
+      ++cycle;
+    }
+    drain(due, p.algorithm);
+    if (p.zero_phase_ramp) device_zero_phase_ramp();
+    synchronize();
+    flush_timing();
+  }
+
+  // The probe's first conjugate-gradient step of min ‖ψ − A·P'‖² + reg·‖P' − P‖² under
+  // FullObject (ptyrs/src/engine/features.rs:673-711), the reads S of every frame not moving
+  // with the probe: g_m = Σ_f conj(S_f·q)·(ψ_{f,m} − W_m·S_f), D = Σ_f |S_f|² + reg (reg =
+  // inertia·max Σ_f |S_f|²), s = g/D and α_m = <g_m, s_m> / (Σ|s_m·q|²·Σ_f |S_f|² + reg·‖s_m‖²)
+  // where <g_m, s_m> > 0, into dPc_ and alpha_ (probe_apply adds them). fo_spec_ holds the
+  // object's spectrum; Wt_ the probe's terms.
+  void probe_step_sampled(size_t stack, int nstack, double inertia) {
+    const int xb = aux_blocks();
+    for (int s = 0; s < nstack; ++s) {
+      const int f0 = int(s * stack);
+      const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+      const int* ord = iota_.as<int>() + f0;
+      run(K_PROBE, [&] {
+        fo_points(ord, nb);
+        fo_object_windows(nb);
+        hipLaunchKernelGGL(k_fo_probe_num<T>, dim3((n + 255) / 256, M_), dim3(256), 0, stream_,
+                           gP_.as<cplx<T>>(), pden_.as<T>(), (const cplx<T>*)fo_win_.as<cplx<T>>(),
+                           (const cplx<T>*)q_.as<cplx<T>>(), (const cplx<T>*)Wt_.as<cplx<T>>(),
+                           (const cplx<T>*)carried_.as<cplx<T>>(), ord, nb, n, M_, int(s > 0));
+      });
+    }
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(MAX_THREADS), 0, stream_,
+                         (const T*)pden_.as<T>(), n, pmax2_.as<T>());
+      hipLaunchKernelGGL(k_cg_probe_sums<T>, dim3(xb, M_), dim3(THREADS), 0, stream_,
+                         (const cplx<T>*)gP_.as<cplx<T>>(), (const T*)pden_.as<T>(),
+                         (const T*)pmax2_.as<T>(), T(inertia), n, dPc_.as<cplx<T>>(),
+                         cg_part_.as<double>());
+      hipLaunchKernelGGL(k_fo_probe_vpow<T>, dim3(xb, M_), dim3(THREADS), 0, stream_,
+                         vpow_.as<double>(), (const cplx<T>*)dPc_.as<cplx<T>>(),
+                         (const cplx<T>*)q_.as<cplx<T>>(), (const T*)pden_.as<T>(), n);
+      hipLaunchKernelGGL(k_cg_probe_alpha, dim3(1), dim3(1), 0, stream_,
+                         (const double*)cg_part_.as<double>(), xb,
+                         (const double*)vpow_.as<double>(), xb, M_, M_, alpha_.as<double>());
+    });
+  }
+  // P += α_m·s_m (the step probe_step_sampled left in dPc_ and alpha_), then the probe terms.
+  void probe_apply() {
+    const int np = int(nprobe_ * n);
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_probe_axpy<T>, dim3((np + 255) / 256), dim3(256), 0, stream_,
+                         P_.as<cplx<T>>(), (const cplx<T>*)dPc_.as<cplx<T>>(),
+                         (const double*)alpha_.as<double>(), n, np);
+    });
+    probe_terms();
+  }
+
+  // out = AᴴA·x under FullObject: x's windows at every frame's point times Σ_m |W_m|² (fo_p2_),
+  // placed back by the sampling's adjoint. Overwrites fo_spec_.
+  void fo_normal(const cplx<T>* x, cplx<T>* out, size_t stack, int nstack) {
+    fo_object_spectrum(x);
+    for (int s = 0; s < nstack; ++s) {
+      const int f0 = int(s * stack);
+      const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+      fo_points(iota_.as<int>() + f0, nb);
+      fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 0, fo_win_.p, (long long)n);
+      hipLaunchKernelGGL(k_fo_times_power<T>, dim3((n + 255) / 256, nb), dim3(256), 0, stream_,
+                         fo_win_.as<cplx<T>>(), (const cplx<T>*)fo_p2_.as<cplx<T>>(), n);
+      fo_place_spectrum(fo_win_.as<cplx<T>>(), (long long)n, nb, s > 0);
+    }
+    fo_add_placed<AS_PLAIN>(out, 1.0);
+  }
+
+  // The object of min ‖ψ − A·O'‖² + reg·‖O' − O‖² under FullObject
+  // (ptyrs/src/engine/features.rs:726-793), A the frames' waves W_m = P_m·q times their reads:
+  // kObjPcg preconditioned conjugate-gradient steps on (AᴴA + reg)·O' = Aᴴψ + reg·O from O, the
+  // preconditioner D + reg with D the exact diagonal of AᴴA (Σ_m |W_m|² placed with the squared
+  // sampling kernels, real part floored at 0) and reg = inertia·max D; pixels with D + reg ≤
+  // 1e-5·max D are held. Every sum in FP64; the scalars stay on the device. Wt_ holds the probe
+  // terms, carried_ the waves ψ; O_ is updated in place; fo_spec_ is overwritten.
+  void object_solve_sampled(size_t stack, int nstack, double inertia) {
+    const long long no = (long long)(nyo_ * nxo_), tp = (long long)N * nxo_;
+    const size_t vplane = size_t(M_) * n;
+    const int grid = int(std::min<long long>(kAbsBlocks, (no + THREADS - 1) / THREADS));
+    const unsigned gno = unsigned((no + 255) / 256);
+    cplx<T>* r = gO_.as<cplx<T>>();
+    cplx<T>* pd = hO_.as<cplx<T>>();
+    cplx<T>* ap = dOc_.as<cplx<T>>();
+    T* D = fo_pcg_d_.as<T>();
+    T* pinv = fo_pcg_pinv_.as<T>();
+    double* sc = fo_pcg_sc_.as<double>();
+    double* part = cg_part_.as<double>();
+    run(K_OBJ_UPDATE, [&] {
+      // D: the spectrum of Σ_m |W_m|² at the plane's origin times Σ_f K_f.
+      hipLaunchKernelGGL(k_fo_power<T>, dim3((n + 255) / 256), dim3(256), 0, stream_,
+                         fo_p2_.as<cplx<T>>(), (const cplx<T>*)Wt_.as<cplx<T>>(), M_, n);
+      axis_pass<AL_PLAIN, AS_PLAIN>(0, false, fo_p2_.as<cplx<T>>(), fo_tmp_.p,
+                                    rows_geom(n, N, tp, int(nxo_), N, N, int(nxo_)), 1);
+      axis_pass<AL_PLAIN, AS_PLAIN>(1, false, fo_tmp_.as<cplx<T>>(), fo_planes_.p,
+                                    cols_geom(tp, N, no, int(nyo_)), 1);
+      for (int s = 0; s < nstack; ++s) {
+        const int f0 = int(s * stack);
+        const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+        fo_points(iota_.as<int>() + f0, nb);
+        hipLaunchKernelGGL(k_fo_diag_acc<T>, dim3(gno), dim3(256), 0, stream_,
+                           fo_acc_.as<cplx<T>>(), (const cplx<T>*)fo_planes_.as<cplx<T>>(),
+                           (const double2*)fo_pts_.as<double2>(), nb, int(nyo_), int(nxo_),
+                           int(s > 0));
+      }
+      fo_add_placed<AS_REAL_FLOOR>(D, 1.0);
+      hipLaunchKernelGGL(k_max_reduce<T>, dim3(1), dim3(MAX_THREADS), 0, stream_, (const T*)D,
+                         int(no), pmax2_.as<T>() + 1);
+      hipLaunchKernelGGL(k_pcg_pinv<T>, dim3(gno), dim3(256), 0, stream_, pinv, (const T*)D,
+                         (const T*)(pmax2_.as<T>() + 1), inertia, no, sc);
+      // The right-hand side Aᴴψ into r.
+      for (int s = 0; s < nstack; ++s) {
+        const int f0 = int(s * stack);
+        const int nb = std::min<int>(int(stack), int(nframe_) - f0);
+        fo_points(iota_.as<int>() + f0, nb);
+        hipLaunchKernelGGL(k_fo_chi_terms<T>, dim3((n + 255) / 256, nb), dim3(256), 0, stream_,
+                           fo_dwin_.as<cplx<T>>(), (const cplx<T>*)Wt_.as<cplx<T>>(),
+                           (const cplx<T>*)(carried_.as<cplx<T>>() + size_t(f0) * vplane), M_,
+                           n);
+        fo_place_spectrum(fo_dwin_.as<cplx<T>>(), (long long)n, nb, s > 0);
+      }
+      fo_add_placed<AS_PLAIN>(r, 1.0);
+      // r = Aᴴψ − AᴴA·O (the regulariser's terms cancel at O), p = r/(D + reg).
+      fo_normal(O_.as<cplx<T>>(), ap, stack, nstack);
+      hipLaunchKernelGGL(k_pcg_start<T>, dim3(grid), dim3(THREADS), 0, stream_, r,
+                         (const cplx<T>*)ap, (const T*)pinv, pd, no, part);
+      hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(THREADS), 0, stream_,
+                         (const double*)part, grid, sc, 0);
+      for (int it = 0; it < kObjPcg; ++it) {
+        fo_normal(pd, ap, stack, nstack);
+        hipLaunchKernelGGL(k_pcg_ap<T>, dim3(grid), dim3(THREADS), 0, stream_, ap,
+                           (const cplx<T>*)pd, (const double*)sc, no, part);
+        hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(THREADS), 0, stream_,
+                           (const double*)part, grid, sc, 1);
+        hipLaunchKernelGGL(k_pcg_step<T>, dim3(grid), dim3(THREADS), 0, stream_,
+                           O_.as<cplx<T>>(), r, (const cplx<T>*)pd, (const cplx<T>*)ap,
+                           (const T*)pinv, (const double*)sc, no, part);
+        if (it + 1 < kObjPcg) {
+          hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(THREADS), 0, stream_,
+                             (const double*)part, grid, sc, 2);
+          hipLaunchKernelGGL(k_pcg_dir<T>, dim3(gno), dim3(256), 0, stream_, pd,
+                             (const cplx<T>*)r, (const T*)pinv, (const double*)sc, no);
+        }
+      }
+    });
   }
 
   // PyNX ML, Poisson conjugate gradient (ptyrs/src/engine/ptycho.rs:757-842): scale_obj_probe,
@@ -1963,8 +2235,10 @@ class Engine : public EngineBase {
   // nxo, [1] columns of nyo).
   AxisPlan fo_plan_[2]{};
   DevBuf fo_tw_[2], fo_spec_, fo_tmp_, fo_planes_, fo_win_, fo_dwin_, fo_F_, fo_acc_, fo_E_,
-      fo_pts_, fo_tall_, fo_D_, fo_part_;
+      fo_pts_, fo_tall_, fo_D_, fo_part_, fo_p2_, fo_pcg_d_, fo_pcg_pinv_, fo_pcg_sc_;
   size_t fo_cap_ = 0;
+  // The object solve's conjugate-gradient steps under FullObject (ptyrs/src/engine/sampling.rs:214).
+  static constexpr int kObjPcg = 10;
   static constexpr int kFoBlocks = 256;
   bool full_object() const { return pos_on_ && sampling == PTYRS_FULL_OBJECT; }
 

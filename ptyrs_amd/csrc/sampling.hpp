@@ -314,6 +314,217 @@ __global__ __launch_bounds__(THREADS) void k_pos_sums_fo(
   }
 }
 
+// ---- AP, DM and RAAR under FullObject: the object's PCG solve (ptyrs/src/engine/features.rs:
+// 726-793) and the probe's step through the sampled windows --------------------------------------
+// One axis of the diagonal's kernel, the spectrum of |h_t[−v]|² (h_t the one-axis kernel that
+// samples at t) at index m of n: (1 − m/n)·exp(−2πi t m/n) + (m/n)·exp(−2πi t (m − n)/n).
+// ptyrs/src/engine/sampling.rs:217-230
+__device__ __forceinline__ double2 square_spectrum(double t, int m, int n) {
+  const double a = double(m) / double(n);
+  double s0, c0, s1, c1;
+  double u = t * a;
+  u -= rint(u);
+  sincospi(-2.0 * u, &s0, &c0);
+  double v = t * (a - 1.0);
+  v -= rint(v);
+  sincospi(-2.0 * v, &s1, &c1);
+  return make_double2((1.0 - a) * c0 + a * c1, (1.0 - a) * s0 + a * s1);
+}
+
+// acc[j] (+)= X[j]·Σ_b K_b[j], K_b the diagonal's kernel at slot b's point (the product of the
+// two axes' square_spectrum): the spectrum of the one patch X (the same at every slot) placed
+// for AᴴA's exact diagonal (Sampler::diag, ptyrs/src/engine/sampling.rs:196-209).
+template <typename T>
+__global__ void k_fo_diag_acc(cplx<T>* __restrict__ acc, const cplx<T>* __restrict__ X,
+                              const double2* __restrict__ pts, int nb, int ny, int nx,
+                              int accumulate) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ny * nx) return;
+  const int jy = j / nx, jx = j % nx;
+  double kr = 0, ki = 0;
+  for (int b = 0; b < nb; ++b) {
+    const double2 a = square_spectrum(pts[b].x, jy, ny), c = square_spectrum(pts[b].y, jx, nx);
+    kr += a.x * c.x - a.y * c.y;
+    ki += a.x * c.y + a.y * c.x;
+  }
+  const cplx<T> x = ld(X + j);
+  double sr = double(x.x) * kr - double(x.y) * ki, si = double(x.x) * ki + double(x.y) * kr;
+  if (accumulate) {
+    const cplx<T> a = ld(acc + j);
+    sr += double(a.x);
+    si += double(a.y);
+  }
+  st(acc + j, cplx<T>{T(sr), T(si)});
+}
+
+// out = Σ_m |W_m|² (a real patch as complex, zero imaginary part). Grid (ceil(n/256)).
+template <typename T>
+__global__ void k_fo_power(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ W, int M, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  T s = 0;
+  for (int m = 0; m < M; ++m) s += norm2(ld(W + (size_t)m * n + j));
+  st(out + j, cplx<T>{s, T(0)});
+}
+
+// planes[b][j] *= p[j].x over nb planes (the reads times Σ_m |W_m|²). Grid (ceil(n/256), nb).
+template <typename T>
+__global__ void k_fo_times_power(cplx<T>* __restrict__ planes, const cplx<T>* __restrict__ p,
+                                 int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  cplx<T>* v = planes + (size_t)blockIdx.y * n + j;
+  st(v, ld(v) * ld(p + j).x);
+}
+
+// The probe step's sums through the reads (slots in order, Kahan): num_m (+)= Σ_b conj(S_b·q)·
+// (ψ_{f_b, m} − W_m·S_b) and, from the mode-0 blocks, norm (+)= Σ_b |S_b|²; ψ the carried waves by
+// frame ([frame·M + m][n]), S the stack's reads ([slot][n]). Grid (ceil(n/256), M).
+template <typename T>
+__global__ void k_fo_probe_num(cplx<T>* __restrict__ num, T* __restrict__ norm,
+                               const cplx<T>* __restrict__ win, const cplx<T>* __restrict__ q,
+                               const cplx<T>* __restrict__ W, const cplx<T>* __restrict__ psi,
+                               const int* __restrict__ order, int nb, int n, int M,
+                               int accumulate) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int m = blockIdx.y;
+  const cplx<T> qj = ld(q + j), w = ld(W + (size_t)m * n + j);
+  kahan<T> sr, si, sn;
+  for (int b = 0; b < nb; ++b) {
+    const cplx<T> o = ld(win + (size_t)b * n + j);
+    const cplx<T> r = ld(psi + ((size_t)order[b] * M + m) * n + j) - w * o;
+    const cplx<T> g = cmulc(o * qj, r);
+    sr.add(g.x);
+    si.add(g.y);
+    if (m == 0) sn.add(norm2(o));
+  }
+  cplx<T> v{sr.s, si.s};
+  if (accumulate) v = v + ld(num + (size_t)m * n + j);
+  st(num + (size_t)m * n + j, v);
+  if (m == 0) norm[j] = accumulate ? norm[j] + sn.s : sn.s;
+}
+
+// Per mode and workgroup Σ |s_m·q|²·norm (FP64): ‖A·s‖² of the probe step (the reads do not
+// move with the probe) into part[m·gridDim.x + blk]. Grid (nblk, M).
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_fo_probe_vpow(double* __restrict__ part,
+                                                           const cplx<T>* __restrict__ s,
+                                                           const cplx<T>* __restrict__ q,
+                                                           const T* __restrict__ norm, int n) {
+  __shared__ double red[THREADS];
+  const int m = blockIdx.y;
+  double a = 0;
+  for (int j = blockIdx.x * THREADS + threadIdx.x; j < n; j += gridDim.x * THREADS)
+    a += double(norm2(ld(s + (size_t)m * n + j) * ld(q + j))) * double(norm[j]);
+  const double t = block_reduce<THREADS>(a, red, AddOp());
+  if (threadIdx.x == 0) part[(size_t)m * gridDim.x + blockIdx.x] = t;
+}
+
+// The PCG's vector steps over the object (no values), FP64 partial sums per workgroup into
+// part[blk]; sc = {α, β, rz, reg} on the device. pinv = 1/(D + reg), 0 where held.
+template <typename T>
+__global__ void k_pcg_pinv(T* __restrict__ pinv, const T* __restrict__ D,
+                           const T* __restrict__ dmax, double inertia, long long no,
+                           double* __restrict__ sc) {
+  const long long i = blockIdx.x * 256ll + threadIdx.x;
+  const double mx = double(dmax[0]), reg = mx * inertia;
+  if (i == 0) sc[3] = reg;
+  if (i >= no) return;
+  const double v = double(D[i]) + reg;
+  pinv[i] = v > 1e-5 * mx ? T(1.0 / v) : T(0);
+}
+// r −= ax (r holding the right-hand side); p = r·pinv; Σ |r|²·pinv.
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_pcg_start(cplx<T>* __restrict__ r,
+                                                       const cplx<T>* __restrict__ ax,
+                                                       const T* __restrict__ pinv,
+                                                       cplx<T>* __restrict__ p, long long no,
+                                                       double* __restrict__ part) {
+  __shared__ double red[THREADS];
+  double a = 0;
+  for (long long i = blockIdx.x * (long long)THREADS + threadIdx.x; i < no;
+       i += (long long)gridDim.x * THREADS) {
+    const cplx<T> v = ld(r + i) - ld(ax + i);
+    st(r + i, v);
+    st(p + i, v * pinv[i]);
+    a += double(norm2(v)) * double(pinv[i]);
+  }
+  a = block_reduce<THREADS>(a, red, AddOp());
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+// ap += reg·p; Re Σ conj(p)·ap.
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_pcg_ap(cplx<T>* __restrict__ ap,
+                                                    const cplx<T>* __restrict__ p,
+                                                    const double* __restrict__ sc, long long no,
+                                                    double* __restrict__ part) {
+  __shared__ double red[THREADS];
+  const T reg = T(sc[3]);
+  double a = 0;
+  for (long long i = blockIdx.x * (long long)THREADS + threadIdx.x; i < no;
+       i += (long long)gridDim.x * THREADS) {
+    const cplx<T> pv = ld(p + i), v = ld(ap + i) + pv * reg;
+    st(ap + i, v);
+    a += double(cmulc(pv, v).x);
+  }
+  a = block_reduce<THREADS>(a, red, AddOp());
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+// x += α·p; r −= α·ap; Σ |r|²·pinv.
+template <typename T>
+__global__ __launch_bounds__(THREADS) void k_pcg_step(cplx<T>* __restrict__ x,
+                                                      cplx<T>* __restrict__ r,
+                                                      const cplx<T>* __restrict__ p,
+                                                      const cplx<T>* __restrict__ ap,
+                                                      const T* __restrict__ pinv,
+                                                      const double* __restrict__ sc,
+                                                      long long no, double* __restrict__ part) {
+  __shared__ double red[THREADS];
+  const double alpha = sc[0];
+  double a = 0;
+  for (long long i = blockIdx.x * (long long)THREADS + threadIdx.x; i < no;
+       i += (long long)gridDim.x * THREADS) {
+    const cplx<T> xv = ld(x + i), pv = ld(p + i), rv = ld(r + i), av = ld(ap + i);
+    st(x + i, cplx<T>{T(double(xv.x) + alpha * pv.x), T(double(xv.y) + alpha * pv.y)});
+    const cplx<T> nr{T(double(rv.x) - alpha * av.x), T(double(rv.y) - alpha * av.y)};
+    st(r + i, nr);
+    a += double(norm2(nr)) * double(pinv[i]);
+  }
+  a = block_reduce<THREADS>(a, red, AddOp());
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
+}
+// p = r·pinv + β·p.
+template <typename T>
+__global__ void k_pcg_dir(cplx<T>* __restrict__ p, const cplx<T>* __restrict__ r,
+                          const T* __restrict__ pinv, const double* __restrict__ sc,
+                          long long no) {
+  const long long i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= no) return;
+  const double beta = sc[1];
+  const cplx<T> rv = ld(r + i) * pinv[i], pv = ld(p + i);
+  st(p + i, cplx<T>{T(rv.x + beta * pv.x), T(rv.y + beta * pv.y)});
+}
+// One workgroup: the sum of part[0..nblk) in a fixed order, then mode 0: rz = sum; mode 1: α =
+// rz/sum where sum > 0, else 0; mode 2: β = sum/rz where rz > 0, else 0, and rz = sum.
+__global__ __launch_bounds__(THREADS) static void k_pcg_scalars(const double* __restrict__ part,
+                                                                int nblk, double* __restrict__ sc,
+                                                                int mode) {
+  __shared__ double red[THREADS];
+  double a = 0;
+  for (int i = threadIdx.x; i < nblk; i += THREADS) a += part[i];
+  a = block_reduce<THREADS>(a, red, AddOp());
+  if (threadIdx.x != 0) return;
+  if (mode == 0) {
+    sc[2] = a;
+  } else if (mode == 1) {
+    sc[0] = a > 0.0 ? sc[2] / a : 0.0;
+  } else {
+    sc[1] = sc[2] > 0.0 ? a / sc[2] : 0.0;
+    sc[2] = a;
+  }
+}
+
 // dO /= √(pre² + floor) (LSQML's preconditioned object direction).
 template <typename T>
 __global__ void k_fo_precondition(cplx<T>* __restrict__ dO, const T* __restrict__ pre,

@@ -164,9 +164,6 @@ def test_full_object_refusals():
     assert g.sampling == "patch"   # far field: Sampling::auto takes PATCH
     g.set_sampling("full_object")
     g.set_features(position=np.zeros((10, 2)))
-    for call in (lambda: g.ap(1), lambda: g.dm(1)):
-        with pytest.raises(ValueError):
-            call()
     with pytest.raises(ValueError):
         g.set_sampling("other")
     g.set_features(position=None)
@@ -269,6 +266,42 @@ def test_ml_full_object_matches_oracle(dtype, nprobe, refine, up):
     tol = PTOL[dtype]
     want = [r[2].poisson for r in o.history]
     print("ml full object", dtype, nprobe, refine, up, max_rel(g.history["poisson"], want),
+          rel_l2(g.object, o.obj), rel_l2(g.probe, o.probe))
+    assert max_rel(g.history["poisson"], want) < tol
+    assert rel_l2(g.object, o.obj) < tol * 10
+    assert rel_l2(g.probe, o.probe) < tol * 10
+    got = g.features["position"]
+    if refine:
+        assert max_rel((got - d0).ravel(), (o.offsets - d0).ravel()) < tol * 100
+    else:
+        assert np.array_equal(got, d0)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("name,nprobe,refine,kw", [
+    ("ap", 1, True, dict(probe_inertia=0.01)), ("ap", 2, False, {}),
+    ("dm", 2, True, {}), ("dm", 1, False, dict(loop_obj_probe=2)),
+    ("raar", 3, True, {}), ("raar", 1, True, dict(loop_obj_probe=2))])
+def test_projections_full_object_match_oracle(dtype, name, nprobe, refine, kw):
+    # AP, DM and RAAR under FULL_OBJECT (the oracle's: the probe's conjugate-gradient step
+    # through the sampled windows, the object's preconditioned CG solve with the exact diagonal
+    # and held pixels, the position sums through the windows' derivatives): far field, masked
+    # pixels, zero-mean offsets within 0.45 px, refined or held, three cycles with LLK records.
+    d = small_scan(n=32, nprobe=nprobe, npos=25, seed=33)
+    d0 = np.random.default_rng(34).uniform(-0.45, 0.45, (25, 2))
+    d0 -= d0.mean(axis=0)
+    o = oracle(d)
+    o.set_sampling("full_object")
+    o.set_position(d0, refine=refine)
+    params = dict(calc_llk=1, **kw)
+    getattr(o, name)(3, **params)
+    g = gpu(d, dtype)
+    g.set_sampling("full_object")
+    g.set_features(position=d0, hold=() if refine else ("position",))
+    getattr(g, name)(3, **params)
+    tol = PTOL[dtype]
+    want = [r[2].poisson for r in o.history]
+    print(name, "full object", nprobe, refine, kw, dtype, max_rel(g.history["poisson"], want),
           rel_l2(g.object, o.obj), rel_l2(g.probe, o.probe))
     assert max_rel(g.history["poisson"], want) < tol
     assert rel_l2(g.object, o.obj) < tol * 10

@@ -133,12 +133,13 @@ def test_position_feature_api():
     for name in ("ap", "dm", "raar"):   # held offsets: the projections model them
         getattr(g, name)(1)
         assert np.array_equal(g.features["position"], d0)
-    g.set_sampling("full_object")   # held offsets under FullObject: rPIE and LSQML run
+    g.set_sampling("full_object")   # held offsets under FullObject: every algorithm runs
     g.rpie(1, batch_size=5)
     g.lsqml(1, 1, 6, "poisson", 1)
+    g.ml(1)
+    for name in ("ap", "dm", "raar"):
+        getattr(g, name)(1)
     assert np.array_equal(g.features["position"], d0)
-    with pytest.raises(ValueError, match="FullObject"):
-        g.ml(1)
     g.set_sampling("patch")
     with pytest.raises(ValueError):
         g.set_features(position=d0[:5])

@@ -446,6 +446,47 @@ def test_full_object_object_solve_descends_its_quadratic():
     assert j10 < j1 < J(p.obj)
 
 
+def test_full_object_object_solve_converges_to_its_minimiser():
+    # Run long, solve_obj's PCG reaches the minimiser of J(O') = ‖psi − A·O'‖² + reg·‖O' − O‖²,
+    # J written with the windows alone (no adjoint): along random directions J's slope there
+    # (exact by central differences, J being quadratic) vanishes against its slope at O, and J
+    # rises both ways. A wrong adjoint, diagonal or conjugation leaves a stationary point of
+    # another quadratic.
+    from oracle.base import to_natural
+    from oracle.ptycho import apply_amplitude, solve_obj
+    from oracle.sampling import Sampler, view_offsets
+    import oracle.ptycho as op
+    p = fresh(2)
+    d0 = np.random.default_rng(17).uniform(-0.45, 0.45, (len(p.iobs), 2))
+    p.set_position(d0, refine=False)
+    p.set_sampling("full_object")
+    nm = p.addr.nmode
+    psi = p.optics.exit_adjoint(apply_amplitude(p.iobs, nm, p.optics.exit_forward(p.calc_psi())))
+    views = p.views_all
+    sm = Sampler(p.obj.shape[-2:], p.probe.shape[-2:], views, view_offsets(views, p.offsets))
+    w = np.stack([p.probe[v[2]] * p.quad for v in views])
+    psi_n = to_natural(psi)
+    reg = 0.01 * sm.diag(w.real ** 2 + w.imag ** 2, p.obj.shape[0]).max()
+
+    def J(o):
+        return float((np.abs(psi_n - w * sm.windows(o)) ** 2).sum() +
+                     reg * (np.abs(o - p.obj) ** 2).sum())
+
+    keep = op.OBJ_PCG
+    try:
+        op.OBJ_PCG = 300
+        x = solve_obj(sm, w, psi_n, p.obj, 0.01)
+    finally:
+        op.OBJ_PCG = keep
+    rng = np.random.default_rng(18)
+    for _ in range(3):
+        dlt = (rng.standard_normal(p.obj.shape) + 1j * rng.standard_normal(p.obj.shape)) * 0.01
+        at_x = (J(x + dlt) - J(x - dlt)) / 2
+        at_o = (J(p.obj + dlt) - J(p.obj - dlt)) / 2
+        assert abs(at_x) < 1e-7 * abs(at_o), (at_x, at_o)
+        assert J(x + dlt) > J(x) and J(x - dlt) > J(x)
+
+
 @pytest.mark.parametrize("algorithm", ["ap", "dm"])
 def test_projections_full_object_refine_offsets_toward_the_truth(algorithm):
     p = fresh(2)
