@@ -242,7 +242,8 @@ typedef struct {
 /* GpuPtycho::lsqml with a ProbeVariation (ptyrs/src/engine/gpu/lsqml.rs:75-120): LSQML epochs
  * that model pty-chi's orthogonal probe relaxation of probe mode 0; with nepoch >= 1 the
  * eigenmodes, weights and intensity of `var` are overwritten with the values the epochs leave.
- * Not with position offsets (PTYRS_EINVAL). */
+ * Runs with one slice or several, with or without position offsets (refined or held) under
+ * PATCH or FullObject sampling (FullObject with one slice). */
 int ptyrs_engine_lsqml_opr(ptyrs_engine* e, const ptyrs_lsqml_params* params, size_t nepoch,
                            size_t calc_llk, ptyrs_opr* var);
 

@@ -2115,8 +2115,6 @@ class Engine : public EngineBase {
                  ptyrs_opr& var) override {
     if (var.k == 0 || !var.eigenmodes || !var.weights)
       throw Invalid("OPR needs k >= 1 eigenmodes and their weights");
-    if (full_object())
-      throw Invalid("LSQML with OPR under FullObject sampling is not run on this GPU engine");
     if (var.k > 64) throw Invalid("OPR takes at most 64 eigenmodes");
     const size_t k = var.k;
     for (size_t i = 0; i < 2 * k * n; ++i)
@@ -2163,15 +2161,20 @@ class Engine : public EngineBase {
   // weights' step; then, with two slots or more, per eigenmode in order its move along the
   // residues' principal direction, its renormalisation, its weights' step and the residues'
   // projection off it.
+  // Under FullObject the windows are the batch's sampled reads (fo_win_ at the tall corners)
+  // and the directions are not moved.
   void opr_step(const int* ord, int nb) {
     const int xb = aux_blocks(), eb = opr_blocks();
     const cplx<T>* q0 = nslice_ == 1 ? q_.as<cplx<T>>() : nullptr;
     const int k = opr_k_;
+    const bool fo = full_object();
     run(K_PROBE, [&] {
       hipLaunchKernelGGL(k_opr_frame_dir<T>, dim3((n + 255) / 256, nb), dim3(256), 0, stream_,
-                         opr_d_.as<cplx<T>>(), opr_vv_.as<T>(), (const cplx<T>*)slice_obj(0),
-                         int(nxo_), q0, (const cplx<T>*)psi_.as<cplx<T>>(), corner_.as<int2>(),
-                         ord, N, M_);
+                         opr_d_.as<cplx<T>>(), opr_vv_.as<T>(),
+                         fo ? (const cplx<T>*)fo_win_.as<cplx<T>>()
+                            : (const cplx<T>*)slice_obj(0),
+                         fo ? N : int(nxo_), q0, (const cplx<T>*)psi_.as<cplx<T>>(),
+                         fo ? fo_tall_.as<int2>() : corner_.as<int2>(), ord, N, M_);
       if (opr_c_on_ && opr_refine_c_) {
         hipLaunchKernelGGL((k_opr_sums<T, 0>), dim3(xb, nb), dim3(kOprThreads), 0, stream_,
                            (const cplx<T>*)P_.as<cplx<T>>(), (const cplx<T>*)opr_d_.as<cplx<T>>(),
@@ -2183,7 +2186,8 @@ class Engine : public EngineBase {
       if (nb < 2) return;
       // The residues of the directions taken back through the frames' moves (the directions
       // themselves without offsets).
-      const cplx<T>* moved = pos_on_ ? opr_m_.as<cplx<T>>() : opr_d_.as<cplx<T>>();
+      const cplx<T>* moved =
+          pos_on_ && !fo ? opr_m_.as<cplx<T>>() : opr_d_.as<cplx<T>>();
       hipLaunchKernelGGL(k_opr_residues<T>, dim3((n + 255) / 256), dim3(256), 0, stream_,
                          opr_r_.as<cplx<T>>(), moved, nb, n);
       const double a_u = std::min(0.1, double(nb) / double(nframe_));
@@ -2509,6 +2513,24 @@ class Engine : public EngineBase {
     const cplx<T>* qlast;
     int q_frame;
   };
+  // Under FullObject with OPR the slots' waves W_b = P_b·q from each frame's own probe planes
+  // (unmoved), plane slot·M + m of inc0_ (ptyrs/src/engine/lsqml.rs:483-536 with the sampled
+  // windows).
+  const cplx<T>* opr_waves_sampled(const int* ord, int nb) {
+    const int K = nb * M_;
+    cplx<T>* pl = inc0_.as<cplx<T>>();
+    run(K_PROBE, [&] {
+      hipLaunchKernelGGL(k_opr_planes<T>, dim3((n + 255) / 256, K), dim3(256), 0, stream_, pl,
+                         (const cplx<T>*)P_.as<cplx<T>>(), (const cplx<T>*)opr_U_.as<cplx<T>>(),
+                         (const double*)opr_w_.as<double>(),
+                         opr_c_on_ ? (const double*)opr_c_.as<double>() : nullptr, ord, opr_k_,
+                         M_, n);
+      hipLaunchKernelGGL((k_times_q<T, false>), dim3((n + 255) / 256, K), dim3(256), 0, stream_,
+                         pl, (const cplx<T>*)q_.as<cplx<T>>(), n);
+    });
+    return pl;
+  }
+
   Waves stack_waves(const int* ord, int nb) {
     Waves w{P_.as<cplx<T>>(), 0, q_.as<cplx<T>>(), 0};
     if (opr_on_) {
@@ -4107,8 +4129,9 @@ class Engine : public EngineBase {
           fo_points(ord, nb);
           fo_object_windows(nb);
         });
-        pass_a(ord, nb, false, MixArgs<T>{MIX_NONE, {}, {}, T(0), T(0)}, nullptr,
-               fo_win_.as<cplx<T>>(), N, fo_tall_.as<int2>());
+        pass_a(ord, nb, false, MixArgs<T>{MIX_NONE, {}, {}, T(0), T(0)},
+               opr_on_ ? opr_waves_sampled(ord, nb) : nullptr, fo_win_.as<cplx<T>>(), N,
+               fo_tall_.as<int2>());
       } else if (nslice_ > 1 || pos_on_ || opr_on_) {
         ms_forward(ord, nb, stack_waves(ord, nb));
         run(K_PASS_A, [&] { rows(false, nb * M_); });
@@ -4323,7 +4346,10 @@ class Engine : public EngineBase {
       fo_object_spectrum(O_.as<cplx<T>>());
       fo_object_windows(nb);
     });
-    pass_a(ord, nb, false, MixArgs<T>{MIX_NONE, {}, {}, T(0), T(0)}, nullptr,
+    // The waves: the shared probe terms, or with OPR each slot's own (W_view).
+    const cplx<T>* W = opr_on_ ? opr_waves_sampled(ord, nb) : Wt_.as<cplx<T>>();
+    const int wview = opr_on_ ? 1 : 0;
+    pass_a(ord, nb, false, MixArgs<T>{MIX_NONE, {}, {}, T(0), T(0)}, opr_on_ ? W : nullptr,
            fo_win_.as<cplx<T>>(), N, fo_tall_.as<int2>());
     BArgs<T> ba{afar_tmp_.as<T>(), prev_.as<T>(), xi_part_.as<double>(), icalc_.as<T>()};
     const int xb = aux_blocks();
@@ -4357,9 +4383,9 @@ class Engine : public EngineBase {
     const int nchunk = (nb + chunk - 1) / chunk;
     run(K_PASS_C, [&] {
       hipLaunchKernelGGL((k_pass_c_chi<T, N>), dim3(SR::NBLK, M_, nchunk), dim3(SR::NT), 0,
-                         stream_, psi_.as<cplx<T>>(), fo_win_.as<cplx<T>>(), N,
-                         Wt_.as<cplx<T>>(), q_.as<cplx<T>>(), fo_tall_.as<int2>(), ord, nb, M_,
-                         chunk, 1, dP_.as<cplx<T>>(), tw_.as<cplx<T>>());
+                         stream_, psi_.as<cplx<T>>(), fo_win_.as<cplx<T>>(), N, W,
+                         q_.as<cplx<T>>(), fo_tall_.as<int2>(), ord, nb, M_, chunk, 1,
+                         dP_.as<cplx<T>>(), tw_.as<cplx<T>>(), 1, wview);
     });
     const cplx<T>* chi = psi_.as<cplx<T>>();
     if (pos_refine_) {
@@ -4369,13 +4395,13 @@ class Engine : public EngineBase {
         fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 1, grad, 2ll * n);
         fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 2, grad + n, 2ll * n);
         hipLaunchKernelGGL(k_pos_sums<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
-                           (const cplx<T>*)grad, (const cplx<T>*)Wt_.as<cplx<T>>(),
-                           (const cplx<T>*)nullptr, 0, chi, N, M_, pos_part_.as<double>(), 0);
+                           (const cplx<T>*)grad, W, (const cplx<T>*)nullptr, 0, chi, N, M_,
+                           pos_part_.as<double>(), wview);
       });
     }
     run(K_OBJ_DIR, [&] {
       hipLaunchKernelGGL(k_fo_chi_terms<T>, dim3((n + 255) / 256, nb), dim3(256), 0, stream_,
-                         fo_dwin_.as<cplx<T>>(), (const cplx<T>*)Wt_.as<cplx<T>>(), chi, M_, n);
+                         fo_dwin_.as<cplx<T>>(), W, chi, M_, n, wview);
       fo_place_spectrum(fo_dwin_.as<cplx<T>>(), (long long)n, nb, false);
       fo_add_placed<AS_PLAIN>(dO_.p, 1.0);
       hipLaunchKernelGGL(k_fo_precondition<T>, dim3(unsigned((no + 255) / 256)), dim3(256), 0,
@@ -4396,13 +4422,15 @@ class Engine : public EngineBase {
       fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 0, fo_dwin_.p, (long long)n);
       hipLaunchKernelGGL(k_lsqml_dots<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
                          (const cplx<T>*)fo_dwin_.as<cplx<T>>(),
-                         (const cplx<T>*)fo_win_.as<cplx<T>>(), N, Wt_.as<cplx<T>>(),
-                         dPm_.as<cplx<T>>(), q_.as<cplx<T>>(), chi, N, M_, fo_tall_.as<int2>(),
-                         ord, dots_part_.as<double>());
+                         (const cplx<T>*)fo_win_.as<cplx<T>>(), N, W, dPm_.as<cplx<T>>(),
+                         q_.as<cplx<T>>(), chi, N, M_, fo_tall_.as<int2>(), ord,
+                         dots_part_.as<double>(), wview);
       hipLaunchKernelGGL(k_lsqml_step<T>, dim3(1), dim3(1024), 0, stream_,
                          dots_part_.as<double>(), xb, nb, lam, M_, sums_.as<double>(),
                          log_.as<double>() + 3 * size_t(off), step_.as<double>(), 3);
     });
+    // OPR's eigenmodes and weights from χ and the batch-start windows and probe.
+    if (opr_on_) opr_step(ord, nb);
     run(K_OBJ_UPDATE, [&] {
       hipLaunchKernelGGL(k_lsqml_apply_obj<T>, dim3(unsigned((no + 255) / 256)), dim3(256), 0,
                          stream_, O_.as<cplx<T>>(), dO_.as<cplx<T>>(), step_.as<double>(),

@@ -530,7 +530,8 @@ __global__ __launch_bounds__((ColShape<T, N>::NT)) void k_pass_t(cplx<T>* __rest
 // LSQML's pass C. Grid (NBLK, M, nchunk). Each workgroup owns rows of mode m and runs over the
 // batch slots of its chunk in order: it transforms the rows back, stores χ = ψ' − W·O over the
 // projected field (sub_wo = 0: χ = ψ', BH's back-projected residual), and adds conj(O·q)·χ into
-// a per-chunk probe direction.
+// a per-chunk probe direction. With w_view every slot's mode has its own wave W (plane slot·M + m
+// of Wt, a frame's own probe).
 // ptyrs/src/engine/lsqml.rs:97-228
 template <typename T, int N>
 __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_chi(
@@ -538,7 +539,7 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_chi(
     const cplx<T>* __restrict__ Wt, const cplx<T>* __restrict__ q,
     const int2* __restrict__ corner, const int* __restrict__ order, int nb, int M, int chunk,
     int probe_dir, cplx<T>* __restrict__ dP_part, const cplx<T>* __restrict__ tw,
-    int sub_wo = 1) {
+    int sub_wo = 1, int w_view = 0) {
   using S = RowShapeC<T, N>;
   constexpr int E = S::E, TL = S::TL;
   constexpr int n = N * N;
@@ -594,7 +595,8 @@ __global__ __launch_bounds__((RowShapeC<T, N>::NT)) void k_pass_c_chi(
     for (int e = 0; e < E; ++e) {
       const int x = jl + TL * e;
       const cplx<T> o = ow[e];
-      const cplx<T> d = sub_wo ? v[e] - w[e] * o : v[e];
+      const cplx<T> we = w_view ? ld(Wt + ((size_t)b * M + m) * n + (size_t)y * N + x) : w[e];
+      const cplx<T> d = sub_wo ? v[e] - we * o : v[e];
       st(prow + x, d);
       if (probe_dir) acc[e] = acc[e] + cmulc(o * qq[e], d);
     }

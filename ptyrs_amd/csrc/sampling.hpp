@@ -272,13 +272,14 @@ __global__ void k_fo_times_sum(cplx<T>* __restrict__ acc, const cplx<T>* __restr
 // (W = P·q), natural layout.
 template <typename T>
 __global__ void k_fo_chi_terms(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ W,
-                               const cplx<T>* __restrict__ chi, int M, int n) {
+                               const cplx<T>* __restrict__ chi, int M, int n, int w_view = 0) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const int b = blockIdx.y;
+  const cplx<T>* wb = w_view ? W + (size_t)b * M * n : W;
   cplx<T> s{T(0), T(0)};
   for (int m = 0; m < M; ++m)
-    s = s + cmulc(ld(W + (size_t)m * n + j), ld(chi + ((size_t)b * M + m) * n + j));
+    s = s + cmulc(ld(wb + (size_t)m * n + j), ld(chi + ((size_t)b * M + m) * n + j));
   st(out + (size_t)b * n + j, s);
 }
 

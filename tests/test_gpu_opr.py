@@ -3,8 +3,8 @@ oracle, whose OPR step the reference's fixtures pin (tests/test_oracle_opr.py): 
 seeded far-field scenario with its fixtures' eigenmodes and weights (one eigenmode; three; three
 with the main mode's weights), both costs, weights and intensity refined or held, two epochs:
 the LLK records, object, probe, eigenmodes, weights and main-mode weights. Also three slices
-(the angular-spectrum multislice fixture state) and refined PATCH offsets (the reference's
-lsqml_pos_opr state). complex128 to FP64 rounding (1e-9), complex64
+(the angular-spectrum multislice fixture state), refined PATCH offsets (the reference's
+lsqml_pos_opr state) and refined offsets under FullObject sampling. complex128 to FP64 rounding (1e-9), complex64
 to 1e-3 of the changes."""
 
 import numpy as np
@@ -150,8 +150,51 @@ def test_opr_refusals():
         g.lsqml(1, 1, 8, "poisson", 1, opr=(eig, w, None), hold=("intensity",))
     with pytest.raises(TypeError):
         g.lsqml(1, 1, 8, "poisson", 1, variation=None)
-    g.set_features(position=np.zeros((40, 2)))
-    g.set_sampling("full_object")
-    with pytest.raises(ValueError, match="OPR"):
-        g.lsqml(1, 1, 8, "poisson", 1, opr=(eig + 1.0, w, None))
     assert g.cycle == 0
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("name,cost,hold", [
+    ("lsqml_opr", POISSON, ()), ("lsqml_opr_int", AMPLITUDE, ()),
+    ("lsqml_opr_int", POISSON, ("weights",))])
+def test_opr_full_object_matches_oracle(dtype, name, cost, hold):
+    # OPR under FULL_OBJECT with refined offsets (the reference runs LSQML's OPR under both
+    # samplings): each frame's own probe planes, unmoved, against its sampled window; the
+    # frames' directions and window powers from the windows. The seeded scenario with the
+    # reference's eigenmodes and weights, zero-mean offsets within 0.4 px, two epochs.
+    from test_oracle_opr import state
+    p = state()
+    r = Ref(name)
+    var = r.variation(refine="weights" not in hold, refine_c=True)
+    start_var = var.copy()
+    d0 = np.random.default_rng(35).uniform(-0.4, 0.4, (len(p.iobs), 2))
+    d0 -= d0.mean(axis=0)
+    run = Lsqml(12, cost, 5, offsets=d0, refine=True, sampling="full_object")
+    obj, probe, llks = p.obj, p.probe, []
+    for _ in range(2):
+        obj, probe, _ = run.epoch(p.addr, p.iobs, p.optics, obj, probe, var)
+        llks.append(slice_llk(p.optics, p.views_all, p.iobs, p.addr.nmode, p.nb_obs, obj, probe,
+                              run, var).poisson)
+    g = engine(dtype)
+    obj0, probe0 = g.object, g.probe
+    g.set_sampling("full_object")
+    g.set_features(position=d0)
+    eig, w, inten = g.lsqml(2, 1, 12, cost, 5,
+                            opr=(start_var.eigenmodes, start_var.weights.reshape(-1, r.k),
+                                 start_var.intensity), hold=hold)
+    tol = TOL[dtype]
+    print("opr full object", name, cost, hold, dtype, max_rel(g.history["poisson"], llks),
+          rel_l2(g.object - obj0, obj - obj0), rel_l2(eig, var.eigenmodes))
+    assert max_rel(g.history["poisson"], llks) < tol
+    assert rel_l2(g.object - obj0, obj - obj0) < tol * 10
+    assert rel_l2(g.probe - probe0, probe - probe0) < tol * 10
+    assert rel_l2(eig, var.eigenmodes) < tol * 10
+    w0 = start_var.weights.reshape(-1, r.k)
+    if "weights" in hold:
+        assert np.array_equal(w, w0)
+    else:
+        assert max_rel((w - w0).ravel(), (var.weights.reshape(-1, r.k) - w0).ravel()) < tol * 10
+    if start_var.intensity is not None:
+        assert max_rel(inten - start_var.intensity, var.intensity - start_var.intensity) < tol * 10
+    got = g.features["position"]
+    assert max_rel((got - d0).ravel(), (run.offsets - d0).ravel()) < tol * 100
