@@ -437,7 +437,7 @@ class Engine : public EngineBase {
 
   void llk(double out[4]) override {
     ensure_workspace(std::min<size_t>(nframe_, kDefaultStack), 1);
-    if (full_object()) fo_ensure(cap_);
+    if (full_object()) fo_ensure(cap_, true);
     probe_terms();
     llk_pass(0);
     PTYRS_HIP(hipMemcpyAsync(out, hist_.p, sizeof(double) * 4, hipMemcpyDeviceToHost, stream_));
@@ -577,9 +577,6 @@ class Engine : public EngineBase {
     if (p.cost != PTYRS_AMPLITUDE && p.cost != PTYRS_POISSON)
       throw Invalid("cost must be amplitude or poisson");
     if (M_ > kMaxModes) throw Invalid(modes_error());
-    if (full_object() && nslice_ > 1)
-      throw Invalid("LSQML with position offsets under FullObject sampling runs one object "
-                    "slice on this GPU engine");
     if (nepoch == 0 || nframe_ == 0) {
       cycle += nepoch;
       return;
@@ -587,7 +584,7 @@ class Engine : public EngineBase {
     const size_t size = std::min(p.batch_size, nframe_);
     ensure_workspace(size, nepoch);
     ensure_lsqml(size);
-    if (full_object()) fo_ensure(cap_);
+    if (full_object()) fo_ensure(cap_, true);
     if (opr_on_) opr_ensure(cap_);
     std::vector<int> order(nepoch * nframe_);
     std::vector<std::vector<std::pair<int, int>>> spans(nepoch);
@@ -615,7 +612,7 @@ class Engine : public EngineBase {
       PTYRS_HIP(hipMemcpyAsync(eord_.p, order_.as<int>() + ic * nframe_, sizeof(int) * nframe_,
                                hipMemcpyDeviceToDevice, stream_));
       const int* ord = eord_.as<int>();
-      if (full_object()) {
+      if (full_object() && nslice_ == 1) {
         for (auto& s : spans[ic]) lsqml_batch_sampled(p, ord + s.first, s.second, s.first);
       } else if (nslice_ > 1 || pos_on_ || opr_on_) {
         for (auto& s : spans[ic]) lsqml_batch_ms(p, ord + s.first, s.second, s.first);
@@ -2264,10 +2261,11 @@ class Engine : public EngineBase {
   // buffers) in 64 KiB of LDS.
   static constexpr size_t fo_max_side() { return sizeof(T) == 4 ? kAxisMax : kAxisMax / 2; }
 
-  void fo_ensure(size_t cap) {
-    if (nslice_ != 1)
-      throw Invalid("FullObject sampling with several object slices is not run on this GPU "
-                    "engine");
+  // `slices`: the caller runs several object slices (LSQML, the LLK and the model intensities).
+  void fo_ensure(size_t cap, bool slices = false) {
+    if (nslice_ != 1 && !slices)
+      throw Invalid("FullObject sampling with several object slices runs LSQML, the LLK and "
+                    "the model intensities on this GPU engine");
     if (nobj_ != 1)
       throw Invalid("FullObject sampling with several object modes is not run on this GPU "
                     "engine");
@@ -2306,6 +2304,42 @@ class Engine : public EngineBase {
     fo_tall_.ensure(sizeof(int2) * std::max<size_t>(1, nframe_));
     fo_part_.ensure(sizeof(T) * kFoBlocks);
     fo_cap_ = std::max(fo_cap_, cap);
+    if (nslice_ > 1) fo_swin_.ensure(sizeof(cplx<T>) * (nslice_ - 1) * fo_cap_ * n);
+  }
+  // Several slices under FullObject: every slice's windows of the batch (slice 0's in fo_win_,
+  // slice s's in fo_swin_ plane s − 1, tall layout at the corners fo_tall_), read by the
+  // multislice passes in place of the slices at the frames' corners.
+  DevBuf fo_swin_;
+  bool ms_fo() const { return full_object() && nslice_ > 1; }
+  cplx<T>* fo_swin(size_t s) {
+    return s == 0 ? fo_win_.as<cplx<T>>() : fo_swin_.as<cplx<T>>() + (s - 1) * fo_cap_ * n;
+  }
+  void fo_slice_windows(const int* ord, int nb) {
+    fo_points(ord, nb);
+    for (size_t sl = 0; sl < nslice_; ++sl) {
+      fo_object_spectrum(slice_obj(sl));
+      fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 0, fo_swin(sl), (long long)n);
+    }
+  }
+  // A slice direction as the batch reads it: its sampled windows into fo_dwin_.
+  void fo_dir_windows(const cplx<T>* dir, int nb) {
+    fo_object_spectrum(dir);
+    fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 0, fo_dwin_.p, (long long)n);
+  }
+  // The position sums on slice s under FullObject: its windows' derivatives over the position
+  // against the wave entering it (w, per slot with w_slot), its phase q and χ (psi_).
+  void fo_position_sums(size_t s, const cplx<T>* w, int w_slot, const cplx<T>* q, int q_frame,
+                        const int* ord, int nb) {
+    (void)ord;
+    cplx<T>* grad = pgrad_.as<cplx<T>>() + cap_ * n;
+    run(K_OTHER, [&] {
+      fo_object_spectrum(slice_obj(s));
+      fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 1, grad, 2ll * n);
+      fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 2, grad + n, 2ll * n);
+      hipLaunchKernelGGL(k_pos_sums<T>, dim3(aux_blocks(), nb), dim3(THREADS), 0, stream_,
+                         (const cplx<T>*)grad, w, q, q_frame, (const cplx<T>*)psi_.as<cplx<T>>(),
+                         N, M_, pos_part_.as<double>(), w_slot);
+    });
   }
 
   // One pass of object-side line transforms along `axis` (0: rows of nxo, 1: columns of nyo).
@@ -2533,6 +2567,8 @@ class Engine : public EngineBase {
 
   Waves stack_waves(const int* ord, int nb) {
     Waves w{P_.as<cplx<T>>(), 0, q_.as<cplx<T>>(), 0};
+    // Under FullObject the probe does not move (the windows do).
+    const bool moving = pos_on_ && !full_object();
     if (opr_on_) {
       // Each frame's own probe planes (OPR), moved by its offset with offsets:
       // conj(q0)·DFT⁻¹(DFT(q0·P_b)·exp(−2πi f·d_b)) (ptyrs/src/engine/lsqml.rs:507-520).
@@ -2546,7 +2582,7 @@ class Engine : public EngineBase {
                            (const double*)opr_w_.as<double>(),
                            opr_c_on_ ? (const double*)opr_c_.as<double>() : nullptr, ord,
                            opr_k_, M_, n);
-        if (pos_on_) {
+        if (moving) {
           if (q0)
             hipLaunchKernelGGL((k_times_q<T, false>), dim3((n + 255) / 256, K), dim3(256), 0,
                                stream_, pl, q0, n);
@@ -2568,9 +2604,9 @@ class Engine : public EngineBase {
       });
       w.w0 = pl;
       w.w0_view = 1;
-      if (!pos_on_) return w;
+      if (!moving) return w;
     } else {
-      if (!pos_on_) return w;
+      if (!moving) return w;
       moved_probes(ord, nb);
     }
     w.w0 = inc0_.as<cplx<T>>();
@@ -2671,17 +2707,22 @@ class Engine : public EngineBase {
   template <int OP>
   void window_rows(cplx<T>* out, size_t s, const cplx<T>* w, int w_view, const cplx<T>* q,
                    int q_frame, const int* ord, int nb) {
+    const bool fo = ms_fo();
     hipLaunchKernelGGL((k_rows_ms<T, N, OP>), dim3(S::NBLK, nb * M_), dim3(S::NT), 0, stream_,
-                       out, (const cplx<T>*)slice_obj(s), int(nxo_), w, w_view, q,
-                       corner_.as<int2>(), ord, M_, q_frame, tw_.as<cplx<T>>());
+                       out, (const cplx<T>*)(fo ? fo_swin(s) : slice_obj(s)),
+                       fo ? N : int(nxo_), w, w_view, q,
+                       fo ? fo_tall_.as<int2>() : corner_.as<int2>(), ord, M_, q_frame,
+                       tw_.as<cplx<T>>());
   }
 
   template <int OP>
   void window(cplx<T>* out, size_t s, const cplx<T>* w, int w_view, const cplx<T>* q,
               int q_frame, const int* ord, int nb) {
+    const bool fo = ms_fo();
     hipLaunchKernelGGL((k_ms_window<T, OP>), dim3((n + 255) / 256, nb * M_), dim3(256), 0,
-                       stream_, out, slice_obj(s), int(nxo_), w, w_view, q, corner_.as<int2>(),
-                       ord, N, M_, q_frame);
+                       stream_, out, fo ? fo_swin(s) : slice_obj(s), fo ? N : int(nxo_), w,
+                       w_view, q, fo ? fo_tall_.as<int2>() : corner_.as<int2>(), ord, N, M_,
+                       q_frame);
   }
 
   // The wave entering slice s: slice 0's from `w`, the others' kept per view.
@@ -3575,15 +3616,15 @@ class Engine : public EngineBase {
     ensure_workspace(stack, 1);
     ensure_lsqml(stack);
     const bool fo = full_object();
-    if (fo) fo_ensure(cap_);
+    if (fo) fo_ensure(cap_, true);
     probe_terms();
-    if (fo) run(K_OTHER, [&] { fo_object_spectrum(O_.as<cplx<T>>()); });
+    if (fo && nslice_ == 1) run(K_OTHER, [&] { fo_object_spectrum(O_.as<cplx<T>>()); });
     zero(prev_.as<T>(), 2);
     std::vector<T> ic(stack * n);
     for (size_t f0 = 0; f0 < nframe_; f0 += stack) {
       const int nb = int(std::min(stack, nframe_ - f0));
       const int* ord = iota_.as<int>() + f0;
-      if (fo) {
+      if (fo && nslice_ == 1) {
         run(K_OTHER, [&] {
           fo_points(ord, nb);
           fo_object_windows(nb);
@@ -3591,6 +3632,7 @@ class Engine : public EngineBase {
         pass_a(ord, nb, false, MixArgs<T>{MIX_NONE, {}, {}, T(0), T(0)}, nullptr,
                fo_win_.as<cplx<T>>(), N, fo_tall_.as<int2>());
       } else if (nslice_ > 1 || pos_on_) {
+        if (fo) run(K_OTHER, [&] { fo_slice_windows(ord, nb); });
         ms_forward(ord, nb, stack_waves(ord, nb));
         run(K_PASS_A, [&] { rows(false, nb * M_); });
       } else {
@@ -3620,13 +3662,24 @@ class Engine : public EngineBase {
   // (Eq. 23b) on the way, slice 0 jointly with the probe (Eq. 22), the position directions on
   // slice S/2, then every update from the state on entry and the position step. One object
   // mode; unfused (one kernel per stage).
+  // Under FullObject (several slices) every slice is read through the batch's sampled windows
+  // (fo_slice_windows; ptyrs/src/engine/multislice.rs:70-171 with the sampler): the slice
+  // directions' terms are placed by the sampling's adjoint, their step sums read the
+  // directions' windows, the position sums the windows' derivatives, and the probe does not
+  // move.
   void lsqml_batch_ms(const ptyrs_lsqml_params& p, const int* ord, int nb, int off) {
     const int nt = ntiles();
     const bool gauss = p.cost == PTYRS_AMPLITUDE;
     const size_t S = nslice_;
     const int ls = int(2 + S);
     const int K = nb * M_;
+    const bool fo = ms_fo();
+    const long long nol = (long long)(nyo_ * nxo_);
     run(K_BIN, [&] {
+      if (fo) {
+        fo_slice_windows(ord, nb);
+        return;
+      }
       hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
                          corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
                          lcount_.as<int>(), (const T*)nullptr, M_, int(nyo_), int(nxo_),
@@ -3678,17 +3731,37 @@ class Engine : public EngineBase {
       const cplx<T>* qs = s + 1 == S ? wv.qlast : nullptr;
       const int qf = s + 1 == S ? wv.q_frame : 0;
       run(K_OBJ_UPDATE, [&] {
+        if (fo) {
+          // Σ_m conj(q_s·in_s)·χ_s on each window, placed, then preconditioned.
+          hipLaunchKernelGGL(k_fo_chi_terms<T>, dim3((n + 255) / 256, nb), dim3(256), 0,
+                             stream_, fo_dwin_.as<cplx<T>>(), in.first, (const cplx<T>*)chi, M_,
+                             n, in.second, qs);
+          fo_place_spectrum(fo_dwin_.as<cplx<T>>(), (long long)n, nb, false);
+          fo_add_placed<AS_PLAIN>(slice_dir(s), 1.0);
+          hipLaunchKernelGGL(k_fo_precondition<T>, dim3(unsigned((nol + 255) / 256)), dim3(256),
+                             0, stream_, slice_dir(s), (const T*)pre_.as<T>(),
+                             (const T*)floor_.as<T>(), nol);
+          return;
+        }
         hipLaunchKernelGGL(k_lsqml_obj_dir<T>, dim3(nt), dim3(THREADS), 0, stream_,
                            slice_dir(s), in.first, (const cplx<T>*)chi, N, M_, corner_.as<int2>(),
                            ord, list_.as<int>(), lcount_.as<int>(), int(cap_), int(nyo_),
                            int(nxo_), ntx_, pre_.as<T>(), floor_.as<T>(), in.second, qs, qf);
       });
-      if (pos_refine_ && s == sp) position_sums(s, in.first, qs, qf, ord, nb);
+      if (pos_refine_ && s == sp) {
+        if (fo)
+          fo_position_sums(s, in.first, in.second, qs, qf, ord, nb);
+        else
+          position_sums(s, in.first, qs, qf, ord, nb);
+      }
       if (s == 0) break;
       run(K_OTHER, [&] {
+        if (fo) fo_dir_windows(slice_dir(s), nb);
         hipLaunchKernelGGL(k_ms_dots<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
-                           (const cplx<T>*)slice_dir(s), int(nxo_), in.first, qs,
-                           (const cplx<T>*)chi, N, M_, corner_.as<int2>(), ord,
+                           fo ? (const cplx<T>*)fo_dwin_.as<cplx<T>>()
+                              : (const cplx<T>*)slice_dir(s),
+                           fo ? N : int(nxo_), in.first, qs, (const cplx<T>*)chi, N, M_,
+                           fo ? fo_tall_.as<int2>() : corner_.as<int2>(), ord,
                            xi_part2_.as<double>(), qf);
         hipLaunchKernelGGL(k_ms_slice_step, dim3(1), dim3(1024), 0, stream_,
                            xi_part2_.as<double>(), xb, nb, int(S) * M_, log, ls, int(1 + s),
@@ -3701,7 +3774,7 @@ class Engine : public EngineBase {
     }
     // Slice 0: the probe direction (moved back per frame with offsets), then the joint step.
     const cplx<T>* q0 = S == 1 ? q_.as<cplx<T>>() : nullptr;
-    if (pos_on_) {
+    if (pos_on_ && !fo) {
       cplx<T>* g = gspec_.as<cplx<T>>();
       run(K_PROBE, [&] {
         PTYRS_HIP(hipMemcpyAsync(g, chi, sizeof(cplx<T>) * size_t(K) * n,
@@ -3740,19 +3813,25 @@ class Engine : public EngineBase {
     } else {
       run(K_PROBE, [&] {
         hipLaunchKernelGGL(k_ms_probe_dir<T>, dim3((n + 255) / 256, M_), dim3(256), 0, stream_,
-                           dPm_.as<cplx<T>>(), (const cplx<T>*)slice_obj(0), int(nxo_), q0,
-                           (const cplx<T>*)chi, corner_.as<int2>(), ord, nb, N, M_,
+                           dPm_.as<cplx<T>>(),
+                           fo ? (const cplx<T>*)fo_swin(0) : (const cplx<T>*)slice_obj(0),
+                           fo ? N : int(nxo_), q0, (const cplx<T>*)chi,
+                           fo ? fo_tall_.as<int2>() : corner_.as<int2>(), ord, nb, N, M_,
                            T(1) / T(nb), 0);
       });
     }
     const int total = int(nprobe_ * n);
     const double lam = 1.2e-7 * scale_ * double(M_);
     run(K_OTHER, [&] {
+      if (fo) fo_dir_windows(slice_dir(0), nb);
       hipLaunchKernelGGL(k_lsqml_dots<T>, dim3(xb, nb), dim3(THREADS), 0, stream_,
-                         (const cplx<T>*)slice_dir(0), (const cplx<T>*)slice_obj(0), int(nxo_),
-                         wv.w0, (const cplx<T>*)dPm_.as<cplx<T>>(),
+                         fo ? (const cplx<T>*)fo_dwin_.as<cplx<T>>()
+                            : (const cplx<T>*)slice_dir(0),
+                         fo ? (const cplx<T>*)fo_swin(0) : (const cplx<T>*)slice_obj(0),
+                         fo ? N : int(nxo_), wv.w0, (const cplx<T>*)dPm_.as<cplx<T>>(),
                          q0 ? q0 : (const cplx<T>*)ones_.as<cplx<T>>(), (const cplx<T>*)chi, N,
-                         M_, corner_.as<int2>(), ord, dots_part_.as<double>(), wv.w0_view, q0);
+                         M_, fo ? fo_tall_.as<int2>() : corner_.as<int2>(), ord,
+                         dots_part_.as<double>(), wv.w0_view, q0);
       hipLaunchKernelGGL(k_lsqml_step<T>, dim3(1), dim3(1024), 0, stream_,
                          dots_part_.as<double>(), xb, nb, lam, int(S) * M_, sums_.as<double>(),
                          log, step, ls);
@@ -4119,11 +4198,11 @@ class Engine : public EngineBase {
   void llk_pass_to(double* result, size_t step_frames) {
     const int step = int(std::max<size_t>(step_frames, 1));
     const bool fo = full_object();
-    if (fo) run(K_OTHER, [&] { fo_object_spectrum(O_.as<cplx<T>>()); });
+    if (fo && nslice_ == 1) run(K_OTHER, [&] { fo_object_spectrum(O_.as<cplx<T>>()); });
     for (int f0 = 0; f0 < int(nframe_); f0 += step) {
       const int nb = std::min(step, int(nframe_) - f0);
       const int* ord = iota_.as<int>() + f0;
-      if (fo) {
+      if (fo && nslice_ == 1) {
         // FullObject: the frames' sampled windows, read at their tall corners.
         run(K_OTHER, [&] {
           fo_points(ord, nb);
@@ -4133,6 +4212,7 @@ class Engine : public EngineBase {
                opr_on_ ? opr_waves_sampled(ord, nb) : nullptr, fo_win_.as<cplx<T>>(), N,
                fo_tall_.as<int2>());
       } else if (nslice_ > 1 || pos_on_ || opr_on_) {
+        if (fo) run(K_OTHER, [&] { fo_slice_windows(ord, nb); });
         ms_forward(ord, nb, stack_waves(ord, nb));
         run(K_PASS_A, [&] { rows(false, nb * M_); });
       } else {

@@ -268,18 +268,21 @@ __global__ void k_fo_times_sum(cplx<T>* __restrict__ acc, const cplx<T>* __restr
   st(acc + j, cplx<T>{T(fr * e.x - fi * e.y), T(fr * e.y + fi * e.x)});
 }
 
-// Per slot the modes' object-direction terms on its window: out[b][j] = Σ_m conj(W_m[j])·χ_bm[j]
-// (W = P·q), natural layout.
+// Per slot the modes' object-direction terms on its window: out[b][j] = Σ_m conj(W_m[j]·q[j])·
+// χ_bm[j] (W = P·q and q null, or a slice's incident waves and its plane phase), natural layout;
+// with w_view every slot's modes have their own W (planes slot·M + m).
 template <typename T>
 __global__ void k_fo_chi_terms(cplx<T>* __restrict__ out, const cplx<T>* __restrict__ W,
-                               const cplx<T>* __restrict__ chi, int M, int n, int w_view = 0) {
+                               const cplx<T>* __restrict__ chi, int M, int n, int w_view = 0,
+                               const cplx<T>* __restrict__ q = nullptr) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const int b = blockIdx.y;
   const cplx<T>* wb = w_view ? W + (size_t)b * M * n : W;
+  const cplx<T> qj = q ? ld(q + j) : cplx<T>{T(1), T(0)};
   cplx<T> s{T(0), T(0)};
   for (int m = 0; m < M; ++m)
-    s = s + cmulc(ld(wb + (size_t)m * n + j), ld(chi + ((size_t)b * M + m) * n + j));
+    s = s + cmulc(ld(wb + (size_t)m * n + j) * qj, ld(chi + ((size_t)b * M + m) * n + j));
   st(out + (size_t)b * n + j, s);
 }
 

@@ -105,6 +105,49 @@ def test_opr_multislice_matches_oracle(dtype):
 
 
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_opr_multislice_full_object_matches_oracle(dtype):
+    # Three slices under FULL_OBJECT with two eigenmodes and refined offsets (the reference runs
+    # this combination): each frame's own probe planes, unmoved, through every slice's sampled
+    # windows. The reference's lsqml_multislice_pos_full state and offsets: its slices carry
+    # structure, so the position steps are well conditioned (on the featureless slices of the
+    # plain multislice state a 1e-15 change of the start moves the refined offsets by 1e-7).
+    from test_gpu_multislice import start as ms_start
+    from test_gpu_positions import offsets_of
+    from test_oracle_multislice import state as multislice_state
+    addr, _, probe, cases = multislice_state()
+    start = np.array(table("lsqml_multislice_pos_start_f64.txt"))
+    objects = [(start[start[:, 0] == s][:, 3] + 1j * start[start[:, 0] == s][:, 4])
+               .reshape(1, 40, 40) for s in range(3)]
+    d0 = offsets_of("lsqml_multislice_pos_full_offsets_f64.txt", 0)
+    optics, iobs, nb_obs, _ = cases[0]
+    rng = np.random.default_rng(31)
+    n = probe.shape[-1]
+    eig = 0.05 * (rng.normal(size=(2, n, n)) + 1j * rng.normal(size=(2, n, n)))
+    w0 = rng.uniform(-0.5, 0.5, (addr.nframe, 2))
+    var = ProbeVariation(eig, w0.ravel())
+    run = Lsqml(8, POISSON, 7, offsets=d0, refine=True, sampling="full_object")
+    obj, pr, llks = list(objects), probe, []
+    for _ in range(2):
+        obj, pr, _ = run.epoch(addr, iobs, optics, obj, pr, var)
+        llks.append(slice_llk(optics, addr.views(), iobs, addr.nmode, nb_obs, obj, pr, run,
+                              var).poisson)
+    g, _, _ = ms_start(0, dtype)
+    g.set_object(np.stack(objects))
+    g.set_sampling("full_object")
+    g.set_features(position=d0)
+    geig, gw, _ = g.lsqml(2, 1, 8, "poisson", 7, opr=(eig, w0, None))
+    tol = TOL[dtype]
+    print("opr multislice full object", dtype, max_rel(g.history["poisson"], llks),
+          rel_l2(g.objects, np.stack(obj)), rel_l2(geig, var.eigenmodes))
+    assert max_rel(g.history["poisson"], llks) < tol
+    assert rel_l2(g.objects, np.stack(obj)) < tol * 10
+    assert rel_l2(geig, var.eigenmodes) < tol * 10
+    assert max_rel((gw - w0).ravel(), (var.weights.reshape(-1, 2) - w0).ravel()) < tol * 10
+    got = g.features["position"]
+    assert max_rel((got - d0).ravel(), (run.offsets - d0).ravel()) < tol * 100
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize("cost", [AMPLITUDE, POISSON])
 def test_opr_with_positions_matches_oracle(dtype, cost):
     # OPR with refined PATCH offsets: the reference's lsqml_pos_opr state (three eigenmodes,

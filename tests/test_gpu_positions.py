@@ -22,8 +22,8 @@ pytestmark = pytest.mark.gpu
 TOL = {"c128": 1e-9, "c64": 1e-3}
 
 
-def run_oracle(addr, optics, iobs, nb_obs, obj, probe, d0, nepoch, batch, seed):
-    run = Lsqml(batch, POISSON, seed, offsets=d0, refine=True)
+def run_oracle(addr, optics, iobs, nb_obs, obj, probe, d0, nepoch, batch, seed, sampling=None):
+    run = Lsqml(batch, POISSON, seed, offsets=d0, refine=True, sampling=sampling)
     llks = []
     for _ in range(nepoch):
         obj, probe, _ = run.epoch(addr, iobs, optics, obj, probe)
@@ -89,6 +89,35 @@ def test_multislice_positions_match_oracle(dtype):
     g.set_features(position=d0)
     g.lsqml(3, 1, 8, "poisson", 7)
     tol = TOL[dtype]
+    assert max_rel(g.history["poisson"], o_llk) < tol
+    assert rel_l2(g.objects, np.stack(o_obj)) < tol * 10
+    assert rel_l2(g.probe, o_probe) < tol * 10
+    got = g.features["position"]
+    assert max_rel((got - d0).ravel(), (o_d - d0).ravel()) < tol * 100
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_multislice_full_object_positions_match_oracle(dtype):
+    # Three slices under FULL_OBJECT (every slice read through the batch's sampled windows, the
+    # slice directions placed by the sampling's adjoint, the probe unmoved): the reference's
+    # lsqml_multislice_pos_full state and offsets, three epochs of batches of 8, refined.
+    addr, objects, probe, cases = multislice_state()
+    start = np.array(table("lsqml_multislice_pos_start_f64.txt"))
+    objects = [(start[start[:, 0] == s][:, 3] + 1j * start[start[:, 0] == s][:, 4])
+               .reshape(1, 40, 40) for s in range(3)]
+    d0 = offsets_of("lsqml_multislice_pos_full_offsets_f64.txt", 0)
+    optics, iobs, nb_obs, _ = cases[0]
+    o_obj, o_probe, o_d, o_llk = run_oracle(addr, optics, iobs, nb_obs, list(objects), probe,
+                                            d0, 3, 8, 7, sampling="full_object")
+    from test_gpu_multislice import start as ms_start
+    g, _, _ = ms_start(0, dtype)
+    g.set_object(np.stack(objects))
+    g.set_sampling("full_object")
+    g.set_features(position=d0)
+    g.lsqml(3, 1, 8, "poisson", 7)
+    tol = TOL[dtype]
+    print("multislice full object", dtype, max_rel(g.history["poisson"], o_llk),
+          rel_l2(g.objects, np.stack(o_obj)), rel_l2(g.probe, o_probe))
     assert max_rel(g.history["poisson"], o_llk) < tol
     assert rel_l2(g.objects, np.stack(o_obj)) < tol * 10
     assert rel_l2(g.probe, o_probe) < tol * 10
