@@ -243,7 +243,7 @@ typedef struct {
  * that model pty-chi's orthogonal probe relaxation of probe mode 0; with nepoch >= 1 the
  * eigenmodes, weights and intensity of `var` are overwritten with the values the epochs leave.
  * Runs with one slice or several, with or without position offsets (refined or held) under
- * PATCH or FullObject sampling (FullObject with one slice). */
+ * PATCH or FullObject sampling. */
 int ptyrs_engine_lsqml_opr(ptyrs_engine* e, const ptyrs_lsqml_params* params, size_t nepoch,
                            size_t calc_llk, ptyrs_opr* var);
 
@@ -314,7 +314,9 @@ int ptyrs_engine_launch(ptyrs_engine* e, ptyrs_launch* out);
  * with complex waves only); under FullObject sampling every algorithm (AP, DM and RAAR through
  * the FullObject psi_to_obj_probe, ptyrs/src/engine/features.rs:653-793: the probe's first
  * conjugate-gradient step through the sampled windows, the object's preconditioned CG solve),
- * the LLK pass and the model intensities, with one object slice and mode. */
+ * the LLK pass and the model intensities, with one object mode (LSQML, the LLK pass and the
+ * model intensities also through several slices, every slice read through its sampled
+ * windows). */
 int ptyrs_engine_set_position(ptyrs_engine* e, const double* offsets, size_t len, int refine);
 
 /* GpuPtycho::features' position (ptyrs/src/engine/gpu/mod.rs:823-825): *state = 0 (none),
