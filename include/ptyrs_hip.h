@@ -360,6 +360,22 @@ int ptyrs_engine_grad(ptyrs_engine* e, int cost, double* value, double* obj_grad
  * (ptyrs/tests/engine_lsqml_multislice.rs:146-182; the reference keeps them on the device). */
 int ptyrs_engine_intensities(ptyrs_engine* e, double* out, size_t len);
 
+/* Live ingestion (LiveAp::push and epoch, ptyrs/src/live/recon.rs:184-266; the frames' ring,
+ * ptyrs/src/live/ring.rs:116-175): the engine's scan is the planned one (its positions or
+ * corners fix every frame's view up front); frames land in scan order and every algorithm,
+ * the LLK and the model intensities run over the frames landed so far (the reference's live
+ * AP is ptyrs_engine_ap with update_probe = 0 on the landed frames).
+ * ptyrs_engine_live_reset leaves no frame landed (the engine holds every planned frame's slot,
+ * so landing is a copy, never a reallocation). ptyrs_engine_live_push lands the next nframes
+ * planned frames: counts [nframes][ny][nx] in detector order, exactly one of counts_u16
+ * (65535 masked) and counts_f32 (negative masked); PTYRS_EINVAL past the planned frames.
+ * ptyrs_engine_live_frames reports the landed and planned counts (either pointer may be
+ * null). An engine starts with every planned frame landed. */
+int ptyrs_engine_live_reset(ptyrs_engine* e);
+int ptyrs_engine_live_push(ptyrs_engine* e, const uint16_t* counts_u16, const float* counts_f32,
+                           size_t nframes);
+int ptyrs_engine_live_frames(ptyrs_engine* e, size_t* landed, size_t* planned);
+
 /* GpuPtycho::llk (ptyrs/src/engine/gpu/mod.rs:853-868): normalised LLK of the current state. */
 int ptyrs_engine_llk(ptyrs_engine* e, ptyrs_llk_record* out);
 /* The LLK of the current state, as ptyrs_engine_llk gives it (the same bits), for callers that

@@ -159,6 +159,10 @@ _SIGNATURES = [
     ("ptyrs_engine_set_sampling", [_P, ctypes.c_int], ctypes.c_int),
     ("ptyrs_engine_grad", [_P, ctypes.c_int, _D, _D, _SZ, _D, _SZ], ctypes.c_int),
     ("ptyrs_engine_intensities", [_P, _D, _SZ], ctypes.c_int),
+    ("ptyrs_engine_live_reset", [_P], ctypes.c_int),
+    ("ptyrs_engine_live_push", [_P, ctypes.POINTER(ctypes.c_uint16),
+                                ctypes.POINTER(ctypes.c_float), _SZ], ctypes.c_int),
+    ("ptyrs_engine_live_frames", [_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)], ctypes.c_int),
     ("ptyrs_group_rpie", [ctypes.POINTER(_P), _SZ, ctypes.POINTER(_RpieC), _SZ], ctypes.c_int),
     ("ptyrs_group_rpie_graph", [ctypes.POINTER(_P), _SZ, ctypes.POINTER(_RpieC), _SZ],
      ctypes.c_int),
@@ -641,11 +645,41 @@ class GpuPtycho:
         return v.value, go, gp
 
     def intensities(self):
-        """The model's detector intensities Σ_m |U ψ_m|² of every frame, `[nframe, ny, nx]`,
-        detector order (the counts' layout)."""
-        out = np.empty((self.nframe,) + tuple(self._probe_shape[1:]))
+        """The model's detector intensities Σ_m |U ψ_m|² of every landed frame,
+        `[nframe, ny, nx]`, detector order (the counts' layout)."""
+        out = np.empty((self.live_frames[0],) + tuple(self._probe_shape[1:]))
         _check(library().ptyrs_engine_intensities(self._h, _ptr(out, ctypes.c_double), out.size))
         return out
+
+    def live_reset(self):
+        """Live ingestion (LiveAp, ptyrs/src/live/recon.rs:184-266): no frame of the planned
+        scan is landed; every algorithm, the LLK and the intensities run over the frames
+        landed so far. An engine starts with every planned frame landed."""
+        _check(library().ptyrs_engine_live_reset(self._h))
+
+    def live_push(self, counts):
+        """Lands the next planned frames: `counts` `[k, ny, nx]` in detector order, uint16
+        (65535 masked) or float (negative masked); ValueError past the planned frames."""
+        c = np.asarray(counts)
+        if c.ndim != 3 or tuple(c.shape[1:]) != tuple(self._probe_shape[1:]):
+            raise ValueError("counts must be [k, ny, nx] frames of the scan's shape")
+        u16 = f32 = None
+        if c.dtype == np.uint16:
+            c = np.ascontiguousarray(c)
+            u16 = _ptr(c, ctypes.c_uint16)
+        elif np.issubdtype(c.dtype, np.floating):
+            c = np.ascontiguousarray(c, dtype=np.float32)
+            f32 = _ptr(c, ctypes.c_float)
+        else:
+            raise TypeError("counts must be uint16 or float")
+        _check(library().ptyrs_engine_live_push(self._h, u16, f32, c.shape[0]))
+
+    @property
+    def live_frames(self):
+        """(landed, planned) frames."""
+        a, b = _SZ(), _SZ()
+        _check(library().ptyrs_engine_live_frames(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def set_checkpoint(self, every):
         """Keeps the wave entering every `every`-th slice for the multislice adjoint and

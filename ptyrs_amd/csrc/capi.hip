@@ -627,6 +627,37 @@ int ptyrs_engine_intensities(ptyrs_engine* h, double* out, size_t len) {
   });
 }
 
+int ptyrs_engine_live_reset(ptyrs_engine* h) {
+  return guarded(h, true, [&](EngineBase& e) { e.live_reset(); });
+}
+
+int ptyrs_engine_live_push(ptyrs_engine* h, const uint16_t* counts_u16, const float* counts_f32,
+                           size_t nframes) {
+  return guarded(h, true, [&](EngineBase& e) {
+    if (nframes > e.planned() - e.landed())
+      throw Invalid("live_push lands more frames than are planned");
+    // The counts' conversion of the scan (prepare_counts) on the landing frames alone.
+    ptyrs_scan s{};
+    s.nframe = nframes;
+    s.ny = s.nx = e.frame_side();
+    s.counts_u16 = counts_u16;
+    s.counts_f32 = counts_f32;
+    ExitModel exit;
+    exit.reverse = e.counts_reverse();
+    exit.dft_plane = e.counts_dft_plane();
+    Prepared p;
+    prepare_counts(s, exit, p);
+    e.live_push(p.counts.data(), nframes, p.nb_obs, p.total, p.max_frame_total);
+  });
+}
+
+int ptyrs_engine_live_frames(ptyrs_engine* h, size_t* landed, size_t* planned) {
+  return guarded(h, false, [&](EngineBase& e) {
+    if (landed) *landed = e.landed();
+    if (planned) *planned = e.planned();
+  });
+}
+
 int ptyrs_engine_llk(ptyrs_engine* h, ptyrs_llk_record* out) {
   return guarded(h, true, [&](EngineBase& e) {
     if (!out) throw Invalid("null output");

@@ -203,6 +203,18 @@ class EngineBase {
   size_t checkpoint = 0;
   virtual void set_checkpoint(size_t every) = 0;
   virtual size_t wave_bytes() const = 0;  // device bytes holding the kept incident waves
+  // Live ingestion (ptyrs/src/live/recon.rs:184-266, ring.rs): the scan's planned frames land
+  // in order; every algorithm runs over the frames landed so far. live_reset leaves none
+  // landed; live_push lands the next `nframes` frames, `counts` in the device's index order
+  // (the counts' conversion of the scan), with their unmasked pixel count, total and largest
+  // frame total.
+  virtual void live_reset() = 0;
+  virtual void live_push(const float* counts, size_t nframes, size_t nb_obs, double total,
+                         double max_frame_total) = 0;
+  size_t landed() const { return nframe_; }
+  size_t planned() const { return corners.size() / 2; }
+  bool counts_reverse() const { return reverse_counts_; }
+  bool counts_dft_plane() const { return dft_plane_counts_; }
 
   void synchronize() { PTYRS_HIP(hipStreamSynchronize(stream_)); }
 
@@ -211,6 +223,7 @@ class EngineBase {
   size_t probe_len() const { return 2 * nprobe_ * ny_ * nx_; }
   size_t nprobe() const { return nprobe_; }
   size_t frame_len() const { return nframe_ * ny_ * nx_; }
+  size_t frame_side() const { return ny_; }
 
   std::vector<Record> history;
   std::vector<StepRecord> lsqml_steps;  // the last LSQML epoch's
@@ -239,6 +252,7 @@ class EngineBase {
   size_t nframe_ = 0, ny_ = 0, nx_ = 0, nprobe_ = 0, nobj_ = 0, nyo_ = 0, nxo_ = 0;
   size_t nslice_ = 1;
   size_t nb_obs_ = 0;
+  bool reverse_counts_ = false, dft_plane_counts_ = true;  // the counts' index map
 
   bool profiling_ = false;
   bool graph_ = true;
@@ -262,6 +276,7 @@ class EngineBase {
     }
     key.push_back(DevBuf::generation.load());
     key.push_back(uint64_t(launch_key()));
+    key.push_back(uint64_t(nframe_));  // frames landed (live ingestion)
     if (g.exec == nullptr || g.key != key) {
       if (g.exec) {
         PTYRS_HIP(hipGraphExecDestroy(g.exec));
@@ -363,8 +378,8 @@ class Engine : public EngineBase {
     hist_slots_ = std::max<size_t>(pre.llk_slots, 1);
     psi_f16_ = pre.psi_f16;
     half_scale_ = half_scale(pre.max_frame_total);
-    reverse_ = pre.exit.reverse;
-    dft_plane_ = pre.exit.dft_plane;
+    reverse_ = reverse_counts_ = pre.exit.reverse;
+    dft_plane_ = dft_plane_counts_ = pre.exit.dft_plane;
     plane_quad_[0] = pre.exit.plane_quad[0];
     plane_quad_[1] = pre.exit.plane_quad[1];
     // Sampling::auto (ptyrs/src/engine/sampling.rs:38-49): FullObject in the near field when
@@ -2484,20 +2499,63 @@ class Engine : public EngineBase {
       pos_on_ = pos_refine_ = false;
       return;
     }
-    for (size_t i = 0; i < 2 * nframe_; ++i)
+    // One offset per planned frame (the frames not landed yet keep theirs for when they land).
+    const size_t nall = planned();
+    for (size_t i = 0; i < 2 * nall; ++i)
       if (!std::isfinite(offsets[i])) throw Invalid("position offsets must be finite");
-    offs_.ensure(sizeof(double) * 2 * std::max<size_t>(1, nframe_));
-    if (nframe_)
-      PTYRS_HIP(hipMemcpy(offs_.p, offsets, sizeof(double) * 2 * nframe_,
-                          hipMemcpyHostToDevice));
+    offs_.ensure(sizeof(double) * 2 * std::max<size_t>(1, nall));
+    if (nall)
+      PTYRS_HIP(hipMemcpy(offs_.p, offsets, sizeof(double) * 2 * nall, hipMemcpyHostToDevice));
     pos_on_ = true;
     pos_refine_ = refine;
   }
+  // Live ingestion: the counts buffer holds every planned frame's slot from the start, so a
+  // landing frame is one copy into its slot (through pinned staging), no reallocation; the
+  // epochs' graphs are keyed by the landed count and captured again when it changes.
+  DevBuf stage_dev_;
+  void* stage_host_ = nullptr;
+  size_t stage_bytes_ = 0;
+  void live_reset() override {
+    synchronize();
+    nframe_ = 0;
+    nb_obs_ = 0;
+    total_ = 0.0;
+    max_frame_total_ = 0.0;
+  }
+  void live_push(const float* counts, size_t nframes, size_t nb_obs, double total,
+                 double max_frame_total) override {
+    if (nframes == 0) return;
+    if (nframe_ + nframes > planned())
+      throw Invalid("live_push lands " + std::to_string(nframes) + " frames past the " +
+                    std::to_string(planned() - nframe_) + " planned frames left");
+    const size_t bytes = sizeof(float) * nframes * n;
+    synchronize();
+    if (bytes > stage_bytes_) {
+      if (stage_host_) PTYRS_HIP(hipHostFree(stage_host_));
+      stage_host_ = nullptr;
+      stage_bytes_ = 0;
+      PTYRS_HIP(hipHostMalloc(&stage_host_, bytes, hipHostMallocDefault));
+      stage_bytes_ = bytes;
+    }
+    std::memcpy(stage_host_, counts, bytes);
+    PTYRS_HIP(hipMemcpyAsync(counts_.as<float>() + nframe_ * n, stage_host_, bytes,
+                             hipMemcpyHostToDevice, stream_));
+    synchronize();
+    nframe_ += nframes;
+    nb_obs_ += nb_obs;
+    total_ += total;
+    max_frame_total_ = std::max(max_frame_total_, max_frame_total);
+  }
+  ~Engine() override {
+    if (stage_host_) (void)hipHostFree(stage_host_);
+  }
+
   int get_position(double* out) override {
     if (!pos_on_) return 0;
-    if (out && nframe_) {
+    if (out && planned()) {
       PTYRS_HIP(hipStreamSynchronize(stream_));
-      PTYRS_HIP(hipMemcpy(out, offs_.p, sizeof(double) * 2 * nframe_, hipMemcpyDeviceToHost));
+      PTYRS_HIP(hipMemcpy(out, offs_.p, sizeof(double) * 2 * planned(),
+                          hipMemcpyDeviceToHost));
     }
     return pos_refine_ ? 2 : 1;
   }
