@@ -264,18 +264,26 @@ class BhBatch:
 
 
 class BhSlices:
-    """BhBatch through several object slices (no offsets): ψ the exit waves of SliceWaves, the
-    object variable every slice ([nslice, nobj, nyo, nxo]). The gradient back-propagates χ =
+    """BhBatch through several object slices: ψ the exit waves of SliceWaves, the object
+    variable every slice ([nslice, nobj, nyo, nxo]). The gradient back-propagates χ =
     Uᴴ(2·td) through the gaps; the Hessian's L_u, L_v and L_uv are the tangent and the mixed
-    second derivative of the slice model along u and v. ptyrs/src/engine/bh.rs:130-577,
-    ptyrs/src/engine/multislice.rs:318-383"""
+    second derivative of the slice model along u and v. With held FullObject `offsets` (one per
+    frame of the batch) every slice is read sampled at corner plus offset: the slices' gradients
+    are placed by the sampling's adjoint and a direction's slice is read through the same
+    windows. ptyrs/src/engine/bh.rs:130-577, ptyrs/src/engine/multislice.rs:318-383, the
+    sampled slices of ptyrs/src/engine/gpu/bh.rs:266-316"""
 
-    def __init__(self, views, iobs, optics, objects, probe):
+    def __init__(self, views, iobs, optics, objects, probe, offsets=None):
         from .multislice import SliceWaves
         self.views, self.optics = views, optics
         self.objects, self.probe = objects, probe
         self.frames, self.nmode = frames_of(views)
-        self.sw = SliceWaves(optics, views, list(objects), probe)
+        self.sm = None
+        if offsets is not None:
+            offsets = np.asarray(offsets, dtype=np.float64)
+            self.sm = Sampler(objects[0].shape[-2:], probe.shape[-2:], views,
+                              [offsets[k // self.nmode] for k in range(len(views))])
+        self.sw = SliceWaves(optics, views, list(objects), probe, sampler=self.sm)
         self.psi = optics.exit_forward(self.sw.psi)
         nb = len(self.frames)
         blk = self.psi.reshape((nb, self.nmode) + self.psi.shape[1:])
@@ -305,11 +313,12 @@ class BhSlices:
         wuv = np.zeros_like(wu)
         last = len(self.objects) - 1
         for s in range(last + 1):
-            o = self.objects[s]
-            ou = sw._times_window(s, o, wu) + sw._times_window(s, u[0][s], sw.incident[s])
-            ov = sw._times_window(s, o, wv) + sw._times_window(s, v[0][s], sw.incident[s])
-            ouv = (sw._times_window(s, o, wuv) + sw._times_window(s, u[0][s], wv)
-                   + sw._times_window(s, v[0][s], wu))
+            o = sw.objects[s]   # what the views read (under FullObject their windows)
+            us, vs = sw.windows(u[0][s]), sw.windows(v[0][s])
+            ou = sw._times_window(s, o, wu) + sw._times_window(s, us, sw.incident[s])
+            ov = sw._times_window(s, o, wv) + sw._times_window(s, vs, sw.incident[s])
+            ouv = (sw._times_window(s, o, wuv) + sw._times_window(s, us, wv)
+                   + sw._times_window(s, vs, wu))
             if s < last:
                 gap = self.optics.gaps[s]
                 wu, wv, wuv = gap.forward(ou), gap.forward(ov), gap.forward(ouv)
@@ -380,9 +389,10 @@ class Bh:
             views = [v for f in frames for v in views_all[f * nmode:(f + 1) * nmode]]
             offs = None if self.offsets is None else self.offsets[frames]
             if optics.nslice() > 1:
-                if offs is not None:
-                    raise ValueError("BH through several slices models no offsets here")
-                b = BhSlices(views, iobs, optics, obj, probe)
+                if offs is not None and (self.refine or self.sampling != "full_object"):
+                    raise ValueError("BH through several slices models held FullObject "
+                                     "offsets only here")
+                b = BhSlices(views, iobs, optics, obj, probe, offs)
             else:
                 b = BhBatch(views, iobs, optics, obj, probe, offs, self.refine, self.sampling)
             g = b.grads()

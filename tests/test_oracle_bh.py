@@ -291,3 +291,53 @@ def test_multislice_epochs_match_fixtures(case, method, batch_size):
         r = np.array([t for t in table(name) if t[0] == case and t[1] == method])
         d = (now - start).ravel()
         assert max_rel(np.stack([d.real, d.imag], axis=1).ravel(), r[:, 5:7].ravel()) < 2e-12
+
+
+def test_multislice_full_object_gradient_and_hessian_match_central_differences():
+    # Three slices read through their sampled windows at held FullObject offsets: the slices'
+    # and the probe's gradient against central differences of the cost, the Hessian against
+    # central differences of the gradient and symmetric; at zero offsets the batch is the
+    # plain multislice one (pinned by the bh_multislice_* fixtures).
+    from oracle.bh import BhSlices
+    from test_oracle_multislice import state as ms_state
+    addr, objects, probe, cases = ms_state()
+    optics, iobs = cases[0][0], cases[0][1]
+    obj = np.stack(objects)
+    rng = np.random.default_rng(21)
+    obj = obj * np.exp(1j * rng.uniform(-0.3, 0.3, obj.shape))
+    nb = 6
+    views = addr.views()[:nb * addr.nmode]
+    d0 = rng.uniform(-0.45, 0.45, (nb, 2))
+
+    def cost(o, p):
+        b = BhSlices(views, iobs, optics, o, p, d0)
+        keep = ~np.isnan(b.root)
+        a = b.amp - 1e-8
+        return float((((a - b.root) ** 2)[keep]).sum() +
+                     (2e-8 * b.root * np.log(b.amp))[keep].sum())
+
+    b = BhSlices(views, iobs, optics, obj, probe, d0)
+    g = b.grads()
+    w = (_direction(obj, rng), _direction(probe, rng))
+    h = 1e-5
+    fd = (cost(obj + h * w[0], probe + h * w[1]) - cost(obj - h * w[0], probe - h * w[1])) / (2 * h)
+    assert abs(dot(g, w) - fd) < 1e-7 * abs(fd)
+    u = (_direction(obj, rng), _direction(probe, rng))
+    v = (_direction(obj, rng), _direction(probe, rng))
+    h_uv, h_vu = b.hessian(u, v), b.hessian(v, u)
+    assert abs(h_uv - h_vu) < 1e-12 * abs(h_uv)
+
+    def du(t):
+        return dot(BhSlices(views, iobs, optics, obj + t * v[0], probe + t * v[1], d0).grads(), u)
+
+    t = 1e-6
+    fd = (du(t) - du(-t)) / (2 * t)
+    assert abs(h_uv - fd) < 1e-6 * abs(fd)
+    zero = BhSlices(views, iobs, optics, obj, probe, np.zeros((nb, 2)))
+    plain = BhSlices(views, iobs, optics, obj, probe)
+    assert np.abs(zero.psi - plain.psi).max() < 1e-12 * np.abs(plain.psi).max()
+    for x, y in zip(zero.grads()[:2], plain.grads()[:2]):
+        assert np.abs(x - y).max() < 1e-12 * np.abs(y).max()
+    assert abs(zero.hessian(u, v) - plain.hessian(u, v)) < 1e-12 * abs(plain.hessian(u, v))
+    # the offsets move what the slices read
+    assert np.abs(b.psi - plain.psi).max() > 1e-3 * np.abs(plain.psi).max()
