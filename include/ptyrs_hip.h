@@ -271,8 +271,9 @@ typedef struct {
 /* GpuPtycho::bh (ptyrs/src/engine/gpu/bh.rs:33-68): nepoch epochs of pty-chi's BH on the
  * amplitude cost from a fresh BH (epoch indices 0.. of the call), an LLK record every calc_llk
  * epochs and after the last. One object mode; one slice with or without position offsets
- * (refined or held) under PATCH or FullObject sampling, or several slices without offsets (the
- * object variable every slice); offsets through several slices give PTYRS_EINVAL. */
+ * (refined or held) under PATCH or FullObject sampling, or several slices (the object variable
+ * every slice) without offsets or with held FullObject offsets; refined offsets or PATCH
+ * offsets through several slices give PTYRS_EINVAL. */
 int ptyrs_engine_bh(ptyrs_engine* e, const ptyrs_bh_params* params, size_t nepoch,
                     size_t calc_llk);
 

@@ -677,9 +677,9 @@ class Engine : public EngineBase {
     if (p.batch_size == 0) throw Invalid("BH batch_size must be >= 1");
     if (!std::isfinite(p.probe_rho) || !std::isfinite(p.position_rho))
       throw Invalid("BH probe_rho and position_rho must be finite");
-    if (nslice_ > 1 && pos_on_)
-      throw Invalid("BH through several object slices with position offsets is not run on this "
-                    "GPU engine");
+    if (nslice_ > 1 && pos_on_ && (pos_refine_ || !full_object()))
+      throw Invalid("BH through several object slices runs without position offsets or with "
+                    "held offsets under FullObject sampling on this GPU engine");
     if (full_object() && nobj_ != 1)
       throw Invalid("FullObject sampling with several object modes is not run on this GPU "
                     "engine");
@@ -709,7 +709,10 @@ class Engine : public EngineBase {
     probe_terms();
     const double rho2 = p.probe_rho * p.probe_rho * scale_;
     const double pos2 = p.position_rho * p.position_rho;
-    if (full_object()) {
+    if (full_object() && nslice_ > 1) {
+      fo_ensure(cap_, true);
+      bh_fd_.ensure(sizeof(cplx<T>) * 2 * cap_ * n);
+    } else if (full_object()) {
       fo_ensure(cap_);
       ensure_bh_sampled(size);
     } else if (pos_on_) {
@@ -3105,7 +3108,13 @@ class Engine : public EngineBase {
     const int nt = ntiles();
     const size_t S = nslice_, no1 = nyo_ * nxo_, no = S * no1, np = nprobe_ * n;
     const int total = int(np), K = nb * M_, NF = cg ? 2 : 1;
+    // Held FullObject offsets: every slice read through the batch's sampled windows.
+    const bool fo = ms_fo();
     run(K_BIN, [&] {
+      if (fo) {
+        fo_slice_windows(ord, nb);
+        return;
+      }
       hipLaunchKernelGGL(k_bin_tiles<T>, dim3(nt), dim3(THREADS), 0, stream_,
                          corner_.as<int2>(), ord, nb, N, ntx_, int(cap_), list_.as<int>(),
                          lcount_.as<int>(), (const T*)nullptr, M_, int(nyo_), int(nxo_),
@@ -3127,6 +3136,15 @@ class Engine : public EngineBase {
       PTYRS_HIP(hipMemcpyAsync(bh_chi_.as<cplx<T>>() + s * vn, chi, sizeof(cplx<T>) * vn,
                                hipMemcpyDeviceToDevice, stream_));
       run(K_OBJ_DIR, [&] {
+        if (fo) {
+          // Σ_m conj(q_s·in_s)·χ_s per slot, placed by the sampling's adjoint.
+          hipLaunchKernelGGL(k_fo_chi_terms<T>, dim3((n + 255) / 256, nb), dim3(256), 0,
+                             stream_, fo_dwin_.as<cplx<T>>(), in.first, (const cplx<T>*)chi, M_,
+                             n, in.second, qs);
+          fo_place_spectrum(fo_dwin_.as<cplx<T>>(), (long long)n, nb, false);
+          fo_add_placed<AS_PLAIN>(bh_gO_.as<cplx<T>>() + s * no1, 1.0);
+          return;
+        }
         hipLaunchKernelGGL(k_lsqml_obj_dir<T>, dim3(nt), dim3(THREADS), 0, stream_,
                            bh_gO_.as<cplx<T>>() + s * no1, in.first, (const cplx<T>*)chi, N, M_,
                            corner_.as<int2>(), ord, list_.as<int>(), lcount_.as<int>(), int(cap_),
@@ -3141,9 +3159,10 @@ class Engine : public EngineBase {
     }
     run(K_PROBE, [&] {
       hipLaunchKernelGGL(k_ms_probe_dir<T>, dim3((n + 255) / 256, M_), dim3(256), 0, stream_,
-                         bh_gP_.as<cplx<T>>(), (const cplx<T>*)slice_obj(0), int(nxo_),
-                         (const cplx<T>*)nullptr, (const cplx<T>*)chi, corner_.as<int2>(), ord, nb,
-                         N, M_, T(1), 0);
+                         bh_gP_.as<cplx<T>>(),
+                         fo ? (const cplx<T>*)fo_swin(0) : (const cplx<T>*)slice_obj(0),
+                         fo ? N : int(nxo_), (const cplx<T>*)nullptr, (const cplx<T>*)chi,
+                         fo ? fo_tall_.as<int2>() : corner_.as<int2>(), ord, nb, N, M_, T(1), 0);
     });
     const int ndots =
         int(std::min<size_t>(kBhDotBlocks, (no + np + THREADS - 1) / THREADS));
@@ -3162,18 +3181,32 @@ class Engine : public EngineBase {
       const bool last = s + 1 == S;
       cplx<T>* out = last ? bh_l_.as<cplx<T>>() : bh_t_.as<cplx<T>>();
       run(K_HESS_A, [&] {
+        // Under FullObject the directions' slices as the slots read them: their sampled
+        // windows (tall, bh_fd_: u then v).
+        const cplx<T>* uo = bh_gO_.as<cplx<T>>() + s * no1;
+        const cplx<T>* vo = bh_dO_.as<cplx<T>>() + s * no1;
+        if (fo) {
+          cplx<T>* fw = bh_fd_.as<cplx<T>>();
+          fo_object_spectrum(uo);
+          fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 0, fw, (long long)n);
+          if (cg) {
+            fo_object_spectrum(vo);
+            fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 0, fw + cap_ * n, (long long)n);
+          }
+          uo = fw;
+          vo = fw + cap_ * n;
+        }
         auto tk = cg ? k_bh_slice_tangent<T, 2> : k_bh_slice_tangent<T, 1>;
         hipLaunchKernelGGL(tk, dim3(nblk, K), dim3(256), 0, stream_, out,
                            (const cplx<T>*)bh_t_.as<cplx<T>>(),
                            (const cplx<T>*)bh_gP_.as<cplx<T>>(), T(rho2),
                            (const cplx<T>*)bh_dP_.as<cplx<T>>(), int(s == 0), in.first,
-                           in.second, (const cplx<T>*)slice_obj(s),
-                           (const cplx<T>*)(bh_gO_.as<cplx<T>>() + s * no1),
-                           (const cplx<T>*)(bh_dO_.as<cplx<T>>() + s * no1), int(nxo_),
+                           in.second, fo ? (const cplx<T>*)fo_swin(s) : (const cplx<T>*)slice_obj(s),
+                           uo, vo, fo ? N : int(nxo_),
                            last ? wv.qlast : (const cplx<T>*)nullptr,
                            (const cplx<T>*)(bh_chi_.as<cplx<T>>() + s * vn),
-                           (const int2*)corner_.as<int2>(), ord, N, M_,
-                           bh_ps_.as<double>() + s * size_t(K) * nblk * 3);
+                           fo ? (const int2*)fo_tall_.as<int2>() : (const int2*)corner_.as<int2>(),
+                           ord, N, M_, bh_ps_.as<double>() + s * size_t(K) * nblk * 3);
         if (!last) gap(bh_t_.as<cplx<T>>(), K * NF, s, false);
       });
     }
