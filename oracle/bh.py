@@ -267,17 +267,20 @@ class BhSlices:
     """BhBatch through several object slices: ψ the exit waves of SliceWaves, the object
     variable every slice ([nslice, nobj, nyo, nxo]). The gradient back-propagates χ =
     Uᴴ(2·td) through the gaps; the Hessian's L_u, L_v and L_uv are the tangent and the mixed
-    second derivative of the slice model along u and v. With held FullObject `offsets` (one per
-    frame of the batch) every slice is read sampled at corner plus offset: the slices' gradients
-    are placed by the sampling's adjoint and a direction's slice is read through the same
-    windows. ptyrs/src/engine/bh.rs:130-577, ptyrs/src/engine/multislice.rs:318-383, the
+    second derivative of the slice model along u and v. With FullObject `offsets` (one per frame
+    of the batch) every slice is read sampled at corner plus offset: the slices' gradients are
+    placed by the sampling's adjoint and a direction's slice is read through the same windows;
+    refined (`refine`), the offsets join the variable: their gradient sums every slice's reads'
+    gradient against the reads' derivatives, and the reads' first- and second-order changes
+    take the offsets' terms (BhBatch._fo_lift and _fo_second, slice by slice). ptyrs/src/engine/bh.rs:130-577, ptyrs/src/engine/multislice.rs:318-383, the
     sampled slices of ptyrs/src/engine/gpu/bh.rs:266-316"""
 
-    def __init__(self, views, iobs, optics, objects, probe, offsets=None):
+    def __init__(self, views, iobs, optics, objects, probe, offsets=None, refine=False):
         from .multislice import SliceWaves
         self.views, self.optics = views, optics
         self.objects, self.probe = objects, probe
         self.frames, self.nmode = frames_of(views)
+        self.refine = refine and offsets is not None
         self.sm = None
         if offsets is not None:
             offsets = np.asarray(offsets, dtype=np.float64)
@@ -301,7 +304,47 @@ class BhSlices:
         np_ = self.probe.shape[0]
         per = self.sw.probe_adjoint(back[0], np_)
         g_probe = per.reshape(-1, np_, *self.probe.shape[1:]).sum(axis=0)
-        return g_obj, g_probe, None
+        g_d = None
+        if self.refine:
+            sw = self.sw
+            g_d = np.zeros((len(self.frames), 2))
+            for s in range(len(self.objects)):
+                for a in range(2):
+                    dw = self.sm.windows(self.objects[s], (a,))
+                    for k in range(len(self.views)):
+                        gw = to_natural(back[s][k]) * np.conj(sw.quad(s, k) * sw.incident[s][k])
+                        g_d[k // self.nmode, a] += float((np.conj(gw) * dw[k]).real.sum())
+        return g_obj, g_probe, g_d
+
+    def _reads_lift(self, s, u):
+        """Slice s's reads' first-order change along u: its windows of u_O,s, plus with refined
+        offsets Σ_a u_da·∂_a S(O_s)."""
+        r = self.sw.windows(u[0][s])
+        if self.refine:
+            for a in range(2):
+                d = self.sm.windows(self.objects[s], (a,))
+                r = r + np.stack([u[2][k // self.nmode][a] * d[k]
+                                  for k in range(len(self.views))])
+        return r
+
+    def _reads_second(self, s, u, v):
+        """Slice s's reads' second-order change: Σ_a (v_da·∂_a S(u_O,s) + u_da·∂_a S(v_O,s)) +
+        Σ_ac u_da·v_dc·∂_a ∂_c S(O_s); None without refined offsets."""
+        if not self.refine:
+            return None
+        nv = len(self.views)
+        cu = [u[2][k // self.nmode] for k in range(nv)]
+        cv = [v[2][k // self.nmode] for k in range(nv)]
+        out = np.zeros((nv,) + tuple(self.probe.shape[-2:]), dtype=np.complex128)
+        for a in range(2):
+            du, dv = self.sm.windows(u[0][s], (a,)), self.sm.windows(v[0][s], (a,))
+            for k in range(nv):
+                out[k] += cv[k][a] * du[k] + cu[k][a] * dv[k]
+            for c in range(2):
+                dd = self.sm.windows(self.objects[s], (a, c))
+                for k in range(nv):
+                    out[k] += cu[k][a] * cv[k][c] * dd[k]
+        return out
 
     def _tangents(self, u, v):
         """[L_u, L_v, L_uv] in the exit plane before U (natural layout): per slice s the changes
@@ -314,11 +357,14 @@ class BhSlices:
         last = len(self.objects) - 1
         for s in range(last + 1):
             o = sw.objects[s]   # what the views read (under FullObject their windows)
-            us, vs = sw.windows(u[0][s]), sw.windows(v[0][s])
+            us, vs = self._reads_lift(s, u), self._reads_lift(s, v)
             ou = sw._times_window(s, o, wu) + sw._times_window(s, us, sw.incident[s])
             ov = sw._times_window(s, o, wv) + sw._times_window(s, vs, sw.incident[s])
             ouv = (sw._times_window(s, o, wuv) + sw._times_window(s, us, wv)
                    + sw._times_window(s, vs, wu))
+            o2 = self._reads_second(s, u, v)
+            if o2 is not None:
+                ouv = ouv + sw._times_window(s, o2, sw.incident[s])
             if s < last:
                 gap = self.optics.gaps[s]
                 wu, wv, wuv = gap.forward(ou), gap.forward(ov), gap.forward(ouv)
@@ -389,10 +435,10 @@ class Bh:
             views = [v for f in frames for v in views_all[f * nmode:(f + 1) * nmode]]
             offs = None if self.offsets is None else self.offsets[frames]
             if optics.nslice() > 1:
-                if offs is not None and (self.refine or self.sampling != "full_object"):
-                    raise ValueError("BH through several slices models held FullObject "
-                                     "offsets only here")
-                b = BhSlices(views, iobs, optics, obj, probe, offs)
+                if offs is not None and self.sampling != "full_object":
+                    raise ValueError("BH through several slices models FullObject offsets "
+                                     "only here")
+                b = BhSlices(views, iobs, optics, obj, probe, offs, self.refine)
             else:
                 b = BhBatch(views, iobs, optics, obj, probe, offs, self.refine, self.sampling)
             g = b.grads()

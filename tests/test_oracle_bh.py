@@ -341,3 +341,50 @@ def test_multislice_full_object_gradient_and_hessian_match_central_differences()
     assert abs(zero.hessian(u, v) - plain.hessian(u, v)) < 1e-12 * abs(plain.hessian(u, v))
     # the offsets move what the slices read
     assert np.abs(b.psi - plain.psi).max() > 1e-3 * np.abs(plain.psi).max()
+
+
+def test_multislice_refined_full_object_offsets_match_central_differences():
+    # Refined FullObject offsets through three slices: the offsets' gradient (every slice's
+    # reads' gradient against the reads' derivatives) and the Hessian over slices, probe and
+    # offsets against central differences of the cost and of the gradient; held, the same waves.
+    from oracle.bh import BhSlices
+    from test_oracle_multislice import state as ms_state
+    addr, objects, probe, cases = ms_state()
+    optics, iobs = cases[0][0], cases[0][1]
+    rng = np.random.default_rng(22)
+    obj = np.stack(objects) * np.exp(1j * rng.uniform(-0.3, 0.3, np.stack(objects).shape))
+    nb = 6
+    views = addr.views()[:nb * addr.nmode]
+    d0 = rng.uniform(-0.45, 0.45, (nb, 2))
+
+    def cost(o, p, d):
+        b = BhSlices(views, iobs, optics, o, p, d)
+        keep = ~np.isnan(b.root)
+        a = b.amp - 1e-8
+        return float((((a - b.root) ** 2)[keep]).sum() +
+                     (2e-8 * b.root * np.log(b.amp))[keep].sum())
+
+    b = BhSlices(views, iobs, optics, obj, probe, d0, True)
+    g = b.grads()
+    w = (_direction(obj, rng), _direction(probe, rng), rng.uniform(-1, 1, (nb, 2)))
+    h = 1e-5
+    fd = (cost(obj + h * w[0], probe + h * w[1], d0 + h * w[2]) -
+          cost(obj - h * w[0], probe - h * w[1], d0 - h * w[2])) / (2 * h)
+    assert abs(dot(g, w) - fd) < 1e-7 * abs(fd)
+    wd = (0 * w[0], 0 * w[1], w[2])   # the offsets' part alone
+    fd = (cost(obj, probe, d0 + h * w[2]) - cost(obj, probe, d0 - h * w[2])) / (2 * h)
+    assert abs(dot(g, wd) - fd) < 1e-6 * abs(fd)
+    u = (_direction(obj, rng), _direction(probe, rng), rng.uniform(-1, 1, (nb, 2)))
+    v = (_direction(obj, rng), _direction(probe, rng), rng.uniform(-1, 1, (nb, 2)))
+    h_uv, h_vu = b.hessian(u, v), b.hessian(v, u)
+    assert abs(h_uv - h_vu) < 1e-12 * abs(h_uv)
+
+    def du(t):
+        bt = BhSlices(views, iobs, optics, obj + t * v[0], probe + t * v[1], d0 + t * v[2], True)
+        return dot(bt.grads(), u)
+
+    t = 1e-6
+    fd = (du(t) - du(-t)) / (2 * t)
+    assert abs(h_uv - fd) < 1e-6 * abs(fd)
+    held = BhSlices(views, iobs, optics, obj, probe, d0)
+    assert held.grads()[2] is None and np.array_equal(held.psi, b.psi)
