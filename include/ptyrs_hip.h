@@ -272,8 +272,8 @@ typedef struct {
  * amplitude cost from a fresh BH (epoch indices 0.. of the call), an LLK record every calc_llk
  * epochs and after the last. One object mode; one slice with or without position offsets
  * (refined or held) under PATCH or FullObject sampling, or several slices (the object variable
- * every slice) without offsets or with held FullObject offsets; refined offsets or PATCH
- * offsets through several slices give PTYRS_EINVAL. */
+ * every slice) without offsets or with FullObject offsets (refined or held); PATCH offsets
+ * through several slices give PTYRS_EINVAL. */
 int ptyrs_engine_bh(ptyrs_engine* e, const ptyrs_bh_params* params, size_t nepoch,
                     size_t calc_llk);
 

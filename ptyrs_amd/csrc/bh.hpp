@@ -339,7 +339,8 @@ __global__ __launch_bounds__((HessShape<T, N, NF>::NT)) void k_bh_hess_a_moved(
 template <typename T>
 __global__ __launch_bounds__(THREADS) void k_bh_win_dot(const cplx<T>* __restrict__ g,
                                                         const cplx<T>* __restrict__ d, int n,
-                                                        double* __restrict__ gd) {
+                                                        double* __restrict__ gd,
+                                                        int accumulate = 0) {
   __shared__ double red[THREADS];
   const int b = blockIdx.x;
   double acc[2] = {0, 0};
@@ -349,7 +350,7 @@ __global__ __launch_bounds__(THREADS) void k_bh_win_dot(const cplx<T>* __restric
   }
   for (int a = 0; a < 2; ++a) {
     const double t = block_reduce<THREADS>(acc[a], red, AddOp());
-    if (threadIdx.x == 0) gd[2 * (size_t)b + a] = t;
+    if (threadIdx.x == 0) gd[2 * (size_t)b + a] = accumulate ? gd[2 * (size_t)b + a] + t : t;
   }
 }
 
@@ -476,7 +477,10 @@ __global__ __launch_bounds__((HessShape<T, N, NF>::NT)) void k_bh_hess_a_fo(
 // Re Σ conj(χ_s)·q_s·(u_s·w_v + v_s·w_u), Re Σ conj(χ_s)·q_s·2·v_s·w_v (FP64, per workgroup
 // into part[(view·gridDim.x + blk)·3 + i]); ⟨r, J″(u, v)⟩ is their sum over the slices. At
 // slice 0 (w_mode) w_u = su·u_P[m] and w_v = v_P[m]; `in` is P[m] there (in_view 0), else the
-// views' incident waves. Grid (ceil(n/256), nb·M).
+// views' incident waves. With `rd` (FullObject with refined offsets) the slot's reads' changes
+// come from rd[slot·NR + p] (k_bh_fo_reads' layout: u′, v′, o″_uu, o″_uv, o″_vv; NF = 1: u′,
+// o″_uu) in place of uO and vO, and the second-order terms add Re Σ conj(χ_s)·q_s·o″·in.
+// Grid (ceil(n/256), nb·M).
 template <typename T, int NF>
 __global__ __launch_bounds__(256) void k_bh_slice_tangent(
     cplx<T>* out, const cplx<T>* w, const cplx<T>* __restrict__ uP, T su,
@@ -484,8 +488,10 @@ __global__ __launch_bounds__(256) void k_bh_slice_tangent(
     const cplx<T>* __restrict__ O, const cplx<T>* __restrict__ uO,
     const cplx<T>* __restrict__ vO, int nxo, const cplx<T>* __restrict__ q,
     const cplx<T>* __restrict__ chi, const int2* __restrict__ corner,
-    const int* __restrict__ order, int N, int M, double* __restrict__ part) {
+    const int* __restrict__ order, int N, int M, double* __restrict__ part,
+    const cplx<T>* __restrict__ rd = nullptr) {
   __shared__ double red[256];
+  constexpr int NR = NF == 2 ? 5 : 2;
   const int n = N * N;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int v = blockIdx.y, b = v / M, m = v % M;
@@ -493,18 +499,24 @@ __global__ __launch_bounds__(256) void k_bh_slice_tangent(
   if (j < n) {
     const int2 c = corner[order[b]];
     const size_t at = (size_t)(c.x + j / N) * nxo + c.y + j % N;
-    const cplx<T> o = ld(O + at), ao = ld(uO + at);
+    const cplx<T>* r = rd ? rd + (size_t)b * NR * n + j : nullptr;
+    const cplx<T> o = ld(O + at), ao = r ? ld(r) : ld(uO + at);
     const cplx<T> qq = q ? ld(q + j) : cplx<T>{T(1), T(0)};
     const cplx<T> iw = ld(in + (size_t)(in_view ? v : m) * n + j);
     const cplx<T> cq = cmulc(qq, ld(chi + (size_t)v * n + j));  // conj(q)·χ
     const cplx<T> wu = w_mode ? ld(uP + (size_t)m * n + j) * su : ld(w + ((size_t)v * NF) * n + j);
     s[0] = 2.0 * double(cmulc(cq, ao * wu).x);
+    if (r) s[0] += double(cmulc(cq, ld(r + (NF == 2 ? 2 : 1) * n) * iw).x);
     const cplx<T> ou = qq * (o * wu + ao * iw);
     if constexpr (NF == 2) {
-      const cplx<T> bo = ld(vO + at);
+      const cplx<T> bo = r ? ld(r + n) : ld(vO + at);
       const cplx<T> wv = w_mode ? ld(vP + (size_t)m * n + j) : ld(w + ((size_t)v * NF + 1) * n + j);
       s[1] = double(cmulc(cq, ao * wv + bo * wu).x);
       s[2] = 2.0 * double(cmulc(cq, bo * wv).x);
+      if (r) {
+        s[1] += double(cmulc(cq, ld(r + 3 * n) * iw).x);
+        s[2] += double(cmulc(cq, ld(r + 4 * n) * iw).x);
+      }
       st(out + ((size_t)v * NF + 1) * n + j, qq * (o * wv + bo * iw));
     }
     st(out + ((size_t)v * NF) * n + j, ou);

@@ -677,9 +677,9 @@ class Engine : public EngineBase {
     if (p.batch_size == 0) throw Invalid("BH batch_size must be >= 1");
     if (!std::isfinite(p.probe_rho) || !std::isfinite(p.position_rho))
       throw Invalid("BH probe_rho and position_rho must be finite");
-    if (nslice_ > 1 && pos_on_ && (pos_refine_ || !full_object()))
+    if (nslice_ > 1 && pos_on_ && !full_object())
       throw Invalid("BH through several object slices runs without position offsets or with "
-                    "held offsets under FullObject sampling on this GPU engine");
+                    "offsets under FullObject sampling on this GPU engine");
     if (full_object() && nobj_ != 1)
       throw Invalid("FullObject sampling with several object modes is not run on this GPU "
                     "engine");
@@ -711,7 +711,7 @@ class Engine : public EngineBase {
     const double pos2 = p.position_rho * p.position_rho;
     if (full_object() && nslice_ > 1) {
       fo_ensure(cap_, true);
-      bh_fd_.ensure(sizeof(cplx<T>) * 2 * cap_ * n);
+      ensure_bh_sampled(size);
     } else if (full_object()) {
       fo_ensure(cap_);
       ensure_bh_sampled(size);
@@ -729,7 +729,7 @@ class Engine : public EngineBase {
         int k = 0;
         for (auto& s : spans[ic])
           bh_batch_ms(rho2, ord + s.first, s.second, double(s.second) / double(size), cg, full,
-                      k++);
+                      k++, pos2);
       } else if (full_object()) {
         int k = 0;
         for (auto& s : spans[ic])
@@ -3104,12 +3104,15 @@ class Engine : public EngineBase {
   // chains along u = (g_O, ρ²·g_P) and v = D′ up through the slices, whose second-order terms
   // meet χ_s slice by slice and whose exit changes L_u, L_v go through the detector-plane pass.
   void bh_batch_ms(double rho2, const int* ord, int nb, double frac, bool cg, bool keep,
-                   int slot) {
+                   int slot, double pos2 = 0.0) {
     const int nt = ntiles();
     const size_t S = nslice_, no1 = nyo_ * nxo_, no = S * no1, np = nprobe_ * n;
     const int total = int(np), K = nb * M_, NF = cg ? 2 : 1;
-    // Held FullObject offsets: every slice read through the batch's sampled windows.
+    // FullObject offsets: every slice read through the batch's sampled windows; refined, the
+    // offsets' gradient sums every slice's reads' gradient against the reads' derivatives and
+    // the reads' changes take the offsets' terms (k_bh_fo_reads per slice).
     const bool fo = ms_fo();
+    const bool refine = fo && pos_refine_;
     run(K_BIN, [&] {
       if (fo) {
         fo_slice_windows(ord, nb);
@@ -3141,6 +3144,15 @@ class Engine : public EngineBase {
           hipLaunchKernelGGL(k_fo_chi_terms<T>, dim3((n + 255) / 256, nb), dim3(256), 0,
                              stream_, fo_dwin_.as<cplx<T>>(), in.first, (const cplx<T>*)chi, M_,
                              n, in.second, qs);
+          if (refine) {
+            cplx<T>* fd = bh_fd_.as<cplx<T>>();
+            fo_object_spectrum(slice_obj(s));
+            fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 1, fd, 2ll * n);
+            fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 2, fd + n, 2ll * n);
+            hipLaunchKernelGGL(k_bh_win_dot<T>, dim3(nb), dim3(THREADS), 0, stream_,
+                               (const cplx<T>*)fo_dwin_.as<cplx<T>>(), (const cplx<T>*)fd, n,
+                               bh_gd_.as<double>(), int(s + 1 < S));
+          }
           fo_place_spectrum(fo_dwin_.as<cplx<T>>(), (long long)n, nb, false);
           fo_add_placed<AS_PLAIN>(bh_gO_.as<cplx<T>>() + s * no1, 1.0);
           return;
@@ -3185,7 +3197,37 @@ class Engine : public EngineBase {
         // windows (tall, bh_fd_: u then v).
         const cplx<T>* uo = bh_gO_.as<cplx<T>>() + s * no1;
         const cplx<T>* vo = bh_dO_.as<cplx<T>>() + s * no1;
-        if (fo) {
+        const cplx<T>* rd = nullptr;
+        if (refine) {
+          // The reads' changes of slice s: the nine windows k_bh_fo_reads takes, from the
+          // spectra of O_s, u_O,s and v_O,s.
+          cplx<T>* fw = bh_fw_.as<cplx<T>>();
+          cplx<T>* fd = bh_fd_.as<cplx<T>>();
+          cplx<T>* su = bh_fs_.as<cplx<T>>();
+          cplx<T>* sv = su + no1;
+          const long long st9 = 9ll * n;
+          fo_object_spectrum(slice_obj(s));
+          fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 1, fd, 2ll * n);
+          fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 2, fd + n, 2ll * n);
+          for (int d = 3; d <= 5; ++d)
+            fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, d, fw + (3 + d) * n, st9);
+          fo_spectrum_to(uo, su);
+          fo_windows<AS_PLAIN>(su, nb, 0, fw, st9);
+          fo_windows<AS_PLAIN>(su, nb, 1, fw + 2 * n, st9);
+          fo_windows<AS_PLAIN>(su, nb, 2, fw + 3 * n, st9);
+          if (cg) {
+            fo_spectrum_to(vo, sv);
+            fo_windows<AS_PLAIN>(sv, nb, 0, fw + n, st9);
+            fo_windows<AS_PLAIN>(sv, nb, 1, fw + 4 * n, st9);
+            fo_windows<AS_PLAIN>(sv, nb, 2, fw + 5 * n, st9);
+          }
+          auto kr = cg ? k_bh_fo_reads<T, 2> : k_bh_fo_reads<T, 1>;
+          hipLaunchKernelGGL(kr, dim3((n + 255) / 256, nb), dim3(256), 0, stream_,
+                             bh_fr_.as<cplx<T>>(), (const cplx<T>*)fw, (const cplx<T>*)fd,
+                             (const double*)bh_gd_.as<double>(), pos2,
+                             (const double*)bh_ddf_.as<double>(), ord, n);
+          rd = bh_fr_.as<cplx<T>>();
+        } else if (fo) {
           cplx<T>* fw = bh_fd_.as<cplx<T>>();
           fo_object_spectrum(uo);
           fo_windows<AS_PLAIN>(fo_spec_.as<cplx<T>>(), nb, 0, fw, (long long)n);
@@ -3206,7 +3248,7 @@ class Engine : public EngineBase {
                            last ? wv.qlast : (const cplx<T>*)nullptr,
                            (const cplx<T>*)(bh_chi_.as<cplx<T>>() + s * vn),
                            fo ? (const int2*)fo_tall_.as<int2>() : (const int2*)corner_.as<int2>(),
-                           ord, N, M_, bh_ps_.as<double>() + s * size_t(K) * nblk * 3);
+                           ord, N, M_, bh_ps_.as<double>() + s * size_t(K) * nblk * 3, rd);
         if (!last) gap(bh_t_.as<cplx<T>>(), K * NF, s, false);
       });
     }
@@ -3215,8 +3257,9 @@ class Engine : public EngineBase {
                          bh_l_.as<cplx<T>>(), tw_.as<cplx<T>>());
     });
     const T norm = T(1) / (angular_ ? T(n) * T(n) : T(n));
-    bh_hess_b_and_step(ord, nb, cg, norm, ndots, frac, slot, nullptr, 0, 0.0,
-                       bh_ps_.as<double>(), int(S) * K * nblk);
+    bh_hess_b_and_step(ord, nb, cg, norm, ndots, frac, slot,
+                       refine ? (const double*)bh_gd_.as<double>() : nullptr, refine ? nb : 0,
+                       pos2, bh_ps_.as<double>(), int(S) * K * nblk);
     run(K_OBJ_UPDATE, [&] {
       const size_t all = no + np;
       hipLaunchKernelGGL(k_bh_update<T>, dim3(unsigned((all + 255) / 256)), dim3(256), 0,
@@ -3224,6 +3267,11 @@ class Engine : public EngineBase {
                          (const cplx<T>*)bh_gO_.as<cplx<T>>(), no, P_.as<cplx<T>>(),
                          bh_dP_.as<cplx<T>>(), (const cplx<T>*)bh_gP_.as<cplx<T>>(), np, rho2,
                          (const double*)bh_step_.as<double>(), int(cg), int(keep));
+      if (refine)
+        hipLaunchKernelGGL(k_bh_update_pos, dim3((nb + 255) / 256), dim3(256), 0, stream_,
+                           offs_.as<double2>(), bh_ddf_.as<double>(),
+                           (const double*)bh_gd_.as<double>(), ord, nb, pos2,
+                           (const double*)bh_step_.as<double>(), int(cg), int(keep));
     });
     run(K_PROBE, [&] {
       hipLaunchKernelGGL(k_probe_terms<T>, dim3((total + 255) / 256), dim3(256), 0, stream_,

@@ -277,11 +277,43 @@ def test_multislice_held_full_object_offsets_match_oracle(dtype, case, batch_siz
     assert rel_l2(g.objects - np.stack(objects), obj - np.stack(objects)) < tol * 10
     assert rel_l2(g.probe - probe0, probe - probe0) < tol * 10
     assert np.array_equal(g.features["position"], d0)
-    # refined offsets through several slices, and PATCH offsets through them, are refused
-    g.set_features(position=d0)
-    with pytest.raises(ValueError, match="several object slices"):
-        g.bh(1, 0, batch_size, 1.0, 7)
+    # PATCH offsets through several slices are refused
     g.set_sampling("patch")
     g.set_features(position=d0, hold=("position",))
     with pytest.raises(ValueError, match="several object slices"):
         g.bh(1, 0, batch_size, 1.0, 7)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("case,batch_size", [(0, 16), (1, 6), (2, 40)])
+def test_multislice_refined_full_object_offsets_match_oracle(dtype, case, batch_size):
+    # Three slices at refined FullObject offsets (the oracle's BhSlices with refine: the
+    # offsets' gradient over every slice's reads, the reads' offset terms slice by slice, the
+    # step 0.3·α·D_d), all three exits, steepest descent and conjugate gradient, three epochs.
+    from test_gpu_multislice import start as ms_start
+    from test_oracle_multislice import state as ms_state
+    addr, objects, probe0, cases = ms_state()
+    optics, iobs = cases[case][0], cases[case][1]
+    d0 = np.random.default_rng(37).uniform(-0.45, 0.45, (addr.nframe, 2))
+    bh = Bh(batch_size, probe_rho=1.0, seed=7, offsets=d0, refine=True, sampling="full_object")
+    obj, probe, got = np.stack(objects), probe0, []
+    for _ in range(3):
+        obj, probe, s = bh.epoch(addr, iobs, optics, obj, probe)
+        got += s
+    g, _, _ = ms_start(case, dtype)
+    g.set_sampling("full_object")
+    g.set_features(position=d0)
+    g.bh(3, 1, batch_size, 1.0, 7)
+    tol = TOL[dtype]
+    want = got[-len(g.bh_steps()):]
+    for s, (frames, beta, alpha, top, bottom) in zip(g.bh_steps(), want):
+        assert s["frames"] == list(frames)
+        for name, w in (("alpha", alpha), ("top", top), ("bottom", bottom)):
+            assert abs(s[name] - w) < tol * 10 * abs(w), (name, s[name], w)
+    dd = g.features["position"] - d0
+    print("bh multislice refined full object", case, batch_size, dtype,
+          rel_l2(g.objects - np.stack(objects), obj - np.stack(objects)),
+          rel_l2(g.probe - probe0, probe - probe0), max_rel(dd.ravel(), (bh.offsets - d0).ravel()))
+    assert rel_l2(g.objects - np.stack(objects), obj - np.stack(objects)) < tol * 10
+    assert rel_l2(g.probe - probe0, probe - probe0) < tol * 10
+    assert max_rel(dd.ravel(), (bh.offsets - d0).ravel()) < tol * 100
