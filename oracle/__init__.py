@@ -27,7 +27,10 @@ What pins each module (the test files named run on the CPU):
   tests/test_oracle_pynx.py (AP, DM, RAAR, ML against `pynx_near_*`);
   tests/test_oracle_scenario.py (`dm_pynx`, `raar_pynx`, `rpie_ptychi`, `ml_grad`; rPIE and
   ML under refined PATCH offsets: `rpie_pos_*` at 1e-12, `ml_pos_*` at 1e-13; rPIE under
-  FULL_OBJECT: `rpie_full_object_{step,epochs,odd}_*` at 1e-12).
+  FULL_OBJECT: `rpie_full_object_{step,epochs,odd}_*` at 1e-12; ML, AP, DM and RAAR under
+  FULL_OBJECT: zero offsets reduce them to the plain runs, ML's gradient against central
+  differences, `solve_obj` descends its quadratic and, run long, reaches its minimiser (the
+  quadratic's slope along random directions vanishes there)).
 - lsqml.py — tests/test_oracle_lsqml.py (`lsqml_near_*`), tests/test_oracle_scenario.py
   (`lsqml_{gaussian,poisson}_*`, `lsqml_epochs_*`), tests/test_oracle_gradient.py
   (`cost_grads` against PyNX's ML gradient and five-point central differences).
@@ -37,7 +40,8 @@ What pins each module (the test files named run on the CPU):
   step (one eigenmode) and LSQML steps with three eigenmodes, with and without the main mode's
   weights (steps, changes, eigenmodes, weights at 1e-12/1e-13), a one-frame batch, held
   weights; LSQML with PATCH offsets and OPR (`lsqml_pos_opr_*`), and the far-field
-  `lsqml_pos_*` step and three epochs.
+  `lsqml_pos_*` step and three epochs; under FULL_OBJECT at zero offsets the OPR step
+  reproduces the `lsqml_opr_*` and `lsqml_opr_int_*` fixtures at 1e-11.
 - bh.py — tests/test_oracle_bh.py restates `ptyrs/tests/engine_bh.rs` and
   `engine_bh_near.rs`: two epochs on the seeded far-field scenario with one batch of all frames
   (conjugate gradient) and with batches of 16 (`bh_full_*`, `bh_mini_*`: β, α, ⟨g, D⟩, H(D, D)
@@ -46,8 +50,9 @@ What pins each module (the test files named run on the CPU):
   steps at 2e-13, every slice's and the probe's change at 2e-12); refined offsets under PATCH (`bh_pos_full_*`, `bh_pos_mini_*`) and FullObject sampling
   (`bh_full_object_full_*`, `bh_full_object_mini_*`, pty-chi's) at 3e-12; the
   gradient against central differences of the cost, the Hessian against central differences of
-  the gradient (with and without offsets), its symmetry and bilinearity, zero offsets reducing
-  to the unmoved batch.
+  the gradient (with and without offsets; through three slices at held and refined
+  FULL_OBJECT offsets, the offsets' part alone), its symmetry and bilinearity, zero offsets
+  reducing to the unmoved batch.
 - multislice.py — tests/test_oracle_multislice.py (`lsqml_multislice_*`: forward
   intensities, one step, 30 epochs; directions and tangent against central differences).
 - positions.py — tests/test_oracle_positions.py (`lsqml_near_pos_*`,
