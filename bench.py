@@ -241,14 +241,23 @@ def kernel_roofline(stats, per_frame, positions):
             "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_s * 1e3, 4)}
 
 
-def algorithm_rates(pa, scan, d, steps=3, algorithms=("ap", "dm", "raar", "ml")):
+def algorithm_rates(pa, scan, d, steps=3, algorithms=("ap", "dm", "raar", "ml"),
+                    full_object=False):
     """AP, DM, RAAR and ML (their default parameters, no LLK records) on the same scan:
-    positions·cycle/s of `steps` cycles after one warm-up cycle, complex64 and complex128."""
+    positions·cycle/s of `steps` cycles after one warm-up cycle, complex64 and complex128.
+    With `full_object` every frame reads its object window sampled at its corner plus a held
+    sub-pixel offset (FullObject sampling; offsets uniform in ±0.45 px, seed 5)."""
     out = {"steps": steps}
+    if full_object:
+        out["sampling"] = "full_object, held offsets uniform in ±0.45 px"
+    offs = np.random.default_rng(5).uniform(-0.45, 0.45, (NPOS, 2))
     for alg in algorithms:
         r = {}
         for dtype in ("c64", "c128"):
             g = pa.GpuPtycho(scan, d["start_obj"], d["start_probe"], dtype=dtype)
+            if full_object:
+                g.set_sampling("full_object")
+                g.set_features(position=offs, hold=("position",))
             getattr(g, alg)(1)
             g.synchronize()
             t = time.perf_counter()
@@ -258,7 +267,7 @@ def algorithm_rates(pa, scan, d, steps=3, algorithms=("ap", "dm", "raar", "ml"))
             r["value" if dtype == "c64" else "fp64"] = round(NPOS * steps / dt, 1)
             del g
         r["speedup_fp32_vs_fp64"] = round(r["value"] / r["fp64"], 3)
-        if alg in ("dm", "raar"):
+        if alg in ("dm", "raar") and not full_object:
             # GpuOptions::psi_f16: the carried exit waves as f16 pairs.
             g = pa.GpuPtycho(scan, d["start_obj"], d["start_probe"], psi_f16=True)
             getattr(g, alg)(1)
@@ -491,6 +500,8 @@ def main():
     ap.add_argument("--no-copy", action="store_true",
                     help="skip the child copy-bandwidth process (counter passes)")
     ap.add_argument("--batch", type=int, default=BATCH, help="rPIE batch size (positions)")
+    ap.add_argument("--full-object", action="store_true",
+                    help="with --algorithm ap|dm|raar|ml: FullObject sampling at held offsets")
     ap.add_argument("--algorithm", choices=("lsqml", "bh", "ap", "dm", "raar", "ml"), default=None,
                     help="time one other algorithm alone on the headline scan (for profiling)")
     a = ap.parse_args()
@@ -530,7 +541,7 @@ def main():
               flush=True)
         return
     if a.algorithm is not None:
-        out = algorithm_rates(pa, scan, d, a.steps, (a.algorithm,))
+        out = algorithm_rates(pa, scan, d, a.steps, (a.algorithm,), a.full_object)
         print(json.dumps({"workload": "headline scan", **out}), flush=True)
         return
 
